@@ -1,0 +1,153 @@
+"""Pin the oracle against outputs the reference itself wrote.
+
+The reference run v25/v251/run1 (N=500 spherocylinders, L=9, one axial dipole
+mu=4.481, B=(0,0,5), K=5, alpha=0.07, NHL thermostat gamma=1, dt=1e-4, 101
+steps, clustering every 10 steps) is replayed from its Signal=0 start:
+ini_pos, 1000 GCAmix moves, set_dip, ini_mom, then the step loop of
+v25/main.cpp.  Every printed value is compared at the printed precision
+(12 decimals): observables (output.dat), the final state and cluster labels
+(conf.dat), 25-particle snapshots every 2 steps (pos_vel_*.dat), the
+neighbour-list reference positions (compConf.dat) and the lattice (iniPos.dat).
+"""
+import ctypes
+import math
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+N = 500
+
+
+def v251_system():
+    S = O.OSys(N, 1)
+    S.L[:] = math.pow(500.0 / ((1.0 * 1.0) * 0.0013), 1.0 / 3)
+    S.len[:] = 9.0
+    S.dia[:] = 1.0
+    S.mass[:] = 1.0
+    S.doc[:, :, :] = (0.0, 0.0, 1.0)
+    S.dmu[:] = 4.481
+    S.set_inertia()
+    S.set_dip()
+    P = O.run_params(dens=0.0013, temp=1.0, dt=1e-4, sr_cut=10.6, alpha=0.07, min_s=0.1,
+                     Q=5.0, gamma=1.0, s0=1.0, B=(0.0, 0.0, 5.0), K=5, cl_fr=10)
+    return S, P
+
+
+def f12(v):
+    return "%.12f" % v
+
+
+def mm(A, B):
+    return [[sum_ordered(A[m][i] * B[i][n] for i in range(len(B))) for n in range(len(B[0]))]
+            for m in range(len(A))]
+
+
+def sum_ordered(it):
+    acc = 0.0
+    for x in it:
+        acc += x
+    return acc
+
+
+def posvel_line(S, i):
+    """pos_vel line of particle i (v25/functions.cpp:2602-2643)."""
+    m = S.mass[i]
+    r, p = S.r[i], S.p[i]
+    o = [list(row) for row in S.ori[i]]
+    I = S.inert[i]
+    I0 = [[1.0 / I[0], 0.0, 0.0], [0.0, 1.0 / I[1], 0.0], [0.0, 0.0, 1.0 / I[2]]]
+    ot = [[o[n][m_] for n in range(3)] for m_ in range(3)]
+    Ib = mm(mm(ot, I0), o)
+    om = mm(Ib, [[S.l[i][0]], [S.l[i][1]], [S.l[i][2]]])
+    vals = [r[0], r[1], r[2], p[0] / m, p[1] / m, p[2] / m]
+    vals2 = [o[2][0], o[2][1], o[2][2], om[0][0], om[1][0], om[2][0]]
+    return " ".join(f12(v) for v in vals) + " | " + " ".join(f12(v) for v in vals2)
+
+
+@pytest.fixture(scope="module")
+def v251_run():
+    S, P = v251_system()
+    R = O.ORun(S, P, seed_init=True)
+    posvel = {}
+    obs = np.zeros((101, O.OBS_N))
+    for step in range(101):
+        obs[step] = R.run(1)[0]
+        if step % 2 == 0:
+            posvel[step // 2] = [posvel_line(S, i) for i in range(25)]
+    yield S, P, R, obs, posvel
+    R.free()
+
+
+def test_output_dat_observables(v251_run):
+    S, P, R, obs, _ = v251_run
+    lines = open(os.path.join(GOLD, "v251_run1_output.txt")).read().splitlines()
+    assert len(lines) == 11
+    for k, line in enumerate(lines):
+        ref = line.split("|")[1].split()
+        mine = [f12(v) for v in obs[10 * k, :11]]
+        assert mine == ref, "step %d" % (10 * k)
+
+
+def test_conf_dat_state_and_cluster_labels(v251_run):
+    S, P, R, obs, _ = v251_run
+    lines = open(os.path.join(GOLD, "v251_run1_conf.txt")).read().splitlines()
+    nlab = 0
+    for i in range(N):
+        tok = lines[1 + i].split()
+        bar = tok.index("|")
+        mine = list(S.r[i]) + list(S.p[i]) + list(S.ori[i].ravel()) + list(S.l[i])
+        assert [f12(v) for v in mine] == tok[:bar], "particle %d" % i
+        assert int(tok[-1]) == S.clust[i], "cluster label of particle %d" % i
+        nlab = max(nlab, int(tok[-1]) + 1)
+    assert nlab == 478
+    for k in range(2):
+        s, strn = lines[501 + k].split()
+        assert f12(R.R.th[k].s) == s and f12(R.R.th[k].strn) == strn
+
+
+def test_pos_vel_snapshots(v251_run):
+    S, P, R, obs, posvel = v251_run
+    blocks = {}
+    cur = None
+    for line in open(os.path.join(GOLD, "v251_run1_posvel.txt")).read().splitlines():
+        if line.startswith("# file"):
+            cur = int(line.split()[2])
+            blocks[cur] = []
+        else:
+            blocks[cur].append(line)
+    assert len(blocks) == 51
+    for n in range(51):
+        assert posvel[n] == blocks[n][:25], "pos_vel_%d" % n
+        assert blocks[n][25] == f12(2 * 1e-4)
+
+
+def test_compconf_neighbour_reference(v251_run):
+    S, P, R, obs, _ = v251_run
+    ref = open(os.path.join(GOLD, "v251_run1_compConf.txt")).read().splitlines()
+    snap = np.ctypeslib.as_array(R.R.snap, shape=(N * 3,)).reshape(N, 3)
+    assert [" ".join(f12(v) for v in snap[i]) for i in range(N)] == ref
+
+
+def test_ini_pos_lattice():
+    S, P = v251_system()
+    assert O.lib().o_ini_pos(S.c()) == 0
+    ref = open(os.path.join(GOLD, "v251_run1_iniPos.txt")).read().splitlines()
+    for i in range(N):
+        tok = ref[i].split()
+        assert ["%g" % v for v in S.r[i]] == tok[:3], "particle %d" % i
+    assert ref[N].split() == ["%g" % v for v in S.L]
+
+
+def test_nve_drift_statistics_of_reference_run():
+    """v25/run1 (N=100 dipolar spheres, Q=1e12, gamma=0) conserves K+U per
+    particle within 4e-3 over 20001 steps; the figure the GPU NVE test is
+    held to is read from here."""
+    d = np.loadtxt(os.path.join(GOLD, "v25_run1_nve.txt"))
+    seg1 = d[d[:, 0] <= 6089]
+    E = seg1[:, 2] + seg1[:, 3]
+    drift = np.max(np.abs(E - E[0]))
+    assert 2e-3 < drift < 4e-3

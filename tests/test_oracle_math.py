@@ -1,0 +1,448 @@
+"""Pin oracle functions against mathematics the reference must obey.
+
+Covers what the whole-run fixtures leave open: branches of the segment
+distance that v251/run1 never reaches, the Ewald split as a whole
+(alpha independence), conservation laws, finite-difference derivatives,
+the free symmetric top, and the MT19937 stream.
+"""
+import ctypes
+import math
+
+import numpy as np
+import pytest
+from scipy.special import erfc
+
+from oracle import oracle as O
+
+L = O.lib()
+
+
+def dist(rA, uA, LA, rB, uB, LB, box=(1e6, 1e6, 1e6)):
+    rA, uA, rB, uB = (np.asarray(x, dtype=np.float64) for x in (rA, uA, rB, uB))
+    box = np.asarray(box, dtype=np.float64)
+    rAB = rA - rB
+    rAB -= box * np.floor(rAB / box + 0.5)
+    dAB = np.zeros(3)
+    st = np.zeros(2)
+    d2 = L.o_segdist(O.dptr(rA), O.dptr(uA), LA, O.dptr(rB), O.dptr(uB), LB, O.dptr(box),
+                     O.dptr(rAB), O.dptr(dAB), O.dptr(st))
+    return d2, dAB, st
+
+
+def brute_min(rA, uA, LA, rB, uB, LB):
+    """Minimum over a fine (s,t) grid refined by golden-section descent."""
+    ss = np.linspace(0, 1, 401)
+    PA = rA[None, :] + (ss[:, None] - 0.5) * LA * uA[None, :]
+    PB = rB[None, :] + (ss[:, None] - 0.5) * LB * uB[None, :]
+    D = ((PA[:, None, :] - PB[None, :, :]) ** 2).sum(-1)
+    i, j = np.unravel_index(np.argmin(D), D.shape)
+    s, t = ss[i], ss[j]
+    f = lambda s, t: np.sum((rA + (s - 0.5) * LA * uA - rB - (t - 0.5) * LB * uB) ** 2)
+    for _ in range(60):  # alternating exact 1-D minimisation on the convex QP
+        a = LA * LA; c = LB * LB; b = LA * LB * uA.dot(uB)
+        r = rA - rB
+        s = np.clip((-(r.dot(uA)) * LA + b * (t - 0.5)) / a + 0.5, 0, 1) if a > 0 else 0.5
+        t = np.clip((r.dot(uB) * LB + b * (s - 0.5)) / c + 0.5, 0, 1) if c > 0 else 0.5
+    return min(f(s, t), D.min())
+
+
+def unit(v):
+    v = np.asarray(v, dtype=np.float64)
+    return v / np.linalg.norm(v)
+
+
+# --------------------------------------------------------------------------
+# segment distance
+# --------------------------------------------------------------------------
+def test_segdist_generic_matches_brute_force():
+    rng = np.random.default_rng(7)
+    checked = 0
+    for _ in range(400):
+        rA, rB = rng.normal(size=3) * 2, rng.normal(size=3) * 2
+        uA, uB = unit(rng.normal(size=3)), unit(rng.normal(size=3))
+        LA, LB = rng.uniform(0.5, 5), rng.uniform(0.5, 5)
+        d2, dAB, st = dist(rA, uA, LA, rB, uB, LB)
+        assert st[0] > -1 and st[1] > -1  # generic branch
+        ref = brute_min(rA, uA, LA, rB, uB, LB)
+        assert d2 == pytest.approx(ref, rel=1e-9, abs=1e-12)
+        # the returned vector joins the two returned segment points
+        pA = rA + (st[0] - 0.5) * LA * uA
+        pB = rB + (st[1] - 0.5) * LB * uB
+        assert np.allclose(dAB, pA - pB, atol=1e-12)
+        checked += 1
+    assert checked == 400
+
+
+def test_segdist_parallel_branches_and_sentinels():
+    u = np.array([0.0, 0.0, 1.0])
+    # side by side, overlapping projections: non-unique, sentinel -1e6
+    d2, dAB, st = dist([1.3, 0, 0.4], u, 3.0, [0, 0, 0], u, 3.0)
+    assert d2 == pytest.approx(1.3 ** 2) and st[0] == -1e6 and st[1] == -1e6
+    assert np.allclose(dAB, [1.3, 0, 0])
+    # head A to tail B (+/+) and tail A to head B (-/-)
+    d2, dAB, st = dist([0, 0, -5.0], u, 3.0, [0, 0, 0], u, 3.0)
+    assert st[0] == -1e10 and d2 == pytest.approx(4.0)
+    d2, dAB, st = dist([0, 0, 5.0], u, 3.0, [0, 0, 0], u, 3.0)
+    assert st[0] == -2e10 and d2 == pytest.approx(4.0)
+    # anti-parallel: head-head (+/-) and tail-tail (-/+)
+    d2, dAB, st = dist([0, 0, -5.0], u, 3.0, [0, 0, 0], -u, 3.0)
+    assert st[0] == -3e10 and d2 == pytest.approx(4.0)
+    d2, dAB, st = dist([0, 0, 5.0], u, 3.0, [0, 0, 0], -u, 3.0)
+    assert st[0] == -4e10 and d2 == pytest.approx(4.0)
+
+
+def test_segdist_spheres_reduce_to_centre_distance():
+    rng = np.random.default_rng(3)
+    for _ in range(50):
+        rA, rB = rng.normal(size=3), rng.normal(size=3)
+        d2, _, _ = dist(rA, unit(rng.normal(size=3)), 0.0, rB, unit(rng.normal(size=3)), 0.0)
+        assert d2 == pytest.approx(np.sum((rA - rB) ** 2), rel=1e-14)
+
+
+def test_segdist_crossing_rods_is_line_distance():
+    # perpendicular rods crossing above each other at height h
+    for h in [0.3, 1.0, 2.5]:
+        d2, _, st = dist([0, 0, h], [1, 0, 0], 4.0, [0, 0, 0], [0, 1, 0], 4.0)
+        assert d2 == pytest.approx(h * h, rel=1e-14)
+        assert st[0] == pytest.approx(0.5) and st[1] == pytest.approx(0.5)
+
+
+def test_segdist_intersecting_lines_branch_is_an_upper_bound():
+    # lines meet at a point outside both segments: the reference's endpoint
+    # search returns a distance between points on the segments
+    # on segments, hence never below the true minimum; it can exceed the
+    # centre distance (the case force_sc aborts on, v25/functions.cpp:1466)
+    rng = np.random.default_rng(11)
+    above_centre = 0
+    for _ in range(100):
+        uA, uB = unit(rng.normal(size=3)), unit(rng.normal(size=3))
+        X = rng.normal(size=3)
+        LA, LB = rng.uniform(1, 3), rng.uniform(1, 3)
+        rA = X + uA * rng.uniform(-4, 4)
+        rB = X + uB * rng.uniform(-4, 4)
+        d2, dAB, st = dist(rA, uA, LA, rB, uB, LB)
+        true = brute_min(rA, uA, LA, rB, uB, LB)
+        assert d2 >= true - 1e-9
+        assert np.sum((rA - rB) ** 2) >= true - 1e-9
+        above_centre += d2 > np.sum((rA - rB) ** 2) * 1.00001
+    assert above_centre > 0
+
+
+# --------------------------------------------------------------------------
+# tables
+# --------------------------------------------------------------------------
+def test_tabul_closed_form():
+    alpha = 0.31
+    T = np.zeros(32768 * 3)
+    L.o_tabul(O.dptr(T), alpha)
+    T = T.reshape(-1, 3)
+    i = np.array([1, 7, 100, 3000, 32768])
+    r = 0.5 * 200 * 1.001 * i / 32768
+    aer = 2 * alpha * np.exp(-alpha ** 2 * r ** 2) / np.sqrt(np.pi)
+    B = erfc(alpha * r) / r ** 3 + aer / r ** 2
+    C = (3 * erfc(alpha * r) / r ** 3 + aer * (2 * alpha ** 2 + 3 / r ** 2)) / r ** 2
+    D = 4 * alpha ** 4 * aer
+    assert np.allclose(T[i - 1, 0], B, rtol=1e-13, atol=0)
+    assert np.allclose(T[i - 1, 1], C, rtol=1e-13, atol=0)
+    assert np.allclose(T[i - 1, 2], D, rtol=1e-13, atol=1e-300)
+
+
+def test_wave_v_closed_form():
+    K, alpha = 6, 0.4
+    box = np.array([10.0, 11.0, 12.0])
+    KV = np.zeros((K + 1) ** 3)
+    L.o_wave_v(O.dptr(KV), O.dptr(box), alpha, K)
+    KV = KV.reshape(K + 1, K + 1, K + 1)
+    n = np.indices((K + 1,) * 3).reshape(3, -1).T
+    k2 = ((2 * np.pi * n / box) ** 2).sum(1)
+    m = k2 > 0
+    ref = 4 * np.pi * np.exp(-k2[m] / (4 * alpha ** 2)) / k2[m]
+    assert np.allclose(KV.reshape(-1)[m], ref, rtol=1e-14)
+    assert KV[0, 0, 0] == 0.0
+
+
+# --------------------------------------------------------------------------
+# small systems for force checks
+# --------------------------------------------------------------------------
+def make_sys(r, u, lens, box, mu=None, dia=None):
+    N = len(r)
+    S = O.OSys(N, 1, L=box)
+    S.r[:] = r
+    for i in range(N):
+        S.ori[i] = frame(u[i])
+    S.len[:] = lens
+    S.dia[:] = 1.0 if dia is None else dia
+    S.mass[:] = 1.0
+    S.dmu[:, 0] = 1.0 if mu is None else mu
+    S.set_inertia()
+    S.set_dip()
+    return S
+
+
+def frame(u):
+    u = unit(u)
+    a = np.array([1.0, 0, 0]) if abs(u[0]) < 0.9 else np.array([0, 1.0, 0])
+    e1 = unit(np.cross(a, u))
+    e0 = np.cross(e1, u)
+    return np.array([e0, e1, u])
+
+
+def rotmat(axis, ang):
+    axis = unit(axis)
+    K = np.array([[0, -axis[2], axis[1]], [axis[2], 0, -axis[0]], [-axis[1], axis[0], 0]])
+    return np.eye(3) + math.sin(ang) * K + (1 - math.cos(ang)) * K @ K
+
+
+def energy_forces(S, kind, alpha=0.5, K=6, rc=0.0, lists=None):
+    S.F[:] = 0
+    S.T[:] = 0
+    U = ctypes.c_double()
+    W = ctypes.c_double()
+    PT = np.zeros(9)
+    if kind == "sc":
+        off, nbr = lists
+        rc_ = L.o_force_sc(S.c(), 10.0, O.iptr(off), O.iptr(nbr), ctypes.byref(U), ctypes.byref(W), O.dptr(PT))
+        assert rc_ == 0
+    elif kind == "R":
+        T = np.zeros(32768 * 3)
+        L.o_tabul(O.dptr(T), alpha)
+        L.o_force_dipR(S.c(), O.dptr(T), rc, ctypes.byref(U), O.dptr(PT))
+    elif kind == "K":
+        KV = np.zeros((K + 1) ** 3)
+        L.o_wave_v(O.dptr(KV), O.dptr(S.L), alpha, K)
+        L.o_force_dipK(S.c(), K, alpha, O.dptr(KV), ctypes.byref(U), O.dptr(PT))
+    return U.value, S.F.copy(), S.T.copy()
+
+
+def full_list(N):
+    off = np.zeros(N + 1, dtype=np.int32)
+    nbr = []
+    for i in range(N):
+        off[i] = len(nbr)
+        nbr += list(range(i + 1, N))
+    off[N] = len(nbr)
+    return off, np.array(nbr, dtype=np.int32)
+
+
+def random_rods(N, box, rng, lens=3.0, mind=1.05):
+    r, u = [], []
+    while len(r) < N:
+        x = rng.uniform(-box / 2, box / 2, size=3)
+        v = unit(rng.normal(size=3))
+        ok = True
+        for y, w in zip(r, u):
+            d2, _, _ = dist(x, v, lens, y, w, lens, (box, box, box))
+            if d2 < mind ** 2:
+                ok = False
+                break
+        if ok:
+            r.append(x)
+            u.append(v)
+    return np.array(r), np.array(u)
+
+
+def test_wca_closed_form_side_by_side():
+    for d in [0.9, 1.0, 1.05, 1.1]:
+        S = make_sys([[0, 0, 0], [d, 0, 0]], [[0, 0, 1], [0, 0, 1]], 3.0, (50, 50, 50))
+        U, F, T = energy_forces(S, "sc", lists=full_list(2))
+        x6 = (1 / d) ** 6
+        assert U == pytest.approx(4 * x6 * (x6 - 1) + 1, rel=1e-13)
+        assert F[1, 0] == pytest.approx(48 / d * (x6 * x6 - 0.5 * x6), rel=1e-12)  # repulsive
+        assert np.allclose(F[0] + F[1], 0, atol=1e-14) and np.allclose(T, 0, atol=1e-14)
+    # beyond 2^(1/6) sigma nothing acts
+    S = make_sys([[0, 0, 0], [1.13, 0, 0]], [[0, 0, 1], [0, 0, 1]], 3.0, (50, 50, 50))
+    U, F, T = energy_forces(S, "sc", lists=full_list(2))
+    assert U == 0 and np.all(F == 0)
+
+
+def contact_pair_system(rng, N, box, lens):
+    """N rods of which 0 and 1 sit inside the WCA range of each other."""
+    while True:
+        r, u = random_rods(N, box, rng, lens=lens, mind=1.3)
+        perp = unit(np.cross(u[0], rng.normal(size=3)))
+        r[1] = r[0] + 1.02 * perp + 0.2 * u[0]
+        u[1] = unit(u[0] + 0.15 * rng.normal(size=3))
+        ok = True
+        for i in range(N):
+            for j in range(i + 1, N):
+                d2, _, _ = dist(r[i], u[i], lens, r[j], u[j], lens, (box, box, box))
+                ok &= d2 > 0.85 ** 2
+        d2, _, _ = dist(r[0], u[0], lens, r[1], u[1], lens, (box, box, box))
+        if ok and d2 < 1.1 ** 2:
+            return r, u
+
+
+@pytest.mark.parametrize("kind", ["sc", "R", "K"])
+def test_forces_torques_are_gradients(kind):
+    """F = -dU/dr and tau.e = -dU/dtheta for rotations about e, by central
+    differences, on a small periodic system.  The real-space energy is a
+    linear interpolation in r (step 3.06e-3), so its difference quotient
+    matches the analytic force only to O(step/r)."""
+    rng = np.random.default_rng({"sc": 1, "R": 2, "K": 3}[kind])
+    box = 9.0
+    N = 6
+    if kind == "sc":
+        r, u = contact_pair_system(rng, N, box, 2.0)
+    else:
+        r, u = random_rods(N, box, rng, lens=2.0, mind=1.2)
+    S = make_sys(r, u, 2.0, (box, box, box), mu=rng.uniform(1, 2, size=N))
+    lists = full_list(N)
+    U0, F0, T0 = energy_forces(S, kind, lists=lists)
+    assert np.abs(F0).max() > 1e-3
+    h = 1e-6
+    tol = 3e-3 if kind == "R" else 1e-6
+    for i in range(N):
+        for a in range(3):
+            Sp = S.copy(); Sp.r[i, a] += h; Sp.set_dip()
+            Sm = S.copy(); Sm.r[i, a] -= h; Sm.set_dip()
+            dU = (energy_forces(Sp, kind, lists=lists)[0] - energy_forces(Sm, kind, lists=lists)[0]) / (2 * h)
+            assert -dU == pytest.approx(F0[i, a], rel=tol, abs=tol * (1 + np.abs(F0).max()))
+        for e in np.eye(3):
+            Sp = S.copy(); Sp.ori[i] = Sp.ori[i] @ rotmat(e, h).T; Sp.set_dip()
+            Sm = S.copy(); Sm.ori[i] = Sm.ori[i] @ rotmat(e, -h).T; Sm.set_dip()
+            dU = (energy_forces(Sp, kind, lists=lists)[0] - energy_forces(Sm, kind, lists=lists)[0]) / (2 * h)
+            assert -dU == pytest.approx(T0[i].dot(e), rel=tol, abs=tol * (1 + np.abs(T0).max()))
+
+
+@pytest.mark.parametrize("kind", ["sc", "R", "K"])
+def test_momentum_conservation(kind):
+    rng = np.random.default_rng(5)
+    box = 10.0
+    if kind == "sc":
+        r, u = contact_pair_system(rng, 8, box, 2.0)
+    else:
+        r, u = random_rods(8, box, rng, lens=2.0, mind=1.0)
+    S = make_sys(r, u, 2.0, (box, box, box), mu=rng.uniform(1, 2, size=8))
+    U, F, T = energy_forces(S, kind, lists=full_list(8))
+    assert np.abs(F.sum(0)).max() < 1e-11 * (1 + np.abs(F).max())
+
+
+def test_reciprocal_self_exclusion_single_particle():
+    S = make_sys([[0.3, -1.2, 2.0]], [[0.2, 0.5, 0.8]], 3.0, (8, 8, 8), mu=1.7)
+    U, F, T = energy_forces(S, "K", alpha=0.8, K=8)
+    assert abs(U) < 1e-15 and np.abs(F).max() < 1e-14
+
+
+def test_real_space_bare_dipoles_small_alpha():
+    """alpha -> 0 reduces the screened kernel to the bare dipole pair."""
+    r = np.array([[0, 0, 0], [1.3, 0.7, -0.4]])
+    u = np.array([unit([0.3, 0.1, 1]), unit([-0.5, 1, 0.2])])
+    S = make_sys(r, u, 0.0, (200, 200, 200), mu=[1.2, 0.9])
+    U, F, T = energy_forces(S, "R", alpha=1e-7)
+    d = r[0] - r[1]
+    R = np.linalg.norm(d)
+    ref = 1.2 * 0.9 * (u[0].dot(u[1]) / R ** 3 - 3 * u[0].dot(d) * u[1].dot(d) / R ** 5)
+    assert U == pytest.approx(ref, rel=2e-5)
+
+
+def test_ewald_split_is_alpha_independent():
+    """U_R + U_K and F_R + F_K do not depend on alpha once both sums have
+    converged (the reference omits only alpha-independent self terms)."""
+    rng = np.random.default_rng(9)
+    box = 8.0
+    r, u = random_rods(6, box, rng, lens=1.0, mind=1.0)
+    mu = rng.uniform(1, 2, size=6)
+    out = []
+    for alpha in [1.0, 1.2]:
+        S = make_sys(r, u, 1.0, (box, box, box), mu=mu)
+        UR, FR, TR = energy_forces(S, "R", alpha=alpha)
+        UK, FK, TK = energy_forces(S, "K", alpha=alpha, K=14)
+        out.append((UR + UK, FR + FK, TR + TK))
+    assert out[0][0] == pytest.approx(out[1][0], rel=1e-6, abs=1e-7)
+    assert np.allclose(out[0][1], out[1][1], rtol=1e-5, atol=1e-6)
+    assert np.allclose(out[0][2], out[1][2], rtol=1e-5, atol=1e-6)
+
+
+def test_pair_torques_balance_angular_momentum():
+    """tau_i + tau_j + r_ij x F_i = 0 for an isolated pair (central + dipolar)."""
+    rng = np.random.default_rng(4)
+    for kind in ["sc", "R"]:
+        r = np.array([[0, 0, 0], [0.4, 1.0, 0.3]]) if kind == "sc" else rng.normal(size=(2, 3)) * 2
+        u = np.array([unit(rng.normal(size=3)) for _ in range(2)])
+        S = make_sys(r, u, 1.0 if kind == "sc" else 0.0, (100, 100, 100), mu=[1.1, 1.4])
+        U, F, T = energy_forces(S, kind, alpha=1e-6, lists=full_list(2))
+        rij = r[0] - r[1]
+        tot = T[0] + T[1] + np.cross(rij, F[0])
+        assert np.abs(tot).max() < 1e-9 * (1 + np.abs(T).max() + np.abs(F).max())
+
+
+# --------------------------------------------------------------------------
+# integrator pieces
+# --------------------------------------------------------------------------
+def test_free_symmetric_top_precession():
+    """With no torque the lab angular momentum is fixed and the axis of a
+    symmetric top precesses about it at |l|/I_perp."""
+    S = make_sys([[0, 0, 0]], [unit([0.3, 0.2, 1.0])], 3.0, (50, 50, 50), mu=0.0)
+    l = np.array([0.4, -0.2, 1.1])
+    S.l[0] = l
+    u0 = S.ori[0, 2].copy()
+    dt, n = 1e-3, 2000
+    for _ in range(n):
+        L.o_moveX(S.c(), dt)
+    w = np.linalg.norm(l) / S.inert[0, 0]
+    R = rotmat(l, w * dt * n)
+    assert np.allclose(S.ori[0, 2], R @ u0, atol=1e-6)
+    assert np.allclose(S.ori[0] @ S.ori[0].T, np.eye(3), atol=1e-12)
+    assert np.array_equal(S.l[0], l)
+
+
+def test_round12_matches_decimal_text_round_trip():
+    rng = np.random.default_rng(0)
+    xs = np.concatenate([rng.uniform(-200, 200, 2000), [1 / 8192 * k for k in range(1, 200, 2)],
+                         [-3.0000000000005, 0.0, -0.0, 12.5e-13]])
+    for x in xs:
+        assert L.o_round12(float(x)) == float("%.12f" % x)
+
+
+def test_mt19937_known_answers():
+    g = O.COmt()
+    L.omt_seed(ctypes.byref(g), 5489)
+    for _ in range(9999):
+        L.omt_u32(ctypes.byref(g))
+    # C++ [rand.predef]: the 10000th output of default-seeded mt19937
+    assert L.omt_u32(ctypes.byref(g)) == 4123659995
+    for seed in [1, 2, 12345]:
+        g = O.COmt()
+        L.omt_seed(ctypes.byref(g), seed)
+        mine = [L.omt_u32(ctypes.byref(g)) for _ in range(700)]
+        ref = np.random.RandomState(seed).randint(0, 2 ** 32, size=700, dtype=np.uint32)
+        assert mine == [int(v) for v in ref]
+    g = O.COmt()
+    L.omt_seed(ctypes.byref(g), 0)  # GSL maps seed 0 to 4357
+    h = O.COmt()
+    L.omt_seed(ctypes.byref(h), 4357)
+    assert L.omt_u32(ctypes.byref(g)) == L.omt_u32(ctypes.byref(h))
+
+
+def test_clusters_are_contact_components():
+    rng = np.random.default_rng(2)
+    box = 14.0
+    r, u = random_rods(40, box, rng, lens=2.0, mind=1.0)
+    S = make_sys(r, u, 2.0, (box, box, box))
+    g = O.COmt()
+    L.omt_seed(ctypes.byref(g), 1)
+    n = L.o_clust_conf(S.c(), 0.4, ctypes.byref(g))
+    # union-find over the same contact test
+    parent = list(range(40))
+    def find(a):
+        while parent[a] != a:
+            parent[a] = parent[parent[a]]
+            a = parent[a]
+        return a
+    for i in range(40):
+        for j in range(i + 1, 40):
+            if L.o_dist_test(S.c(), i, j, 0.4):
+                parent[find(i)] = find(j)
+    comp = {}
+    for i in range(40):
+        comp.setdefault(find(i), set()).add(i)
+    lab = {}
+    for i in range(40):
+        lab.setdefault(int(S.clust[i]), set()).add(i)
+    assert n == len(comp) == len(lab)
+    assert sorted(map(sorted, comp.values())) == sorted(map(sorted, lab.values()))
+    # labels are creation order: the first seed is floor(u1 * 40)
+    g = O.COmt()
+    L.omt_seed(ctypes.byref(g), 1)
+    first = int(math.floor(L.omt_uniform(ctypes.byref(g)) * 40))
+    assert S.clust[first] == 0
