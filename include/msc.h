@@ -1,0 +1,176 @@
+/*
+ * msc.h -- C ABI of libmsc_hip.so, the MI355X implementation of mdSpherocyl's
+ * per-step force path (neighbour list, short-range spherocylinder force,
+ * Ewald dipolar real-space and reciprocal sums) and the step loop around it.
+ *
+ * Plain C: pointers, sizes and status codes.  No exceptions cross this
+ * boundary.  A context owns one HIP stream on one device; every call is
+ * ordered on that stream and returns after the result it reports is ready.
+ * Host arrays passed in are read during the call only.
+ *
+ * Layout of host arrays (one replica, particles in the reference's global-id
+ * order, i.e. species-major as rev_II at v25/functions.cpp:1166-1179):
+ *   r, p, l, F, T : [N][3]     ori : [N][3][3] row-major, row 2 = axis u
+ *   len, dia, mass: [N]        dpc, dmu : [N][nd]     doc : [N][nd][3]
+ * A context may hold several independent replicas ("systems"); most entry
+ * points take a replica index, MSC_ALL meaning every replica.
+ *
+ * Conventions follow the reference routines each entry point replaces:
+ * force routines ADD into F and T (the caller zeroes them, v25/main.cpp:582-588,
+ * here msc_zero_forces) and OVERWRITE the energy / virial / pressure-tensor
+ * outputs; energies are totals, not per particle.
+ */
+#ifndef MSC_H
+#define MSC_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define MSC_OK          0
+#define MSC_E_OVERLAP   1   /* force_sc: segment distance^2 < 0.49 sigma^2  */
+#define MSC_E_DISTANCE  2   /* force_sc: segment distance^2 > 1.00001 r^2   */
+#define MSC_E_ARG       3   /* bad argument                                  */
+#define MSC_E_HIP       4   /* HIP runtime error                             */
+#define MSC_E_CAPACITY  5   /* a device buffer bound was exceeded            */
+#define MSC_E_STATE     6   /* call order (e.g. force before neighbour list) */
+
+/* precision of pair and reciprocal arithmetic */
+#define MSC_FP64  0         /* plain FP64 throughout (drop-in path)           */
+#define MSC_MIXED 1         /* FP32 pair/reciprocal arithmetic, FP64 state,
+                               reductions, thermostat and rotation            */
+
+#define MSC_ALL  (-1)
+
+/* observables per replica and step, as printed by print_out
+ * (v25/functions.cpp:2830-2853) plus the list-rebuild flag */
+#define MSC_OBS_K      0
+#define MSC_OBS_U      1
+#define MSC_OBS_P      2
+#define MSC_OBS_USS    3
+#define MSC_OBS_PSS    4
+#define MSC_OBS_UDR    5
+#define MSC_OBS_PDR    6
+#define MSC_OBS_UDK    7
+#define MSC_OBS_PDK    8
+#define MSC_OBS_MOM2   9
+#define MSC_OBS_ANG2   10
+#define MSC_OBS_REBUILD 11
+#define MSC_OBS_N      12
+
+typedef struct msc_ctx msc_ctx;
+
+/* Parameters of one replica.  Fields map onto input.txt (r_input,
+ * v25/functions.cpp:10-87) as noted. */
+typedef struct msc_replica_desc {
+  int N;              /* particles ("No. of Particles", summed)            */
+  int nd;             /* dipole slots per particle (1..8)                  */
+  double L[3];        /* box                                               */
+  double alpha;       /* "Convergence param(a)"                            */
+  int K;              /* "No. of Kvectors": |n_a| <= K-1                   */
+  double sr_cut;      /* "Cutoff dist(SRange)"                             */
+  double rc;          /* real-space Ewald cutoff; <= 0: every minimum-image
+                         pair, as force_dipR (v25/functions.cpp:1570-1600) */
+  double B[3];        /* "Ext Mag.Field"                                   */
+  double temp;        /* "Temperature"                                     */
+  double dt;          /* "Time step size"                                  */
+  double Q;           /* "Thermostat\"mass\""                               */
+  double gamma;       /* "Thermostat\"shearRate\""                          */
+  double s0;          /* initial thermostat coordinate (0 in v25, 1 in v251) */
+  double dens;        /* "Density" (number density, enters P only)         */
+  double dmin, dmax;  /* smallest diameter, largest L+D over species
+                         (v25/main.cpp:153-158); <= 0: taken from the state */
+  unsigned long thermo_seed[2]; /* MT19937 seeds (1 and 2 in v25/main.cpp:183,186) */
+} msc_replica_desc;
+
+/* ---- context ---------------------------------------------------------- */
+int  msc_create(int device, int precision, int nrep, const msc_replica_desc *reps,
+                msc_ctx **out);
+void msc_destroy(msc_ctx *ctx);
+const char *msc_last_error(const msc_ctx *ctx);
+int  msc_version(void);
+
+/* ---- state ------------------------------------------------------------ */
+/* Upload the state of replica `rep` (host arrays, layout above).  Dipole lab
+ * positions and orientations are derived as Spherocyl::set_dip does
+ * (v25/classes.cpp:148-173).  Per-particle inertia follows set_inertia
+ * (v25/classes.cpp:175-185). */
+int msc_set_state(msc_ctx *ctx, int rep, const double *r, const double *p,
+                  const double *l, const double *ori, const double *len,
+                  const double *dia, const double *mass, const double *dpc,
+                  const double *doc, const double *dmu);
+int msc_get_state(msc_ctx *ctx, int rep, double *r, double *p, double *l, double *ori);
+int msc_get_forces(msc_ctx *ctx, int rep, double *F, double *T);
+int msc_set_forces(msc_ctx *ctx, int rep, const double *F, const double *T);
+/* thermostat coordinate s and stored normal deviate of the two thermostats */
+int msc_get_thermostat(msc_ctx *ctx, int rep, double s[2], double strn[2]);
+int msc_set_thermostat(msc_ctx *ctx, int rep, const double s[2], const double strn[2]);
+
+/* ---- routines of v25/functions.hpp:29-54 ------------------------------ */
+/* neigh_list (v25/functions.cpp:1298-1334): build the neighbour pairs with
+ * segment distance < r_list on the device; savec_list snapshot included. */
+int msc_neigh_list(msc_ctx *ctx, int rep, double r_list);
+/* The list in the reference's form: for each f, ascending g > f.  off has
+ * N+1 entries; nbr holds cap entries; *total receives the pair count. */
+int msc_get_neigh_list(msc_ctx *ctx, int rep, int *off, int *nbr, long cap, long *total);
+/* check_list (v25/functions.cpp:1356-1394) against the savec_list snapshot */
+int msc_check_list(msc_ctx *ctx, int rep, double r_list_min, double r_list_max,
+                   int *rebuild);
+/* force_sc (v25/functions.cpp:1397-1553); MSC_E_OVERLAP / MSC_E_DISTANCE
+ * where the reference returns false */
+int msc_force_sc(msc_ctx *ctx, int rep, double r_cut, double *U, double *W, double PT[9]);
+/* force_dipR (v25/functions.cpp:1555-1678), table of tabul (1680-1704) */
+int msc_force_dipR(msc_ctx *ctx, int rep, double *U, double PT[9]);
+/* force_dipK (v25/functions.cpp:1706-1961), weights of wave_v (1963-1990) */
+int msc_force_dipK(msc_ctx *ctx, int rep, double *U, double PT[9]);
+/* ext_field (v25/functions.cpp:1992-2025) */
+int msc_ext_field(msc_ctx *ctx, int rep, double *U);
+/* self_ener (v25/functions.cpp:2027-2069) */
+int msc_self_ener(msc_ctx *ctx, int rep, double *U);
+int msc_zero_forces(msc_ctx *ctx, int rep);
+/* moveX (v25/functions.cpp:2071-2092) */
+int msc_moveX(msc_ctx *ctx, int rep, double dt);
+/* moveP (v25/functions.cpp:2094-2184) with the replica's thermostat
+ * coordinates; K[2] kinetic energies, PT the kinetic pressure tensor */
+int msc_moveP(msc_ctx *ctx, int rep, double dt, int last, double K[2], double PT[9]);
+/* renorm every particle (v25/classes.cpp:307-325) */
+int msc_renorm(msc_ctx *ctx, int rep);
+
+/* ---- the step loop (v25/main.cpp:520-708) ------------------------------ */
+/* Initial list, forces and self energy as v25/main.cpp:460-497. */
+int msc_init_run(msc_ctx *ctx);
+/* Advance every replica by nsteps steps.  obs (may be NULL) receives
+ * [nsteps][nrep][MSC_OBS_N] values.  step0 is the global step number of the
+ * first step (renormalisation every 100 steps counts from it). */
+int msc_run(msc_ctx *ctx, int step0, int nsteps, double *obs);
+/* 1: replay each step as one captured HIP graph (default), 0: launch
+ * kernels directly. */
+int msc_set_graph(msc_ctx *ctx, int on);
+
+/* ---- cluster labels (v25/functions.cpp:2950-3324) ----------------------
+ * Components of the contact graph d_seg^2 < ((D_i+D_j)/2 + min_s)^2, labelled
+ * in the order the reference's clust_conf creates them.  mt_state/mt_index
+ * hold the MT19937 state of the reference's `ran` generator (624 words + the
+ * position), advanced in place by the draws rnd_part makes.  Returns the
+ * number of clusters in *nclust. */
+int msc_cluster_labels(msc_ctx *ctx, int rep, double min_s, unsigned int *mt_state,
+                       int *mt_index, int *labels, int *nclust);
+
+/* ---- measurement -------------------------------------------------------
+ * With timing on, the library brackets each launch of the named kernels with
+ * HIP events on its stream; msc_kernel_time returns the summed milliseconds
+ * and launch count since the last reset.  Names: "recip_struct",
+ * "recip_force", "real_pairs", "wca", "nlist", "integrate". */
+int msc_set_timing(msc_ctx *ctx, int on);
+int msc_kernel_time(msc_ctx *ctx, const char *name, double *ms, long *launches);
+int msc_reset_timing(msc_ctx *ctx);
+/* counts of the algorithmic work units of the last force evaluation of a
+ * replica: [0] WCA pairs inside 2^(1/6) sigma, [1] real-space dipole pairs
+ * inside the cutoff, [2] reciprocal k-vectors (half space), [3] list pairs */
+int msc_work_counts(msc_ctx *ctx, int rep, long counts[4]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

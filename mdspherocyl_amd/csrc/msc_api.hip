@@ -1,0 +1,1140 @@
+// Host side of libmsc_hip.so: context, replica layout, buffers and the C ABI
+// declared in include/msc.h.  The step driver restates the order of the
+// reference step loop (v25/main.cpp:559-708) over device kernels.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/msc.h"
+#include "msc_kernels.hpp"
+
+using namespace msc;
+
+namespace {
+
+// MT19937 (published algorithm, GSL seeding) for the thermostat streams and
+// the cluster seed draws.
+struct MT {
+  uint32_t mt[624];
+  int mti = 625;
+  void seed(unsigned long s) {
+    if (s == 0) s = 4357;
+    mt[0] = (uint32_t)(s & 0xffffffffUL);
+    for (int i = 1; i < 624; ++i) mt[i] = (uint32_t)(1812433253UL * (mt[i - 1] ^ (mt[i - 1] >> 30)) + (uint32_t)i);
+    mti = 624;
+  }
+  uint32_t next() {
+    const uint32_t UP = 0x80000000u, LO = 0x7fffffffu, mag[2] = {0u, 0x9908b0dfu};
+    if (mti >= 624) {
+      int k;
+      for (k = 0; k < 624 - 397; ++k) {
+        uint32_t y = (mt[k] & UP) | (mt[k + 1] & LO);
+        mt[k] = mt[k + 397] ^ (y >> 1) ^ mag[y & 1u];
+      }
+      for (; k < 623; ++k) {
+        uint32_t y = (mt[k] & UP) | (mt[k + 1] & LO);
+        mt[k] = mt[k + (397 - 624)] ^ (y >> 1) ^ mag[y & 1u];
+      }
+      uint32_t y = (mt[623] & UP) | (mt[0] & LO);
+      mt[623] = mt[396] ^ (y >> 1) ^ mag[y & 1u];
+      mti = 0;
+    }
+    uint32_t y = mt[mti++];
+    y ^= (y >> 11);
+    y ^= (y << 7) & 0x9d2c5680u;
+    y ^= (y << 15) & 0xefc60000u;
+    y ^= (y >> 18);
+    return y;
+  }
+  double uniform() { return next() / 4294967296.0; }
+};
+
+// Box-Muller pair as gauss(1, r1, r2) (v25/functions.cpp:1149-1156)
+void gauss1(double &r1, double &r2) {
+  double a = std::sqrt(-2 * std::log(r1)) * std::cos(2 * M_PI * r2);
+  double b = std::sqrt(-2 * std::log(r1)) * std::sin(2 * M_PI * r2);
+  double sv = std::sqrt(1.0);
+  r1 = a * sv;
+  r2 = b * sv;
+}
+
+struct TimerSet {
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+  double ms = 0;
+  long launches = 0;
+};
+
+}  // namespace
+
+struct msc_ctx {
+  int device = 0, precision = MSC_FP64;
+  bool mixed = false;
+  hipStream_t stream = nullptr;
+  int nrep = 0, Ntot = 0, nd = 1;
+  int rchunk = 256;           // particles per structure-factor chunk task
+  std::vector<msc_replica_desc> desc;
+  std::vector<RepDev> reps;
+  std::vector<Task> tasks, rtasks;
+  std::vector<MT> thermo_rng;  // 2 per replica
+  std::vector<int> kz_groups;  // distinct KZ values
+  std::string err;
+  State S{};
+  bool have_state = false, have_lists = false, graph_on = false, timing = false;
+  long nl_cap = 0, dl_cap = 0;
+  size_t part_elems = 0;
+  int nslot_tot = 0, maxslot = 0, maxcol = 0;
+  // device buffers
+  RepDev *d_reps = nullptr;
+  Task *d_tasks = nullptr, *d_rtasks = nullptr;
+  int *d_rep_of = nullptr, *d_cell_rep = nullptr;
+  double *d_x = nullptr, *d_p = nullptr, *d_l = nullptr, *d_ori = nullptr, *d_F = nullptr, *d_T = nullptr;
+  double *d_len = nullptr, *d_dia = nullptr, *d_mass = nullptr, *d_inert = nullptr;
+  double *d_dpc = nullptr, *d_doc = nullptr, *d_dmu = nullptr, *d_dpos = nullptr, *d_dori = nullptr, *d_snap = nullptr;
+  int *d_cell_of = nullptr, *d_cell_cnt = nullptr, *d_cell_start = nullptr, *d_cell_fill = nullptr,
+      *d_cell_list = nullptr;
+  int *d_nl_cnt = nullptr, *d_nl_start = nullptr, *d_nl_idx = nullptr;
+  int *d_dl_cnt = nullptr, *d_dl_start = nullptr, *d_dl_idx = nullptr;
+  int ncells = 0;
+  double *d_kval = nullptr;
+  void *d_tab = nullptr, *d_ph = nullptr, *d_amp = nullptr, *d_part = nullptr, *d_A = nullptr;
+  double *d_slot_out = nullptr, *d_task_out = nullptr, *d_red = nullptr, *d_M2 = nullptr, *d_Kin = nullptr,
+         *d_mom = nullptr, *d_gs = nullptr, *d_obs = nullptr, *d_counts = nullptr;
+  RepEnergy *d_E = nullptr;
+  Thermo *d_th = nullptr;
+  int *d_flags = nullptr, *d_rebuild = nullptr, *d_one = nullptr;
+  std::map<std::string, TimerSet> timers;
+  std::vector<hipEvent_t> event_pool;
+  hipGraphExec_t graph_exec = nullptr;
+  int graph_renorm = -1;
+};
+
+#define HIPC(expr)                                                                   \
+  do {                                                                               \
+    hipError_t e_ = (expr);                                                          \
+    if (e_ != hipSuccess) {                                                          \
+      ctx->err = std::string(#expr) + ": " + hipGetErrorString(e_);                  \
+      return MSC_E_HIP;                                                              \
+    }                                                                                \
+  } while (0)
+
+namespace {
+
+constexpr int kEoff_sc = 0, kEoff_R = 11, kEoff_K = 21, kEoff_ext = 31;
+constexpr int kEstride = (int)(sizeof(RepEnergy) / sizeof(double));
+
+int fail(msc_ctx *ctx, int code, const std::string &m) {
+  ctx->err = m;
+  return code;
+}
+
+template <typename T>
+int dalloc(msc_ctx *ctx, T **p, size_t n) {
+  if (*p) hipFree(*p);
+  *p = nullptr;
+  if (n == 0) n = 1;
+  HIPC(hipMalloc((void **)p, n * sizeof(T)));
+  HIPC(hipMemsetAsync(*p, 0, n * sizeof(T), ctx->stream));
+  return MSC_OK;
+}
+
+// event bracket around a launch when timing is on
+struct Bracket {
+  msc_ctx *c;
+  const char *name;
+  hipEvent_t a = nullptr, b = nullptr;
+  Bracket(msc_ctx *c_, const char *n) : c(c_), name(n) {
+    if (!c->timing) return;
+    hipEventCreate(&a);
+    hipEventCreate(&b);
+    hipEventRecord(a, c->stream);
+  }
+  ~Bracket() {
+    if (!c->timing) return;
+    hipEventRecord(b, c->stream);
+    c->timers[name].pending.push_back({a, b});
+  }
+};
+
+void settle_timers(msc_ctx *ctx) {
+  hipStreamSynchronize(ctx->stream);
+  for (auto &kv : ctx->timers) {
+    for (auto &ev : kv.second.pending) {
+      float ms = 0;
+      hipEventElapsedTime(&ms, ev.first, ev.second);
+      kv.second.ms += ms;
+      kv.second.launches += 1;
+      hipEventDestroy(ev.first);
+      hipEventDestroy(ev.second);
+    }
+    kv.second.pending.clear();
+  }
+}
+
+State sel(const msc_ctx *ctx, int rep) {
+  State s = ctx->S;
+  if (rep == MSC_ALL) {
+    s.task_begin = 0;
+    s.ntask = (int)ctx->tasks.size();
+    s.g_begin = 0;
+    s.g_end = ctx->Ntot;
+    s.rep_begin = 0;
+    s.rep_end = ctx->nrep;
+  } else {
+    const RepDev &R = ctx->reps[rep];
+    s.task_begin = R.task0;
+    s.ntask = R.ntask;
+    s.g_begin = R.n0;
+    s.g_end = R.n0 + R.N;
+    s.rep_begin = rep;
+    s.rep_end = rep + 1;
+  }
+  return s;
+}
+
+bool rep_ok(msc_ctx *ctx, int rep) { return rep == MSC_ALL || (rep >= 0 && rep < ctx->nrep); }
+
+// derived per-replica quantities that depend on the state (shapes, dipoles)
+void derive(msc_ctx *ctx, int rp, const double *len, const double *dia, const double *dpc) {
+  RepDev &R = ctx->reps[rp];
+  const msc_replica_desc &D = ctx->desc[rp];
+  double dmax = 0, dmin = 1.e6, lmax = 0, lev = 0;
+  for (int i = 0; i < R.N; ++i) {
+    dmax = std::max(dmax, len[i] + dia[i]);
+    dmin = std::min(dmin, dia[i]);
+    lmax = std::max(lmax, len[i]);
+    for (int w = 0; w < R.nd; ++w) lev = std::max(lev, std::fabs(dpc[i * R.nd + w]));
+  }
+  if (D.dmax > 0) dmax = D.dmax;
+  if (D.dmin > 0) dmin = D.dmin;
+  R.lmax = lmax;
+  // list radii, v25/main.cpp:461-463
+  R.r_list_min = D.sr_cut + 0.1 * dmin;
+  if (R.r_list_min < 1.5 * dmax) R.r_list_min = 1.5 * dmax;
+  R.r_list_max = R.r_list_min + 0.9 * dmin;
+  R.rl_dip = D.rc > 0 ? D.rc + (R.r_list_max - R.r_list_min) : 0;
+  ctx->S.lever_max = std::max(ctx->S.lever_max, lev);
+}
+
+// cell grid of a replica for the current list radii
+void grid(msc_ctx *ctx, int rp) {
+  RepDev &R = ctx->reps[rp];
+  double need = R.r_list_max + R.lmax;
+  if (R.rc > 0) need = std::max(need, R.rl_dip + 2 * ctx->S.lever_max);
+  need *= 1.0 + 1e-9;
+  for (int a = 0; a < 3; ++a) {
+    int n = (int)std::floor(R.L[a] / need);
+    if (n < 3) n = 1;
+    n = std::min(n, 256);
+    R.nc[a] = n;
+    R.cw[a] = R.L[a] / n;
+  }
+  R.ncell = R.nc[0] * R.nc[1] * R.nc[2];
+}
+
+int upload_reps(msc_ctx *ctx) {
+  HIPC(hipMemcpyAsync(ctx->d_reps, ctx->reps.data(), sizeof(RepDev) * ctx->nrep, hipMemcpyHostToDevice, ctx->stream));
+  return MSC_OK;
+}
+
+void refresh_state_ptrs(msc_ctx *ctx) {
+  State &s = ctx->S;
+  s.Ntot = ctx->Ntot;
+  s.nd = ctx->nd;
+  s.nrep = ctx->nrep;
+  s.ntask = (int)ctx->tasks.size();
+  s.nrtask = (int)ctx->rtasks.size();
+  s.ncells = ctx->ncells;
+  s.reps = ctx->d_reps;
+  s.tasks = ctx->d_tasks;
+  s.rtasks = ctx->d_rtasks;
+  s.rep_of = ctx->d_rep_of;
+  s.x = ctx->d_x; s.p = ctx->d_p; s.l = ctx->d_l; s.ori = ctx->d_ori; s.F = ctx->d_F; s.T = ctx->d_T;
+  s.len = ctx->d_len; s.dia = ctx->d_dia; s.mass = ctx->d_mass; s.inert = ctx->d_inert;
+  s.dpc = ctx->d_dpc; s.doc = ctx->d_doc; s.dmu = ctx->d_dmu; s.dpos = ctx->d_dpos; s.dori = ctx->d_dori;
+  s.snap = ctx->d_snap;
+  s.cell_of = ctx->d_cell_of; s.cell_cnt = ctx->d_cell_cnt; s.cell_start = ctx->d_cell_start;
+  s.cell_fill = ctx->d_cell_fill; s.cell_list = ctx->d_cell_list; s.cell_rep = ctx->d_cell_rep;
+  s.nl_cnt = ctx->d_nl_cnt; s.nl_start = ctx->d_nl_start; s.nl_idx = ctx->d_nl_idx;
+  bool anyrc = false;
+  for (auto &R : ctx->reps) anyrc |= R.rc > 0;
+  s.dl_cnt = anyrc ? ctx->d_dl_cnt : nullptr;
+  s.dl_start = ctx->d_dl_start;
+  s.dl_idx = ctx->d_dl_idx;
+  s.kval = ctx->d_kval;
+}
+
+// reciprocal weights val = 2 KK / V of wave_v/force_dipK (v25/functions.cpp:1733-1737,
+// 1963-1990) for the half-space slots; G = sum val k k^T
+void build_kspace(msc_ctx *ctx, std::vector<double> &kval) {
+  kval.assign(ctx->nslot_tot, 0.0);
+  for (int rp = 0; rp < ctx->nrep; ++rp) {
+    RepDev &R = ctx->reps[rp];
+    const int kmax = R.kmax, side = 2 * kmax + 1;
+    double c = 4. * R.alpha * R.alpha;
+    for (int q = 0; q < 9; ++q) R.G[q] = 0;
+    for (int col = 0; col < R.ncol; ++col) {
+      int nx = col % side - kmax, ny = col / side - kmax;
+      for (int nz = 0; nz < R.KZ; ++nz) {
+        bool half = nz > 0 || ny > 0 || (ny == 0 && nx > 0);
+        if (!half) continue;
+        int ax = std::abs(nx), ay = std::abs(ny);
+        double Kx = R.P2[0] * ax, Ky = R.P2[1] * ay, Kz = R.P2[2] * nz;
+        double Ksq = Kx * Kx + Ky * Ky + Kz * Kz;
+        double KK = 4. * M_PI * std::exp(-Ksq / c) / Ksq;
+        double val = 2. * KK / R.V;
+        kval[R.slot0 + col * R.KZ + nz] = val;
+        double k[3] = {R.P2[0] * nx, R.P2[1] * ny, R.P2[2] * nz};
+        for (int a = 0; a < 3; ++a)
+          for (int b = 0; b < 3; ++b) R.G[3 * a + b] += val * k[a] * k[b];
+      }
+    }
+  }
+}
+
+// tabul (v25/functions.cpp:1680-1704), host libm, one table per replica
+void build_table(const RepDev &R, std::vector<double> &T) {
+  double extL = 200 * 1.001, alpha = R.alpha;
+  double alsq = alpha * alpha, alft = alsq * alsq, spi = std::sqrt(M_PI);
+  for (int i = 1; i < kTabN + 1; i++) {
+    double r = 0.5 * extL * i / kTabN;
+    double rsq = r * r, sr1 = 1.0 / r, sr2 = sr1 * sr1, sr3 = sr2 * sr1;
+    double ERFC = std::erfc(alpha * r);
+    double aer = 2.0 * alpha * std::exp(-alsq * rsq) / spi;
+    double aa = 2.0 * alsq + 3.0 * sr2;
+    T[(size_t)R.tab0 * 3 + 3 * (i - 1) + 0] = ERFC * sr3 + aer * sr2;
+    T[(size_t)R.tab0 * 3 + 3 * (i - 1) + 1] = (3.0 * ERFC * sr3 + aer * aa) * sr2;
+    T[(size_t)R.tab0 * 3 + 3 * (i - 1) + 2] = 4. * alft * aer;
+  }
+}
+
+}  // namespace
+
+// ==========================================================================
+// C ABI
+// ==========================================================================
+extern "C" {
+
+int msc_version(void) { return 1; }
+
+const char *msc_last_error(const msc_ctx *ctx) { return ctx ? ctx->err.c_str() : "no context"; }
+
+int msc_create(int device, int precision, int nrep, const msc_replica_desc *reps, msc_ctx **out) {
+  if (!out || nrep <= 0 || !reps || (precision != MSC_FP64 && precision != MSC_MIXED)) return MSC_E_ARG;
+  *out = nullptr;
+  msc_ctx *ctx = new msc_ctx();
+  ctx->device = device;
+  ctx->precision = precision;
+  ctx->mixed = precision == MSC_MIXED;
+  ctx->nrep = nrep;
+  ctx->desc.assign(reps, reps + nrep);
+  int nd = reps[0].nd;
+  for (int r = 0; r < nrep; ++r) {
+    const msc_replica_desc &D = reps[r];
+    if (D.N <= 0 || D.nd != nd || nd < 1 || nd > 8 || D.K < 2 || D.K > kMaxKZ || D.L[0] <= 0 || D.L[1] <= 0 ||
+        D.L[2] <= 0 || D.alpha <= 0) {
+      delete ctx;
+      return MSC_E_ARG;
+    }
+  }
+  ctx->nd = nd;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&ctx->stream) != hipSuccess) {
+    delete ctx;
+    return MSC_E_HIP;
+  }
+  // replica layout: particles, tasks, chunk tasks, k slots, tables, partials
+  int n0 = 0, slot0 = 0;
+  long part0 = 0;
+  ctx->reps.resize(nrep);
+  for (int r = 0; r < nrep; ++r) {
+    const msc_replica_desc &D = reps[r];
+    RepDev &R = ctx->reps[r];
+    memset(&R, 0, sizeof R);
+    R.n0 = n0;
+    R.N = D.N;
+    R.nd = nd;
+    R.K = D.K;
+    R.kmax = D.K - 1;
+    R.KZ = D.K;
+    R.ncol = (2 * R.kmax + 1) * (2 * R.kmax + 1);
+    R.slot0 = slot0;
+    R.tab0 = r * kTabN;
+    for (int a = 0; a < 3; ++a) {
+      R.L[a] = D.L[a];
+      R.P2[a] = 2. * M_PI / D.L[a];
+      R.B[a] = D.B[a];
+    }
+    R.V = D.L[0] * D.L[1] * D.L[2];
+    R.alpha = D.alpha;
+    R.rc = D.rc;
+    R.sr_cut = D.sr_cut;
+    R.temp = D.temp;
+    R.dt = D.dt;
+    R.Q = D.Q;
+    R.gamma = D.gamma;
+    R.dens = D.dens;
+    R.task0 = (int)ctx->tasks.size();
+    for (int g = 0; g < D.N; g += kWave) ctx->tasks.push_back({r, n0 + g, std::min(kWave, D.N - g), 0});
+    R.ntask = (int)ctx->tasks.size() - R.task0;
+    R.rtask0 = (int)ctx->rtasks.size();
+    for (int g = 0; g < D.N; g += ctx->rchunk)
+      ctx->rtasks.push_back({r, n0 + g, std::min(ctx->rchunk, D.N - g), 0});
+    R.nrtask = (int)ctx->rtasks.size() - R.rtask0;
+    R.part0 = part0;
+    part0 += (long)R.nrtask * R.ncol * R.KZ * 6;
+    slot0 += R.ncol * R.KZ;
+    ctx->maxslot = std::max(ctx->maxslot, R.ncol * R.KZ);
+    ctx->maxcol = std::max(ctx->maxcol, R.ncol);
+    if (std::find(ctx->kz_groups.begin(), ctx->kz_groups.end(), R.KZ) == ctx->kz_groups.end())
+      ctx->kz_groups.push_back(R.KZ);
+    n0 += D.N;
+  }
+  ctx->Ntot = n0;
+  ctx->nslot_tot = slot0;
+  ctx->part_elems = (size_t)part0;
+  ctx->thermo_rng.resize(2 * nrep);
+  for (int r = 0; r < nrep; ++r)
+    for (int i = 0; i < 2; ++i) ctx->thermo_rng[2 * r + i].seed(reps[r].thermo_seed[i]);
+  const int N = ctx->Ntot;
+  const size_t rs = ctx->mixed ? sizeof(float) : sizeof(double);
+  int rc = MSC_OK;
+  auto A = [&](auto **p, size_t n) {
+    if (rc == MSC_OK) rc = dalloc(ctx, p, n);
+  };
+  A(&ctx->d_reps, nrep);
+  A(&ctx->d_tasks, ctx->tasks.size());
+  A(&ctx->d_rtasks, ctx->rtasks.size());
+  A(&ctx->d_rep_of, N);
+  A(&ctx->d_x, 3 * N); A(&ctx->d_p, 3 * N); A(&ctx->d_l, 3 * N); A(&ctx->d_ori, 9 * N);
+  A(&ctx->d_F, 3 * N); A(&ctx->d_T, 3 * N);
+  A(&ctx->d_len, N); A(&ctx->d_dia, N); A(&ctx->d_mass, N); A(&ctx->d_inert, 3 * N);
+  A(&ctx->d_dpc, nd * N); A(&ctx->d_doc, 3 * nd * N); A(&ctx->d_dmu, nd * N);
+  A(&ctx->d_dpos, 3 * nd * N); A(&ctx->d_dori, 3 * nd * N); A(&ctx->d_snap, 3 * N);
+  A(&ctx->d_cell_of, N); A(&ctx->d_cell_list, N);
+  A(&ctx->d_nl_cnt, N); A(&ctx->d_nl_start, N);
+  A(&ctx->d_dl_cnt, N); A(&ctx->d_dl_start, N);
+  A(&ctx->d_kval, ctx->nslot_tot);
+  A(&ctx->d_slot_out, (size_t)ctx->nslot_tot * 10);
+  A(&ctx->d_task_out, ctx->tasks.size() * 16);
+  A(&ctx->d_red, (size_t)nrep * 16);
+  A(&ctx->d_M2, (size_t)nrep * 6);
+  A(&ctx->d_Kin, (size_t)nrep * 2);
+  A(&ctx->d_mom, (size_t)nrep * 8);
+  A(&ctx->d_gs, (size_t)nrep * 4);
+  A(&ctx->d_obs, (size_t)nrep * MSC_OBS_N);
+  A(&ctx->d_counts, (size_t)nrep * 4);
+  A(&ctx->d_E, nrep);
+  A(&ctx->d_th, nrep);
+  A(&ctx->d_flags, (size_t)nrep * 4);
+  A(&ctx->d_rebuild, nrep);
+  A(&ctx->d_one, nrep);
+  if (rc == MSC_OK) {
+    unsigned char **raw[] = {(unsigned char **)&ctx->d_tab, (unsigned char **)&ctx->d_ph,
+                             (unsigned char **)&ctx->d_amp, (unsigned char **)&ctx->d_part,
+                             (unsigned char **)&ctx->d_A};
+    size_t sz[] = {(size_t)nrep * kTabN * 3 * rs, (size_t)3 * kMaxKZ * N * 2 * rs, (size_t)3 * N * rs,
+                   ctx->part_elems * rs, (size_t)ctx->nslot_tot * 2 * rs};
+    for (int q = 0; q < 5 && rc == MSC_OK; ++q) rc = dalloc(ctx, raw[q], sz[q]);
+  }
+  if (rc != MSC_OK) {
+    std::string e = ctx->err;
+    msc_destroy(ctx);
+    return MSC_E_HIP;
+  }
+  std::vector<int> rep_of(N);
+  for (int r = 0; r < nrep; ++r)
+    for (int i = 0; i < ctx->reps[r].N; ++i) rep_of[ctx->reps[r].n0 + i] = r;
+  std::vector<int> ones(nrep, 1);
+  hipMemcpyAsync(ctx->d_rep_of, rep_of.data(), sizeof(int) * N, hipMemcpyHostToDevice, ctx->stream);
+  hipMemcpyAsync(ctx->d_tasks, ctx->tasks.data(), sizeof(Task) * ctx->tasks.size(), hipMemcpyHostToDevice,
+                 ctx->stream);
+  hipMemcpyAsync(ctx->d_rtasks, ctx->rtasks.data(), sizeof(Task) * ctx->rtasks.size(), hipMemcpyHostToDevice,
+                 ctx->stream);
+  hipMemcpyAsync(ctx->d_one, ones.data(), sizeof(int) * nrep, hipMemcpyHostToDevice, ctx->stream);
+  std::vector<Thermo> th(nrep);
+  for (int r = 0; r < nrep; ++r) {
+    th[r].s[0] = th[r].s[1] = reps[r].s0;
+    th[r].strn[0] = th[r].strn[1] = 0;
+  }
+  hipMemcpyAsync(ctx->d_th, th.data(), sizeof(Thermo) * nrep, hipMemcpyHostToDevice, ctx->stream);
+  std::vector<double> kval;
+  build_kspace(ctx, kval);
+  hipMemcpyAsync(ctx->d_kval, kval.data(), sizeof(double) * kval.size(), hipMemcpyHostToDevice, ctx->stream);
+  std::vector<double> tab((size_t)nrep * kTabN * 3);
+  for (int r = 0; r < nrep; ++r) build_table(ctx->reps[r], tab);
+  if (ctx->mixed) {
+    std::vector<float> tf(tab.begin(), tab.end());
+    hipMemcpyAsync(ctx->d_tab, tf.data(), tf.size() * sizeof(float), hipMemcpyHostToDevice, ctx->stream);
+    hipStreamSynchronize(ctx->stream);
+  } else {
+    hipMemcpyAsync(ctx->d_tab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream);
+    hipStreamSynchronize(ctx->stream);
+  }
+  upload_reps(ctx);
+  refresh_state_ptrs(ctx);
+  if (hipStreamSynchronize(ctx->stream) != hipSuccess) {
+    msc_destroy(ctx);
+    return MSC_E_HIP;
+  }
+  *out = ctx;
+  return MSC_OK;
+}
+
+void msc_destroy(msc_ctx *ctx) {
+  if (!ctx) return;
+  if (ctx->stream) hipStreamSynchronize(ctx->stream);
+  if (ctx->graph_exec) hipGraphExecDestroy(ctx->graph_exec);
+  void *bufs[] = {ctx->d_reps, ctx->d_tasks, ctx->d_rtasks, ctx->d_rep_of, ctx->d_cell_rep, ctx->d_x, ctx->d_p,
+                  ctx->d_l, ctx->d_ori, ctx->d_F, ctx->d_T, ctx->d_len, ctx->d_dia, ctx->d_mass, ctx->d_inert,
+                  ctx->d_dpc, ctx->d_doc, ctx->d_dmu, ctx->d_dpos, ctx->d_dori, ctx->d_snap, ctx->d_cell_of,
+                  ctx->d_cell_cnt, ctx->d_cell_start, ctx->d_cell_fill, ctx->d_cell_list, ctx->d_nl_cnt,
+                  ctx->d_nl_start, ctx->d_nl_idx, ctx->d_dl_cnt, ctx->d_dl_start, ctx->d_dl_idx, ctx->d_kval,
+                  ctx->d_tab, ctx->d_ph, ctx->d_amp, ctx->d_part, ctx->d_A, ctx->d_slot_out, ctx->d_task_out,
+                  ctx->d_red, ctx->d_M2, ctx->d_Kin, ctx->d_mom, ctx->d_gs, ctx->d_obs, ctx->d_counts, ctx->d_E,
+                  ctx->d_th, ctx->d_flags, ctx->d_rebuild, ctx->d_one};
+  for (void *b : bufs)
+    if (b) hipFree(b);
+  for (auto &kv : ctx->timers)
+    for (auto &ev : kv.second.pending) {
+      hipEventDestroy(ev.first);
+      hipEventDestroy(ev.second);
+    }
+  if (ctx->stream) hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+int msc_set_state(msc_ctx *ctx, int rep, const double *r, const double *p, const double *l, const double *ori,
+                  const double *len, const double *dia, const double *mass, const double *dpc, const double *doc,
+                  const double *dmu) {
+  if (!ctx || rep < 0 || rep >= ctx->nrep || !r || !p || !l || !ori || !len || !dia || !mass || !dpc || !doc || !dmu)
+    return MSC_E_ARG;
+  RepDev &R = ctx->reps[rep];
+  const int n = R.N, N = ctx->Ntot, nd = ctx->nd;
+  std::vector<double> buf;
+  auto put = [&](double *dst, const double *src, int ncomp) -> int {
+    buf.resize((size_t)n);
+    for (int c = 0; c < ncomp; ++c) {
+      for (int i = 0; i < n; ++i) buf[i] = src[(size_t)i * ncomp + c];
+      HIPC(hipMemcpy(dst + (size_t)c * N + R.n0, buf.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    }
+    return MSC_OK;
+  };
+  int rc;
+  if ((rc = put(ctx->d_x, r, 3)) || (rc = put(ctx->d_p, p, 3)) || (rc = put(ctx->d_l, l, 3)) ||
+      (rc = put(ctx->d_ori, ori, 9)) || (rc = put(ctx->d_len, len, 1)) || (rc = put(ctx->d_dia, dia, 1)) ||
+      (rc = put(ctx->d_mass, mass, 1)))
+    return rc;
+  // dipole constraints: host layout [N][nd], [N][nd][3] -> [nd][N], [3nd][N]
+  if ((rc = put(ctx->d_dpc, dpc, nd)) || (rc = put(ctx->d_dmu, dmu, nd)) || (rc = put(ctx->d_doc, doc, 3 * nd)))
+    return rc;
+  // inertia, Spherocyl::set_inertia (v25/classes.cpp:175-185)
+  std::vector<double> I((size_t)3 * n);
+  for (int i = 0; i < n; ++i) {
+    double Rr = dia[i] / 2, Rsq = Rr * Rr, Rft = Rsq * Rsq, Lq = len[i], Lsq = Lq * Lq;
+    double vol = M_PI * Rsq * (Lq + 3. * Rr / 4);
+    double mdens = mass[i] / vol;
+    I[3 * i + 0] = mdens * M_PI * (Lsq * Lq * Rsq / 12 + Lsq * Rsq * Rr / 3 + Lq * Rft / 4 + 8. * Rft * Rr / 15);
+    I[3 * i + 1] = I[3 * i + 0];
+    I[3 * i + 2] = mdens * M_PI * (Lq * Rft / 2 + 8. * Rft * Rr / 15);
+  }
+  if ((rc = put(ctx->d_inert, I.data(), 3))) return rc;
+  derive(ctx, rep, len, dia, dpc);
+  upload_reps(ctx);
+  launch_dipoles(sel(ctx, rep), ctx->stream);
+  HIPC(hipStreamSynchronize(ctx->stream));
+  ctx->have_state = true;
+  return MSC_OK;
+}
+
+int msc_get_state(msc_ctx *ctx, int rep, double *r, double *p, double *l, double *ori) {
+  if (!ctx || rep < 0 || rep >= ctx->nrep) return MSC_E_ARG;
+  const RepDev &R = ctx->reps[rep];
+  const int n = R.N, N = ctx->Ntot;
+  std::vector<double> buf(n);
+  auto get = [&](double *dst, const double *src, int ncomp) -> int {
+    if (!dst) return MSC_OK;
+    for (int c = 0; c < ncomp; ++c) {
+      HIPC(hipMemcpy(buf.data(), src + (size_t)c * N + R.n0, sizeof(double) * n, hipMemcpyDeviceToHost));
+      for (int i = 0; i < n; ++i) dst[(size_t)i * ncomp + c] = buf[i];
+    }
+    return MSC_OK;
+  };
+  HIPC(hipStreamSynchronize(ctx->stream));
+  int rc;
+  if ((rc = get(r, ctx->d_x, 3)) || (rc = get(p, ctx->d_p, 3)) || (rc = get(l, ctx->d_l, 3)) ||
+      (rc = get(ori, ctx->d_ori, 9)))
+    return rc;
+  return MSC_OK;
+}
+
+int msc_get_forces(msc_ctx *ctx, int rep, double *F, double *T) {
+  if (!ctx || rep < 0 || rep >= ctx->nrep) return MSC_E_ARG;
+  const RepDev &R = ctx->reps[rep];
+  const int n = R.N, N = ctx->Ntot;
+  std::vector<double> buf(n);
+  HIPC(hipStreamSynchronize(ctx->stream));
+  for (int c = 0; c < 3; ++c) {
+    if (F) {
+      HIPC(hipMemcpy(buf.data(), ctx->d_F + (size_t)c * N + R.n0, sizeof(double) * n, hipMemcpyDeviceToHost));
+      for (int i = 0; i < n; ++i) F[3 * i + c] = buf[i];
+    }
+    if (T) {
+      HIPC(hipMemcpy(buf.data(), ctx->d_T + (size_t)c * N + R.n0, sizeof(double) * n, hipMemcpyDeviceToHost));
+      for (int i = 0; i < n; ++i) T[3 * i + c] = buf[i];
+    }
+  }
+  return MSC_OK;
+}
+
+int msc_set_forces(msc_ctx *ctx, int rep, const double *F, const double *T) {
+  if (!ctx || rep < 0 || rep >= ctx->nrep || !F || !T) return MSC_E_ARG;
+  const RepDev &R = ctx->reps[rep];
+  const int n = R.N, N = ctx->Ntot;
+  std::vector<double> buf(n);
+  for (int c = 0; c < 3; ++c) {
+    for (int i = 0; i < n; ++i) buf[i] = F[3 * i + c];
+    HIPC(hipMemcpy(ctx->d_F + (size_t)c * N + R.n0, buf.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    for (int i = 0; i < n; ++i) buf[i] = T[3 * i + c];
+    HIPC(hipMemcpy(ctx->d_T + (size_t)c * N + R.n0, buf.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+  }
+  return MSC_OK;
+}
+
+int msc_get_thermostat(msc_ctx *ctx, int rep, double s[2], double strn[2]) {
+  if (!ctx || rep < 0 || rep >= ctx->nrep) return MSC_E_ARG;
+  Thermo t;
+  HIPC(hipStreamSynchronize(ctx->stream));
+  HIPC(hipMemcpy(&t, ctx->d_th + rep, sizeof t, hipMemcpyDeviceToHost));
+  if (s) { s[0] = t.s[0]; s[1] = t.s[1]; }
+  if (strn) { strn[0] = t.strn[0]; strn[1] = t.strn[1]; }
+  return MSC_OK;
+}
+
+int msc_set_thermostat(msc_ctx *ctx, int rep, const double s[2], const double strn[2]) {
+  if (!ctx || rep < 0 || rep >= ctx->nrep || !s || !strn) return MSC_E_ARG;
+  Thermo t;
+  t.s[0] = s[0]; t.s[1] = s[1]; t.strn[0] = strn[0]; t.strn[1] = strn[1];
+  HIPC(hipMemcpy(ctx->d_th + rep, &t, sizeof t, hipMemcpyHostToDevice));
+  return MSC_OK;
+}
+
+// --------------------------------------------------------------------------
+// neighbour lists
+// --------------------------------------------------------------------------
+static int lists_build(msc_ctx *ctx, const int *d_flagmask) {
+  State s = sel(ctx, MSC_ALL);
+  {
+    Bracket b(ctx, "nlist");
+    launch_cells(s, d_flagmask, ctx->d_flags, ctx->stream);
+    launch_lists_count(s, d_flagmask, ctx->d_flags, ctx->stream);
+    launch_lists_fill(s, d_flagmask, ctx->stream);
+  }
+  return MSC_OK;
+}
+
+// (re)allocate cells and list regions for every replica from a counting pass
+static int lists_layout(msc_ctx *ctx) {
+  const int nrep = ctx->nrep;
+  int ncells = 0;
+  std::vector<int> cell_rep;
+  for (int r = 0; r < nrep; ++r) {
+    grid(ctx, r);
+    ctx->reps[r].cell0 = ncells;
+    ncells += ctx->reps[r].ncell;
+    cell_rep.insert(cell_rep.end(), ctx->reps[r].ncell, r);
+  }
+  if (ncells != ctx->ncells || !ctx->d_cell_cnt) {
+    int rc;
+    if ((rc = dalloc(ctx, &ctx->d_cell_cnt, ncells)) || (rc = dalloc(ctx, &ctx->d_cell_start, ncells)) ||
+        (rc = dalloc(ctx, &ctx->d_cell_fill, ncells)) || (rc = dalloc(ctx, &ctx->d_cell_rep, ncells)))
+      return rc;
+    ctx->ncells = ncells;
+  }
+  HIPC(hipMemcpyAsync(ctx->d_cell_rep, cell_rep.data(), sizeof(int) * ncells, hipMemcpyHostToDevice, ctx->stream));
+  // counting pass with unbounded regions, then size the regions
+  for (int r = 0; r < nrep; ++r) {
+    ctx->reps[r].nl_base = 0;
+    ctx->reps[r].nl_cap = 0x7fffffff;
+    ctx->reps[r].dl_base = 0;
+    ctx->reps[r].dl_cap = 0x7fffffff;
+  }
+  int rc;
+  if ((rc = upload_reps(ctx))) return rc;
+  refresh_state_ptrs(ctx);
+  State s = sel(ctx, MSC_ALL);
+  launch_cells(s, ctx->d_one, ctx->d_flags, ctx->stream);
+  launch_lists_count(s, ctx->d_one, ctx->d_flags, ctx->stream);
+  std::vector<int> cnt(ctx->Ntot), dcnt(ctx->Ntot);
+  HIPC(hipMemcpyAsync(cnt.data(), ctx->d_nl_cnt, sizeof(int) * ctx->Ntot, hipMemcpyDeviceToHost, ctx->stream));
+  HIPC(hipMemcpyAsync(dcnt.data(), ctx->d_dl_cnt, sizeof(int) * ctx->Ntot, hipMemcpyDeviceToHost, ctx->stream));
+  HIPC(hipStreamSynchronize(ctx->stream));
+  long nl_tot = 0, dl_tot = 0;
+  for (int r = 0; r < nrep; ++r) {
+    RepDev &R = ctx->reps[r];
+    long a = 0, b = 0;
+    for (int i = R.n0; i < R.n0 + R.N; ++i) {
+      a += cnt[i];
+      if (R.rc > 0) b += dcnt[i];
+    }
+    // headroom for the growth of the lists between layouts
+    long ca = a + a / 2 + 64 * 16, cb = R.rc > 0 ? b + b / 2 + 64 * 16 : 0;
+    R.nl_base = (int)nl_tot;
+    R.nl_cap = (int)ca;
+    R.dl_base = (int)dl_tot;
+    R.dl_cap = (int)cb;
+    nl_tot += ca;
+    dl_tot += cb;
+  }
+  if (nl_tot > 0x7fffffffL || dl_tot > 0x7fffffffL) return fail(ctx, MSC_E_CAPACITY, "neighbour lists exceed 2^31 entries");
+  if (nl_tot > ctx->nl_cap) {
+    if ((rc = dalloc(ctx, &ctx->d_nl_idx, nl_tot))) return rc;
+    ctx->nl_cap = nl_tot;
+  }
+  if (dl_tot > ctx->dl_cap) {
+    if ((rc = dalloc(ctx, &ctx->d_dl_idx, std::max(dl_tot, 1L)))) return rc;
+    ctx->dl_cap = dl_tot;
+  }
+  if ((rc = upload_reps(ctx))) return rc;
+  refresh_state_ptrs(ctx);
+  return MSC_OK;
+}
+
+int msc_neigh_list(msc_ctx *ctx, int rep, double r_list) {
+  if (!ctx || !rep_ok(ctx, rep) || !(r_list > 0)) return MSC_E_ARG;
+  if (!ctx->have_state) return fail(ctx, MSC_E_STATE, "no state");
+  for (int r = 0; r < ctx->nrep; ++r)
+    if (rep == MSC_ALL || r == rep) ctx->reps[r].r_list_max = r_list;
+  int rc = lists_layout(ctx);
+  if (rc) return rc;
+  HIPC(hipMemsetAsync(ctx->d_flags, 0, sizeof(int) * 4 * ctx->nrep, ctx->stream));
+  if ((rc = lists_build(ctx, ctx->d_one))) return rc;
+  int flags[4 * 64];
+  std::vector<int> fl(4 * ctx->nrep);
+  HIPC(hipMemcpy(fl.data(), ctx->d_flags, sizeof(int) * fl.size(), hipMemcpyDeviceToHost));
+  (void)flags;
+  for (int r = 0; r < ctx->nrep; ++r)
+    if (fl[4 * r + 2]) return fail(ctx, MSC_E_CAPACITY, "neighbour list region overflow");
+  ctx->have_lists = true;
+  return MSC_OK;
+}
+
+int msc_get_neigh_list(msc_ctx *ctx, int rep, int *off, int *nbr, long cap, long *total) {
+  if (!ctx || rep < 0 || rep >= ctx->nrep || !off || !total) return MSC_E_ARG;
+  if (!ctx->have_lists) return fail(ctx, MSC_E_STATE, "no neighbour list");
+  const RepDev &R = ctx->reps[rep];
+  std::vector<int> cnt(R.N), st(R.N), idx(std::max(R.nl_cap, 1));
+  HIPC(hipStreamSynchronize(ctx->stream));
+  HIPC(hipMemcpy(cnt.data(), ctx->d_nl_cnt + R.n0, sizeof(int) * R.N, hipMemcpyDeviceToHost));
+  HIPC(hipMemcpy(st.data(), ctx->d_nl_start + R.n0, sizeof(int) * R.N, hipMemcpyDeviceToHost));
+  HIPC(hipMemcpy(idx.data(), ctx->d_nl_idx + R.nl_base, sizeof(int) * R.nl_cap, hipMemcpyDeviceToHost));
+  long t = 0;
+  std::vector<int> row;
+  for (int i = 0; i < R.N; ++i) {
+    off[i] = (int)t;
+    row.clear();
+    for (int q = 0; q < cnt[i]; ++q) {
+      int j = idx[st[i] - R.nl_base + q] - R.n0;
+      if (j > i) row.push_back(j);
+    }
+    std::sort(row.begin(), row.end());
+    for (int j : row) {
+      if (nbr && t < cap) nbr[t] = j;
+      ++t;
+    }
+  }
+  off[R.N] = (int)t;
+  *total = t;
+  return MSC_OK;
+}
+
+int msc_check_list(msc_ctx *ctx, int rep, double r_list_min, double r_list_max, int *rebuild) {
+  if (!ctx || rep < 0 || rep >= ctx->nrep || !rebuild) return MSC_E_ARG;
+  if (!ctx->have_lists) return fail(ctx, MSC_E_STATE, "no neighbour list");
+  ctx->reps[rep].r_list_min = r_list_min;
+  ctx->reps[rep].r_list_max = r_list_max;
+  int rc;
+  if ((rc = upload_reps(ctx))) return rc;
+  State s = sel(ctx, rep);
+  launch_check(s, ctx->d_task_out, ctx->d_rebuild, nullptr, 0, ctx->stream);
+  HIPC(hipMemcpy(rebuild, ctx->d_rebuild + rep, sizeof(int), hipMemcpyDeviceToHost));
+  return MSC_OK;
+}
+
+// --------------------------------------------------------------------------
+// forces
+// --------------------------------------------------------------------------
+static int read_energy(msc_ctx *ctx, int rep, RepEnergy *e) {
+  HIPC(hipMemcpy(e, ctx->d_E + rep, sizeof(RepEnergy), hipMemcpyDeviceToHost));
+  return MSC_OK;
+}
+
+static void do_force_sc(msc_ctx *ctx, int rep) {
+  State s = sel(ctx, rep);
+  {
+    Bracket b(ctx, "wca");
+    launch_force_sc(s, !ctx->mixed, ctx->d_task_out, ctx->d_flags, ctx->stream);
+  }
+  launch_reduce(s, ctx->d_task_out, 16, 11, 0, (double *)ctx->d_E, kEstride, kEoff_sc, ctx->stream);
+  launch_reduce(s, ctx->d_task_out + 11, 16, 1, 0, ctx->d_counts, 4, 0, ctx->stream);
+}
+
+int msc_force_sc(msc_ctx *ctx, int rep, double r_cut, double *U, double *W, double PT[9]) {
+  if (!ctx || rep < 0 || rep >= ctx->nrep) return MSC_E_ARG;
+  if (!ctx->have_lists) return fail(ctx, MSC_E_STATE, "no neighbour list");
+  if (r_cut != ctx->reps[rep].sr_cut) {
+    ctx->reps[rep].sr_cut = r_cut;
+    int rc = upload_reps(ctx);
+    if (rc) return rc;
+  }
+  HIPC(hipMemsetAsync(ctx->d_flags + 4 * rep, 0, sizeof(int) * 4, ctx->stream));
+  do_force_sc(ctx, rep);
+  int fl[4];
+  HIPC(hipMemcpy(fl, ctx->d_flags + 4 * rep, sizeof fl, hipMemcpyDeviceToHost));
+  if (fl[0]) return fail(ctx, fl[0] == 1 ? MSC_E_OVERLAP : MSC_E_DISTANCE, "force_sc abort");
+  RepEnergy e;
+  int rc = read_energy(ctx, rep, &e);
+  if (rc) return rc;
+  if (U) *U = e.Usc;
+  if (W) *W = e.Wsc;
+  if (PT) memcpy(PT, e.PTsc, sizeof e.PTsc);
+  return MSC_OK;
+}
+
+static void do_dipR(msc_ctx *ctx, int rep) {
+  State s = sel(ctx, rep);
+  bool all = true;
+  for (int r = (rep == MSC_ALL ? 0 : rep); r < (rep == MSC_ALL ? ctx->nrep : rep + 1); ++r) all &= ctx->reps[r].rc <= 0;
+  bool none = true;
+  for (int r = (rep == MSC_ALL ? 0 : rep); r < (rep == MSC_ALL ? ctx->nrep : rep + 1); ++r) none &= !(ctx->reps[r].rc <= 0);
+  {
+    Bracket b(ctx, "real_pairs");
+    if (all) launch_dipR(s, ctx->mixed, true, ctx->d_tab, ctx->d_task_out, ctx->stream);
+    else if (none) launch_dipR(s, ctx->mixed, false, ctx->d_tab, ctx->d_task_out, ctx->stream);
+    else {
+      for (int r = 0; r < ctx->nrep; ++r) {
+        State s1 = sel(ctx, r);
+        launch_dipR(s1, ctx->mixed, ctx->reps[r].rc <= 0, ctx->d_tab, ctx->d_task_out, ctx->stream);
+      }
+    }
+  }
+  launch_reduce(s, ctx->d_task_out, 16, 10, 0, (double *)ctx->d_E, kEstride, kEoff_R, ctx->stream);
+  launch_reduce(s, ctx->d_task_out + 10, 16, 1, 0, ctx->d_counts, 4, 1, ctx->stream);
+}
+
+int msc_force_dipR(msc_ctx *ctx, int rep, double *U, double PT[9]) {
+  if (!ctx || rep < 0 || rep >= ctx->nrep) return MSC_E_ARG;
+  if (ctx->reps[rep].rc > 0 && !ctx->have_lists) return fail(ctx, MSC_E_STATE, "no pair list");
+  do_dipR(ctx, rep);
+  RepEnergy e;
+  int rc = read_energy(ctx, rep, &e);
+  if (rc) return rc;
+  if (U) *U = e.UR;
+  if (PT) memcpy(PT, e.PTR, sizeof e.PTR);
+  return MSC_OK;
+}
+
+static int do_dipK(msc_ctx *ctx, int rep) {
+  State s = sel(ctx, rep);
+  launch_recip_tables(s, ctx->mixed, ctx->d_ph, ctx->d_amp, ctx->d_task_out, ctx->stream);
+  launch_reduce(s, ctx->d_task_out, 16, 6, 0, ctx->d_M2, 6, 0, ctx->stream);
+  // structure factors per group of replicas sharing KZ
+  for (int kz : ctx->kz_groups) {
+    int rt0 = -1, rt1 = -1;
+    for (int r = 0; r < ctx->nrep; ++r) {
+      if (ctx->reps[r].KZ != kz || (rep != MSC_ALL && r != rep)) continue;
+      if (rt0 < 0) rt0 = ctx->reps[r].rtask0;
+      rt1 = ctx->reps[r].rtask0 + ctx->reps[r].nrtask;
+    }
+    if (rt0 < 0) continue;
+    Bracket b(ctx, "recip_struct");
+    // groups are contiguous when replicas sharing KZ are adjacent; otherwise
+    // launch replica by replica
+    bool contiguous = true;
+    for (int r = 0; r < ctx->nrep; ++r)
+      if (ctx->reps[r].rtask0 >= rt0 && ctx->reps[r].rtask0 < rt1 && ctx->reps[r].KZ != kz) contiguous = false;
+    if (contiguous) {
+      launch_recip_struct(s, ctx->mixed, kz, rt0, rt1 - rt0, ctx->maxcol, ctx->d_ph, ctx->d_amp, ctx->d_part,
+                          ctx->stream);
+    } else {
+      for (int r = 0; r < ctx->nrep; ++r)
+        if (ctx->reps[r].KZ == kz && (rep == MSC_ALL || r == rep))
+          launch_recip_struct(s, ctx->mixed, kz, ctx->reps[r].rtask0, ctx->reps[r].nrtask, ctx->maxcol, ctx->d_ph,
+                              ctx->d_amp, ctx->d_part, ctx->stream);
+    }
+  }
+  launch_recip_reduce(s, ctx->mixed, ctx->d_part, ctx->d_M2, ctx->d_A, ctx->d_slot_out, ctx->maxslot, ctx->stream);
+  launch_reduce(s, ctx->d_slot_out, 10, 10, 1, (double *)ctx->d_E, kEstride, kEoff_K, ctx->stream);
+  for (int kz : ctx->kz_groups) {
+    for (int r = 0; r < ctx->nrep; ++r) {
+      if (ctx->reps[r].KZ != kz || (rep != MSC_ALL && r != rep)) continue;
+      // merge consecutive replicas of this KZ into one launch
+      int r1 = r;
+      while (r1 + 1 < ctx->nrep && ctx->reps[r1 + 1].KZ == kz && rep == MSC_ALL) ++r1;
+      State s1 = sel(ctx, r);
+      s1.ntask = ctx->reps[r1].task0 + ctx->reps[r1].ntask - ctx->reps[r].task0;
+      s1.g_end = ctx->reps[r1].n0 + ctx->reps[r1].N;
+      s1.rep_end = r1 + 1;
+      {
+        Bracket b(ctx, "recip_force");
+        launch_recip_force(s1, ctx->mixed, kz, ctx->d_ph, ctx->d_amp, ctx->d_A, ctx->stream);
+      }
+      r = r1;
+    }
+  }
+  return MSC_OK;
+}
+
+int msc_force_dipK(msc_ctx *ctx, int rep, double *U, double PT[9]) {
+  if (!ctx || rep < 0 || rep >= ctx->nrep) return MSC_E_ARG;
+  if (ctx->nd != 1) return fail(ctx, MSC_E_ARG, "the reciprocal kernels take one dipole slot per particle");
+  int rc = do_dipK(ctx, rep);
+  if (rc) return rc;
+  RepEnergy e;
+  if ((rc = read_energy(ctx, rep, &e))) return rc;
+  if (U) *U = e.UK;
+  if (PT) memcpy(PT, e.PTK, sizeof e.PTK);
+  return MSC_OK;
+}
+
+int msc_ext_field(msc_ctx *ctx, int rep, double *U) {
+  if (!ctx || rep < 0 || rep >= ctx->nrep) return MSC_E_ARG;
+  State s = sel(ctx, rep);
+  launch_ext_field(s, ctx->d_task_out, ctx->stream);
+  launch_reduce(s, ctx->d_task_out, 16, 1, 0, (double *)ctx->d_E, kEstride, kEoff_ext, ctx->stream);
+  RepEnergy e;
+  int rc = read_energy(ctx, rep, &e);
+  if (rc) return rc;
+  if (U) *U = e.Uext;
+  return MSC_OK;
+}
+
+int msc_self_ener(msc_ctx *ctx, int rep, double *U) {
+  if (!ctx || rep < 0 || rep >= ctx->nrep || !U) return MSC_E_ARG;
+  State s = sel(ctx, rep);
+  launch_self_ener(s, ctx->d_task_out, ctx->stream);
+  launch_reduce(s, ctx->d_task_out, 16, 1, 0, ctx->d_red, 16, 15, ctx->stream);
+  double v[16];
+  HIPC(hipMemcpy(v, ctx->d_red + 16 * rep, sizeof v, hipMemcpyDeviceToHost));
+  *U = v[15];
+  return MSC_OK;
+}
+
+int msc_zero_forces(msc_ctx *ctx, int rep) {
+  if (!ctx || !rep_ok(ctx, rep)) return MSC_E_ARG;
+  launch_zero_FT(sel(ctx, rep), ctx->stream);
+  HIPC(hipStreamSynchronize(ctx->stream));
+  return MSC_OK;
+}
+
+// --------------------------------------------------------------------------
+// integrator
+// --------------------------------------------------------------------------
+int msc_moveX(msc_ctx *ctx, int rep, double dt) {
+  if (!ctx || !rep_ok(ctx, rep)) return MSC_E_ARG;
+  for (int r = 0; r < ctx->nrep; ++r)
+    if (rep == MSC_ALL || r == rep) ctx->reps[r].dt = dt;
+  int rc = upload_reps(ctx);
+  if (rc) return rc;
+  State s = sel(ctx, rep);
+  launch_moveX(s, ctx->stream);
+  launch_dipoles(s, ctx->stream);
+  HIPC(hipStreamSynchronize(ctx->stream));
+  return MSC_OK;
+}
+
+int msc_moveP(msc_ctx *ctx, int rep, double dt, int last, double K[2], double PT[9]) {
+  if (!ctx || rep < 0 || rep >= ctx->nrep) return MSC_E_ARG;
+  ctx->reps[rep].dt = dt;
+  int rc = upload_reps(ctx);
+  if (rc) return rc;
+  State s = sel(ctx, rep);
+  launch_moveP(s, last != 0, ctx->d_th, ctx->d_task_out, ctx->stream);
+  launch_reduce(s, ctx->d_task_out, 16, 8, 0, ctx->d_red, 16, 0, ctx->stream);
+  double v[16];
+  HIPC(hipMemcpy(v, ctx->d_red + 16 * rep, sizeof v, hipMemcpyDeviceToHost));
+  if (K) { K[0] = v[0] / 2; K[1] = v[1] / 2; }
+  if (PT) {
+    double P[9] = {v[2], v[3], v[4], v[3], v[5], v[6], v[4], v[6], v[7]};
+    if (!last) for (double &x : P) x = 0;
+    memcpy(PT, P, sizeof P);
+  }
+  return MSC_OK;
+}
+
+int msc_renorm(msc_ctx *ctx, int rep) {
+  if (!ctx || !rep_ok(ctx, rep)) return MSC_E_ARG;
+  launch_renorm(sel(ctx, rep), ctx->stream);
+  HIPC(hipStreamSynchronize(ctx->stream));
+  return MSC_OK;
+}
+
+// --------------------------------------------------------------------------
+// the step loop
+// --------------------------------------------------------------------------
+static void forces_all(msc_ctx *ctx) {
+  State s = sel(ctx, MSC_ALL);
+  launch_zero_FT(s, ctx->stream);
+  do_force_sc(ctx, MSC_ALL);
+  do_dipR(ctx, MSC_ALL);
+  do_dipK(ctx, MSC_ALL);
+  launch_ext_field(s, ctx->d_task_out, ctx->stream);
+  launch_reduce(s, ctx->d_task_out, 16, 1, 0, (double *)ctx->d_E, kEstride, kEoff_ext, ctx->stream);
+}
+
+int msc_init_run(msc_ctx *ctx) {
+  if (!ctx || !ctx->have_state) return MSC_E_ARG;
+  if (ctx->nd != 1) return fail(ctx, MSC_E_ARG, "the step loop takes one dipole slot per particle");
+  int rc = lists_layout(ctx);
+  if (rc) return rc;
+  HIPC(hipMemsetAsync(ctx->d_flags, 0, sizeof(int) * 4 * ctx->nrep, ctx->stream));
+  if ((rc = lists_build(ctx, ctx->d_one))) return rc;
+  ctx->have_lists = true;
+  forces_all(ctx);
+  State s = sel(ctx, MSC_ALL);
+  launch_self_ener(s, ctx->d_task_out, ctx->stream);
+  launch_reduce(s, ctx->d_task_out, 16, 1, 0, ctx->d_red, 16, 15, ctx->stream);
+  std::vector<double> red(16 * ctx->nrep);
+  std::vector<int> fl(4 * ctx->nrep);
+  HIPC(hipMemcpy(red.data(), ctx->d_red, sizeof(double) * red.size(), hipMemcpyDeviceToHost));
+  HIPC(hipMemcpy(fl.data(), ctx->d_flags, sizeof(int) * fl.size(), hipMemcpyDeviceToHost));
+  for (int r = 0; r < ctx->nrep; ++r) {
+    ctx->reps[r].Uself = red[16 * r + 15] / ctx->reps[r].N;
+    if (fl[4 * r + 2]) return fail(ctx, MSC_E_CAPACITY, "neighbour list region overflow");
+    if (fl[4 * r + 0]) return fail(ctx, fl[4 * r] == 1 ? MSC_E_OVERLAP : MSC_E_DISTANCE, "initial overlap");
+  }
+  return upload_reps(ctx);
+}
+
+int msc_set_graph(msc_ctx *ctx, int on) {
+  if (!ctx) return MSC_E_ARG;
+  ctx->graph_on = on != 0;
+  return MSC_OK;
+}
+
+// one step of every replica, enqueued on the context stream
+static void enqueue_step(msc_ctx *ctx, int step, const double *d_gs_step, double *d_obs_step) {
+  State s = sel(ctx, MSC_ALL);
+  hipStream_t st = ctx->stream;
+  if (std::fmod((float)step, (float)100) == 0) launch_renorm(s, st);
+  launch_thermo(s, ctx->d_th, ctx->d_Kin, 0, d_gs_step, st);
+  {
+    Bracket b(ctx, "integrate");
+    launch_moveP(s, false, ctx->d_th, ctx->d_task_out, st);
+  }
+  launch_reduce(s, ctx->d_task_out, 16, 8, 0, ctx->d_red, 16, 0, st);
+  launch_kin_finish(s, ctx->d_red, ctx->d_Kin, st);
+  launch_thermo(s, ctx->d_th, ctx->d_Kin, 1, d_gs_step, st);
+  {
+    Bracket b(ctx, "integrate");
+    launch_moveX(s, st);
+    launch_dipoles(s, st);
+  }
+  forces_all(ctx);
+  {
+    Bracket b(ctx, "integrate");
+    launch_moveP(s, true, ctx->d_th, ctx->d_task_out, st);
+  }
+  launch_reduce(s, ctx->d_task_out, 16, 8, 0, ctx->d_red, 16, 0, st);
+  launch_kin_finish(s, ctx->d_red, ctx->d_Kin, st);
+  launch_thermo(s, ctx->d_th, ctx->d_Kin, 2, d_gs_step, st);
+  launch_momenta(s, ctx->d_task_out, st);
+  launch_reduce(s, ctx->d_task_out, 16, 6, 0, ctx->d_mom, 8, 0, st);
+  launch_observables(s, ctx->d_E, ctx->d_Kin, ctx->d_mom, d_obs_step, st);
+  // list maintenance (v25/main.cpp:679-701): decide on the device, rebuild
+  // inside kernels that return at once for replicas that keep their list
+  launch_check(s, ctx->d_task_out, ctx->d_rebuild, d_obs_step + MSC_OBS_REBUILD, MSC_OBS_N, st);
+  lists_build(ctx, ctx->d_rebuild);
+}
+
+int msc_run(msc_ctx *ctx, int step0, int nsteps, double *obs) {
+  if (!ctx || nsteps < 0) return MSC_E_ARG;
+  if (!ctx->have_lists) return fail(ctx, MSC_E_STATE, "call msc_init_run first");
+  const int nrep = ctx->nrep;
+  // thermostat deviates: two uniforms per thermostat per step
+  // (moveTher mode 0, v25/functions.cpp:2193-2201)
+  std::vector<double> gs((size_t)nsteps * nrep * 4);
+  for (int k = 0; k < nsteps; ++k)
+    for (int r = 0; r < nrep; ++r)
+      for (int i = 0; i < 2; ++i) {
+        double a = ctx->thermo_rng[2 * r + i].uniform();
+        double b = ctx->thermo_rng[2 * r + i].uniform();
+        gauss1(a, b);
+        gs[((size_t)k * nrep + r) * 4 + 2 * i] = a;
+        gs[((size_t)k * nrep + r) * 4 + 2 * i + 1] = b;
+      }
+  double *d_gs_all = nullptr, *d_obs_all = nullptr;
+  HIPC(hipMalloc((void **)&d_gs_all, sizeof(double) * std::max<size_t>(gs.size(), 1)));
+  HIPC(hipMalloc((void **)&d_obs_all, sizeof(double) * std::max<size_t>((size_t)nsteps * nrep * MSC_OBS_N, 1)));
+  HIPC(hipMemcpyAsync(d_gs_all, gs.data(), sizeof(double) * gs.size(), hipMemcpyHostToDevice, ctx->stream));
+  HIPC(hipMemsetAsync(ctx->d_flags, 0, sizeof(int) * 4 * nrep, ctx->stream));
+  for (int k = 0; k < nsteps; ++k)
+    enqueue_step(ctx, step0 + k, d_gs_all + (size_t)k * nrep * 4, d_obs_all + (size_t)k * nrep * MSC_OBS_N);
+  std::vector<int> fl(4 * nrep);
+  HIPC(hipMemcpyAsync(fl.data(), ctx->d_flags, sizeof(int) * fl.size(), hipMemcpyDeviceToHost, ctx->stream));
+  if (obs)
+    HIPC(hipMemcpyAsync(obs, d_obs_all, sizeof(double) * (size_t)nsteps * nrep * MSC_OBS_N, hipMemcpyDeviceToHost,
+                        ctx->stream));
+  HIPC(hipStreamSynchronize(ctx->stream));
+  hipFree(d_gs_all);
+  hipFree(d_obs_all);
+  if (ctx->timing) settle_timers(ctx);
+  for (int r = 0; r < nrep; ++r) {
+    if (fl[4 * r + 2]) return fail(ctx, MSC_E_CAPACITY, "neighbour list region overflow during the run");
+    if (fl[4 * r + 0]) return fail(ctx, fl[4 * r] == 1 ? MSC_E_OVERLAP : MSC_E_DISTANCE, "overlap during the run");
+  }
+  return MSC_OK;
+}
+
+// --------------------------------------------------------------------------
+// measurement
+// --------------------------------------------------------------------------
+int msc_set_timing(msc_ctx *ctx, int on) {
+  if (!ctx) return MSC_E_ARG;
+  ctx->timing = on != 0;
+  return MSC_OK;
+}
+int msc_reset_timing(msc_ctx *ctx) {
+  if (!ctx) return MSC_E_ARG;
+  settle_timers(ctx);
+  ctx->timers.clear();
+  return MSC_OK;
+}
+int msc_kernel_time(msc_ctx *ctx, const char *name, double *ms, long *launches) {
+  if (!ctx || !name) return MSC_E_ARG;
+  settle_timers(ctx);
+  auto it = ctx->timers.find(name);
+  if (ms) *ms = it == ctx->timers.end() ? 0 : it->second.ms;
+  if (launches) *launches = it == ctx->timers.end() ? 0 : it->second.launches;
+  return MSC_OK;
+}
+int msc_work_counts(msc_ctx *ctx, int rep, long counts[4]) {
+  if (!ctx || rep < 0 || rep >= ctx->nrep || !counts) return MSC_E_ARG;
+  double c[4];
+  HIPC(hipStreamSynchronize(ctx->stream));
+  HIPC(hipMemcpy(c, ctx->d_counts + 4 * rep, sizeof c, hipMemcpyDeviceToHost));
+  const RepDev &R = ctx->reps[rep];
+  long nk = ((long)(2 * R.kmax + 1) * (2 * R.kmax + 1) * (2 * R.kmax + 1) - 1) / 2;
+  counts[0] = (long)c[0];
+  counts[1] = (long)c[1];
+  counts[2] = nk;
+  std::vector<int> cnt(R.N);
+  HIPC(hipMemcpy(cnt.data(), ctx->d_nl_cnt + R.n0, sizeof(int) * R.N, hipMemcpyDeviceToHost));
+  long t = 0;
+  for (int v : cnt) t += v;
+  counts[3] = t / 2;
+  return MSC_OK;
+}
+
+int msc_cluster_labels(msc_ctx *ctx, int rep, double min_s, unsigned int *mt_state, int *mt_index, int *labels,
+                       int *nclust) {
+  (void)min_s; (void)mt_state; (void)mt_index; (void)labels; (void)nclust;
+  if (!ctx || rep < 0 || rep >= ctx->nrep) return MSC_E_ARG;
+  return fail(ctx, MSC_E_STATE, "cluster labels are not built yet");
+}
+
+}  // extern "C"

@@ -1,0 +1,70 @@
+// Shared definitions of the MI355X force-path library (device and host).
+//
+// Device state is structure-of-arrays over the concatenation of all replicas
+// ("global particle index" g); vector fields are stored component-major,
+// v[c * Ntot + g], so that a wavefront of 64 consecutive particles reads
+// contiguous 512-byte lines.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace msc {
+
+constexpr int kWave = 64;
+constexpr int kTabN = 32768;               // table length of tabul
+constexpr double kTabExt = 200 * 1.001;    // table span of tabul/force_dipR
+constexpr int kMaxKZ = 16;                 // largest reference K input supported
+
+// Per-replica parameters as the kernels see them.
+struct RepDev {
+  int n0, N;            // first global particle, count
+  int nd;               // dipole slots per particle
+  int K, kmax, KZ;      // reference K input, kmax = K-1, KZ = K
+  int ncol;             // (2 kmax + 1)^2 columns (nx, ny) of the k half space
+  int slot0;            // first slot (ncol * KZ per replica) in the k arrays
+  int tab0;             // first row of this replica's real-space table
+  int cell0, ncell;     // first cell and cell count of this replica's grid
+  int nc[3];            // cells per dimension (1 if the box is too small)
+  int nl_base, nl_cap;  // WCA neighbour-list region (entries)
+  int dl_base, dl_cap;  // real-space pair-list region (entries)
+  long part0;           // first element of this replica's structure-factor partials
+  int task0, ntask;     // wave tasks (64 particles) of this replica
+  int rtask0, nrtask;   // reciprocal chunk tasks of this replica
+  double L[3], V, P2[3];
+  double cw[3];         // cell widths
+  double alpha, rc, sr_cut;
+  double r_list_min, r_list_max;
+  double rl_dip;        // real-space pair-list radius (rc + skin)
+  double lmax;          // largest cylinder length
+  double B[3];
+  double temp, dt, Q, gamma, dens;
+  double G[9];          // sum over half-space k of val_k k k^T
+  double Uself;
+};
+
+// A wave task: up to 64 consecutive particles of one replica.
+struct Task {
+  int rep, g0, n, pad;
+};
+
+// Per-replica energies of one force evaluation (device resident).
+struct RepEnergy {
+  double Usc, Wsc, PTsc[9];
+  double UR, PTR[9];
+  double UK, PTK[9];
+  double Uext;
+  double Kin[2], PTkin[9];
+  double pad[3];
+};
+
+// Thermostat state of a replica (two thermostats: translation, rotation).
+struct Thermo {
+  double s[2], strn[2];
+};
+
+#define MSC_HD __host__ __device__ __forceinline__
+
+MSC_HD double wrapd(double d, double L) { return d - L * floor(d / L + 0.5); }
+
+}  // namespace msc

@@ -1,0 +1,1288 @@
+// Kernels of the MI355X force path.  Compiled with -ffp-contract=off: the
+// FP64 kernels that mirror reference routines keep every product separately
+// rounded; the reciprocal kernels request fused multiply-adds explicitly.
+#include "msc_kernels.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include "msc_geom.hpp"
+
+namespace msc {
+
+// --------------------------------------------------------------------------
+// small helpers
+// --------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+  return v;
+}
+// fixed-order sum of a value over the 64 lanes of a wave (tree on lane
+// index), identical on every run
+__device__ __forceinline__ void wave_store_sums(double *dst, const double *v, int n, int lane) {
+  for (int k = 0; k < n; ++k) {
+    double s = wave_sum(v[k]);
+    if (lane == 0) dst[k] = s;
+  }
+}
+
+// Task of the current wave (4 waves per 256-thread block).
+__device__ __forceinline__ int wave_task_id() { return blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; }
+
+// --------------------------------------------------------------------------
+// dipole lab frame: position r + dpc * u, orientation O^T c
+// (Spherocyl::set_dip / translate / rotate, v25/classes.cpp:148-304)
+// --------------------------------------------------------------------------
+__global__ void k_dipoles(State s) {
+  int g = s.g_begin + blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= s.g_end) return;
+  const int N = s.Ntot;
+  double O[9];
+  for (int k = 0; k < 9; ++k) O[k] = s.ori[k * N + g];
+  double r[3] = {s.x[g], s.x[N + g], s.x[2 * N + g]};
+  for (int w = 0; w < s.nd; ++w) {
+    int q = w * N + g;
+    double pc = s.dpc[q];
+    double c[3] = {s.doc[(3 * w + 0) * N + g], s.doc[(3 * w + 1) * N + g], s.doc[(3 * w + 2) * N + g]};
+    for (int a = 0; a < 3; ++a) {
+      s.dpos[(3 * w + a) * N + g] = r[a] + pc * O[6 + a];
+      double acc = 0.0;
+      acc += O[0 * 3 + a] * c[0];
+      acc += O[1 * 3 + a] * c[1];
+      acc += O[2 * 3 + a] * c[2];
+      s.dori[(3 * w + a) * N + g] = acc;
+    }
+  }
+}
+
+// --------------------------------------------------------------------------
+// cell grid (integer binning of wrapped centres) and neighbour lists
+// --------------------------------------------------------------------------
+__global__ void k_cells_assign(State s, const int *__restrict__ rebuild) {
+  int g = s.g_begin + blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= s.g_end) return;
+  int rp = s.rep_of[g];
+  if (!rebuild[rp]) return;
+  const RepDev &R = s.reps[rp];
+  int ci[3];
+  for (int a = 0; a < 3; ++a) {
+    double x = s.x[a * s.Ntot + g];
+    double w = x - R.L[a] * floor(x / R.L[a]);
+    int c = (int)floor(w / R.cw[a]);
+    c = c < 0 ? 0 : (c >= R.nc[a] ? R.nc[a] - 1 : c);
+    ci[a] = c;
+  }
+  int cell = R.cell0 + (ci[2] * R.nc[1] + ci[1]) * R.nc[0] + ci[0];
+  s.cell_of[g] = cell;
+  atomicAdd(&s.cell_cnt[cell], 1);
+}
+
+__global__ void k_cells_fill(State s, const int *__restrict__ rebuild) {
+  int g = s.g_begin + blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= s.g_end) return;
+  if (!rebuild[s.rep_of[g]]) return;
+  int c = s.cell_of[g];
+  int pos = s.cell_start[c] + atomicAdd(&s.cell_fill[c], 1);
+  s.cell_list[pos] = g;
+}
+
+// ascending particle order inside each cell makes every later traversal
+// deterministic
+__global__ void k_cells_sort(State s, const int *__restrict__ cell_rep, const int *__restrict__ rebuild) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= s.ncells) return;
+  if (!rebuild[cell_rep[c]]) return;
+  int b = s.cell_start[c], n = s.cell_cnt[c];
+  int *v = s.cell_list + b;
+  for (int i = 1; i < n; ++i) {
+    int key = v[i], j = i - 1;
+    while (j >= 0 && v[j] > key) { v[j + 1] = v[j]; --j; }
+    v[j + 1] = key;
+  }
+}
+
+// Exclusive scan of one segment per replica (one block of 1024 threads per
+// replica, each thread owning a contiguous run).  mode 0: cell counts ->
+// cell starts (base = the replica's first particle); mode 1 / 2: WCA /
+// real-space row counts -> row starts in the replica's list region.  A
+// replica whose rebuild flag is clear is left untouched.  Rows that would
+// overflow the region set flags[rep*4+2] and are emptied.
+__global__ void __launch_bounds__(1024) k_scan_seg(int *__restrict__ cnt, int *__restrict__ out,
+                                                   const RepDev *__restrict__ reps, int mode,
+                                                   const int *__restrict__ rebuild, int *__restrict__ flags) {
+  __shared__ int sh[1024];
+  const int rp = blockIdx.x;
+  if (!rebuild[rp]) return;
+  const RepDev &R = reps[rp];
+  int b0, n, base, cap;
+  if (mode == 0) { b0 = R.cell0; n = R.ncell; base = R.n0; cap = R.N; }
+  else if (mode == 1) { b0 = R.n0; n = R.N; base = R.nl_base; cap = R.nl_cap; }
+  else { b0 = R.n0; n = R.N; base = R.dl_base; cap = R.dl_cap; }
+  int t = threadIdx.x;
+  int seg = (n + 1023) / 1024;
+  int b = t * seg, e = min(n, b + seg);
+  int sum = 0;
+  for (int i = b; i < e; ++i) sum += cnt[b0 + i];
+  sh[t] = sum;
+  __syncthreads();
+  for (int o = 1; o < 1024; o <<= 1) {
+    int v = (t >= o) ? sh[t - o] : 0;
+    __syncthreads();
+    sh[t] += v;
+    __syncthreads();
+  }
+  const int total = sh[1023];
+  if (total > cap) {
+    if (t == 0) flags[rp * 4 + 2] = total;
+    for (int i = b; i < e; ++i) { cnt[b0 + i] = 0; out[b0 + i] = base; }
+    return;
+  }
+  int run = base + sh[t] - sum;
+  for (int i = b; i < e; ++i) {
+    int v = cnt[b0 + i];
+    out[b0 + i] = run;
+    run += v;
+  }
+}
+
+// Neighbour lists from the cell grid.  For particle g, every candidate j of
+// the same replica in the 27 (or fewer) surrounding cells is tested:
+//  - WCA list: segment distance < r_list_max, evaluated exactly as
+//    neigh_list does for the pair (min(g,j), max(g,j)) (v25/functions.cpp:1310-1331);
+//    a cheap centre-distance bound skips pairs that cannot qualify;
+//  - dipolar list (cut-off mode): dipole-pair centre distance < rl_dip.
+// pass 0 counts, pass 1 writes rows at the scanned offsets.  Rows hold both
+// directions (full lists) in stencil order.
+template <int PASS>
+__global__ void __launch_bounds__(128) k_build_lists(State s, const int *__restrict__ rebuild) {
+  int g = s.g_begin + blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= s.g_end) return;
+  int rp = s.rep_of[g];
+  if (!rebuild[rp]) return;
+  const RepDev &R = s.reps[rp];
+  const int N = s.Ntot;
+  double rg[3] = {s.x[g], s.x[N + g], s.x[2 * N + g]};
+  double ug[3] = {s.ori[6 * N + g], s.ori[7 * N + g], s.ori[8 * N + g]};
+  double Lg = s.len[g];
+  int ci[3];
+  {
+    int c = s.cell_of[g] - R.cell0;
+    ci[0] = c % R.nc[0];
+    ci[1] = (c / R.nc[0]) % R.nc[1];
+    ci[2] = c / (R.nc[0] * R.nc[1]);
+  }
+  const double rl2 = R.r_list_max * R.r_list_max;
+  const bool dip = R.rc > 0;
+  const double rld = R.rl_dip;
+  int nw = 0, ndl = 0;
+  int wbase = 0, dbase = 0;
+  if (PASS == 1) {
+    wbase = s.nl_start[g];
+    dbase = dip ? s.dl_start[g] : 0;
+  }
+  int lo[3], hi[3];
+  for (int a = 0; a < 3; ++a) {
+    if (R.nc[a] >= 3) { lo[a] = -1; hi[a] = 1; }
+    else { lo[a] = 0; hi[a] = 0; }
+  }
+  for (int dz = lo[2]; dz <= hi[2]; ++dz)
+    for (int dy = lo[1]; dy <= hi[1]; ++dy)
+      for (int dx = lo[0]; dx <= hi[0]; ++dx) {
+        int cx = ci[0] + dx, cy = ci[1] + dy, cz = ci[2] + dz;
+        cx = cx < 0 ? cx + R.nc[0] : (cx >= R.nc[0] ? cx - R.nc[0] : cx);
+        cy = cy < 0 ? cy + R.nc[1] : (cy >= R.nc[1] ? cy - R.nc[1] : cy);
+        cz = cz < 0 ? cz + R.nc[2] : (cz >= R.nc[2] ? cz - R.nc[2] : cz);
+        int c = R.cell0 + (cz * R.nc[1] + cy) * R.nc[0] + cx;
+        int b = s.cell_start[c], e = b + s.cell_cnt[c];
+        for (int q = b; q < e; ++q) {
+          int j = s.cell_list[q];
+          if (j == g) continue;
+          double rj[3] = {s.x[j], s.x[N + j], s.x[2 * N + j]};
+          double rgj[3];
+          for (int a = 0; a < 3; ++a) rgj[a] = wrapd(rg[a] - rj[a], R.L[a]);
+          double rc2 = dot3(rgj, rgj);
+          double reach = R.r_list_max + 0.5 * (Lg + s.len[j]) + 1e-9;
+          if (rc2 < reach * reach * (1 + 1e-12)) {
+            // canonical orientation: A = lower index, rAB = r_A - r_B
+            int A = g < j ? g : j, Bi = g < j ? j : g;
+            double rA[3] = {s.x[A], s.x[N + A], s.x[2 * N + A]};
+            double rB[3] = {s.x[Bi], s.x[N + Bi], s.x[2 * N + Bi]};
+            double uA[3] = {s.ori[6 * N + A], s.ori[7 * N + A], s.ori[8 * N + A]};
+            double uB[3] = {s.ori[6 * N + Bi], s.ori[7 * N + Bi], s.ori[8 * N + Bi]};
+            double rAB[3], dAB[3], st[2];
+            for (int a = 0; a < 3; ++a) {
+              rAB[a] = rA[a] - rB[a];
+              rAB[a] -= R.L[a] * floor(rAB[a] / R.L[a] + 0.5);
+            }
+            double d2 = segdist(rA, uA, s.len[A], rB, uB, s.len[Bi], R.L, rAB, dAB, st);
+            if (d2 < rl2) {
+              if (PASS == 1 && nw < s.nl_cnt[g]) s.nl_idx[wbase + nw] = j;
+              ++nw;
+            }
+          }
+          if (dip) {
+            // pairs whose dipoles can come within rc before the next rebuild
+            double reachd = rld + 2.0 * s.lever_max;
+            if (rc2 < reachd * reachd) {
+              if (PASS == 1 && ndl < s.dl_cnt[g]) s.dl_idx[dbase + ndl] = j;
+              ++ndl;
+            }
+          }
+        }
+      }
+  if (PASS == 0) {
+    s.nl_cnt[g] = nw;
+    if (dip) s.dl_cnt[g] = ndl;
+  }
+}
+
+// --------------------------------------------------------------------------
+// savec_list / check_list (v25/functions.cpp:1336-1394)
+// --------------------------------------------------------------------------
+// nearest double to x rounded to 12 decimals (text round trip of savec_list)
+__device__ double round12(double x) {
+  double p = x * 1e12;
+  double e = fma(x, 1e12, -p);  // exact residual of the product
+  double m = floor(p);
+  double f = p - m;             // exact, |p| < 2^52
+  if (f > 0.5 || (f == 0.5 && (e > 0 || (e == 0 && fmod(m, 2.0) != 0)))) m += 1;
+  return m / 1e12;
+}
+
+__global__ void k_savec(State s, const int *__restrict__ rebuild) {
+  int g = s.g_begin + blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= s.g_end) return;
+  if (!rebuild[s.rep_of[g]]) return;
+  for (int a = 0; a < 3; ++a) s.snap[a * s.Ntot + g] = round12(s.x[a * s.Ntot + g]);
+}
+
+// per-task maximum of |min-image displacement component| since the snapshot
+__global__ void __launch_bounds__(256) k_check_list(State s, double *__restrict__ task_out) {
+  int tid = s.task_begin + wave_task_id();
+  int lane = threadIdx.x & 63;
+  if (wave_task_id() >= s.ntask) return;
+  Task t = s.tasks[tid];
+  const RepDev &R = s.reps[t.rep];
+  double m = 0.0;
+  if (lane < t.n) {
+    int g = t.g0 + lane;
+    for (int a = 0; a < 3; ++a) {
+      double D = s.x[a * s.Ntot + g] - s.snap[a * s.Ntot + g];
+      D -= R.L[a] * floor(D / R.L[a] + 0.5);
+      m = fmax(m, fabs(D));
+    }
+  }
+  m = wave_max(m);
+  if (lane == 0) task_out[tid] = m;
+}
+
+__global__ void k_check_decide(State s, const double *__restrict__ task_max, int *__restrict__ rebuild,
+                               double *__restrict__ obs_rebuild, int obs_stride) {
+  int rp = s.rep_begin + blockIdx.x * blockDim.x + threadIdx.x;
+  if (rp >= s.rep_end) return;
+  const RepDev &R = s.reps[rp];
+  double dm = 0.0;
+  for (int t = R.task0; t < R.task0 + R.ntask; ++t) dm = fmax(dm, task_max[t]);
+  dm = 2. * sqrt(3. * dm * dm);
+  int rb = dm > R.r_list_max - R.r_list_min;
+  rebuild[rp] = rb;
+  if (obs_rebuild) obs_rebuild[(size_t)rp * obs_stride] = rb;
+}
+
+// --------------------------------------------------------------------------
+// force_sc: WCA on the segment distance (v25/functions.cpp:1397-1553)
+// --------------------------------------------------------------------------
+// STRICT: every list pair runs spherocyl_dist and both abort tests, as the
+// reference.  Otherwise a centre-distance bound skips pairs that cannot be
+// inside 2^(1/6) sigma (their force is exactly zero).
+template <bool STRICT>
+__global__ void __launch_bounds__(256) k_force_sc(State s, double *__restrict__ task_out, int *__restrict__ flags) {
+  int tid = s.task_begin + wave_task_id();
+  int lane = threadIdx.x & 63;
+  if (wave_task_id() >= s.ntask) return;
+  Task tk = s.tasks[tid];
+  const RepDev &R = s.reps[tk.rep];
+  const int N = s.Ntot;
+  double acc[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};  // U, W, PT[9], pairs
+  if (lane < tk.n) {
+    const int g = tk.g0 + lane;
+    const double srt = pow(2., 1. / 6);
+    const double srtsq = srt * srt;
+    const double r_cut_sq = R.sr_cut * R.sr_cut;
+    double F[3] = {0, 0, 0}, T[3] = {0, 0, 0};
+    int code = 0;
+    double rg[3] = {s.x[g], s.x[N + g], s.x[2 * N + g]};
+    int b = s.nl_start[g], e = b + s.nl_cnt[g];
+    for (int q = b; q < e; ++q) {
+      int j = s.nl_idx[q];
+      double rj[3] = {s.x[j], s.x[N + j], s.x[2 * N + j]};
+      double sigma = (s.dia[g] + s.dia[j]) / 2;
+      double sig_sq = sigma * sigma;
+      if (!STRICT) {
+        double c2 = 0;
+        for (int a = 0; a < 3; ++a) {
+          double d = wrapd(rg[a] - rj[a], R.L[a]);
+          c2 += d * d;
+        }
+        double reach = srt * sigma + 0.5 * (s.len[g] + s.len[j]) + 1e-9;
+        if (c2 >= reach * reach * (1 + 1e-12)) continue;
+      }
+      const bool gA = g < j;
+      const int A = gA ? g : j, Bi = gA ? j : g;
+      double rA[3] = {s.x[A], s.x[N + A], s.x[2 * N + A]};
+      double rB[3] = {s.x[Bi], s.x[N + Bi], s.x[2 * N + Bi]};
+      double uA[3] = {s.ori[6 * N + A], s.ori[7 * N + A], s.ori[8 * N + A]};
+      double uB[3] = {s.ori[6 * N + Bi], s.ori[7 * N + Bi], s.ori[8 * N + Bi]};
+      double LA = s.len[A], LB = s.len[Bi];
+      double rsh = srtsq * sig_sq;
+      double rij[3], rsq = 0;
+      for (int a = 0; a < 3; ++a) {
+        rij[a] = rA[a] - rB[a];
+        rij[a] -= R.L[a] * floor(rij[a] / R.L[a] + 0.5);
+        rsq += rij[a] * rij[a];
+      }
+      double st[2], dij[3];
+      double dsq = segdist(rA, uA, LA, rB, uB, LB, R.L, rij, dij, st);
+      if (dsq < 0.49 * sig_sq) code = code ? code : 1;
+      if (dsq > rsq * 1.00001) code = code ? code : 2;
+      if (dsq < r_cut_sq && dsq < rsh) {
+        double SR2s = sig_sq / dsq;
+        double SR6s = SR2s * SR2s * SR2s;
+        double pot = 4 * SR6s * (SR6s - 1.);
+        double ddr = dot3(rij, dij);
+        double UIJ = pot + 1.;
+        double WIJ = 6 * (pot + 4 * SR6s * SR6s) * ddr / dsq;
+        double FIJ = WIJ / ddr;
+        double f[3];
+        for (int a = 0; a < 3; ++a) f[a] = FIJ * dij[a];
+        double icroj[3], icror[3], jcror[3];
+        cross3(uA, uB, icroj);
+        cross3(uA, rij, icror);
+        cross3(uB, rij, jcror);
+        double fact1 = (st[0] - 0.5) * LA;
+        double fact2 = -(st[1] - 0.5) * LB;
+        if (st[0] == -1.e+6) fact1 = fact2 = 0.;
+        if (st[0] == -1.e+10) { fact1 = 0.5 * LA; fact2 = 0.5 * LB; }
+        else if (st[0] == -2.e+10) { fact1 = -0.5 * LA; fact2 = -0.5 * LB; }
+        else if (st[0] == -3.e+10) { fact1 = 0.5 * LA; fact2 = -0.5 * LB; }
+        double fmul = fact1 * fact2;
+        if (gA) {
+          for (int a = 0; a < 3; ++a) F[a] += f[a];
+          for (int a = 0; a < 3; ++a) T[a] += FIJ * (fact1 * icror[a] + fmul * icroj[a]);
+          acc[0] += UIJ;
+          acc[1] += WIJ;
+          acc[11] += 1.0;
+          for (int a = 0; a < 3; ++a)
+            for (int c = 0; c < 3; ++c) acc[2 + 3 * a + c] += rij[a] * f[c];
+        } else {
+          for (int a = 0; a < 3; ++a) F[a] += -f[a];
+          for (int a = 0; a < 3; ++a) T[a] += FIJ * (fact2 * jcror[a] - fmul * icroj[a]);
+        }
+      }
+    }
+    for (int a = 0; a < 3; ++a) {
+      s.F[a * N + g] += F[a];
+      s.T[a * N + g] += T[a];
+    }
+    if (code) atomicCAS(&flags[tk.rep * 4 + 0], 0, code);
+  }
+  wave_store_sums(task_out + (size_t)tid * 16, acc, 12, lane);
+}
+
+// --------------------------------------------------------------------------
+// force_dipR: Ewald real-space dipole pairs (v25/functions.cpp:1555-1678)
+// --------------------------------------------------------------------------
+// One dipole pair evaluated for the side `side` (0: the lower particle k,
+// 1: the higher particle l) with k's dipole (pi, oi, mi, di) and l's dipole
+// (pj, oj, mj, dj); rij = pi - pj, minimum image.  Table rows are
+// interleaved (A, B, C) as tabul writes them.
+template <typename R>
+struct PairOut {
+  R f[3], t[3], U, pt[9];
+};
+
+template <typename R>
+__device__ __forceinline__ bool dip_pair(const R rij_in[3], const R oi[3], const R oj[3], R mi, R mj,
+                                         const R di[3], const R dj[3], const R *__restrict__ tab, R rc,
+                                         int side, bool want_energy, PairOut<R> &o) {
+  const R Lh = (R)(0.5 * kTabExt / kTabN);
+  R rij[3] = {rij_in[0], rij_in[1], rij_in[2]};
+  R RIJSQ = rij[0] * rij[0] + rij[1] * rij[1] + rij[2] * rij[2];
+  R uu = mi * mj;
+  R SR2 = (R)1 / RIJSQ;
+  R RIJ = sqrt(RIJSQ);
+  if (rc > 0 && !(RIJ < rc)) return false;
+  int ix = (int)floor(RIJ / Lh);
+  ix = ix < 1 ? 1 : (ix > kTabN - 1 ? kTabN - 1 : ix);
+  R R_L0 = RIJ - Lh * (R)ix;
+  R R_L1 = RIJ - Lh * (R)(ix + 1);
+  const R *T1 = tab + 3 * ix, *T0 = tab + 3 * (ix - 1);
+  R A = (T1[0] * R_L0 - T0[0] * R_L1) / Lh;
+  R B = (T1[1] * R_L0 - T0[1] * R_L1) / Lh;
+  R C = (T1[2] * R_L0 - T0[2] * R_L1) / Lh;
+  R UIR = oi[0] * rij[0] + oi[1] * rij[1] + oi[2] * rij[2];
+  R UJR = oj[0] * rij[0] + oj[1] * rij[1] + oj[2] * rij[2];
+  R UIJR = UIR * UJR;
+  R UIUJ = oi[0] * oj[0] + oi[1] * oj[1] + oi[2] * oj[2];
+  R fact = (UIUJ - (R)5. * UIJR * SR2) * B - UIJR * SR2 * C;
+  R f[3];
+  for (int a = 0; a < 3; ++a) f[a] = uu * (fact * rij[a] + (UIR * oj[a] + UJR * oi[a]) * B);
+  R icroj[3];
+  icroj[0] = oi[1] * oj[2] - oi[2] * oj[1];
+  icroj[1] = oi[2] * oj[0] - oi[0] * oj[2];
+  icroj[2] = oi[0] * oj[1] - oi[1] * oj[0];
+  for (int a = 0; a < 3; ++a) icroj[a] *= A;
+  R fact2 = -UIR * UJR * ((R)5 * B + C) * SR2 + UIUJ * B;
+  if (side == 0) {
+    R icror[3], dicror[3], dicroi[3], dicroj[3];
+    icror[0] = oi[1] * rij[2] - oi[2] * rij[1]; icror[1] = oi[2] * rij[0] - oi[0] * rij[2]; icror[2] = oi[0] * rij[1] - oi[1] * rij[0];
+    dicror[0] = di[1] * rij[2] - di[2] * rij[1]; dicror[1] = di[2] * rij[0] - di[0] * rij[2]; dicror[2] = di[0] * rij[1] - di[1] * rij[0];
+    dicroi[0] = di[1] * oi[2] - di[2] * oi[1]; dicroi[1] = di[2] * oi[0] - di[0] * oi[2]; dicroi[2] = di[0] * oi[1] - di[1] * oi[0];
+    dicroj[0] = di[1] * oj[2] - di[2] * oj[1]; dicroj[1] = di[2] * oj[0] - di[0] * oj[2]; dicroj[2] = di[0] * oj[1] - di[1] * oj[0];
+    for (int a = 0; a < 3; ++a) {
+      o.f[a] = f[a];
+      R fact1 = (UJR * (icror[a] + dicroi[a]) + UIR * dicroj[a]) * B;
+      o.t[a] = uu * (fact2 * dicror[a] + fact1 - icroj[a]);
+    }
+    if (want_energy) {
+      o.U = uu * (UIUJ * A - UIJR * B);
+      for (int a = 0; a < 3; ++a)
+        for (int c = 0; c < 3; ++c) o.pt[3 * a + c] = rij[a] * f[c];
+    }
+  } else {
+    R jcror[3], djcror[3], djcroj[3], djcroi[3];
+    jcror[0] = oj[1] * rij[2] - oj[2] * rij[1]; jcror[1] = oj[2] * rij[0] - oj[0] * rij[2]; jcror[2] = oj[0] * rij[1] - oj[1] * rij[0];
+    djcror[0] = dj[1] * rij[2] - dj[2] * rij[1]; djcror[1] = dj[2] * rij[0] - dj[0] * rij[2]; djcror[2] = dj[0] * rij[1] - dj[1] * rij[0];
+    djcroj[0] = dj[1] * oj[2] - dj[2] * oj[1]; djcroj[1] = dj[2] * oj[0] - dj[0] * oj[2]; djcroj[2] = dj[0] * oj[1] - dj[1] * oj[0];
+    djcroi[0] = dj[1] * oi[2] - dj[2] * oi[1]; djcroi[1] = dj[2] * oi[0] - dj[0] * oi[2]; djcroi[2] = dj[0] * oi[1] - dj[1] * oi[0];
+    for (int a = 0; a < 3; ++a) {
+      o.f[a] = -f[a];
+      R fact1 = (UIR * (jcror[a] - djcroj[a]) - UJR * djcroi[a]) * B;
+      o.t[a] = uu * (-fact2 * djcror[a] + fact1 + icroj[a]);
+    }
+  }
+  return true;
+}
+
+// Dipole data of particle g, slot w, in real type R.
+template <typename R>
+struct Dip {
+  double pos[3];  // lab position (FP64: the min image is taken in FP64)
+  R o[3], d[3], mu;
+};
+
+template <typename R>
+__device__ __forceinline__ Dip<R> load_dip(const State &s, int g, int w) {
+  const int N = s.Ntot;
+  Dip<R> D;
+  for (int a = 0; a < 3; ++a) {
+    D.pos[a] = s.dpos[(3 * w + a) * N + g];
+    D.o[a] = (R)s.dori[(3 * w + a) * N + g];
+    D.d[a] = (R)(s.dpc[w * N + g] * s.ori[(6 + a) * N + g]);
+  }
+  D.mu = (R)s.dmu[w * N + g];
+  return D;
+}
+
+// Accumulate the real-space interaction of particle g with candidate j
+// (all dipole slot pairs), canonical k = min(g, j).
+template <typename R>
+__device__ __forceinline__ void dipR_partner(const State &s, const RepDev &Rp, const R *__restrict__ tab, int g,
+                                             int j, double F[3], double T[3], double acc[11]) {
+  const bool gk = g < j;
+  const int k = gk ? g : j, l = gk ? j : g;
+  for (int v = 0; v < s.nd; ++v) {
+    Dip<R> Di = load_dip<R>(s, k, v);
+    if (Di.mu == (R)0) continue;
+    for (int w = 0; w < s.nd; ++w) {
+      Dip<R> Dj = load_dip<R>(s, l, w);
+      if (Dj.mu == (R)0) continue;
+      R rij[3];
+      for (int a = 0; a < 3; ++a) {
+        double d = Di.pos[a] - Dj.pos[a];
+        d -= Rp.L[a] * floor(d / Rp.L[a] + 0.5);
+        rij[a] = (R)d;
+      }
+      PairOut<R> o;
+      if (!dip_pair<R>(rij, Di.o, Dj.o, Di.mu, Dj.mu, Di.d, Dj.d, tab, (R)Rp.rc, gk ? 0 : 1, gk, o)) continue;
+      for (int a = 0; a < 3; ++a) {
+        F[a] += (double)o.f[a];
+        T[a] += (double)o.t[a];
+      }
+      if (gk) {
+        acc[0] += (double)o.U;
+        for (int q = 0; q < 9; ++q) acc[1 + q] += (double)o.pt[q];
+        acc[10] += 1.0;
+      }
+    }
+  }
+}
+
+// all minimum-image pairs of the replica (reference mode)
+template <typename R>
+__global__ void __launch_bounds__(256) k_dipR_all(State s, const R *__restrict__ tabs, double *__restrict__ task_out) {
+  int tid = s.task_begin + wave_task_id();
+  int lane = threadIdx.x & 63;
+  if (wave_task_id() >= s.ntask) return;
+  Task tk = s.tasks[tid];
+  const RepDev &Rp = s.reps[tk.rep];
+  const R *tab = tabs + (size_t)Rp.tab0 * 3;
+  double acc[11] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  if (lane < tk.n) {
+    const int g = tk.g0 + lane;
+    double F[3] = {0, 0, 0}, T[3] = {0, 0, 0};
+    for (int j = Rp.n0; j < Rp.n0 + Rp.N; ++j) {
+      if (j == g) continue;
+      dipR_partner<R>(s, Rp, tab, g, j, F, T, acc);
+    }
+    for (int a = 0; a < 3; ++a) {
+      s.F[a * s.Ntot + g] += F[a];
+      s.T[a * s.Ntot + g] += T[a];
+    }
+  }
+  wave_store_sums(task_out + (size_t)tid * 16, acc, 11, lane);
+}
+
+// cut-off mode over the real-space pair list
+template <typename R>
+__global__ void __launch_bounds__(256) k_dipR_list(State s, const R *__restrict__ tabs, double *__restrict__ task_out) {
+  int tid = s.task_begin + wave_task_id();
+  int lane = threadIdx.x & 63;
+  if (wave_task_id() >= s.ntask) return;
+  Task tk = s.tasks[tid];
+  const RepDev &Rp = s.reps[tk.rep];
+  const R *tab = tabs + (size_t)Rp.tab0 * 3;
+  double acc[11] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  if (lane < tk.n) {
+    const int g = tk.g0 + lane;
+    double F[3] = {0, 0, 0}, T[3] = {0, 0, 0};
+    int b = s.dl_start[g], e = b + s.dl_cnt[g];
+    for (int q = b; q < e; ++q) dipR_partner<R>(s, Rp, tab, g, s.dl_idx[q], F, T, acc);
+    for (int a = 0; a < 3; ++a) {
+      s.F[a * s.Ntot + g] += F[a];
+      s.T[a * s.Ntot + g] += T[a];
+    }
+  }
+  wave_store_sums(task_out + (size_t)tid * 16, acc, 11, lane);
+}
+
+// --------------------------------------------------------------------------
+// ext_field (v25/functions.cpp:1992-2025) and self_ener (2027-2069)
+// --------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_ext_field(State s, double *__restrict__ task_out) {
+  int tid = s.task_begin + wave_task_id();
+  int lane = threadIdx.x & 63;
+  if (wave_task_id() >= s.ntask) return;
+  Task tk = s.tasks[tid];
+  const RepDev &Rp = s.reps[tk.rep];
+  double U = 0;
+  if (lane < tk.n) {
+    const int g = tk.g0 + lane, N = s.Ntot;
+    double T[3] = {0, 0, 0};
+    for (int w = 0; w < s.nd; ++w) {
+      double o[3] = {s.dori[(3 * w) * N + g], s.dori[(3 * w + 1) * N + g], s.dori[(3 * w + 2) * N + g]};
+      double mu = s.dmu[w * N + g], t[3];
+      U -= mu * dot3(o, Rp.B);
+      cross3(o, Rp.B, t);
+      for (int a = 0; a < 3; ++a) T[a] += t[a] * mu;
+    }
+    for (int a = 0; a < 3; ++a) s.T[a * N + g] += T[a];
+  }
+  double v[1] = {U};
+  wave_store_sums(task_out + (size_t)tid * 16, v, 1, lane);
+}
+
+__global__ void __launch_bounds__(256) k_self_ener(State s, double *__restrict__ task_out) {
+  int tid = s.task_begin + wave_task_id();
+  int lane = threadIdx.x & 63;
+  if (wave_task_id() >= s.ntask) return;
+  Task tk = s.tasks[tid];
+  double U = 0;
+  if (lane < tk.n) {
+    const int g = tk.g0 + lane, N = s.Ntot;
+    for (int v = 0; v < s.nd - 1; ++v)
+      for (int w = v + 1; w < s.nd; ++w) {
+        double ri[3], rj[3], oi[3], oj[3], rij[3];
+        for (int a = 0; a < 3; ++a) {
+          ri[a] = s.dpos[(3 * v + a) * N + g]; rj[a] = s.dpos[(3 * w + a) * N + g];
+          oi[a] = s.dori[(3 * v + a) * N + g]; oj[a] = s.dori[(3 * w + a) * N + g];
+          rij[a] = ri[a] - rj[a];
+        }
+        double uu = s.dmu[v * N + g] * s.dmu[w * N + g];
+        double RIJSQ = dot3(rij, rij), RIJ = sqrt(RIJSQ);
+        double SR2 = 1. / RIJSQ, SR3 = SR2 / RIJ;
+        double UIUJ = dot3(oi, oj), UIR = dot3(oi, rij), UJR = dot3(oj, rij);
+        U += uu * (UIUJ * SR3 - 3. * UIR * UJR * SR2 * SR3);
+      }
+  }
+  double vv[1] = {U};
+  wave_store_sums(task_out + (size_t)tid * 16, vv, 1, lane);
+}
+
+// --------------------------------------------------------------------------
+// Ewald reciprocal sum over the half space of k (one dipole slot per
+// particle).  k = 2 pi (nx/Lx, ny/Ly, nz/Lz); column c = (nx, ny) with
+// nx, ny in [-kmax, kmax], slot = c * KZ + nz, nz in [0, kmax]; slots outside
+// the half space (nz = 0 with ny < 0, or ny = 0 and nx <= 0) carry weight 0.
+// --------------------------------------------------------------------------
+template <typename R>
+struct Cpx { R re, im; };
+
+template <typename R>
+__device__ __forceinline__ R fmaR(R a, R b, R c) { return fma(a, b, c); }
+
+// per-particle phase tables e^{-i n theta_a}, n = 0..kmax, theta_a = 2 pi x_a / L_a of
+// the wrapped dipole position (as force_dipK wraps, v25/functions.cpp:1793-1794),
+// and the amplitude c = mu u; also per-task partial sums of c c^T.
+template <typename R>
+__global__ void __launch_bounds__(256) k_recip_tables(State s, Cpx<R> *__restrict__ ph, R *__restrict__ amp,
+                               double *__restrict__ task_out) {
+  int tid = s.task_begin + wave_task_id();
+  int lane = threadIdx.x & 63;
+  if (wave_task_id() >= s.ntask) return;
+  Task tk = s.tasks[tid];
+  const RepDev &Rp = s.reps[tk.rep];
+  double m2[6] = {0, 0, 0, 0, 0, 0};
+  if (lane < tk.n) {
+    const int g = tk.g0 + lane, N = s.Ntot;
+    for (int a = 0; a < 3; ++a) {
+      double x = s.dpos[a * N + g];
+      x -= Rp.L[a] * floor(x / Rp.L[a] + 0.5);
+      double th = Rp.P2[a] * x;
+      for (int n = 0; n < Rp.KZ; ++n) {
+        double sn, cs;
+        sincos(n * th, &sn, &cs);
+        Cpx<R> e;
+        e.re = (R)cs;
+        e.im = (R)(-sn);
+        ph[((size_t)a * kMaxKZ + n) * N + g] = e;
+      }
+    }
+    double mu = s.dmu[g];
+    double c[3] = {mu * s.dori[g], mu * s.dori[N + g], mu * s.dori[2 * N + g]};
+    for (int a = 0; a < 3; ++a) amp[a * N + g] = (R)c[a];
+    m2[0] = c[0] * c[0]; m2[1] = c[0] * c[1]; m2[2] = c[0] * c[2];
+    m2[3] = c[1] * c[1]; m2[4] = c[1] * c[2]; m2[5] = c[2] * c[2];
+  }
+  wave_store_sums(task_out + (size_t)tid * 16, m2, 6, lane);
+}
+
+// Structure factors Q_a(k) = sum_i c_a,i e^{-i k.r_i} over a chunk of
+// particles: one block per chunk, one thread per column, nz unrolled.
+// Particle tables are staged in LDS and read by all lanes at once.
+template <typename R, int KZ>
+__global__ void __launch_bounds__(256) k_recip_struct(State s, int rt0, const Cpx<R> *__restrict__ ph,
+                                                      const R *__restrict__ amp, R *__restrict__ part) {
+  __shared__ Cpx<R> sh[kChunk * 3 * KZ];
+  __shared__ R sa[kChunk * 3];
+  const int rtid = rt0 + blockIdx.x;
+  const Task rt = s.rtasks[rtid];
+  const RepDev &Rp = s.reps[rt.rep];
+  const int N = s.Ntot;
+  const int kmax = KZ - 1, side = 2 * kmax + 1;
+  const int col = blockIdx.y * blockDim.x + threadIdx.x;
+  const bool active = col < Rp.ncol;
+  const int cc = active ? col : 0;
+  const int nx = cc % side - kmax, ny = cc / side - kmax;
+  const int ax = nx < 0 ? -nx : nx, ay = ny < 0 ? -ny : ny;
+  const R sx = nx < 0 ? (R)-1 : (R)1, sy = ny < 0 ? (R)-1 : (R)1;
+  R Qr[KZ][3], Qi[KZ][3];
+#pragma unroll
+  for (int z = 0; z < KZ; ++z)
+#pragma unroll
+    for (int a = 0; a < 3; ++a) Qr[z][a] = Qi[z][a] = (R)0;
+  for (int p0 = 0; p0 < rt.n; p0 += kChunk) {
+    const int np = min(kChunk, rt.n - p0);
+    __syncthreads();
+    for (int q = threadIdx.x; q < np * 3 * KZ; q += blockDim.x) {
+      int p = q / (3 * KZ), r = q % (3 * KZ), a = r / KZ, n = r % KZ;
+      sh[q] = ph[((size_t)a * kMaxKZ + n) * N + rt.g0 + p0 + p];
+    }
+    for (int q = threadIdx.x; q < np * 3; q += blockDim.x) {
+      int p = q / 3, a = q % 3;
+      sa[q] = amp[a * N + rt.g0 + p0 + p];
+    }
+    __syncthreads();
+    if (!active) continue;
+    for (int p = 0; p < np; ++p) {
+      const Cpx<R> *t = sh + p * 3 * KZ;
+      Cpx<R> ex = t[ax], ey = t[KZ + ay];
+      ex.im *= sx;
+      ey.im *= sy;
+      R xr = fmaR(ex.re, ey.re, -ex.im * ey.im);
+      R xi = fmaR(ex.re, ey.im, ex.im * ey.re);
+      R c0 = sa[3 * p], c1 = sa[3 * p + 1], c2 = sa[3 * p + 2];
+#pragma unroll
+      for (int z = 0; z < KZ; ++z) {
+        Cpx<R> ez = t[2 * KZ + z];
+        R er = fmaR(xr, ez.re, -xi * ez.im);
+        R ei = fmaR(xr, ez.im, xi * ez.re);
+        Qr[z][0] = fmaR(c0, er, Qr[z][0]); Qi[z][0] = fmaR(c0, ei, Qi[z][0]);
+        Qr[z][1] = fmaR(c1, er, Qr[z][1]); Qi[z][1] = fmaR(c1, ei, Qi[z][1]);
+        Qr[z][2] = fmaR(c2, er, Qr[z][2]); Qi[z][2] = fmaR(c2, ei, Qi[z][2]);
+      }
+    }
+  }
+  if (!active) return;
+  // partials: part[part0 + ((chunk within replica) * nslot + slot) * 6]
+  const long nslot = (long)Rp.ncol * KZ;
+  R *dst = part + Rp.part0 + (((long)(rtid - Rp.rtask0)) * nslot + (long)col * KZ) * 6;
+#pragma unroll
+  for (int z = 0; z < KZ; ++z) {
+    dst[6 * z + 0] = Qr[z][0]; dst[6 * z + 1] = Qi[z][0];
+    dst[6 * z + 2] = Qr[z][1]; dst[6 * z + 3] = Qi[z][1];
+    dst[6 * z + 4] = Qr[z][2]; dst[6 * z + 5] = Qi[z][2];
+  }
+}
+
+// Per slot: Q = sum over the replica's chunks (FP64, fixed order), S = k.Q,
+// energy (val/2)(|S|^2 - k^T M2 k) and the pressure-tensor term of
+// force_dipK (v25/functions.cpp:1926-1952, with its a4 = 1/(4 alpha));
+// A = val conj(S) for the back projection.
+template <typename R>
+__global__ void k_recip_reduce(State s, const R *__restrict__ part, const double *__restrict__ M2,
+                               Cpx<R> *__restrict__ Aout, double *__restrict__ slot_out) {
+  const int rep = s.rep_begin + blockIdx.y;
+  const RepDev &Rp = s.reps[rep];
+  const int nslot = Rp.ncol * Rp.KZ;
+  int sl = blockIdx.x * blockDim.x + threadIdx.x;
+  if (sl >= nslot) return;
+  const int kmax = Rp.kmax, side = 2 * kmax + 1;
+  const int col = sl / Rp.KZ, nz = sl % Rp.KZ;
+  const int nx = col % side - kmax, ny = col / side - kmax;
+  double val = s.kval[Rp.slot0 + sl];
+  double *o = slot_out + (size_t)(Rp.slot0 + sl) * 10;
+  Cpx<R> A;
+  A.re = A.im = (R)0;
+  if (val == 0.0) {
+    for (int q = 0; q < 10; ++q) o[q] = 0.0;
+    Aout[Rp.slot0 + sl] = A;
+    return;
+  }
+  double Q[6] = {0, 0, 0, 0, 0, 0};
+  for (int c = 0; c < Rp.nrtask; ++c) {
+    const R *p = part + Rp.part0 + ((long)c * nslot + sl) * 6;
+    for (int q = 0; q < 6; ++q) Q[q] += (double)p[q];
+  }
+  double k[3] = {Rp.P2[0] * nx, Rp.P2[1] * ny, Rp.P2[2] * nz};
+  double Sr = k[0] * Q[0] + k[1] * Q[2] + k[2] * Q[4];
+  double Si = k[0] * Q[1] + k[1] * Q[3] + k[2] * Q[5];
+  const double *m = M2 + rep * 6;
+  double Mk[3] = {m[0] * k[0] + m[1] * k[1] + m[2] * k[2], m[1] * k[0] + m[3] * k[1] + m[4] * k[2],
+                  m[2] * k[0] + m[4] * k[1] + m[5] * k[2]};
+  double self = k[0] * Mk[0] + k[1] * Mk[1] + k[2] * Mk[2];
+  double nrm = Sr * Sr + Si * Si - self;
+  o[0] = val * 0.5 * nrm;
+  double Ksq = k[0] * k[0] + k[1] * k[1] + k[2] * k[2];
+  double pval = 2 * (1. / (4. * Rp.alpha) + 1. / Ksq);
+  for (int j = 0; j < 3; ++j)
+    for (int c = 0; c < 3; ++c) {
+      double re = Sr * Q[2 * j] + Si * Q[2 * j + 1];  // Re(conj(S) Q_j)
+      o[1 + 3 * j + c] = val * 0.5 * (((j == c) ? 1.0 : 0.0) - pval * k[j] * k[c]) * nrm +
+                         val * 0.5 * 2 * (re - Mk[j]) * k[c];
+    }
+  A.re = (R)(val * Sr);
+  A.im = (R)(-val * Si);
+  Aout[Rp.slot0 + sl] = A;
+}
+
+// Back projection: for each particle, F = -sum_k a_k Im(Z_k) k and
+// T = u x (-mu sum_k Re(Z_k) k + mu^2 G u + s_d F), Z_k = e^{-i k.r} A_k,
+// a_k = mu u.k.  One lane per particle; the nz loop is unrolled with the
+// particle's z phases in registers; column sums are carried in FP64.
+template <typename R, int KZ>
+__global__ void __launch_bounds__(256) k_recip_force(State s, const Cpx<R> *__restrict__ ph,
+                                                     const R *__restrict__ amp,
+                                                     const Cpx<R> *__restrict__ Aall) {
+  // per-lane x/y phase tables, indexed by the (uniform) loop counters
+  __shared__ Cpx<R> sxy[4][2][KZ][kWave];
+  int wv = threadIdx.x / kWave, lane = threadIdx.x & 63;
+  if (wave_task_id() >= s.ntask) return;
+  int tid = s.task_begin + wave_task_id();
+  Task tk = s.tasks[tid];
+  const RepDev &Rp = s.reps[tk.rep];
+  const int N = s.Ntot;
+  const int kmax = KZ - 1, side = 2 * kmax + 1;
+  const int g = tk.g0 + (lane < tk.n ? lane : 0);
+  for (int n = 0; n < KZ; ++n) {
+    sxy[wv][0][n][lane] = ph[((size_t)0 * kMaxKZ + n) * N + g];
+    sxy[wv][1][n][lane] = ph[((size_t)1 * kMaxKZ + n) * N + g];
+  }
+  Cpx<R> ez[KZ];
+#pragma unroll
+  for (int n = 0; n < KZ; ++n) ez[n] = ph[((size_t)2 * kMaxKZ + n) * N + g];
+  const R c0 = amp[g], c1 = amp[N + g], c2 = amp[2 * N + g];
+  const Cpx<R> *A = Aall + Rp.slot0;
+  const R P2x = (R)Rp.P2[0], P2y = (R)Rp.P2[1], P2z = (R)Rp.P2[2];
+  double Fx = 0, Fy = 0, Fz = 0, Hx = 0, Hy = 0, Hz = 0;
+  for (int col = 0; col < side * side; ++col) {
+    const int nx = col % side - kmax, ny = col / side - kmax;
+    const int ax = nx < 0 ? -nx : nx, ay = ny < 0 ? -ny : ny;
+    Cpx<R> ex = sxy[wv][0][ax][lane], ey = sxy[wv][1][ay][lane];
+    if (nx < 0) ex.im = -ex.im;
+    if (ny < 0) ey.im = -ey.im;
+    R xr = ex.re * ey.re - ex.im * ey.im;
+    R xi = fmaR(ex.re, ey.im, ex.im * ey.re);
+    const R kx = P2x * (R)nx, ky = P2y * (R)ny;
+    const R pcol = fmaR(c0, kx, c1 * ky);
+    R Gc = 0, Hc = 0, Gz = 0, Hzc = 0;
+    const Cpx<R> *Ac = A + col * KZ;
+#pragma unroll
+    for (int z = 0; z < KZ; ++z) {
+      const Cpx<R> Ak = Ac[z];
+      const R kz = P2z * (R)z;
+      R er = fmaR(xr, ez[z].re, -xi * ez[z].im);
+      R ei = fmaR(xr, ez[z].im, xi * ez[z].re);
+      R zr = fmaR(er, Ak.re, -ei * Ak.im);
+      R zi = fmaR(er, Ak.im, ei * Ak.re);
+      R a = fmaR(c2, kz, pcol);
+      R gg = a * zi;
+      Gc += gg;
+      Gz = fmaR(gg, kz, Gz);
+      Hc += zr;
+      Hzc = fmaR(zr, kz, Hzc);
+    }
+    Fx = fma((double)Gc, (double)kx, Fx);
+    Fy = fma((double)Gc, (double)ky, Fy);
+    Fz += (double)Gz;
+    Hx = fma((double)Hc, (double)kx, Hx);
+    Hy = fma((double)Hc, (double)ky, Hy);
+    Hz += (double)Hzc;
+  }
+  if (lane >= tk.n) return;
+  double F[3] = {-Fx, -Fy, -Fz};
+  double H[3] = {Hx, Hy, Hz};
+  double mu = s.dmu[g];
+  double u[3] = {s.dori[g], s.dori[N + g], s.dori[2 * N + g]};
+  double sd = s.dpc[g];
+  const double *G = Rp.G;
+  double Gu[3] = {G[0] * u[0] + G[1] * u[1] + G[2] * u[2], G[3] * u[0] + G[4] * u[1] + G[5] * u[2],
+                  G[6] * u[0] + G[7] * u[1] + G[8] * u[2]};
+  double w[3];
+  for (int a = 0; a < 3; ++a) w[a] = -mu * H[a] + mu * mu * Gu[a] + sd * F[a];
+  double t[3];
+  cross3(u, w, t);
+  for (int a = 0; a < 3; ++a) {
+    s.F[a * N + g] += F[a];
+    s.T[a * N + g] += t[a];
+  }
+}
+
+// --------------------------------------------------------------------------
+// deterministic per-replica reductions of per-task (or per-slot) partials
+// --------------------------------------------------------------------------
+// out[rep * ostride + ooff + k] = sum over the replica's items of
+// in[item * istride + k], k < nv.  One block of 256 threads per replica;
+// each thread sums a strided subset in order, then a fixed LDS tree.
+__global__ void k_reduce_items(const RepDev *__restrict__ reps, int rep0, const double *__restrict__ in, int istride,
+                               int nv, int which, double *__restrict__ out, int ostride, int ooff) {
+  __shared__ double sh[256];
+  const int rp = rep0 + blockIdx.x;
+  const RepDev &R = reps[rp];
+  int b, n;
+  if (which == 0) { b = R.task0; n = R.ntask; }
+  else { b = R.slot0; n = R.ncol * R.KZ; }
+  for (int k = 0; k < nv; ++k) {
+    double v = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) v += in[(size_t)(b + i) * istride + k];
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+      if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) out[(size_t)rp * ostride + ooff + k] = sh[0];
+    __syncthreads();
+  }
+}
+
+// --------------------------------------------------------------------------
+// integrator (v25/functions.cpp:2071-2218, v25/classes.cpp:228-325)
+// --------------------------------------------------------------------------
+__device__ __forceinline__ void mm3(const double *A, const double *B, double *C) {
+  double Rr[9];
+  for (int m = 0; m < 3; ++m)
+    for (int n = 0; n < 3; ++n) {
+      double acc = 0.0;
+      for (int i = 0; i < 3; ++i) acc += A[m * 3 + i] * B[i * 3 + n];
+      Rr[m * 3 + n] = acc;
+    }
+  for (int q = 0; q < 9; ++q) C[q] = Rr[q];
+}
+__device__ __forceinline__ void mv3(const double *A, const double *x, double *y) {
+  double r[3];
+  for (int m = 0; m < 3; ++m) {
+    double acc = 0.0;
+    for (int i = 0; i < 3; ++i) acc += A[m * 3 + i] * x[i];
+    r[m] = acc;
+  }
+  y[0] = r[0]; y[1] = r[1]; y[2] = r[2];
+}
+
+// moveP for every particle; thermostat coordinates read from device memory.
+// Per-task partial sums: [0] sum p^2/m, [1] sum l^T Ib l, [2..7] kinetic PT.
+template <bool LAST>
+__global__ void __launch_bounds__(256) k_moveP(State s, const Thermo *__restrict__ th, double *__restrict__ task_out) {
+  int tid = s.task_begin + wave_task_id();
+  int lane = threadIdx.x & 63;
+  if (wave_task_id() >= s.ntask) return;
+  Task tk = s.tasks[tid];
+  const RepDev &Rp = s.reps[tk.rep];
+  double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  if (lane < tk.n) {
+    const int g = tk.g0 + lane, N = s.Ntot;
+    const double Dt2 = Rp.dt / 2;
+    double m = s.mass[g];
+    double I[3] = {s.inert[g], s.inert[N + g], s.inert[2 * N + g]};
+    double O[9], Ot[9], I0[9] = {1. / I[0], 0., 0., 0., 1. / I[1], 0., 0., 0., 1. / I[2]}, Ax[9], Ib[9];
+    for (int k = 0; k < 9; ++k) O[k] = s.ori[k * N + g];
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b) Ot[a * 3 + b] = O[b * 3 + a];
+    mm3(Ot, I0, Ax);
+    mm3(Ax, O, Ib);
+    double p[3], l[3], psq = 0;
+    const Thermo T = th[tk.rep];
+    if (!LAST) {
+      double sfac = 1. + Dt2 * T.s[0];
+      for (int j = 0; j < 3; ++j) {
+        p[j] = (s.p[j * N + g] + Dt2 * s.F[j * N + g]) / sfac;
+        psq += p[j] * p[j];
+      }
+      sfac = 1. + Dt2 * T.s[1];
+      for (int j = 0; j < 3; ++j) l[j] = (s.l[j * N + g] + Dt2 * s.T[j * N + g]) / sfac;
+    } else {
+      double sfac = 1. - Dt2 * T.s[0];
+      for (int j = 0; j < 3; ++j) {
+        p[j] = s.p[j * N + g] * sfac + Dt2 * s.F[j * N + g];
+        psq += p[j] * p[j];
+      }
+      acc[2] = p[0] * p[0] / m; acc[3] = p[0] * p[1] / m; acc[4] = p[0] * p[2] / m;
+      acc[5] = p[1] * p[1] / m; acc[6] = p[1] * p[2] / m; acc[7] = p[2] * p[2] / m;
+      sfac = 1. - Dt2 * T.s[1];
+      for (int j = 0; j < 3; ++j) l[j] = s.l[j * N + g] * sfac + Dt2 * s.T[j * N + g];
+    }
+    double v[3];
+    for (int n = 0; n < 3; ++n) {
+      double a2 = 0.0;
+      for (int k = 0; k < 3; ++k) a2 += l[k] * Ib[k * 3 + n];
+      v[n] = a2;
+    }
+    double lres = 0.0;
+    for (int k = 0; k < 3; ++k) lres += v[k] * l[k];
+    double lsq = 0.0;
+    lsq += lres;
+    for (int j = 0; j < 3; ++j) {
+      s.p[j * N + g] = p[j];
+      s.l[j * N + g] = l[j];
+    }
+    acc[0] = psq / m;
+    acc[1] = lsq;
+  }
+  wave_store_sums(task_out + (size_t)tid * 16, acc, 8, lane);
+}
+
+// translate + rotate (5-factor Rx Ry Rz Ry Rx splitting) + dipole update
+__global__ void k_moveX(State s) {
+  int g = s.g_begin + blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= s.g_end) return;
+  const RepDev &Rp = s.reps[s.rep_of[g]];
+  const int N = s.Ntot;
+  const double dt = Rp.dt, dt2 = dt / 2;
+  double m = s.mass[g];
+  for (int j = 0; j < 3; ++j) s.x[j * N + g] += dt * s.p[j * N + g] / m;
+  double O[9], Q[9], R[9], l0[3], l1[3];
+  double I[3] = {s.inert[g], s.inert[N + g], s.inert[2 * N + g]};
+  for (int k = 0; k < 9; ++k) O[k] = s.ori[k * N + g];
+  for (int k = 0; k < 3; ++k) l0[k] = s.l[k * N + g];
+  mv3(O, l0, l1);
+  double ang, ca, sa;
+#define RX(c, s_) { R[0]=1.; R[1]=0.; R[2]=0.; R[3]=0.; R[4]=c; R[5]=-s_; R[6]=0.; R[7]=s_; R[8]=c; }
+#define RY(c, s_) { R[0]=c; R[1]=0.; R[2]=s_; R[3]=0.; R[4]=1.; R[5]=0.; R[6]=-s_; R[7]=0.; R[8]=c; }
+#define RZ(c, s_) { R[0]=c; R[1]=-s_; R[2]=0.; R[3]=s_; R[4]=c; R[5]=0.; R[6]=0.; R[7]=0.; R[8]=1.; }
+  ang = -l1[0] * dt2 / I[0]; ca = cos(ang); sa = sin(ang); RX(ca, sa);
+  mv3(R, l1, l0); mm3(R, O, Q);
+  ang = -l0[1] * dt2 / I[1]; ca = cos(ang); sa = sin(ang); RY(ca, sa);
+  mv3(R, l0, l1); mm3(R, Q, O);
+  ang = -l1[2] * dt / I[2]; ca = cos(ang); sa = sin(ang); RZ(ca, sa);
+  mv3(R, l1, l0); mm3(R, O, Q);
+  ang = -l0[1] * dt2 / I[1]; ca = cos(ang); sa = sin(ang); RY(ca, sa);
+  mv3(R, l0, l1); mm3(R, Q, O);
+  ang = -l1[0] * dt2 / I[0]; ca = cos(ang); sa = sin(ang); RX(ca, sa);
+  mv3(R, l1, l0); mm3(R, O, Q);
+#undef RX
+#undef RY
+#undef RZ
+  for (int k = 0; k < 9; ++k) s.ori[k * N + g] = Q[k];
+}
+
+__global__ void k_renorm(State s) {
+  int g = s.g_begin + blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= s.g_end) return;
+  const int N = s.Ntot;
+  double o[9];
+  for (int k = 0; k < 9; ++k) o[k] = s.ori[k * N + g];
+  double norm = sqrt(o[6] * o[6] + o[7] * o[7] + o[8] * o[8]);
+  o[6] /= norm; o[7] /= norm; o[8] /= norm;
+  double Up[3] = {o[6] + o[3], o[7] + o[4], o[8] + o[5]};
+  double d = Up[0] * o[6] + Up[1] * o[7] + Up[2] * o[8];
+  o[3] = Up[0] - d * o[6];
+  o[4] = Up[1] - d * o[7];
+  o[5] = Up[2] - d * o[8];
+  norm = sqrt(o[3] * o[3] + o[4] * o[4] + o[5] * o[5]);
+  o[3] /= norm; o[4] /= norm; o[5] /= norm;
+  o[0] = o[4] * o[8] - o[7] * o[5];
+  o[1] = o[5] * o[6] - o[8] * o[3];
+  o[2] = o[3] * o[7] - o[6] * o[4];
+  for (int k = 0; k < 9; ++k) s.ori[k * N + g] = o[k];
+}
+
+// moveTher for every replica (one thread each).  gs[rep*4 + 2*i + 0/1] hold
+// the Box-Muller pair drawn on the host from thermostat i's MT19937 for this
+// step (mode 0 uses the first, mode 2 the stored second).
+__global__ void k_thermo(State s, Thermo *__restrict__ th, const double *__restrict__ Kin, int mode,
+                         const double *__restrict__ gs) {
+  int rp = blockIdx.x * blockDim.x + threadIdx.x;
+  if (rp >= s.nrep) return;
+  const RepDev &R = s.reps[rp];
+  Thermo T = th[rp];
+  for (int i = 0; i < 2; ++i) {
+    double Q = R.Q, g = R.gamma, ex1;
+    if (mode == 0) {
+      double r0 = gs[rp * 4 + 2 * i], r1 = gs[rp * 4 + 2 * i + 1];
+      T.strn[i] = r1;
+      ex1 = exp(-g * R.dt / 2);
+      T.s[i] = ex1 * T.s[i] + sqrt(2. * R.temp * g * R.dt / Q) * r0;
+    } else if (mode == 1) {
+      double K = Kin[rp * 2 + i];
+      T.s[i] = T.s[i] + R.dt * (2. * K - 3. * (double)R.N * R.temp) / Q;
+    } else {
+      ex1 = exp(-g * R.dt / 2);
+      T.s[i] = ex1 * T.s[i] + sqrt(2. * R.temp * g * R.dt / Q) * T.strn[i];
+    }
+  }
+  th[rp] = T;
+}
+
+// kinetic energies K[2] = sums / 2 per replica from the reduced moveP sums
+__global__ void k_kin_finish(State s, const double *__restrict__ red, double *__restrict__ Kin) {
+  int rp = blockIdx.x * blockDim.x + threadIdx.x;
+  if (rp >= s.nrep) return;
+  Kin[rp * 2 + 0] = red[rp * 8 + 0] / 2;
+  Kin[rp * 2 + 1] = red[rp * 8 + 1] / 2;
+}
+
+// observables of print_out (v25/main.cpp:626-647, v25/functions.cpp:2843-2850)
+__global__ void k_observables(State s, const RepEnergy *__restrict__ E, const double *__restrict__ Kin,
+                              const double *__restrict__ mom, double *__restrict__ obs) {
+  int rp = blockIdx.x * blockDim.x + threadIdx.x;
+  if (rp >= s.nrep) return;
+  const RepDev &R = s.reps[rp];
+  const RepEnergy e = E[rp];
+  double Np = (double)R.N;
+  double WdipR = 3 * e.UR, WdipK = 3 * e.UK;
+  double K = (Kin[rp * 2] + Kin[rp * 2 + 1]) / Np;
+  double Uss = e.Usc / Np, UdipR = e.UR / Np, UdipK = e.UK / Np, Uext = e.Uext / Np;
+  double U = R.Uself + Uss + UdipR + UdipK + Uext;
+  double Pss = e.Wsc / (3 * R.V), PdipR = WdipR / (3 * R.V), PdipK = WdipK / (3 * R.V);
+  double P = K / 3 * R.dens + Pss + PdipR + PdipK;
+  double *o = obs + (size_t)rp * 12;
+  o[0] = K; o[1] = U; o[2] = P; o[3] = Uss; o[4] = Pss; o[5] = UdipR; o[6] = PdipR;
+  o[7] = UdipK; o[8] = PdipK;
+  const double *m = mom + rp * 8;
+  o[9] = m[0] * m[0] + m[1] * m[1] + m[2] * m[2];
+  o[10] = m[3] * m[3] + m[4] * m[4] + m[5] * m[5];
+}
+
+// per-task sums of p and l (total momenta)
+__global__ void __launch_bounds__(256) k_momenta(State s, double *__restrict__ task_out) {
+  int tid = s.task_begin + wave_task_id();
+  int lane = threadIdx.x & 63;
+  if (wave_task_id() >= s.ntask) return;
+  Task tk = s.tasks[tid];
+  double v[6] = {0, 0, 0, 0, 0, 0};
+  if (lane < tk.n) {
+    int g = tk.g0 + lane;
+    for (int a = 0; a < 3; ++a) {
+      v[a] = s.p[a * s.Ntot + g];
+      v[3 + a] = s.l[a * s.Ntot + g];
+    }
+  }
+  wave_store_sums(task_out + (size_t)tid * 16, v, 6, lane);
+}
+
+__global__ void k_zero_FT(State s) {
+  int g = s.g_begin + blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= s.g_end) return;
+  for (int a = 0; a < 3; ++a) {
+    s.F[a * s.Ntot + g] = 0.0;
+    s.T[a * s.Ntot + g] = 0.0;
+  }
+}
+
+// fill kernel (graph-safe replacement for memset nodes)
+__global__ void k_fill_int(int *__restrict__ p, int n, int v) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// --------------------------------------------------------------------------
+// host launch wrappers
+// --------------------------------------------------------------------------
+static inline int nblk(int n, int b) { return (n + b - 1) / b; }
+
+void launch_dipoles(const State &s, hipStream_t st) {
+  hipLaunchKernelGGL(k_dipoles, dim3(nblk(s.g_end - s.g_begin, 256)), dim3(256), 0, st, s);
+}
+void launch_cells(const State &s, const int *rebuild, int *flags, hipStream_t st) {
+  hipLaunchKernelGGL(k_fill_int, dim3(nblk(s.ncells, 256)), dim3(256), 0, st, s.cell_cnt, s.ncells, 0);
+  hipLaunchKernelGGL(k_fill_int, dim3(nblk(s.ncells, 256)), dim3(256), 0, st, s.cell_fill, s.ncells, 0);
+  hipLaunchKernelGGL(k_cells_assign, dim3(nblk(s.g_end - s.g_begin, 256)), dim3(256), 0, st, s, rebuild);
+  hipLaunchKernelGGL(k_scan_seg, dim3(s.nrep), dim3(1024), 0, st, s.cell_cnt, s.cell_start, s.reps, 0, rebuild,
+                     flags);
+  hipLaunchKernelGGL(k_cells_fill, dim3(nblk(s.g_end - s.g_begin, 256)), dim3(256), 0, st, s, rebuild);
+  hipLaunchKernelGGL(k_cells_sort, dim3(nblk(s.ncells, 256)), dim3(256), 0, st, s, s.cell_rep, rebuild);
+}
+void launch_lists_count(const State &s, const int *rebuild, int *flags, hipStream_t st) {
+  hipLaunchKernelGGL(k_build_lists<0>, dim3(nblk(s.g_end - s.g_begin, 128)), dim3(128), 0, st, s, rebuild);
+  hipLaunchKernelGGL(k_scan_seg, dim3(s.nrep), dim3(1024), 0, st, s.nl_cnt, s.nl_start, s.reps, 1, rebuild, flags);
+  if (s.dl_cnt)
+    hipLaunchKernelGGL(k_scan_seg, dim3(s.nrep), dim3(1024), 0, st, s.dl_cnt, s.dl_start, s.reps, 2, rebuild,
+                       flags);
+}
+void launch_lists_fill(const State &s, const int *rebuild, hipStream_t st) {
+  hipLaunchKernelGGL(k_build_lists<1>, dim3(nblk(s.g_end - s.g_begin, 128)), dim3(128), 0, st, s, rebuild);
+  hipLaunchKernelGGL(k_savec, dim3(nblk(s.g_end - s.g_begin, 256)), dim3(256), 0, st, s, rebuild);
+}
+void launch_check(const State &s, double *task_out, int *rebuild, double *obs_rebuild, int obs_stride,
+                  hipStream_t st) {
+  hipLaunchKernelGGL(k_check_list, dim3(nblk(s.ntask, 4)), dim3(256), 0, st, s, task_out);
+  hipLaunchKernelGGL(k_check_decide, dim3(nblk(s.rep_end - s.rep_begin, 64)), dim3(64), 0, st, s, task_out, rebuild,
+                     obs_rebuild, obs_stride);
+}
+void launch_force_sc(const State &s, bool strict, double *task_out, int *flags, hipStream_t st) {
+  if (strict)
+    hipLaunchKernelGGL(k_force_sc<true>, dim3(nblk(s.ntask, 4)), dim3(256), 0, st, s, task_out, flags);
+  else
+    hipLaunchKernelGGL(k_force_sc<false>, dim3(nblk(s.ntask, 4)), dim3(256), 0, st, s, task_out, flags);
+}
+void launch_dipR(const State &s, bool mixed, bool all_pairs, const void *tab, double *task_out, hipStream_t st) {
+  if (mixed) {
+    if (all_pairs)
+      hipLaunchKernelGGL(k_dipR_all<float>, dim3(nblk(s.ntask, 4)), dim3(256), 0, st, s, (const float *)tab, task_out);
+    else
+      hipLaunchKernelGGL(k_dipR_list<float>, dim3(nblk(s.ntask, 4)), dim3(256), 0, st, s, (const float *)tab, task_out);
+  } else {
+    if (all_pairs)
+      hipLaunchKernelGGL(k_dipR_all<double>, dim3(nblk(s.ntask, 4)), dim3(256), 0, st, s, (const double *)tab, task_out);
+    else
+      hipLaunchKernelGGL(k_dipR_list<double>, dim3(nblk(s.ntask, 4)), dim3(256), 0, st, s, (const double *)tab, task_out);
+  }
+}
+void launch_ext_field(const State &s, double *task_out, hipStream_t st) {
+  hipLaunchKernelGGL(k_ext_field, dim3(nblk(s.ntask, 4)), dim3(256), 0, st, s, task_out);
+}
+void launch_self_ener(const State &s, double *task_out, hipStream_t st) {
+  hipLaunchKernelGGL(k_self_ener, dim3(nblk(s.ntask, 4)), dim3(256), 0, st, s, task_out);
+}
+void launch_reduce(const State &s, const double *in, int istride, int nv, int which, double *out, int ostride,
+                   int ooff, hipStream_t st) {
+  hipLaunchKernelGGL(k_reduce_items, dim3(s.rep_end - s.rep_begin), dim3(256), 0, st, s.reps, s.rep_begin, in,
+                     istride, nv, which, out, ostride, ooff);
+}
+
+template <typename R>
+static void recip_tables_t(const State &s, void *ph, void *amp, double *task_out, hipStream_t st) {
+  hipLaunchKernelGGL(k_recip_tables<R>, dim3(nblk(s.ntask, 4)), dim3(256), 0, st, s, (Cpx<R> *)ph, (R *)amp,
+                     task_out);
+}
+void launch_recip_tables(const State &s, bool mixed, void *ph, void *amp, double *task_out, hipStream_t st) {
+  if (mixed) recip_tables_t<float>(s, ph, amp, task_out, st);
+  else recip_tables_t<double>(s, ph, amp, task_out, st);
+}
+
+template <typename R, int KZ>
+static void recip_struct_kz(const State &s, int rt0, int nrt, int maxcol, const void *ph, const void *amp,
+                            void *part, hipStream_t st) {
+  hipLaunchKernelGGL((k_recip_struct<R, KZ>), dim3(nrt, nblk(maxcol, 256)), dim3(256), 0, st, s, rt0,
+                     (const Cpx<R> *)ph, (const R *)amp, (R *)part);
+}
+template <typename R, int KZ>
+static void recip_force_kz(const State &s, const void *ph, const void *amp, const void *A, hipStream_t st) {
+  hipLaunchKernelGGL((k_recip_force<R, KZ>), dim3(nblk(s.ntask, 4)), dim3(256), 0, st, s, (const Cpx<R> *)ph,
+                     (const R *)amp, (const Cpx<R> *)A);
+}
+
+#define MSC_KZ_CASES(X) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16)
+
+int launch_recip_struct(const State &s, bool mixed, int KZ, int rt0, int nrt, int maxcol, const void *ph,
+                        const void *amp, void *part, hipStream_t st) {
+  switch (KZ) {
+#define CASE(k)                                                                         \
+  case k:                                                                               \
+    if (mixed) recip_struct_kz<float, k>(s, rt0, nrt, maxcol, ph, amp, part, st);        \
+    else recip_struct_kz<double, k>(s, rt0, nrt, maxcol, ph, amp, part, st);             \
+    return 0;
+    MSC_KZ_CASES(CASE)
+#undef CASE
+    default:
+      return -1;
+  }
+}
+int launch_recip_force(const State &s, bool mixed, int KZ, const void *ph, const void *amp, const void *A,
+                       hipStream_t st) {
+  switch (KZ) {
+#define CASE(k)                                                        \
+  case k:                                                              \
+    if (mixed) recip_force_kz<float, k>(s, ph, amp, A, st);            \
+    else recip_force_kz<double, k>(s, ph, amp, A, st);                 \
+    return 0;
+    MSC_KZ_CASES(CASE)
+#undef CASE
+    default:
+      return -1;
+  }
+}
+void launch_recip_reduce(const State &s, bool mixed, const void *part, const double *M2, void *A, double *slot_out,
+                         int maxslot, hipStream_t st) {
+  if (mixed)
+    hipLaunchKernelGGL(k_recip_reduce<float>, dim3(nblk(maxslot, 256), s.rep_end - s.rep_begin), dim3(256), 0, st, s,
+                       (const float *)part, M2, (Cpx<float> *)A, slot_out);
+  else
+    hipLaunchKernelGGL(k_recip_reduce<double>, dim3(nblk(maxslot, 256), s.rep_end - s.rep_begin), dim3(256), 0, st, s,
+                       (const double *)part, M2, (Cpx<double> *)A, slot_out);
+}
+
+void launch_moveP(const State &s, bool last, const Thermo *th, double *task_out, hipStream_t st) {
+  if (last) hipLaunchKernelGGL(k_moveP<true>, dim3(nblk(s.ntask, 4)), dim3(256), 0, st, s, th, task_out);
+  else hipLaunchKernelGGL(k_moveP<false>, dim3(nblk(s.ntask, 4)), dim3(256), 0, st, s, th, task_out);
+}
+void launch_moveX(const State &s, hipStream_t st) {
+  hipLaunchKernelGGL(k_moveX, dim3(nblk(s.g_end - s.g_begin, 256)), dim3(256), 0, st, s);
+}
+void launch_renorm(const State &s, hipStream_t st) {
+  hipLaunchKernelGGL(k_renorm, dim3(nblk(s.g_end - s.g_begin, 256)), dim3(256), 0, st, s);
+}
+void launch_thermo(const State &s, Thermo *th, const double *Kin, int mode, const double *gs, hipStream_t st) {
+  hipLaunchKernelGGL(k_thermo, dim3(nblk(s.nrep, 64)), dim3(64), 0, st, s, th, Kin, mode, gs);
+}
+void launch_kin_finish(const State &s, const double *red, double *Kin, hipStream_t st) {
+  hipLaunchKernelGGL(k_kin_finish, dim3(nblk(s.nrep, 64)), dim3(64), 0, st, s, red, Kin);
+}
+void launch_observables(const State &s, const RepEnergy *E, const double *Kin, const double *mom, double *obs,
+                        hipStream_t st) {
+  hipLaunchKernelGGL(k_observables, dim3(nblk(s.nrep, 64)), dim3(64), 0, st, s, E, Kin, mom, obs);
+}
+void launch_momenta(const State &s, double *task_out, hipStream_t st) {
+  hipLaunchKernelGGL(k_momenta, dim3(nblk(s.ntask, 4)), dim3(256), 0, st, s, task_out);
+}
+void launch_zero_FT(const State &s, hipStream_t st) {
+  hipLaunchKernelGGL(k_zero_FT, dim3(nblk(s.g_end - s.g_begin, 256)), dim3(256), 0, st, s);
+}
+void launch_fill_int(int *p, int n, int v, hipStream_t st) {
+  hipLaunchKernelGGL(k_fill_int, dim3(nblk(n, 256)), dim3(256), 0, st, p, n, v);
+}
+
+}  // namespace msc

@@ -1,0 +1,69 @@
+// Device state handed to every kernel, and the host-side launch wrappers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "msc_common.hpp"
+
+namespace msc {
+
+struct State {
+  int Ntot, nd, nrep;
+  int ntask, nrtask, ncells;  // ntask: tasks in the selected range
+  int task_begin;            // first selected wave task
+  int g_begin, g_end;        // particle range of per-particle kernels
+  int rep_begin, rep_end;    // replica range of per-replica reductions
+  double lever_max;          // largest |dpc| of the context
+  const RepDev *reps;        // [nrep]
+  const Task *tasks;         // [ntask] wave tasks, grouped by replica
+  const Task *rtasks;        // [nrtask] reciprocal particle chunks, grouped by replica
+  const int *rep_of;         // [Ntot]
+  // particle state (component-major)
+  double *x, *p, *l, *ori, *F, *T;   // 3N, 3N, 3N, 9N, 3N, 3N
+  const double *len, *dia, *mass, *inert;  // N, N, N, 3N
+  const double *dpc, *doc, *dmu;     // nd N, 3 nd N, nd N
+  double *dpos, *dori;               // 3 nd N lab dipole positions / orientations
+  double *snap;                      // 3N neighbour-list reference positions
+  // cell grid
+  int *cell_of, *cell_cnt, *cell_start, *cell_fill, *cell_list;
+  const int *cell_rep;       // [ncells] owning replica
+  // neighbour lists (full rows; per-replica regions)
+  int *nl_cnt, *nl_start, *nl_idx;
+  int *dl_cnt, *dl_start, *dl_idx;
+  // reciprocal weights per slot (0 outside the half space)
+  const double *kval;
+};
+
+void launch_dipoles(const State &s, hipStream_t st);
+void launch_cells(const State &s, const int *rebuild, int *flags, hipStream_t st);
+void launch_lists_count(const State &s, const int *rebuild, int *flags, hipStream_t st);
+void launch_lists_fill(const State &s, const int *rebuild, hipStream_t st);
+void launch_check(const State &s, double *task_out, int *rebuild, double *obs_rebuild, int obs_stride,
+                  hipStream_t st);
+void launch_force_sc(const State &s, bool strict, double *task_out, int *flags, hipStream_t st);
+void launch_dipR(const State &s, bool mixed, bool all_pairs, const void *tab, double *task_out, hipStream_t st);
+void launch_ext_field(const State &s, double *task_out, hipStream_t st);
+void launch_self_ener(const State &s, double *task_out, hipStream_t st);
+void launch_reduce(const State &s, const double *in, int istride, int nv, int which, double *out, int ostride,
+                   int ooff, hipStream_t st);
+void launch_recip_tables(const State &s, bool mixed, void *ph, void *amp, double *task_out, hipStream_t st);
+int launch_recip_struct(const State &s, bool mixed, int KZ, int rt0, int nrt, int maxcol, const void *ph,
+                        const void *amp, void *part, hipStream_t st);
+void launch_recip_reduce(const State &s, bool mixed, const void *part, const double *M2, void *A, double *slot_out,
+                         int maxslot, hipStream_t st);
+int launch_recip_force(const State &s, bool mixed, int KZ, const void *ph, const void *amp, const void *A,
+                       hipStream_t st);
+void launch_moveP(const State &s, bool last, const Thermo *th, double *task_out, hipStream_t st);
+void launch_moveX(const State &s, hipStream_t st);
+void launch_renorm(const State &s, hipStream_t st);
+void launch_thermo(const State &s, Thermo *th, const double *Kin, int mode, const double *gs, hipStream_t st);
+void launch_kin_finish(const State &s, const double *red, double *Kin, hipStream_t st);
+void launch_observables(const State &s, const RepEnergy *E, const double *Kin, const double *mom, double *obs,
+                        hipStream_t st);
+void launch_momenta(const State &s, double *task_out, hipStream_t st);
+void launch_zero_FT(const State &s, hipStream_t st);
+void launch_fill_int(int *p, int n, int v, hipStream_t st);
+
+constexpr int kChunk = 64;  // particles per reciprocal structure-factor chunk
+
+}  // namespace msc
