@@ -1,0 +1,279 @@
+#!/usr/bin/env python3
+"""Throughput of the mdSpherocyl per-step force path on one MI355X.
+
+Workload (value): N=32768 spherocylinders, lengths Gaussian(3, 0.3)
+truncated at +-3 sd, packing fraction 0.20, mu^2=4, Ewald alpha=3/3.5 with
+real-space cut-off r_c=3.5 and k_max=10 (reference K input 11), T=1 Nose-Hoover
+thermostat, dt=0.002, mixed precision.  One step is the whole device step of
+v25/main.cpp:559-622 (thermostat, kicks, drift, forces, observables, list
+check and rebuild when due).
+
+Beside it, on the same initial state: the plain FP64 GPU path (steps/s and the
+mixed/FP64 ratio), the force/torque/energy error of both against the FP64
+oracle, the batched workload (64 replicas, N_i in [512, 8192], packing
+fraction in [0.05, 0.30], mu^2 in [1, 6], k_max=7, mixed) as replica-steps/s
+and particle-steps/s, and the oracle timed on the host (cpu_baseline).
+
+Prints one JSON line.  Inputs are resident on the device before the timed
+region starts; the timed region is msc_run over K steps, which returns after
+its stream has drained.
+"""
+import argparse
+import ctypes
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+# Algorithmic flops per unit, frozen from the operation count of the force
+# formulas (never tuned against measurements):
+#   reciprocal structure factor: per (particle, +-k pair) one complex product
+#     forming e^{-ik.r} from the separable x/y/z factors (6) and three
+#     complex-by-real accumulations of mu u_a e^{-ik.r} (12)          -> 18
+#   reciprocal back-projection: per (particle, +-k pair) the phase product
+#     (6), its product with the folded amplitude A(k) (6), mu u.k (2 with the
+#     x/y part hoisted per column), force and field accumulations (7)  -> 21
+#   WCA pair (d < 2^{1/6} sigma_ij): segment distance and force/torque    120
+#   dipolar real-space pair within r_c: table lerp, force, torques        70
+FLOPS = {"recip_struct": 18, "recip_force": 21, "wca": 120, "real_pairs": 70}
+PEAK_TFLOPS = {"fp32": 157.3, "fp64": 78.6}   # MI355X vector peaks
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def n_pm_k(kmax):
+    return ((2 * kmax + 1) ** 3 - 1) // 2
+
+
+def force_split(sysm, rep, sr_cut):
+    """Per-routine forces and energies at the current state (lists from
+    init_run), as the reference's step calls them."""
+    sysm.zero_forces(rep)
+    Usc, _, _ = sysm.force_sc(rep, sr_cut)
+    UR, _ = sysm.force_dipR(rep)
+    UK, _ = sysm.force_dipK(rep)
+    F, T = sysm.get_forces(rep)
+    return F, T, Usc + UR + UK
+
+
+def rel_l2(a, b):
+    d = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / d) if d > 0 else float(np.linalg.norm(a - b))
+
+
+def timed_run(sysm, steps, warmup, timing):
+    sysm.run(warmup, observe=False)
+    sysm.reset_timing()
+    sysm.set_timing(timing)
+    t0 = time.perf_counter()
+    obs = sysm.run(steps, step0=warmup)
+    dt = time.perf_counter() - t0
+    sysm.set_timing(False)
+    return dt, obs
+
+
+def kernel_table(sysm, names):
+    out = {}
+    for n in names:
+        ms, cnt = sysm.kernel_time(n)
+        if cnt:
+            out[n] = (ms, cnt)
+    return out
+
+
+def cpu_baseline(desc, st, off, nbr, budget_s):
+    """The oracle (serial C, FP64) on one force evaluation of the same
+    workload: WCA over the same neighbour list, real space over all pairs
+    within r_c, reciprocal sum over the same k-vectors."""
+    from oracle import oracle as O
+    L = O.lib()
+    N = desc.N
+    S = O.OSys(N, 1, L=tuple(desc.L))
+    S.r[:] = st["r"]
+    S.ori[:] = st["ori"]
+    S.len[:] = st["length"]
+    S.dia[:] = st["dia"]
+    S.mass[:] = st["mass"]
+    S.dpc[:] = st["dpc"]
+    S.doc[:] = st["doc"]
+    S.dmu[:] = st["dmu"]
+    S.set_inertia()
+    S.set_dip()
+    c = S.c()
+    U, W, PT = ctypes.c_double(), ctypes.c_double(), np.zeros(9)
+    tab = np.zeros(32768 * 3)
+    L.o_tabul(O.dptr(tab), desc.alpha)
+    kv = np.zeros((desc.K + 1) ** 3)
+    Lb = np.array(desc.L[:3], dtype=np.float64)
+    L.o_wave_v(O.dptr(kv), O.dptr(Lb), desc.alpha, desc.K)
+    S.F[:] = 0
+    S.T[:] = 0
+    t0 = time.perf_counter()
+    rc = L.o_force_sc(c, desc.sr_cut, O.iptr(off), O.iptr(nbr), ctypes.byref(U), ctypes.byref(W), O.dptr(PT))
+    Usc = U.value
+    L.o_force_dipR(c, O.dptr(tab), desc.rc, ctypes.byref(U), O.dptr(PT))
+    UR = U.value
+    L.o_force_dipK(c, desc.K, desc.alpha, O.dptr(kv), ctypes.byref(U), O.dptr(PT))
+    UK = U.value
+    t = time.perf_counter() - t0
+    if rc != 0:
+        raise RuntimeError("oracle force_sc reported an overlap on the initial state")
+    return t, S.F.copy(), S.T.copy(), Usc + UR + UK
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--fp64-steps", type=int, default=0, help="timed FP64 steps (default: steps / 2)")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=30.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (replicas would shard across GPUs, "
+              "a single system by spatial domains plus one S(k) all-reduce); run with --gpus 1",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import mdspherocyl_amd as m
+    from mdspherocyl_amd import configs
+
+    m.load()
+    t0 = time.perf_counter()
+    desc, st = configs.build("c3")[0]
+    log("bench: workload N=%d built in %.1f s (box %.3f, alpha %.4f, r_c %.2f, K %d)"
+        % (desc.N, time.perf_counter() - t0, desc.L[0], desc.alpha, desc.rc, desc.K))
+    kmax = desc.K - 1
+    nk = n_pm_k(kmax)
+
+    # --- mixed context (the headline) --------------------------------------
+    mix = m.MdSystem([desc], precision="mixed")
+    configs.load_into(mix, [(desc, st)])
+    mix.init_run()
+    Fm, Tm, Um = force_split(mix, 0, desc.sr_cut)
+
+    # --- FP64 context on the same state --------------------------------------
+    f64 = m.MdSystem([desc], precision="fp64")
+    configs.load_into(f64, [(desc, st)])
+    f64.init_run()
+    Fd, Td, Ud = force_split(f64, 0, desc.sr_cut)
+    off, nbr = f64.nblist(0)
+    counts = f64.work_counts(0)
+    err_mix_vs_fp64 = {"F": rel_l2(Fm, Fd), "T": rel_l2(Tm, Td), "U": abs(Um - Ud) / abs(Ud)}
+    log("bench: mixed vs FP64 GPU at the initial state: rel L2 F %.2e T %.2e, U %.2e"
+        % (err_mix_vs_fp64["F"], err_mix_vs_fp64["T"], err_mix_vs_fp64["U"]))
+
+    names = ["recip_struct", "recip_force", "real_pairs", "wca", "nlist", "integrate"]
+    log("bench: mixed, %d warmup + %d timed steps" % (args.warmup, args.steps))
+    t_mix, obs = timed_run(mix, args.steps, args.warmup, True)
+    ktab = kernel_table(mix, names)
+    counts_mix = mix.work_counts(0)
+    rebuilds = int(np.sum(obs[:, 0, m.OBS_NAMES.index("rebuild")]))
+    sps_mix = args.steps / t_mix
+    log("bench: mixed %.2f steps/s (%.3f ms/step, %d list rebuilds)" % (sps_mix, 1e3 * t_mix / args.steps, rebuilds))
+    for n, (ms, cnt) in sorted(ktab.items(), key=lambda kv: -kv[1][0]):
+        log("  %-13s %8.3f ms/launch  (%d launches, %.1f%% of the timed region)"
+            % (n, ms / cnt, cnt, 100 * ms / (1e3 * t_mix)))
+
+    k64 = args.fp64_steps or max(args.steps // 2, 10)
+    log("bench: FP64, %d warmup + %d timed steps" % (min(args.warmup, 10), k64))
+    t_d, _ = timed_run(f64, k64, min(args.warmup, 10), False)
+    sps_d = k64 / t_d
+    log("bench: FP64 %.2f steps/s; mixed/FP64 = %.2f" % (sps_d, sps_mix / sps_d))
+    mix.close()
+    f64.close()
+
+    # --- dominant kernel roofline -----------------------------------------------
+    units = {"recip_struct": desc.N * nk, "recip_force": desc.N * nk, "wca": counts_mix[0],
+             "real_pairs": counts_mix[1]}
+    dom = max((n for n in ktab if n in FLOPS), key=lambda n: ktab[n][0])
+    ms, cnt = ktab[dom]
+    flop = FLOPS[dom] * units[dom]
+    achieved = flop / (ms / cnt * 1e-3) / 1e12 if ms > 0 else 0.0
+    # the kernel runs on the vector ALUs; MI355X's published FP32 figure is the
+    # same 157.3 TFLOP/s for vector and matrix issue
+    roofline = {"kernel": dom, "bound": "mfma", "achieved": round(achieved, 3),
+                "peak": PEAK_TFLOPS["fp32"], "unit": "TFLOP/s", "frac": round(achieved / PEAK_TFLOPS["fp32"], 4),
+                "traffic": None, "flop_per_launch": flop, "ms_per_launch": round(ms / cnt, 4)}
+
+    # --- batched replicas ---------------------------------------------------------
+    batched = None
+    if not args.no_batched:
+        t0 = time.perf_counter()
+        reps = configs.build("c4")
+        ntot = sum(d.N for d, _ in reps)
+        log("bench: batched workload, %d replicas, sum N = %d, built in %.1f s"
+            % (len(reps), ntot, time.perf_counter() - t0))
+        bat = m.MdSystem([d for d, _ in reps], precision="mixed")
+        configs.load_into(bat, reps)
+        bat.init_run()
+        kb = max(args.steps // 2, 10)
+        t_b, _ = timed_run(bat, kb, min(args.warmup, 10), False)
+        bat.close()
+        sps_b = kb / t_b
+        pps_b = sps_b * ntot
+        pps_single = sps_mix * desc.N
+        batched = {"replicas": len(reps), "sum_N": ntot, "steps": kb, "steps_per_s": round(sps_b, 3),
+                   "replica_steps_per_s": round(sps_b * len(reps), 2), "particle_steps_per_s": round(pps_b, 1),
+                   "vs_single_particle_steps": round(pps_b / pps_single, 3)}
+        log("bench: batched %.2f steps/s = %.1f replica-steps/s; particle-steps/s %.3e = %.2f x single system"
+            % (sps_b, sps_b * len(reps), pps_b, pps_b / pps_single))
+
+    # --- oracle on the host ----------------------------------------------------------
+    cpu = None
+    err_oracle = None
+    if not args.no_cpu:
+        log("bench: oracle force evaluation on the host (serial FP64)")
+        t_cpu, Fo, To, Uo = cpu_baseline(desc, st, off, nbr, args.cpu_budget)
+        cpu = {"value": round(1.0 / t_cpu, 5), "unit": "steps/s", "cores": 1, "kind": "port",
+               "sample": "one force evaluation (WCA on the neighbour list, real space within r_c=3.5, reciprocal "
+                         "k_max=10) of the N=32768 workload, serial C oracle, %.1f s; the integrator "
+                         "(about 1%% of a reference step) is not timed" % t_cpu}
+        err_oracle = {"mixed": {"F": rel_l2(Fm, Fo), "T": rel_l2(Tm, To), "U": abs(Um - Uo) / abs(Uo)},
+                      "fp64": {"F": rel_l2(Fd, Fo), "T": rel_l2(Td, To), "U": abs(Ud - Uo) / abs(Uo)}}
+        log("bench: oracle %.1f s per force evaluation; error vs oracle: mixed F %.2e T %.2e U %.2e, "
+            "FP64 F %.2e T %.2e U %.2e" % (t_cpu, *err_oracle["mixed"].values(), *err_oracle["fp64"].values()))
+
+    line = {
+        "metric": "steps/s @N=32768 mixed vs FP64 GPU (force err≤1e-5); batched replica-steps/s",
+        "value": round(sps_mix, 3),
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "list_rebuilds": rebuilds,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * t_mix / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "N=32768 spherocylinders, L~N(3,0.3), phi=0.20, mu^2=4, alpha=3/3.5, r_c=3.5, "
+                               "k_max=10, mixed precision", "N": desc.N, "box": round(desc.L[0], 4),
+                   "k_pairs": nk, "wca_pairs": counts[0], "real_pairs": counts[1], "list_pairs": counts[3]},
+        "fp64_steps_per_s": round(sps_d, 3),
+        "mixed_over_fp64": round(sps_mix / sps_d, 3),
+        "err_mixed_vs_fp64": err_mix_vs_fp64,
+        "err_vs_oracle": err_oracle,
+        "batched": batched,
+        "kernels_ms_per_launch": {n: round(v[0] / v[1], 4) for n, v in ktab.items()},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

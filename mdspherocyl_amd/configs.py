@@ -6,7 +6,11 @@ diameter: c1 puts centres on a simple-cubic lattice with axes uniform on the
 sphere; c2-c5 use random sequential addition of uniform centres and axes
 (random axes on any lattice jam at phi >= 0.15 for these lengths: even rods
 aligned along the cube diagonal of a simple-cubic lattice at phi = 0.2 touch
-once L > 3.2).  Maxwell momenta at T, zero total momentum.  One axial point dipole
+once L > 3.2); replicas denser than phi = 0.22, where addition slows down
+near jamming, start as an orthorhombic lattice of rods along +-z with random
+polarity and a 0.02 rad random tilt (the reference's own ini_pos start is an
+aligned lattice too, v25/functions.cpp:491-584).  Maxwell momenta at T, zero
+total momentum.  One axial point dipole
 per particle at the centre.  Lengths are Gaussian, truncated at +-3 sd, and
 the box follows from the packing fraction phi = sum(v_i) / V with
 v = pi D^2 L / 4 + pi D^3 / 6.
@@ -205,6 +209,22 @@ def place_rsa(rng, N, lengths, box, dmin=1.0, batch=512, max_batches=200000):
     return r, u
 
 
+def place_aligned(rng, N, lengths, box, dmin=1.0, tilt=0.02):
+    """Orthorhombic lattice of rods along +-z (random polarity, small random
+    tilt): layers one longest rod apart, square in-plane grid."""
+    c = lengths.max() + dmin + 0.05
+    nz = max(1, int(box // c))
+    nxy = int(math.ceil(math.sqrt(N / nz)))
+    a = box / nxy
+    if a < dmin * 1.02 or box / nz < c:
+        raise RuntimeError("aligned lattice does not fit %d rods in a box of %.3f" % (N, box))
+    idx = np.arange(nxy * nxy * nz)[:N]
+    r = np.stack([(idx % nxy) * a, ((idx // nxy) % nxy) * a, (idx // (nxy * nxy)) * (box / nz)], axis=1)
+    r += np.array([0.5 * a, 0.5 * a, 0.5 * box / nz]) - 0.5 * box
+    u = fisher_axes(rng, N, 1.0 / (tilt * tilt))
+    return r, u
+
+
 def maxwell(rng, N, mass, inert, ori, temp):
     p = rng.normal(size=(N, 3)) * np.sqrt(mass * temp)[:, None]
     p -= p.mean(axis=0)
@@ -231,7 +251,12 @@ def make_replica(rng, N, lengths, phi, mu, K, alpha_fn, rc, temp=1.0, dt=0.002, 
     vol = np.pi * dia ** 2 * lengths / 4 + np.pi * dia ** 3 / 6
     box = (vol.sum() / phi) ** (1.0 / 3)
     lengths = lengths.copy()
-    r, u = (place_lattice if lattice else place_rsa)(rng, N, lengths, box)
+    if lattice:
+        r, u = place_lattice(rng, N, lengths, box)
+    elif phi > 0.22:
+        r, u = place_aligned(rng, N, lengths, box)
+    else:
+        r, u = place_rsa(rng, N, lengths, box)
     ori = frames(u)
     p, l = maxwell(rng, N, mass, inertia(lengths, dia, mass), ori, temp)
     dmax = float((lengths + dia).max())

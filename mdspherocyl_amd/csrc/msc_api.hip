@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -110,18 +111,61 @@ struct msc_ctx {
   int *d_flags = nullptr, *d_rebuild = nullptr, *d_one = nullptr;
   std::map<std::string, TimerSet> timers;
   std::vector<hipEvent_t> event_pool;
+  double *d_run_gs = nullptr, *d_run_obs = nullptr;
+  size_t run_gs_cap = 0, run_obs_cap = 0;
   hipGraphExec_t graph_exec = nullptr;
   int graph_renorm = -1;
 };
+
+static std::string g_err;  // failures that have no context to hold them
+
+// One stream per device, created on first use and kept until the process
+// ends: contexts on a device share it, so creating and destroying contexts
+// never creates or destroys streams.
+static hipError_t device_stream(int device, hipStream_t *out) {
+  static std::mutex mu;
+  static std::map<int, hipStream_t> streams;
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = streams.find(device);
+  if (it != streams.end()) {
+    *out = it->second;
+    return hipSuccess;
+  }
+  hipStream_t st = nullptr;
+  hipError_t e = hipStreamCreate(&st);
+  if (e == hipSuccess) streams[device] = st;
+  *out = st;
+  return e;
+}
 
 #define HIPC(expr)                                                                   \
   do {                                                                               \
     hipError_t e_ = (expr);                                                          \
     if (e_ != hipSuccess) {                                                          \
       ctx->err = std::string(#expr) + ": " + hipGetErrorString(e_);                  \
+      g_err = ctx->err;                                                              \
       return MSC_E_HIP;                                                              \
     }                                                                                \
   } while (0)
+
+// launch failures are reported by the next runtime query; every entry point
+// that launches kernels checks before it returns
+#define LAUNCHED(what)                                                               \
+  do {                                                                               \
+    hipError_t e_ = hipGetLastError();                                               \
+    if (e_ != hipSuccess) {                                                          \
+      ctx->err = std::string("kernel launch in ") + (what) + ": " + hipGetErrorString(e_); \
+      g_err = ctx->err;                                                              \
+      return MSC_E_HIP;                                                              \
+    }                                                                                \
+  } while (0)
+
+// device -> host copy ordered after the work queued on the context stream
+static int d2h(msc_ctx *ctx, void *dst, const void *src, size_t bytes) {
+  HIPC(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  HIPC(hipStreamSynchronize(ctx->stream));
+  return MSC_OK;
+}
 
 namespace {
 
@@ -150,9 +194,19 @@ struct Bracket {
   hipEvent_t a = nullptr, b = nullptr;
   Bracket(msc_ctx *c_, const char *n) : c(c_), name(n) {
     if (!c->timing) return;
-    hipEventCreate(&a);
-    hipEventCreate(&b);
+    a = take();
+    b = take();
     hipEventRecord(a, c->stream);
+  }
+  hipEvent_t take() {
+    hipEvent_t e = nullptr;
+    if (!c->event_pool.empty()) {
+      e = c->event_pool.back();
+      c->event_pool.pop_back();
+    } else {
+      hipEventCreate(&e);
+    }
+    return e;
   }
   ~Bracket() {
     if (!c->timing) return;
@@ -169,8 +223,8 @@ void settle_timers(msc_ctx *ctx) {
       hipEventElapsedTime(&ms, ev.first, ev.second);
       kv.second.ms += ms;
       kv.second.launches += 1;
-      hipEventDestroy(ev.first);
-      hipEventDestroy(ev.second);
+      ctx->event_pool.push_back(ev.first);
+      ctx->event_pool.push_back(ev.second);
     }
     kv.second.pending.clear();
   }
@@ -322,7 +376,10 @@ extern "C" {
 
 int msc_version(void) { return 1; }
 
-const char *msc_last_error(const msc_ctx *ctx) { return ctx ? ctx->err.c_str() : "no context"; }
+const char *msc_last_error(const msc_ctx *ctx) {
+  if (ctx) return ctx->err.c_str();
+  return g_err.empty() ? "no context" : g_err.c_str();
+}
 
 int msc_create(int device, int precision, int nrep, const msc_replica_desc *reps, msc_ctx **out) {
   if (!out || nrep <= 0 || !reps || (precision != MSC_FP64 && precision != MSC_MIXED)) return MSC_E_ARG;
@@ -343,9 +400,14 @@ int msc_create(int device, int precision, int nrep, const msc_replica_desc *reps
     }
   }
   ctx->nd = nd;
-  if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&ctx->stream) != hipSuccess) {
-    delete ctx;
-    return MSC_E_HIP;
+  {
+    hipError_t e1 = hipSetDevice(device);
+    hipError_t e2 = e1 == hipSuccess ? device_stream(device, &ctx->stream) : e1;
+    if (e2 != hipSuccess) {
+      g_err = std::string(e1 != hipSuccess ? "hipSetDevice: " : "hipStreamCreate: ") + hipGetErrorString(e2);
+      delete ctx;
+      return MSC_E_HIP;
+    }
   }
   // replica layout: particles, tasks, chunk tasks, k slots, tables, partials
   int n0 = 0, slot0 = 0;
@@ -442,7 +504,7 @@ int msc_create(int device, int precision, int nrep, const msc_replica_desc *reps
     for (int q = 0; q < 5 && rc == MSC_OK; ++q) rc = dalloc(ctx, raw[q], sz[q]);
   }
   if (rc != MSC_OK) {
-    std::string e = ctx->err;
+    g_err = "msc_create: " + ctx->err;
     msc_destroy(ctx);
     return MSC_E_HIP;
   }
@@ -477,7 +539,9 @@ int msc_create(int device, int precision, int nrep, const msc_replica_desc *reps
   }
   upload_reps(ctx);
   refresh_state_ptrs(ctx);
-  if (hipStreamSynchronize(ctx->stream) != hipSuccess) {
+  hipError_t es = hipStreamSynchronize(ctx->stream);
+  if (es != hipSuccess) {
+    g_err = std::string("msc_create: hipStreamSynchronize: ") + hipGetErrorString(es);
     msc_destroy(ctx);
     return MSC_E_HIP;
   }
@@ -488,6 +552,7 @@ int msc_create(int device, int precision, int nrep, const msc_replica_desc *reps
 void msc_destroy(msc_ctx *ctx) {
   if (!ctx) return;
   if (ctx->stream) hipStreamSynchronize(ctx->stream);
+  hipDeviceSynchronize();
   if (ctx->graph_exec) hipGraphExecDestroy(ctx->graph_exec);
   void *bufs[] = {ctx->d_reps, ctx->d_tasks, ctx->d_rtasks, ctx->d_rep_of, ctx->d_cell_rep, ctx->d_x, ctx->d_p,
                   ctx->d_l, ctx->d_ori, ctx->d_F, ctx->d_T, ctx->d_len, ctx->d_dia, ctx->d_mass, ctx->d_inert,
@@ -496,7 +561,7 @@ void msc_destroy(msc_ctx *ctx) {
                   ctx->d_nl_start, ctx->d_nl_idx, ctx->d_dl_cnt, ctx->d_dl_start, ctx->d_dl_idx, ctx->d_kval,
                   ctx->d_tab, ctx->d_ph, ctx->d_amp, ctx->d_part, ctx->d_A, ctx->d_slot_out, ctx->d_task_out,
                   ctx->d_red, ctx->d_M2, ctx->d_Kin, ctx->d_mom, ctx->d_gs, ctx->d_obs, ctx->d_counts, ctx->d_E,
-                  ctx->d_th, ctx->d_flags, ctx->d_rebuild, ctx->d_one};
+                  ctx->d_th, ctx->d_flags, ctx->d_rebuild, ctx->d_one, ctx->d_run_gs, ctx->d_run_obs};
   for (void *b : bufs)
     if (b) hipFree(b);
   for (auto &kv : ctx->timers)
@@ -504,7 +569,7 @@ void msc_destroy(msc_ctx *ctx) {
       hipEventDestroy(ev.first);
       hipEventDestroy(ev.second);
     }
-  if (ctx->stream) hipStreamDestroy(ctx->stream);
+  for (hipEvent_t e : ctx->event_pool) hipEventDestroy(e);
   delete ctx;
 }
 
@@ -520,7 +585,7 @@ int msc_set_state(msc_ctx *ctx, int rep, const double *r, const double *p, const
     buf.resize((size_t)n);
     for (int c = 0; c < ncomp; ++c) {
       for (int i = 0; i < n; ++i) buf[i] = src[(size_t)i * ncomp + c];
-      HIPC(hipMemcpy(dst + (size_t)c * N + R.n0, buf.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+      HIPC(hipMemcpyAsync(dst + (size_t)c * N + R.n0, buf.data(), sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream));
     }
     return MSC_OK;
   };
@@ -559,7 +624,7 @@ int msc_get_state(msc_ctx *ctx, int rep, double *r, double *p, double *l, double
   auto get = [&](double *dst, const double *src, int ncomp) -> int {
     if (!dst) return MSC_OK;
     for (int c = 0; c < ncomp; ++c) {
-      HIPC(hipMemcpy(buf.data(), src + (size_t)c * N + R.n0, sizeof(double) * n, hipMemcpyDeviceToHost));
+      if (int rc_ = d2h(ctx, buf.data(), src + (size_t)c * N + R.n0, sizeof(double) * n)) return rc_;
       for (int i = 0; i < n; ++i) dst[(size_t)i * ncomp + c] = buf[i];
     }
     return MSC_OK;
@@ -580,11 +645,11 @@ int msc_get_forces(msc_ctx *ctx, int rep, double *F, double *T) {
   HIPC(hipStreamSynchronize(ctx->stream));
   for (int c = 0; c < 3; ++c) {
     if (F) {
-      HIPC(hipMemcpy(buf.data(), ctx->d_F + (size_t)c * N + R.n0, sizeof(double) * n, hipMemcpyDeviceToHost));
+      if (int rc_ = d2h(ctx, buf.data(), ctx->d_F + (size_t)c * N + R.n0, sizeof(double) * n)) return rc_;
       for (int i = 0; i < n; ++i) F[3 * i + c] = buf[i];
     }
     if (T) {
-      HIPC(hipMemcpy(buf.data(), ctx->d_T + (size_t)c * N + R.n0, sizeof(double) * n, hipMemcpyDeviceToHost));
+      if (int rc_ = d2h(ctx, buf.data(), ctx->d_T + (size_t)c * N + R.n0, sizeof(double) * n)) return rc_;
       for (int i = 0; i < n; ++i) T[3 * i + c] = buf[i];
     }
   }
@@ -598,9 +663,9 @@ int msc_set_forces(msc_ctx *ctx, int rep, const double *F, const double *T) {
   std::vector<double> buf(n);
   for (int c = 0; c < 3; ++c) {
     for (int i = 0; i < n; ++i) buf[i] = F[3 * i + c];
-    HIPC(hipMemcpy(ctx->d_F + (size_t)c * N + R.n0, buf.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    HIPC(hipMemcpyAsync(ctx->d_F + (size_t)c * N + R.n0, buf.data(), sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream));
     for (int i = 0; i < n; ++i) buf[i] = T[3 * i + c];
-    HIPC(hipMemcpy(ctx->d_T + (size_t)c * N + R.n0, buf.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    HIPC(hipMemcpyAsync(ctx->d_T + (size_t)c * N + R.n0, buf.data(), sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream));
   }
   return MSC_OK;
 }
@@ -609,7 +674,7 @@ int msc_get_thermostat(msc_ctx *ctx, int rep, double s[2], double strn[2]) {
   if (!ctx || rep < 0 || rep >= ctx->nrep) return MSC_E_ARG;
   Thermo t;
   HIPC(hipStreamSynchronize(ctx->stream));
-  HIPC(hipMemcpy(&t, ctx->d_th + rep, sizeof t, hipMemcpyDeviceToHost));
+  if (int rc_ = d2h(ctx, &t, ctx->d_th + rep, sizeof t)) return rc_;
   if (s) { s[0] = t.s[0]; s[1] = t.s[1]; }
   if (strn) { strn[0] = t.strn[0]; strn[1] = t.strn[1]; }
   return MSC_OK;
@@ -619,7 +684,7 @@ int msc_set_thermostat(msc_ctx *ctx, int rep, const double s[2], const double st
   if (!ctx || rep < 0 || rep >= ctx->nrep || !s || !strn) return MSC_E_ARG;
   Thermo t;
   t.s[0] = s[0]; t.s[1] = s[1]; t.strn[0] = strn[0]; t.strn[1] = strn[1];
-  HIPC(hipMemcpy(ctx->d_th + rep, &t, sizeof t, hipMemcpyHostToDevice));
+  HIPC(hipMemcpyAsync(ctx->d_th + rep, &t, sizeof t, hipMemcpyHostToDevice, ctx->stream));
   return MSC_OK;
 }
 
@@ -715,7 +780,7 @@ int msc_neigh_list(msc_ctx *ctx, int rep, double r_list) {
   if ((rc = lists_build(ctx, ctx->d_one))) return rc;
   int flags[4 * 64];
   std::vector<int> fl(4 * ctx->nrep);
-  HIPC(hipMemcpy(fl.data(), ctx->d_flags, sizeof(int) * fl.size(), hipMemcpyDeviceToHost));
+  if (int rc_ = d2h(ctx, fl.data(), ctx->d_flags, sizeof(int) * fl.size())) return rc_;
   (void)flags;
   for (int r = 0; r < ctx->nrep; ++r)
     if (fl[4 * r + 2]) return fail(ctx, MSC_E_CAPACITY, "neighbour list region overflow");
@@ -729,9 +794,9 @@ int msc_get_neigh_list(msc_ctx *ctx, int rep, int *off, int *nbr, long cap, long
   const RepDev &R = ctx->reps[rep];
   std::vector<int> cnt(R.N), st(R.N), idx(std::max(R.nl_cap, 1));
   HIPC(hipStreamSynchronize(ctx->stream));
-  HIPC(hipMemcpy(cnt.data(), ctx->d_nl_cnt + R.n0, sizeof(int) * R.N, hipMemcpyDeviceToHost));
-  HIPC(hipMemcpy(st.data(), ctx->d_nl_start + R.n0, sizeof(int) * R.N, hipMemcpyDeviceToHost));
-  HIPC(hipMemcpy(idx.data(), ctx->d_nl_idx + R.nl_base, sizeof(int) * R.nl_cap, hipMemcpyDeviceToHost));
+  if (int rc_ = d2h(ctx, cnt.data(), ctx->d_nl_cnt + R.n0, sizeof(int) * R.N)) return rc_;
+  if (int rc_ = d2h(ctx, st.data(), ctx->d_nl_start + R.n0, sizeof(int) * R.N)) return rc_;
+  if (int rc_ = d2h(ctx, idx.data(), ctx->d_nl_idx + R.nl_base, sizeof(int) * R.nl_cap)) return rc_;
   long t = 0;
   std::vector<int> row;
   for (int i = 0; i < R.N; ++i) {
@@ -761,7 +826,7 @@ int msc_check_list(msc_ctx *ctx, int rep, double r_list_min, double r_list_max, 
   if ((rc = upload_reps(ctx))) return rc;
   State s = sel(ctx, rep);
   launch_check(s, ctx->d_task_out, ctx->d_rebuild, nullptr, 0, ctx->stream);
-  HIPC(hipMemcpy(rebuild, ctx->d_rebuild + rep, sizeof(int), hipMemcpyDeviceToHost));
+  if (int rc_ = d2h(ctx, rebuild, ctx->d_rebuild + rep, sizeof(int))) return rc_;
   return MSC_OK;
 }
 
@@ -769,8 +834,8 @@ int msc_check_list(msc_ctx *ctx, int rep, double r_list_min, double r_list_max, 
 // forces
 // --------------------------------------------------------------------------
 static int read_energy(msc_ctx *ctx, int rep, RepEnergy *e) {
-  HIPC(hipMemcpy(e, ctx->d_E + rep, sizeof(RepEnergy), hipMemcpyDeviceToHost));
-  return MSC_OK;
+  LAUNCHED("force routine");
+  return d2h(ctx, e, ctx->d_E + rep, sizeof(RepEnergy));
 }
 
 static void do_force_sc(msc_ctx *ctx, int rep) {
@@ -794,7 +859,7 @@ int msc_force_sc(msc_ctx *ctx, int rep, double r_cut, double *U, double *W, doub
   HIPC(hipMemsetAsync(ctx->d_flags + 4 * rep, 0, sizeof(int) * 4, ctx->stream));
   do_force_sc(ctx, rep);
   int fl[4];
-  HIPC(hipMemcpy(fl, ctx->d_flags + 4 * rep, sizeof fl, hipMemcpyDeviceToHost));
+  if (int rc_ = d2h(ctx, fl, ctx->d_flags + 4 * rep, sizeof fl)) return rc_;
   if (fl[0]) return fail(ctx, fl[0] == 1 ? MSC_E_OVERLAP : MSC_E_DISTANCE, "force_sc abort");
   RepEnergy e;
   int rc = read_energy(ctx, rep, &e);
@@ -919,7 +984,7 @@ int msc_self_ener(msc_ctx *ctx, int rep, double *U) {
   launch_self_ener(s, ctx->d_task_out, ctx->stream);
   launch_reduce(s, ctx->d_task_out, 16, 1, 0, ctx->d_red, 16, 15, ctx->stream);
   double v[16];
-  HIPC(hipMemcpy(v, ctx->d_red + 16 * rep, sizeof v, hipMemcpyDeviceToHost));
+  if (int rc_ = d2h(ctx, v, ctx->d_red + 16 * rep, sizeof v)) return rc_;
   *U = v[15];
   return MSC_OK;
 }
@@ -956,7 +1021,7 @@ int msc_moveP(msc_ctx *ctx, int rep, double dt, int last, double K[2], double PT
   launch_moveP(s, last != 0, ctx->d_th, ctx->d_task_out, ctx->stream);
   launch_reduce(s, ctx->d_task_out, 16, 8, 0, ctx->d_red, 16, 0, ctx->stream);
   double v[16];
-  HIPC(hipMemcpy(v, ctx->d_red + 16 * rep, sizeof v, hipMemcpyDeviceToHost));
+  if (int rc_ = d2h(ctx, v, ctx->d_red + 16 * rep, sizeof v)) return rc_;
   if (K) { K[0] = v[0] / 2; K[1] = v[1] / 2; }
   if (PT) {
     double P[9] = {v[2], v[3], v[4], v[3], v[5], v[6], v[4], v[6], v[7]};
@@ -1000,8 +1065,8 @@ int msc_init_run(msc_ctx *ctx) {
   launch_reduce(s, ctx->d_task_out, 16, 1, 0, ctx->d_red, 16, 15, ctx->stream);
   std::vector<double> red(16 * ctx->nrep);
   std::vector<int> fl(4 * ctx->nrep);
-  HIPC(hipMemcpy(red.data(), ctx->d_red, sizeof(double) * red.size(), hipMemcpyDeviceToHost));
-  HIPC(hipMemcpy(fl.data(), ctx->d_flags, sizeof(int) * fl.size(), hipMemcpyDeviceToHost));
+  if (int rc_ = d2h(ctx, red.data(), ctx->d_red, sizeof(double) * red.size())) return rc_;
+  if (int rc_ = d2h(ctx, fl.data(), ctx->d_flags, sizeof(int) * fl.size())) return rc_;
   for (int r = 0; r < ctx->nrep; ++r) {
     ctx->reps[r].Uself = red[16 * r + 15] / ctx->reps[r].N;
     if (fl[4 * r + 2]) return fail(ctx, MSC_E_CAPACITY, "neighbour list region overflow");
@@ -1067,21 +1132,26 @@ int msc_run(msc_ctx *ctx, int step0, int nsteps, double *obs) {
         gs[((size_t)k * nrep + r) * 4 + 2 * i] = a;
         gs[((size_t)k * nrep + r) * 4 + 2 * i + 1] = b;
       }
-  double *d_gs_all = nullptr, *d_obs_all = nullptr;
-  HIPC(hipMalloc((void **)&d_gs_all, sizeof(double) * std::max<size_t>(gs.size(), 1)));
-  HIPC(hipMalloc((void **)&d_obs_all, sizeof(double) * std::max<size_t>((size_t)nsteps * nrep * MSC_OBS_N, 1)));
-  HIPC(hipMemcpyAsync(d_gs_all, gs.data(), sizeof(double) * gs.size(), hipMemcpyHostToDevice, ctx->stream));
+  // run buffers grow with the longest run and are kept by the context
+  const size_t nobs = (size_t)nsteps * nrep * MSC_OBS_N;
+  if (gs.size() > ctx->run_gs_cap) {
+    if (int rc_ = dalloc(ctx, &ctx->d_run_gs, gs.size())) return rc_;
+    ctx->run_gs_cap = gs.size();
+  }
+  if (nobs > ctx->run_obs_cap) {
+    if (int rc_ = dalloc(ctx, &ctx->d_run_obs, nobs)) return rc_;
+    ctx->run_obs_cap = nobs;
+  }
+  double *d_gs_all = ctx->d_run_gs, *d_obs_all = ctx->d_run_obs;
+  if (!gs.empty())
+    HIPC(hipMemcpyAsync(d_gs_all, gs.data(), sizeof(double) * gs.size(), hipMemcpyHostToDevice, ctx->stream));
   HIPC(hipMemsetAsync(ctx->d_flags, 0, sizeof(int) * 4 * nrep, ctx->stream));
   for (int k = 0; k < nsteps; ++k)
     enqueue_step(ctx, step0 + k, d_gs_all + (size_t)k * nrep * 4, d_obs_all + (size_t)k * nrep * MSC_OBS_N);
   std::vector<int> fl(4 * nrep);
   HIPC(hipMemcpyAsync(fl.data(), ctx->d_flags, sizeof(int) * fl.size(), hipMemcpyDeviceToHost, ctx->stream));
-  if (obs)
-    HIPC(hipMemcpyAsync(obs, d_obs_all, sizeof(double) * (size_t)nsteps * nrep * MSC_OBS_N, hipMemcpyDeviceToHost,
-                        ctx->stream));
+  if (obs && nobs) HIPC(hipMemcpyAsync(obs, d_obs_all, sizeof(double) * nobs, hipMemcpyDeviceToHost, ctx->stream));
   HIPC(hipStreamSynchronize(ctx->stream));
-  hipFree(d_gs_all);
-  hipFree(d_obs_all);
   if (ctx->timing) settle_timers(ctx);
   for (int r = 0; r < nrep; ++r) {
     if (fl[4 * r + 2]) return fail(ctx, MSC_E_CAPACITY, "neighbour list region overflow during the run");
@@ -1116,14 +1186,14 @@ int msc_work_counts(msc_ctx *ctx, int rep, long counts[4]) {
   if (!ctx || rep < 0 || rep >= ctx->nrep || !counts) return MSC_E_ARG;
   double c[4];
   HIPC(hipStreamSynchronize(ctx->stream));
-  HIPC(hipMemcpy(c, ctx->d_counts + 4 * rep, sizeof c, hipMemcpyDeviceToHost));
+  if (int rc_ = d2h(ctx, c, ctx->d_counts + 4 * rep, sizeof c)) return rc_;
   const RepDev &R = ctx->reps[rep];
   long nk = ((long)(2 * R.kmax + 1) * (2 * R.kmax + 1) * (2 * R.kmax + 1) - 1) / 2;
   counts[0] = (long)c[0];
   counts[1] = (long)c[1];
   counts[2] = nk;
   std::vector<int> cnt(R.N);
-  HIPC(hipMemcpy(cnt.data(), ctx->d_nl_cnt + R.n0, sizeof(int) * R.N, hipMemcpyDeviceToHost));
+  if (int rc_ = d2h(ctx, cnt.data(), ctx->d_nl_cnt + R.n0, sizeof(int) * R.N)) return rc_;
   long t = 0;
   for (int v : cnt) t += v;
   counts[3] = t / 2;

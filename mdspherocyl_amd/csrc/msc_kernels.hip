@@ -32,6 +32,9 @@ __device__ __forceinline__ void wave_store_sums(double *dst, const double *v, in
 // Task of the current wave (4 waves per 256-thread block).
 __device__ __forceinline__ int wave_task_id() { return blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; }
 
+template <typename R>
+__device__ __forceinline__ R fmaR(R a, R b, R c) { return fma(a, b, c); }
+
 // --------------------------------------------------------------------------
 // dipole lab frame: position r + dpc * u, orientation O^T c
 // (Spherocyl::set_dip / translate / rotate, v25/classes.cpp:148-304)
@@ -415,14 +418,30 @@ __device__ __forceinline__ bool dip_pair(const R rij_in[3], const R oi[3], const
   R SR2 = (R)1 / RIJSQ;
   R RIJ = sqrt(RIJSQ);
   if (rc > 0 && !(RIJ < rc)) return false;
-  int ix = (int)floor(RIJ / Lh);
-  ix = ix < 1 ? 1 : (ix > kTabN - 1 ? kTabN - 1 : ix);
-  R R_L0 = RIJ - Lh * (R)ix;
-  R R_L1 = RIJ - Lh * (R)(ix + 1);
-  const R *T1 = tab + 3 * ix, *T0 = tab + 3 * (ix - 1);
-  R A = (T1[0] * R_L0 - T0[0] * R_L1) / Lh;
-  R B = (T1[1] * R_L0 - T0[1] * R_L1) / Lh;
-  R C = (T1[2] * R_L0 - T0[2] * R_L1) / Lh;
+  R A, B, C;
+  if constexpr (sizeof(R) == sizeof(double)) {
+    // the reference's interpolation, operation for operation
+    int ix = (int)floor(RIJ / Lh);
+    ix = ix < 1 ? 1 : (ix > kTabN - 1 ? kTabN - 1 : ix);
+    R R_L0 = RIJ - Lh * (R)ix;
+    R R_L1 = RIJ - Lh * (R)(ix + 1);
+    const R *T1 = tab + 3 * ix, *T0 = tab + 3 * (ix - 1);
+    A = (T1[0] * R_L0 - T0[0] * R_L1) / Lh;
+    B = (T1[1] * R_L0 - T0[1] * R_L1) / Lh;
+    C = (T1[2] * R_L0 - T0[2] * R_L1) / Lh;
+  } else {
+    // same linear interpolation with weights (1 - t, t) that sum to one
+    // exactly; the reference's form loses ~1e-4 of its weights to the
+    // cancellation in r - Lh*i at single precision
+    const R q = RIJ * (R)(kTabN / (0.5 * kTabExt));
+    int ix = (int)floor(q);
+    ix = ix < 1 ? 1 : (ix > kTabN - 1 ? kTabN - 1 : ix);
+    const R t = q - (R)ix;
+    const R *T1 = tab + 3 * ix, *T0 = tab + 3 * (ix - 1);
+    A = fmaR(t, T1[0] - T0[0], T0[0]);
+    B = fmaR(t, T1[1] - T0[1], T0[1]);
+    C = fmaR(t, T1[2] - T0[2], T0[2]);
+  }
   R UIR = oi[0] * rij[0] + oi[1] * rij[1] + oi[2] * rij[2];
   R UJR = oj[0] * rij[0] + oj[1] * rij[1] + oj[2] * rij[2];
   R UIJR = UIR * UJR;
@@ -631,8 +650,6 @@ __global__ void __launch_bounds__(256) k_self_ener(State s, double *__restrict__
 template <typename R>
 struct Cpx { R re, im; };
 
-template <typename R>
-__device__ __forceinline__ R fmaR(R a, R b, R c) { return fma(a, b, c); }
 
 // per-particle phase tables e^{-i n theta_a}, n = 0..kmax, theta_a = 2 pi x_a / L_a of
 // the wrapped dipole position (as force_dipK wraps, v25/functions.cpp:1793-1794),

@@ -56,7 +56,7 @@ class MdSystem:
         self._ctx = ctypes.c_void_p()
         rc = self.L.msc_create(device, self.precision, self.nrep, arr, ctypes.byref(self._ctx))
         if rc:
-            raise MscError(rc, "msc_create failed")
+            raise MscError(rc, "msc_create failed: " + (self.L.msc_last_error(None) or b"").decode())
 
     # -- plumbing ----------------------------------------------------------
     def _chk(self, rc):

@@ -113,4 +113,5 @@ def max_rel_row(a, b):
     typical magnitude)."""
     a, b = np.asarray(a), np.asarray(b)
     rms = np.sqrt(np.mean(np.sum(b * b, axis=1)))
-    return float(np.max(np.linalg.norm(a - b, axis=1)) / rms)
+    err = float(np.max(np.linalg.norm(a - b, axis=1)))
+    return err / rms if rms > 0 else err

@@ -1,0 +1,148 @@
+// CPU emulation of the subset of the HIP runtime the library uses -- TEST
+// INFRASTRUCTURE for bounds checking under AddressSanitizer on a machine
+// without a GPU.  Blocks run one after another; the threads of a block run
+// as OS threads, with real barriers for __syncthreads and a per-wavefront
+// exchange for __shfl_xor.  Device memory is host heap memory.
+#pragma once
+#include <algorithm>
+#include <atomic>
+#include <barrier>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <functional>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#define __global__
+#define __device__
+#define __host__
+#define __forceinline__ inline
+#define __launch_bounds__(...)
+#define __shared__ static
+
+using std::floor;
+using std::sqrt;
+using std::fabs;
+using std::fmax;
+using std::fma;
+using std::exp;
+using std::min;
+using std::max;
+
+struct dim3 {
+  unsigned x = 1, y = 1, z = 1;
+  dim3(unsigned a = 1, unsigned b = 1, unsigned c = 1) : x(a), y(b), z(c) {}
+};
+
+namespace hipemu {
+struct Block {
+  std::unique_ptr<std::barrier<>> bar;
+  std::vector<std::unique_ptr<std::barrier<>>> wbar;
+  std::vector<double> xch;  // [thread] exchange slots for shuffles
+};
+inline Block *&cur_block() { static Block *b = nullptr; return b; }
+inline thread_local dim3 t_threadIdx, t_blockIdx, t_blockDim, t_gridDim;
+inline thread_local bool t_done = false;
+}  // namespace hipemu
+
+#define threadIdx (hipemu::t_threadIdx)
+#define blockIdx (hipemu::t_blockIdx)
+#define blockDim (hipemu::t_blockDim)
+#define gridDim (hipemu::t_gridDim)
+
+inline void __syncthreads() { hipemu::cur_block()->bar->arrive_and_wait(); }
+
+template <typename T>
+inline T __shfl_xor(T v, int o) {
+  hipemu::Block *b = hipemu::cur_block();
+  unsigned t = hipemu::t_threadIdx.x, w = t / 64, lane = t % 64;
+  double *x = b->xch.data() + w * 64;
+  x[lane] = (double)v;
+  b->wbar[w]->arrive_and_wait();
+  double r = x[lane ^ (unsigned)o];
+  b->wbar[w]->arrive_and_wait();
+  return (T)r;
+}
+
+inline int atomicAdd(int *p, int v) { return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST); }
+inline int atomicCAS(int *p, int cmp, int val) {
+  __atomic_compare_exchange_n(p, &cmp, val, false, __ATOMIC_SEQ_CST, __ATOMIC_SEQ_CST);
+  return cmp;
+}
+
+typedef int hipError_t;
+typedef void *hipStream_t;
+typedef void *hipEvent_t;
+typedef void *hipGraphExec_t;
+enum { hipSuccess = 0, hipErrorInvalidValue = 1, hipErrorOutOfMemory = 2 };
+enum hipMemcpyKind { hipMemcpyHostToDevice, hipMemcpyDeviceToHost, hipMemcpyDeviceToDevice, hipMemcpyHostToHost };
+
+inline const char *hipGetErrorString(hipError_t e) { return e ? "emulated error" : "no error"; }
+inline hipError_t hipSetDevice(int) { return hipSuccess; }
+inline hipError_t hipStreamCreate(hipStream_t *s) { *s = (void *)0x1; return hipSuccess; }
+inline hipError_t hipStreamDestroy(hipStream_t) { return hipSuccess; }
+inline hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
+inline hipError_t hipGraphExecDestroy(hipGraphExec_t) { return hipSuccess; }
+inline hipError_t hipMalloc(void **p, size_t n) {
+  *p = std::malloc(n ? n : 1);
+  return *p ? hipSuccess : hipErrorOutOfMemory;
+}
+inline hipError_t hipFree(void *p) { std::free(p); return hipSuccess; }
+inline hipError_t hipMemsetAsync(void *p, int v, size_t n, hipStream_t) { std::memset(p, v, n); return hipSuccess; }
+inline hipError_t hipMemcpyAsync(void *d, const void *s, size_t n, hipMemcpyKind, hipStream_t) {
+  std::memcpy(d, s, n);
+  return hipSuccess;
+}
+inline hipError_t hipMemcpy(void *d, const void *s, size_t n, hipMemcpyKind) { std::memcpy(d, s, n); return hipSuccess; }
+inline hipError_t hipEventCreate(hipEvent_t *e) { *e = nullptr; return hipSuccess; }
+inline hipError_t hipEventDestroy(hipEvent_t) { return hipSuccess; }
+inline hipError_t hipEventRecord(hipEvent_t, hipStream_t) { return hipSuccess; }
+inline hipError_t hipEventElapsedTime(float *ms, hipEvent_t, hipEvent_t) { *ms = 0; return hipSuccess; }
+inline hipError_t hipGetLastError() { return hipSuccess; }
+inline hipError_t hipDeviceSynchronize() { return hipSuccess; }
+
+namespace hipemu {
+template <typename F, typename... A>
+void launch(F f, dim3 grid, dim3 block, size_t, hipStream_t, A... args) {
+  unsigned nt = block.x * block.y * block.z;
+  if (nt == 0 || grid.x == 0 || grid.y == 0 || grid.z == 0) {
+    std::fprintf(stderr, "hipemu: empty launch\n");
+    std::abort();
+  }
+  if (nt % 64 != 0 && nt > 64) {
+    std::fprintf(stderr, "hipemu: block size %u not a multiple of 64\n", nt);
+    std::abort();
+  }
+  for (unsigned bz = 0; bz < grid.z; ++bz)
+    for (unsigned by = 0; by < grid.y; ++by)
+      for (unsigned bx = 0; bx < grid.x; ++bx) {
+        Block B;
+        B.bar = std::make_unique<std::barrier<>>(nt);
+        unsigned nw = (nt + 63) / 64;
+        for (unsigned w = 0; w < nw; ++w) B.wbar.push_back(std::make_unique<std::barrier<>>(std::min(64u, nt - 64 * w)));
+        B.xch.assign(nw * 64, 0.0);
+        cur_block() = &B;
+        std::vector<std::thread> th;
+        th.reserve(nt);
+        for (unsigned t = 0; t < nt; ++t)
+          th.emplace_back([&, t]() {
+            t_threadIdx = dim3(t, 0, 0);
+            t_blockIdx = dim3(bx, by, bz);
+            t_blockDim = block;
+            t_gridDim = grid;
+            f(args...);
+            // a thread that leaves drops out of every barrier it could meet
+            B.bar->arrive_and_drop();
+            B.wbar[t / 64]->arrive_and_drop();
+          });
+        for (auto &x : th) x.join();
+        cur_block() = nullptr;
+      }
+}
+}  // namespace hipemu
+
+#define hipLaunchKernelGGL(K, G, B, S, ST, ...) hipemu::launch(K, dim3(G), dim3(B), S, ST, __VA_ARGS__)
