@@ -112,12 +112,14 @@ struct msc_ctx {
   std::map<std::string, TimerSet> timers;
   std::vector<hipEvent_t> event_pool;
   double *d_run_gs = nullptr, *d_run_obs = nullptr;
+  void *pin = nullptr;  // pinned staging buffer of kPinBytes
   size_t run_gs_cap = 0, run_obs_cap = 0;
   hipGraphExec_t graph_exec = nullptr;
   int graph_renorm = -1;
 };
 
 static std::string g_err;  // failures that have no context to hold them
+static constexpr size_t kPinBytes = (size_t)4 << 20;
 
 // One stream per device, created on first use and kept until the process
 // ends: contexts on a device share it, so creating and destroying contexts
@@ -160,9 +162,32 @@ static hipError_t device_stream(int device, hipStream_t *out) {
     }                                                                                \
   } while (0)
 
-// device -> host copy ordered after the work queued on the context stream
+// Host <-> device transfers.  The runtime is never handed pageable host
+// memory: every transfer goes through the context's pinned staging buffer
+// (allocated with hipHostMalloc), chunk by chunk, ordered on the context
+// stream and complete when the call returns.  Pageable transfers of a
+// megabyte or more would otherwise make the runtime lock the caller's pages
+// in place for the DMA engine.
 static int d2h(msc_ctx *ctx, void *dst, const void *src, size_t bytes) {
-  HIPC(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  char *d = (char *)dst;
+  const char *s_ = (const char *)src;
+  for (size_t off = 0; off < bytes; off += kPinBytes) {
+    size_t n = std::min(kPinBytes, bytes - off);
+    HIPC(hipMemcpyAsync(ctx->pin, s_ + off, n, hipMemcpyDeviceToHost, ctx->stream));
+    HIPC(hipStreamSynchronize(ctx->stream));
+    memcpy(d + off, ctx->pin, n);
+  }
+  return MSC_OK;
+}
+static int h2d(msc_ctx *ctx, void *dst, const void *src, size_t bytes) {
+  char *d = (char *)dst;
+  const char *s_ = (const char *)src;
+  for (size_t off = 0; off < bytes; off += kPinBytes) {
+    size_t n = std::min(kPinBytes, bytes - off);
+    HIPC(hipStreamSynchronize(ctx->stream));  // the staging buffer is free again
+    memcpy(ctx->pin, s_ + off, n);
+    HIPC(hipMemcpyAsync(d + off, ctx->pin, n, hipMemcpyHostToDevice, ctx->stream));
+  }
   HIPC(hipStreamSynchronize(ctx->stream));
   return MSC_OK;
 }
@@ -292,8 +317,7 @@ void grid(msc_ctx *ctx, int rp) {
 }
 
 int upload_reps(msc_ctx *ctx) {
-  HIPC(hipMemcpyAsync(ctx->d_reps, ctx->reps.data(), sizeof(RepDev) * ctx->nrep, hipMemcpyHostToDevice, ctx->stream));
-  return MSC_OK;
+  return h2d(ctx, ctx->d_reps, ctx->reps.data(), sizeof(RepDev) * ctx->nrep);
 }
 
 void refresh_state_ptrs(msc_ctx *ctx) {
@@ -408,6 +432,13 @@ int msc_create(int device, int precision, int nrep, const msc_replica_desc *reps
       delete ctx;
       return MSC_E_HIP;
     }
+    hipError_t e3 = hipHostMalloc(&ctx->pin, kPinBytes, hipHostMallocDefault);
+    if (e3 != hipSuccess) {
+      g_err = std::string("hipHostMalloc: ") + hipGetErrorString(e3);
+      ctx->pin = nullptr;
+      delete ctx;
+      return MSC_E_HIP;
+    }
   }
   // replica layout: particles, tasks, chunk tasks, k slots, tables, partials
   int n0 = 0, slot0 = 0;
@@ -512,39 +543,37 @@ int msc_create(int device, int precision, int nrep, const msc_replica_desc *reps
   for (int r = 0; r < nrep; ++r)
     for (int i = 0; i < ctx->reps[r].N; ++i) rep_of[ctx->reps[r].n0 + i] = r;
   std::vector<int> ones(nrep, 1);
-  hipMemcpyAsync(ctx->d_rep_of, rep_of.data(), sizeof(int) * N, hipMemcpyHostToDevice, ctx->stream);
-  hipMemcpyAsync(ctx->d_tasks, ctx->tasks.data(), sizeof(Task) * ctx->tasks.size(), hipMemcpyHostToDevice,
-                 ctx->stream);
-  hipMemcpyAsync(ctx->d_rtasks, ctx->rtasks.data(), sizeof(Task) * ctx->rtasks.size(), hipMemcpyHostToDevice,
-                 ctx->stream);
-  hipMemcpyAsync(ctx->d_one, ones.data(), sizeof(int) * nrep, hipMemcpyHostToDevice, ctx->stream);
   std::vector<Thermo> th(nrep);
   for (int r = 0; r < nrep; ++r) {
     th[r].s[0] = th[r].s[1] = reps[r].s0;
     th[r].strn[0] = th[r].strn[1] = 0;
   }
-  hipMemcpyAsync(ctx->d_th, th.data(), sizeof(Thermo) * nrep, hipMemcpyHostToDevice, ctx->stream);
   std::vector<double> kval;
   build_kspace(ctx, kval);
-  hipMemcpyAsync(ctx->d_kval, kval.data(), sizeof(double) * kval.size(), hipMemcpyHostToDevice, ctx->stream);
   std::vector<double> tab((size_t)nrep * kTabN * 3);
   for (int r = 0; r < nrep; ++r) build_table(ctx->reps[r], tab);
-  if (ctx->mixed) {
-    std::vector<float> tf(tab.begin(), tab.end());
-    hipMemcpyAsync(ctx->d_tab, tf.data(), tf.size() * sizeof(float), hipMemcpyHostToDevice, ctx->stream);
-    hipStreamSynchronize(ctx->stream);
-  } else {
-    hipMemcpyAsync(ctx->d_tab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream);
-    hipStreamSynchronize(ctx->stream);
-  }
-  upload_reps(ctx);
-  refresh_state_ptrs(ctx);
-  hipError_t es = hipStreamSynchronize(ctx->stream);
-  if (es != hipSuccess) {
-    g_err = std::string("msc_create: hipStreamSynchronize: ") + hipGetErrorString(es);
+  std::vector<float> tf;
+  if (ctx->mixed) tf.assign(tab.begin(), tab.end());
+  auto U = [&](void *dst, const void *src, size_t bytes) {
+    if (rc == MSC_OK) rc = h2d(ctx, dst, src, bytes);
+  };
+  U(ctx->d_rep_of, rep_of.data(), sizeof(int) * N);
+  U(ctx->d_tasks, ctx->tasks.data(), sizeof(Task) * ctx->tasks.size());
+  U(ctx->d_rtasks, ctx->rtasks.data(), sizeof(Task) * ctx->rtasks.size());
+  U(ctx->d_one, ones.data(), sizeof(int) * nrep);
+  U(ctx->d_th, th.data(), sizeof(Thermo) * nrep);
+  U(ctx->d_kval, kval.data(), sizeof(double) * kval.size());
+  if (ctx->mixed)
+    U(ctx->d_tab, tf.data(), tf.size() * sizeof(float));
+  else
+    U(ctx->d_tab, tab.data(), tab.size() * sizeof(double));
+  if (rc == MSC_OK) rc = upload_reps(ctx);
+  if (rc != MSC_OK) {
+    g_err = "msc_create: " + ctx->err;
     msc_destroy(ctx);
     return MSC_E_HIP;
   }
+  refresh_state_ptrs(ctx);
   *out = ctx;
   return MSC_OK;
 }
@@ -570,6 +599,7 @@ void msc_destroy(msc_ctx *ctx) {
       hipEventDestroy(ev.second);
     }
   for (hipEvent_t e : ctx->event_pool) hipEventDestroy(e);
+  if (ctx->pin) hipHostFree(ctx->pin);
   delete ctx;
 }
 
@@ -585,7 +615,7 @@ int msc_set_state(msc_ctx *ctx, int rep, const double *r, const double *p, const
     buf.resize((size_t)n);
     for (int c = 0; c < ncomp; ++c) {
       for (int i = 0; i < n; ++i) buf[i] = src[(size_t)i * ncomp + c];
-      HIPC(hipMemcpyAsync(dst + (size_t)c * N + R.n0, buf.data(), sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream));
+      if (int rc_ = h2d(ctx, dst + (size_t)c * N + R.n0, buf.data(), sizeof(double) * n)) return rc_;
     }
     return MSC_OK;
   };
@@ -663,9 +693,9 @@ int msc_set_forces(msc_ctx *ctx, int rep, const double *F, const double *T) {
   std::vector<double> buf(n);
   for (int c = 0; c < 3; ++c) {
     for (int i = 0; i < n; ++i) buf[i] = F[3 * i + c];
-    HIPC(hipMemcpyAsync(ctx->d_F + (size_t)c * N + R.n0, buf.data(), sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream));
+    if (int rc_ = h2d(ctx, ctx->d_F + (size_t)c * N + R.n0, buf.data(), sizeof(double) * n)) return rc_;
     for (int i = 0; i < n; ++i) buf[i] = T[3 * i + c];
-    HIPC(hipMemcpyAsync(ctx->d_T + (size_t)c * N + R.n0, buf.data(), sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream));
+    if (int rc_ = h2d(ctx, ctx->d_T + (size_t)c * N + R.n0, buf.data(), sizeof(double) * n)) return rc_;
   }
   return MSC_OK;
 }
@@ -684,8 +714,7 @@ int msc_set_thermostat(msc_ctx *ctx, int rep, const double s[2], const double st
   if (!ctx || rep < 0 || rep >= ctx->nrep || !s || !strn) return MSC_E_ARG;
   Thermo t;
   t.s[0] = s[0]; t.s[1] = s[1]; t.strn[0] = strn[0]; t.strn[1] = strn[1];
-  HIPC(hipMemcpyAsync(ctx->d_th + rep, &t, sizeof t, hipMemcpyHostToDevice, ctx->stream));
-  return MSC_OK;
+  return h2d(ctx, ctx->d_th + rep, &t, sizeof t);
 }
 
 // --------------------------------------------------------------------------
@@ -720,7 +749,7 @@ static int lists_layout(msc_ctx *ctx) {
       return rc;
     ctx->ncells = ncells;
   }
-  HIPC(hipMemcpyAsync(ctx->d_cell_rep, cell_rep.data(), sizeof(int) * ncells, hipMemcpyHostToDevice, ctx->stream));
+  if (int rc_ = h2d(ctx, ctx->d_cell_rep, cell_rep.data(), sizeof(int) * ncells)) return rc_;
   // counting pass with unbounded regions, then size the regions
   for (int r = 0; r < nrep; ++r) {
     ctx->reps[r].nl_base = 0;
@@ -735,9 +764,8 @@ static int lists_layout(msc_ctx *ctx) {
   launch_cells(s, ctx->d_one, ctx->d_flags, ctx->stream);
   launch_lists_count(s, ctx->d_one, ctx->d_flags, ctx->stream);
   std::vector<int> cnt(ctx->Ntot), dcnt(ctx->Ntot);
-  HIPC(hipMemcpyAsync(cnt.data(), ctx->d_nl_cnt, sizeof(int) * ctx->Ntot, hipMemcpyDeviceToHost, ctx->stream));
-  HIPC(hipMemcpyAsync(dcnt.data(), ctx->d_dl_cnt, sizeof(int) * ctx->Ntot, hipMemcpyDeviceToHost, ctx->stream));
-  HIPC(hipStreamSynchronize(ctx->stream));
+  if (int rc_ = d2h(ctx, cnt.data(), ctx->d_nl_cnt, sizeof(int) * ctx->Ntot)) return rc_;
+  if (int rc_ = d2h(ctx, dcnt.data(), ctx->d_dl_cnt, sizeof(int) * ctx->Ntot)) return rc_;
   long nl_tot = 0, dl_tot = 0;
   for (int r = 0; r < nrep; ++r) {
     RepDev &R = ctx->reps[r];
@@ -1144,14 +1172,14 @@ int msc_run(msc_ctx *ctx, int step0, int nsteps, double *obs) {
   }
   double *d_gs_all = ctx->d_run_gs, *d_obs_all = ctx->d_run_obs;
   if (!gs.empty())
-    HIPC(hipMemcpyAsync(d_gs_all, gs.data(), sizeof(double) * gs.size(), hipMemcpyHostToDevice, ctx->stream));
+    if (int rc_ = h2d(ctx, d_gs_all, gs.data(), sizeof(double) * gs.size())) return rc_;
   HIPC(hipMemsetAsync(ctx->d_flags, 0, sizeof(int) * 4 * nrep, ctx->stream));
   for (int k = 0; k < nsteps; ++k)
     enqueue_step(ctx, step0 + k, d_gs_all + (size_t)k * nrep * 4, d_obs_all + (size_t)k * nrep * MSC_OBS_N);
   std::vector<int> fl(4 * nrep);
-  HIPC(hipMemcpyAsync(fl.data(), ctx->d_flags, sizeof(int) * fl.size(), hipMemcpyDeviceToHost, ctx->stream));
-  if (obs && nobs) HIPC(hipMemcpyAsync(obs, d_obs_all, sizeof(double) * nobs, hipMemcpyDeviceToHost, ctx->stream));
-  HIPC(hipStreamSynchronize(ctx->stream));
+  if (int rc_ = d2h(ctx, fl.data(), ctx->d_flags, sizeof(int) * fl.size())) return rc_;
+  if (obs && nobs)
+    if (int rc_ = d2h(ctx, obs, d_obs_all, sizeof(double) * nobs)) return rc_;
   if (ctx->timing) settle_timers(ctx);
   for (int r = 0; r < nrep; ++r) {
     if (fl[4 * r + 2]) return fail(ctx, MSC_E_CAPACITY, "neighbour list region overflow during the run");

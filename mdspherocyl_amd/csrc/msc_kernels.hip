@@ -5,9 +5,41 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstdio>
+#include <cstdlib>
+
 #include "msc_geom.hpp"
 
 namespace msc {
+
+// With MSC_SYNC_CHECK=1 in the environment every launch is followed by a
+// device synchronisation, and a failure names the kernel on stderr (MSC_SYNC_CHECK=2
+// also names every kernel before it starts).  Off by default.
+static int sync_check_mode() {
+  static int mode = [] {
+    const char *v = std::getenv("MSC_SYNC_CHECK");
+    return v ? std::atoi(v) : 0;
+  }();
+  return mode;
+}
+static void sync_check(const char *kernel) {
+  if (sync_check_mode() == 0) return;
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e != hipSuccess) std::fprintf(stderr, "msc: kernel %s failed: %s\n", kernel, hipGetErrorString(e));
+  if (sync_check_mode() >= 2 || e != hipSuccess) std::fflush(stderr);
+}
+static void sync_announce(const char *kernel) {
+  if (sync_check_mode() < 2) return;
+  std::fprintf(stderr, "msc: launch %s\n", kernel);
+  std::fflush(stderr);
+}
+#define MSC_LAUNCH(K, ...)          \
+  do {                              \
+    sync_announce(#K);              \
+    hipLaunchKernelGGL(K, __VA_ARGS__); \
+    sync_check(#K);                 \
+  } while (0)
 
 // --------------------------------------------------------------------------
 // small helpers
@@ -113,17 +145,22 @@ __global__ void k_cells_sort(State s, const int *__restrict__ cell_rep, const in
 // real-space row counts -> row starts in the replica's list region.  A
 // replica whose rebuild flag is clear is left untouched.  Rows that would
 // overflow the region set flags[rep*4+2] and are emptied.
+// The mode is a template parameter: a three-way branch on a uniform
+// kernel argument here was compiled for gfx950 into code that, for mode 2,
+// loaded the segment bounds through uninitialised scalar address registers
+// (an aperture-violation fault on the device).
+template <int MODE>
 __global__ void __launch_bounds__(1024) k_scan_seg(int *__restrict__ cnt, int *__restrict__ out,
-                                                   const RepDev *__restrict__ reps, int mode,
+                                                   const RepDev *__restrict__ reps,
                                                    const int *__restrict__ rebuild, int *__restrict__ flags) {
   __shared__ int sh[1024];
   const int rp = blockIdx.x;
   if (!rebuild[rp]) return;
   const RepDev &R = reps[rp];
-  int b0, n, base, cap;
-  if (mode == 0) { b0 = R.cell0; n = R.ncell; base = R.n0; cap = R.N; }
-  else if (mode == 1) { b0 = R.n0; n = R.N; base = R.nl_base; cap = R.nl_cap; }
-  else { b0 = R.n0; n = R.N; base = R.dl_base; cap = R.dl_cap; }
+  const int b0 = MODE == 0 ? R.cell0 : R.n0;
+  const int n = MODE == 0 ? R.ncell : R.N;
+  const int base = MODE == 0 ? R.n0 : (MODE == 1 ? R.nl_base : R.dl_base);
+  const int cap = MODE == 0 ? R.N : (MODE == 1 ? R.nl_cap : R.dl_cap);
   int t = threadIdx.x;
   int seg = (n + 1023) / 1024;
   int b = t * seg, e = min(n, b + seg);
@@ -894,14 +931,16 @@ __global__ void __launch_bounds__(256) k_recip_force(State s, const Cpx<R> *__re
 // out[rep * ostride + ooff + k] = sum over the replica's items of
 // in[item * istride + k], k < nv.  One block of 256 threads per replica;
 // each thread sums a strided subset in order, then a fixed LDS tree.
+// WHICH 0: wave-task partials, 1: k-slot partials (compile-time, like every
+// mode switch of these kernels)
+template <int WHICH>
 __global__ void k_reduce_items(const RepDev *__restrict__ reps, int rep0, const double *__restrict__ in, int istride,
-                               int nv, int which, double *__restrict__ out, int ostride, int ooff) {
+                               int nv, double *__restrict__ out, int ostride, int ooff) {
   __shared__ double sh[256];
   const int rp = rep0 + blockIdx.x;
   const RepDev &R = reps[rp];
-  int b, n;
-  if (which == 0) { b = R.task0; n = R.ntask; }
-  else { b = R.slot0; n = R.ncol * R.KZ; }
+  const int b = WHICH == 0 ? R.task0 : R.slot0;
+  const int n = WHICH == 0 ? R.ntask : R.ncol * R.KZ;
   for (int k = 0; k < nv; ++k) {
     double v = 0.0;
     for (int i = threadIdx.x; i < n; i += 256) v += in[(size_t)(b + i) * istride + k];
@@ -1059,7 +1098,8 @@ __global__ void k_renorm(State s) {
 // moveTher for every replica (one thread each).  gs[rep*4 + 2*i + 0/1] hold
 // the Box-Muller pair drawn on the host from thermostat i's MT19937 for this
 // step (mode 0 uses the first, mode 2 the stored second).
-__global__ void k_thermo(State s, Thermo *__restrict__ th, const double *__restrict__ Kin, int mode,
+template <int MODE>
+__global__ void k_thermo(State s, Thermo *__restrict__ th, const double *__restrict__ Kin,
                          const double *__restrict__ gs) {
   int rp = blockIdx.x * blockDim.x + threadIdx.x;
   if (rp >= s.nrep) return;
@@ -1067,12 +1107,12 @@ __global__ void k_thermo(State s, Thermo *__restrict__ th, const double *__restr
   Thermo T = th[rp];
   for (int i = 0; i < 2; ++i) {
     double Q = R.Q, g = R.gamma, ex1;
-    if (mode == 0) {
+    if constexpr (MODE == 0) {
       double r0 = gs[rp * 4 + 2 * i], r1 = gs[rp * 4 + 2 * i + 1];
       T.strn[i] = r1;
       ex1 = exp(-g * R.dt / 2);
       T.s[i] = ex1 * T.s[i] + sqrt(2. * R.temp * g * R.dt / Q) * r0;
-    } else if (mode == 1) {
+    } else if constexpr (MODE == 1) {
       double K = Kin[rp * 2 + i];
       T.s[i] = T.s[i] + R.dt * (2. * K - 3. * (double)R.N * R.temp) / Q;
     } else {
@@ -1151,68 +1191,70 @@ __global__ void k_fill_int(int *__restrict__ p, int n, int v) {
 static inline int nblk(int n, int b) { return (n + b - 1) / b; }
 
 void launch_dipoles(const State &s, hipStream_t st) {
-  hipLaunchKernelGGL(k_dipoles, dim3(nblk(s.g_end - s.g_begin, 256)), dim3(256), 0, st, s);
+  MSC_LAUNCH(k_dipoles, dim3(nblk(s.g_end - s.g_begin, 256)), dim3(256), 0, st, s);
 }
 void launch_cells(const State &s, const int *rebuild, int *flags, hipStream_t st) {
-  hipLaunchKernelGGL(k_fill_int, dim3(nblk(s.ncells, 256)), dim3(256), 0, st, s.cell_cnt, s.ncells, 0);
-  hipLaunchKernelGGL(k_fill_int, dim3(nblk(s.ncells, 256)), dim3(256), 0, st, s.cell_fill, s.ncells, 0);
-  hipLaunchKernelGGL(k_cells_assign, dim3(nblk(s.g_end - s.g_begin, 256)), dim3(256), 0, st, s, rebuild);
-  hipLaunchKernelGGL(k_scan_seg, dim3(s.nrep), dim3(1024), 0, st, s.cell_cnt, s.cell_start, s.reps, 0, rebuild,
-                     flags);
-  hipLaunchKernelGGL(k_cells_fill, dim3(nblk(s.g_end - s.g_begin, 256)), dim3(256), 0, st, s, rebuild);
-  hipLaunchKernelGGL(k_cells_sort, dim3(nblk(s.ncells, 256)), dim3(256), 0, st, s, s.cell_rep, rebuild);
+  MSC_LAUNCH(k_fill_int, dim3(nblk(s.ncells, 256)), dim3(256), 0, st, s.cell_cnt, s.ncells, 0);
+  MSC_LAUNCH(k_fill_int, dim3(nblk(s.ncells, 256)), dim3(256), 0, st, s.cell_fill, s.ncells, 0);
+  MSC_LAUNCH(k_cells_assign, dim3(nblk(s.g_end - s.g_begin, 256)), dim3(256), 0, st, s, rebuild);
+  MSC_LAUNCH(k_scan_seg<0>, dim3(s.nrep), dim3(1024), 0, st, s.cell_cnt, s.cell_start, s.reps, rebuild, flags);
+  MSC_LAUNCH(k_cells_fill, dim3(nblk(s.g_end - s.g_begin, 256)), dim3(256), 0, st, s, rebuild);
+  MSC_LAUNCH(k_cells_sort, dim3(nblk(s.ncells, 256)), dim3(256), 0, st, s, s.cell_rep, rebuild);
 }
 void launch_lists_count(const State &s, const int *rebuild, int *flags, hipStream_t st) {
-  hipLaunchKernelGGL(k_build_lists<0>, dim3(nblk(s.g_end - s.g_begin, 128)), dim3(128), 0, st, s, rebuild);
-  hipLaunchKernelGGL(k_scan_seg, dim3(s.nrep), dim3(1024), 0, st, s.nl_cnt, s.nl_start, s.reps, 1, rebuild, flags);
+  MSC_LAUNCH(k_build_lists<0>, dim3(nblk(s.g_end - s.g_begin, 128)), dim3(128), 0, st, s, rebuild);
+  MSC_LAUNCH(k_scan_seg<1>, dim3(s.nrep), dim3(1024), 0, st, s.nl_cnt, s.nl_start, s.reps, rebuild, flags);
   if (s.dl_cnt)
-    hipLaunchKernelGGL(k_scan_seg, dim3(s.nrep), dim3(1024), 0, st, s.dl_cnt, s.dl_start, s.reps, 2, rebuild,
-                       flags);
+    MSC_LAUNCH(k_scan_seg<2>, dim3(s.nrep), dim3(1024), 0, st, s.dl_cnt, s.dl_start, s.reps, rebuild, flags);
 }
 void launch_lists_fill(const State &s, const int *rebuild, hipStream_t st) {
-  hipLaunchKernelGGL(k_build_lists<1>, dim3(nblk(s.g_end - s.g_begin, 128)), dim3(128), 0, st, s, rebuild);
-  hipLaunchKernelGGL(k_savec, dim3(nblk(s.g_end - s.g_begin, 256)), dim3(256), 0, st, s, rebuild);
+  MSC_LAUNCH(k_build_lists<1>, dim3(nblk(s.g_end - s.g_begin, 128)), dim3(128), 0, st, s, rebuild);
+  MSC_LAUNCH(k_savec, dim3(nblk(s.g_end - s.g_begin, 256)), dim3(256), 0, st, s, rebuild);
 }
 void launch_check(const State &s, double *task_out, int *rebuild, double *obs_rebuild, int obs_stride,
                   hipStream_t st) {
-  hipLaunchKernelGGL(k_check_list, dim3(nblk(s.ntask, 4)), dim3(256), 0, st, s, task_out);
-  hipLaunchKernelGGL(k_check_decide, dim3(nblk(s.rep_end - s.rep_begin, 64)), dim3(64), 0, st, s, task_out, rebuild,
+  MSC_LAUNCH(k_check_list, dim3(nblk(s.ntask, 4)), dim3(256), 0, st, s, task_out);
+  MSC_LAUNCH(k_check_decide, dim3(nblk(s.rep_end - s.rep_begin, 64)), dim3(64), 0, st, s, task_out, rebuild,
                      obs_rebuild, obs_stride);
 }
 void launch_force_sc(const State &s, bool strict, double *task_out, int *flags, hipStream_t st) {
   if (strict)
-    hipLaunchKernelGGL(k_force_sc<true>, dim3(nblk(s.ntask, 4)), dim3(256), 0, st, s, task_out, flags);
+    MSC_LAUNCH(k_force_sc<true>, dim3(nblk(s.ntask, 4)), dim3(256), 0, st, s, task_out, flags);
   else
-    hipLaunchKernelGGL(k_force_sc<false>, dim3(nblk(s.ntask, 4)), dim3(256), 0, st, s, task_out, flags);
+    MSC_LAUNCH(k_force_sc<false>, dim3(nblk(s.ntask, 4)), dim3(256), 0, st, s, task_out, flags);
 }
 void launch_dipR(const State &s, bool mixed, bool all_pairs, const void *tab, double *task_out, hipStream_t st) {
   if (mixed) {
     if (all_pairs)
-      hipLaunchKernelGGL(k_dipR_all<float>, dim3(nblk(s.ntask, 4)), dim3(256), 0, st, s, (const float *)tab, task_out);
+      MSC_LAUNCH(k_dipR_all<float>, dim3(nblk(s.ntask, 4)), dim3(256), 0, st, s, (const float *)tab, task_out);
     else
-      hipLaunchKernelGGL(k_dipR_list<float>, dim3(nblk(s.ntask, 4)), dim3(256), 0, st, s, (const float *)tab, task_out);
+      MSC_LAUNCH(k_dipR_list<float>, dim3(nblk(s.ntask, 4)), dim3(256), 0, st, s, (const float *)tab, task_out);
   } else {
     if (all_pairs)
-      hipLaunchKernelGGL(k_dipR_all<double>, dim3(nblk(s.ntask, 4)), dim3(256), 0, st, s, (const double *)tab, task_out);
+      MSC_LAUNCH(k_dipR_all<double>, dim3(nblk(s.ntask, 4)), dim3(256), 0, st, s, (const double *)tab, task_out);
     else
-      hipLaunchKernelGGL(k_dipR_list<double>, dim3(nblk(s.ntask, 4)), dim3(256), 0, st, s, (const double *)tab, task_out);
+      MSC_LAUNCH(k_dipR_list<double>, dim3(nblk(s.ntask, 4)), dim3(256), 0, st, s, (const double *)tab, task_out);
   }
 }
 void launch_ext_field(const State &s, double *task_out, hipStream_t st) {
-  hipLaunchKernelGGL(k_ext_field, dim3(nblk(s.ntask, 4)), dim3(256), 0, st, s, task_out);
+  MSC_LAUNCH(k_ext_field, dim3(nblk(s.ntask, 4)), dim3(256), 0, st, s, task_out);
 }
 void launch_self_ener(const State &s, double *task_out, hipStream_t st) {
-  hipLaunchKernelGGL(k_self_ener, dim3(nblk(s.ntask, 4)), dim3(256), 0, st, s, task_out);
+  MSC_LAUNCH(k_self_ener, dim3(nblk(s.ntask, 4)), dim3(256), 0, st, s, task_out);
 }
 void launch_reduce(const State &s, const double *in, int istride, int nv, int which, double *out, int ostride,
                    int ooff, hipStream_t st) {
-  hipLaunchKernelGGL(k_reduce_items, dim3(s.rep_end - s.rep_begin), dim3(256), 0, st, s.reps, s.rep_begin, in,
-                     istride, nv, which, out, ostride, ooff);
+  if (which == 0)
+    MSC_LAUNCH(k_reduce_items<0>, dim3(s.rep_end - s.rep_begin), dim3(256), 0, st, s.reps, s.rep_begin, in, istride,
+               nv, out, ostride, ooff);
+  else
+    MSC_LAUNCH(k_reduce_items<1>, dim3(s.rep_end - s.rep_begin), dim3(256), 0, st, s.reps, s.rep_begin, in, istride,
+               nv, out, ostride, ooff);
 }
 
 template <typename R>
 static void recip_tables_t(const State &s, void *ph, void *amp, double *task_out, hipStream_t st) {
-  hipLaunchKernelGGL(k_recip_tables<R>, dim3(nblk(s.ntask, 4)), dim3(256), 0, st, s, (Cpx<R> *)ph, (R *)amp,
+  MSC_LAUNCH(k_recip_tables<R>, dim3(nblk(s.ntask, 4)), dim3(256), 0, st, s, (Cpx<R> *)ph, (R *)amp,
                      task_out);
 }
 void launch_recip_tables(const State &s, bool mixed, void *ph, void *amp, double *task_out, hipStream_t st) {
@@ -1223,12 +1265,12 @@ void launch_recip_tables(const State &s, bool mixed, void *ph, void *amp, double
 template <typename R, int KZ>
 static void recip_struct_kz(const State &s, int rt0, int nrt, int maxcol, const void *ph, const void *amp,
                             void *part, hipStream_t st) {
-  hipLaunchKernelGGL((k_recip_struct<R, KZ>), dim3(nrt, nblk(maxcol, 256)), dim3(256), 0, st, s, rt0,
+  MSC_LAUNCH((k_recip_struct<R, KZ>), dim3(nrt, nblk(maxcol, 256)), dim3(256), 0, st, s, rt0,
                      (const Cpx<R> *)ph, (const R *)amp, (R *)part);
 }
 template <typename R, int KZ>
 static void recip_force_kz(const State &s, const void *ph, const void *amp, const void *A, hipStream_t st) {
-  hipLaunchKernelGGL((k_recip_force<R, KZ>), dim3(nblk(s.ntask, 4)), dim3(256), 0, st, s, (const Cpx<R> *)ph,
+  MSC_LAUNCH((k_recip_force<R, KZ>), dim3(nblk(s.ntask, 4)), dim3(256), 0, st, s, (const Cpx<R> *)ph,
                      (const R *)amp, (const Cpx<R> *)A);
 }
 
@@ -1265,41 +1307,46 @@ int launch_recip_force(const State &s, bool mixed, int KZ, const void *ph, const
 void launch_recip_reduce(const State &s, bool mixed, const void *part, const double *M2, void *A, double *slot_out,
                          int maxslot, hipStream_t st) {
   if (mixed)
-    hipLaunchKernelGGL(k_recip_reduce<float>, dim3(nblk(maxslot, 256), s.rep_end - s.rep_begin), dim3(256), 0, st, s,
+    MSC_LAUNCH(k_recip_reduce<float>, dim3(nblk(maxslot, 256), s.rep_end - s.rep_begin), dim3(256), 0, st, s,
                        (const float *)part, M2, (Cpx<float> *)A, slot_out);
   else
-    hipLaunchKernelGGL(k_recip_reduce<double>, dim3(nblk(maxslot, 256), s.rep_end - s.rep_begin), dim3(256), 0, st, s,
+    MSC_LAUNCH(k_recip_reduce<double>, dim3(nblk(maxslot, 256), s.rep_end - s.rep_begin), dim3(256), 0, st, s,
                        (const double *)part, M2, (Cpx<double> *)A, slot_out);
 }
 
 void launch_moveP(const State &s, bool last, const Thermo *th, double *task_out, hipStream_t st) {
-  if (last) hipLaunchKernelGGL(k_moveP<true>, dim3(nblk(s.ntask, 4)), dim3(256), 0, st, s, th, task_out);
-  else hipLaunchKernelGGL(k_moveP<false>, dim3(nblk(s.ntask, 4)), dim3(256), 0, st, s, th, task_out);
+  if (last) MSC_LAUNCH(k_moveP<true>, dim3(nblk(s.ntask, 4)), dim3(256), 0, st, s, th, task_out);
+  else MSC_LAUNCH(k_moveP<false>, dim3(nblk(s.ntask, 4)), dim3(256), 0, st, s, th, task_out);
 }
 void launch_moveX(const State &s, hipStream_t st) {
-  hipLaunchKernelGGL(k_moveX, dim3(nblk(s.g_end - s.g_begin, 256)), dim3(256), 0, st, s);
+  MSC_LAUNCH(k_moveX, dim3(nblk(s.g_end - s.g_begin, 256)), dim3(256), 0, st, s);
 }
 void launch_renorm(const State &s, hipStream_t st) {
-  hipLaunchKernelGGL(k_renorm, dim3(nblk(s.g_end - s.g_begin, 256)), dim3(256), 0, st, s);
+  MSC_LAUNCH(k_renorm, dim3(nblk(s.g_end - s.g_begin, 256)), dim3(256), 0, st, s);
 }
 void launch_thermo(const State &s, Thermo *th, const double *Kin, int mode, const double *gs, hipStream_t st) {
-  hipLaunchKernelGGL(k_thermo, dim3(nblk(s.nrep, 64)), dim3(64), 0, st, s, th, Kin, mode, gs);
+  if (mode == 0)
+    MSC_LAUNCH(k_thermo<0>, dim3(nblk(s.nrep, 64)), dim3(64), 0, st, s, th, Kin, gs);
+  else if (mode == 1)
+    MSC_LAUNCH(k_thermo<1>, dim3(nblk(s.nrep, 64)), dim3(64), 0, st, s, th, Kin, gs);
+  else
+    MSC_LAUNCH(k_thermo<2>, dim3(nblk(s.nrep, 64)), dim3(64), 0, st, s, th, Kin, gs);
 }
 void launch_kin_finish(const State &s, const double *red, double *Kin, hipStream_t st) {
-  hipLaunchKernelGGL(k_kin_finish, dim3(nblk(s.nrep, 64)), dim3(64), 0, st, s, red, Kin);
+  MSC_LAUNCH(k_kin_finish, dim3(nblk(s.nrep, 64)), dim3(64), 0, st, s, red, Kin);
 }
 void launch_observables(const State &s, const RepEnergy *E, const double *Kin, const double *mom, double *obs,
                         hipStream_t st) {
-  hipLaunchKernelGGL(k_observables, dim3(nblk(s.nrep, 64)), dim3(64), 0, st, s, E, Kin, mom, obs);
+  MSC_LAUNCH(k_observables, dim3(nblk(s.nrep, 64)), dim3(64), 0, st, s, E, Kin, mom, obs);
 }
 void launch_momenta(const State &s, double *task_out, hipStream_t st) {
-  hipLaunchKernelGGL(k_momenta, dim3(nblk(s.ntask, 4)), dim3(256), 0, st, s, task_out);
+  MSC_LAUNCH(k_momenta, dim3(nblk(s.ntask, 4)), dim3(256), 0, st, s, task_out);
 }
 void launch_zero_FT(const State &s, hipStream_t st) {
-  hipLaunchKernelGGL(k_zero_FT, dim3(nblk(s.g_end - s.g_begin, 256)), dim3(256), 0, st, s);
+  MSC_LAUNCH(k_zero_FT, dim3(nblk(s.g_end - s.g_begin, 256)), dim3(256), 0, st, s);
 }
 void launch_fill_int(int *p, int n, int v, hipStream_t st) {
-  hipLaunchKernelGGL(k_fill_int, dim3(nblk(n, 256)), dim3(256), 0, st, p, n, v);
+  MSC_LAUNCH(k_fill_int, dim3(nblk(n, 256)), dim3(256), 0, st, p, n, v);
 }
 
 }  // namespace msc

@@ -92,6 +92,12 @@ inline hipError_t hipMalloc(void **p, size_t n) {
   return *p ? hipSuccess : hipErrorOutOfMemory;
 }
 inline hipError_t hipFree(void *p) { std::free(p); return hipSuccess; }
+enum { hipHostMallocDefault = 0 };
+inline hipError_t hipHostMalloc(void **p, size_t n, unsigned) {
+  *p = std::malloc(n ? n : 1);
+  return *p ? hipSuccess : hipErrorOutOfMemory;
+}
+inline hipError_t hipHostFree(void *p) { std::free(p); return hipSuccess; }
 inline hipError_t hipMemsetAsync(void *p, int v, size_t n, hipStream_t) { std::memset(p, v, n); return hipSuccess; }
 inline hipError_t hipMemcpyAsync(void *d, const void *s, size_t n, hipMemcpyKind, hipStream_t) {
   std::memcpy(d, s, n);
