@@ -147,6 +147,12 @@ int msc_run(msc_ctx *ctx, int step0, int nsteps, double *obs);
 /* 1: replay each step as one captured HIP graph (default), 0: launch
  * kernels directly. */
 int msc_set_graph(msc_ctx *ctx, int on);
+/* msc_run advances in chunks of 50 steps.  A chunk during which a list
+ * rebuild outgrew its device region is rolled back, the regions are enlarged
+ * and the lists rebuilt from the chunk's first state, and the chunk is run
+ * again; *n receives how many chunks were repeated since msc_create.  No
+ * reference counterpart (the reference's lists are host deques). */
+int msc_chunk_repeats(msc_ctx *ctx, long *n);
 
 /* ---- cluster labels (v25/functions.cpp:2950-3324) ----------------------
  * Components of the contact graph d_seg^2 < ((D_i+D_j)/2 + min_s)^2, labelled

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -112,6 +113,9 @@ struct msc_ctx {
   std::map<std::string, TimerSet> timers;
   std::vector<hipEvent_t> event_pool;
   double *d_run_gs = nullptr, *d_run_obs = nullptr;
+  char *d_save = nullptr;  // chunk rollback copy of the step state
+  long chunk_repeats = 0;  // chunks of msc_run repeated after a list overflow
+  size_t save_cap = 0;
   void *pin = nullptr;  // pinned staging buffer of kPinBytes
   size_t run_gs_cap = 0, run_obs_cap = 0;
   hipGraphExec_t graph_exec = nullptr;
@@ -590,7 +594,7 @@ void msc_destroy(msc_ctx *ctx) {
                   ctx->d_nl_start, ctx->d_nl_idx, ctx->d_dl_cnt, ctx->d_dl_start, ctx->d_dl_idx, ctx->d_kval,
                   ctx->d_tab, ctx->d_ph, ctx->d_amp, ctx->d_part, ctx->d_A, ctx->d_slot_out, ctx->d_task_out,
                   ctx->d_red, ctx->d_M2, ctx->d_Kin, ctx->d_mom, ctx->d_gs, ctx->d_obs, ctx->d_counts, ctx->d_E,
-                  ctx->d_th, ctx->d_flags, ctx->d_rebuild, ctx->d_one, ctx->d_run_gs, ctx->d_run_obs};
+                  ctx->d_th, ctx->d_flags, ctx->d_rebuild, ctx->d_one, ctx->d_run_gs, ctx->d_run_obs, ctx->d_save};
   for (void *b : bufs)
     if (b) hipFree(b);
   for (auto &kv : ctx->timers)
@@ -732,7 +736,26 @@ static int lists_build(msc_ctx *ctx, const int *d_flagmask) {
 }
 
 // (re)allocate cells and list regions for every replica from a counting pass
-static int lists_layout(msc_ctx *ctx) {
+// Expected full-list entries per particle of a fluid at the replica's number
+// density: rho times the volume within segment distance r of a rod, averaged
+// over random relative orientations, 4/3 pi r^3 + pi r^2 (L_i + L_j) +
+// (pi/2) r L_i L_j, with L_i = L_j = the largest length.
+static double fluid_neighbours(const RepDev &R, double r) {
+  double rho = R.N / R.V, L = R.lmax;
+  return rho * (4.0 / 3.0 * M_PI * r * r * r + M_PI * r * r * 2 * L + 0.5 * M_PI * r * L * L);
+}
+
+// MSC_TIGHT_LISTS=1 in the environment sizes list regions to the present
+// count only, so that any growth takes the overflow path of msc_run.
+static bool tight_lists() {
+  const char *v = std::getenv("MSC_TIGHT_LISTS");
+  return v && std::atoi(v) != 0;
+}
+
+// need_nl / need_dl: per-replica entries that a later rebuild found missing
+// (empty on a first layout)
+static int lists_layout(msc_ctx *ctx, const std::vector<long> &need_nl = {},
+                        const std::vector<long> &need_dl = {}) {
   const int nrep = ctx->nrep;
   int ncells = 0;
   std::vector<int> cell_rep;
@@ -774,8 +797,21 @@ static int lists_layout(msc_ctx *ctx) {
       a += cnt[i];
       if (R.rc > 0) b += dcnt[i];
     }
-    // headroom for the growth of the lists between layouts
-    long ca = a + a / 2 + 64 * 16, cb = R.rc > 0 ? b + b / 2 + 64 * 16 : 0;
+    // headroom for the growth of the lists between layouts: half again the
+    // present count, at least 1.25 times the count of a fluid at this
+    // density, and 1.25 times what a rebuild found missing
+    long ca = std::max(a + a / 2, (long)(1.25 * R.N * fluid_neighbours(R, R.r_list_max))) + 64 * 16;
+    long cb = 0;
+    if (R.rc > 0) {
+      double rd = R.rl_dip + 2 * ctx->S.lever_max;
+      cb = std::max(b + b / 2, (long)(1.25 * R.N * (R.N / R.V) * 4.0 / 3.0 * M_PI * rd * rd * rd)) + 64 * 16;
+    }
+    if (tight_lists()) {  // regions exactly as counted (exercises the overflow path)
+      ca = a;
+      cb = R.rc > 0 ? b : 0;
+    }
+    if (!need_nl.empty()) ca = std::max(ca, need_nl[r] + need_nl[r] / 4);
+    if (!need_dl.empty() && R.rc > 0) cb = std::max(cb, need_dl[r] + need_dl[r] / 4);
     R.nl_base = (int)nl_tot;
     R.nl_cap = (int)ca;
     R.dl_base = (int)dl_tot;
@@ -811,7 +847,7 @@ int msc_neigh_list(msc_ctx *ctx, int rep, double r_list) {
   if (int rc_ = d2h(ctx, fl.data(), ctx->d_flags, sizeof(int) * fl.size())) return rc_;
   (void)flags;
   for (int r = 0; r < ctx->nrep; ++r)
-    if (fl[4 * r + 2]) return fail(ctx, MSC_E_CAPACITY, "neighbour list region overflow");
+    if (fl[4 * r + 2] || fl[4 * r + 3]) return fail(ctx, MSC_E_CAPACITY, "neighbour list region overflow");
   ctx->have_lists = true;
   return MSC_OK;
 }
@@ -1097,7 +1133,7 @@ int msc_init_run(msc_ctx *ctx) {
   if (int rc_ = d2h(ctx, fl.data(), ctx->d_flags, sizeof(int) * fl.size())) return rc_;
   for (int r = 0; r < ctx->nrep; ++r) {
     ctx->reps[r].Uself = red[16 * r + 15] / ctx->reps[r].N;
-    if (fl[4 * r + 2]) return fail(ctx, MSC_E_CAPACITY, "neighbour list region overflow");
+    if (fl[4 * r + 2] || fl[4 * r + 3]) return fail(ctx, MSC_E_CAPACITY, "neighbour list region overflow");
     if (fl[4 * r + 0]) return fail(ctx, fl[4 * r] == 1 ? MSC_E_OVERLAP : MSC_E_DISTANCE, "initial overlap");
   }
   return upload_reps(ctx);
@@ -1144,6 +1180,40 @@ static void enqueue_step(msc_ctx *ctx, int step, const double *d_gs_step, double
   lists_build(ctx, ctx->d_rebuild);
 }
 
+// Device copy of everything a step carries into the next one (positions,
+// momenta, orientations, forces, list snapshot, thermostats, energies), for
+// rolling back a chunk of steps.
+static int state_copy(msc_ctx *ctx, bool restore) {
+  const size_t N = ctx->Ntot, nrep = ctx->nrep;
+  struct Item { void *p; size_t bytes; } items[] = {
+      {ctx->d_x, 3 * N * 8}, {ctx->d_p, 3 * N * 8}, {ctx->d_l, 3 * N * 8}, {ctx->d_ori, 9 * N * 8},
+      {ctx->d_F, 3 * N * 8}, {ctx->d_T, 3 * N * 8}, {ctx->d_snap, 3 * N * 8},
+      {ctx->d_th, nrep * sizeof(Thermo)}, {ctx->d_E, nrep * sizeof(RepEnergy)}, {ctx->d_Kin, 2 * nrep * 8}};
+  size_t total = 0;
+  for (auto &it : items) total += (it.bytes + 255) / 256 * 256;
+  if (total > ctx->save_cap) {
+    if (int rc_ = dalloc(ctx, &ctx->d_save, total)) return rc_;
+    ctx->save_cap = total;
+  }
+  size_t off = 0;
+  for (auto &it : items) {
+    char *b = ctx->d_save + off;
+    if (restore)
+      HIPC(hipMemcpyAsync(it.p, b, it.bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    else
+      HIPC(hipMemcpyAsync(b, it.p, it.bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    off += (it.bytes + 255) / 256 * 256;
+  }
+  return MSC_OK;
+}
+
+// Steps run in chunks of kRunChunk.  A chunk in which a list rebuild found
+// its region too small is rolled back, the regions are enlarged to what the
+// rebuild needed, every list is rebuilt from the chunk's first state, and the
+// chunk is run again (the redone steps agree with an uninterrupted run up to
+// the summation order of the rebuilt lists).
+static constexpr int kRunChunk = 50;
+
 int msc_run(msc_ctx *ctx, int step0, int nsteps, double *obs) {
   if (!ctx || nsteps < 0) return MSC_E_ARG;
   if (!ctx->have_lists) return fail(ctx, MSC_E_STATE, "call msc_init_run first");
@@ -1173,18 +1243,40 @@ int msc_run(msc_ctx *ctx, int step0, int nsteps, double *obs) {
   double *d_gs_all = ctx->d_run_gs, *d_obs_all = ctx->d_run_obs;
   if (!gs.empty())
     if (int rc_ = h2d(ctx, d_gs_all, gs.data(), sizeof(double) * gs.size())) return rc_;
-  HIPC(hipMemsetAsync(ctx->d_flags, 0, sizeof(int) * 4 * nrep, ctx->stream));
-  for (int k = 0; k < nsteps; ++k)
-    enqueue_step(ctx, step0 + k, d_gs_all + (size_t)k * nrep * 4, d_obs_all + (size_t)k * nrep * MSC_OBS_N);
   std::vector<int> fl(4 * nrep);
-  if (int rc_ = d2h(ctx, fl.data(), ctx->d_flags, sizeof(int) * fl.size())) return rc_;
+  int retries = 0;
+  for (int k0 = 0; k0 < nsteps;) {
+    const int nk = std::min(kRunChunk, nsteps - k0);
+    if (int rc_ = state_copy(ctx, false)) return rc_;
+    launch_fill_int(ctx->d_flags, 4 * nrep, 0, ctx->stream);
+    for (int k = k0; k < k0 + nk; ++k)
+      enqueue_step(ctx, step0 + k, d_gs_all + (size_t)k * nrep * 4, d_obs_all + (size_t)k * nrep * MSC_OBS_N);
+    LAUNCHED("msc_run");
+    if (int rc_ = d2h(ctx, fl.data(), ctx->d_flags, sizeof(int) * fl.size())) return rc_;
+    bool overflow = false;
+    for (int r = 0; r < nrep; ++r) {
+      if (fl[4 * r + 0]) return fail(ctx, fl[4 * r] == 1 ? MSC_E_OVERLAP : MSC_E_DISTANCE, "overlap during the run");
+      overflow |= fl[4 * r + 2] != 0 || fl[4 * r + 3] != 0;
+    }
+    if (overflow) {
+      if (++retries > 4) return fail(ctx, MSC_E_CAPACITY, "neighbour list regions keep overflowing");
+      ++ctx->chunk_repeats;
+      std::vector<long> need_nl(nrep), need_dl(nrep);
+      for (int r = 0; r < nrep; ++r) {
+        need_nl[r] = fl[4 * r + 2];
+        need_dl[r] = fl[4 * r + 3];
+      }
+      if (int rc_ = state_copy(ctx, true)) return rc_;
+      if (int rc_ = lists_layout(ctx, need_nl, need_dl)) return rc_;
+      launch_fill_int(ctx->d_flags, 4 * nrep, 0, ctx->stream);
+      if (int rc_ = lists_build(ctx, ctx->d_one)) return rc_;
+      continue;
+    }
+    k0 += nk;
+  }
   if (obs && nobs)
     if (int rc_ = d2h(ctx, obs, d_obs_all, sizeof(double) * nobs)) return rc_;
   if (ctx->timing) settle_timers(ctx);
-  for (int r = 0; r < nrep; ++r) {
-    if (fl[4 * r + 2]) return fail(ctx, MSC_E_CAPACITY, "neighbour list region overflow during the run");
-    if (fl[4 * r + 0]) return fail(ctx, fl[4 * r] == 1 ? MSC_E_OVERLAP : MSC_E_DISTANCE, "overlap during the run");
-  }
   return MSC_OK;
 }
 
@@ -1225,6 +1317,12 @@ int msc_work_counts(msc_ctx *ctx, int rep, long counts[4]) {
   long t = 0;
   for (int v : cnt) t += v;
   counts[3] = t / 2;
+  return MSC_OK;
+}
+
+int msc_chunk_repeats(msc_ctx *ctx, long *n) {
+  if (!ctx || !n) return MSC_E_ARG;
+  *n = ctx->chunk_repeats;
   return MSC_OK;
 }
 

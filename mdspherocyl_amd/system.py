@@ -200,6 +200,11 @@ class MdSystem:
         self._chk(self.L.msc_kernel_time(self._ctx, name.encode(), ctypes.byref(ms), ctypes.byref(n)))
         return ms.value, n.value
 
+    def chunk_repeats(self):
+        n = ctypes.c_long()
+        self._chk(self.L.msc_chunk_repeats(self._ctx, ctypes.byref(n)))
+        return n.value
+
     def work_counts(self, rep):
         c = (ctypes.c_long * 4)()
         self._chk(self.L.msc_work_counts(self._ctx, rep, c))

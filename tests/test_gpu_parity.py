@@ -323,3 +323,32 @@ def test_nve_energy_drift(msc):
         drift = float(np.max(np.abs(E - E[0])))
         _say("  max |E - E0| per particle = %.3e" % drift)
         assert drift < 4e-3
+
+
+def test_list_regions_grow_during_run(msc, monkeypatch):
+    """A replica that starts as an aligned lattice at packing fraction 0.3
+    melts and its neighbour lists grow.  With regions sized to the initial
+    count only (MSC_TIGHT_LISTS=1) a growing rebuild overflows: msc_run rolls
+    the chunk back, enlarges the regions and repeats it.  The observables
+    equal those of a run with roomy regions to summation-order rounding."""
+    from mdspherocyl_amd import configs
+    rng = np.random.default_rng(21)
+    N = 400
+    d, st = configs.make_replica(rng, N, configs.truncated_normal(rng, N, 3.0, 0.3), 0.30, 1.5, 5,
+                                 lambda b: 3.0 / 3.5, 3.5)
+    runs, repeats = [], []
+    for tight in ("1", "0"):
+        monkeypatch.setenv("MSC_TIGHT_LISTS", tight)
+        _say("case: list regions, N=%d aligned start, tight=%s, 400 steps fp64" % (N, tight))
+        sysm = msc.MdSystem([d], precision="fp64")
+        try:
+            configs.load_into(sysm, [(d, st)])
+            sysm.init_run()
+            runs.append(sysm.run(400))
+            repeats.append(sysm.chunk_repeats())
+        finally:
+            sysm.close()
+    _say("  chunks repeated: tight %d, roomy %d" % tuple(repeats))
+    assert repeats[0] >= 1 and repeats[1] == 0
+    a, b = runs
+    assert np.allclose(a[:, 0, :11], b[:, 0, :11], rtol=1e-8, atol=1e-10)
