@@ -221,13 +221,14 @@ def main():
         bat.init_run()
         kb = max(args.steps // 2, 10)
         t_b, _ = timed_run(bat, kb, min(args.warmup, 10), False)
+        repeats = bat.chunk_repeats()
         bat.close()
         sps_b = kb / t_b
         pps_b = sps_b * ntot
         pps_single = sps_mix * desc.N
         batched = {"replicas": len(reps), "sum_N": ntot, "steps": kb, "steps_per_s": round(sps_b, 3),
                    "replica_steps_per_s": round(sps_b * len(reps), 2), "particle_steps_per_s": round(pps_b, 1),
-                   "vs_single_particle_steps": round(pps_b / pps_single, 3)}
+                   "vs_single_particle_steps": round(pps_b / pps_single, 3), "chunk_repeats": repeats}
         log("bench: batched %.2f steps/s = %.1f replica-steps/s; particle-steps/s %.3e = %.2f x single system"
             % (sps_b, sps_b * len(reps), pps_b, pps_b / pps_single))
 

@@ -1253,11 +1253,13 @@ int msc_run(msc_ctx *ctx, int step0, int nsteps, double *obs) {
       enqueue_step(ctx, step0 + k, d_gs_all + (size_t)k * nrep * 4, d_obs_all + (size_t)k * nrep * MSC_OBS_N);
     LAUNCHED("msc_run");
     if (int rc_ = d2h(ctx, fl.data(), ctx->d_flags, sizeof(int) * fl.size())) return rc_;
+    // an overflowing rebuild empties the replica's rows, so an overlap later in
+    // the same chunk may be its consequence: overflow is handled first
     bool overflow = false;
-    for (int r = 0; r < nrep; ++r) {
-      if (fl[4 * r + 0]) return fail(ctx, fl[4 * r] == 1 ? MSC_E_OVERLAP : MSC_E_DISTANCE, "overlap during the run");
-      overflow |= fl[4 * r + 2] != 0 || fl[4 * r + 3] != 0;
-    }
+    for (int r = 0; r < nrep; ++r) overflow |= fl[4 * r + 2] != 0 || fl[4 * r + 3] != 0;
+    if (!overflow)
+      for (int r = 0; r < nrep; ++r)
+        if (fl[4 * r + 0]) return fail(ctx, fl[4 * r] == 1 ? MSC_E_OVERLAP : MSC_E_DISTANCE, "overlap during the run");
     if (overflow) {
       if (++retries > 4) return fail(ctx, MSC_E_CAPACITY, "neighbour list regions keep overflowing");
       ++ctx->chunk_repeats;
