@@ -1261,12 +1261,22 @@ int msc_run(msc_ctx *ctx, int step0, int nsteps, double *obs) {
       for (int r = 0; r < nrep; ++r)
         if (fl[4 * r + 0]) return fail(ctx, fl[4 * r] == 1 ? MSC_E_OVERLAP : MSC_E_DISTANCE, "overlap during the run");
     if (overflow) {
-      if (++retries > 4) return fail(ctx, MSC_E_CAPACITY, "neighbour list regions keep overflowing");
+      if (++retries > 4) {
+        std::string m = "neighbour list regions keep overflowing:";
+        for (int r = 0; r < nrep; ++r)
+          if (fl[4 * r + 2] || fl[4 * r + 3])
+            m += " replica " + std::to_string(r) + " needs " + std::to_string(fl[4 * r + 2]) + "/" +
+                 std::to_string(fl[4 * r + 3]) + " of " + std::to_string(ctx->reps[r].nl_cap) + "/" +
+                 std::to_string(ctx->reps[r].dl_cap) + ";";
+        return fail(ctx, MSC_E_CAPACITY, m);
+      }
       ++ctx->chunk_repeats;
+      // a region that overflowed grows to at least 1.5 times its old size
       std::vector<long> need_nl(nrep), need_dl(nrep);
       for (int r = 0; r < nrep; ++r) {
-        need_nl[r] = fl[4 * r + 2];
-        need_dl[r] = fl[4 * r + 3];
+        const RepDev &R = ctx->reps[r];
+        need_nl[r] = fl[4 * r + 2] ? std::max<long>(fl[4 * r + 2], R.nl_cap + R.nl_cap / 2) : 0;
+        need_dl[r] = fl[4 * r + 3] ? std::max<long>(fl[4 * r + 3], R.dl_cap + R.dl_cap / 2) : 0;
       }
       if (int rc_ = state_copy(ctx, true)) return rc_;
       if (int rc_ = lists_layout(ctx, need_nl, need_dl)) return rc_;

@@ -144,7 +144,7 @@ __global__ void k_cells_sort(State s, const int *__restrict__ cell_rep, const in
 // cell starts (base = the replica's first particle); mode 1 / 2: WCA /
 // real-space row counts -> row starts in the replica's list region.  A
 // replica whose rebuild flag is clear is left untouched.  Rows that would
-// overflow the region set flags[rep*4+2] (WCA list) or flags[rep*4+3]
+// overflow the region raise flags[rep*4+2] (WCA list) or flags[rep*4+3]
 // (real-space list) to the entries needed and are emptied.
 // The mode is a template parameter: a three-way branch on a uniform
 // kernel argument here was compiled for gfx950 into code that, for mode 2,
@@ -177,7 +177,7 @@ __global__ void __launch_bounds__(1024) k_scan_seg(int *__restrict__ cnt, int *_
   }
   const int total = sh[1023];
   if (total > cap) {
-    if (t == 0) flags[rp * 4 + (MODE == 2 ? 3 : 2)] = total;
+    if (t == 0) atomicMax(&flags[rp * 4 + (MODE == 2 ? 3 : 2)], total);
     for (int i = b; i < e; ++i) { cnt[b0 + i] = 0; out[b0 + i] = base; }
     return;
   }
