@@ -518,7 +518,7 @@ int msc_create(int device, int precision, int nrep, const msc_replica_desc *reps
   A(&ctx->d_kval, ctx->nslot_tot);
   A(&ctx->d_slot_out, (size_t)ctx->nslot_tot * 10);
   A(&ctx->d_task_out, ctx->tasks.size() * 16);
-  A(&ctx->d_red, (size_t)nrep * 16);
+  A(&ctx->d_red, (size_t)nrep * kRedStride);
   A(&ctx->d_M2, (size_t)nrep * 6);
   A(&ctx->d_Kin, (size_t)nrep * 2);
   A(&ctx->d_mom, (size_t)nrep * 8);
@@ -1046,9 +1046,9 @@ int msc_self_ener(msc_ctx *ctx, int rep, double *U) {
   if (!ctx || rep < 0 || rep >= ctx->nrep || !U) return MSC_E_ARG;
   State s = sel(ctx, rep);
   launch_self_ener(s, ctx->d_task_out, ctx->stream);
-  launch_reduce(s, ctx->d_task_out, 16, 1, 0, ctx->d_red, 16, 15, ctx->stream);
+  launch_reduce(s, ctx->d_task_out, 16, 1, 0, ctx->d_red, kRedStride, 15, ctx->stream);
   double v[16];
-  if (int rc_ = d2h(ctx, v, ctx->d_red + 16 * rep, sizeof v)) return rc_;
+  if (int rc_ = d2h(ctx, v, ctx->d_red + kRedStride * rep, sizeof v)) return rc_;
   *U = v[15];
   return MSC_OK;
 }
@@ -1083,9 +1083,9 @@ int msc_moveP(msc_ctx *ctx, int rep, double dt, int last, double K[2], double PT
   if (rc) return rc;
   State s = sel(ctx, rep);
   launch_moveP(s, last != 0, ctx->d_th, ctx->d_task_out, ctx->stream);
-  launch_reduce(s, ctx->d_task_out, 16, 8, 0, ctx->d_red, 16, 0, ctx->stream);
+  launch_reduce(s, ctx->d_task_out, 16, 8, 0, ctx->d_red, kRedStride, 0, ctx->stream);
   double v[16];
-  if (int rc_ = d2h(ctx, v, ctx->d_red + 16 * rep, sizeof v)) return rc_;
+  if (int rc_ = d2h(ctx, v, ctx->d_red + kRedStride * rep, sizeof v)) return rc_;
   if (K) { K[0] = v[0] / 2; K[1] = v[1] / 2; }
   if (PT) {
     double P[9] = {v[2], v[3], v[4], v[3], v[5], v[6], v[4], v[6], v[7]};
@@ -1126,13 +1126,13 @@ int msc_init_run(msc_ctx *ctx) {
   forces_all(ctx);
   State s = sel(ctx, MSC_ALL);
   launch_self_ener(s, ctx->d_task_out, ctx->stream);
-  launch_reduce(s, ctx->d_task_out, 16, 1, 0, ctx->d_red, 16, 15, ctx->stream);
-  std::vector<double> red(16 * ctx->nrep);
+  launch_reduce(s, ctx->d_task_out, 16, 1, 0, ctx->d_red, kRedStride, 15, ctx->stream);
+  std::vector<double> red(kRedStride * ctx->nrep);
   std::vector<int> fl(4 * ctx->nrep);
   if (int rc_ = d2h(ctx, red.data(), ctx->d_red, sizeof(double) * red.size())) return rc_;
   if (int rc_ = d2h(ctx, fl.data(), ctx->d_flags, sizeof(int) * fl.size())) return rc_;
   for (int r = 0; r < ctx->nrep; ++r) {
-    ctx->reps[r].Uself = red[16 * r + 15] / ctx->reps[r].N;
+    ctx->reps[r].Uself = red[kRedStride * r + 15] / ctx->reps[r].N;
     if (fl[4 * r + 2] || fl[4 * r + 3]) return fail(ctx, MSC_E_CAPACITY, "neighbour list region overflow");
     if (fl[4 * r + 0]) return fail(ctx, fl[4 * r] == 1 ? MSC_E_OVERLAP : MSC_E_DISTANCE, "initial overlap");
   }
@@ -1155,7 +1155,7 @@ static void enqueue_step(msc_ctx *ctx, int step, const double *d_gs_step, double
     Bracket b(ctx, "integrate");
     launch_moveP(s, false, ctx->d_th, ctx->d_task_out, st);
   }
-  launch_reduce(s, ctx->d_task_out, 16, 8, 0, ctx->d_red, 16, 0, st);
+  launch_reduce(s, ctx->d_task_out, 16, 8, 0, ctx->d_red, kRedStride, 0, st);
   launch_kin_finish(s, ctx->d_red, ctx->d_Kin, st);
   launch_thermo(s, ctx->d_th, ctx->d_Kin, 1, d_gs_step, st);
   {
@@ -1168,7 +1168,7 @@ static void enqueue_step(msc_ctx *ctx, int step, const double *d_gs_step, double
     Bracket b(ctx, "integrate");
     launch_moveP(s, true, ctx->d_th, ctx->d_task_out, st);
   }
-  launch_reduce(s, ctx->d_task_out, 16, 8, 0, ctx->d_red, 16, 0, st);
+  launch_reduce(s, ctx->d_task_out, 16, 8, 0, ctx->d_red, kRedStride, 0, st);
   launch_kin_finish(s, ctx->d_red, ctx->d_Kin, st);
   launch_thermo(s, ctx->d_th, ctx->d_Kin, 2, d_gs_step, st);
   launch_momenta(s, ctx->d_task_out, st);
@@ -1257,9 +1257,17 @@ int msc_run(msc_ctx *ctx, int step0, int nsteps, double *obs) {
     // the same chunk may be its consequence: overflow is handled first
     bool overflow = false;
     for (int r = 0; r < nrep; ++r) overflow |= fl[4 * r + 2] != 0 || fl[4 * r + 3] != 0;
-    if (!overflow)
+    if (!overflow) {
+      std::string m;
+      int code = 0;
       for (int r = 0; r < nrep; ++r)
-        if (fl[4 * r + 0]) return fail(ctx, fl[4 * r] == 1 ? MSC_E_OVERLAP : MSC_E_DISTANCE, "overlap during the run");
+        if (fl[4 * r + 0]) {
+          if (!code) code = fl[4 * r] == 1 ? MSC_E_OVERLAP : MSC_E_DISTANCE;
+          m += " replica " + std::to_string(r) + (fl[4 * r] == 1 ? " (overlap)" : " (distance)");
+        }
+      if (code) return fail(ctx, code, "force_sc abort during steps " + std::to_string(step0 + k0) + ".." +
+                                           std::to_string(step0 + k0 + nk - 1) + ":" + m);
+    }
     if (overflow) {
       if (++retries > 4) {
         std::string m = "neighbour list regions keep overflowing:";

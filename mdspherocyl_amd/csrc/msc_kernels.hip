@@ -1125,11 +1125,12 @@ __global__ void k_thermo(State s, Thermo *__restrict__ th, const double *__restr
 }
 
 // kinetic energies K[2] = sums / 2 per replica from the reduced moveP sums
+// (kRedStride doubles per replica)
 __global__ void k_kin_finish(State s, const double *__restrict__ red, double *__restrict__ Kin) {
   int rp = blockIdx.x * blockDim.x + threadIdx.x;
   if (rp >= s.nrep) return;
-  Kin[rp * 2 + 0] = red[rp * 8 + 0] / 2;
-  Kin[rp * 2 + 1] = red[rp * 8 + 1] / 2;
+  Kin[rp * 2 + 0] = red[rp * kRedStride + 0] / 2;
+  Kin[rp * 2 + 1] = red[rp * kRedStride + 1] / 2;
 }
 
 // observables of print_out (v25/main.cpp:626-647, v25/functions.cpp:2843-2850)

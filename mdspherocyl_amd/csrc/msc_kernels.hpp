@@ -65,5 +65,6 @@ void launch_zero_FT(const State &s, hipStream_t st);
 void launch_fill_int(int *p, int n, int v, hipStream_t st);
 
 constexpr int kChunk = 64;  // particles per reciprocal structure-factor chunk
+constexpr int kRedStride = 16;  // doubles per replica in the reduction scratch d_red
 
 }  // namespace msc

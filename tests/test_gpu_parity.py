@@ -278,7 +278,8 @@ def test_batched_replicas_match_oracle(msc, precision):
     """Three replicas of different size and real-space mode in one context:
     each replica's total force and torque equal the oracle's for that replica
     alone (FP64: 1e-10 of the RMS per-particle value; mixed: relative L2
-    error 1e-5)."""
+    error 1e-5), and 60 steps of the batch reproduce each replica's run in a
+    context of its own (1e-12 relative)."""
     from mdspherocyl_amd import configs
     rng = np.random.default_rng(11)
     reps = [configs.make_replica(rng, n, configs.truncated_normal(rng, n, 3.0, 0.3), 0.1, 1.5, 6,
@@ -302,10 +303,19 @@ def test_batched_replicas_match_oracle(msc, precision):
             else:
                 assert H.rel_err(F, F1 + F2 + F3) < 1e-5, "replica %d" % k
                 assert H.rel_err(T, T1 + T2 + T3) < 1e-5, "replica %d" % k
-        obs = big.run(20)
-        assert np.all(np.isfinite(obs))
+        obs = big.run(60)
     finally:
         big.close()
+    # the step loop of every replica in the batch equals its run alone
+    for k, (d, st) in enumerate(reps):
+        one = msc.MdSystem([d], precision=precision)
+        try:
+            configs.load_into(one, [(d, st)])
+            one.init_run()
+            alone = one.run(60)
+        finally:
+            one.close()
+        assert np.allclose(obs[:, k, :11], alone[:, 0, :11], rtol=1e-12, atol=1e-14), "replica %d" % k
 
 
 def test_nve_energy_drift(msc):
