@@ -88,7 +88,7 @@ struct msc_ctx {
   std::string err;
   State S{};
   bool have_state = false, have_lists = false, graph_on = false, timing = false;
-  long nl_cap = 0, dl_cap = 0;
+  long nl_alloc = 0, sl_alloc = 0, dl_alloc = 0;  // entries allocated for the row arrays
   size_t part_elems = 0;
   int nslot_tot = 0, maxslot = 0, maxcol = 0;
   // device buffers
@@ -97,11 +97,14 @@ struct msc_ctx {
   int *d_rep_of = nullptr, *d_cell_rep = nullptr;
   double *d_x = nullptr, *d_p = nullptr, *d_l = nullptr, *d_ori = nullptr, *d_F = nullptr, *d_T = nullptr;
   double *d_len = nullptr, *d_dia = nullptr, *d_mass = nullptr, *d_inert = nullptr;
-  double *d_dpc = nullptr, *d_doc = nullptr, *d_dmu = nullptr, *d_dpos = nullptr, *d_dori = nullptr, *d_snap = nullptr;
+  double *d_dpc = nullptr, *d_doc = nullptr, *d_dmu = nullptr, *d_dpos = nullptr, *d_dori = nullptr, *d_snap = nullptr,
+         *d_snap_e = nullptr;
   int *d_cell_of = nullptr, *d_cell_cnt = nullptr, *d_cell_start = nullptr, *d_cell_fill = nullptr,
       *d_cell_list = nullptr;
-  int *d_nl_cnt = nullptr, *d_nl_start = nullptr, *d_nl_idx = nullptr;
-  int *d_dl_cnt = nullptr, *d_dl_start = nullptr, *d_dl_idx = nullptr;
+  int *d_cell_tmp = nullptr;
+  int *d_nl_cnt = nullptr, *d_nl_idx = nullptr;
+  int *d_sl_cnt = nullptr, *d_sl_idx = nullptr;
+  int *d_dl_cnt = nullptr, *d_dl_idx = nullptr;
   int ncells = 0;
   double *d_kval = nullptr;
   void *d_tab = nullptr, *d_ph = nullptr, *d_amp = nullptr, *d_part = nullptr, *d_A = nullptr;
@@ -109,7 +112,7 @@ struct msc_ctx {
          *d_mom = nullptr, *d_gs = nullptr, *d_obs = nullptr, *d_counts = nullptr;
   RepEnergy *d_E = nullptr;
   Thermo *d_th = nullptr;
-  int *d_flags = nullptr, *d_rebuild = nullptr, *d_one = nullptr;
+  int *d_flags = nullptr, *d_rebuild = nullptr, *d_srebuild = nullptr, *d_one = nullptr;
   std::map<std::string, TimerSet> timers;
   std::vector<hipEvent_t> event_pool;
   double *d_run_gs = nullptr, *d_run_obs = nullptr;
@@ -286,10 +289,11 @@ bool rep_ok(msc_ctx *ctx, int rep) { return rep == MSC_ALL || (rep >= 0 && rep <
 void derive(msc_ctx *ctx, int rp, const double *len, const double *dia, const double *dpc) {
   RepDev &R = ctx->reps[rp];
   const msc_replica_desc &D = ctx->desc[rp];
-  double dmax = 0, dmin = 1.e6, lmax = 0, lev = 0;
+  double dmax = 0, dmin = 1.e6, lmax = 0, lev = 0, diamax = 0;
   for (int i = 0; i < R.N; ++i) {
     dmax = std::max(dmax, len[i] + dia[i]);
     dmin = std::min(dmin, dia[i]);
+    diamax = std::max(diamax, dia[i]);
     lmax = std::max(lmax, len[i]);
     for (int w = 0; w < R.nd; ++w) lev = std::max(lev, std::fabs(dpc[i * R.nd + w]));
   }
@@ -301,6 +305,10 @@ void derive(msc_ctx *ctx, int rp, const double *len, const double *dia, const do
   if (R.r_list_min < 1.5 * dmax) R.r_list_min = 1.5 * dmax;
   R.r_list_max = R.r_list_min + 0.9 * dmin;
   R.rl_dip = D.rc > 0 ? D.rc + (R.r_list_max - R.r_list_min) : 0;
+  // contact rows: every pair inside the WCA range 2^(1/6) sigma_ij plus a
+  // skin of half the smallest diameter (never beyond the reference list)
+  R.skin_short = 0.5 * dmin;
+  R.r_short = std::min(std::pow(2.0, 1.0 / 6.0) * diamax + R.skin_short, R.r_list_max);
   ctx->S.lever_max = std::max(ctx->S.lever_max, lev);
 }
 
@@ -340,13 +348,16 @@ void refresh_state_ptrs(msc_ctx *ctx) {
   s.len = ctx->d_len; s.dia = ctx->d_dia; s.mass = ctx->d_mass; s.inert = ctx->d_inert;
   s.dpc = ctx->d_dpc; s.doc = ctx->d_doc; s.dmu = ctx->d_dmu; s.dpos = ctx->d_dpos; s.dori = ctx->d_dori;
   s.snap = ctx->d_snap;
+  s.snap_e = ctx->d_snap_e;
   s.cell_of = ctx->d_cell_of; s.cell_cnt = ctx->d_cell_cnt; s.cell_start = ctx->d_cell_start;
-  s.cell_fill = ctx->d_cell_fill; s.cell_list = ctx->d_cell_list; s.cell_rep = ctx->d_cell_rep;
-  s.nl_cnt = ctx->d_nl_cnt; s.nl_start = ctx->d_nl_start; s.nl_idx = ctx->d_nl_idx;
+  s.cell_fill = ctx->d_cell_fill; s.cell_tmp = ctx->d_cell_tmp; s.cell_list = ctx->d_cell_list;
+  s.cell_rep = ctx->d_cell_rep;
+  s.nl_cnt = ctx->d_nl_cnt; s.nl_idx = ctx->d_nl_idx;
+  s.sl_cnt = ctx->d_sl_cnt; s.sl_idx = ctx->d_sl_idx;
   bool anyrc = false;
   for (auto &R : ctx->reps) anyrc |= R.rc > 0;
-  s.dl_cnt = anyrc ? ctx->d_dl_cnt : nullptr;
-  s.dl_start = ctx->d_dl_start;
+  (void)anyrc;
+  s.dl_cnt = ctx->d_dl_cnt;
   s.dl_idx = ctx->d_dl_idx;
   s.kval = ctx->d_kval;
 }
@@ -512,9 +523,10 @@ int msc_create(int device, int precision, int nrep, const msc_replica_desc *reps
   A(&ctx->d_len, N); A(&ctx->d_dia, N); A(&ctx->d_mass, N); A(&ctx->d_inert, 3 * N);
   A(&ctx->d_dpc, nd * N); A(&ctx->d_doc, 3 * nd * N); A(&ctx->d_dmu, nd * N);
   A(&ctx->d_dpos, 3 * nd * N); A(&ctx->d_dori, 3 * nd * N); A(&ctx->d_snap, 3 * N);
+  A(&ctx->d_snap_e, 6 * N);
   A(&ctx->d_cell_of, N); A(&ctx->d_cell_list, N);
-  A(&ctx->d_nl_cnt, N); A(&ctx->d_nl_start, N);
-  A(&ctx->d_dl_cnt, N); A(&ctx->d_dl_start, N);
+  A(&ctx->d_cell_tmp, N);
+  A(&ctx->d_nl_cnt, N); A(&ctx->d_sl_cnt, N); A(&ctx->d_dl_cnt, N);
   A(&ctx->d_kval, ctx->nslot_tot);
   A(&ctx->d_slot_out, (size_t)ctx->nslot_tot * 10);
   A(&ctx->d_task_out, ctx->tasks.size() * 16);
@@ -529,6 +541,7 @@ int msc_create(int device, int precision, int nrep, const msc_replica_desc *reps
   A(&ctx->d_th, nrep);
   A(&ctx->d_flags, (size_t)nrep * 4);
   A(&ctx->d_rebuild, nrep);
+  A(&ctx->d_srebuild, nrep);
   A(&ctx->d_one, nrep);
   if (rc == MSC_OK) {
     unsigned char **raw[] = {(unsigned char **)&ctx->d_tab, (unsigned char **)&ctx->d_ph,
@@ -591,7 +604,8 @@ void msc_destroy(msc_ctx *ctx) {
                   ctx->d_l, ctx->d_ori, ctx->d_F, ctx->d_T, ctx->d_len, ctx->d_dia, ctx->d_mass, ctx->d_inert,
                   ctx->d_dpc, ctx->d_doc, ctx->d_dmu, ctx->d_dpos, ctx->d_dori, ctx->d_snap, ctx->d_cell_of,
                   ctx->d_cell_cnt, ctx->d_cell_start, ctx->d_cell_fill, ctx->d_cell_list, ctx->d_nl_cnt,
-                  ctx->d_nl_start, ctx->d_nl_idx, ctx->d_dl_cnt, ctx->d_dl_start, ctx->d_dl_idx, ctx->d_kval,
+                  ctx->d_nl_idx, ctx->d_dl_cnt, ctx->d_dl_idx, ctx->d_kval, ctx->d_cell_tmp, ctx->d_sl_cnt,
+                  ctx->d_sl_idx, ctx->d_snap_e, ctx->d_srebuild,
                   ctx->d_tab, ctx->d_ph, ctx->d_amp, ctx->d_part, ctx->d_A, ctx->d_slot_out, ctx->d_task_out,
                   ctx->d_red, ctx->d_M2, ctx->d_Kin, ctx->d_mom, ctx->d_gs, ctx->d_obs, ctx->d_counts, ctx->d_E,
                   ctx->d_th, ctx->d_flags, ctx->d_rebuild, ctx->d_one, ctx->d_run_gs, ctx->d_run_obs, ctx->d_save};
@@ -728,14 +742,12 @@ static int lists_build(msc_ctx *ctx, const int *d_flagmask) {
   State s = sel(ctx, MSC_ALL);
   {
     Bracket b(ctx, "nlist");
-    launch_cells(s, d_flagmask, ctx->d_flags, ctx->stream);
-    launch_lists_count(s, d_flagmask, ctx->d_flags, ctx->stream);
-    launch_lists_fill(s, d_flagmask, ctx->stream);
+    launch_cells(s, d_flagmask, ctx->stream);
+    launch_lists(s, d_flagmask, ctx->d_flags, ctx->stream);
   }
   return MSC_OK;
 }
 
-// (re)allocate cells and list regions for every replica from a counting pass
 // Expected full-list entries per particle of a fluid at the replica's number
 // density: rho times the volume within segment distance r of a rod, averaged
 // over random relative orientations, 4/3 pi r^3 + pi r^2 (L_i + L_j) +
@@ -745,17 +757,21 @@ static double fluid_neighbours(const RepDev &R, double r) {
   return rho * (4.0 / 3.0 * M_PI * r * r * r + M_PI * r * r * 2 * L + 0.5 * M_PI * r * L * L);
 }
 
-// MSC_TIGHT_LISTS=1 in the environment sizes list regions to the present
-// count only, so that any growth takes the overflow path of msc_run.
+// MSC_TIGHT_LISTS=1 in the environment sizes rows to the present longest row
+// only, so that any growth takes the overflow path of msc_run.
 static bool tight_lists() {
   const char *v = std::getenv("MSC_TIGHT_LISTS");
   return v && std::atoi(v) != 0;
 }
 
-// need_nl / need_dl: per-replica entries that a later rebuild found missing
-// (empty on a first layout)
+// Cells and neighbour-row capacities of every replica.  A measuring pass
+// builds the rows with capacity 0, which records the longest row of each
+// list per replica; the capacity is that length with headroom (and at least
+// 1.5 times the mean row of a fluid at the replica's density), or what a
+// rebuild of a running system found missing (need_*: longest rows seen by
+// an overflowing rebuild, 0 where nothing overflowed).
 static int lists_layout(msc_ctx *ctx, const std::vector<long> &need_nl = {},
-                        const std::vector<long> &need_dl = {}) {
+                        const std::vector<long> &need_sl = {}, const std::vector<long> &need_dl = {}) {
   const int nrep = ctx->nrep;
   int ncells = 0;
   std::vector<int> cell_rep;
@@ -773,60 +789,56 @@ static int lists_layout(msc_ctx *ctx, const std::vector<long> &need_nl = {},
     ctx->ncells = ncells;
   }
   if (int rc_ = h2d(ctx, ctx->d_cell_rep, cell_rep.data(), sizeof(int) * ncells)) return rc_;
-  // counting pass with unbounded regions, then size the regions
+  // measuring pass
   for (int r = 0; r < nrep; ++r) {
-    ctx->reps[r].nl_base = 0;
-    ctx->reps[r].nl_cap = 0x7fffffff;
-    ctx->reps[r].dl_base = 0;
-    ctx->reps[r].dl_cap = 0x7fffffff;
+    RepDev &R = ctx->reps[r];
+    R.nl_w = R.sl_w = R.dl_w = 0;
+    R.nl_base = R.sl_base = R.dl_base = 0;
   }
   int rc;
   if ((rc = upload_reps(ctx))) return rc;
   refresh_state_ptrs(ctx);
   State s = sel(ctx, MSC_ALL);
-  launch_cells(s, ctx->d_one, ctx->d_flags, ctx->stream);
-  launch_lists_count(s, ctx->d_one, ctx->d_flags, ctx->stream);
-  std::vector<int> cnt(ctx->Ntot), dcnt(ctx->Ntot);
-  if (int rc_ = d2h(ctx, cnt.data(), ctx->d_nl_cnt, sizeof(int) * ctx->Ntot)) return rc_;
-  if (int rc_ = d2h(ctx, dcnt.data(), ctx->d_dl_cnt, sizeof(int) * ctx->Ntot)) return rc_;
-  long nl_tot = 0, dl_tot = 0;
+  launch_fill_int(ctx->d_flags, 4 * nrep, 0, ctx->stream);
+  launch_cells(s, ctx->d_one, ctx->stream);
+  launch_lists(s, ctx->d_one, ctx->d_flags, ctx->stream);
+  std::vector<int> fl(4 * nrep);
+  if (int rc_ = d2h(ctx, fl.data(), ctx->d_flags, sizeof(int) * fl.size())) return rc_;
+  const bool tight = tight_lists();
+  long nl_tot = 0, sl_tot = 0, dl_tot = 0;
   for (int r = 0; r < nrep; ++r) {
     RepDev &R = ctx->reps[r];
-    long a = 0, b = 0;
-    for (int i = R.n0; i < R.n0 + R.N; ++i) {
-      a += cnt[i];
-      if (R.rc > 0) b += dcnt[i];
-    }
-    // headroom for the growth of the lists between layouts: half again the
-    // present count, at least 1.25 times the count of a fluid at this
-    // density, and 1.25 times what a rebuild found missing
-    long ca = std::max(a + a / 2, (long)(1.25 * R.N * fluid_neighbours(R, R.r_list_max))) + 64 * 16;
-    long cb = 0;
+    long mw = fl[4 * r + 2], ms = fl[4 * r + 1], md = fl[4 * r + 3];
+    auto width = [&](long longest, double fluid, long need) {
+      long w = tight ? longest : std::max(longest + longest / 4 + 8, (long)std::ceil(1.5 * fluid));
+      if (need > 0) w = std::max(w, need + need / 4 + 8);
+      return std::max(w, 1L);
+    };
+    R.nl_w = (int)width(mw, fluid_neighbours(R, R.r_list_max), need_nl.empty() ? 0 : need_nl[r]);
+    R.sl_w = (int)width(ms, fluid_neighbours(R, R.r_short), need_sl.empty() ? 0 : need_sl[r]);
+    R.dl_w = 0;
     if (R.rc > 0) {
       double rd = R.rl_dip + 2 * ctx->S.lever_max;
-      cb = std::max(b + b / 2, (long)(1.25 * R.N * (R.N / R.V) * 4.0 / 3.0 * M_PI * rd * rd * rd)) + 64 * 16;
+      R.dl_w = (int)width(md, R.N / R.V * 4.0 / 3.0 * M_PI * rd * rd * rd, need_dl.empty() ? 0 : need_dl[r]);
     }
-    if (tight_lists()) {  // regions exactly as counted (exercises the overflow path)
-      ca = a;
-      cb = R.rc > 0 ? b : 0;
-    }
-    if (!need_nl.empty()) ca = std::max(ca, need_nl[r] + need_nl[r] / 4);
-    if (!need_dl.empty() && R.rc > 0) cb = std::max(cb, need_dl[r] + need_dl[r] / 4);
-    R.nl_base = (int)nl_tot;
-    R.nl_cap = (int)ca;
-    R.dl_base = (int)dl_tot;
-    R.dl_cap = (int)cb;
-    nl_tot += ca;
-    dl_tot += cb;
+    R.nl_base = nl_tot;
+    R.sl_base = sl_tot;
+    R.dl_base = dl_tot;
+    nl_tot += (long)R.nl_w * R.N;
+    sl_tot += (long)R.sl_w * R.N;
+    dl_tot += (long)R.dl_w * R.N;
   }
-  if (nl_tot > 0x7fffffffL || dl_tot > 0x7fffffffL) return fail(ctx, MSC_E_CAPACITY, "neighbour lists exceed 2^31 entries");
-  if (nl_tot > ctx->nl_cap) {
+  if (nl_tot > ctx->nl_alloc) {
     if ((rc = dalloc(ctx, &ctx->d_nl_idx, nl_tot))) return rc;
-    ctx->nl_cap = nl_tot;
+    ctx->nl_alloc = nl_tot;
   }
-  if (dl_tot > ctx->dl_cap) {
+  if (sl_tot > ctx->sl_alloc) {
+    if ((rc = dalloc(ctx, &ctx->d_sl_idx, sl_tot))) return rc;
+    ctx->sl_alloc = sl_tot;
+  }
+  if (dl_tot > ctx->dl_alloc) {
     if ((rc = dalloc(ctx, &ctx->d_dl_idx, std::max(dl_tot, 1L)))) return rc;
-    ctx->dl_cap = dl_tot;
+    ctx->dl_alloc = dl_tot;
   }
   if ((rc = upload_reps(ctx))) return rc;
   refresh_state_ptrs(ctx);
@@ -840,14 +852,13 @@ int msc_neigh_list(msc_ctx *ctx, int rep, double r_list) {
     if (rep == MSC_ALL || r == rep) ctx->reps[r].r_list_max = r_list;
   int rc = lists_layout(ctx);
   if (rc) return rc;
-  HIPC(hipMemsetAsync(ctx->d_flags, 0, sizeof(int) * 4 * ctx->nrep, ctx->stream));
+  launch_fill_int(ctx->d_flags, 4 * ctx->nrep, 0, ctx->stream);
   if ((rc = lists_build(ctx, ctx->d_one))) return rc;
-  int flags[4 * 64];
   std::vector<int> fl(4 * ctx->nrep);
   if (int rc_ = d2h(ctx, fl.data(), ctx->d_flags, sizeof(int) * fl.size())) return rc_;
-  (void)flags;
   for (int r = 0; r < ctx->nrep; ++r)
-    if (fl[4 * r + 2] || fl[4 * r + 3]) return fail(ctx, MSC_E_CAPACITY, "neighbour list region overflow");
+    if (fl[4 * r + 1] || fl[4 * r + 2] || fl[4 * r + 3])
+      return fail(ctx, MSC_E_CAPACITY, "neighbour rows longer than measured");
   ctx->have_lists = true;
   return MSC_OK;
 }
@@ -856,18 +867,16 @@ int msc_get_neigh_list(msc_ctx *ctx, int rep, int *off, int *nbr, long cap, long
   if (!ctx || rep < 0 || rep >= ctx->nrep || !off || !total) return MSC_E_ARG;
   if (!ctx->have_lists) return fail(ctx, MSC_E_STATE, "no neighbour list");
   const RepDev &R = ctx->reps[rep];
-  std::vector<int> cnt(R.N), st(R.N), idx(std::max(R.nl_cap, 1));
-  HIPC(hipStreamSynchronize(ctx->stream));
+  std::vector<int> cnt(R.N), idx(std::max<size_t>((size_t)R.nl_w * R.N, 1));
   if (int rc_ = d2h(ctx, cnt.data(), ctx->d_nl_cnt + R.n0, sizeof(int) * R.N)) return rc_;
-  if (int rc_ = d2h(ctx, st.data(), ctx->d_nl_start + R.n0, sizeof(int) * R.N)) return rc_;
-  if (int rc_ = d2h(ctx, idx.data(), ctx->d_nl_idx + R.nl_base, sizeof(int) * R.nl_cap)) return rc_;
+  if (int rc_ = d2h(ctx, idx.data(), ctx->d_nl_idx + R.nl_base, sizeof(int) * (size_t)R.nl_w * R.N)) return rc_;
   long t = 0;
   std::vector<int> row;
   for (int i = 0; i < R.N; ++i) {
     off[i] = (int)t;
     row.clear();
     for (int q = 0; q < cnt[i]; ++q) {
-      int j = idx[st[i] - R.nl_base + q] - R.n0;
+      int j = idx[(size_t)q * R.N + i] - R.n0;
       if (j > i) row.push_back(j);
     }
     std::sort(row.begin(), row.end());
@@ -889,7 +898,7 @@ int msc_check_list(msc_ctx *ctx, int rep, double r_list_min, double r_list_max, 
   int rc;
   if ((rc = upload_reps(ctx))) return rc;
   State s = sel(ctx, rep);
-  launch_check(s, ctx->d_task_out, ctx->d_rebuild, nullptr, 0, ctx->stream);
+  launch_check(s, ctx->d_task_out, ctx->d_rebuild, nullptr, nullptr, 0, false, ctx->stream);
   if (int rc_ = d2h(ctx, rebuild, ctx->d_rebuild + rep, sizeof(int))) return rc_;
   return MSC_OK;
 }
@@ -904,6 +913,12 @@ static int read_energy(msc_ctx *ctx, int rep, RepEnergy *e) {
 
 static void do_force_sc(msc_ctx *ctx, int rep) {
   State s = sel(ctx, rep);
+  if (ctx->mixed && rep != MSC_ALL) {
+    // a force call outside the step loop: refresh contact rows whose skin
+    // the particles have used up since they were built
+    launch_check(s, ctx->d_task_out, ctx->d_rebuild, ctx->d_srebuild, nullptr, 0, true, ctx->stream);
+    launch_short(s, ctx->d_srebuild, ctx->d_flags, ctx->stream);
+  }
   {
     Bracket b(ctx, "wca");
     launch_force_sc(s, !ctx->mixed, ctx->d_task_out, ctx->d_flags, ctx->stream);
@@ -1120,7 +1135,7 @@ int msc_init_run(msc_ctx *ctx) {
   if (ctx->nd != 1) return fail(ctx, MSC_E_ARG, "the step loop takes one dipole slot per particle");
   int rc = lists_layout(ctx);
   if (rc) return rc;
-  HIPC(hipMemsetAsync(ctx->d_flags, 0, sizeof(int) * 4 * ctx->nrep, ctx->stream));
+  launch_fill_int(ctx->d_flags, 4 * ctx->nrep, 0, ctx->stream);
   if ((rc = lists_build(ctx, ctx->d_one))) return rc;
   ctx->have_lists = true;
   forces_all(ctx);
@@ -1133,7 +1148,8 @@ int msc_init_run(msc_ctx *ctx) {
   if (int rc_ = d2h(ctx, fl.data(), ctx->d_flags, sizeof(int) * fl.size())) return rc_;
   for (int r = 0; r < ctx->nrep; ++r) {
     ctx->reps[r].Uself = red[kRedStride * r + 15] / ctx->reps[r].N;
-    if (fl[4 * r + 2] || fl[4 * r + 3]) return fail(ctx, MSC_E_CAPACITY, "neighbour list region overflow");
+    if (fl[4 * r + 1] || fl[4 * r + 2] || fl[4 * r + 3])
+      return fail(ctx, MSC_E_CAPACITY, "neighbour rows longer than measured");
     if (fl[4 * r + 0]) return fail(ctx, fl[4 * r] == 1 ? MSC_E_OVERLAP : MSC_E_DISTANCE, "initial overlap");
   }
   return upload_reps(ctx);
@@ -1176,8 +1192,10 @@ static void enqueue_step(msc_ctx *ctx, int step, const double *d_gs_step, double
   launch_observables(s, ctx->d_E, ctx->d_Kin, ctx->d_mom, d_obs_step, st);
   // list maintenance (v25/main.cpp:679-701): decide on the device, rebuild
   // inside kernels that return at once for replicas that keep their list
-  launch_check(s, ctx->d_task_out, ctx->d_rebuild, d_obs_step + MSC_OBS_REBUILD, MSC_OBS_N, st);
+  launch_check(s, ctx->d_task_out, ctx->d_rebuild, ctx->d_srebuild, d_obs_step + MSC_OBS_REBUILD, MSC_OBS_N, false,
+               st);
   lists_build(ctx, ctx->d_rebuild);
+  if (ctx->mixed) launch_short(s, ctx->d_srebuild, ctx->d_flags, st);
 }
 
 // Device copy of everything a step carries into the next one (positions,
@@ -1187,7 +1205,7 @@ static int state_copy(msc_ctx *ctx, bool restore) {
   const size_t N = ctx->Ntot, nrep = ctx->nrep;
   struct Item { void *p; size_t bytes; } items[] = {
       {ctx->d_x, 3 * N * 8}, {ctx->d_p, 3 * N * 8}, {ctx->d_l, 3 * N * 8}, {ctx->d_ori, 9 * N * 8},
-      {ctx->d_F, 3 * N * 8}, {ctx->d_T, 3 * N * 8}, {ctx->d_snap, 3 * N * 8},
+      {ctx->d_F, 3 * N * 8}, {ctx->d_T, 3 * N * 8}, {ctx->d_snap, 3 * N * 8}, {ctx->d_snap_e, 6 * N * 8},
       {ctx->d_th, nrep * sizeof(Thermo)}, {ctx->d_E, nrep * sizeof(RepEnergy)}, {ctx->d_Kin, 2 * nrep * 8}};
   size_t total = 0;
   for (auto &it : items) total += (it.bytes + 255) / 256 * 256;
@@ -1256,7 +1274,7 @@ int msc_run(msc_ctx *ctx, int step0, int nsteps, double *obs) {
     // an overflowing rebuild empties the replica's rows, so an overlap later in
     // the same chunk may be its consequence: overflow is handled first
     bool overflow = false;
-    for (int r = 0; r < nrep; ++r) overflow |= fl[4 * r + 2] != 0 || fl[4 * r + 3] != 0;
+    for (int r = 0; r < nrep; ++r) overflow |= fl[4 * r + 1] != 0 || fl[4 * r + 2] != 0 || fl[4 * r + 3] != 0;
     if (!overflow) {
       std::string m;
       int code = 0;
@@ -1272,22 +1290,25 @@ int msc_run(msc_ctx *ctx, int step0, int nsteps, double *obs) {
       if (++retries > 4) {
         std::string m = "neighbour list regions keep overflowing:";
         for (int r = 0; r < nrep; ++r)
-          if (fl[4 * r + 2] || fl[4 * r + 3])
-            m += " replica " + std::to_string(r) + " needs " + std::to_string(fl[4 * r + 2]) + "/" +
-                 std::to_string(fl[4 * r + 3]) + " of " + std::to_string(ctx->reps[r].nl_cap) + "/" +
-                 std::to_string(ctx->reps[r].dl_cap) + ";";
+          if (fl[4 * r + 1] || fl[4 * r + 2] || fl[4 * r + 3])
+            m += " replica " + std::to_string(r) + " rows need " + std::to_string(fl[4 * r + 2]) + "/" +
+                 std::to_string(fl[4 * r + 1]) + "/" + std::to_string(fl[4 * r + 3]) + " of " +
+                 std::to_string(ctx->reps[r].nl_w) + "/" + std::to_string(ctx->reps[r].sl_w) + "/" +
+                 std::to_string(ctx->reps[r].dl_w) + ";";
         return fail(ctx, MSC_E_CAPACITY, m);
       }
       ++ctx->chunk_repeats;
-      // a region that overflowed grows to at least 1.5 times its old size
-      std::vector<long> need_nl(nrep), need_dl(nrep);
+      // overflowing rows grow to at least 1.5 times their old capacity; the
+      // others keep theirs
+      std::vector<long> need_nl(nrep), need_sl(nrep), need_dl(nrep);
       for (int r = 0; r < nrep; ++r) {
         const RepDev &R = ctx->reps[r];
-        need_nl[r] = fl[4 * r + 2] ? std::max<long>(fl[4 * r + 2], R.nl_cap + R.nl_cap / 2) : 0;
-        need_dl[r] = fl[4 * r + 3] ? std::max<long>(fl[4 * r + 3], R.dl_cap + R.dl_cap / 2) : 0;
+        need_nl[r] = fl[4 * r + 2] ? std::max<long>(fl[4 * r + 2], R.nl_w + R.nl_w / 2) : R.nl_w;
+        need_sl[r] = fl[4 * r + 1] ? std::max<long>(fl[4 * r + 1], R.sl_w + R.sl_w / 2) : R.sl_w;
+        need_dl[r] = fl[4 * r + 3] ? std::max<long>(fl[4 * r + 3], R.dl_w + R.dl_w / 2) : R.dl_w;
       }
       if (int rc_ = state_copy(ctx, true)) return rc_;
-      if (int rc_ = lists_layout(ctx, need_nl, need_dl)) return rc_;
+      if (int rc_ = lists_layout(ctx, need_nl, need_sl, need_dl)) return rc_;
       launch_fill_int(ctx->d_flags, 4 * nrep, 0, ctx->stream);
       if (int rc_ = lists_build(ctx, ctx->d_one)) return rc_;
       continue;

@@ -26,8 +26,13 @@ struct RepDev {
   int tab0;             // first row of this replica's real-space table
   int cell0, ncell;     // first cell and cell count of this replica's grid
   int nc[3];            // cells per dimension (1 if the box is too small)
-  int nl_base, nl_cap;  // WCA neighbour-list region (entries)
-  int dl_base, dl_cap;  // real-space pair-list region (entries)
+  // Neighbour rows: entry q of particle g sits at idx[base + q*N + (g - n0)]
+  // (column-major, so the lanes of a wave read consecutive words); w is the
+  // row capacity.  nl: reference list (segment distance < r_list_max); sl:
+  // contact list (segment distance < r_short); dl: real-space dipole pairs.
+  int nl_w, sl_w, dl_w;
+  int pad_w;
+  long nl_base, sl_base, dl_base;
   long part0;           // first element of this replica's structure-factor partials
   int task0, ntask;     // wave tasks (64 particles) of this replica
   int rtask0, nrtask;   // reciprocal chunk tasks of this replica
@@ -36,6 +41,8 @@ struct RepDev {
   double alpha, rc, sr_cut;
   double r_list_min, r_list_max;
   double rl_dip;        // real-space pair-list radius (rc + skin)
+  double r_short;       // contact-list radius: 2^(1/6) max diameter + skin_short
+  double skin_short;
   double lmax;          // largest cylinder length
   double B[3];
   double temp, dt, Q, gamma, dens;

@@ -115,53 +115,16 @@ __global__ void k_cells_assign(State s, const int *__restrict__ rebuild) {
   atomicAdd(&s.cell_cnt[cell], 1);
 }
 
-__global__ void k_cells_fill(State s, const int *__restrict__ rebuild) {
-  int g = s.g_begin + blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= s.g_end) return;
-  if (!rebuild[s.rep_of[g]]) return;
-  int c = s.cell_of[g];
-  int pos = s.cell_start[c] + atomicAdd(&s.cell_fill[c], 1);
-  s.cell_list[pos] = g;
-}
-
-// ascending particle order inside each cell makes every later traversal
-// deterministic
-__global__ void k_cells_sort(State s, const int *__restrict__ cell_rep, const int *__restrict__ rebuild) {
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= s.ncells) return;
-  if (!rebuild[cell_rep[c]]) return;
-  int b = s.cell_start[c], n = s.cell_cnt[c];
-  int *v = s.cell_list + b;
-  for (int i = 1; i < n; ++i) {
-    int key = v[i], j = i - 1;
-    while (j >= 0 && v[j] > key) { v[j + 1] = v[j]; --j; }
-    v[j + 1] = key;
-  }
-}
-
-// Exclusive scan of one segment per replica (one block of 1024 threads per
-// replica, each thread owning a contiguous run).  mode 0: cell counts ->
-// cell starts (base = the replica's first particle); mode 1 / 2: WCA /
-// real-space row counts -> row starts in the replica's list region.  A
-// replica whose rebuild flag is clear is left untouched.  Rows that would
-// overflow the region raise flags[rep*4+2] (WCA list) or flags[rep*4+3]
-// (real-space list) to the entries needed and are emptied.
-// The mode is a template parameter: a three-way branch on a uniform
-// kernel argument here was compiled for gfx950 into code that, for mode 2,
-// loaded the segment bounds through uninitialised scalar address registers
-// (an aperture-violation fault on the device).
-template <int MODE>
-__global__ void __launch_bounds__(1024) k_scan_seg(int *__restrict__ cnt, int *__restrict__ out,
-                                                   const RepDev *__restrict__ reps,
-                                                   const int *__restrict__ rebuild, int *__restrict__ flags) {
+// Exclusive scan of the cell counts of one replica into cell starts (one
+// block of 1024 threads per replica, each thread owning a contiguous run of
+// cells); the replica's particles occupy [n0, n0 + N) of the cell order.
+__global__ void __launch_bounds__(1024) k_scan_cells(int *__restrict__ cnt, int *__restrict__ out,
+                                                     const RepDev *__restrict__ reps,
+                                                     const int *__restrict__ rebuild) {
   __shared__ int sh[1024];
   const int rp = blockIdx.x;
   if (!rebuild[rp]) return;
-  const RepDev &R = reps[rp];
-  const int b0 = MODE == 0 ? R.cell0 : R.n0;
-  const int n = MODE == 0 ? R.ncell : R.N;
-  const int base = MODE == 0 ? R.n0 : (MODE == 1 ? R.nl_base : R.dl_base);
-  const int cap = MODE == 0 ? R.N : (MODE == 1 ? R.nl_cap : R.dl_cap);
+  const int b0 = reps[rp].cell0, n = reps[rp].ncell, base = reps[rp].n0;
   int t = threadIdx.x;
   int seg = (n + 1023) / 1024;
   int b = t * seg, e = min(n, b + seg);
@@ -175,12 +138,6 @@ __global__ void __launch_bounds__(1024) k_scan_seg(int *__restrict__ cnt, int *_
     sh[t] += v;
     __syncthreads();
   }
-  const int total = sh[1023];
-  if (total > cap) {
-    if (t == 0) atomicMax(&flags[rp * 4 + (MODE == 2 ? 3 : 2)], total);
-    for (int i = b; i < e; ++i) { cnt[b0 + i] = 0; out[b0 + i] = base; }
-    return;
-  }
   int run = base + sh[t] - sum;
   for (int i = b; i < e; ++i) {
     int v = cnt[b0 + i];
@@ -189,24 +146,72 @@ __global__ void __launch_bounds__(1024) k_scan_seg(int *__restrict__ cnt, int *_
   }
 }
 
-// Neighbour lists from the cell grid.  For particle g, every candidate j of
-// the same replica in the 27 (or fewer) surrounding cells is tested:
-//  - WCA list: segment distance < r_list_max, evaluated exactly as
-//    neigh_list does for the pair (min(g,j), max(g,j)) (v25/functions.cpp:1310-1331);
-//    a cheap centre-distance bound skips pairs that cannot qualify;
-//  - dipolar list (cut-off mode): dipole-pair centre distance < rl_dip.
-// pass 0 counts, pass 1 writes rows at the scanned offsets.  Rows hold both
-// directions (full lists) in stencil order.
-template <int PASS>
-__global__ void __launch_bounds__(128) k_build_lists(State s, const int *__restrict__ rebuild) {
+// particles into their cells in arrival order ...
+__global__ void k_cells_fill(State s, const int *__restrict__ rebuild) {
   int g = s.g_begin + blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= s.g_end) return;
-  int rp = s.rep_of[g];
+  if (!rebuild[s.rep_of[g]]) return;
+  int c = s.cell_of[g];
+  int pos = s.cell_start[c] + atomicAdd(&s.cell_fill[c], 1);
+  s.cell_tmp[pos] = g;
+}
+
+// ... then into ascending index order: each particle's rank among the
+// members of its cell is the number of members with a smaller index.  Every
+// later traversal of a cell is in a fixed order.
+__global__ void k_cells_rank(State s, const int *__restrict__ rebuild) {
+  int g = s.g_begin + blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= s.g_end) return;
+  if (!rebuild[s.rep_of[g]]) return;
+  int c = s.cell_of[g];
+  int b = s.cell_start[c], n = s.cell_cnt[c];
+  int rank = 0;
+  for (int k = 0; k < n; ++k) rank += s.cell_tmp[b + k] < g;
+  s.cell_list[b + rank] = g;
+}
+
+__device__ __forceinline__ size_t row_at(long base, int N, int n0, int q, int g) {
+  return (size_t)base + (size_t)q * (size_t)N + (size_t)(g - n0);
+}
+
+// Segment distance of the pair (g, j) evaluated as neigh_list and force_sc
+// do: A = the lower index, rAB = minimum image of r_A - r_B.
+__device__ __forceinline__ double pair_segdist(const State &s, const RepDev &R, int g, int j) {
+  const int N = s.Ntot;
+  const int A = g < j ? g : j, Bi = g < j ? j : g;
+  double rA[3] = {s.x[A], s.x[N + A], s.x[2 * N + A]};
+  double rB[3] = {s.x[Bi], s.x[N + Bi], s.x[2 * N + Bi]};
+  double uA[3] = {s.ori[6 * N + A], s.ori[7 * N + A], s.ori[8 * N + A]};
+  double uB[3] = {s.ori[6 * N + Bi], s.ori[7 * N + Bi], s.ori[8 * N + Bi]};
+  double rAB[3], dAB[3], st[2];
+  for (int a = 0; a < 3; ++a) {
+    rAB[a] = rA[a] - rB[a];
+    rAB[a] -= R.L[a] * floor(rAB[a] / R.L[a] + 0.5);
+  }
+  return segdist(rA, uA, s.len[A], rB, uB, s.len[Bi], R.L, rAB, dAB, st);
+}
+
+// Neighbour rows from the cell grid in one pass.  Thread t takes the t-th
+// particle in cell order, so the lanes of a wave share their candidate
+// cells.  For particle g every candidate j of the same replica in the 27 (or
+// fewer) surrounding cells is tested:
+//  - reference list: segment distance < r_list_max, evaluated exactly as
+//    neigh_list does for the pair (min(g,j), max(g,j)) (v25/functions.cpp:1310-1331);
+//    a centre-distance bound skips pairs that cannot qualify;
+//  - contact list: segment distance < r_short (a subset of the row above);
+//  - real-space list (cut-off mode): dipole-pair centre distance < rl_dip.
+// Rows hold both directions in stencil order.  A row longer than its
+// capacity is cut and raises flags[rep*4 + 2 / 1 / 3] to the length needed
+// (a layout pass runs with capacity 0 to measure the longest rows).
+__global__ void __launch_bounds__(128) k_build_lists(State s, const int *__restrict__ rebuild, int *__restrict__ flags) {
+  int t = s.g_begin + blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= s.g_end) return;
+  int rp = s.rep_of[t];
   if (!rebuild[rp]) return;
+  const int g = s.cell_list[t];
   const RepDev &R = s.reps[rp];
   const int N = s.Ntot;
   double rg[3] = {s.x[g], s.x[N + g], s.x[2 * N + g]};
-  double ug[3] = {s.ori[6 * N + g], s.ori[7 * N + g], s.ori[8 * N + g]};
   double Lg = s.len[g];
   int ci[3];
   {
@@ -216,14 +221,10 @@ __global__ void __launch_bounds__(128) k_build_lists(State s, const int *__restr
     ci[2] = c / (R.nc[0] * R.nc[1]);
   }
   const double rl2 = R.r_list_max * R.r_list_max;
+  const double rs2 = R.r_short * R.r_short;
   const bool dip = R.rc > 0;
-  const double rld = R.rl_dip;
-  int nw = 0, ndl = 0;
-  int wbase = 0, dbase = 0;
-  if (PASS == 1) {
-    wbase = s.nl_start[g];
-    dbase = dip ? s.dl_start[g] : 0;
-  }
+  const double reachd = R.rl_dip + 2.0 * s.lever_max;
+  int nw = 0, ns = 0, ndl = 0;
   int lo[3], hi[3];
   for (int a = 0; a < 3; ++a) {
     if (R.nc[a] >= 3) { lo[a] = -1; hi[a] = 1; }
@@ -241,42 +242,63 @@ __global__ void __launch_bounds__(128) k_build_lists(State s, const int *__restr
         for (int q = b; q < e; ++q) {
           int j = s.cell_list[q];
           if (j == g) continue;
-          double rj[3] = {s.x[j], s.x[N + j], s.x[2 * N + j]};
           double rgj[3];
-          for (int a = 0; a < 3; ++a) rgj[a] = wrapd(rg[a] - rj[a], R.L[a]);
+          for (int a = 0; a < 3; ++a) rgj[a] = wrapd(rg[a] - s.x[a * N + j], R.L[a]);
           double rc2 = dot3(rgj, rgj);
           double reach = R.r_list_max + 0.5 * (Lg + s.len[j]) + 1e-9;
           if (rc2 < reach * reach * (1 + 1e-12)) {
-            // canonical orientation: A = lower index, rAB = r_A - r_B
-            int A = g < j ? g : j, Bi = g < j ? j : g;
-            double rA[3] = {s.x[A], s.x[N + A], s.x[2 * N + A]};
-            double rB[3] = {s.x[Bi], s.x[N + Bi], s.x[2 * N + Bi]};
-            double uA[3] = {s.ori[6 * N + A], s.ori[7 * N + A], s.ori[8 * N + A]};
-            double uB[3] = {s.ori[6 * N + Bi], s.ori[7 * N + Bi], s.ori[8 * N + Bi]};
-            double rAB[3], dAB[3], st[2];
-            for (int a = 0; a < 3; ++a) {
-              rAB[a] = rA[a] - rB[a];
-              rAB[a] -= R.L[a] * floor(rAB[a] / R.L[a] + 0.5);
-            }
-            double d2 = segdist(rA, uA, s.len[A], rB, uB, s.len[Bi], R.L, rAB, dAB, st);
+            double d2 = pair_segdist(s, R, g, j);
             if (d2 < rl2) {
-              if (PASS == 1 && nw < s.nl_cnt[g]) s.nl_idx[wbase + nw] = j;
+              if (nw < R.nl_w) s.nl_idx[row_at(R.nl_base, R.N, R.n0, nw, g)] = j;
               ++nw;
+              if (d2 < rs2) {
+                if (ns < R.sl_w) s.sl_idx[row_at(R.sl_base, R.N, R.n0, ns, g)] = j;
+                ++ns;
+              }
             }
           }
-          if (dip) {
-            // pairs whose dipoles can come within rc before the next rebuild
-            double reachd = rld + 2.0 * s.lever_max;
-            if (rc2 < reachd * reachd) {
-              if (PASS == 1 && ndl < s.dl_cnt[g]) s.dl_idx[dbase + ndl] = j;
-              ++ndl;
-            }
+          // pairs whose dipoles can come within rc before the next rebuild
+          if (dip && rc2 < reachd * reachd) {
+            if (ndl < R.dl_w) s.dl_idx[row_at(R.dl_base, R.N, R.n0, ndl, g)] = j;
+            ++ndl;
           }
         }
       }
-  if (PASS == 0) {
-    s.nl_cnt[g] = nw;
-    if (dip) s.dl_cnt[g] = ndl;
+  s.nl_cnt[g] = min(nw, R.nl_w);
+  s.sl_cnt[g] = min(ns, R.sl_w);
+  if (nw > R.nl_w) atomicMax(&flags[rp * 4 + 2], nw);
+  if (ns > R.sl_w) atomicMax(&flags[rp * 4 + 1], ns);
+  if (dip) {
+    s.dl_cnt[g] = min(ndl, R.dl_w);
+    if (ndl > R.dl_w) atomicMax(&flags[rp * 4 + 3], ndl);
+  }
+}
+
+// Contact rows refreshed from the reference rows (same pair order), for
+// replicas whose contact skin is used up while the reference list is still
+// valid.
+__global__ void __launch_bounds__(128) k_short_build(State s, const int *__restrict__ srebuild, int *__restrict__ flags) {
+  int g = s.g_begin + blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= s.g_end) return;
+  int rp = s.rep_of[g];
+  if (!srebuild[rp]) return;
+  const RepDev &R = s.reps[rp];
+  const double rs2 = R.r_short * R.r_short;
+  const int n = s.nl_cnt[g];
+  int ns = 0;
+  for (int q = 0; q < n; ++q) {
+    int j = s.nl_idx[row_at(R.nl_base, R.N, R.n0, q, g)];
+    if (pair_segdist(s, R, g, j) < rs2) {
+      if (ns < R.sl_w) s.sl_idx[row_at(R.sl_base, R.N, R.n0, ns, g)] = j;
+      ++ns;
+    }
+  }
+  s.sl_cnt[g] = min(ns, R.sl_w);
+  if (ns > R.sl_w) atomicMax(&flags[rp * 4 + 1], ns);
+  const int N = s.Ntot;
+  for (int a = 0; a < 3; ++a) {
+    s.snap_e[a * N + g] = s.x[a * N + g];
+    s.snap_e[(3 + a) * N + g] = s.ori[(6 + a) * N + g];
   }
 }
 
@@ -293,52 +315,98 @@ __device__ double round12(double x) {
   return m / 1e12;
 }
 
+// list snapshot of a rebuilt replica: rounded centres (savec_list) and the
+// centre and axis from which contact-list displacements are measured
 __global__ void k_savec(State s, const int *__restrict__ rebuild) {
   int g = s.g_begin + blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= s.g_end) return;
   if (!rebuild[s.rep_of[g]]) return;
-  for (int a = 0; a < 3; ++a) s.snap[a * s.Ntot + g] = round12(s.x[a * s.Ntot + g]);
+  const int N = s.Ntot;
+  for (int a = 0; a < 3; ++a) {
+    s.snap[a * N + g] = round12(s.x[a * N + g]);
+    s.snap_e[a * N + g] = s.x[a * N + g];
+    s.snap_e[(3 + a) * N + g] = s.ori[(6 + a) * N + g];
+  }
 }
 
-// per-task maximum of |min-image displacement component| since the snapshot
+// per wave task: [0] largest centre-component displacement since savec_list
+// (check_list), [1] largest displacement of any point of a segment since the
+// contact list was built: |dr| + L/2 |du|
 __global__ void __launch_bounds__(256) k_check_list(State s, double *__restrict__ task_out) {
   int tid = s.task_begin + wave_task_id();
   int lane = threadIdx.x & 63;
   if (wave_task_id() >= s.ntask) return;
   Task t = s.tasks[tid];
   const RepDev &R = s.reps[t.rep];
-  double m = 0.0;
+  double m = 0.0, me = 0.0;
   if (lane < t.n) {
-    int g = t.g0 + lane;
+    const int g = t.g0 + lane, N = s.Ntot;
+    double dr2 = 0, du2 = 0;
     for (int a = 0; a < 3; ++a) {
-      double D = s.x[a * s.Ntot + g] - s.snap[a * s.Ntot + g];
+      double D = s.x[a * N + g] - s.snap[a * N + g];
       D -= R.L[a] * floor(D / R.L[a] + 0.5);
       m = fmax(m, fabs(D));
+      double E = s.x[a * N + g] - s.snap_e[a * N + g];
+      E -= R.L[a] * floor(E / R.L[a] + 0.5);
+      dr2 += E * E;
+      double U = s.ori[(6 + a) * N + g] - s.snap_e[(3 + a) * N + g];
+      du2 += U * U;
     }
+    me = sqrt(dr2) + 0.5 * s.len[g] * sqrt(du2);
   }
   m = wave_max(m);
-  if (lane == 0) task_out[tid] = m;
+  me = wave_max(me);
+  if (lane == 0) {
+    task_out[(size_t)tid * 2 + 0] = m;
+    task_out[(size_t)tid * 2 + 1] = me;
+  }
 }
 
-__global__ void k_check_decide(State s, const double *__restrict__ task_max, int *__restrict__ rebuild,
-                               double *__restrict__ obs_rebuild, int obs_stride) {
-  int rp = s.rep_begin + blockIdx.x * blockDim.x + threadIdx.x;
-  if (rp >= s.rep_end) return;
+// one block per replica: rebuild the reference list when 2 sqrt(3) max|dx| >
+// r_list_max - r_list_min (v25/main.cpp:679-701); refresh the contact list
+// when two segments may have closed the contact skin, unless the reference
+// list is rebuilt anyway (which rebuilds the contact rows with it).  With
+// `standalone` (a force call outside the step loop, which never rebuilds the
+// reference list) the contact test alone decides.
+__global__ void __launch_bounds__(256) k_check_decide(State s, const double *__restrict__ task_max,
+                                                      int *__restrict__ rebuild, int *__restrict__ srebuild,
+                                                      double *__restrict__ obs_rebuild, int obs_stride,
+                                                      int standalone) {
+  __shared__ double sh[2][256];
+  const int rp = s.rep_begin + blockIdx.x;
   const RepDev &R = s.reps[rp];
-  double dm = 0.0;
-  for (int t = R.task0; t < R.task0 + R.ntask; ++t) dm = fmax(dm, task_max[t]);
-  dm = 2. * sqrt(3. * dm * dm);
-  int rb = dm > R.r_list_max - R.r_list_min;
-  rebuild[rp] = rb;
-  if (obs_rebuild) obs_rebuild[(size_t)rp * obs_stride] = rb;
+  double dm = 0.0, de = 0.0;
+  for (int t = R.task0 + threadIdx.x; t < R.task0 + R.ntask; t += 256) {
+    dm = fmax(dm, task_max[(size_t)t * 2]);
+    de = fmax(de, task_max[(size_t)t * 2 + 1]);
+  }
+  sh[0][threadIdx.x] = dm;
+  sh[1][threadIdx.x] = de;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      sh[0][threadIdx.x] = fmax(sh[0][threadIdx.x], sh[0][threadIdx.x + o]);
+      sh[1][threadIdx.x] = fmax(sh[1][threadIdx.x], sh[1][threadIdx.x + o]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    dm = sh[0][0];
+    dm = 2. * sqrt(3. * dm * dm);
+    int rb = dm > R.r_list_max - R.r_list_min;
+    rebuild[rp] = rb;
+    if (srebuild) srebuild[rp] = (standalone || !rb) && 2. * sh[1][0] > R.skin_short;
+    if (obs_rebuild) obs_rebuild[(size_t)rp * obs_stride] = rb;
+  }
 }
 
 // --------------------------------------------------------------------------
 // force_sc: WCA on the segment distance (v25/functions.cpp:1397-1553)
 // --------------------------------------------------------------------------
-// STRICT: every list pair runs spherocyl_dist and both abort tests, as the
-// reference.  Otherwise a centre-distance bound skips pairs that cannot be
-// inside 2^(1/6) sigma (their force is exactly zero).
+// STRICT: every pair of the reference list runs spherocyl_dist and both
+// abort tests, as the reference.  Otherwise only the contact rows (pairs
+// within r_short, which hold every pair inside 2^(1/6) sigma until the
+// contact skin is used up) are evaluated; the others contribute exactly zero.
 template <bool STRICT>
 __global__ void __launch_bounds__(256) k_force_sc(State s, double *__restrict__ task_out, int *__restrict__ flags) {
   int tid = s.task_begin + wave_task_id();
@@ -356,21 +424,13 @@ __global__ void __launch_bounds__(256) k_force_sc(State s, double *__restrict__ 
     double F[3] = {0, 0, 0}, T[3] = {0, 0, 0};
     int code = 0;
     double rg[3] = {s.x[g], s.x[N + g], s.x[2 * N + g]};
-    int b = s.nl_start[g], e = b + s.nl_cnt[g];
-    for (int q = b; q < e; ++q) {
-      int j = s.nl_idx[q];
-      double rj[3] = {s.x[j], s.x[N + j], s.x[2 * N + j]};
+    const int n = STRICT ? s.nl_cnt[g] : s.sl_cnt[g];
+    const int *idx = STRICT ? s.nl_idx : s.sl_idx;
+    const long base = STRICT ? R.nl_base : R.sl_base;
+    for (int q = 0; q < n; ++q) {
+      int j = idx[row_at(base, R.N, R.n0, q, g)];
       double sigma = (s.dia[g] + s.dia[j]) / 2;
       double sig_sq = sigma * sigma;
-      if (!STRICT) {
-        double c2 = 0;
-        for (int a = 0; a < 3; ++a) {
-          double d = wrapd(rg[a] - rj[a], R.L[a]);
-          c2 += d * d;
-        }
-        double reach = srt * sigma + 0.5 * (s.len[g] + s.len[j]) + 1e-9;
-        if (c2 >= reach * reach * (1 + 1e-12)) continue;
-      }
       const bool gA = g < j;
       const int A = gA ? g : j, Bi = gA ? j : g;
       double rA[3] = {s.x[A], s.x[N + A], s.x[2 * N + A]};
@@ -616,8 +676,8 @@ __global__ void __launch_bounds__(256) k_dipR_list(State s, const R *__restrict_
   if (lane < tk.n) {
     const int g = tk.g0 + lane;
     double F[3] = {0, 0, 0}, T[3] = {0, 0, 0};
-    int b = s.dl_start[g], e = b + s.dl_cnt[g];
-    for (int q = b; q < e; ++q) dipR_partner<R>(s, Rp, tab, g, s.dl_idx[q], F, T, acc);
+    const int n = s.dl_cnt[g];
+    for (int q = 0; q < n; ++q) dipR_partner<R>(s, Rp, tab, g, s.dl_idx[row_at(Rp.dl_base, Rp.N, Rp.n0, q, g)], F, T, acc);
     for (int a = 0; a < 3; ++a) {
       s.F[a * s.Ntot + g] += F[a];
       s.T[a * s.Ntot + g] += T[a];
@@ -935,24 +995,36 @@ __global__ void __launch_bounds__(256) k_recip_force(State s, const Cpx<R> *__re
 // WHICH 0: wave-task partials, 1: k-slot partials (compile-time, like every
 // mode switch of these kernels)
 template <int WHICH>
-__global__ void k_reduce_items(const RepDev *__restrict__ reps, int rep0, const double *__restrict__ in, int istride,
-                               int nv, double *__restrict__ out, int ostride, int ooff) {
-  __shared__ double sh[256];
+__global__ void __launch_bounds__(256) k_reduce_items(const RepDev *__restrict__ reps, int rep0,
+                                                      const double *__restrict__ in, int istride, int nv,
+                                                      double *__restrict__ out, int ostride, int ooff) {
+  constexpr int kMaxV = 16;
+  __shared__ double sh[4][kMaxV];
   const int rp = rep0 + blockIdx.x;
   const RepDev &R = reps[rp];
   const int b = WHICH == 0 ? R.task0 : R.slot0;
   const int n = WHICH == 0 ? R.ntask : R.ncol * R.KZ;
-  for (int k = 0; k < nv; ++k) {
-    double v = 0.0;
-    for (int i = threadIdx.x; i < n; i += 256) v += in[(size_t)(b + i) * istride + k];
-    sh[threadIdx.x] = v;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-      if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
-      __syncthreads();
+  double v[kMaxV];
+#pragma unroll
+  for (int k = 0; k < kMaxV; ++k) v[k] = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) {
+    const double *row = in + (size_t)(b + i) * istride;
+#pragma unroll
+    for (int k = 0; k < kMaxV; ++k)
+      if (k < nv) v[k] += row[k];
+  }
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+  for (int k = 0; k < kMaxV; ++k) {
+    if (k < nv) {
+      double t = wave_sum(v[k]);
+      if (lane == 0) sh[wv][k] = t;
     }
-    if (threadIdx.x == 0) out[(size_t)rp * ostride + ooff + k] = sh[0];
-    __syncthreads();
+  }
+  __syncthreads();
+  if (threadIdx.x < nv) {
+    const int k = threadIdx.x;
+    out[(size_t)rp * ostride + ooff + k] = ((sh[0][k] + sh[1][k]) + sh[2][k]) + sh[3][k];
   }
 }
 
@@ -1195,29 +1267,26 @@ static inline int nblk(int n, int b) { return (n + b - 1) / b; }
 void launch_dipoles(const State &s, hipStream_t st) {
   MSC_LAUNCH(k_dipoles, dim3(nblk(s.g_end - s.g_begin, 256)), dim3(256), 0, st, s);
 }
-void launch_cells(const State &s, const int *rebuild, int *flags, hipStream_t st) {
+void launch_cells(const State &s, const int *rebuild, hipStream_t st) {
   MSC_LAUNCH(k_fill_int, dim3(nblk(s.ncells, 256)), dim3(256), 0, st, s.cell_cnt, s.ncells, 0);
   MSC_LAUNCH(k_fill_int, dim3(nblk(s.ncells, 256)), dim3(256), 0, st, s.cell_fill, s.ncells, 0);
   MSC_LAUNCH(k_cells_assign, dim3(nblk(s.g_end - s.g_begin, 256)), dim3(256), 0, st, s, rebuild);
-  MSC_LAUNCH(k_scan_seg<0>, dim3(s.nrep), dim3(1024), 0, st, s.cell_cnt, s.cell_start, s.reps, rebuild, flags);
+  MSC_LAUNCH(k_scan_cells, dim3(s.nrep), dim3(1024), 0, st, s.cell_cnt, s.cell_start, s.reps, rebuild);
   MSC_LAUNCH(k_cells_fill, dim3(nblk(s.g_end - s.g_begin, 256)), dim3(256), 0, st, s, rebuild);
-  MSC_LAUNCH(k_cells_sort, dim3(nblk(s.ncells, 256)), dim3(256), 0, st, s, s.cell_rep, rebuild);
+  MSC_LAUNCH(k_cells_rank, dim3(nblk(s.g_end - s.g_begin, 256)), dim3(256), 0, st, s, rebuild);
 }
-void launch_lists_count(const State &s, const int *rebuild, int *flags, hipStream_t st) {
-  MSC_LAUNCH(k_build_lists<0>, dim3(nblk(s.g_end - s.g_begin, 128)), dim3(128), 0, st, s, rebuild);
-  MSC_LAUNCH(k_scan_seg<1>, dim3(s.nrep), dim3(1024), 0, st, s.nl_cnt, s.nl_start, s.reps, rebuild, flags);
-  if (s.dl_cnt)
-    MSC_LAUNCH(k_scan_seg<2>, dim3(s.nrep), dim3(1024), 0, st, s.dl_cnt, s.dl_start, s.reps, rebuild, flags);
-}
-void launch_lists_fill(const State &s, const int *rebuild, hipStream_t st) {
-  MSC_LAUNCH(k_build_lists<1>, dim3(nblk(s.g_end - s.g_begin, 128)), dim3(128), 0, st, s, rebuild);
+void launch_lists(const State &s, const int *rebuild, int *flags, hipStream_t st) {
+  MSC_LAUNCH(k_build_lists, dim3(nblk(s.g_end - s.g_begin, 128)), dim3(128), 0, st, s, rebuild, flags);
   MSC_LAUNCH(k_savec, dim3(nblk(s.g_end - s.g_begin, 256)), dim3(256), 0, st, s, rebuild);
 }
-void launch_check(const State &s, double *task_out, int *rebuild, double *obs_rebuild, int obs_stride,
-                  hipStream_t st) {
+void launch_short(const State &s, const int *srebuild, int *flags, hipStream_t st) {
+  MSC_LAUNCH(k_short_build, dim3(nblk(s.g_end - s.g_begin, 128)), dim3(128), 0, st, s, srebuild, flags);
+}
+void launch_check(const State &s, double *task_out, int *rebuild, int *srebuild, double *obs_rebuild,
+                  int obs_stride, bool standalone, hipStream_t st) {
   MSC_LAUNCH(k_check_list, dim3(nblk(s.ntask, 4)), dim3(256), 0, st, s, task_out);
-  MSC_LAUNCH(k_check_decide, dim3(nblk(s.rep_end - s.rep_begin, 64)), dim3(64), 0, st, s, task_out, rebuild,
-                     obs_rebuild, obs_stride);
+  MSC_LAUNCH(k_check_decide, dim3(s.rep_end - s.rep_begin), dim3(256), 0, st, s, task_out, rebuild, srebuild,
+             obs_rebuild, obs_stride, standalone ? 1 : 0);
 }
 void launch_force_sc(const State &s, bool strict, double *task_out, int *flags, hipStream_t st) {
   if (strict)

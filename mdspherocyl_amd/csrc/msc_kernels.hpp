@@ -24,22 +24,25 @@ struct State {
   const double *dpc, *doc, *dmu;     // nd N, 3 nd N, nd N
   double *dpos, *dori;               // 3 nd N lab dipole positions / orientations
   double *snap;                      // 3N neighbour-list reference positions
+  double *snap_e;                    // 6N centre and axis when the contact list was built
   // cell grid
-  int *cell_of, *cell_cnt, *cell_start, *cell_fill, *cell_list;
+  int *cell_of, *cell_cnt, *cell_start, *cell_fill, *cell_tmp, *cell_list;
   const int *cell_rep;       // [ncells] owning replica
-  // neighbour lists (full rows; per-replica regions)
-  int *nl_cnt, *nl_start, *nl_idx;
-  int *dl_cnt, *dl_start, *dl_idx;
+  // neighbour rows (layout in RepDev): reference list, contact list,
+  // real-space dipole pairs; counts per particle
+  int *nl_cnt, *nl_idx;
+  int *sl_cnt, *sl_idx;
+  int *dl_cnt, *dl_idx;
   // reciprocal weights per slot (0 outside the half space)
   const double *kval;
 };
 
 void launch_dipoles(const State &s, hipStream_t st);
-void launch_cells(const State &s, const int *rebuild, int *flags, hipStream_t st);
-void launch_lists_count(const State &s, const int *rebuild, int *flags, hipStream_t st);
-void launch_lists_fill(const State &s, const int *rebuild, hipStream_t st);
-void launch_check(const State &s, double *task_out, int *rebuild, double *obs_rebuild, int obs_stride,
-                  hipStream_t st);
+void launch_cells(const State &s, const int *rebuild, hipStream_t st);
+void launch_lists(const State &s, const int *rebuild, int *flags, hipStream_t st);
+void launch_short(const State &s, const int *srebuild, int *flags, hipStream_t st);
+void launch_check(const State &s, double *task_out, int *rebuild, int *srebuild, double *obs_rebuild,
+                  int obs_stride, bool standalone, hipStream_t st);
 void launch_force_sc(const State &s, bool strict, double *task_out, int *flags, hipStream_t st);
 void launch_dipR(const State &s, bool mixed, bool all_pairs, const void *tab, double *task_out, hipStream_t st);
 void launch_ext_field(const State &s, double *task_out, hipStream_t st);

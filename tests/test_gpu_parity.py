@@ -229,6 +229,71 @@ def test_mixed_precision_errors(mixed, c2_state, part):
     assert U == pytest.approx(Uo, rel=1e-6)
 
 
+def _bring_into_contact(S, rl_short, box):
+    """Move one rod of S rigidly towards a neighbour whose segment distance
+    lies between rl_short and 2.6 until they are 1.05 apart (inside the WCA
+    range 2^(1/6)), keeping it at least 0.9 from every other rod.  Returns
+    the moved index or None."""
+    from mdspherocyl_amd import configs
+    N = S.N
+    u = S.ori[:, 2, :]
+    for i in range(N):
+        others = np.arange(N) != i
+        d2 = configs.segdist2(np.repeat(S.r[i:i + 1], N, 0), np.repeat(u[i:i + 1], N, 0), np.full(N, S.len[i]),
+                              S.r, u, S.len, box)
+        cand = np.nonzero(others & (d2 > rl_short ** 2) & (d2 < 2.6 ** 2))[0]
+        for j in cand[:3]:
+            v = S.r[j] - S.r[i]
+            v -= box * np.round(v / box)
+            v /= np.linalg.norm(v)
+            lo, hi = 0.0, 3.0
+            for _ in range(60):
+                mid = 0.5 * (lo + hi)
+                r = S.r[i] + mid * v
+                dij = configs.segdist2(r[None], u[i:i + 1], S.len[i:i + 1], S.r[j:j + 1], u[j:j + 1],
+                                       S.len[j:j + 1], box)[0]
+                lo, hi = (mid, hi) if dij > 1.05 ** 2 else (lo, mid)
+            r = S.r[i] + lo * v
+            dall = configs.segdist2(np.repeat(r[None], N, 0), np.repeat(u[i:i + 1], N, 0), np.full(N, S.len[i]),
+                                    S.r, u, S.len, box)
+            dall[i] = np.inf
+            dall[j] = np.inf
+            if dall.min() > 0.9 ** 2 and lo > 0.3:
+                S.r[i] = r
+                return i
+    return None
+
+
+@pytest.mark.parametrize("moved", [False, True])
+def test_force_sc_mixed_contact_rows(mixed, c2_state, moved):
+    """The mixed path evaluates WCA (in FP64) over contact rows cut from the
+    reference list (segment distance < 2^(1/6) sigma + skin).  With `moved`
+    one rod is carried, without a new list, from outside the contact radius
+    into WCA contact with a neighbour: force_sc must notice the used-up skin,
+    refresh the contact rows and still equal the oracle's force_sc over the
+    same reference list (FP64 tolerance)."""
+    desc, st, S = c2_state
+    rlmin, rlmax = H.list_radii(S, desc.sr_cut)
+    _say("case: mixed force_sc over contact rows, c2 state, moved=%s" % moved)
+    H.load_osys(mixed, M_C2, S)
+    mixed.neigh_list(M_C2, rlmax)
+    off, nbr = mixed.nblist(M_C2)
+    S2 = S.copy()
+    if moved:
+        i = _bring_into_contact(S2, 2 ** (1 / 6) + 0.5, desc.L[0])
+        assert i is not None
+        H.load_osys(mixed, M_C2, S2)
+    mixed.zero_forces()
+    U, W, PT = mixed.force_sc(M_C2, desc.sr_cut)
+    F, T = mixed.get_forces(M_C2)
+    rc, Uo, Wo, PTo, Fo, To = H.oracle_forces(S2.copy(), "sc", lists=(off, nbr), r_cut=desc.sr_cut)
+    assert rc == 0
+    assert U == pytest.approx(Uo, rel=TOL64, abs=1e-13)
+    assert W == pytest.approx(Wo, rel=TOL64, abs=1e-13)
+    assert H.max_rel_row(F, Fo) < TOL64
+    assert H.max_rel_row(T, To) < TOL64
+
+
 def test_force_sc_overlap_reported(msc):
     """Two rods 0.6 sigma apart: force_sc reports the overlap the reference
     aborts on (v25/functions.cpp:1454)."""
