@@ -79,7 +79,7 @@ struct msc_ctx {
   bool mixed = false;
   hipStream_t stream = nullptr;
   int nrep = 0, Ntot = 0, nd = 1;
-  int rchunk = 256;           // particles per structure-factor chunk task
+  int rchunk = 128;           // particles per structure-factor chunk task
   std::vector<msc_replica_desc> desc;
   std::vector<RepDev> reps;
   std::vector<Task> tasks, rtasks;

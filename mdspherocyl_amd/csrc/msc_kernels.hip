@@ -61,8 +61,12 @@ __device__ __forceinline__ void wave_store_sums(double *dst, const double *v, in
   }
 }
 
-// Task of the current wave (4 waves per 256-thread block).
-__device__ __forceinline__ int wave_task_id() { return blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; }
+// Task of the current wave (4 waves per 256-thread block).  The value is the
+// same in every lane; readfirstlane lets the compiler keep it, and all task
+// data derived from it, in scalar registers.
+__device__ __forceinline__ int wave_task_id() {
+  return __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave);
+}
 
 template <typename R>
 __device__ __forceinline__ R fmaR(R a, R b, R c) { return fma(a, b, c); }
@@ -604,18 +608,23 @@ __device__ __forceinline__ Dip<R> load_dip(const State &s, int g, int w) {
   return D;
 }
 
-// Accumulate the real-space interaction of particle g with candidate j
-// (all dipole slot pairs), canonical k = min(g, j).
+// Accumulate the real-space interaction of particle g with candidate j over
+// all dipole slot pairs.  The pair is evaluated from g's side: rij = r_g -
+// r_j and the side-0 formulas of dip_pair give the force and torque on g.
+// (For g the higher index this equals the reference's side-1 expressions
+// bit for bit: every term changes sign exactly under the swap.)  The energy
+// and pressure tensor are counted once, from the lower index.  Dg0 holds g's
+// slot 0 (the only slot in every workload), loaded once per particle.
 template <typename R>
-__device__ __forceinline__ void dipR_partner(const State &s, const RepDev &Rp, const R *__restrict__ tab, int g,
-                                             int j, double F[3], double T[3], double acc[11]) {
+__device__ __forceinline__ void dipR_partner(const State &s, const RepDev &Rp, const R *__restrict__ tab,
+                                             const Dip<R> &Dg0, int g, int j, double F[3], double T[3],
+                                             double acc[11]) {
   const bool gk = g < j;
-  const int k = gk ? g : j, l = gk ? j : g;
   for (int v = 0; v < s.nd; ++v) {
-    Dip<R> Di = load_dip<R>(s, k, v);
+    const Dip<R> Di = s.nd == 1 ? Dg0 : load_dip<R>(s, g, v);
     if (Di.mu == (R)0) continue;
     for (int w = 0; w < s.nd; ++w) {
-      Dip<R> Dj = load_dip<R>(s, l, w);
+      Dip<R> Dj = load_dip<R>(s, j, w);
       if (Dj.mu == (R)0) continue;
       R rij[3];
       for (int a = 0; a < 3; ++a) {
@@ -624,7 +633,7 @@ __device__ __forceinline__ void dipR_partner(const State &s, const RepDev &Rp, c
         rij[a] = (R)d;
       }
       PairOut<R> o;
-      if (!dip_pair<R>(rij, Di.o, Dj.o, Di.mu, Dj.mu, Di.d, Dj.d, tab, (R)Rp.rc, gk ? 0 : 1, gk, o)) continue;
+      if (!dip_pair<R>(rij, Di.o, Dj.o, Di.mu, Dj.mu, Di.d, Dj.d, tab, (R)Rp.rc, 0, gk, o)) continue;
       for (int a = 0; a < 3; ++a) {
         F[a] += (double)o.f[a];
         T[a] += (double)o.t[a];
@@ -651,9 +660,10 @@ __global__ void __launch_bounds__(256) k_dipR_all(State s, const R *__restrict__
   if (lane < tk.n) {
     const int g = tk.g0 + lane;
     double F[3] = {0, 0, 0}, T[3] = {0, 0, 0};
+    const Dip<R> Dg0 = load_dip<R>(s, g, 0);
     for (int j = Rp.n0; j < Rp.n0 + Rp.N; ++j) {
       if (j == g) continue;
-      dipR_partner<R>(s, Rp, tab, g, j, F, T, acc);
+      dipR_partner<R>(s, Rp, tab, Dg0, g, j, F, T, acc);
     }
     for (int a = 0; a < 3; ++a) {
       s.F[a * s.Ntot + g] += F[a];
@@ -676,8 +686,10 @@ __global__ void __launch_bounds__(256) k_dipR_list(State s, const R *__restrict_
   if (lane < tk.n) {
     const int g = tk.g0 + lane;
     double F[3] = {0, 0, 0}, T[3] = {0, 0, 0};
+    const Dip<R> Dg0 = load_dip<R>(s, g, 0);
     const int n = s.dl_cnt[g];
-    for (int q = 0; q < n; ++q) dipR_partner<R>(s, Rp, tab, g, s.dl_idx[row_at(Rp.dl_base, Rp.N, Rp.n0, q, g)], F, T, acc);
+    for (int q = 0; q < n; ++q)
+      dipR_partner<R>(s, Rp, tab, Dg0, g, s.dl_idx[row_at(Rp.dl_base, Rp.N, Rp.n0, q, g)], F, T, acc);
     for (int a = 0; a < 3; ++a) {
       s.F[a * s.Ntot + g] += F[a];
       s.T[a * s.Ntot + g] += T[a];
@@ -857,14 +869,34 @@ __global__ void __launch_bounds__(256) k_recip_struct(State s, int rt0, const Cp
 // energy (val/2)(|S|^2 - k^T M2 k) and the pressure-tensor term of
 // force_dipK (v25/functions.cpp:1926-1952, with its a4 = 1/(4 alpha));
 // A = val conj(S) for the back projection.
+// One block per (64 slots, replica): 16 waves each sum every 16th chunk
+// partial of their 64 slots in FP64 (coalesced rows of 64 slots), then one
+// wave combines the 16 sums in fixed order and finishes the slot.
 template <typename R>
-__global__ void k_recip_reduce(State s, const R *__restrict__ part, const double *__restrict__ M2,
-                               Cpx<R> *__restrict__ Aout, double *__restrict__ slot_out) {
+__global__ void __launch_bounds__(1024) k_recip_reduce(State s, const R *__restrict__ part, const double *__restrict__ M2,
+                                                       Cpx<R> *__restrict__ Aout, double *__restrict__ slot_out) {
+  constexpr int kGroups = 16;
+  __shared__ double sh[kGroups][6][64];
   const int rep = s.rep_begin + blockIdx.y;
   const RepDev &Rp = s.reps[rep];
   const int nslot = Rp.ncol * Rp.KZ;
-  int sl = blockIdx.x * blockDim.x + threadIdx.x;
-  if (sl >= nslot) return;
+  const int lane = threadIdx.x & 63, grp = threadIdx.x >> 6;
+  const int sl = blockIdx.x * 64 + lane;
+  if (blockIdx.x * 64 >= nslot) return;  // whole block idle (uniform)
+  double Q[6] = {0, 0, 0, 0, 0, 0};
+  if (sl < nslot)
+    for (int c = grp; c < Rp.nrtask; c += kGroups) {
+      const R *p = part + Rp.part0 + ((long)c * nslot + sl) * 6;
+      for (int q = 0; q < 6; ++q) Q[q] += (double)p[q];
+    }
+  for (int q = 0; q < 6; ++q) sh[grp][q][lane] = Q[q];
+  __syncthreads();
+  if (grp != 0 || sl >= nslot) return;
+  for (int q = 0; q < 6; ++q) {
+    double t = 0;
+    for (int g2 = 0; g2 < kGroups; ++g2) t += sh[g2][q][lane];
+    Q[q] = t;
+  }
   const int kmax = Rp.kmax, side = 2 * kmax + 1;
   const int col = sl / Rp.KZ, nz = sl % Rp.KZ;
   const int nx = col % side - kmax, ny = col / side - kmax;
@@ -876,11 +908,6 @@ __global__ void k_recip_reduce(State s, const R *__restrict__ part, const double
     for (int q = 0; q < 10; ++q) o[q] = 0.0;
     Aout[Rp.slot0 + sl] = A;
     return;
-  }
-  double Q[6] = {0, 0, 0, 0, 0, 0};
-  for (int c = 0; c < Rp.nrtask; ++c) {
-    const R *p = part + Rp.part0 + ((long)c * nslot + sl) * 6;
-    for (int q = 0; q < 6; ++q) Q[q] += (double)p[q];
   }
   double k[3] = {Rp.P2[0] * nx, Rp.P2[1] * ny, Rp.P2[2] * nz};
   double Sr = k[0] * Q[0] + k[1] * Q[2] + k[2] * Q[4];
@@ -906,66 +933,81 @@ __global__ void k_recip_reduce(State s, const R *__restrict__ part, const double
 
 // Back projection: for each particle, F = -sum_k a_k Im(Z_k) k and
 // T = u x (-mu sum_k Re(Z_k) k + mu^2 G u + s_d F), Z_k = e^{-i k.r} A_k,
-// a_k = mu u.k.  One lane per particle; the nz loop is unrolled with the
-// particle's z phases in registers; column sums are carried in FP64.
+// a_k = mu u.k.  One lane per particle, k visited row by row (ny outer, nx
+// inner).  Within a column (nx, ny) the phase factor x = e_x e_y is common,
+// so the column needs only the three sums S_m = sum_nz nz^m e_z(nz) A(nz),
+// m = 0, 1, 2, with a_k = c_z k_z + (c_x k_x + c_y k_y):
+//   sum a Im Z = c_z P2z Im(x S_1) + p_col Im(x S_0),
+//   sum a k_z Im Z = P2z (c_z P2z Im(x S_2) + p_col Im(x S_1)),
+//   sum Re Z = Re(x S_0),  sum k_z Re Z = P2z Re(x S_1).
+// e_z stays in registers, e_x in LDS (per lane), e_y is read once per row;
+// column sums are carried in FP64.
 template <typename R, int KZ>
 __global__ void __launch_bounds__(256) k_recip_force(State s, const Cpx<R> *__restrict__ ph,
                                                      const R *__restrict__ amp,
                                                      const Cpx<R> *__restrict__ Aall) {
-  // per-lane x/y phase tables, indexed by the (uniform) loop counters
-  __shared__ Cpx<R> sxy[4][2][KZ][kWave];
-  int wv = threadIdx.x / kWave, lane = threadIdx.x & 63;
+  __shared__ Cpx<R> sx[4][KZ][kWave];
+  const int wv = threadIdx.x / kWave, lane = threadIdx.x & 63;
   if (wave_task_id() >= s.ntask) return;
-  int tid = s.task_begin + wave_task_id();
-  Task tk = s.tasks[tid];
+  const int tid = s.task_begin + wave_task_id();
+  const Task tk = s.tasks[tid];
   const RepDev &Rp = s.reps[tk.rep];
   const int N = s.Ntot;
   const int kmax = KZ - 1, side = 2 * kmax + 1;
   const int g = tk.g0 + (lane < tk.n ? lane : 0);
-  for (int n = 0; n < KZ; ++n) {
-    sxy[wv][0][n][lane] = ph[((size_t)0 * kMaxKZ + n) * N + g];
-    sxy[wv][1][n][lane] = ph[((size_t)1 * kMaxKZ + n) * N + g];
-  }
+#pragma unroll
+  for (int n = 0; n < KZ; ++n) sx[wv][n][lane] = ph[((size_t)0 * kMaxKZ + n) * N + g];
   Cpx<R> ez[KZ];
 #pragma unroll
   for (int n = 0; n < KZ; ++n) ez[n] = ph[((size_t)2 * kMaxKZ + n) * N + g];
   const R c0 = amp[g], c1 = amp[N + g], c2 = amp[2 * N + g];
   const Cpx<R> *A = Aall + Rp.slot0;
   const R P2x = (R)Rp.P2[0], P2y = (R)Rp.P2[1], P2z = (R)Rp.P2[2];
+  const R c2z = c2 * P2z;
   double Fx = 0, Fy = 0, Fz = 0, Hx = 0, Hy = 0, Hz = 0;
-  for (int col = 0; col < side * side; ++col) {
-    const int nx = col % side - kmax, ny = col / side - kmax;
-    const int ax = nx < 0 ? -nx : nx, ay = ny < 0 ? -ny : ny;
-    Cpx<R> ex = sxy[wv][0][ax][lane], ey = sxy[wv][1][ay][lane];
-    if (nx < 0) ex.im = -ex.im;
+  for (int iy = 0; iy < side; ++iy) {
+    const int ny = iy - kmax;
+    Cpx<R> ey = ph[((size_t)1 * kMaxKZ + (ny < 0 ? -ny : ny)) * N + g];
     if (ny < 0) ey.im = -ey.im;
-    R xr = ex.re * ey.re - ex.im * ey.im;
-    R xi = fmaR(ex.re, ey.im, ex.im * ey.re);
-    const R kx = P2x * (R)nx, ky = P2y * (R)ny;
-    const R pcol = fmaR(c0, kx, c1 * ky);
-    R Gc = 0, Hc = 0, Gz = 0, Hzc = 0;
-    const Cpx<R> *Ac = A + col * KZ;
+    const R ky = P2y * (R)ny;
+    const R cy = c1 * ky;
+    for (int ix = 0; ix < side; ++ix) {
+      const int nx = ix - kmax;
+      Cpx<R> ex = sx[wv][nx < 0 ? -nx : nx][lane];
+      if (nx < 0) ex.im = -ex.im;
+      const R xr = fmaR(ex.re, ey.re, -ex.im * ey.im);
+      const R xi = fmaR(ex.re, ey.im, ex.im * ey.re);
+      const Cpx<R> *Ac = A + (iy * side + ix) * KZ;
+      R s0r = 0, s0i = 0, s1r = 0, s1i = 0, s2r = 0, s2i = 0;
 #pragma unroll
-    for (int z = 0; z < KZ; ++z) {
-      const Cpx<R> Ak = Ac[z];
-      const R kz = P2z * (R)z;
-      R er = fmaR(xr, ez[z].re, -xi * ez[z].im);
-      R ei = fmaR(xr, ez[z].im, xi * ez[z].re);
-      R zr = fmaR(er, Ak.re, -ei * Ak.im);
-      R zi = fmaR(er, Ak.im, ei * Ak.re);
-      R a = fmaR(c2, kz, pcol);
-      R gg = a * zi;
-      Gc += gg;
-      Gz = fmaR(gg, kz, Gz);
-      Hc += zr;
-      Hzc = fmaR(zr, kz, Hzc);
+      for (int z = 0; z < KZ; ++z) {
+        const Cpx<R> Ak = Ac[z];
+        const R tr = fmaR(ez[z].re, Ak.re, -ez[z].im * Ak.im);
+        const R ti = fmaR(ez[z].re, Ak.im, ez[z].im * Ak.re);
+        s0r += tr;
+        s0i += ti;
+        if (z > 0) {
+          s1r = fmaR((R)z, tr, s1r);
+          s1i = fmaR((R)z, ti, s1i);
+          s2r = fmaR((R)(z * z), tr, s2r);
+          s2i = fmaR((R)(z * z), ti, s2i);
+        }
+      }
+      const R kx = P2x * (R)nx;
+      const R pcol = fmaR(c0, kx, cy);
+      // Re and Im of x S_m
+      const R r0 = fmaR(xr, s0r, -xi * s0i), i0 = fmaR(xr, s0i, xi * s0r);
+      const R r1 = fmaR(xr, s1r, -xi * s1i), i1 = fmaR(xr, s1i, xi * s1r);
+      const R i2 = fmaR(xr, s2i, xi * s2r);
+      const R Gc = fmaR(c2z, i1, pcol * i0);
+      const R Gz = P2z * fmaR(c2z, i2, pcol * i1);
+      Fx = fma((double)Gc, (double)kx, Fx);
+      Fy = fma((double)Gc, (double)ky, Fy);
+      Fz += (double)Gz;
+      Hx = fma((double)r0, (double)kx, Hx);
+      Hy = fma((double)r0, (double)ky, Hy);
+      Hz += (double)(P2z * r1);
     }
-    Fx = fma((double)Gc, (double)kx, Fx);
-    Fy = fma((double)Gc, (double)ky, Fy);
-    Fz += (double)Gz;
-    Hx = fma((double)Hc, (double)kx, Hx);
-    Hy = fma((double)Hc, (double)ky, Hy);
-    Hz += (double)Hzc;
   }
   if (lane >= tk.n) return;
   double F[3] = {-Fx, -Fy, -Fz};
@@ -1378,11 +1420,11 @@ int launch_recip_force(const State &s, bool mixed, int KZ, const void *ph, const
 void launch_recip_reduce(const State &s, bool mixed, const void *part, const double *M2, void *A, double *slot_out,
                          int maxslot, hipStream_t st) {
   if (mixed)
-    MSC_LAUNCH(k_recip_reduce<float>, dim3(nblk(maxslot, 256), s.rep_end - s.rep_begin), dim3(256), 0, st, s,
-                       (const float *)part, M2, (Cpx<float> *)A, slot_out);
+    MSC_LAUNCH(k_recip_reduce<float>, dim3(nblk(maxslot, 64), s.rep_end - s.rep_begin), dim3(1024), 0, st, s,
+               (const float *)part, M2, (Cpx<float> *)A, slot_out);
   else
-    MSC_LAUNCH(k_recip_reduce<double>, dim3(nblk(maxslot, 256), s.rep_end - s.rep_begin), dim3(256), 0, st, s,
-                       (const double *)part, M2, (Cpx<double> *)A, slot_out);
+    MSC_LAUNCH(k_recip_reduce<double>, dim3(nblk(maxslot, 64), s.rep_end - s.rep_begin), dim3(1024), 0, st, s,
+               (const double *)part, M2, (Cpx<double> *)A, slot_out);
 }
 
 void launch_moveP(const State &s, bool last, const Thermo *th, double *task_out, hipStream_t st) {

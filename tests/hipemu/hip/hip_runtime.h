@@ -23,6 +23,7 @@
 #define __forceinline__ inline
 #define __launch_bounds__(...)
 #define __shared__ static
+#define __builtin_amdgcn_readfirstlane(x) (x)
 
 using std::floor;
 using std::sqrt;
