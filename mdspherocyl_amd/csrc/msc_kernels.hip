@@ -940,31 +940,34 @@ __global__ void __launch_bounds__(1024) k_recip_reduce(State s, const R *__restr
 //   sum a Im Z = c_z P2z Im(x S_1) + p_col Im(x S_0),
 //   sum a k_z Im Z = P2z (c_z P2z Im(x S_2) + p_col Im(x S_1)),
 //   sum Re Z = Re(x S_0),  sum k_z Re Z = P2z Re(x S_1).
-// e_z stays in registers, e_x in LDS (per lane), e_y is read once per row;
-// column sums are carried in FP64.
+// e_x and e_z stay in registers, e_y is read once per row; column sums are
+// carried in FP64.
 template <typename R, int KZ>
 __global__ void __launch_bounds__(256) k_recip_force(State s, const Cpx<R> *__restrict__ ph,
                                                      const R *__restrict__ amp,
                                                      const Cpx<R> *__restrict__ Aall) {
-  __shared__ Cpx<R> sx[4][KZ][kWave];
-  const int wv = threadIdx.x / kWave, lane = threadIdx.x & 63;
+  const int lane = threadIdx.x & 63;
   if (wave_task_id() >= s.ntask) return;
   const int tid = s.task_begin + wave_task_id();
   const Task tk = s.tasks[tid];
   const RepDev &Rp = s.reps[tk.rep];
   const int N = s.Ntot;
-  const int kmax = KZ - 1, side = 2 * kmax + 1;
+  constexpr int kmax = KZ - 1, side = 2 * kmax + 1;
   const int g = tk.g0 + (lane < tk.n ? lane : 0);
+  Cpx<R> ex[KZ], ez[KZ];
 #pragma unroll
-  for (int n = 0; n < KZ; ++n) sx[wv][n][lane] = ph[((size_t)0 * kMaxKZ + n) * N + g];
-  Cpx<R> ez[KZ];
-#pragma unroll
-  for (int n = 0; n < KZ; ++n) ez[n] = ph[((size_t)2 * kMaxKZ + n) * N + g];
+  for (int n = 0; n < KZ; ++n) {
+    ex[n] = ph[((size_t)0 * kMaxKZ + n) * N + g];
+    ez[n] = ph[((size_t)2 * kMaxKZ + n) * N + g];
+  }
   const R c0 = amp[g], c1 = amp[N + g], c2 = amp[2 * N + g];
   const Cpx<R> *A = Aall + Rp.slot0;
   const R P2x = (R)Rp.P2[0], P2y = (R)Rp.P2[1], P2z = (R)Rp.P2[2];
   const R c2z = c2 * P2z;
   double Fx = 0, Fy = 0, Fz = 0, Hx = 0, Hy = 0, Hz = 0;
+  Cpx<R> An[KZ];
+#pragma unroll
+  for (int z = 0; z < KZ; ++z) An[z] = A[z];
   for (int iy = 0; iy < side; ++iy) {
     const int ny = iy - kmax;
     Cpx<R> ey = ph[((size_t)1 * kMaxKZ + (ny < 0 ? -ny : ny)) * N + g];
@@ -973,11 +976,19 @@ __global__ void __launch_bounds__(256) k_recip_force(State s, const Cpx<R> *__re
     const R cy = c1 * ky;
     for (int ix = 0; ix < side; ++ix) {
       const int nx = ix - kmax;
-      Cpx<R> ex = sx[wv][nx < 0 ? -nx : nx][lane];
-      if (nx < 0) ex.im = -ex.im;
-      const R xr = fmaR(ex.re, ey.re, -ex.im * ey.im);
-      const R xi = fmaR(ex.re, ey.im, ex.im * ey.re);
-      const Cpx<R> *Ac = A + (iy * side + ix) * KZ;
+      // coefficients of this column (prefetched), then those of the next
+      Cpx<R> Ac[KZ];
+#pragma unroll
+      for (int z = 0; z < KZ; ++z) Ac[z] = An[z];
+      const int next = iy * side + ix + 1;
+      if (next < side * side) {
+#pragma unroll
+        for (int z = 0; z < KZ; ++z) An[z] = A[next * KZ + z];
+      }
+      Cpx<R> exv = ex[nx < 0 ? -nx : nx];
+      if (nx < 0) exv.im = -exv.im;
+      const R xr = fmaR(exv.re, ey.re, -exv.im * ey.im);
+      const R xi = fmaR(exv.re, ey.im, exv.im * ey.re);
       R s0r = 0, s0i = 0, s1r = 0, s1i = 0, s2r = 0, s2i = 0;
 #pragma unroll
       for (int z = 0; z < KZ; ++z) {
