@@ -102,6 +102,7 @@ struct msc_ctx {
   int *d_cell_of = nullptr, *d_cell_cnt = nullptr, *d_cell_start = nullptr, *d_cell_fill = nullptr,
       *d_cell_list = nullptr;
   int *d_cell_tmp = nullptr;
+  double *d_cell_pack = nullptr;
   int *d_nl_cnt = nullptr, *d_nl_idx = nullptr;
   int *d_sl_cnt = nullptr, *d_sl_idx = nullptr;
   int *d_dl_cnt = nullptr, *d_dl_idx = nullptr;
@@ -350,7 +351,7 @@ void refresh_state_ptrs(msc_ctx *ctx) {
   s.snap = ctx->d_snap;
   s.snap_e = ctx->d_snap_e;
   s.cell_of = ctx->d_cell_of; s.cell_cnt = ctx->d_cell_cnt; s.cell_start = ctx->d_cell_start;
-  s.cell_fill = ctx->d_cell_fill; s.cell_tmp = ctx->d_cell_tmp; s.cell_list = ctx->d_cell_list;
+  s.cell_fill = ctx->d_cell_fill; s.cell_tmp = ctx->d_cell_tmp; s.cell_pack = ctx->d_cell_pack; s.cell_list = ctx->d_cell_list;
   s.cell_rep = ctx->d_cell_rep;
   s.nl_cnt = ctx->d_nl_cnt; s.nl_idx = ctx->d_nl_idx;
   s.sl_cnt = ctx->d_sl_cnt; s.sl_idx = ctx->d_sl_idx;
@@ -526,6 +527,7 @@ int msc_create(int device, int precision, int nrep, const msc_replica_desc *reps
   A(&ctx->d_snap_e, 6 * N);
   A(&ctx->d_cell_of, N); A(&ctx->d_cell_list, N);
   A(&ctx->d_cell_tmp, N);
+  A(&ctx->d_cell_pack, (size_t)8 * N);
   A(&ctx->d_nl_cnt, N); A(&ctx->d_sl_cnt, N); A(&ctx->d_dl_cnt, N);
   A(&ctx->d_kval, ctx->nslot_tot);
   A(&ctx->d_slot_out, (size_t)ctx->nslot_tot * 10);
@@ -605,7 +607,7 @@ void msc_destroy(msc_ctx *ctx) {
                   ctx->d_dpc, ctx->d_doc, ctx->d_dmu, ctx->d_dpos, ctx->d_dori, ctx->d_snap, ctx->d_cell_of,
                   ctx->d_cell_cnt, ctx->d_cell_start, ctx->d_cell_fill, ctx->d_cell_list, ctx->d_nl_cnt,
                   ctx->d_nl_idx, ctx->d_dl_cnt, ctx->d_dl_idx, ctx->d_kval, ctx->d_cell_tmp, ctx->d_sl_cnt,
-                  ctx->d_sl_idx, ctx->d_snap_e, ctx->d_srebuild,
+                  ctx->d_sl_idx, ctx->d_snap_e, ctx->d_srebuild, ctx->d_cell_pack,
                   ctx->d_tab, ctx->d_ph, ctx->d_amp, ctx->d_part, ctx->d_A, ctx->d_slot_out, ctx->d_task_out,
                   ctx->d_red, ctx->d_M2, ctx->d_Kin, ctx->d_mom, ctx->d_gs, ctx->d_obs, ctx->d_counts, ctx->d_E,
                   ctx->d_th, ctx->d_flags, ctx->d_rebuild, ctx->d_one, ctx->d_run_gs, ctx->d_run_obs, ctx->d_save};

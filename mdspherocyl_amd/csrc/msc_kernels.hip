@@ -174,6 +174,24 @@ __global__ void k_cells_rank(State s, const int *__restrict__ rebuild) {
   s.cell_list[b + rank] = g;
 }
 
+// centre, axis, length and index of the particle at cell-order position t,
+// packed into one 64-byte record so that a candidate costs one cache line
+__global__ void k_cells_pack(State s, const int *__restrict__ rebuild) {
+  int t = s.g_begin + blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= s.g_end) return;
+  if (!rebuild[s.rep_of[t]]) return;
+  const int g = s.cell_list[t], N = s.Ntot;
+  double *o = s.cell_pack + (size_t)t * 8;
+  o[0] = s.x[g];
+  o[1] = s.x[N + g];
+  o[2] = s.x[2 * N + g];
+  o[3] = s.ori[6 * N + g];
+  o[4] = s.ori[7 * N + g];
+  o[5] = s.ori[8 * N + g];
+  o[6] = s.len[g];
+  o[7] = (double)g;
+}
+
 __device__ __forceinline__ size_t row_at(long base, int N, int n0, int q, int g) {
   return (size_t)base + (size_t)q * (size_t)N + (size_t)(g - n0);
 }
@@ -212,11 +230,12 @@ __global__ void __launch_bounds__(128) k_build_lists(State s, const int *__restr
   if (t >= s.g_end) return;
   int rp = s.rep_of[t];
   if (!rebuild[rp]) return;
-  const int g = s.cell_list[t];
   const RepDev &R = s.reps[rp];
-  const int N = s.Ntot;
-  double rg[3] = {s.x[g], s.x[N + g], s.x[2 * N + g]};
-  double Lg = s.len[g];
+  const double *pg = s.cell_pack + (size_t)t * 8;
+  const double rg[3] = {pg[0], pg[1], pg[2]};
+  const double ug[3] = {pg[3], pg[4], pg[5]};
+  const double Lg = pg[6];
+  const int g = (int)pg[7];
   int ci[3];
   {
     int c = s.cell_of[g] - R.cell0;
@@ -244,14 +263,27 @@ __global__ void __launch_bounds__(128) k_build_lists(State s, const int *__restr
         int c = R.cell0 + (cz * R.nc[1] + cy) * R.nc[0] + cx;
         int b = s.cell_start[c], e = b + s.cell_cnt[c];
         for (int q = b; q < e; ++q) {
-          int j = s.cell_list[q];
+          const double *pj = s.cell_pack + (size_t)q * 8;
+          const int j = (int)pj[7];
           if (j == g) continue;
+          const double rj[3] = {pj[0], pj[1], pj[2]};
           double rgj[3];
-          for (int a = 0; a < 3; ++a) rgj[a] = wrapd(rg[a] - s.x[a * N + j], R.L[a]);
+          for (int a = 0; a < 3; ++a) rgj[a] = wrapd(rg[a] - rj[a], R.L[a]);
           double rc2 = dot3(rgj, rgj);
-          double reach = R.r_list_max + 0.5 * (Lg + s.len[j]) + 1e-9;
+          const double Lj = pj[6];
+          double reach = R.r_list_max + 0.5 * (Lg + Lj) + 1e-9;
           if (rc2 < reach * reach * (1 + 1e-12)) {
-            double d2 = pair_segdist(s, R, g, j);
+            // the pair as neigh_list evaluates it: A = lower index, rAB = r_A - r_B
+            const double uj[3] = {pj[3], pj[4], pj[5]};
+            const bool gA = g < j;
+            const double *rA = gA ? rg : rj, *rB = gA ? rj : rg;
+            const double *uA = gA ? ug : uj, *uB = gA ? uj : ug;
+            double rAB[3], dAB[3], st[2];
+            for (int a = 0; a < 3; ++a) {
+              rAB[a] = rA[a] - rB[a];
+              rAB[a] -= R.L[a] * floor(rAB[a] / R.L[a] + 0.5);
+            }
+            double d2 = segdist(rA, uA, gA ? Lg : Lj, rB, uB, gA ? Lj : Lg, R.L, rAB, dAB, st);
             if (d2 < rl2) {
               if (nw < R.nl_w) s.nl_idx[row_at(R.nl_base, R.N, R.n0, nw, g)] = j;
               ++nw;
@@ -412,26 +444,30 @@ __global__ void __launch_bounds__(256) k_check_decide(State s, const double *__r
 // within r_short, which hold every pair inside 2^(1/6) sigma until the
 // contact skin is used up) are evaluated; the others contribute exactly zero.
 template <bool STRICT>
-__global__ void __launch_bounds__(256) k_force_sc(State s, double *__restrict__ task_out, int *__restrict__ flags) {
-  int tid = s.task_begin + wave_task_id();
-  int lane = threadIdx.x & 63;
-  if (wave_task_id() >= s.ntask) return;
-  Task tk = s.tasks[tid];
+__global__ void __launch_bounds__(64 * kPairSplit) k_force_sc(State s, double *__restrict__ task_out,
+                                                               int *__restrict__ flags) {
+  // one block per wave task; wave sw of the block takes row entries
+  // q = sw (mod kPairSplit); partial sums meet in LDS in a fixed order
+  __shared__ double red[kPairSplit][18][kWave];
+  const int lane = threadIdx.x & 63, sw = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  if ((int)blockIdx.x >= s.ntask) return;
+  const int tid = s.task_begin + blockIdx.x;
+  const Task tk = s.tasks[tid];
   const RepDev &R = s.reps[tk.rep];
   const int N = s.Ntot;
   double acc[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};  // U, W, PT[9], pairs
+  double F[3] = {0, 0, 0}, T[3] = {0, 0, 0};
   if (lane < tk.n) {
     const int g = tk.g0 + lane;
     const double srt = pow(2., 1. / 6);
     const double srtsq = srt * srt;
     const double r_cut_sq = R.sr_cut * R.sr_cut;
-    double F[3] = {0, 0, 0}, T[3] = {0, 0, 0};
     int code = 0;
     double rg[3] = {s.x[g], s.x[N + g], s.x[2 * N + g]};
     const int n = STRICT ? s.nl_cnt[g] : s.sl_cnt[g];
     const int *idx = STRICT ? s.nl_idx : s.sl_idx;
     const long base = STRICT ? R.nl_base : R.sl_base;
-    for (int q = 0; q < n; ++q) {
+    for (int q = sw; q < n; q += kPairSplit) {
       int j = idx[row_at(base, R.N, R.n0, q, g)];
       double sigma = (s.dia[g] + s.dia[j]) / 2;
       double sig_sq = sigma * sigma;
@@ -488,11 +524,28 @@ __global__ void __launch_bounds__(256) k_force_sc(State s, double *__restrict__ 
         }
       }
     }
+    if (code) atomicCAS(&flags[tk.rep * 4 + 0], 0, code);
+  }
+  for (int a = 0; a < 3; ++a) {
+    red[sw][a][lane] = F[a];
+    red[sw][3 + a][lane] = T[a];
+  }
+  for (int q = 0; q < 12; ++q) red[sw][6 + q][lane] = acc[q];
+  __syncthreads();
+  if (sw != 0) return;
+  for (int v = 0; v < 18; ++v) {
+    double t = 0;
+    for (int c = 0; c < kPairSplit; ++c) t += red[c][v][lane];
+    if (v < 3) F[v] = t;
+    else if (v < 6) T[v - 3] = t;
+    else acc[v - 6] = t;
+  }
+  if (lane < tk.n) {
+    const int g = tk.g0 + lane;
     for (int a = 0; a < 3; ++a) {
       s.F[a * N + g] += F[a];
       s.T[a * N + g] += T[a];
     }
-    if (code) atomicCAS(&flags[tk.rep * 4 + 0], 0, code);
   }
   wave_store_sums(task_out + (size_t)tid * 16, acc, 12, lane);
 }
@@ -675,21 +728,41 @@ __global__ void __launch_bounds__(256) k_dipR_all(State s, const R *__restrict__
 
 // cut-off mode over the real-space pair list
 template <typename R>
-__global__ void __launch_bounds__(256) k_dipR_list(State s, const R *__restrict__ tabs, double *__restrict__ task_out) {
-  int tid = s.task_begin + wave_task_id();
-  int lane = threadIdx.x & 63;
-  if (wave_task_id() >= s.ntask) return;
-  Task tk = s.tasks[tid];
+__global__ void __launch_bounds__(64 * kPairSplit) k_dipR_list(State s, const R *__restrict__ tabs,
+                                                                double *__restrict__ task_out) {
+  // one block per wave task, row entries split over its waves as in k_force_sc
+  __shared__ double red[kPairSplit][17][kWave];
+  const int lane = threadIdx.x & 63, sw = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  if ((int)blockIdx.x >= s.ntask) return;
+  const int tid = s.task_begin + blockIdx.x;
+  const Task tk = s.tasks[tid];
   const RepDev &Rp = s.reps[tk.rep];
   const R *tab = tabs + (size_t)Rp.tab0 * 3;
   double acc[11] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  double F[3] = {0, 0, 0}, T[3] = {0, 0, 0};
   if (lane < tk.n) {
     const int g = tk.g0 + lane;
-    double F[3] = {0, 0, 0}, T[3] = {0, 0, 0};
     const Dip<R> Dg0 = load_dip<R>(s, g, 0);
     const int n = s.dl_cnt[g];
-    for (int q = 0; q < n; ++q)
+    for (int q = sw; q < n; q += kPairSplit)
       dipR_partner<R>(s, Rp, tab, Dg0, g, s.dl_idx[row_at(Rp.dl_base, Rp.N, Rp.n0, q, g)], F, T, acc);
+  }
+  for (int a = 0; a < 3; ++a) {
+    red[sw][a][lane] = F[a];
+    red[sw][3 + a][lane] = T[a];
+  }
+  for (int q = 0; q < 11; ++q) red[sw][6 + q][lane] = acc[q];
+  __syncthreads();
+  if (sw != 0) return;
+  for (int v = 0; v < 17; ++v) {
+    double t = 0;
+    for (int c = 0; c < kPairSplit; ++c) t += red[c][v][lane];
+    if (v < 3) F[v] = t;
+    else if (v < 6) T[v - 3] = t;
+    else acc[v - 6] = t;
+  }
+  if (lane < tk.n) {
+    const int g = tk.g0 + lane;
     for (int a = 0; a < 3; ++a) {
       s.F[a * s.Ntot + g] += F[a];
       s.T[a * s.Ntot + g] += T[a];
@@ -940,19 +1013,24 @@ __global__ void __launch_bounds__(1024) k_recip_reduce(State s, const R *__restr
 //   sum a Im Z = c_z P2z Im(x S_1) + p_col Im(x S_0),
 //   sum a k_z Im Z = P2z (c_z P2z Im(x S_2) + p_col Im(x S_1)),
 //   sum Re Z = Re(x S_0),  sum k_z Re Z = P2z Re(x S_1).
-// e_x and e_z stay in registers, e_y is read once per row; column sums are
-// carried in FP64.
+// e_x and e_z stay in (private) registers and e_y is read per column; column
+// sums are carried in FP64.  A task's columns are spread over the waves of
+// a block: at N = 32768 one lane per particle alone gives only half a wave
+// per SIMD.
 template <typename R, int KZ>
-__global__ void __launch_bounds__(256) k_recip_force(State s, const Cpx<R> *__restrict__ ph,
-                                                     const R *__restrict__ amp,
-                                                     const Cpx<R> *__restrict__ Aall) {
-  const int lane = threadIdx.x & 63;
-  if (wave_task_id() >= s.ntask) return;
-  const int tid = s.task_begin + wave_task_id();
+__global__ void __launch_bounds__(64 * kRfGroups) k_recip_force(State s, const Cpx<R> *__restrict__ ph,
+                                                                const R *__restrict__ amp,
+                                                                const Cpx<R> *__restrict__ Aall) {
+  // one block per wave task; wave cg of the block takes every column
+  // c = cg (mod kRfGroups); partial sums meet in LDS in a fixed order
+  __shared__ double red[kRfGroups][6][kWave];
+  const int lane = threadIdx.x & 63, cg = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  if ((int)blockIdx.x >= s.ntask) return;
+  const int tid = s.task_begin + blockIdx.x;
   const Task tk = s.tasks[tid];
   const RepDev &Rp = s.reps[tk.rep];
   const int N = s.Ntot;
-  constexpr int kmax = KZ - 1, side = 2 * kmax + 1;
+  constexpr int kmax = KZ - 1, side = 2 * kmax + 1, ncol = side * side;
   const int g = tk.g0 + (lane < tk.n ? lane : 0);
   Cpx<R> ex[KZ], ez[KZ];
 #pragma unroll
@@ -965,64 +1043,75 @@ __global__ void __launch_bounds__(256) k_recip_force(State s, const Cpx<R> *__re
   const R P2x = (R)Rp.P2[0], P2y = (R)Rp.P2[1], P2z = (R)Rp.P2[2];
   const R c2z = c2 * P2z;
   double Fx = 0, Fy = 0, Fz = 0, Hx = 0, Hy = 0, Hz = 0;
+  // the coefficients of the next column are loaded while this one is used
   Cpx<R> An[KZ];
+  if (cg < ncol) {
 #pragma unroll
-  for (int z = 0; z < KZ; ++z) An[z] = A[z];
-  for (int iy = 0; iy < side; ++iy) {
-    const int ny = iy - kmax;
+    for (int z = 0; z < KZ; ++z) An[z] = A[cg * KZ + z];
+  }
+  for (int col = cg; col < ncol; col += kRfGroups) {
+    const int iy = col / side, ix = col - iy * side;
+    const int nx = ix - kmax, ny = iy - kmax;
+    Cpx<R> Ac[KZ];
+#pragma unroll
+    for (int z = 0; z < KZ; ++z) Ac[z] = An[z];
+    const int next = col + kRfGroups;
+    if (next < ncol) {
+#pragma unroll
+      for (int z = 0; z < KZ; ++z) An[z] = A[next * KZ + z];
+    }
     Cpx<R> ey = ph[((size_t)1 * kMaxKZ + (ny < 0 ? -ny : ny)) * N + g];
     if (ny < 0) ey.im = -ey.im;
-    const R ky = P2y * (R)ny;
-    const R cy = c1 * ky;
-    for (int ix = 0; ix < side; ++ix) {
-      const int nx = ix - kmax;
-      // coefficients of this column (prefetched), then those of the next
-      Cpx<R> Ac[KZ];
+    Cpx<R> exv = ex[nx < 0 ? -nx : nx];
+    if (nx < 0) exv.im = -exv.im;
+    const R xr = fmaR(exv.re, ey.re, -exv.im * ey.im);
+    const R xi = fmaR(exv.re, ey.im, exv.im * ey.re);
+    R s0r = 0, s0i = 0, s1r = 0, s1i = 0, s2r = 0, s2i = 0;
 #pragma unroll
-      for (int z = 0; z < KZ; ++z) Ac[z] = An[z];
-      const int next = iy * side + ix + 1;
-      if (next < side * side) {
-#pragma unroll
-        for (int z = 0; z < KZ; ++z) An[z] = A[next * KZ + z];
+    for (int z = 0; z < KZ; ++z) {
+      const Cpx<R> Ak = Ac[z];
+      const R tr = fmaR(ez[z].re, Ak.re, -ez[z].im * Ak.im);
+      const R ti = fmaR(ez[z].re, Ak.im, ez[z].im * Ak.re);
+      s0r += tr;
+      s0i += ti;
+      if (z > 0) {
+        s1r = fmaR((R)z, tr, s1r);
+        s1i = fmaR((R)z, ti, s1i);
+        s2r = fmaR((R)(z * z), tr, s2r);
+        s2i = fmaR((R)(z * z), ti, s2i);
       }
-      Cpx<R> exv = ex[nx < 0 ? -nx : nx];
-      if (nx < 0) exv.im = -exv.im;
-      const R xr = fmaR(exv.re, ey.re, -exv.im * ey.im);
-      const R xi = fmaR(exv.re, ey.im, exv.im * ey.re);
-      R s0r = 0, s0i = 0, s1r = 0, s1i = 0, s2r = 0, s2i = 0;
-#pragma unroll
-      for (int z = 0; z < KZ; ++z) {
-        const Cpx<R> Ak = Ac[z];
-        const R tr = fmaR(ez[z].re, Ak.re, -ez[z].im * Ak.im);
-        const R ti = fmaR(ez[z].re, Ak.im, ez[z].im * Ak.re);
-        s0r += tr;
-        s0i += ti;
-        if (z > 0) {
-          s1r = fmaR((R)z, tr, s1r);
-          s1i = fmaR((R)z, ti, s1i);
-          s2r = fmaR((R)(z * z), tr, s2r);
-          s2i = fmaR((R)(z * z), ti, s2i);
-        }
-      }
-      const R kx = P2x * (R)nx;
-      const R pcol = fmaR(c0, kx, cy);
-      // Re and Im of x S_m
-      const R r0 = fmaR(xr, s0r, -xi * s0i), i0 = fmaR(xr, s0i, xi * s0r);
-      const R r1 = fmaR(xr, s1r, -xi * s1i), i1 = fmaR(xr, s1i, xi * s1r);
-      const R i2 = fmaR(xr, s2i, xi * s2r);
-      const R Gc = fmaR(c2z, i1, pcol * i0);
-      const R Gz = P2z * fmaR(c2z, i2, pcol * i1);
-      Fx = fma((double)Gc, (double)kx, Fx);
-      Fy = fma((double)Gc, (double)ky, Fy);
-      Fz += (double)Gz;
-      Hx = fma((double)r0, (double)kx, Hx);
-      Hy = fma((double)r0, (double)ky, Hy);
-      Hz += (double)(P2z * r1);
     }
+    const R kx = P2x * (R)nx, ky = P2y * (R)ny;
+    const R pcol = fmaR(c0, kx, c1 * ky);
+    // Re and Im of x S_m
+    const R r0 = fmaR(xr, s0r, -xi * s0i), i0 = fmaR(xr, s0i, xi * s0r);
+    const R r1 = fmaR(xr, s1r, -xi * s1i), i1 = fmaR(xr, s1i, xi * s1r);
+    const R i2 = fmaR(xr, s2i, xi * s2r);
+    const R Gc = fmaR(c2z, i1, pcol * i0);
+    const R Gz = P2z * fmaR(c2z, i2, pcol * i1);
+    Fx = fma((double)Gc, (double)kx, Fx);
+    Fy = fma((double)Gc, (double)ky, Fy);
+    Fz += (double)Gz;
+    Hx = fma((double)r0, (double)kx, Hx);
+    Hy = fma((double)r0, (double)ky, Hy);
+    Hz += (double)(P2z * r1);
   }
-  if (lane >= tk.n) return;
-  double F[3] = {-Fx, -Fy, -Fz};
-  double H[3] = {Hx, Hy, Hz};
+  red[cg][0][lane] = Fx;
+  red[cg][1][lane] = Fy;
+  red[cg][2][lane] = Fz;
+  red[cg][3][lane] = Hx;
+  red[cg][4][lane] = Hy;
+  red[cg][5][lane] = Hz;
+  __syncthreads();
+  if (cg != 0 || lane >= tk.n) return;
+  double tot[6];
+  for (int q = 0; q < 6; ++q) {
+    double t = 0;
+    for (int c = 0; c < kRfGroups; ++c) t += red[c][q][lane];
+    tot[q] = t;
+  }
+  double F[3] = {-tot[0], -tot[1], -tot[2]};
+  double H[3] = {tot[3], tot[4], tot[5]};
   double mu = s.dmu[g];
   double u[3] = {s.dori[g], s.dori[N + g], s.dori[2 * N + g]};
   double sd = s.dpc[g];
@@ -1327,6 +1416,7 @@ void launch_cells(const State &s, const int *rebuild, hipStream_t st) {
   MSC_LAUNCH(k_scan_cells, dim3(s.nrep), dim3(1024), 0, st, s.cell_cnt, s.cell_start, s.reps, rebuild);
   MSC_LAUNCH(k_cells_fill, dim3(nblk(s.g_end - s.g_begin, 256)), dim3(256), 0, st, s, rebuild);
   MSC_LAUNCH(k_cells_rank, dim3(nblk(s.g_end - s.g_begin, 256)), dim3(256), 0, st, s, rebuild);
+  MSC_LAUNCH(k_cells_pack, dim3(nblk(s.g_end - s.g_begin, 256)), dim3(256), 0, st, s, rebuild);
 }
 void launch_lists(const State &s, const int *rebuild, int *flags, hipStream_t st) {
   MSC_LAUNCH(k_build_lists, dim3(nblk(s.g_end - s.g_begin, 128)), dim3(128), 0, st, s, rebuild, flags);
@@ -1343,21 +1433,22 @@ void launch_check(const State &s, double *task_out, int *rebuild, int *srebuild,
 }
 void launch_force_sc(const State &s, bool strict, double *task_out, int *flags, hipStream_t st) {
   if (strict)
-    MSC_LAUNCH(k_force_sc<true>, dim3(nblk(s.ntask, 4)), dim3(256), 0, st, s, task_out, flags);
+    MSC_LAUNCH(k_force_sc<true>, dim3(s.ntask), dim3(64 * kPairSplit), 0, st, s, task_out, flags);
   else
-    MSC_LAUNCH(k_force_sc<false>, dim3(nblk(s.ntask, 4)), dim3(256), 0, st, s, task_out, flags);
+    MSC_LAUNCH(k_force_sc<false>, dim3(s.ntask), dim3(64 * kPairSplit), 0, st, s, task_out, flags);
 }
 void launch_dipR(const State &s, bool mixed, bool all_pairs, const void *tab, double *task_out, hipStream_t st) {
   if (mixed) {
     if (all_pairs)
       MSC_LAUNCH(k_dipR_all<float>, dim3(nblk(s.ntask, 4)), dim3(256), 0, st, s, (const float *)tab, task_out);
     else
-      MSC_LAUNCH(k_dipR_list<float>, dim3(nblk(s.ntask, 4)), dim3(256), 0, st, s, (const float *)tab, task_out);
+      MSC_LAUNCH(k_dipR_list<float>, dim3(s.ntask), dim3(64 * kPairSplit), 0, st, s, (const float *)tab, task_out);
   } else {
     if (all_pairs)
       MSC_LAUNCH(k_dipR_all<double>, dim3(nblk(s.ntask, 4)), dim3(256), 0, st, s, (const double *)tab, task_out);
     else
-      MSC_LAUNCH(k_dipR_list<double>, dim3(nblk(s.ntask, 4)), dim3(256), 0, st, s, (const double *)tab, task_out);
+      MSC_LAUNCH(k_dipR_list<double>, dim3(s.ntask), dim3(64 * kPairSplit), 0, st, s, (const double *)tab,
+                 task_out);
   }
 }
 void launch_ext_field(const State &s, double *task_out, hipStream_t st) {
@@ -1394,8 +1485,8 @@ static void recip_struct_kz(const State &s, int rt0, int nrt, int maxcol, const 
 }
 template <typename R, int KZ>
 static void recip_force_kz(const State &s, const void *ph, const void *amp, const void *A, hipStream_t st) {
-  MSC_LAUNCH((k_recip_force<R, KZ>), dim3(nblk(s.ntask, 4)), dim3(256), 0, st, s, (const Cpx<R> *)ph,
-                     (const R *)amp, (const Cpx<R> *)A);
+  MSC_LAUNCH((k_recip_force<R, KZ>), dim3(s.ntask), dim3(64 * kRfGroups), 0, st, s, (const Cpx<R> *)ph,
+             (const R *)amp, (const Cpx<R> *)A);
 }
 
 #define MSC_KZ_CASES(X) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16)

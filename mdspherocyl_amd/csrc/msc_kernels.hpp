@@ -27,6 +27,7 @@ struct State {
   double *snap_e;                    // 6N centre and axis when the contact list was built
   // cell grid
   int *cell_of, *cell_cnt, *cell_start, *cell_fill, *cell_tmp, *cell_list;
+  double *cell_pack;         // [Ntot][8] centre, axis, length, index in cell order
   const int *cell_rep;       // [ncells] owning replica
   // neighbour rows (layout in RepDev): reference list, contact list,
   // real-space dipole pairs; counts per particle
@@ -68,6 +69,8 @@ void launch_zero_FT(const State &s, hipStream_t st);
 void launch_fill_int(int *p, int n, int v, hipStream_t st);
 
 constexpr int kChunk = 64;  // particles per reciprocal structure-factor chunk
-constexpr int kRedStride = 16;  // doubles per replica in the reduction scratch d_red
+constexpr int kRedStride = 16;
+constexpr int kRfGroups = 8;    // waves sharing one task in the reciprocal back projection
+constexpr int kPairSplit = 4;   // waves sharing one task in the WCA and real-space pair kernels  // doubles per replica in the reduction scratch d_red
 
 }  // namespace msc
