@@ -213,29 +213,38 @@ __device__ __forceinline__ double pair_segdist(const State &s, const RepDev &R, 
   return segdist(rA, uA, s.len[A], rB, uB, s.len[Bi], R.L, rAB, dAB, st);
 }
 
-// Neighbour rows from the cell grid in one pass.  Thread t takes the t-th
-// particle in cell order, so the lanes of a wave share their candidate
-// cells.  For particle g every candidate j of the same replica in the 27 (or
-// fewer) surrounding cells is tested:
-//  - reference list: segment distance < r_list_max, evaluated exactly as
-//    neigh_list does for the pair (min(g,j), max(g,j)) (v25/functions.cpp:1310-1331);
-//    a centre-distance bound skips pairs that cannot qualify;
-//  - contact list: segment distance < r_short (a subset of the row above);
-//  - real-space list (cut-off mode): dipole-pair centre distance < rl_dip.
-// Rows hold both directions in stencil order.  A row longer than its
-// capacity is cut and raises flags[rep*4 + 2 / 1 / 3] to the length needed
-// (a layout pass runs with capacity 0 to measure the longest rows).
-__global__ void __launch_bounds__(128) k_build_lists(State s, const int *__restrict__ rebuild, int *__restrict__ flags) {
-  int t = s.g_begin + blockIdx.x * blockDim.x + threadIdx.x;
+// Neighbour rows from the cell grid in one pass, one wave per particle
+// (particles taken in cell order).  The lanes walk the candidates j of the
+// same replica in the 27 (or fewer) surrounding cells, 64 at a time:
+//  - real-space list (cut-off mode): dipole-pair centre distance < rl_dip,
+//    written at once;
+//  - candidates whose centre distance allows a segment distance below
+//    r_list_max go to a per-wave queue in LDS; every 64 queued candidates
+//    run the segment distance together (dense lanes), evaluated exactly as
+//    neigh_list does for the pair (min(g,j), max(g,j)) (v25/functions.cpp:1310-1331):
+//    reference list d < r_list_max, contact list d < r_short.
+// Accepted entries are compacted with ballots in candidate order, so a row
+// holds both directions in stencil order, as a lane-per-particle build
+// would write it.  A row longer than its capacity is cut and raises
+// flags[rep*4 + 2 / 1 / 3] to the length needed (a layout pass runs with
+// capacity 0 to measure the longest rows).
+__device__ __forceinline__ int lane_prefix(unsigned long long m, int lane) {
+  return __popcll(m & ((1ull << lane) - 1ull));
+}
+
+__global__ void __launch_bounds__(256) k_build_lists(State s, const int *__restrict__ rebuild, int *__restrict__ flags) {
+  __shared__ int queue[4][128];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int t = s.g_begin + __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + wv);
   if (t >= s.g_end) return;
-  int rp = s.rep_of[t];
+  const int rp = __builtin_amdgcn_readfirstlane(s.rep_of[t]);
   if (!rebuild[rp]) return;
   const RepDev &R = s.reps[rp];
   const double *pg = s.cell_pack + (size_t)t * 8;
   const double rg[3] = {pg[0], pg[1], pg[2]};
   const double ug[3] = {pg[3], pg[4], pg[5]};
   const double Lg = pg[6];
-  const int g = (int)pg[7];
+  const int g = __builtin_amdgcn_readfirstlane((int)pg[7]);
   int ci[3];
   {
     int c = s.cell_of[g] - R.cell0;
@@ -247,7 +256,41 @@ __global__ void __launch_bounds__(128) k_build_lists(State s, const int *__restr
   const double rs2 = R.r_short * R.r_short;
   const bool dip = R.rc > 0;
   const double reachd = R.rl_dip + 2.0 * s.lever_max;
-  int nw = 0, ns = 0, ndl = 0;
+  int *Q = queue[wv];
+  int nw = 0, ns = 0, ndl = 0, qn = 0;
+  // segment distances of the first `count` queued candidates
+  auto flush = [&](int count) {
+    const bool act = lane < count;
+    const int q = act ? Q[lane] : t;
+    const double *pj = s.cell_pack + (size_t)q * 8;
+    const int j = (int)pj[7];
+    const double rj[3] = {pj[0], pj[1], pj[2]};
+    const double uj[3] = {pj[3], pj[4], pj[5]};
+    const double Lj = pj[6];
+    double d2 = rl2;
+    if (act) {
+      // the pair as neigh_list evaluates it: A = lower index, rAB = r_A - r_B
+      const bool gA = g < j;
+      const double *rA = gA ? rg : rj, *rB = gA ? rj : rg;
+      const double *uA = gA ? ug : uj, *uB = gA ? uj : ug;
+      double rAB[3], dAB[3], st[2];
+      for (int a = 0; a < 3; ++a) {
+        rAB[a] = rA[a] - rB[a];
+        rAB[a] -= R.L[a] * floor(rAB[a] / R.L[a] + 0.5);
+      }
+      d2 = segdist(rA, uA, gA ? Lg : Lj, rB, uB, gA ? Lj : Lg, R.L, rAB, dAB, st);
+    }
+    const bool inl = act && d2 < rl2;
+    const bool ins = inl && d2 < rs2;
+    unsigned long long m = __ballot(inl);
+    int pos = nw + lane_prefix(m, lane);
+    if (inl && pos < R.nl_w) s.nl_idx[row_at(R.nl_base, R.N, R.n0, pos, g)] = j;
+    nw += __popcll(m);
+    m = __ballot(ins);
+    pos = ns + lane_prefix(m, lane);
+    if (ins && pos < R.sl_w) s.sl_idx[row_at(R.sl_base, R.N, R.n0, pos, g)] = j;
+    ns += __popcll(m);
+  };
   int lo[3], hi[3];
   for (int a = 0; a < 3; ++a) {
     if (R.nc[a] >= 3) { lo[a] = -1; hi[a] = 1; }
@@ -260,53 +303,58 @@ __global__ void __launch_bounds__(128) k_build_lists(State s, const int *__restr
         cx = cx < 0 ? cx + R.nc[0] : (cx >= R.nc[0] ? cx - R.nc[0] : cx);
         cy = cy < 0 ? cy + R.nc[1] : (cy >= R.nc[1] ? cy - R.nc[1] : cy);
         cz = cz < 0 ? cz + R.nc[2] : (cz >= R.nc[2] ? cz - R.nc[2] : cz);
-        int c = R.cell0 + (cz * R.nc[1] + cy) * R.nc[0] + cx;
-        int b = s.cell_start[c], e = b + s.cell_cnt[c];
-        for (int q = b; q < e; ++q) {
-          const double *pj = s.cell_pack + (size_t)q * 8;
+        const int c = R.cell0 + (cz * R.nc[1] + cy) * R.nc[0] + cx;
+        const int b = s.cell_start[c], e = b + s.cell_cnt[c];
+        for (int q0 = b; q0 < e; q0 += 64) {
+          const int q = q0 + lane;
+          bool act = q < e;
+          const double *pj = s.cell_pack + (size_t)(act ? q : t) * 8;
           const int j = (int)pj[7];
-          if (j == g) continue;
-          const double rj[3] = {pj[0], pj[1], pj[2]};
+          act = act && j != g;
           double rgj[3];
-          for (int a = 0; a < 3; ++a) rgj[a] = wrapd(rg[a] - rj[a], R.L[a]);
-          double rc2 = dot3(rgj, rgj);
-          const double Lj = pj[6];
-          double reach = R.r_list_max + 0.5 * (Lg + Lj) + 1e-9;
-          if (rc2 < reach * reach * (1 + 1e-12)) {
-            // the pair as neigh_list evaluates it: A = lower index, rAB = r_A - r_B
-            const double uj[3] = {pj[3], pj[4], pj[5]};
-            const bool gA = g < j;
-            const double *rA = gA ? rg : rj, *rB = gA ? rj : rg;
-            const double *uA = gA ? ug : uj, *uB = gA ? uj : ug;
-            double rAB[3], dAB[3], st[2];
-            for (int a = 0; a < 3; ++a) {
-              rAB[a] = rA[a] - rB[a];
-              rAB[a] -= R.L[a] * floor(rAB[a] / R.L[a] + 0.5);
-            }
-            double d2 = segdist(rA, uA, gA ? Lg : Lj, rB, uB, gA ? Lj : Lg, R.L, rAB, dAB, st);
-            if (d2 < rl2) {
-              if (nw < R.nl_w) s.nl_idx[row_at(R.nl_base, R.N, R.n0, nw, g)] = j;
-              ++nw;
-              if (d2 < rs2) {
-                if (ns < R.sl_w) s.sl_idx[row_at(R.sl_base, R.N, R.n0, ns, g)] = j;
-                ++ns;
-              }
-            }
-          }
+          for (int a = 0; a < 3; ++a) rgj[a] = wrapd(rg[a] - pj[a], R.L[a]);
+          const double rc2 = dot3(rgj, rgj);
+          const double reach = R.r_list_max + 0.5 * (Lg + pj[6]) + 1e-9;
+          const bool pass = act && rc2 < reach * reach * (1 + 1e-12);
           // pairs whose dipoles can come within rc before the next rebuild
-          if (dip && rc2 < reachd * reachd) {
-            if (ndl < R.dl_w) s.dl_idx[row_at(R.dl_base, R.N, R.n0, ndl, g)] = j;
-            ++ndl;
+          if (dip) {
+            const bool dp = act && rc2 < reachd * reachd;
+            const unsigned long long m = __ballot(dp);
+            const int pos = ndl + lane_prefix(m, lane);
+            if (dp && pos < R.dl_w) s.dl_idx[row_at(R.dl_base, R.N, R.n0, pos, g)] = j;
+            ndl += __popcll(m);
+          }
+          const unsigned long long m = __ballot(pass);
+          if (pass) Q[qn + lane_prefix(m, lane)] = q;
+          qn += __popcll(m);
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+          __builtin_amdgcn_wave_barrier();
+          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+          if (qn >= 64) {
+            flush(64);
+            const int rest = qn - 64;
+            const int keep = lane < rest ? Q[64 + lane] : 0;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            if (lane < rest) Q[lane] = keep;
+            qn = rest;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
           }
         }
       }
-  s.nl_cnt[g] = min(nw, R.nl_w);
-  s.sl_cnt[g] = min(ns, R.sl_w);
-  if (nw > R.nl_w) atomicMax(&flags[rp * 4 + 2], nw);
-  if (ns > R.sl_w) atomicMax(&flags[rp * 4 + 1], ns);
-  if (dip) {
-    s.dl_cnt[g] = min(ndl, R.dl_w);
-    if (ndl > R.dl_w) atomicMax(&flags[rp * 4 + 3], ndl);
+  if (qn > 0) flush(qn);
+  if (lane == 0) {
+    s.nl_cnt[g] = min(nw, R.nl_w);
+    s.sl_cnt[g] = min(ns, R.sl_w);
+    if (nw > R.nl_w) atomicMax(&flags[rp * 4 + 2], nw);
+    if (ns > R.sl_w) atomicMax(&flags[rp * 4 + 1], ns);
+    if (dip) {
+      s.dl_cnt[g] = min(ndl, R.dl_w);
+      if (ndl > R.dl_w) atomicMax(&flags[rp * 4 + 3], ndl);
+    }
   }
 }
 
@@ -1419,7 +1467,7 @@ void launch_cells(const State &s, const int *rebuild, hipStream_t st) {
   MSC_LAUNCH(k_cells_pack, dim3(nblk(s.g_end - s.g_begin, 256)), dim3(256), 0, st, s, rebuild);
 }
 void launch_lists(const State &s, const int *rebuild, int *flags, hipStream_t st) {
-  MSC_LAUNCH(k_build_lists, dim3(nblk(s.g_end - s.g_begin, 128)), dim3(128), 0, st, s, rebuild, flags);
+  MSC_LAUNCH(k_build_lists, dim3(nblk(s.g_end - s.g_begin, 4)), dim3(256), 0, st, s, rebuild, flags);
   MSC_LAUNCH(k_savec, dim3(nblk(s.g_end - s.g_begin, 256)), dim3(256), 0, st, s, rebuild);
 }
 void launch_short(const State &s, const int *srebuild, int *flags, hipStream_t st) {

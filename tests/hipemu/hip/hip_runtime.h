@@ -69,6 +69,28 @@ inline T __shfl_xor(T v, int o) {
   return (T)r;
 }
 
+// all 64 lanes of a wave take part (the kernels only call these from
+// wave-uniform control flow)
+inline unsigned long long __ballot(int pred) {
+  hipemu::Block *b = hipemu::cur_block();
+  unsigned t = hipemu::t_threadIdx.x, w = t / 64, lane = t % 64;
+  double *x = b->xch.data() + w * 64;
+  x[lane] = pred ? 1.0 : 0.0;
+  b->wbar[w]->arrive_and_wait();
+  unsigned long long m = 0;
+  for (unsigned i = 0; i < 64; ++i)
+    if (x[i] != 0.0) m |= 1ull << i;
+  b->wbar[w]->arrive_and_wait();
+  return m;
+}
+inline int __popcll(unsigned long long m) { return __builtin_popcountll(m); }
+inline void hipemu_wave_barrier() {
+  hipemu::Block *b = hipemu::cur_block();
+  b->wbar[hipemu::t_threadIdx.x / 64]->arrive_and_wait();
+}
+#define __builtin_amdgcn_wave_barrier() hipemu_wave_barrier()
+#define __builtin_amdgcn_fence(...) ((void)0)
+
 inline int atomicAdd(int *p, int v) { return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST); }
 inline int atomicMax(int *p, int v) {
   int old = __atomic_load_n(p, __ATOMIC_SEQ_CST);
