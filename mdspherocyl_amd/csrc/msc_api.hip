@@ -110,7 +110,7 @@ struct msc_ctx {
   double *d_kval = nullptr;
   void *d_tab = nullptr, *d_ph = nullptr, *d_amp = nullptr, *d_part = nullptr, *d_A = nullptr;
   double *d_slot_out = nullptr, *d_task_out = nullptr, *d_red = nullptr, *d_M2 = nullptr, *d_Kin = nullptr,
-         *d_mom = nullptr, *d_gs = nullptr, *d_obs = nullptr, *d_counts = nullptr;
+         *d_gs = nullptr, *d_obs = nullptr, *d_counts = nullptr;
   RepEnergy *d_E = nullptr;
   Thermo *d_th = nullptr;
   int *d_flags = nullptr, *d_rebuild = nullptr, *d_srebuild = nullptr, *d_one = nullptr;
@@ -535,7 +535,6 @@ int msc_create(int device, int precision, int nrep, const msc_replica_desc *reps
   A(&ctx->d_red, (size_t)nrep * kRedStride);
   A(&ctx->d_M2, (size_t)nrep * 6);
   A(&ctx->d_Kin, (size_t)nrep * 2);
-  A(&ctx->d_mom, (size_t)nrep * 8);
   A(&ctx->d_gs, (size_t)nrep * 4);
   A(&ctx->d_obs, (size_t)nrep * MSC_OBS_N);
   A(&ctx->d_counts, (size_t)nrep * 4);
@@ -609,7 +608,7 @@ void msc_destroy(msc_ctx *ctx) {
                   ctx->d_nl_idx, ctx->d_dl_cnt, ctx->d_dl_idx, ctx->d_kval, ctx->d_cell_tmp, ctx->d_sl_cnt,
                   ctx->d_sl_idx, ctx->d_snap_e, ctx->d_srebuild, ctx->d_cell_pack,
                   ctx->d_tab, ctx->d_ph, ctx->d_amp, ctx->d_part, ctx->d_A, ctx->d_slot_out, ctx->d_task_out,
-                  ctx->d_red, ctx->d_M2, ctx->d_Kin, ctx->d_mom, ctx->d_gs, ctx->d_obs, ctx->d_counts, ctx->d_E,
+                  ctx->d_red, ctx->d_M2, ctx->d_Kin, ctx->d_gs, ctx->d_obs, ctx->d_counts, ctx->d_E,
                   ctx->d_th, ctx->d_flags, ctx->d_rebuild, ctx->d_one, ctx->d_run_gs, ctx->d_run_obs, ctx->d_save};
   for (void *b : bufs)
     if (b) hipFree(b);
@@ -925,8 +924,8 @@ static void do_force_sc(msc_ctx *ctx, int rep) {
     Bracket b(ctx, "wca");
     launch_force_sc(s, !ctx->mixed, ctx->d_task_out, ctx->d_flags, ctx->stream);
   }
-  launch_reduce(s, ctx->d_task_out, 16, 11, 0, (double *)ctx->d_E, kEstride, kEoff_sc, ctx->stream);
-  launch_reduce(s, ctx->d_task_out + 11, 16, 1, 0, ctx->d_counts, 4, 0, ctx->stream);
+  launch_reduce(s, ctx->d_task_out, 16, 11, 0, (double *)ctx->d_E, kEstride, kEoff_sc, ctx->stream, 1, ctx->d_counts,
+                4, 0);
 }
 
 int msc_force_sc(msc_ctx *ctx, int rep, double r_cut, double *U, double *W, double PT[9]) {
@@ -968,8 +967,8 @@ static void do_dipR(msc_ctx *ctx, int rep) {
       }
     }
   }
-  launch_reduce(s, ctx->d_task_out, 16, 10, 0, (double *)ctx->d_E, kEstride, kEoff_R, ctx->stream);
-  launch_reduce(s, ctx->d_task_out + 10, 16, 1, 0, ctx->d_counts, 4, 1, ctx->stream);
+  launch_reduce(s, ctx->d_task_out, 16, 10, 0, (double *)ctx->d_E, kEstride, kEoff_R, ctx->stream, 1, ctx->d_counts,
+                4, 1);
 }
 
 int msc_force_dipR(msc_ctx *ctx, int rep, double *U, double PT[9]) {
@@ -1128,8 +1127,14 @@ static void forces_all(msc_ctx *ctx) {
   do_force_sc(ctx, MSC_ALL);
   do_dipR(ctx, MSC_ALL);
   do_dipK(ctx, MSC_ALL);
-  launch_ext_field(s, ctx->d_task_out, ctx->stream);
-  launch_reduce(s, ctx->d_task_out, 16, 1, 0, (double *)ctx->d_E, kEstride, kEoff_ext, ctx->stream);
+  // ext_field adds exactly nothing without a field: skipped then (U_ext
+  // stays 0)
+  bool field = false;
+  for (auto &R : ctx->reps) field |= R.B[0] != 0 || R.B[1] != 0 || R.B[2] != 0;
+  if (field) {
+    launch_ext_field(s, ctx->d_task_out, ctx->stream);
+    launch_reduce(s, ctx->d_task_out, 16, 1, 0, (double *)ctx->d_E, kEstride, kEoff_ext, ctx->stream);
+  }
 }
 
 int msc_init_run(msc_ctx *ctx) {
@@ -1168,14 +1173,13 @@ static void enqueue_step(msc_ctx *ctx, int step, const double *d_gs_step, double
   State s = sel(ctx, MSC_ALL);
   hipStream_t st = ctx->stream;
   if (std::fmod((float)step, (float)100) == 0) launch_renorm(s, st);
-  launch_thermo(s, ctx->d_th, ctx->d_Kin, 0, d_gs_step, st);
+  launch_thermo0(s, ctx->d_th, d_gs_step, st);
   {
     Bracket b(ctx, "integrate");
     launch_moveP(s, false, ctx->d_th, ctx->d_task_out, st);
   }
   launch_reduce(s, ctx->d_task_out, 16, 8, 0, ctx->d_red, kRedStride, 0, st);
-  launch_kin_finish(s, ctx->d_red, ctx->d_Kin, st);
-  launch_thermo(s, ctx->d_th, ctx->d_Kin, 1, d_gs_step, st);
+  launch_step_scalars(s, 1, ctx->d_red, ctx->d_Kin, ctx->d_th, ctx->d_E, nullptr, st);
   {
     Bracket b(ctx, "integrate");
     launch_moveX(s, st);
@@ -1186,12 +1190,8 @@ static void enqueue_step(msc_ctx *ctx, int step, const double *d_gs_step, double
     Bracket b(ctx, "integrate");
     launch_moveP(s, true, ctx->d_th, ctx->d_task_out, st);
   }
-  launch_reduce(s, ctx->d_task_out, 16, 8, 0, ctx->d_red, kRedStride, 0, st);
-  launch_kin_finish(s, ctx->d_red, ctx->d_Kin, st);
-  launch_thermo(s, ctx->d_th, ctx->d_Kin, 2, d_gs_step, st);
-  launch_momenta(s, ctx->d_task_out, st);
-  launch_reduce(s, ctx->d_task_out, 16, 6, 0, ctx->d_mom, 8, 0, st);
-  launch_observables(s, ctx->d_E, ctx->d_Kin, ctx->d_mom, d_obs_step, st);
+  launch_reduce(s, ctx->d_task_out, 16, 14, 0, ctx->d_red, kRedStride, 0, st);
+  launch_step_scalars(s, 2, ctx->d_red, ctx->d_Kin, ctx->d_th, ctx->d_E, d_obs_step, st);
   // list maintenance (v25/main.cpp:679-701): decide on the device, rebuild
   // inside kernels that return at once for replicas that keep their list
   launch_check(s, ctx->d_task_out, ctx->d_rebuild, ctx->d_srebuild, d_obs_step + MSC_OBS_REBUILD, MSC_OBS_N, false,

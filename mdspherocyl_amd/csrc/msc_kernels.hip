@@ -361,28 +361,35 @@ __global__ void __launch_bounds__(256) k_build_lists(State s, const int *__restr
 // Contact rows refreshed from the reference rows (same pair order), for
 // replicas whose contact skin is used up while the reference list is still
 // valid.
-__global__ void __launch_bounds__(128) k_short_build(State s, const int *__restrict__ srebuild, int *__restrict__ flags) {
-  int g = s.g_begin + blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(256) k_short_build(State s, const int *__restrict__ srebuild, int *__restrict__ flags) {
+  // one wave per particle, the lanes over its reference row (kept in order)
+  const int lane = threadIdx.x & 63;
+  const int g = s.g_begin + __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
   if (g >= s.g_end) return;
-  int rp = s.rep_of[g];
+  const int rp = __builtin_amdgcn_readfirstlane(s.rep_of[g]);
   if (!srebuild[rp]) return;
   const RepDev &R = s.reps[rp];
   const double rs2 = R.r_short * R.r_short;
   const int n = s.nl_cnt[g];
   int ns = 0;
-  for (int q = 0; q < n; ++q) {
-    int j = s.nl_idx[row_at(R.nl_base, R.N, R.n0, q, g)];
-    if (pair_segdist(s, R, g, j) < rs2) {
-      if (ns < R.sl_w) s.sl_idx[row_at(R.sl_base, R.N, R.n0, ns, g)] = j;
-      ++ns;
-    }
+  for (int q0 = 0; q0 < n; q0 += 64) {
+    const int q = q0 + lane;
+    const bool act = q < n;
+    const int j = act ? s.nl_idx[row_at(R.nl_base, R.N, R.n0, q, g)] : g;
+    const bool in = act && pair_segdist(s, R, g, j) < rs2;
+    const unsigned long long m = __ballot(in);
+    const int pos = ns + lane_prefix(m, lane);
+    if (in && pos < R.sl_w) s.sl_idx[row_at(R.sl_base, R.N, R.n0, pos, g)] = j;
+    ns += __popcll(m);
   }
-  s.sl_cnt[g] = min(ns, R.sl_w);
-  if (ns > R.sl_w) atomicMax(&flags[rp * 4 + 1], ns);
+  if (lane == 0) {
+    s.sl_cnt[g] = min(ns, R.sl_w);
+    if (ns > R.sl_w) atomicMax(&flags[rp * 4 + 1], ns);
+  }
   const int N = s.Ntot;
-  for (int a = 0; a < 3; ++a) {
-    s.snap_e[a * N + g] = s.x[a * N + g];
-    s.snap_e[(3 + a) * N + g] = s.ori[(6 + a) * N + g];
+  if (lane < 3) {
+    s.snap_e[lane * N + g] = s.x[lane * N + g];
+    s.snap_e[(3 + lane) * N + g] = s.ori[(6 + lane) * N + g];
   }
 }
 
@@ -1187,34 +1194,38 @@ __global__ void __launch_bounds__(64 * kRfGroups) k_recip_force(State s, const C
 template <int WHICH>
 __global__ void __launch_bounds__(256) k_reduce_items(const RepDev *__restrict__ reps, int rep0,
                                                       const double *__restrict__ in, int istride, int nv,
-                                                      double *__restrict__ out, int ostride, int ooff) {
+                                                      double *__restrict__ out, int ostride, int ooff, int nv2,
+                                                      double *__restrict__ out2, int o2stride, int o2off) {
   constexpr int kMaxV = 16;
   __shared__ double sh[4][kMaxV];
   const int rp = rep0 + blockIdx.x;
   const RepDev &R = reps[rp];
   const int b = WHICH == 0 ? R.task0 : R.slot0;
   const int n = WHICH == 0 ? R.ntask : R.ncol * R.KZ;
+  const int nt = nv + nv2;
   double v[kMaxV];
 #pragma unroll
   for (int k = 0; k < kMaxV; ++k) v[k] = 0.0;
   for (int i = threadIdx.x; i < n; i += 256) {
-    const double *row = in + (size_t)(b + i) * istride;
+      const double *row = in + (size_t)(b + i) * istride;
 #pragma unroll
     for (int k = 0; k < kMaxV; ++k)
-      if (k < nv) v[k] += row[k];
+      if (k < nt) v[k] += row[k];
   }
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
 #pragma unroll
   for (int k = 0; k < kMaxV; ++k) {
-    if (k < nv) {
+    if (k < nt) {
       double t = wave_sum(v[k]);
       if (lane == 0) sh[wv][k] = t;
     }
   }
   __syncthreads();
-  if (threadIdx.x < nv) {
+  if (threadIdx.x < nt) {
     const int k = threadIdx.x;
-    out[(size_t)rp * ostride + ooff + k] = ((sh[0][k] + sh[1][k]) + sh[2][k]) + sh[3][k];
+    const double t = ((sh[0][k] + sh[1][k]) + sh[2][k]) + sh[3][k];
+    if (k < nv) out[(size_t)rp * ostride + ooff + k] = t;
+    else out2[(size_t)rp * o2stride + o2off + (k - nv)] = t;
   }
 }
 
@@ -1250,7 +1261,9 @@ __global__ void __launch_bounds__(256) k_moveP(State s, const Thermo *__restrict
   if (wave_task_id() >= s.ntask) return;
   Task tk = s.tasks[tid];
   const RepDev &Rp = s.reps[tk.rep];
-  double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  // [0] sum p^2/m, [1] sum l.I^-1.l, [2..7] kinetic pressure tensor, and
+  // after the last kick [8..13] the total momentum and angular momentum
+  double acc[14] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
   if (lane < tk.n) {
     const int g = tk.g0 + lane, N = s.Ntot;
     const double Dt2 = Rp.dt / 2;
@@ -1299,8 +1312,13 @@ __global__ void __launch_bounds__(256) k_moveP(State s, const Thermo *__restrict
     }
     acc[0] = psq / m;
     acc[1] = lsq;
+    if (LAST)
+      for (int j = 0; j < 3; ++j) {
+        acc[8 + j] = p[j];
+        acc[11 + j] = l[j];
+      }
   }
-  wave_store_sums(task_out + (size_t)tid * 16, acc, 8, lane);
+  wave_store_sums(task_out + (size_t)tid * 16, acc, LAST ? 14 : 8, lane);
 }
 
 // translate + rotate (5-factor Rx Ry Rz Ry Rx splitting) + dipole update
@@ -1361,77 +1379,66 @@ __global__ void k_renorm(State s) {
 // moveTher for every replica (one thread each).  gs[rep*4 + 2*i + 0/1] hold
 // the Box-Muller pair drawn on the host from thermostat i's MT19937 for this
 // step (mode 0 uses the first, mode 2 the stored second).
-template <int MODE>
-__global__ void k_thermo(State s, Thermo *__restrict__ th, const double *__restrict__ Kin,
-                         const double *__restrict__ gs) {
+// moveTher mode 0 (v25/functions.cpp:2186-2218) for every replica (one
+// thread each): gs[rep*4 + 2*i + 0/1] hold the Box-Muller pair drawn on the
+// host from thermostat i's MT19937 for this step.
+__global__ void k_thermo0(State s, Thermo *__restrict__ th, const double *__restrict__ gs) {
   int rp = blockIdx.x * blockDim.x + threadIdx.x;
   if (rp >= s.nrep) return;
   const RepDev &R = s.reps[rp];
   Thermo T = th[rp];
   for (int i = 0; i < 2; ++i) {
-    double Q = R.Q, g = R.gamma, ex1;
-    if constexpr (MODE == 0) {
-      double r0 = gs[rp * 4 + 2 * i], r1 = gs[rp * 4 + 2 * i + 1];
-      T.strn[i] = r1;
-      ex1 = exp(-g * R.dt / 2);
-      T.s[i] = ex1 * T.s[i] + sqrt(2. * R.temp * g * R.dt / Q) * r0;
-    } else if constexpr (MODE == 1) {
-      double K = Kin[rp * 2 + i];
-      T.s[i] = T.s[i] + R.dt * (2. * K - 3. * (double)R.N * R.temp) / Q;
-    } else {
-      ex1 = exp(-g * R.dt / 2);
-      T.s[i] = ex1 * T.s[i] + sqrt(2. * R.temp * g * R.dt / Q) * T.strn[i];
-    }
+    double r0 = gs[rp * 4 + 2 * i], r1 = gs[rp * 4 + 2 * i + 1];
+    T.strn[i] = r1;
+    double ex1 = exp(-R.gamma * R.dt / 2);
+    T.s[i] = ex1 * T.s[i] + sqrt(2. * R.temp * R.gamma * R.dt / R.Q) * r0;
   }
   th[rp] = T;
 }
 
-// kinetic energies K[2] = sums / 2 per replica from the reduced moveP sums
-// (kRedStride doubles per replica)
-__global__ void k_kin_finish(State s, const double *__restrict__ red, double *__restrict__ Kin) {
-  int rp = blockIdx.x * blockDim.x + threadIdx.x;
-  if (rp >= s.nrep) return;
-  Kin[rp * 2 + 0] = red[rp * kRedStride + 0] / 2;
-  Kin[rp * 2 + 1] = red[rp * kRedStride + 1] / 2;
-}
-
-// observables of print_out (v25/main.cpp:626-647, v25/functions.cpp:2843-2850)
-__global__ void k_observables(State s, const RepEnergy *__restrict__ E, const double *__restrict__ Kin,
-                              const double *__restrict__ mom, double *__restrict__ obs) {
+// Per-replica scalars after a kick (one thread per replica), from the
+// reduced moveP sums red[rep*kRedStride + ...]: the kinetic energies
+// K[2] = sums / 2, then
+//  MODE 1: moveTher mode 1, driven by those energies;
+//  MODE 2: moveTher mode 2 (the stored deviate) and the observables of
+//          print_out (v25/main.cpp:626-647, v25/functions.cpp:2843-2850).
+template <int MODE>
+__global__ void k_step_scalars(State s, const double *__restrict__ red, double *__restrict__ Kin,
+                               Thermo *__restrict__ th, const RepEnergy *__restrict__ E,
+                               double *__restrict__ obs) {
   int rp = blockIdx.x * blockDim.x + threadIdx.x;
   if (rp >= s.nrep) return;
   const RepDev &R = s.reps[rp];
-  const RepEnergy e = E[rp];
-  double Np = (double)R.N;
-  double WdipR = 3 * e.UR, WdipK = 3 * e.UK;
-  double K = (Kin[rp * 2] + Kin[rp * 2 + 1]) / Np;
-  double Uss = e.Usc / Np, UdipR = e.UR / Np, UdipK = e.UK / Np, Uext = e.Uext / Np;
-  double U = R.Uself + Uss + UdipR + UdipK + Uext;
-  double Pss = e.Wsc / (3 * R.V), PdipR = WdipR / (3 * R.V), PdipK = WdipK / (3 * R.V);
-  double P = K / 3 * R.dens + Pss + PdipR + PdipK;
-  double *o = obs + (size_t)rp * 12;
-  o[0] = K; o[1] = U; o[2] = P; o[3] = Uss; o[4] = Pss; o[5] = UdipR; o[6] = PdipR;
-  o[7] = UdipK; o[8] = PdipK;
-  const double *m = mom + rp * 8;
-  o[9] = m[0] * m[0] + m[1] * m[1] + m[2] * m[2];
-  o[10] = m[3] * m[3] + m[4] * m[4] + m[5] * m[5];
-}
-
-// per-task sums of p and l (total momenta)
-__global__ void __launch_bounds__(256) k_momenta(State s, double *__restrict__ task_out) {
-  int tid = s.task_begin + wave_task_id();
-  int lane = threadIdx.x & 63;
-  if (wave_task_id() >= s.ntask) return;
-  Task tk = s.tasks[tid];
-  double v[6] = {0, 0, 0, 0, 0, 0};
-  if (lane < tk.n) {
-    int g = tk.g0 + lane;
-    for (int a = 0; a < 3; ++a) {
-      v[a] = s.p[a * s.Ntot + g];
-      v[3 + a] = s.l[a * s.Ntot + g];
+  const double *r = red + (size_t)rp * kRedStride;
+  const double K2[2] = {r[0] / 2, r[1] / 2};
+  Kin[rp * 2 + 0] = K2[0];
+  Kin[rp * 2 + 1] = K2[1];
+  Thermo T = th[rp];
+  for (int i = 0; i < 2; ++i) {
+    if constexpr (MODE == 1) {
+      T.s[i] = T.s[i] + R.dt * (2. * K2[i] - 3. * (double)R.N * R.temp) / R.Q;
+    } else {
+      double ex1 = exp(-R.gamma * R.dt / 2);
+      T.s[i] = ex1 * T.s[i] + sqrt(2. * R.temp * R.gamma * R.dt / R.Q) * T.strn[i];
     }
   }
-  wave_store_sums(task_out + (size_t)tid * 16, v, 6, lane);
+  th[rp] = T;
+  if constexpr (MODE == 2) {
+    const RepEnergy e = E[rp];
+    double Np = (double)R.N;
+    double WdipR = 3 * e.UR, WdipK = 3 * e.UK;
+    double K = (K2[0] + K2[1]) / Np;
+    double Uss = e.Usc / Np, UdipR = e.UR / Np, UdipK = e.UK / Np, Uext = e.Uext / Np;
+    double U = R.Uself + Uss + UdipR + UdipK + Uext;
+    double Pss = e.Wsc / (3 * R.V), PdipR = WdipR / (3 * R.V), PdipK = WdipK / (3 * R.V);
+    double P = K / 3 * R.dens + Pss + PdipR + PdipK;
+    double *o = obs + (size_t)rp * 12;
+    o[0] = K; o[1] = U; o[2] = P; o[3] = Uss; o[4] = Pss; o[5] = UdipR; o[6] = PdipR;
+    o[7] = UdipK; o[8] = PdipK;
+    const double *m = r + 8;
+    o[9] = m[0] * m[0] + m[1] * m[1] + m[2] * m[2];
+    o[10] = m[3] * m[3] + m[4] * m[4] + m[5] * m[5];
+  }
 }
 
 __global__ void k_zero_FT(State s) {
@@ -1471,7 +1478,7 @@ void launch_lists(const State &s, const int *rebuild, int *flags, hipStream_t st
   MSC_LAUNCH(k_savec, dim3(nblk(s.g_end - s.g_begin, 256)), dim3(256), 0, st, s, rebuild);
 }
 void launch_short(const State &s, const int *srebuild, int *flags, hipStream_t st) {
-  MSC_LAUNCH(k_short_build, dim3(nblk(s.g_end - s.g_begin, 128)), dim3(128), 0, st, s, srebuild, flags);
+  MSC_LAUNCH(k_short_build, dim3(nblk(s.g_end - s.g_begin, 4)), dim3(256), 0, st, s, srebuild, flags);
 }
 void launch_check(const State &s, double *task_out, int *rebuild, int *srebuild, double *obs_rebuild,
                   int obs_stride, bool standalone, hipStream_t st) {
@@ -1506,13 +1513,13 @@ void launch_self_ener(const State &s, double *task_out, hipStream_t st) {
   MSC_LAUNCH(k_self_ener, dim3(nblk(s.ntask, 4)), dim3(256), 0, st, s, task_out);
 }
 void launch_reduce(const State &s, const double *in, int istride, int nv, int which, double *out, int ostride,
-                   int ooff, hipStream_t st) {
+                   int ooff, hipStream_t st, int nv2, double *out2, int o2stride, int o2off) {
   if (which == 0)
     MSC_LAUNCH(k_reduce_items<0>, dim3(s.rep_end - s.rep_begin), dim3(256), 0, st, s.reps, s.rep_begin, in, istride,
-               nv, out, ostride, ooff);
+               nv, out, ostride, ooff, nv2, out2, o2stride, o2off);
   else
     MSC_LAUNCH(k_reduce_items<1>, dim3(s.rep_end - s.rep_begin), dim3(256), 0, st, s.reps, s.rep_begin, in, istride,
-               nv, out, ostride, ooff);
+               nv, out, ostride, ooff, nv2, out2, o2stride, o2off);
 }
 
 template <typename R>
@@ -1587,23 +1594,15 @@ void launch_moveX(const State &s, hipStream_t st) {
 void launch_renorm(const State &s, hipStream_t st) {
   MSC_LAUNCH(k_renorm, dim3(nblk(s.g_end - s.g_begin, 256)), dim3(256), 0, st, s);
 }
-void launch_thermo(const State &s, Thermo *th, const double *Kin, int mode, const double *gs, hipStream_t st) {
-  if (mode == 0)
-    MSC_LAUNCH(k_thermo<0>, dim3(nblk(s.nrep, 64)), dim3(64), 0, st, s, th, Kin, gs);
-  else if (mode == 1)
-    MSC_LAUNCH(k_thermo<1>, dim3(nblk(s.nrep, 64)), dim3(64), 0, st, s, th, Kin, gs);
+void launch_thermo0(const State &s, Thermo *th, const double *gs, hipStream_t st) {
+  MSC_LAUNCH(k_thermo0, dim3(nblk(s.nrep, 64)), dim3(64), 0, st, s, th, gs);
+}
+void launch_step_scalars(const State &s, int mode, const double *red, double *Kin, Thermo *th, const RepEnergy *E,
+                         double *obs, hipStream_t st) {
+  if (mode == 1)
+    MSC_LAUNCH(k_step_scalars<1>, dim3(nblk(s.nrep, 64)), dim3(64), 0, st, s, red, Kin, th, E, obs);
   else
-    MSC_LAUNCH(k_thermo<2>, dim3(nblk(s.nrep, 64)), dim3(64), 0, st, s, th, Kin, gs);
-}
-void launch_kin_finish(const State &s, const double *red, double *Kin, hipStream_t st) {
-  MSC_LAUNCH(k_kin_finish, dim3(nblk(s.nrep, 64)), dim3(64), 0, st, s, red, Kin);
-}
-void launch_observables(const State &s, const RepEnergy *E, const double *Kin, const double *mom, double *obs,
-                        hipStream_t st) {
-  MSC_LAUNCH(k_observables, dim3(nblk(s.nrep, 64)), dim3(64), 0, st, s, E, Kin, mom, obs);
-}
-void launch_momenta(const State &s, double *task_out, hipStream_t st) {
-  MSC_LAUNCH(k_momenta, dim3(nblk(s.ntask, 4)), dim3(256), 0, st, s, task_out);
+    MSC_LAUNCH(k_step_scalars<2>, dim3(nblk(s.nrep, 64)), dim3(64), 0, st, s, red, Kin, th, E, obs);
 }
 void launch_zero_FT(const State &s, hipStream_t st) {
   MSC_LAUNCH(k_zero_FT, dim3(nblk(s.g_end - s.g_begin, 256)), dim3(256), 0, st, s);

@@ -48,8 +48,10 @@ void launch_force_sc(const State &s, bool strict, double *task_out, int *flags, 
 void launch_dipR(const State &s, bool mixed, bool all_pairs, const void *tab, double *task_out, hipStream_t st);
 void launch_ext_field(const State &s, double *task_out, hipStream_t st);
 void launch_self_ener(const State &s, double *task_out, hipStream_t st);
+// out[rep*ostride + ooff + k] = replica sum of in[item*istride + k], k < nv;
+// with nv2 > 0 the following nv2 values go to out2[rep*o2stride + o2off + k]
 void launch_reduce(const State &s, const double *in, int istride, int nv, int which, double *out, int ostride,
-                   int ooff, hipStream_t st);
+                   int ooff, hipStream_t st, int nv2 = 0, double *out2 = nullptr, int o2stride = 0, int o2off = 0);
 void launch_recip_tables(const State &s, bool mixed, void *ph, void *amp, double *task_out, hipStream_t st);
 int launch_recip_struct(const State &s, bool mixed, int KZ, int rt0, int nrt, int maxcol, const void *ph,
                         const void *amp, void *part, hipStream_t st);
@@ -60,11 +62,9 @@ int launch_recip_force(const State &s, bool mixed, int KZ, const void *ph, const
 void launch_moveP(const State &s, bool last, const Thermo *th, double *task_out, hipStream_t st);
 void launch_moveX(const State &s, hipStream_t st);
 void launch_renorm(const State &s, hipStream_t st);
-void launch_thermo(const State &s, Thermo *th, const double *Kin, int mode, const double *gs, hipStream_t st);
-void launch_kin_finish(const State &s, const double *red, double *Kin, hipStream_t st);
-void launch_observables(const State &s, const RepEnergy *E, const double *Kin, const double *mom, double *obs,
-                        hipStream_t st);
-void launch_momenta(const State &s, double *task_out, hipStream_t st);
+void launch_thermo0(const State &s, Thermo *th, const double *gs, hipStream_t st);
+void launch_step_scalars(const State &s, int mode, const double *red, double *Kin, Thermo *th, const RepEnergy *E,
+                         double *obs, hipStream_t st);
 void launch_zero_FT(const State &s, hipStream_t st);
 void launch_fill_int(int *p, int n, int v, hipStream_t st);
 
