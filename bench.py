@@ -82,6 +82,16 @@ def timed_run(sysm, steps, warmup, timing):
     return dt, obs
 
 
+def mt19937_state(seed):
+    """MT19937 words after init_genrand(seed), position 624 (gsl_rng_set)."""
+    mt = np.zeros(624, dtype=np.uint32)
+    mt[0] = seed
+    for i in range(1, 624):
+        prev = int(mt[i - 1])
+        mt[i] = (1812433253 * (prev ^ (prev >> 30)) + i) & 0xffffffff
+    return mt
+
+
 def kernel_table(sysm, names):
     out = {}
     for n in names:
@@ -187,6 +197,23 @@ def main():
         log("  %-13s %8.3f ms/launch  (%d launches, %.1f%% of the timed region)"
             % (n, ms / cnt, cnt, 100 * ms / (1e3 * t_mix)))
 
+    # cluster labels (clust_conf) on the final state, min_s = 0.2: device
+    # components plus the host's numbering by the generator's draws
+    mix.reset_timing()
+    mix.set_timing(True)
+    mt, mti = mt19937_state(5), 624
+    t0 = time.perf_counter()
+    ncl = 3
+    for _ in range(ncl):
+        labels, nclust, mt, mti, _ = mix.cluster_labels(0, 0.2, mt, mti)
+    t_cl = (time.perf_counter() - t0) / ncl
+    mix.set_timing(False)
+    cl_dev = mix.kernel_time("clusters")
+    clusters = {"ms_per_call": round(1e3 * t_cl, 3), "device_ms": round(cl_dev[0] / max(cl_dev[1], 1), 4),
+                "clusters": nclust, "largest": int(np.bincount(labels).max()), "min_s": 0.2}
+    log("bench: cluster labels %.3f ms per call (device %.3f ms), %d clusters, largest %d"
+        % (1e3 * t_cl, clusters["device_ms"], nclust, clusters["largest"]))
+
     k64 = args.fp64_steps or max(args.steps // 2, 10)
     log("bench: FP64, %d warmup + %d timed steps" % (min(args.warmup, 10), k64))
     t_d, _ = timed_run(f64, k64, min(args.warmup, 10), False)
@@ -269,6 +296,7 @@ def main():
         "err_mixed_vs_fp64": err_mix_vs_fp64,
         "err_vs_oracle": err_oracle,
         "batched": batched,
+        "cluster_labels": clusters,
         "kernels_ms_per_launch": {n: round(v[0] / v[1], 4) for n, v in ktab.items()},
         "roofline": roofline,
         "cpu_baseline": cpu,

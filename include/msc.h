@@ -154,14 +154,24 @@ int msc_set_graph(msc_ctx *ctx, int on);
  * reference counterpart (the reference's lists are host deques). */
 int msc_chunk_repeats(msc_ctx *ctx, long *n);
 
-/* ---- cluster labels (v25/functions.cpp:2950-3324) ----------------------
- * Components of the contact graph d_seg^2 < ((D_i+D_j)/2 + min_s)^2, labelled
- * in the order the reference's clust_conf creates them.  mt_state/mt_index
- * hold the MT19937 state of the reference's `ran` generator (624 words + the
- * position), advanced in place by the draws rnd_part makes.  Returns the
- * number of clusters in *nclust. */
+/* ---- cluster labels (clust_conf, v25/functions.cpp:3270-3324) ----------
+ * labels[i] (replica-local i) = the cluster clust_conf assigns particle i:
+ * clusters are the components of the contact graph
+ * d_seg^2 < ((D_i+D_j)/2 + min_s)^2 (dist_test), numbered in the order the
+ * reference creates them.  mt_state[624]/mt_index are the MT19937 state of
+ * the reference's `ran` generator (gsl_rng_mt19937: words mt[] and position
+ * mti), advanced in place by the one gsl_rng_uniform draw rnd_part makes per
+ * cluster.  *nclust receives the number of clusters.  n_asym (may be NULL)
+ * receives the number of pairs whose contact test gives different answers
+ * for (i, j) and (j, i); such a pair is taken as a contact, where the
+ * reference's answer depends on which of the two its search meets first
+ * (0 unless a distance lies within rounding of the threshold).
+ * Needs the neighbour list of the current state (msc_neigh_list, or the
+ * list msc_run keeps) and min_s + largest diameter < r_list_max; fails with
+ * MSC_E_STATE if particles moved too far since the list was built for its
+ * rows to hold every contact. */
 int msc_cluster_labels(msc_ctx *ctx, int rep, double min_s, unsigned int *mt_state,
-                       int *mt_index, int *labels, int *nclust);
+                       int *mt_index, int *labels, int *nclust, long *n_asym);
 
 /* ---- measurement -------------------------------------------------------
  * With timing on, the library brackets each launch of the named kernels with

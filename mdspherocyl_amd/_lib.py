@@ -86,7 +86,7 @@ def load():
         "msc_run": ([vp, ctypes.c_int, ctypes.c_int, _dp], ctypes.c_int),
         "msc_set_graph": ([vp, ctypes.c_int], ctypes.c_int),
         "msc_cluster_labels": ([vp, ctypes.c_int, ctypes.c_double, ctypes.POINTER(ctypes.c_uint),
-                                _ip, _ip, _ip], ctypes.c_int),
+                                _ip, _ip, _ip, ctypes.POINTER(ctypes.c_long)], ctypes.c_int),
         "msc_set_timing": ([vp, ctypes.c_int], ctypes.c_int),
         "msc_kernel_time": ([vp, ctypes.c_char_p, _dp, ctypes.POINTER(ctypes.c_long)], ctypes.c_int),
         "msc_reset_timing": ([vp], ctypes.c_int),

@@ -98,7 +98,8 @@ struct msc_ctx {
   double *d_x = nullptr, *d_p = nullptr, *d_l = nullptr, *d_ori = nullptr, *d_F = nullptr, *d_T = nullptr;
   double *d_len = nullptr, *d_dia = nullptr, *d_mass = nullptr, *d_inert = nullptr;
   double *d_dpc = nullptr, *d_doc = nullptr, *d_dmu = nullptr, *d_dpos = nullptr, *d_dori = nullptr, *d_snap = nullptr,
-         *d_snap_e = nullptr;
+         *d_snap_e = nullptr, *d_snap_u = nullptr;
+  int *d_clu = nullptr, *d_clu_flag = nullptr;
   int *d_cell_of = nullptr, *d_cell_cnt = nullptr, *d_cell_start = nullptr, *d_cell_fill = nullptr,
       *d_cell_list = nullptr;
   int *d_cell_tmp = nullptr;
@@ -301,6 +302,7 @@ void derive(msc_ctx *ctx, int rp, const double *len, const double *dia, const do
   if (D.dmax > 0) dmax = D.dmax;
   if (D.dmin > 0) dmin = D.dmin;
   R.lmax = lmax;
+  R.diamax = diamax;
   // list radii, v25/main.cpp:461-463
   R.r_list_min = D.sr_cut + 0.1 * dmin;
   if (R.r_list_min < 1.5 * dmax) R.r_list_min = 1.5 * dmax;
@@ -350,6 +352,7 @@ void refresh_state_ptrs(msc_ctx *ctx) {
   s.dpc = ctx->d_dpc; s.doc = ctx->d_doc; s.dmu = ctx->d_dmu; s.dpos = ctx->d_dpos; s.dori = ctx->d_dori;
   s.snap = ctx->d_snap;
   s.snap_e = ctx->d_snap_e;
+  s.snap_u = ctx->d_snap_u;
   s.cell_of = ctx->d_cell_of; s.cell_cnt = ctx->d_cell_cnt; s.cell_start = ctx->d_cell_start;
   s.cell_fill = ctx->d_cell_fill; s.cell_tmp = ctx->d_cell_tmp; s.cell_pack = ctx->d_cell_pack; s.cell_list = ctx->d_cell_list;
   s.cell_rep = ctx->d_cell_rep;
@@ -412,6 +415,83 @@ void build_table(const RepDev &R, std::vector<double> &T) {
 // ==========================================================================
 // C ABI
 // ==========================================================================
+// MT19937 of Matsumoto and Nishimura (ACM TOMACS 8, 1998) with GSL's state
+// convention: words mt[] and the position mti of the next word (624: the
+// block is used up); gsl_rng_uniform is the tempered word / 2^32.
+struct Mt19937 {
+  static constexpr int kN = 624, kM = 397;
+  uint32_t mt[kN];
+  int mti = kN;
+  uint32_t next() {
+    if (mti >= kN) {
+      for (int k = 0; k < kN; ++k) {
+        const uint32_t y = (mt[k] & 0x80000000u) | (mt[(k + 1) % kN] & 0x7fffffffu);
+        mt[k] = mt[(k + kM) % kN] ^ (y >> 1) ^ ((y & 1u) ? 0x9908b0dfu : 0u);
+      }
+      mti = 0;
+    }
+    uint32_t y = mt[mti++];
+    y ^= y >> 11;
+    y ^= (y << 7) & 0x9d2c5680u;
+    y ^= (y << 15) & 0xefc60000u;
+    y ^= y >> 18;
+    return y;
+  }
+  double uniform() { return next() / 4294967296.0; }
+};
+
+// Cluster numbering of clust_conf (v25/functions.cpp:3270-3301): while
+// particles are left, rnd_part (v25/functions.cpp:2950-2970) draws
+// ip = floor(u * left) and takes the ip-th particle left in global order
+// (species-major, as the per-species index lists NI concatenate); its whole
+// component gets the next number.  The particles left are a bit set with a
+// Fenwick tree over the popcounts of its words, so each draw finds its
+// particle in O(log N).
+static int order_clusters(const std::vector<int> &root, Mt19937 &mt, int *labels) {
+  const int N = (int)root.size();
+  std::vector<int> start(N + 1, 0), members(N);
+  for (int r : root) ++start[r + 1];
+  for (int i = 0; i < N; ++i) start[i + 1] += start[i];
+  {
+    std::vector<int> fill(start.begin(), start.end() - 1);
+    for (int i = 0; i < N; ++i) members[fill[root[i]]++] = i;
+  }
+  const int W = (N + 63) / 64;
+  std::vector<uint64_t> bits(W, ~0ull);
+  if (N % 64) bits[W - 1] = (1ull << (N % 64)) - 1;
+  std::vector<int> tree(W + 1, 0);  // Fenwick tree, 1-based
+  for (int w = 0; w < W; ++w) {
+    tree[w + 1] += __builtin_popcountll(bits[w]);
+    const int up = (w + 1) + ((w + 1) & -(w + 1));
+    if (up <= W) tree[up] += tree[w + 1];
+  }
+  int top = 1;
+  while (top * 2 <= W) top *= 2;
+  int left = N, nc = 0;
+  while (left > 0) {
+    int ip = (int)std::floor(mt.uniform() * left);
+    int pos = 0;
+    for (int step = top; step; step >>= 1)
+      if (pos + step <= W && tree[pos + step] <= ip) {
+        pos += step;
+        ip -= tree[pos];
+      }
+    uint64_t word = bits[pos];
+    for (int k = 0; k < ip; ++k) word &= word - 1;
+    const int seed = pos * 64 + __builtin_ctzll(word);
+    const int r = root[seed];
+    for (int k = start[r]; k < start[r + 1]; ++k) {
+      const int m = members[k];
+      labels[m] = nc;
+      bits[m >> 6] &= ~(1ull << (m & 63));
+      for (int i = (m >> 6) + 1; i <= W; i += i & -i) --tree[i];
+    }
+    left -= start[r + 1] - start[r];
+    ++nc;
+  }
+  return nc;
+}
+
 extern "C" {
 
 int msc_version(void) { return 1; }
@@ -525,6 +605,9 @@ int msc_create(int device, int precision, int nrep, const msc_replica_desc *reps
   A(&ctx->d_dpc, nd * N); A(&ctx->d_doc, 3 * nd * N); A(&ctx->d_dmu, nd * N);
   A(&ctx->d_dpos, 3 * nd * N); A(&ctx->d_dori, 3 * nd * N); A(&ctx->d_snap, 3 * N);
   A(&ctx->d_snap_e, 6 * N);
+  A(&ctx->d_snap_u, 3 * N);
+  A(&ctx->d_clu, N);
+  A(&ctx->d_clu_flag, 2);
   A(&ctx->d_cell_of, N); A(&ctx->d_cell_list, N);
   A(&ctx->d_cell_tmp, N);
   A(&ctx->d_cell_pack, (size_t)8 * N);
@@ -606,7 +689,8 @@ void msc_destroy(msc_ctx *ctx) {
                   ctx->d_dpc, ctx->d_doc, ctx->d_dmu, ctx->d_dpos, ctx->d_dori, ctx->d_snap, ctx->d_cell_of,
                   ctx->d_cell_cnt, ctx->d_cell_start, ctx->d_cell_fill, ctx->d_cell_list, ctx->d_nl_cnt,
                   ctx->d_nl_idx, ctx->d_dl_cnt, ctx->d_dl_idx, ctx->d_kval, ctx->d_cell_tmp, ctx->d_sl_cnt,
-                  ctx->d_sl_idx, ctx->d_snap_e, ctx->d_srebuild, ctx->d_cell_pack,
+                  ctx->d_sl_idx, ctx->d_snap_e, ctx->d_srebuild, ctx->d_cell_pack, 
+                  ctx->d_snap_u, ctx->d_clu, ctx->d_clu_flag,
                   ctx->d_tab, ctx->d_ph, ctx->d_amp, ctx->d_part, ctx->d_A, ctx->d_slot_out, ctx->d_task_out,
                   ctx->d_red, ctx->d_M2, ctx->d_Kin, ctx->d_gs, ctx->d_obs, ctx->d_counts, ctx->d_E,
                   ctx->d_th, ctx->d_flags, ctx->d_rebuild, ctx->d_one, ctx->d_run_gs, ctx->d_run_obs, ctx->d_save};
@@ -1208,6 +1292,7 @@ static int state_copy(msc_ctx *ctx, bool restore) {
   struct Item { void *p; size_t bytes; } items[] = {
       {ctx->d_x, 3 * N * 8}, {ctx->d_p, 3 * N * 8}, {ctx->d_l, 3 * N * 8}, {ctx->d_ori, 9 * N * 8},
       {ctx->d_F, 3 * N * 8}, {ctx->d_T, 3 * N * 8}, {ctx->d_snap, 3 * N * 8}, {ctx->d_snap_e, 6 * N * 8},
+      {ctx->d_snap_u, 3 * N * 8},
       {ctx->d_th, nrep * sizeof(Thermo)}, {ctx->d_E, nrep * sizeof(RepEnergy)}, {ctx->d_Kin, 2 * nrep * 8}};
   size_t total = 0;
   for (auto &it : items) total += (it.bytes + 255) / 256 * 256;
@@ -1370,10 +1455,37 @@ int msc_chunk_repeats(msc_ctx *ctx, long *n) {
 }
 
 int msc_cluster_labels(msc_ctx *ctx, int rep, double min_s, unsigned int *mt_state, int *mt_index, int *labels,
-                       int *nclust) {
-  (void)min_s; (void)mt_state; (void)mt_index; (void)labels; (void)nclust;
-  if (!ctx || rep < 0 || rep >= ctx->nrep) return MSC_E_ARG;
-  return fail(ctx, MSC_E_STATE, "cluster labels are not built yet");
+                       int *nclust, long *n_asym) {
+  if (!ctx || rep < 0 || rep >= ctx->nrep || !(min_s >= 0) || !mt_state || !mt_index || !labels || !nclust)
+    return MSC_E_ARG;
+  if (*mt_index < 0 || *mt_index > Mt19937::kN) return fail(ctx, MSC_E_ARG, "mt_index outside [0, 624]");
+  if (!ctx->have_state) return fail(ctx, MSC_E_STATE, "no state");
+  if (!ctx->have_lists) return fail(ctx, MSC_E_STATE, "no neighbour list (msc_neigh_list or msc_init_run)");
+  const RepDev &R = ctx->reps[rep];
+  const double reach_max = R.diamax + min_s;
+  if (!(reach_max < R.r_list_max))
+    return fail(ctx, MSC_E_ARG, "min_s + largest diameter must stay below the list radius r_list_max");
+  const State s = sel(ctx, rep);
+  {
+    Bracket b(ctx, "clusters");
+    launch_clusters(s, min_s, reach_max, ctx->d_clu, ctx->d_clu_flag, ctx->stream);
+  }
+  LAUNCHED("clusters");
+  std::vector<int> root(R.N);
+  int fl[2];
+  if (int rc_ = d2h(ctx, root.data(), ctx->d_clu + R.n0, sizeof(int) * R.N)) return rc_;
+  if (int rc_ = d2h(ctx, fl, ctx->d_clu_flag, sizeof fl)) return rc_;
+  if (fl[0])
+    return fail(ctx, MSC_E_STATE, "particles moved too far since the neighbour list was built to find every contact");
+  for (int &r : root) r -= R.n0;
+  Mt19937 mt;
+  std::copy(mt_state, mt_state + Mt19937::kN, mt.mt);
+  mt.mti = *mt_index;
+  *nclust = order_clusters(root, mt, labels);
+  std::copy(mt.mt, mt.mt + Mt19937::kN, mt_state);
+  *mt_index = mt.mti;
+  if (n_asym) *n_asym = fl[1];
+  return MSC_OK;
 }
 
 }  // extern "C"

@@ -44,6 +44,7 @@ struct RepDev {
   double r_short;       // contact-list radius: 2^(1/6) max diameter + skin_short
   double skin_short;
   double lmax;          // largest cylinder length
+  double diamax;        // largest diameter
   double B[3];
   double temp, dt, Q, gamma, dens;
   double G[9];          // sum over half-space k of val_k k k^T

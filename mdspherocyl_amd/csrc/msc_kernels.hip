@@ -198,9 +198,10 @@ __device__ __forceinline__ size_t row_at(long base, int N, int n0, int q, int g)
 
 // Segment distance of the pair (g, j) evaluated as neigh_list and force_sc
 // do: A = the lower index, rAB = minimum image of r_A - r_B.
-__device__ __forceinline__ double pair_segdist(const State &s, const RepDev &R, int g, int j) {
+// segment distance with A = particle a, B = particle b and r_AB the minimum
+// image of r_a - r_b, as spherocyl_dist is called (v25/functions.cpp:89)
+__device__ __forceinline__ double dir_segdist(const State &s, const RepDev &R, int A, int Bi) {
   const int N = s.Ntot;
-  const int A = g < j ? g : j, Bi = g < j ? j : g;
   double rA[3] = {s.x[A], s.x[N + A], s.x[2 * N + A]};
   double rB[3] = {s.x[Bi], s.x[N + Bi], s.x[2 * N + Bi]};
   double uA[3] = {s.ori[6 * N + A], s.ori[7 * N + A], s.ori[8 * N + A]};
@@ -211,6 +212,11 @@ __device__ __forceinline__ double pair_segdist(const State &s, const RepDev &R, 
     rAB[a] -= R.L[a] * floor(rAB[a] / R.L[a] + 0.5);
   }
   return segdist(rA, uA, s.len[A], rB, uB, s.len[Bi], R.L, rAB, dAB, st);
+}
+
+// the pair as neigh_list evaluates it: A = lower index
+__device__ __forceinline__ double pair_segdist(const State &s, const RepDev &R, int g, int j) {
+  return g < j ? dir_segdist(s, R, g, j) : dir_segdist(s, R, j, g);
 }
 
 // Neighbour rows from the cell grid in one pass, one wave per particle
@@ -417,6 +423,7 @@ __global__ void k_savec(State s, const int *__restrict__ rebuild) {
     s.snap[a * N + g] = round12(s.x[a * N + g]);
     s.snap_e[a * N + g] = s.x[a * N + g];
     s.snap_e[(3 + a) * N + g] = s.ori[(6 + a) * N + g];
+    s.snap_u[a * N + g] = s.ori[(6 + a) * N + g];
   }
 }
 
@@ -1230,6 +1237,105 @@ __global__ void __launch_bounds__(256) k_reduce_items(const RepDev *__restrict__
 }
 
 // --------------------------------------------------------------------------
+// cluster labels (v25/functions.cpp:2988-3113): the clusters clust_gen grows
+// are the connected components of the contact graph dist_test(i, j):
+// d_seg^2 < ((D_i + D_j)/2 + min_s)^2.  The components come from a
+// union-find over the reference rows; every union hooks the larger root
+// under the smaller, so each component ends with its smallest particle as
+// root whatever order the unions run in.  Parent words are read and written
+// with device-scope atomics, as waves on other XCDs change them.
+
+__device__ __forceinline__ int uf_load(int *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ int uf_find(int *par, int x) {
+  int p = uf_load(&par[x]);
+  while (p != x) {
+    const int gp = uf_load(&par[p]);
+    // path halving; x is no root, so only halving writes ever touch par[x],
+    // and every one of them stores an ancestor
+    if (gp != p) __hip_atomic_store(&par[x], gp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    x = gp;
+    p = uf_load(&par[x]);
+  }
+  return x;
+}
+
+__device__ void uf_union(int *par, int a, int b) {
+  for (;;) {
+    a = uf_find(par, a);
+    b = uf_find(par, b);
+    if (a == b) return;
+    if (a < b) {
+      const int t = a;
+      a = b;
+      b = t;
+    }
+    int expect = a;
+    if (__hip_atomic_compare_exchange_strong(&par[a], &expect, b, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT))
+      return;
+  }
+}
+
+__global__ void k_clust_init(State s, int *__restrict__ par) {
+  const int g = s.g_begin + blockIdx.x * blockDim.x + threadIdx.x;
+  if (g < s.g_end) par[g] = g;
+}
+
+// One thread per particle g over its reference row, each pair once (j > g).
+// The row holds every pair closer than r_list_max when the list was built;
+// a pair in contact now was closer than reach_max + de_g + de_j then, with
+// de the largest displacement of a point of the segment since, so a particle
+// with 2 de >= r_list_max - reach_max is counted in flags[0].
+__global__ void __launch_bounds__(256) k_clust_edges(State s, double min_s, double reach_max, int *__restrict__ par,
+                                                     int *__restrict__ flags) {
+  const int g = s.g_begin + blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= s.g_end) return;
+  const RepDev &R = s.reps[s.rep_of[g]];
+  const int N = s.Ntot;
+  double rg[3], dr2 = 0.0, du2 = 0.0;
+  for (int a = 0; a < 3; ++a) {
+    rg[a] = s.x[a * N + g];
+    double D = rg[a] - s.snap[a * N + g];
+    D -= R.L[a] * floor(D / R.L[a] + 0.5);
+    dr2 += D * D;
+    const double E = s.ori[(6 + a) * N + g] - s.snap_u[a * N + g];
+    du2 += E * E;
+  }
+  const double Lg = s.len[g], Dg = s.dia[g];
+  // the 1e-9 covers the 12-decimal rounding of the reference positions
+  const double de = sqrt(dr2) + 0.5 * Lg * sqrt(du2) + 1e-9;
+  if (2.0 * de >= R.r_list_max - reach_max) atomicAdd(&flags[0], 1);
+  const int n = s.nl_cnt[g];
+  for (int q = 0; q < n; ++q) {
+    const int j = s.nl_idx[row_at(R.nl_base, R.N, R.n0, q, g)];
+    if (j <= g) continue;
+    // dist_test, v25/functions.cpp:2987-3013
+    const double sigma = (Dg + s.dia[j]) / 2 + min_s;
+    const double sig_sq = sigma * sigma;
+    // centre distance bound: d_seg >= |r_gj| - (L_g + L_j)/2
+    double rc2 = 0.0;
+    for (int a = 0; a < 3; ++a) {
+      const double d = wrapd(rg[a] - s.x[a * N + j], R.L[a]);
+      rc2 += d * d;
+    }
+    const double reach = sigma + 0.5 * (Lg + s.len[j]) + 1e-9;
+    if (rc2 >= reach * reach * (1 + 1e-12)) continue;
+    const bool t1 = dir_segdist(s, R, g, j) < sig_sq;
+    const bool t2 = dir_segdist(s, R, j, g) < sig_sq;
+    if (t1 != t2) atomicAdd(&flags[1], 1);
+    if (t1 || t2) uf_union(par, g, j);
+  }
+}
+
+__global__ void k_clust_final(State s, int *__restrict__ par) {
+  const int g = s.g_begin + blockIdx.x * blockDim.x + threadIdx.x;
+  if (g < s.g_end) par[g] = uf_find(par, g);
+}
+
+// --------------------------------------------------------------------------
 // integrator (v25/functions.cpp:2071-2218, v25/classes.cpp:228-325)
 // --------------------------------------------------------------------------
 __device__ __forceinline__ void mm3(const double *A, const double *B, double *C) {
@@ -1606,6 +1712,13 @@ void launch_step_scalars(const State &s, int mode, const double *red, double *Ki
 }
 void launch_zero_FT(const State &s, hipStream_t st) {
   MSC_LAUNCH(k_zero_FT, dim3(nblk(s.g_end - s.g_begin, 256)), dim3(256), 0, st, s);
+}
+void launch_clusters(const State &s, double min_s, double reach_max, int *par, int *flags, hipStream_t st) {
+  const dim3 grid(nblk(s.g_end - s.g_begin, 256));
+  launch_fill_int(flags, 2, 0, st);
+  MSC_LAUNCH(k_clust_init, grid, dim3(256), 0, st, s, par);
+  MSC_LAUNCH(k_clust_edges, grid, dim3(256), 0, st, s, min_s, reach_max, par, flags);
+  MSC_LAUNCH(k_clust_final, grid, dim3(256), 0, st, s, par);
 }
 void launch_fill_int(int *p, int n, int v, hipStream_t st) {
   MSC_LAUNCH(k_fill_int, dim3(nblk(n, 256)), dim3(256), 0, st, p, n, v);

@@ -25,6 +25,7 @@ struct State {
   double *dpos, *dori;               // 3 nd N lab dipole positions / orientations
   double *snap;                      // 3N neighbour-list reference positions
   double *snap_e;                    // 6N centre and axis when the contact list was built
+  double *snap_u;                    // 3N axis when the reference list was built
   // cell grid
   int *cell_of, *cell_cnt, *cell_start, *cell_fill, *cell_tmp, *cell_list;
   double *cell_pack;         // [Ntot][8] centre, axis, length, index in cell order
@@ -67,10 +68,15 @@ void launch_step_scalars(const State &s, int mode, const double *red, double *Ki
                          double *obs, hipStream_t st);
 void launch_zero_FT(const State &s, hipStream_t st);
 void launch_fill_int(int *p, int n, int v, hipStream_t st);
+// contact-graph components of the selected replica: par[g] = smallest
+// particle of g's component (global index); flags[0] counts particles that
+// moved too far for the reference rows to hold every contact, flags[1] pairs
+// whose contact test depends on the order of the two particles
+void launch_clusters(const State &s, double min_s, double reach_max, int *par, int *flags, hipStream_t st);
 
 constexpr int kChunk = 64;  // particles per reciprocal structure-factor chunk
-constexpr int kRedStride = 16;
+constexpr int kRedStride = 16;  // doubles per replica in the reduction scratch d_red
 constexpr int kRfGroups = 8;    // waves sharing one task in the reciprocal back projection
-constexpr int kPairSplit = 4;   // waves sharing one task in the WCA and real-space pair kernels  // doubles per replica in the reduction scratch d_red
+constexpr int kPairSplit = 4;   // waves sharing one task in the WCA and real-space pair kernels
 
 }  // namespace msc

@@ -205,6 +205,26 @@ class MdSystem:
         self._chk(self.L.msc_chunk_repeats(self._ctx, ctypes.byref(n)))
         return n.value
 
+    def cluster_labels(self, rep, min_s, mt_state, mt_index):
+        """clust_conf (v25/functions.cpp:3270): per-particle cluster numbers.
+
+        mt_state (624 words) and mt_index are the MT19937 state of the
+        reference's `ran` generator; returns (labels, nclust, mt_state,
+        mt_index, n_asym) with the state advanced by the draws made.
+        """
+        st = np.ascontiguousarray(mt_state, dtype=np.uint32).copy()
+        if st.shape != (624,):
+            raise ValueError("mt_state must hold 624 words")
+        idx = ctypes.c_int(int(mt_index))
+        n = self.descs[rep].N
+        labels = np.zeros(n, dtype=np.int32)
+        nc, na = ctypes.c_int(), ctypes.c_long()
+        self._chk(self.L.msc_cluster_labels(self._ctx, rep, float(min_s),
+                                            st.ctypes.data_as(ctypes.POINTER(ctypes.c_uint)), ctypes.byref(idx),
+                                            labels.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), ctypes.byref(nc),
+                                            ctypes.byref(na)))
+        return labels, nc.value, st, idx.value, na.value
+
     def work_counts(self, rep):
         c = (ctypes.c_long * 4)()
         self._chk(self.L.msc_work_counts(self._ctx, rep, c))

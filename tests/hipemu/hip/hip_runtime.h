@@ -90,8 +90,15 @@ inline void hipemu_wave_barrier() {
 }
 #define __builtin_amdgcn_wave_barrier() hipemu_wave_barrier()
 #define __builtin_amdgcn_fence(...) ((void)0)
+inline void __threadfence() { std::atomic_thread_fence(std::memory_order_seq_cst); }
 
 inline int atomicAdd(int *p, int v) { return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST); }
+#define __HIP_MEMORY_SCOPE_AGENT 3
+template <class T> inline T __hip_atomic_load(T *p, int, int) { return __atomic_load_n(p, __ATOMIC_SEQ_CST); }
+template <class T> inline void __hip_atomic_store(T *p, T v, int, int) { __atomic_store_n(p, v, __ATOMIC_SEQ_CST); }
+template <class T> inline bool __hip_atomic_compare_exchange_strong(T *p, T *e, T d, int, int, int) {
+  return __atomic_compare_exchange_n(p, e, d, false, __ATOMIC_SEQ_CST, __ATOMIC_SEQ_CST);
+}
 inline int atomicMax(int *p, int v) {
   int old = __atomic_load_n(p, __ATOMIC_SEQ_CST);
   while (old < v && !__atomic_compare_exchange_n(p, &old, v, false, __ATOMIC_SEQ_CST, __ATOMIC_SEQ_CST)) {

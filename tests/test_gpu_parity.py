@@ -433,3 +433,83 @@ def test_list_regions_grow_during_run(msc, monkeypatch):
     assert repeats[0] >= 1 and repeats[1] == 0
     a, b = runs
     assert np.allclose(a[:, 0, :11], b[:, 0, :11], rtol=1e-8, atol=1e-10)
+
+
+def _mt_copy(g):
+    return np.array(g.mt, dtype=np.uint32), int(g.mti)
+
+
+def test_cluster_labels_v251_every_call(msc):
+    """clust_conf of v25/v251/run1 (min_s = 0.1, every 10 of 101 steps) on the
+    GPU: the oracle replays the run and, at each of the 11 calls, the library
+    gets the same state and `ran` generator state.  Labels, cluster count and
+    the advanced generator state equal the oracle's bit for bit, and the last
+    call reproduces the 478 clusters of the reference's conf.dat."""
+    import ctypes
+    S, P = H.v251_state()
+    P.cl_fr = 0  # the clustering calls are made here, in the reference's order
+    R = O.ORun(S, P, seed_init=True)
+    rlmax = R.R.r_list_max
+    sysm = msc.MdSystem([H.v251_desc()], precision="fp64")
+    try:
+        for step in range(101):
+            R.run(1)
+            if step % 10:
+                continue
+            _say("case: cluster labels v251 after step %d" % step)
+            H.load_osys(sysm, 0, S)
+            sysm.neigh_list(0, rlmax)
+            st, mti = _mt_copy(R.R.ran)
+            labels, nc, st2, mti2, nasym = sysm.cluster_labels(0, 0.1, st, mti)
+            n_ref = O.lib().o_clust_conf(S.c(), 0.1, ctypes.byref(R.R.ran))
+            assert nasym == 0
+            assert nc == n_ref
+            assert np.array_equal(labels, S.clust)
+            st_ref, mti_ref = _mt_copy(R.R.ran)
+            assert mti2 == mti_ref and np.array_equal(st2, st_ref)
+    finally:
+        sysm.close()
+        R.free()
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "v251_run1_conf.txt")
+    lines = open(path).read().splitlines()
+    ref = np.array([int(lines[1 + i].split()[-1]) for i in range(500)])
+    assert np.array_equal(labels, ref) and nc == 478
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_cluster_labels_dense_after_run(msc, precision):
+    """A denser system (N=1500, packing fraction 0.25, min_s = 0.6, so that
+    clusters of many particles form) after 120 steps of msc_run, which keeps
+    the neighbour list: labels and cluster count equal the oracle's
+    clust_conf on the state read back, for three successive calls on one
+    generator stream."""
+    import ctypes
+    from mdspherocyl_amd import configs
+    rng = np.random.default_rng(31)
+    N = 1500
+    d, st = configs.make_replica(rng, N, configs.truncated_normal(rng, N, 3.0, 0.3), 0.25, 1.5, 5,
+                                 lambda b: 3.0 / 3.5, 3.5)
+    _say("case: cluster labels N=%d phi=0.25 after 120 steps %s" % (N, precision))
+    sysm = msc.MdSystem([d], precision=precision)
+    try:
+        configs.load_into(sysm, [(d, st)])
+        sysm.init_run()
+        sysm.run(120, observe=False)
+        g = O.COmt()
+        O.lib().omt_seed(ctypes.byref(g), 12345)
+        r, p, l, ori = sysm.get_state(0)
+        S = H.osys_from_config(d, dict(st, r=r, p=p, l=l, ori=ori))
+        sizes = []
+        for call in range(3):
+            st0, mti = _mt_copy(g)
+            labels, nc, st2, mti2, nasym = sysm.cluster_labels(0, 0.6, st0, mti)
+            n_ref = O.lib().o_clust_conf(S.c(), 0.6, ctypes.byref(g))
+            assert nasym == 0
+            assert nc == n_ref
+            assert np.array_equal(labels, S.clust), "call %d" % call
+            assert (mti2, list(st2)) == (_mt_copy(g)[1], list(_mt_copy(g)[0]))
+            sizes.append(np.bincount(labels).max())
+        _say("  clusters %d, largest %d" % (nc, sizes[-1]))
+        assert sizes[-1] > 5
+    finally:
+        sysm.close()
