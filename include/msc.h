@@ -144,8 +144,9 @@ int msc_init_run(msc_ctx *ctx);
  * [nsteps][nrep][MSC_OBS_N] values.  step0 is the global step number of the
  * first step (renormalisation every 100 steps counts from it). */
 int msc_run(msc_ctx *ctx, int step0, int nsteps, double *obs);
-/* 1: replay each step as one captured HIP graph (default), 0: launch
- * kernels directly. */
+/* Step submission mode.  The step is enqueued as direct kernel launches
+ * (about 15 per step); the flag is stored and has no other effect in this
+ * version. */
 int msc_set_graph(msc_ctx *ctx, int on);
 /* msc_run advances in chunks of 50 steps.  A chunk during which a list
  * rebuild outgrew its device region is rolled back, the regions are enlarged

@@ -100,6 +100,7 @@ struct msc_ctx {
   double *d_dpc = nullptr, *d_doc = nullptr, *d_dmu = nullptr, *d_dpos = nullptr, *d_dori = nullptr, *d_snap = nullptr,
          *d_snap_e = nullptr, *d_snap_u = nullptr;
   int *d_clu = nullptr, *d_clu_flag = nullptr;
+  double *d_rows = nullptr, *d_kblk = nullptr;  // per-task row sets and k-space block sums of the step loop
   int *d_cell_of = nullptr, *d_cell_cnt = nullptr, *d_cell_start = nullptr, *d_cell_fill = nullptr,
       *d_cell_list = nullptr;
   int *d_cell_tmp = nullptr;
@@ -608,6 +609,8 @@ int msc_create(int device, int precision, int nrep, const msc_replica_desc *reps
   A(&ctx->d_snap_u, 3 * N);
   A(&ctx->d_clu, N);
   A(&ctx->d_clu_flag, 2);
+  A(&ctx->d_rows, ctx->tasks.size() * 16 * kRowSets);
+  A(&ctx->d_kblk, (size_t)nrep * ((ctx->maxslot + 63) / 64) * 10);
   A(&ctx->d_cell_of, N); A(&ctx->d_cell_list, N);
   A(&ctx->d_cell_tmp, N);
   A(&ctx->d_cell_pack, (size_t)8 * N);
@@ -690,7 +693,7 @@ void msc_destroy(msc_ctx *ctx) {
                   ctx->d_cell_cnt, ctx->d_cell_start, ctx->d_cell_fill, ctx->d_cell_list, ctx->d_nl_cnt,
                   ctx->d_nl_idx, ctx->d_dl_cnt, ctx->d_dl_idx, ctx->d_kval, ctx->d_cell_tmp, ctx->d_sl_cnt,
                   ctx->d_sl_idx, ctx->d_snap_e, ctx->d_srebuild, ctx->d_cell_pack, 
-                  ctx->d_snap_u, ctx->d_clu, ctx->d_clu_flag,
+                  ctx->d_snap_u, ctx->d_clu, ctx->d_clu_flag, ctx->d_rows, ctx->d_kblk,
                   ctx->d_tab, ctx->d_ph, ctx->d_amp, ctx->d_part, ctx->d_A, ctx->d_slot_out, ctx->d_task_out,
                   ctx->d_red, ctx->d_M2, ctx->d_Kin, ctx->d_gs, ctx->d_obs, ctx->d_counts, ctx->d_E,
                   ctx->d_th, ctx->d_flags, ctx->d_rebuild, ctx->d_one, ctx->d_run_gs, ctx->d_run_obs, ctx->d_save};
@@ -823,11 +826,11 @@ int msc_set_thermostat(msc_ctx *ctx, int rep, const double s[2], const double st
 // --------------------------------------------------------------------------
 // neighbour lists
 // --------------------------------------------------------------------------
-static int lists_build(msc_ctx *ctx, const int *d_flagmask) {
+static int lists_build(msc_ctx *ctx, const int *d_flagmask, bool zero_cells = true) {
   State s = sel(ctx, MSC_ALL);
   {
     Bracket b(ctx, "nlist");
-    launch_cells(s, d_flagmask, ctx->stream);
+    launch_cells(s, d_flagmask, zero_cells, ctx->stream);
     launch_lists(s, d_flagmask, ctx->d_flags, ctx->stream);
   }
   return MSC_OK;
@@ -885,7 +888,7 @@ static int lists_layout(msc_ctx *ctx, const std::vector<long> &need_nl = {},
   refresh_state_ptrs(ctx);
   State s = sel(ctx, MSC_ALL);
   launch_fill_int(ctx->d_flags, 4 * nrep, 0, ctx->stream);
-  launch_cells(s, ctx->d_one, ctx->stream);
+  launch_cells(s, ctx->d_one, true, ctx->stream);
   launch_lists(s, ctx->d_one, ctx->d_flags, ctx->stream);
   std::vector<int> fl(4 * nrep);
   if (int rc_ = d2h(ctx, fl.data(), ctx->d_flags, sizeof(int) * fl.size())) return rc_;
@@ -1034,7 +1037,8 @@ int msc_force_sc(msc_ctx *ctx, int rep, double r_cut, double *U, double *W, doub
   return MSC_OK;
 }
 
-static void do_dipR(msc_ctx *ctx, int rep) {
+// real-space launches; per-task sums into task_out
+static void dipR_launches(msc_ctx *ctx, int rep, double *task_out) {
   State s = sel(ctx, rep);
   bool all = true;
   for (int r = (rep == MSC_ALL ? 0 : rep); r < (rep == MSC_ALL ? ctx->nrep : rep + 1); ++r) all &= ctx->reps[r].rc <= 0;
@@ -1042,15 +1046,20 @@ static void do_dipR(msc_ctx *ctx, int rep) {
   for (int r = (rep == MSC_ALL ? 0 : rep); r < (rep == MSC_ALL ? ctx->nrep : rep + 1); ++r) none &= !(ctx->reps[r].rc <= 0);
   {
     Bracket b(ctx, "real_pairs");
-    if (all) launch_dipR(s, ctx->mixed, true, ctx->d_tab, ctx->d_task_out, ctx->stream);
-    else if (none) launch_dipR(s, ctx->mixed, false, ctx->d_tab, ctx->d_task_out, ctx->stream);
+    if (all) launch_dipR(s, ctx->mixed, true, ctx->d_tab, task_out, ctx->stream);
+    else if (none) launch_dipR(s, ctx->mixed, false, ctx->d_tab, task_out, ctx->stream);
     else {
       for (int r = 0; r < ctx->nrep; ++r) {
         State s1 = sel(ctx, r);
-        launch_dipR(s1, ctx->mixed, ctx->reps[r].rc <= 0, ctx->d_tab, ctx->d_task_out, ctx->stream);
+        launch_dipR(s1, ctx->mixed, ctx->reps[r].rc <= 0, ctx->d_tab, task_out, ctx->stream);
       }
     }
   }
+}
+
+static void do_dipR(msc_ctx *ctx, int rep) {
+  State s = sel(ctx, rep);
+  dipR_launches(ctx, rep, ctx->d_task_out);
   launch_reduce(s, ctx->d_task_out, 16, 10, 0, (double *)ctx->d_E, kEstride, kEoff_R, ctx->stream, 1, ctx->d_counts,
                 4, 1);
 }
@@ -1067,10 +1076,15 @@ int msc_force_dipR(msc_ctx *ctx, int rep, double *U, double PT[9]) {
   return MSC_OK;
 }
 
-static int do_dipK(msc_ctx *ctx, int rep) {
+// Reciprocal launches.  Standalone (m2rows null): phase tables, M2 and the
+// per-replica energy sums here.  In the step loop the tables and the c c^T
+// rows come from k_drift (m2rows) and the energies go to kblk for k_end.
+static int dipK_launches(msc_ctx *ctx, int rep, const double *m2rows, double *kblk) {
   State s = sel(ctx, rep);
-  launch_recip_tables(s, ctx->mixed, ctx->d_ph, ctx->d_amp, ctx->d_task_out, ctx->stream);
-  launch_reduce(s, ctx->d_task_out, 16, 6, 0, ctx->d_M2, 6, 0, ctx->stream);
+  if (!m2rows) {
+    launch_recip_tables(s, ctx->mixed, ctx->d_ph, ctx->d_amp, ctx->d_task_out, ctx->stream);
+    launch_reduce(s, ctx->d_task_out, 16, 6, 0, ctx->d_M2, 6, 0, ctx->stream);
+  }
   // structure factors per group of replicas sharing KZ
   for (int kz : ctx->kz_groups) {
     int rt0 = -1, rt1 = -1;
@@ -1096,8 +1110,9 @@ static int do_dipK(msc_ctx *ctx, int rep) {
                               ctx->d_amp, ctx->d_part, ctx->stream);
     }
   }
-  launch_recip_reduce(s, ctx->mixed, ctx->d_part, ctx->d_M2, ctx->d_A, ctx->d_slot_out, ctx->maxslot, ctx->stream);
-  launch_reduce(s, ctx->d_slot_out, 10, 10, 1, (double *)ctx->d_E, kEstride, kEoff_K, ctx->stream);
+  launch_recip_reduce(s, ctx->mixed, ctx->d_part, m2rows ? nullptr : ctx->d_M2, m2rows, ctx->d_A, ctx->d_slot_out,
+                      kblk, ctx->maxslot, ctx->stream);
+  if (!kblk) launch_reduce(s, ctx->d_slot_out, 10, 10, 1, (double *)ctx->d_E, kEstride, kEoff_K, ctx->stream);
   for (int kz : ctx->kz_groups) {
     for (int r = 0; r < ctx->nrep; ++r) {
       if (ctx->reps[r].KZ != kz || (rep != MSC_ALL && r != rep)) continue;
@@ -1117,6 +1132,8 @@ static int do_dipK(msc_ctx *ctx, int rep) {
   }
   return MSC_OK;
 }
+
+static int do_dipK(msc_ctx *ctx, int rep) { return dipK_launches(ctx, rep, nullptr, nullptr); }
 
 int msc_force_dipK(msc_ctx *ctx, int rep, double *U, double PT[9]) {
   if (!ctx || rep < 0 || rep >= ctx->nrep) return MSC_E_ARG;
@@ -1205,6 +1222,12 @@ int msc_renorm(msc_ctx *ctx, int rep) {
 // --------------------------------------------------------------------------
 // the step loop
 // --------------------------------------------------------------------------
+static bool any_field(const msc_ctx *ctx) {
+  bool field = false;
+  for (auto &R : ctx->reps) field |= R.B[0] != 0 || R.B[1] != 0 || R.B[2] != 0;
+  return field;
+}
+
 static void forces_all(msc_ctx *ctx) {
   State s = sel(ctx, MSC_ALL);
   launch_zero_FT(s, ctx->stream);
@@ -1213,9 +1236,7 @@ static void forces_all(msc_ctx *ctx) {
   do_dipK(ctx, MSC_ALL);
   // ext_field adds exactly nothing without a field: skipped then (U_ext
   // stays 0)
-  bool field = false;
-  for (auto &R : ctx->reps) field |= R.B[0] != 0 || R.B[1] != 0 || R.B[2] != 0;
-  if (field) {
+  if (any_field(ctx)) {
     launch_ext_field(s, ctx->d_task_out, ctx->stream);
     launch_reduce(s, ctx->d_task_out, 16, 1, 0, (double *)ctx->d_E, kEstride, kEoff_ext, ctx->stream);
   }
@@ -1252,35 +1273,37 @@ int msc_set_graph(msc_ctx *ctx, int on) {
   return MSC_OK;
 }
 
-// one step of every replica, enqueued on the context stream
+// One step of every replica, enqueued on the context stream, in the order of
+// v25/main.cpp:559-708 (the fused kernels are described in msc_kernels.hip).
 static void enqueue_step(msc_ctx *ctx, int step, const double *d_gs_step, double *d_obs_step) {
   State s = sel(ctx, MSC_ALL);
   hipStream_t st = ctx->stream;
+  const long region = (long)ctx->tasks.size() * 16;
+  double *rows = ctx->d_rows;
   if (std::fmod((float)step, (float)100) == 0) launch_renorm(s, st);
-  launch_thermo0(s, ctx->d_th, d_gs_step, st);
   {
     Bracket b(ctx, "integrate");
-    launch_moveP(s, false, ctx->d_th, ctx->d_task_out, st);
+    launch_kick1(s, ctx->d_th, d_gs_step, rows + kRowKick1 * region, st);
+    launch_drift(s, ctx->mixed, ctx->d_ph, ctx->d_amp, rows + kRowDrift * region, st);
   }
-  launch_reduce(s, ctx->d_task_out, 16, 8, 0, ctx->d_red, kRedStride, 0, st);
-  launch_step_scalars(s, 1, ctx->d_red, ctx->d_Kin, ctx->d_th, ctx->d_E, nullptr, st);
+  {
+    Bracket b(ctx, "wca");
+    launch_force_sc(s, !ctx->mixed, rows + kRowSC * region, ctx->d_flags, st);
+  }
+  dipR_launches(ctx, MSC_ALL, rows + kRowR * region);
+  dipK_launches(ctx, MSC_ALL, rows + kRowDrift * region + 2, ctx->d_kblk);
+  // ext_field adds exactly nothing without a field: skipped then
+  const bool field = any_field(ctx);
+  if (field) launch_ext_field(s, rows + kRowExt * region, st);
   {
     Bracket b(ctx, "integrate");
-    launch_moveX(s, st);
-    launch_dipoles(s, st);
+    launch_kick2(s, ctx->d_th, d_gs_step, rows + kRowKick1 * region, rows + kRowKick2 * region, st);
+    launch_end(s, ctx->d_th, d_gs_step, rows, region, field, ctx->d_kblk, ctx->maxslot, ctx->d_E, ctx->d_Kin,
+               ctx->d_counts, d_obs_step, ctx->d_rebuild, ctx->d_srebuild, st);
   }
-  forces_all(ctx);
-  {
-    Bracket b(ctx, "integrate");
-    launch_moveP(s, true, ctx->d_th, ctx->d_task_out, st);
-  }
-  launch_reduce(s, ctx->d_task_out, 16, 14, 0, ctx->d_red, kRedStride, 0, st);
-  launch_step_scalars(s, 2, ctx->d_red, ctx->d_Kin, ctx->d_th, ctx->d_E, d_obs_step, st);
-  // list maintenance (v25/main.cpp:679-701): decide on the device, rebuild
-  // inside kernels that return at once for replicas that keep their list
-  launch_check(s, ctx->d_task_out, ctx->d_rebuild, ctx->d_srebuild, d_obs_step + MSC_OBS_REBUILD, MSC_OBS_N, false,
-               st);
-  lists_build(ctx, ctx->d_rebuild);
+  // list maintenance (v25/main.cpp:679-701): decided in k_end; the rebuild
+  // kernels return at once for replicas that keep their list
+  lists_build(ctx, ctx->d_rebuild, false);
   if (ctx->mixed) launch_short(s, ctx->d_srebuild, ctx->d_flags, st);
 }
 

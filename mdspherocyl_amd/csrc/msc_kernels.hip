@@ -75,9 +75,7 @@ __device__ __forceinline__ R fmaR(R a, R b, R c) { return fma(a, b, c); }
 // dipole lab frame: position r + dpc * u, orientation O^T c
 // (Spherocyl::set_dip / translate / rotate, v25/classes.cpp:148-304)
 // --------------------------------------------------------------------------
-__global__ void k_dipoles(State s) {
-  int g = s.g_begin + blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= s.g_end) return;
+__device__ __forceinline__ void dipoles_particle(const State &s, int g) {
   const int N = s.Ntot;
   double O[9];
   for (int k = 0; k < 9; ++k) O[k] = s.ori[k * N + g];
@@ -95,6 +93,11 @@ __global__ void k_dipoles(State s) {
       s.dori[(3 * w + a) * N + g] = acc;
     }
   }
+}
+
+__global__ void k_dipoles(State s) {
+  int g = s.g_begin + blockIdx.x * blockDim.x + threadIdx.x;
+  if (g < s.g_end) dipoles_particle(s, g);
 }
 
 // --------------------------------------------------------------------------
@@ -162,7 +165,9 @@ __global__ void k_cells_fill(State s, const int *__restrict__ rebuild) {
 
 // ... then into ascending index order: each particle's rank among the
 // members of its cell is the number of members with a smaller index.  Every
-// later traversal of a cell is in a fixed order.
+// later traversal of a cell is in a fixed order.  The particle also writes
+// its centre, axis, length and index at its cell-order position t, packed
+// into one 64-byte record so that a candidate costs one cache line.
 __global__ void k_cells_rank(State s, const int *__restrict__ rebuild) {
   int g = s.g_begin + blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= s.g_end) return;
@@ -171,16 +176,8 @@ __global__ void k_cells_rank(State s, const int *__restrict__ rebuild) {
   int b = s.cell_start[c], n = s.cell_cnt[c];
   int rank = 0;
   for (int k = 0; k < n; ++k) rank += s.cell_tmp[b + k] < g;
-  s.cell_list[b + rank] = g;
-}
-
-// centre, axis, length and index of the particle at cell-order position t,
-// packed into one 64-byte record so that a candidate costs one cache line
-__global__ void k_cells_pack(State s, const int *__restrict__ rebuild) {
-  int t = s.g_begin + blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= s.g_end) return;
-  if (!rebuild[s.rep_of[t]]) return;
-  const int g = s.cell_list[t], N = s.Ntot;
+  const int t = b + rank, N = s.Ntot;
+  s.cell_list[t] = g;
   double *o = s.cell_pack + (size_t)t * 8;
   o[0] = s.x[g];
   o[1] = s.x[N + g];
@@ -190,6 +187,16 @@ __global__ void k_cells_pack(State s, const int *__restrict__ rebuild) {
   o[5] = s.ori[8 * N + g];
   o[6] = s.len[g];
   o[7] = (double)g;
+}
+
+// nearest double to x rounded to 12 decimals (text round trip of savec_list)
+__device__ double round12(double x) {
+  double p = x * 1e12;
+  double e = fma(x, 1e12, -p);  // exact residual of the product
+  double m = floor(p);
+  double f = p - m;             // exact, |p| < 2^52
+  if (f > 0.5 || (f == 0.5 && (e > 0 || (e == 0 && fmod(m, 2.0) != 0)))) m += 1;
+  return m / 1e12;
 }
 
 __device__ __forceinline__ size_t row_at(long base, int N, int n0, int q, int g) {
@@ -251,6 +258,15 @@ __global__ void __launch_bounds__(256) k_build_lists(State s, const int *__restr
   const double ug[3] = {pg[3], pg[4], pg[5]};
   const double Lg = pg[6];
   const int g = __builtin_amdgcn_readfirstlane((int)pg[7]);
+  if (lane < 3) {
+    // list snapshot (savec_list): rounded centre, and the centre and axis
+    // from which contact-row and cluster displacements are measured
+    const int a = lane, N = s.Ntot;
+    s.snap[a * N + g] = round12(s.x[a * N + g]);
+    s.snap_e[a * N + g] = s.x[a * N + g];
+    s.snap_e[(3 + a) * N + g] = s.ori[(6 + a) * N + g];
+    s.snap_u[a * N + g] = s.ori[(6 + a) * N + g];
+  }
   int ci[3];
   {
     int c = s.cell_of[g] - R.cell0;
@@ -402,31 +418,6 @@ __global__ void __launch_bounds__(256) k_short_build(State s, const int *__restr
 // --------------------------------------------------------------------------
 // savec_list / check_list (v25/functions.cpp:1336-1394)
 // --------------------------------------------------------------------------
-// nearest double to x rounded to 12 decimals (text round trip of savec_list)
-__device__ double round12(double x) {
-  double p = x * 1e12;
-  double e = fma(x, 1e12, -p);  // exact residual of the product
-  double m = floor(p);
-  double f = p - m;             // exact, |p| < 2^52
-  if (f > 0.5 || (f == 0.5 && (e > 0 || (e == 0 && fmod(m, 2.0) != 0)))) m += 1;
-  return m / 1e12;
-}
-
-// list snapshot of a rebuilt replica: rounded centres (savec_list) and the
-// centre and axis from which contact-list displacements are measured
-__global__ void k_savec(State s, const int *__restrict__ rebuild) {
-  int g = s.g_begin + blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= s.g_end) return;
-  if (!rebuild[s.rep_of[g]]) return;
-  const int N = s.Ntot;
-  for (int a = 0; a < 3; ++a) {
-    s.snap[a * N + g] = round12(s.x[a * N + g]);
-    s.snap_e[a * N + g] = s.x[a * N + g];
-    s.snap_e[(3 + a) * N + g] = s.ori[(6 + a) * N + g];
-    s.snap_u[a * N + g] = s.ori[(6 + a) * N + g];
-  }
-}
-
 // per wave task: [0] largest centre-component displacement since savec_list
 // (check_list), [1] largest displacement of any point of a segment since the
 // contact list was built: |dr| + L/2 |du|
@@ -896,9 +887,43 @@ template <typename R>
 struct Cpx { R re, im; };
 
 
-// per-particle phase tables e^{-i n theta_a}, n = 0..kmax, theta_a = 2 pi x_a / L_a of
-// the wrapped dipole position (as force_dipK wraps, v25/functions.cpp:1793-1794),
-// and the amplitude c = mu u; also per-task partial sums of c c^T.
+// Phase table e^{-i n theta}, n = 0..KZ-1, of one axis a of particle g, for
+// theta = 2 pi x_a / L_a of the wrapped dipole position (as force_dipK
+// wraps, v25/functions.cpp:1793-1794), by the recurrence
+// e^{-i (n+1) theta} = e^{-i n theta} e^{-i theta} in FP64 (one sincos).
+__device__ __forceinline__ double recip_theta(const State &s, const RepDev &Rp, int g, int a) {
+  double x = s.dpos[a * s.Ntot + g];
+  x -= Rp.L[a] * floor(x / Rp.L[a] + 0.5);
+  return Rp.P2[a] * x;
+}
+template <typename R>
+__device__ __forceinline__ void recip_phase_axis(const State &s, const RepDev &Rp, int g, int a, double th,
+                                                 Cpx<R> *__restrict__ ph) {
+  double sn, cs;
+  sincos(th, &sn, &cs);
+  double er = 1.0, ei = 0.0;
+  for (int n = 0; n < Rp.KZ; ++n) {
+    Cpx<R> e;
+    e.re = (R)er;
+    e.im = (R)ei;
+    ph[((size_t)a * kMaxKZ + n) * s.Ntot + g] = e;
+    const double nr = er * cs + ei * sn;
+    ei = ei * cs - er * sn;
+    er = nr;
+  }
+}
+// amplitude c = mu u of particle g; m2 receives c c^T
+template <typename R>
+__device__ __forceinline__ void recip_amp_particle(const State &s, int g, R *__restrict__ amp, double m2[6]) {
+  const int N = s.Ntot;
+  const double mu = s.dmu[g];
+  const double c[3] = {mu * s.dori[g], mu * s.dori[N + g], mu * s.dori[2 * N + g]};
+  for (int a = 0; a < 3; ++a) amp[a * N + g] = (R)c[a];
+  m2[0] = c[0] * c[0]; m2[1] = c[0] * c[1]; m2[2] = c[0] * c[2];
+  m2[3] = c[1] * c[1]; m2[4] = c[1] * c[2]; m2[5] = c[2] * c[2];
+}
+
+// phase tables and amplitudes of every particle; per-task sums of c c^T
 template <typename R>
 __global__ void __launch_bounds__(256) k_recip_tables(State s, Cpx<R> *__restrict__ ph, R *__restrict__ amp,
                                double *__restrict__ task_out) {
@@ -909,25 +934,9 @@ __global__ void __launch_bounds__(256) k_recip_tables(State s, Cpx<R> *__restric
   const RepDev &Rp = s.reps[tk.rep];
   double m2[6] = {0, 0, 0, 0, 0, 0};
   if (lane < tk.n) {
-    const int g = tk.g0 + lane, N = s.Ntot;
-    for (int a = 0; a < 3; ++a) {
-      double x = s.dpos[a * N + g];
-      x -= Rp.L[a] * floor(x / Rp.L[a] + 0.5);
-      double th = Rp.P2[a] * x;
-      for (int n = 0; n < Rp.KZ; ++n) {
-        double sn, cs;
-        sincos(n * th, &sn, &cs);
-        Cpx<R> e;
-        e.re = (R)cs;
-        e.im = (R)(-sn);
-        ph[((size_t)a * kMaxKZ + n) * N + g] = e;
-      }
-    }
-    double mu = s.dmu[g];
-    double c[3] = {mu * s.dori[g], mu * s.dori[N + g], mu * s.dori[2 * N + g]};
-    for (int a = 0; a < 3; ++a) amp[a * N + g] = (R)c[a];
-    m2[0] = c[0] * c[0]; m2[1] = c[0] * c[1]; m2[2] = c[0] * c[2];
-    m2[3] = c[1] * c[1]; m2[4] = c[1] * c[2]; m2[5] = c[2] * c[2];
+    const int g = tk.g0 + lane;
+    for (int a = 0; a < 3; ++a) recip_phase_axis<R>(s, Rp, g, a, recip_theta(s, Rp, g, a), ph);
+    recip_amp_particle<R>(s, g, amp, m2);
   }
   wave_store_sums(task_out + (size_t)tid * 16, m2, 6, lane);
 }
@@ -1000,17 +1009,55 @@ __global__ void __launch_bounds__(256) k_recip_struct(State s, int rt0, const Cp
   }
 }
 
+// Sum over the tasks t0..t0+nt-1 of rows[t*16 + k], k < NV, by one wave:
+// lane l takes tasks l, l+64, ... in order, then an xor butterfly.  Every
+// lane ends with the same sums (each butterfly stage adds the same two
+// partials in either lane), and every wave that calls this with the same
+// rows gets the same result.
+template <int NV>
+__device__ __forceinline__ void wave_rows_sum(const double *__restrict__ rows, int t0, int nt, int lane, double *out) {
+  double v[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) v[k] = 0.0;
+  // eight rows per lane in flight at a time; the additions stay in row order
+  int t = lane;
+  for (; t + 7 * 64 < nt; t += 8 * 64) {
+    double r[8][NV];
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+#pragma unroll
+      for (int k = 0; k < NV; ++k) r[u][k] = rows[(size_t)(t0 + t + u * 64) * 16 + k];
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+#pragma unroll
+      for (int k = 0; k < NV; ++k) v[k] += r[u][k];
+  }
+  for (; t < nt; t += 64) {
+    const double *r = rows + (size_t)(t0 + t) * 16;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) v[k] += r[k];
+  }
+#pragma unroll
+  for (int k = 0; k < NV; ++k) out[k] = wave_sum(v[k]);
+}
+
 // Per slot: Q = sum over the replica's chunks (FP64, fixed order), S = k.Q,
 // energy (val/2)(|S|^2 - k^T M2 k) and the pressure-tensor term of
 // force_dipK (v25/functions.cpp:1926-1952, with its a4 = 1/(4 alpha));
 // A = val conj(S) for the back projection.
-// One block per (64 slots, replica): 16 waves each sum every 16th chunk
+// One block per (64 slots, replica): 8 waves each sum every 8th chunk
 // partial of their 64 slots in FP64 (coalesced rows of 64 slots), then one
-// wave combines the 16 sums in fixed order and finishes the slot.
+// wave combines the 8 sums in fixed order and finishes the slot.
+// M2 (sum of c c^T per replica) is read from M2 or, when M2 is null, summed
+// from the per-task rows m2rows (6 values each).  With kblk, wave 0 also
+// stores the sum of its 64 slots' 10 values as row blockIdx.x of the
+// replica (kblk_stride rows per replica).
 template <typename R>
-__global__ void __launch_bounds__(1024) k_recip_reduce(State s, const R *__restrict__ part, const double *__restrict__ M2,
-                                                       Cpx<R> *__restrict__ Aout, double *__restrict__ slot_out) {
-  constexpr int kGroups = 16;
+__global__ void __launch_bounds__(512) k_recip_reduce(State s, const R *__restrict__ part, const double *__restrict__ M2,
+                                                       const double *__restrict__ m2rows, Cpx<R> *__restrict__ Aout,
+                                                       double *__restrict__ slot_out, double *__restrict__ kblk,
+                                                       int kblk_stride) {
+  constexpr int kGroups = 8;
   __shared__ double sh[kGroups][6][64];
   const int rep = s.rep_begin + blockIdx.y;
   const RepDev &Rp = s.reps[rep];
@@ -1018,52 +1065,90 @@ __global__ void __launch_bounds__(1024) k_recip_reduce(State s, const R *__restr
   const int lane = threadIdx.x & 63, grp = threadIdx.x >> 6;
   const int sl = blockIdx.x * 64 + lane;
   if (blockIdx.x * 64 >= nslot) return;  // whole block idle (uniform)
+  const bool act = sl < nslot;
   double Q[6] = {0, 0, 0, 0, 0, 0};
-  if (sl < nslot)
-    for (int c = grp; c < Rp.nrtask; c += kGroups) {
+  if (act) {
+    // four chunk partials in flight at a time, added in chunk order
+    int c = grp;
+    for (; c + 3 * kGroups < Rp.nrtask; c += 4 * kGroups) {
+      R v[4][6];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const R *p = part + Rp.part0 + ((long)(c + u * kGroups) * nslot + sl) * 6;
+#pragma unroll
+        for (int q = 0; q < 6; ++q) v[u][q] = p[q];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int q = 0; q < 6; ++q) Q[q] += (double)v[u][q];
+    }
+    for (; c < Rp.nrtask; c += kGroups) {
       const R *p = part + Rp.part0 + ((long)c * nslot + sl) * 6;
+#pragma unroll
       for (int q = 0; q < 6; ++q) Q[q] += (double)p[q];
     }
+  }
+#pragma unroll
   for (int q = 0; q < 6; ++q) sh[grp][q][lane] = Q[q];
   __syncthreads();
-  if (grp != 0 || sl >= nslot) return;
+  if (grp != 0) return;
+#pragma unroll
   for (int q = 0; q < 6; ++q) {
     double t = 0;
     for (int g2 = 0; g2 < kGroups; ++g2) t += sh[g2][q][lane];
     Q[q] = t;
   }
+  double m[6];
+  if (M2) {
+#pragma unroll
+    for (int q = 0; q < 6; ++q) m[q] = M2[rep * 6 + q];
+  } else {
+    wave_rows_sum<6>(m2rows, Rp.task0, Rp.ntask, lane, m);
+  }
   const int kmax = Rp.kmax, side = 2 * kmax + 1;
-  const int col = sl / Rp.KZ, nz = sl % Rp.KZ;
+  const int col = (act ? sl : 0) / Rp.KZ, nz = (act ? sl : 0) % Rp.KZ;
   const int nx = col % side - kmax, ny = col / side - kmax;
-  double val = s.kval[Rp.slot0 + sl];
-  double *o = slot_out + (size_t)(Rp.slot0 + sl) * 10;
+  const double val = act ? s.kval[Rp.slot0 + sl] : 0.0;
+  double o[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
   Cpx<R> A;
   A.re = A.im = (R)0;
-  if (val == 0.0) {
-    for (int q = 0; q < 10; ++q) o[q] = 0.0;
-    Aout[Rp.slot0 + sl] = A;
-    return;
+  if (val != 0.0) {
+    double k[3] = {Rp.P2[0] * nx, Rp.P2[1] * ny, Rp.P2[2] * nz};
+    double Sr = k[0] * Q[0] + k[1] * Q[2] + k[2] * Q[4];
+    double Si = k[0] * Q[1] + k[1] * Q[3] + k[2] * Q[5];
+    double Mk[3] = {m[0] * k[0] + m[1] * k[1] + m[2] * k[2], m[1] * k[0] + m[3] * k[1] + m[4] * k[2],
+                    m[2] * k[0] + m[4] * k[1] + m[5] * k[2]};
+    double self = k[0] * Mk[0] + k[1] * Mk[1] + k[2] * Mk[2];
+    double nrm = Sr * Sr + Si * Si - self;
+    o[0] = val * 0.5 * nrm;
+    double Ksq = k[0] * k[0] + k[1] * k[1] + k[2] * k[2];
+    double pval = 2 * (1. / (4. * Rp.alpha) + 1. / Ksq);
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        double re = Sr * Q[2 * j] + Si * Q[2 * j + 1];  // Re(conj(S) Q_j)
+        o[1 + 3 * j + c] = val * 0.5 * (((j == c) ? 1.0 : 0.0) - pval * k[j] * k[c]) * nrm +
+                           val * 0.5 * 2 * (re - Mk[j]) * k[c];
+      }
+    A.re = (R)(val * Sr);
+    A.im = (R)(-val * Si);
   }
-  double k[3] = {Rp.P2[0] * nx, Rp.P2[1] * ny, Rp.P2[2] * nz};
-  double Sr = k[0] * Q[0] + k[1] * Q[2] + k[2] * Q[4];
-  double Si = k[0] * Q[1] + k[1] * Q[3] + k[2] * Q[5];
-  const double *m = M2 + rep * 6;
-  double Mk[3] = {m[0] * k[0] + m[1] * k[1] + m[2] * k[2], m[1] * k[0] + m[3] * k[1] + m[4] * k[2],
-                  m[2] * k[0] + m[4] * k[1] + m[5] * k[2]};
-  double self = k[0] * Mk[0] + k[1] * Mk[1] + k[2] * Mk[2];
-  double nrm = Sr * Sr + Si * Si - self;
-  o[0] = val * 0.5 * nrm;
-  double Ksq = k[0] * k[0] + k[1] * k[1] + k[2] * k[2];
-  double pval = 2 * (1. / (4. * Rp.alpha) + 1. / Ksq);
-  for (int j = 0; j < 3; ++j)
-    for (int c = 0; c < 3; ++c) {
-      double re = Sr * Q[2 * j] + Si * Q[2 * j + 1];  // Re(conj(S) Q_j)
-      o[1 + 3 * j + c] = val * 0.5 * (((j == c) ? 1.0 : 0.0) - pval * k[j] * k[c]) * nrm +
-                         val * 0.5 * 2 * (re - Mk[j]) * k[c];
+  if (act) {
+    double *os = slot_out + (size_t)(Rp.slot0 + sl) * 10;
+#pragma unroll
+    for (int q = 0; q < 10; ++q) os[q] = o[q];
+    Aout[Rp.slot0 + sl] = A;
+  }
+  if (kblk) {
+    double *kb = kblk + ((size_t)rep * kblk_stride + blockIdx.x) * 10;
+#pragma unroll
+    for (int q = 0; q < 10; ++q) {
+      const double t = wave_sum(o[q]);
+      if (lane == 0) kb[q] = t;
     }
-  A.re = (R)(val * Sr);
-  A.im = (R)(-val * Si);
-  Aout[Rp.slot0 + sl] = A;
+  }
 }
 
 // Back projection: for each particle, F = -sum_k a_k Im(Z_k) k and
@@ -1358,8 +1443,66 @@ __device__ __forceinline__ void mv3(const double *A, const double *x, double *y)
   y[0] = r[0]; y[1] = r[1]; y[2] = r[2];
 }
 
+// moveP of one particle (v25/functions.cpp:2071-2184) with thermostat
+// coordinates ts[2]; acc receives [0] p^2/m, [1] l^T Ib l and, after the
+// last kick, [2..7] the kinetic pressure tensor and [8..13] p and l.
+template <bool LAST>
+__device__ __forceinline__ void kick_particle(const State &s, const RepDev &Rp, int g, const double ts[2],
+                                              double acc[14]) {
+  const int N = s.Ntot;
+  const double Dt2 = Rp.dt / 2;
+  double m = s.mass[g];
+  double I[3] = {s.inert[g], s.inert[N + g], s.inert[2 * N + g]};
+  double O[9], Ot[9], I0[9] = {1. / I[0], 0., 0., 0., 1. / I[1], 0., 0., 0., 1. / I[2]}, Ax[9], Ib[9];
+  for (int k = 0; k < 9; ++k) O[k] = s.ori[k * N + g];
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b) Ot[a * 3 + b] = O[b * 3 + a];
+  mm3(Ot, I0, Ax);
+  mm3(Ax, O, Ib);
+  double p[3], l[3], psq = 0;
+  if (!LAST) {
+    double sfac = 1. + Dt2 * ts[0];
+    for (int j = 0; j < 3; ++j) {
+      p[j] = (s.p[j * N + g] + Dt2 * s.F[j * N + g]) / sfac;
+      psq += p[j] * p[j];
+    }
+    sfac = 1. + Dt2 * ts[1];
+    for (int j = 0; j < 3; ++j) l[j] = (s.l[j * N + g] + Dt2 * s.T[j * N + g]) / sfac;
+  } else {
+    double sfac = 1. - Dt2 * ts[0];
+    for (int j = 0; j < 3; ++j) {
+      p[j] = s.p[j * N + g] * sfac + Dt2 * s.F[j * N + g];
+      psq += p[j] * p[j];
+    }
+    acc[2] = p[0] * p[0] / m; acc[3] = p[0] * p[1] / m; acc[4] = p[0] * p[2] / m;
+    acc[5] = p[1] * p[1] / m; acc[6] = p[1] * p[2] / m; acc[7] = p[2] * p[2] / m;
+    sfac = 1. - Dt2 * ts[1];
+    for (int j = 0; j < 3; ++j) l[j] = s.l[j * N + g] * sfac + Dt2 * s.T[j * N + g];
+  }
+  double v[3];
+  for (int n = 0; n < 3; ++n) {
+    double a2 = 0.0;
+    for (int k = 0; k < 3; ++k) a2 += l[k] * Ib[k * 3 + n];
+    v[n] = a2;
+  }
+  double lres = 0.0;
+  for (int k = 0; k < 3; ++k) lres += v[k] * l[k];
+  double lsq = 0.0;
+  lsq += lres;
+  for (int j = 0; j < 3; ++j) {
+    s.p[j * N + g] = p[j];
+    s.l[j * N + g] = l[j];
+  }
+  acc[0] = psq / m;
+  acc[1] = lsq;
+  if (LAST)
+    for (int j = 0; j < 3; ++j) {
+      acc[8 + j] = p[j];
+      acc[11 + j] = l[j];
+    }
+}
+
 // moveP for every particle; thermostat coordinates read from device memory.
-// Per-task partial sums: [0] sum p^2/m, [1] sum l^T Ib l, [2..7] kinetic PT.
 template <bool LAST>
 __global__ void __launch_bounds__(256) k_moveP(State s, const Thermo *__restrict__ th, double *__restrict__ task_out) {
   int tid = s.task_begin + wave_task_id();
@@ -1367,71 +1510,16 @@ __global__ void __launch_bounds__(256) k_moveP(State s, const Thermo *__restrict
   if (wave_task_id() >= s.ntask) return;
   Task tk = s.tasks[tid];
   const RepDev &Rp = s.reps[tk.rep];
-  // [0] sum p^2/m, [1] sum l.I^-1.l, [2..7] kinetic pressure tensor, and
-  // after the last kick [8..13] the total momentum and angular momentum
   double acc[14] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
   if (lane < tk.n) {
-    const int g = tk.g0 + lane, N = s.Ntot;
-    const double Dt2 = Rp.dt / 2;
-    double m = s.mass[g];
-    double I[3] = {s.inert[g], s.inert[N + g], s.inert[2 * N + g]};
-    double O[9], Ot[9], I0[9] = {1. / I[0], 0., 0., 0., 1. / I[1], 0., 0., 0., 1. / I[2]}, Ax[9], Ib[9];
-    for (int k = 0; k < 9; ++k) O[k] = s.ori[k * N + g];
-    for (int a = 0; a < 3; ++a)
-      for (int b = 0; b < 3; ++b) Ot[a * 3 + b] = O[b * 3 + a];
-    mm3(Ot, I0, Ax);
-    mm3(Ax, O, Ib);
-    double p[3], l[3], psq = 0;
     const Thermo T = th[tk.rep];
-    if (!LAST) {
-      double sfac = 1. + Dt2 * T.s[0];
-      for (int j = 0; j < 3; ++j) {
-        p[j] = (s.p[j * N + g] + Dt2 * s.F[j * N + g]) / sfac;
-        psq += p[j] * p[j];
-      }
-      sfac = 1. + Dt2 * T.s[1];
-      for (int j = 0; j < 3; ++j) l[j] = (s.l[j * N + g] + Dt2 * s.T[j * N + g]) / sfac;
-    } else {
-      double sfac = 1. - Dt2 * T.s[0];
-      for (int j = 0; j < 3; ++j) {
-        p[j] = s.p[j * N + g] * sfac + Dt2 * s.F[j * N + g];
-        psq += p[j] * p[j];
-      }
-      acc[2] = p[0] * p[0] / m; acc[3] = p[0] * p[1] / m; acc[4] = p[0] * p[2] / m;
-      acc[5] = p[1] * p[1] / m; acc[6] = p[1] * p[2] / m; acc[7] = p[2] * p[2] / m;
-      sfac = 1. - Dt2 * T.s[1];
-      for (int j = 0; j < 3; ++j) l[j] = s.l[j * N + g] * sfac + Dt2 * s.T[j * N + g];
-    }
-    double v[3];
-    for (int n = 0; n < 3; ++n) {
-      double a2 = 0.0;
-      for (int k = 0; k < 3; ++k) a2 += l[k] * Ib[k * 3 + n];
-      v[n] = a2;
-    }
-    double lres = 0.0;
-    for (int k = 0; k < 3; ++k) lres += v[k] * l[k];
-    double lsq = 0.0;
-    lsq += lres;
-    for (int j = 0; j < 3; ++j) {
-      s.p[j * N + g] = p[j];
-      s.l[j * N + g] = l[j];
-    }
-    acc[0] = psq / m;
-    acc[1] = lsq;
-    if (LAST)
-      for (int j = 0; j < 3; ++j) {
-        acc[8 + j] = p[j];
-        acc[11 + j] = l[j];
-      }
+    kick_particle<LAST>(s, Rp, tk.g0 + lane, T.s, acc);
   }
   wave_store_sums(task_out + (size_t)tid * 16, acc, LAST ? 14 : 8, lane);
 }
 
 // translate + rotate (5-factor Rx Ry Rz Ry Rx splitting) + dipole update
-__global__ void k_moveX(State s) {
-  int g = s.g_begin + blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= s.g_end) return;
-  const RepDev &Rp = s.reps[s.rep_of[g]];
+__device__ __forceinline__ void drift_particle(const State &s, const RepDev &Rp, int g) {
   const int N = s.Ntot;
   const double dt = Rp.dt, dt2 = dt / 2;
   double m = s.mass[g];
@@ -1461,6 +1549,11 @@ __global__ void k_moveX(State s) {
   for (int k = 0; k < 9; ++k) s.ori[k * N + g] = Q[k];
 }
 
+__global__ void k_moveX(State s) {
+  int g = s.g_begin + blockIdx.x * blockDim.x + threadIdx.x;
+  if (g < s.g_end) drift_particle(s, s.reps[s.rep_of[g]], g);
+}
+
 __global__ void k_renorm(State s) {
   int g = s.g_begin + blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= s.g_end) return;
@@ -1488,16 +1581,22 @@ __global__ void k_renorm(State s) {
 // moveTher mode 0 (v25/functions.cpp:2186-2218) for every replica (one
 // thread each): gs[rep*4 + 2*i + 0/1] hold the Box-Muller pair drawn on the
 // host from thermostat i's MT19937 for this step.
+__device__ __forceinline__ double thermo_half(const RepDev &R, double s, double r) {
+  double ex1 = exp(-R.gamma * R.dt / 2);
+  return ex1 * s + sqrt(2. * R.temp * R.gamma * R.dt / R.Q) * r;
+}
+__device__ __forceinline__ double thermo_mode1(const RepDev &R, double s, double K2) {
+  return s + R.dt * (2. * K2 - 3. * (double)R.N * R.temp) / R.Q;
+}
+
 __global__ void k_thermo0(State s, Thermo *__restrict__ th, const double *__restrict__ gs) {
   int rp = blockIdx.x * blockDim.x + threadIdx.x;
   if (rp >= s.nrep) return;
   const RepDev &R = s.reps[rp];
   Thermo T = th[rp];
   for (int i = 0; i < 2; ++i) {
-    double r0 = gs[rp * 4 + 2 * i], r1 = gs[rp * 4 + 2 * i + 1];
-    T.strn[i] = r1;
-    double ex1 = exp(-R.gamma * R.dt / 2);
-    T.s[i] = ex1 * T.s[i] + sqrt(2. * R.temp * R.gamma * R.dt / R.Q) * r0;
+    T.strn[i] = gs[rp * 4 + 2 * i + 1];
+    T.s[i] = thermo_half(R, T.s[i], gs[rp * 4 + 2 * i]);
   }
   th[rp] = T;
 }
@@ -1521,12 +1620,8 @@ __global__ void k_step_scalars(State s, const double *__restrict__ red, double *
   Kin[rp * 2 + 1] = K2[1];
   Thermo T = th[rp];
   for (int i = 0; i < 2; ++i) {
-    if constexpr (MODE == 1) {
-      T.s[i] = T.s[i] + R.dt * (2. * K2[i] - 3. * (double)R.N * R.temp) / R.Q;
-    } else {
-      double ex1 = exp(-R.gamma * R.dt / 2);
-      T.s[i] = ex1 * T.s[i] + sqrt(2. * R.temp * R.gamma * R.dt / R.Q) * T.strn[i];
-    }
+    if constexpr (MODE == 1) T.s[i] = thermo_mode1(R, T.s[i], K2[i]);
+    else T.s[i] = thermo_half(R, T.s[i], T.strn[i]);
   }
   th[rp] = T;
   if constexpr (MODE == 2) {
@@ -1556,6 +1651,218 @@ __global__ void k_zero_FT(State s) {
   }
 }
 
+// --------------------------------------------------------------------------
+// The step loop in few launches (v25/main.cpp:559-708).  The per-replica
+// sums a kernel needs are re-summed from per-task rows by every wave that
+// needs them (wave_rows_sum: same rows, same order, same result), so no
+// kernel waits for a separate reduction launch:
+//   k_kick1  moveTher mode 0 + first moveP           -> rows kick1 [0..1]
+//   k_drift  moveX, dipoles, F = T = 0, the list check and the reciprocal
+//            phase tables                             -> rows drift [0..7]
+//   (force kernels)                                   -> rows sc, R, ext
+//   k_kick2  moveTher mode 1 (from rows kick1) + last moveP -> rows kick2
+//   k_end    every per-replica sum, moveTher mode 2, energies, observables
+//            and the rebuild decision (one block per replica).
+// --------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_kick1(State s, const Thermo *__restrict__ th, const double *__restrict__ gs,
+                                               double *__restrict__ rows) {
+  const int tid = s.task_begin + wave_task_id();
+  const int lane = threadIdx.x & 63;
+  if (wave_task_id() >= s.ntask) return;
+  const Task tk = s.tasks[tid];
+  const RepDev &Rp = s.reps[tk.rep];
+  double acc[14] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  if (lane < tk.n) {
+    const Thermo T = th[tk.rep];
+    const double ts[2] = {thermo_half(Rp, T.s[0], gs[tk.rep * 4 + 0]), thermo_half(Rp, T.s[1], gs[tk.rep * 4 + 2])};
+    kick_particle<false>(s, Rp, tk.g0 + lane, ts, acc);
+  }
+  wave_store_sums(rows + (size_t)tid * 16, acc, 2, lane);
+}
+
+// One block of 4 waves per task.  Wave 0 moves the task's particles and
+// writes rows: [0] largest centre-component displacement since savec_list,
+// [1] largest segment-point displacement since the contact rows were built
+// (as k_check_list), [2..7] sums of c c^T (as k_recip_tables); waves 0-2
+// then write the phase tables of axes 0-2 (one particle per lane).
+template <typename R>
+__global__ void __launch_bounds__(256) k_drift(State s, Cpx<R> *__restrict__ ph, R *__restrict__ amp,
+                                               double *__restrict__ rows) {
+  __shared__ double th_sh[3][64];
+  const int tid = s.task_begin + __builtin_amdgcn_readfirstlane(blockIdx.x);
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const Task tk = s.tasks[tid];
+  const RepDev &Rp = s.reps[tk.rep];
+  const int N = s.Ntot;
+  if (wv == 0) {
+    double m = 0.0, me = 0.0, m2[6] = {0, 0, 0, 0, 0, 0};
+    if (lane < tk.n) {
+      const int g = tk.g0 + lane;
+      drift_particle(s, Rp, g);
+      dipoles_particle(s, g);
+      double dr2 = 0, du2 = 0;
+      for (int a = 0; a < 3; ++a) {
+        s.F[a * N + g] = 0.0;
+        s.T[a * N + g] = 0.0;
+        const double x = s.x[a * N + g];
+        double D = x - s.snap[a * N + g];
+        D -= Rp.L[a] * floor(D / Rp.L[a] + 0.5);
+        m = fmax(m, fabs(D));
+        double E = x - s.snap_e[a * N + g];
+        E -= Rp.L[a] * floor(E / Rp.L[a] + 0.5);
+        dr2 += E * E;
+        double U = s.ori[(6 + a) * N + g] - s.snap_e[(3 + a) * N + g];
+        du2 += U * U;
+        th_sh[a][lane] = recip_theta(s, Rp, g, a);
+      }
+      me = sqrt(dr2) + 0.5 * s.len[g] * sqrt(du2);
+      recip_amp_particle<R>(s, g, amp, m2);
+    }
+    m = wave_max(m);
+    me = wave_max(me);
+    double *o = rows + (size_t)tid * 16;
+    if (lane == 0) {
+      o[0] = m;
+      o[1] = me;
+    }
+    for (int k = 0; k < 6; ++k) {
+      const double t = wave_sum(m2[k]);
+      if (lane == 0) o[2 + k] = t;
+    }
+  }
+  __syncthreads();
+  if (wv < 3 && lane < tk.n) recip_phase_axis<R>(s, Rp, tk.g0 + lane, wv, th_sh[wv][lane], ph);
+}
+
+// moveTher mode 1 from the first kick's energies, re-summed by each wave,
+// then the last moveP.  rows2: [0..13] as kick_particle<true>.
+__global__ void __launch_bounds__(256) k_kick2(State s, const Thermo *__restrict__ th, const double *__restrict__ gs,
+                                               const double *__restrict__ rows1, double *__restrict__ rows2) {
+  const int tid = s.task_begin + wave_task_id();
+  const int lane = threadIdx.x & 63;
+  if (wave_task_id() >= s.ntask) return;
+  const Task tk = s.tasks[tid];
+  const RepDev &Rp = s.reps[tk.rep];
+  double K1[2];
+  wave_rows_sum<2>(rows1, Rp.task0, Rp.ntask, lane, K1);
+  const Thermo T = th[tk.rep];
+  double ts[2];
+  for (int i = 0; i < 2; ++i) ts[i] = thermo_mode1(Rp, thermo_half(Rp, T.s[i], gs[tk.rep * 4 + 2 * i]), K1[i] / 2);
+  double acc[14] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  if (lane < tk.n) kick_particle<true>(s, Rp, tk.g0 + lane, ts, acc);
+  wave_store_sums(rows2 + (size_t)tid * 16, acc, 14, lane);
+}
+
+// One block (4 waves) per replica.  rows + k*region is row set k (kick1,
+// drift, sc, R, ext, kick2).  Writes the thermostat after modes 0, 1 and 2
+// (k_thermo0 / k_step_scalars), the energies in the layout of RepEnergy,
+// Kin, the pair counts, the observables of print_out and the list decision
+// of k_check_decide; zeroes the cell counters of a replica whose list is
+// rebuilt next.
+__global__ void __launch_bounds__(256) k_end(State s, Thermo *__restrict__ th, const double *__restrict__ gs,
+                                             const double *__restrict__ rows, long region, int has_field,
+                                             const double *__restrict__ kblk, int kblk_stride,
+                                             RepEnergy *__restrict__ E, double *__restrict__ Kin,
+                                             double *__restrict__ counts, double *__restrict__ obs,
+                                             int *__restrict__ rebuild, int *__restrict__ srebuild) {
+  __shared__ double v1[2], v2[14], vsc[12], vr[11], vext[1], vchk[2], vk[10];
+  __shared__ int rb_sh;
+  const int rp = s.rep_begin + blockIdx.x;
+  const RepDev &R = s.reps[rp];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (wv == 0) {
+    double a[2], b[14];
+    wave_rows_sum<2>(rows, R.task0, R.ntask, lane, a);
+    wave_rows_sum<14>(rows + 5 * region, R.task0, R.ntask, lane, b);
+    if (lane < 2) v1[lane] = lane == 0 ? a[0] : a[1];
+    if (lane == 0)
+      for (int k = 0; k < 14; ++k) v2[k] = b[k];
+  } else if (wv == 1) {
+    double a[12];
+    wave_rows_sum<12>(rows + 2 * region, R.task0, R.ntask, lane, a);
+    if (lane == 0)
+      for (int k = 0; k < 12; ++k) vsc[k] = a[k];
+  } else if (wv == 2) {
+    double a[11], e[1] = {0.0};
+    wave_rows_sum<11>(rows + 3 * region, R.task0, R.ntask, lane, a);
+    if (has_field) wave_rows_sum<1>(rows + 4 * region, R.task0, R.ntask, lane, e);
+    double m = 0.0, me = 0.0;
+    const double *dr = rows + region;
+    for (int t = lane; t < R.ntask; t += 64) {
+      m = fmax(m, dr[(size_t)(R.task0 + t) * 16]);
+      me = fmax(me, dr[(size_t)(R.task0 + t) * 16 + 1]);
+    }
+    m = wave_max(m);
+    me = wave_max(me);
+    if (lane == 0) {
+      for (int k = 0; k < 11; ++k) vr[k] = a[k];
+      vext[0] = e[0];
+      vchk[0] = m;
+      vchk[1] = me;
+    }
+  } else {
+    const int nb = (R.ncol * R.KZ + 63) / 64;
+    double k[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int b = lane; b < nb; b += 64) {
+      const double *r = kblk + ((size_t)rp * kblk_stride + b) * 10;
+      for (int q = 0; q < 10; ++q) k[q] += r[q];
+    }
+    for (int q = 0; q < 10; ++q) {
+      const double t = wave_sum(k[q]);
+      if (lane == 0) vk[q] = t;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    Thermo T = th[rp];
+    for (int i = 0; i < 2; ++i) {
+      const double s1 = thermo_mode1(R, thermo_half(R, T.s[i], gs[rp * 4 + 2 * i]), v1[i] / 2);
+      T.strn[i] = gs[rp * 4 + 2 * i + 1];
+      T.s[i] = thermo_half(R, s1, T.strn[i]);
+    }
+    th[rp] = T;
+    const double K2[2] = {v2[0] / 2, v2[1] / 2};
+    Kin[rp * 2 + 0] = K2[0];
+    Kin[rp * 2 + 1] = K2[1];
+    double *e = reinterpret_cast<double *>(E + rp);
+    for (int k = 0; k < 11; ++k) e[k] = vsc[k];
+    for (int k = 0; k < 10; ++k) e[11 + k] = vr[k];
+    for (int k = 0; k < 10; ++k) e[21 + k] = vk[k];
+    e[31] = vext[0];
+    counts[rp * 4 + 0] = vsc[11];
+    counts[rp * 4 + 1] = vr[10];
+    // observables of print_out (as k_step_scalars<2>)
+    const double Np = (double)R.N;
+    const double UR = vr[0], UK = vk[0];
+    const double WdipR = 3 * UR, WdipK = 3 * UK;
+    const double K = (K2[0] + K2[1]) / Np;
+    const double Uss = vsc[0] / Np, UdipR = UR / Np, UdipK = UK / Np, Uext = vext[0] / Np;
+    const double U = R.Uself + Uss + UdipR + UdipK + Uext;
+    const double Pss = vsc[1] / (3 * R.V), PdipR = WdipR / (3 * R.V), PdipK = WdipK / (3 * R.V);
+    const double P = K / 3 * R.dens + Pss + PdipR + PdipK;
+    double *o = obs + (size_t)rp * 12;
+    o[0] = K; o[1] = U; o[2] = P; o[3] = Uss; o[4] = Pss; o[5] = UdipR; o[6] = PdipR;
+    o[7] = UdipK; o[8] = PdipK;
+    const double *mo = v2 + 8;
+    o[9] = mo[0] * mo[0] + mo[1] * mo[1] + mo[2] * mo[2];
+    o[10] = mo[3] * mo[3] + mo[4] * mo[4] + mo[5] * mo[5];
+    // list decision (as k_check_decide in the step loop)
+    double dm = vchk[0];
+    dm = 2. * sqrt(3. * dm * dm);
+    const int rb = dm > R.r_list_max - R.r_list_min;
+    rebuild[rp] = rb;
+    srebuild[rp] = !rb && 2. * vchk[1] > R.skin_short;
+    o[11] = rb;
+    rb_sh = rb;
+  }
+  __syncthreads();
+  if (rb_sh)
+    for (int c = threadIdx.x; c < R.ncell; c += 256) {
+      s.cell_cnt[R.cell0 + c] = 0;
+      s.cell_fill[R.cell0 + c] = 0;
+    }
+}
+
 // fill kernel (graph-safe replacement for memset nodes)
 __global__ void k_fill_int(int *__restrict__ p, int n, int v) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1570,18 +1877,18 @@ static inline int nblk(int n, int b) { return (n + b - 1) / b; }
 void launch_dipoles(const State &s, hipStream_t st) {
   MSC_LAUNCH(k_dipoles, dim3(nblk(s.g_end - s.g_begin, 256)), dim3(256), 0, st, s);
 }
-void launch_cells(const State &s, const int *rebuild, hipStream_t st) {
-  MSC_LAUNCH(k_fill_int, dim3(nblk(s.ncells, 256)), dim3(256), 0, st, s.cell_cnt, s.ncells, 0);
-  MSC_LAUNCH(k_fill_int, dim3(nblk(s.ncells, 256)), dim3(256), 0, st, s.cell_fill, s.ncells, 0);
+void launch_cells(const State &s, const int *rebuild, bool zero, hipStream_t st) {
+  if (zero) {
+    MSC_LAUNCH(k_fill_int, dim3(nblk(s.ncells, 256)), dim3(256), 0, st, s.cell_cnt, s.ncells, 0);
+    MSC_LAUNCH(k_fill_int, dim3(nblk(s.ncells, 256)), dim3(256), 0, st, s.cell_fill, s.ncells, 0);
+  }
   MSC_LAUNCH(k_cells_assign, dim3(nblk(s.g_end - s.g_begin, 256)), dim3(256), 0, st, s, rebuild);
   MSC_LAUNCH(k_scan_cells, dim3(s.nrep), dim3(1024), 0, st, s.cell_cnt, s.cell_start, s.reps, rebuild);
   MSC_LAUNCH(k_cells_fill, dim3(nblk(s.g_end - s.g_begin, 256)), dim3(256), 0, st, s, rebuild);
   MSC_LAUNCH(k_cells_rank, dim3(nblk(s.g_end - s.g_begin, 256)), dim3(256), 0, st, s, rebuild);
-  MSC_LAUNCH(k_cells_pack, dim3(nblk(s.g_end - s.g_begin, 256)), dim3(256), 0, st, s, rebuild);
 }
 void launch_lists(const State &s, const int *rebuild, int *flags, hipStream_t st) {
   MSC_LAUNCH(k_build_lists, dim3(nblk(s.g_end - s.g_begin, 4)), dim3(256), 0, st, s, rebuild, flags);
-  MSC_LAUNCH(k_savec, dim3(nblk(s.g_end - s.g_begin, 256)), dim3(256), 0, st, s, rebuild);
 }
 void launch_short(const State &s, const int *srebuild, int *flags, hipStream_t st) {
   MSC_LAUNCH(k_short_build, dim3(nblk(s.g_end - s.g_begin, 4)), dim3(256), 0, st, s, srebuild, flags);
@@ -1680,14 +1987,15 @@ int launch_recip_force(const State &s, bool mixed, int KZ, const void *ph, const
       return -1;
   }
 }
-void launch_recip_reduce(const State &s, bool mixed, const void *part, const double *M2, void *A, double *slot_out,
-                         int maxslot, hipStream_t st) {
+void launch_recip_reduce(const State &s, bool mixed, const void *part, const double *M2, const double *m2rows,
+                         void *A, double *slot_out, double *kblk, int maxslot, hipStream_t st) {
+  const dim3 grid(nblk(maxslot, 64), s.rep_end - s.rep_begin);
   if (mixed)
-    MSC_LAUNCH(k_recip_reduce<float>, dim3(nblk(maxslot, 64), s.rep_end - s.rep_begin), dim3(1024), 0, st, s,
-               (const float *)part, M2, (Cpx<float> *)A, slot_out);
+    MSC_LAUNCH(k_recip_reduce<float>, grid, dim3(512), 0, st, s, (const float *)part, M2, m2rows,
+               (Cpx<float> *)A, slot_out, kblk, nblk(maxslot, 64));
   else
-    MSC_LAUNCH(k_recip_reduce<double>, dim3(nblk(maxslot, 64), s.rep_end - s.rep_begin), dim3(1024), 0, st, s,
-               (const double *)part, M2, (Cpx<double> *)A, slot_out);
+    MSC_LAUNCH(k_recip_reduce<double>, grid, dim3(512), 0, st, s, (const double *)part, M2, m2rows,
+               (Cpx<double> *)A, slot_out, kblk, nblk(maxslot, 64));
 }
 
 void launch_moveP(const State &s, bool last, const Thermo *th, double *task_out, hipStream_t st) {
@@ -1719,6 +2027,25 @@ void launch_clusters(const State &s, double min_s, double reach_max, int *par, i
   MSC_LAUNCH(k_clust_init, grid, dim3(256), 0, st, s, par);
   MSC_LAUNCH(k_clust_edges, grid, dim3(256), 0, st, s, min_s, reach_max, par, flags);
   MSC_LAUNCH(k_clust_final, grid, dim3(256), 0, st, s, par);
+}
+void launch_kick1(const State &s, const Thermo *th, const double *gs, double *rows, hipStream_t st) {
+  MSC_LAUNCH(k_kick1, dim3(nblk(s.ntask, 4)), dim3(256), 0, st, s, th, gs, rows);
+}
+void launch_drift(const State &s, bool mixed, void *ph, void *amp, double *rows, hipStream_t st) {
+  if (mixed)
+    MSC_LAUNCH(k_drift<float>, dim3(s.ntask), dim3(256), 0, st, s, (Cpx<float> *)ph, (float *)amp, rows);
+  else
+    MSC_LAUNCH(k_drift<double>, dim3(s.ntask), dim3(256), 0, st, s, (Cpx<double> *)ph, (double *)amp, rows);
+}
+void launch_kick2(const State &s, const Thermo *th, const double *gs, const double *rows1, double *rows2,
+                  hipStream_t st) {
+  MSC_LAUNCH(k_kick2, dim3(nblk(s.ntask, 4)), dim3(256), 0, st, s, th, gs, rows1, rows2);
+}
+void launch_end(const State &s, Thermo *th, const double *gs, const double *rows, long region, bool has_field,
+                const double *kblk, int maxslot, RepEnergy *E, double *Kin, double *counts, double *obs,
+                int *rebuild, int *srebuild, hipStream_t st) {
+  MSC_LAUNCH(k_end, dim3(s.rep_end - s.rep_begin), dim3(256), 0, st, s, th, gs, rows, region, has_field ? 1 : 0,
+             kblk, nblk(maxslot, 64), E, Kin, counts, obs, rebuild, srebuild);
 }
 void launch_fill_int(int *p, int n, int v, hipStream_t st) {
   MSC_LAUNCH(k_fill_int, dim3(nblk(n, 256)), dim3(256), 0, st, p, n, v);

@@ -40,7 +40,9 @@ struct State {
 };
 
 void launch_dipoles(const State &s, hipStream_t st);
-void launch_cells(const State &s, const int *rebuild, hipStream_t st);
+// zero: clear every cell counter first (the step loop's k_end clears those
+// of the replicas it rebuilds)
+void launch_cells(const State &s, const int *rebuild, bool zero, hipStream_t st);
 void launch_lists(const State &s, const int *rebuild, int *flags, hipStream_t st);
 void launch_short(const State &s, const int *srebuild, int *flags, hipStream_t st);
 void launch_check(const State &s, double *task_out, int *rebuild, int *srebuild, double *obs_rebuild,
@@ -56,8 +58,11 @@ void launch_reduce(const State &s, const double *in, int istride, int nv, int wh
 void launch_recip_tables(const State &s, bool mixed, void *ph, void *amp, double *task_out, hipStream_t st);
 int launch_recip_struct(const State &s, bool mixed, int KZ, int rt0, int nrt, int maxcol, const void *ph,
                         const void *amp, void *part, hipStream_t st);
-void launch_recip_reduce(const State &s, bool mixed, const void *part, const double *M2, void *A, double *slot_out,
-                         int maxslot, hipStream_t st);
+// M2: per-replica sums of c c^T, or null to sum them from the per-task rows
+// m2rows; kblk (may be null): per-block sums of the 10 slot values,
+// nblk(maxslot, 64) rows per replica
+void launch_recip_reduce(const State &s, bool mixed, const void *part, const double *M2, const double *m2rows,
+                         void *A, double *slot_out, double *kblk, int maxslot, hipStream_t st);
 int launch_recip_force(const State &s, bool mixed, int KZ, const void *ph, const void *amp, const void *A,
                        hipStream_t st);
 void launch_moveP(const State &s, bool last, const Thermo *th, double *task_out, hipStream_t st);
@@ -68,6 +73,16 @@ void launch_step_scalars(const State &s, int mode, const double *red, double *Ki
                          double *obs, hipStream_t st);
 void launch_zero_FT(const State &s, hipStream_t st);
 void launch_fill_int(int *p, int n, int v, hipStream_t st);
+// the step loop's fused kernels; rows + k*region is per-task row set k:
+// 0 kick1, 1 drift, 2 sc, 3 R, 4 ext, 5 kick2 (16 doubles per task)
+constexpr int kRowKick1 = 0, kRowDrift = 1, kRowSC = 2, kRowR = 3, kRowExt = 4, kRowKick2 = 5, kRowSets = 6;
+void launch_kick1(const State &s, const Thermo *th, const double *gs, double *rows, hipStream_t st);
+void launch_drift(const State &s, bool mixed, void *ph, void *amp, double *rows, hipStream_t st);
+void launch_kick2(const State &s, const Thermo *th, const double *gs, const double *rows1, double *rows2,
+                  hipStream_t st);
+void launch_end(const State &s, Thermo *th, const double *gs, const double *rows, long region, bool has_field,
+                const double *kblk, int maxslot, RepEnergy *E, double *Kin, double *counts, double *obs,
+                int *rebuild, int *srebuild, hipStream_t st);
 // contact-graph components of the selected replica: par[g] = smallest
 // particle of g's component (global index); flags[0] counts particles that
 // moved too far for the reference rows to hold every contact, flags[1] pairs
