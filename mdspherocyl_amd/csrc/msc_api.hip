@@ -160,16 +160,24 @@ static hipError_t device_stream(int device, hipStream_t *out) {
     }                                                                                \
   } while (0)
 
-// launch failures are reported by the next runtime query; every entry point
-// that launches kernels checks before it returns
-#define LAUNCHED(what)                                                               \
-  do {                                                                               \
-    hipError_t e_ = hipGetLastError();                                               \
-    if (e_ != hipSuccess) {                                                          \
-      ctx->err = std::string("kernel launch in ") + (what) + ": " + hipGetErrorString(e_); \
-      g_err = ctx->err;                                                              \
-      return MSC_E_HIP;                                                              \
-    }                                                                                \
+// A failed launch is recorded by the launch wrapper (take_launch_error);
+// every result read back from the device (d2h) and the entry points below
+// check that record first.
+static int launch_status(msc_ctx *ctx, const char *what) {
+  const char *kernel = "";
+  hipError_t e = take_launch_error(&kernel);
+  if (e == hipSuccess) {
+    e = hipGetLastError();
+    kernel = "(runtime)";
+  }
+  if (e == hipSuccess) return MSC_OK;
+  ctx->err = std::string("kernel launch ") + kernel + " in " + what + ": " + hipGetErrorString(e);
+  g_err = ctx->err;
+  return MSC_E_HIP;
+}
+#define LAUNCHED(what)                                              \
+  do {                                                              \
+    if (int rc_l_ = launch_status(ctx, (what))) return rc_l_;       \
   } while (0)
 
 // Host <-> device transfers.  The runtime is never handed pageable host
@@ -179,6 +187,7 @@ static hipError_t device_stream(int device, hipStream_t *out) {
 // megabyte or more would otherwise make the runtime lock the caller's pages
 // in place for the DMA engine.
 static int d2h(msc_ctx *ctx, void *dst, const void *src, size_t bytes) {
+  LAUNCHED("a device read-back");
   char *d = (char *)dst;
   const char *s_ = (const char *)src;
   for (size_t off = 0; off < bytes; off += kPinBytes) {
@@ -322,11 +331,22 @@ void grid(msc_ctx *ctx, int rp) {
   double need = R.r_list_max + R.lmax;
   if (R.rc > 0) need = std::max(need, R.rl_dip + 2 * ctx->S.lever_max);
   need *= 1.0 + 1e-9;
+  // cells of half the search radius with a 5-cell stencil where 5 fit (a
+  // search volume (5/2)^3 need^3 instead of 3^3 need^3), else cells of the
+  // whole radius with a 3-cell stencil, else the whole box
   for (int a = 0; a < 3; ++a) {
-    int n = (int)std::floor(R.L[a] / need);
-    if (n < 3) n = 1;
+    int n = (int)std::floor(R.L[a] / (0.5 * need)), reach = 2;
+    if (n < 5) {
+      n = (int)std::floor(R.L[a] / need);
+      reach = 1;
+    }
+    if (n < 3) {
+      n = 1;
+      reach = 0;
+    }
     n = std::min(n, 256);
     R.nc[a] = n;
+    R.reach[a] = reach;
     R.cw[a] = R.L[a] / n;
   }
   R.ncell = R.nc[0] * R.nc[1] * R.nc[2];

@@ -26,6 +26,7 @@ struct RepDev {
   int tab0;             // first row of this replica's real-space table
   int cell0, ncell;     // first cell and cell count of this replica's grid
   int nc[3];            // cells per dimension (1 if the box is too small)
+  int reach[3];         // stencil half-width per dimension: cells within +-reach are searched
   // Neighbour rows: entry q of particle g sits at idx[base + q*N + (g - n0)]
   // (column-major, so the lanes of a wave read consecutive words); w is the
   // row capacity.  nl: reference list (segment distance < r_list_max); sl:

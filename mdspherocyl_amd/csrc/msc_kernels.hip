@@ -7,6 +7,7 @@
 
 #include <cstdio>
 #include <cstdlib>
+#include <mutex>
 
 #include "msc_geom.hpp"
 
@@ -34,10 +35,34 @@ static void sync_announce(const char *kernel) {
   std::fprintf(stderr, "msc: launch %s\n", kernel);
   std::fflush(stderr);
 }
+// The runtime's last-error slot is overwritten by every later call, so each
+// launch's own status is kept here: the first failure since the last
+// take_launch_error(), with the kernel's name.
+static std::mutex g_launch_mu;
+static hipError_t g_launch_err = hipSuccess;
+static const char *g_launch_kernel = "";
+static void note_launch(const char *kernel) {
+  const hipError_t e = hipGetLastError();
+  if (e == hipSuccess) return;
+  std::lock_guard<std::mutex> lk(g_launch_mu);
+  if (g_launch_err == hipSuccess) {
+    g_launch_err = e;
+    g_launch_kernel = kernel;
+  }
+}
+hipError_t take_launch_error(const char **kernel) {
+  std::lock_guard<std::mutex> lk(g_launch_mu);
+  const hipError_t e = g_launch_err;
+  *kernel = g_launch_kernel;
+  g_launch_err = hipSuccess;
+  g_launch_kernel = "";
+  return e;
+}
 #define MSC_LAUNCH(K, ...)          \
   do {                              \
     sync_announce(#K);              \
     hipLaunchKernelGGL(K, __VA_ARGS__); \
+    note_launch(#K);                \
     sync_check(#K);                 \
   } while (0)
 
@@ -313,20 +338,30 @@ __global__ void __launch_bounds__(256) k_build_lists(State s, const int *__restr
     if (ins && pos < R.sl_w) s.sl_idx[row_at(R.sl_base, R.N, R.n0, pos, g)] = j;
     ns += __popcll(m);
   };
-  int lo[3], hi[3];
-  for (int a = 0; a < 3; ++a) {
-    if (R.nc[a] >= 3) { lo[a] = -1; hi[a] = 1; }
-    else { lo[a] = 0; hi[a] = 0; }
+  // The stencil row (dy, dz) covers cells cx - reach .. cx + reach, which
+  // are consecutive in cell order except where the row wraps: at most two
+  // runs of consecutive cells, whose particles are consecutive records.
+  const int rx = R.reach[0], ry = R.reach[1], rz = R.reach[2], ncx = R.nc[0];
+  int runs[2][2], nrun;
+  {
+    const int x0 = ci[0] - rx, x1 = ci[0] + rx;
+    if (x0 < 0) {
+      runs[0][0] = x0 + ncx; runs[0][1] = ncx - 1; runs[1][0] = 0; runs[1][1] = x1; nrun = 2;
+    } else if (x1 >= ncx) {
+      runs[0][0] = x0; runs[0][1] = ncx - 1; runs[1][0] = 0; runs[1][1] = x1 - ncx; nrun = 2;
+    } else {
+      runs[0][0] = x0; runs[0][1] = x1; runs[1][0] = runs[1][1] = 0; nrun = 1;
+    }
   }
-  for (int dz = lo[2]; dz <= hi[2]; ++dz)
-    for (int dy = lo[1]; dy <= hi[1]; ++dy)
-      for (int dx = lo[0]; dx <= hi[0]; ++dx) {
-        int cx = ci[0] + dx, cy = ci[1] + dy, cz = ci[2] + dz;
-        cx = cx < 0 ? cx + R.nc[0] : (cx >= R.nc[0] ? cx - R.nc[0] : cx);
+  for (int dz = -rz; dz <= rz; ++dz)
+    for (int dy = -ry; dy <= ry; ++dy)
+      for (int ru = 0; ru < nrun; ++ru) {
+        int cy = ci[1] + dy, cz = ci[2] + dz;
         cy = cy < 0 ? cy + R.nc[1] : (cy >= R.nc[1] ? cy - R.nc[1] : cy);
         cz = cz < 0 ? cz + R.nc[2] : (cz >= R.nc[2] ? cz - R.nc[2] : cz);
-        const int c = R.cell0 + (cz * R.nc[1] + cy) * R.nc[0] + cx;
-        const int b = s.cell_start[c], e = b + s.cell_cnt[c];
+        const int crow = R.cell0 + (cz * R.nc[1] + cy) * ncx;
+        const int cl = crow + runs[ru][0], ch = crow + runs[ru][1];
+        const int b = s.cell_start[cl], e = s.cell_start[ch] + s.cell_cnt[ch];
         for (int q0 = b; q0 < e; q0 += 64) {
           const int q = q0 + lane;
           bool act = q < e;

@@ -39,6 +39,10 @@ struct State {
   const double *kval;
 };
 
+// first kernel launch that failed since the last call (hipSuccess if none),
+// with the kernel's name; clears the record
+hipError_t take_launch_error(const char **kernel);
+
 void launch_dipoles(const State &s, hipStream_t st);
 // zero: clear every cell counter first (the step loop's k_end clears those
 // of the replicas it rebuilds)
