@@ -332,6 +332,7 @@ void grid(msc_ctx *ctx, int rp) {
   if (R.rc > 0) need = std::max(need, R.rl_dip + 2 * ctx->S.lever_max);
   need *= 1.0 + 1e-9;
   // cells of half the search radius with a 5-cell stencil where 5 fit (a
+  // reach of at most 2 cells, which the list kernel relies on; a
   // search volume (5/2)^3 need^3 instead of 3^3 need^3), else cells of the
   // whole radius with a 3-cell stencil, else the whole box
   for (int a = 0; a < 3; ++a) {
@@ -1321,10 +1322,12 @@ static void enqueue_step(msc_ctx *ctx, int step, const double *d_gs_step, double
     launch_end(s, ctx->d_th, d_gs_step, rows, region, field, ctx->d_kblk, ctx->maxslot, ctx->d_E, ctx->d_Kin,
                ctx->d_counts, d_obs_step, ctx->d_rebuild, ctx->d_srebuild, st);
   }
-  // list maintenance (v25/main.cpp:679-701): decided in k_end; the rebuild
-  // kernels return at once for replicas that keep their list
-  lists_build(ctx, ctx->d_rebuild, false);
-  if (ctx->mixed) launch_short(s, ctx->d_srebuild, ctx->d_flags, st);
+  // list maintenance (v25/main.cpp:679-701): decided in k_end; the launches
+  // return at once for replicas that keep their rows
+  {
+    Bracket b(ctx, "nlist");
+    launch_rebuild(s, ctx->d_rebuild, ctx->mixed ? ctx->d_srebuild : nullptr, ctx->d_flags, st);
+  }
 }
 
 // Device copy of everything a step carries into the next one (positions,

@@ -128,9 +128,7 @@ __global__ void k_dipoles(State s) {
 // --------------------------------------------------------------------------
 // cell grid (integer binning of wrapped centres) and neighbour lists
 // --------------------------------------------------------------------------
-__global__ void k_cells_assign(State s, const int *__restrict__ rebuild) {
-  int g = s.g_begin + blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= s.g_end) return;
+__device__ __forceinline__ void cells_assign_one(const State &s, int g, const int *__restrict__ rebuild) {
   int rp = s.rep_of[g];
   if (!rebuild[rp]) return;
   const RepDev &R = s.reps[rp];
@@ -146,25 +144,26 @@ __global__ void k_cells_assign(State s, const int *__restrict__ rebuild) {
   s.cell_of[g] = cell;
   atomicAdd(&s.cell_cnt[cell], 1);
 }
+__global__ void k_cells_assign(State s, const int *__restrict__ rebuild) {
+  int g = s.g_begin + blockIdx.x * blockDim.x + threadIdx.x;
+  if (g < s.g_end) cells_assign_one(s, g, rebuild);
+}
 
-// Exclusive scan of the cell counts of one replica into cell starts (one
-// block of 1024 threads per replica, each thread owning a contiguous run of
-// cells); the replica's particles occupy [n0, n0 + N) of the cell order.
-__global__ void __launch_bounds__(1024) k_scan_cells(int *__restrict__ cnt, int *__restrict__ out,
-                                                     const RepDev *__restrict__ reps,
-                                                     const int *__restrict__ rebuild) {
-  __shared__ int sh[1024];
-  const int rp = blockIdx.x;
-  if (!rebuild[rp]) return;
-  const int b0 = reps[rp].cell0, n = reps[rp].ncell, base = reps[rp].n0;
-  int t = threadIdx.x;
-  int seg = (n + 1023) / 1024;
-  int b = t * seg, e = min(n, b + seg);
+// Exclusive scan of the cell counts of replica rp into cell starts by one
+// block (each of its nt threads owns a contiguous run of cells; sh holds nt
+// ints); the replica's particles occupy [n0, n0 + N) of the cell order.
+// Block-uniform: every thread of the block calls it with the same rp.
+__device__ void scan_cells_block(const int *__restrict__ cnt, int *__restrict__ out, const RepDev &R, int *sh,
+                                 int nt) {
+  const int b0 = R.cell0, n = R.ncell, base = R.n0;
+  const int t = threadIdx.x;
+  const int seg = (n + nt - 1) / nt;
+  const int b = t * seg, e = min(n, b + seg);
   int sum = 0;
   for (int i = b; i < e; ++i) sum += cnt[b0 + i];
   sh[t] = sum;
   __syncthreads();
-  for (int o = 1; o < 1024; o <<= 1) {
+  for (int o = 1; o < nt; o <<= 1) {
     int v = (t >= o) ? sh[t - o] : 0;
     __syncthreads();
     sh[t] += v;
@@ -176,16 +175,27 @@ __global__ void __launch_bounds__(1024) k_scan_cells(int *__restrict__ cnt, int 
     out[b0 + i] = run;
     run += v;
   }
+  __syncthreads();
+}
+__global__ void __launch_bounds__(1024) k_scan_cells(int *__restrict__ cnt, int *__restrict__ out,
+                                                     const RepDev *__restrict__ reps,
+                                                     const int *__restrict__ rebuild) {
+  __shared__ int sh[1024];
+  const int rp = blockIdx.x;
+  if (!rebuild[rp]) return;
+  scan_cells_block(cnt, out, reps[rp], sh, 1024);
 }
 
 // particles into their cells in arrival order ...
-__global__ void k_cells_fill(State s, const int *__restrict__ rebuild) {
-  int g = s.g_begin + blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= s.g_end) return;
+__device__ __forceinline__ void cells_fill_one(const State &s, int g, const int *__restrict__ rebuild) {
   if (!rebuild[s.rep_of[g]]) return;
   int c = s.cell_of[g];
   int pos = s.cell_start[c] + atomicAdd(&s.cell_fill[c], 1);
   s.cell_tmp[pos] = g;
+}
+__global__ void k_cells_fill(State s, const int *__restrict__ rebuild) {
+  int g = s.g_begin + blockIdx.x * blockDim.x + threadIdx.x;
+  if (g < s.g_end) cells_fill_one(s, g, rebuild);
 }
 
 // ... then into ascending index order: each particle's rank among the
@@ -193,9 +203,7 @@ __global__ void k_cells_fill(State s, const int *__restrict__ rebuild) {
 // later traversal of a cell is in a fixed order.  The particle also writes
 // its centre, axis, length and index at its cell-order position t, packed
 // into one 64-byte record so that a candidate costs one cache line.
-__global__ void k_cells_rank(State s, const int *__restrict__ rebuild) {
-  int g = s.g_begin + blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= s.g_end) return;
+__device__ __forceinline__ void cells_rank_one(const State &s, int g, const int *__restrict__ rebuild) {
   if (!rebuild[s.rep_of[g]]) return;
   int c = s.cell_of[g];
   int b = s.cell_start[c], n = s.cell_cnt[c];
@@ -212,6 +220,10 @@ __global__ void k_cells_rank(State s, const int *__restrict__ rebuild) {
   o[5] = s.ori[8 * N + g];
   o[6] = s.len[g];
   o[7] = (double)g;
+}
+__global__ void k_cells_rank(State s, const int *__restrict__ rebuild) {
+  int g = s.g_begin + blockIdx.x * blockDim.x + threadIdx.x;
+  if (g < s.g_end) cells_rank_one(s, g, rebuild);
 }
 
 // nearest double to x rounded to 12 decimals (text round trip of savec_list)
@@ -270,11 +282,11 @@ __device__ __forceinline__ int lane_prefix(unsigned long long m, int lane) {
   return __popcll(m & ((1ull << lane) - 1ull));
 }
 
-__global__ void __launch_bounds__(256) k_build_lists(State s, const int *__restrict__ rebuild, int *__restrict__ flags) {
-  __shared__ int queue[4][128];
-  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int t = s.g_begin + __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + wv);
-  if (t >= s.g_end) return;
+// The rows of the particle at cell-order position t, by one wave.  Q (128
+// ints) and RB, RE (64 ints each) are the wave's own LDS.
+__device__ void build_lists_wave(const State &s, int t, const int *__restrict__ rebuild, int *__restrict__ flags,
+                                 int *Q, int *RB, int *RE) {
+  const int lane = threadIdx.x & 63;
   const int rp = __builtin_amdgcn_readfirstlane(s.rep_of[t]);
   if (!rebuild[rp]) return;
   const RepDev &R = s.reps[rp];
@@ -303,7 +315,6 @@ __global__ void __launch_bounds__(256) k_build_lists(State s, const int *__restr
   const double rs2 = R.r_short * R.r_short;
   const bool dip = R.rc > 0;
   const double reachd = R.rl_dip + 2.0 * s.lever_max;
-  int *Q = queue[wv];
   int nw = 0, ns = 0, ndl = 0, qn = 0;
   // segment distances of the first `count` queued candidates
   auto flush = [&](int count) {
@@ -341,6 +352,8 @@ __global__ void __launch_bounds__(256) k_build_lists(State s, const int *__restr
   // The stencil row (dy, dz) covers cells cx - reach .. cx + reach, which
   // are consecutive in cell order except where the row wraps: at most two
   // runs of consecutive cells, whose particles are consecutive records.
+  // Lane l finds the record range of run l (dz outer, dy, then the runs of
+  // the row; at most 5 x 5 x 2 = 50 runs, reach <= 2) for the whole wave.
   const int rx = R.reach[0], ry = R.reach[1], rz = R.reach[2], ncx = R.nc[0];
   int runs[2][2], nrun;
   {
@@ -353,55 +366,110 @@ __global__ void __launch_bounds__(256) k_build_lists(State s, const int *__restr
       runs[0][0] = x0; runs[0][1] = x1; runs[1][0] = runs[1][1] = 0; nrun = 1;
     }
   }
-  for (int dz = -rz; dz <= rz; ++dz)
-    for (int dy = -ry; dy <= ry; ++dy)
-      for (int ru = 0; ru < nrun; ++ru) {
-        int cy = ci[1] + dy, cz = ci[2] + dz;
-        cy = cy < 0 ? cy + R.nc[1] : (cy >= R.nc[1] ? cy - R.nc[1] : cy);
-        cz = cz < 0 ? cz + R.nc[2] : (cz >= R.nc[2] ? cz - R.nc[2] : cz);
-        const int crow = R.cell0 + (cz * R.nc[1] + cy) * ncx;
-        const int cl = crow + runs[ru][0], ch = crow + runs[ru][1];
-        const int b = s.cell_start[cl], e = s.cell_start[ch] + s.cell_cnt[ch];
-        for (int q0 = b; q0 < e; q0 += 64) {
-          const int q = q0 + lane;
-          bool act = q < e;
-          const double *pj = s.cell_pack + (size_t)(act ? q : t) * 8;
-          const int j = (int)pj[7];
-          act = act && j != g;
-          double rgj[3];
-          for (int a = 0; a < 3; ++a) rgj[a] = wrapd(rg[a] - pj[a], R.L[a]);
-          const double rc2 = dot3(rgj, rgj);
-          const double reach = R.r_list_max + 0.5 * (Lg + pj[6]) + 1e-9;
-          const bool pass = act && rc2 < reach * reach * (1 + 1e-12);
-          // pairs whose dipoles can come within rc before the next rebuild
-          if (dip) {
-            const bool dp = act && rc2 < reachd * reachd;
-            const unsigned long long m = __ballot(dp);
-            const int pos = ndl + lane_prefix(m, lane);
-            if (dp && pos < R.dl_w) s.dl_idx[row_at(R.dl_base, R.N, R.n0, pos, g)] = j;
-            ndl += __popcll(m);
-          }
-          const unsigned long long m = __ballot(pass);
-          if (pass) Q[qn + lane_prefix(m, lane)] = q;
-          qn += __popcll(m);
-          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-          __builtin_amdgcn_wave_barrier();
-          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-          if (qn >= 64) {
-            flush(64);
-            const int rest = qn - 64;
-            const int keep = lane < rest ? Q[64 + lane] : 0;
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            if (lane < rest) Q[lane] = keep;
-            qn = rest;
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-          }
-        }
+  const int nr = (2 * rz + 1) * (2 * ry + 1) * nrun;
+  if (lane < nr) {
+    const int ru = lane % nrun, row = lane / nrun;
+    const int dy = row % (2 * ry + 1) - ry, dz = row / (2 * ry + 1) - rz;
+    int cy = ci[1] + dy, cz = ci[2] + dz;
+    cy = cy < 0 ? cy + R.nc[1] : (cy >= R.nc[1] ? cy - R.nc[1] : cy);
+    cz = cz < 0 ? cz + R.nc[2] : (cz >= R.nc[2] ? cz - R.nc[2] : cz);
+    const int crow = R.cell0 + (cz * R.nc[1] + cy) * ncx;
+    const int cl = crow + (ru ? runs[1][0] : runs[0][0]), ch = crow + (ru ? runs[1][1] : runs[0][1]);
+    RB[lane] = s.cell_start[cl];
+    RE[lane] = s.cell_start[ch] + s.cell_cnt[ch];
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  // Passes of 64 candidates over the runs in order; the records of the next
+  // pass are loaded while the current one is processed.
+  int r = 0, q0 = 0, e = 0;
+  for (; r < nr; ++r) {
+    q0 = __builtin_amdgcn_readfirstlane(RB[r]);
+    e = __builtin_amdgcn_readfirstlane(RE[r]);
+    if (q0 < e) break;
+  }
+  double cur[8];
+  {
+    const double *pj = s.cell_pack + (size_t)(r < nr && q0 + lane < e ? q0 + lane : t) * 8;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) cur[k] = pj[k];
+  }
+  while (r < nr) {
+    int r2 = r, q2 = q0 + 64, e2 = e;
+    if (q2 >= e2)
+      for (++r2; r2 < nr; ++r2) {
+        q2 = __builtin_amdgcn_readfirstlane(RB[r2]);
+        e2 = __builtin_amdgcn_readfirstlane(RE[r2]);
+        if (q2 < e2) break;
       }
+    double nxt[8];
+    {
+      const double *pj = s.cell_pack + (size_t)(r2 < nr && q2 + lane < e2 ? q2 + lane : t) * 8;
+#pragma unroll
+      for (int k = 0; k < 8; ++k) nxt[k] = pj[k];
+    }
+    {
+      const int q = q0 + lane;
+      bool act = q < e;
+      const int j = (int)cur[7];
+      act = act && j != g;
+      double rgj[3];
+      for (int a = 0; a < 3; ++a) rgj[a] = wrapd(rg[a] - cur[a], R.L[a]);
+      const double rc2 = dot3(rgj, rgj);
+      const double reach = R.r_list_max + 0.5 * (Lg + cur[6]) + 1e-9;
+      bool pass = act && rc2 < reach * reach * (1 + 1e-12);
+      // d_seg >= d(centre_j, segment_g) - L_j/2 and >= d(centre_g, segment_j)
+      // - L_g/2 (triangle inequality): a candidate either bound puts beyond
+      // r_list_max (with a margin far above rounding) cannot be a list pair
+      {
+        const double hg = 0.5 * Lg, hj = 0.5 * cur[6];
+        double t1 = -dot3(rgj, ug), t2 = rgj[0] * cur[3] + rgj[1] * cur[4] + rgj[2] * cur[5];
+        t1 = fmin(fmax(t1, -hg), hg);
+        t2 = fmin(fmax(t2, -hj), hj);
+        double d1 = 0.0, d2 = 0.0;
+        for (int a = 0; a < 3; ++a) {
+          const double w1 = -rgj[a] - t1 * ug[a], w2 = rgj[a] - t2 * cur[3 + a];
+          d1 += w1 * w1;
+          d2 += w2 * w2;
+        }
+        const double R1 = R.r_list_max + hj + 1e-9, R2 = R.r_list_max + hg + 1e-9;
+        pass = pass && d1 < R1 * R1 * (1 + 1e-12) && d2 < R2 * R2 * (1 + 1e-12);
+      }
+      // pairs whose dipoles can come within rc before the next rebuild
+      if (dip) {
+        const bool dp = act && rc2 < reachd * reachd;
+        const unsigned long long m = __ballot(dp);
+        const int pos = ndl + lane_prefix(m, lane);
+        if (dp && pos < R.dl_w) s.dl_idx[row_at(R.dl_base, R.N, R.n0, pos, g)] = j;
+        ndl += __popcll(m);
+      }
+      const unsigned long long m = __ballot(pass);
+      if (pass) Q[qn + lane_prefix(m, lane)] = q;
+      qn += __popcll(m);
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      if (qn >= 64) {
+        flush(64);
+        const int rest = qn - 64;
+        const int keep = lane < rest ? Q[64 + lane] : 0;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (lane < rest) Q[lane] = keep;
+        qn = rest;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) cur[k] = nxt[k];
+    r = r2;
+    q0 = q2;
+    e = e2;
+  }
   if (qn > 0) flush(qn);
   if (lane == 0) {
     s.nl_cnt[g] = min(nw, R.nl_w);
@@ -414,15 +482,20 @@ __global__ void __launch_bounds__(256) k_build_lists(State s, const int *__restr
     }
   }
 }
+__global__ void __launch_bounds__(256) k_build_lists(State s, const int *__restrict__ rebuild, int *__restrict__ flags) {
+  __shared__ int queue[4][128];
+  __shared__ int run_b[4][64], run_e[4][64];
+  const int wv = threadIdx.x >> 6;
+  const int t = s.g_begin + __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + wv);
+  if (t < s.g_end) build_lists_wave(s, t, rebuild, flags, queue[wv], run_b[wv], run_e[wv]);
+}
 
 // Contact rows refreshed from the reference rows (same pair order), for
 // replicas whose contact skin is used up while the reference list is still
 // valid.
-__global__ void __launch_bounds__(256) k_short_build(State s, const int *__restrict__ srebuild, int *__restrict__ flags) {
-  // one wave per particle, the lanes over its reference row (kept in order)
+// one wave per particle, the lanes over its reference row (kept in order)
+__device__ void short_build_wave(const State &s, int g, const int *__restrict__ srebuild, int *__restrict__ flags) {
   const int lane = threadIdx.x & 63;
-  const int g = s.g_begin + __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
-  if (g >= s.g_end) return;
   const int rp = __builtin_amdgcn_readfirstlane(s.rep_of[g]);
   if (!srebuild[rp]) return;
   const RepDev &R = s.reps[rp];
@@ -449,6 +522,11 @@ __global__ void __launch_bounds__(256) k_short_build(State s, const int *__restr
     s.snap_e[(3 + lane) * N + g] = s.ori[(6 + lane) * N + g];
   }
 }
+__global__ void __launch_bounds__(256) k_short_build(State s, const int *__restrict__ srebuild, int *__restrict__ flags) {
+  const int g = s.g_begin + __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
+  if (g < s.g_end) short_build_wave(s, g, srebuild, flags);
+}
+
 
 // --------------------------------------------------------------------------
 // savec_list / check_list (v25/functions.cpp:1336-1394)
@@ -1924,6 +2002,11 @@ void launch_cells(const State &s, const int *rebuild, bool zero, hipStream_t st)
 }
 void launch_lists(const State &s, const int *rebuild, int *flags, hipStream_t st) {
   MSC_LAUNCH(k_build_lists, dim3(nblk(s.g_end - s.g_begin, 4)), dim3(256), 0, st, s, rebuild, flags);
+}
+void launch_rebuild(const State &s, const int *rebuild, const int *srebuild, int *flags, hipStream_t st) {
+  launch_cells(s, rebuild, false, st);
+  launch_lists(s, rebuild, flags, st);
+  if (srebuild) launch_short(s, srebuild, flags, st);
 }
 void launch_short(const State &s, const int *srebuild, int *flags, hipStream_t st) {
   MSC_LAUNCH(k_short_build, dim3(nblk(s.g_end - s.g_begin, 4)), dim3(256), 0, st, s, srebuild, flags);

@@ -49,6 +49,10 @@ void launch_dipoles(const State &s, hipStream_t st);
 void launch_cells(const State &s, const int *rebuild, bool zero, hipStream_t st);
 void launch_lists(const State &s, const int *rebuild, int *flags, hipStream_t st);
 void launch_short(const State &s, const int *srebuild, int *flags, hipStream_t st);
+// the step's list maintenance: cell grid and rows of the replicas with
+// rebuild set (cell counters already zero), contact rows of those with
+// srebuild set (srebuild may be null)
+void launch_rebuild(const State &s, const int *rebuild, const int *srebuild, int *flags, hipStream_t st);
 void launch_check(const State &s, double *task_out, int *rebuild, int *srebuild, double *obs_rebuild,
                   int obs_stride, bool standalone, hipStream_t st);
 void launch_force_sc(const State &s, bool strict, double *task_out, int *flags, hipStream_t st);
