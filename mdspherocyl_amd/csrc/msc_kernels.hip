@@ -1154,6 +1154,177 @@ __device__ __forceinline__ void wave_rows_sum(const double *__restrict__ rows, i
   for (int k = 0; k < NV; ++k) out[k] = wave_sum(v[k]);
 }
 
+// Structure factors on the matrix cores (mixed precision).  For a chunk of
+// particles, S_a(nx, ny, nz) = sum_p B_p(a, nx) A_p(ny, nz) with
+// A = e_y(ny) e_z(nz) and B = c_a e_x(nx): a complex matrix product over the
+// particles, rows m = (ny, nz), columns n = (a, nx).  Each FP32 operand x is
+// split as x = hi + lo with hi = fp16(x), lo = fp16(x - hi), and a real
+// product XY is taken as Xh Yh + Xh Yl + Xl Yh (the dropped Xl Yl and the
+// split itself are below 2^-21 |X||Y|) with FP32 accumulation in
+// v_mfma_f32_16x16x16f16; the complex product is four real products, so a
+// 16 x 16 output tile of (Re S, Im S) takes 12 instructions per 16 particles.
+// Operand layout of the instruction (checked on gfx950): A[i][k] in lane
+// i + 16 (k/4), element k % 4; B[k][n] in lane n + 16 (k/4), element k % 4;
+// D[i][n] in lane n + 16 (i/4), element i % 4.
+// Block: 4 waves; wave w owns M-tiles 8 blockIdx.y + 2w + {0, 1} and every
+// N-tile.  Per 64 particles the phase tables are staged in LDS, the B
+// operands of all four 16-particle steps are built once for the block (wave
+// w builds N-tiles w, w + 4, ...) and read by all waves.
+typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
+typedef float float4_t __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ void split_f16(float x, _Float16 &h, _Float16 &l) {
+  h = (_Float16)x;
+  l = (_Float16)(x - (float)h);
+}
+
+template <int KZ>
+__global__ void __launch_bounds__(256) k_recip_struct_mfma(State s, int rt0, const Cpx<float> *__restrict__ ph,
+                                                           const float *__restrict__ amp, float *__restrict__ part) {
+  constexpr int kmax = KZ - 1, side = 2 * kmax + 1;
+  constexpr int MTOT = side * KZ, NTOT = 3 * side;
+  constexpr int MT = (MTOT + 15) / 16, NT = (NTOT + 15) / 16;
+  constexpr int MW = 2;  // M-tiles per wave
+  __shared__ Cpx<float> sh[64 * 3 * KZ];
+  __shared__ float sa[64 * 3];
+  // B operands of the 4 steps of a 64-particle stage: [step][N-tile][6][lane]
+  __shared__ half4_t sb[4][NT][6][64];
+  const int rtid = rt0 + blockIdx.x;
+  const Task rt = s.rtasks[rtid];
+  const RepDev &Rp = s.reps[rt.rep];
+  const int N = s.Ntot;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int li = lane & 15, lq = lane >> 4;
+  float4_t cr[MW][NT], ci[MW][NT];
+#pragma unroll
+  for (int u = 0; u < MW; ++u)
+#pragma unroll
+    for (int t = 0; t < NT; ++t) cr[u][t] = ci[u][t] = float4_t{0.f, 0.f, 0.f, 0.f};
+  // this lane's A rows (one per owned M-tile) and B column
+  int mrow[MW];
+#pragma unroll
+  for (int u = 0; u < MW; ++u) mrow[u] = (8 * blockIdx.y + 2 * wv + u) * 16 + li;
+  for (int p0 = 0; p0 < rt.n; p0 += 64) {
+    const int np = min(64, rt.n - p0);
+    __syncthreads();
+    for (int q = threadIdx.x; q < 64 * 3 * KZ; q += 256) {
+      const int p = q / (3 * KZ), r = q % (3 * KZ), a = r / KZ, n = r % KZ;
+      Cpx<float> v;
+      v.re = v.im = 0.f;
+      if (p < np) v = ph[((size_t)a * kMaxKZ + n) * N + rt.g0 + p0 + p];
+      sh[q] = v;
+    }
+    for (int q = threadIdx.x; q < 64 * 3; q += 256) {
+      const int p = q / 3, a = q % 3;
+      sa[q] = p < np ? amp[a * N + rt.g0 + p0 + p] : 0.f;
+    }
+    __syncthreads();
+    // B = c_a e_x(nx) for the 4 steps (particles of padding have c = 0)
+    for (int t = wv; t < NT; t += 4) {
+      const int n = t * 16 + li;
+      const bool ok = n < NTOT;
+      const int a = ok ? n / side : 0, nxi = ok ? n % side : kmax;
+      const int nx = nxi - kmax, ax = nx < 0 ? -nx : nx;
+      const float sx = nx < 0 ? -1.f : 1.f;
+#pragma unroll
+      for (int step = 0; step < 4; ++step) {
+        half4_t rh, rl, ih, il, nih, nil;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int p = step * 16 + 4 * lq + j;
+          const Cpx<float> ex = sh[p * 3 * KZ + ax];
+          const float c = ok ? sa[3 * p + a] : 0.f;
+          const float br = c * ex.re, bi = c * (sx * ex.im);
+          _Float16 h, l;
+          split_f16(br, h, l);
+          rh[j] = h;
+          rl[j] = l;
+          split_f16(bi, h, l);
+          ih[j] = h;
+          il[j] = l;
+          nih[j] = -h;
+          nil[j] = -l;
+        }
+        sb[step][t][0][lane] = rh;
+        sb[step][t][1][lane] = rl;
+        sb[step][t][2][lane] = ih;
+        sb[step][t][3][lane] = il;
+        sb[step][t][4][lane] = nih;
+        sb[step][t][5][lane] = nil;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int step = 0; step < 4; ++step) {
+      if (step * 16 >= np) break;  // block-uniform
+#pragma unroll
+      for (int u = 0; u < MW; ++u) {
+        const int m = mrow[u];
+        const bool ok = m < MTOT;
+        const int nyi = ok ? m / KZ : kmax, nz = ok ? m % KZ : 0;
+        const int ny = nyi - kmax, ay = ny < 0 ? -ny : ny;
+        const float sy = ny < 0 ? -1.f : 1.f;
+        half4_t arh, arl, aih, ail;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int p = step * 16 + 4 * lq + j;
+          const Cpx<float> *tp = sh + p * 3 * KZ;
+          Cpx<float> ey = tp[KZ + ay], ez = tp[2 * KZ + nz];
+          ey.im *= sy;
+          float ar = fmaf(ey.re, ez.re, -ey.im * ez.im);
+          float ai = fmaf(ey.re, ez.im, ey.im * ez.re);
+          if (!ok) ar = ai = 0.f;
+          _Float16 h, l;
+          split_f16(ar, h, l);
+          arh[j] = h;
+          arl[j] = l;
+          split_f16(ai, h, l);
+          aih[j] = h;
+          ail[j] = l;
+        }
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+          const half4_t brh = sb[step][t][0][lane], brl = sb[step][t][1][lane];
+          const half4_t bih = sb[step][t][2][lane], bil = sb[step][t][3][lane];
+          const half4_t nbih = sb[step][t][4][lane], nbil = sb[step][t][5][lane];
+          float4_t r = cr[u][t], i = ci[u][t];
+          r = __builtin_amdgcn_mfma_f32_16x16x16f16(arh, brh, r, 0, 0, 0);
+          r = __builtin_amdgcn_mfma_f32_16x16x16f16(arh, brl, r, 0, 0, 0);
+          r = __builtin_amdgcn_mfma_f32_16x16x16f16(arl, brh, r, 0, 0, 0);
+          r = __builtin_amdgcn_mfma_f32_16x16x16f16(aih, nbih, r, 0, 0, 0);
+          r = __builtin_amdgcn_mfma_f32_16x16x16f16(aih, nbil, r, 0, 0, 0);
+          r = __builtin_amdgcn_mfma_f32_16x16x16f16(ail, nbih, r, 0, 0, 0);
+          i = __builtin_amdgcn_mfma_f32_16x16x16f16(arh, bih, i, 0, 0, 0);
+          i = __builtin_amdgcn_mfma_f32_16x16x16f16(arh, bil, i, 0, 0, 0);
+          i = __builtin_amdgcn_mfma_f32_16x16x16f16(arl, bih, i, 0, 0, 0);
+          i = __builtin_amdgcn_mfma_f32_16x16x16f16(aih, brh, i, 0, 0, 0);
+          i = __builtin_amdgcn_mfma_f32_16x16x16f16(aih, brl, i, 0, 0, 0);
+          i = __builtin_amdgcn_mfma_f32_16x16x16f16(ail, brh, i, 0, 0, 0);
+          cr[u][t] = r;
+          ci[u][t] = i;
+        }
+      }
+    }
+  }
+  // partials in the layout of k_recip_struct: slot (col, nz), col = (ny, nx)
+  const long nslot = (long)Rp.ncol * KZ;
+  float *dst = part + Rp.part0 + (long)(rtid - Rp.rtask0) * nslot * 6;
+#pragma unroll
+  for (int u = 0; u < MW; ++u)
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int m = (8 * blockIdx.y + 2 * wv + u) * 16 + 4 * lq + j, n = t * 16 + li;
+        if (m < MTOT && n < NTOT) {
+          const int nyi = m / KZ, nz = m % KZ, a = n / side, nxi = n % side;
+          float *o = dst + ((long)(nyi * side + nxi) * KZ + nz) * 6 + 2 * a;
+          o[0] = cr[u][t][j];
+          o[1] = ci[u][t][j];
+        }
+      }
+}
+
 // Per slot: Q = sum over the replica's chunks (FP64, fixed order), S = k.Q,
 // energy (val/2)(|S|^2 - k^T M2 k) and the pressure-tensor term of
 // force_dipK (v25/functions.cpp:1926-1952, with its a4 = 1/(4 alpha));
@@ -2066,8 +2237,14 @@ void launch_recip_tables(const State &s, bool mixed, void *ph, void *amp, double
 template <typename R, int KZ>
 static void recip_struct_kz(const State &s, int rt0, int nrt, int maxcol, const void *ph, const void *amp,
                             void *part, hipStream_t st) {
-  MSC_LAUNCH((k_recip_struct<R, KZ>), dim3(nrt, nblk(maxcol, 256)), dim3(256), 0, st, s, rt0,
-                     (const Cpx<R> *)ph, (const R *)amp, (R *)part);
+  if constexpr (sizeof(R) == sizeof(float)) {
+    constexpr int MT = ((2 * KZ - 1) * KZ + 15) / 16;
+    MSC_LAUNCH((k_recip_struct_mfma<KZ>), dim3(nrt, (MT + 7) / 8), dim3(256), 0, st, s, rt0,
+               (const Cpx<float> *)ph, (const float *)amp, (float *)part);
+  } else {
+    MSC_LAUNCH((k_recip_struct<R, KZ>), dim3(nrt, nblk(maxcol, 256)), dim3(256), 0, st, s, rt0,
+               (const Cpx<R> *)ph, (const R *)amp, (R *)part);
+  }
 }
 template <typename R, int KZ>
 static void recip_force_kz(const State &s, const void *ph, const void *amp, const void *A, hipStream_t st) {

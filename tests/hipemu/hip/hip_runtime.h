@@ -44,6 +44,7 @@ struct Block {
   std::unique_ptr<std::barrier<>> bar;
   std::vector<std::unique_ptr<std::barrier<>>> wbar;
   std::vector<double> xch;  // [thread] exchange slots for shuffles
+  std::vector<float> mxa, mxb;  // [thread][4] operand exchange of matrix instructions
 };
 inline Block *&cur_block() { static Block *b = nullptr; return b; }
 inline thread_local dim3 t_threadIdx, t_blockIdx, t_blockDim, t_gridDim;
@@ -89,16 +90,51 @@ inline void hipemu_wave_barrier() {
   b->wbar[hipemu::t_threadIdx.x / 64]->arrive_and_wait();
 }
 #define __builtin_amdgcn_wave_barrier() hipemu_wave_barrier()
+
+#ifdef __clang__
+// v_mfma_f32_16x16x16f16 on the lanes of one wave: A[i][k] in lane
+// i + 16 (k/4), element k % 4; B[k][n] in lane n + 16 (k/4); D[i][n] in lane
+// n + 16 (i/4), element i % 4 (the products of two halves are exact in float)
+typedef _Float16 hipemu_half4 __attribute__((ext_vector_type(4)));
+typedef float hipemu_float4 __attribute__((ext_vector_type(4)));
+inline hipemu_float4 hipemu_mfma16(hipemu_half4 a, hipemu_half4 b, hipemu_float4 c) {
+  hipemu::Block *bk = hipemu::cur_block();
+  const unsigned t = hipemu::t_threadIdx.x, w = t / 64, lane = t % 64;
+  float *xa = bk->mxa.data() + w * 256, *xb = bk->mxb.data() + w * 256;
+  for (int j = 0; j < 4; ++j) {
+    xa[lane * 4 + j] = (float)a[j];
+    xb[lane * 4 + j] = (float)b[j];
+  }
+  bk->wbar[w]->arrive_and_wait();
+  hipemu_float4 d;
+  for (int j = 0; j < 4; ++j) {
+    const int i = 4 * (lane / 16) + j, n = lane % 16;
+    float acc = c[j];
+    for (int k = 0; k < 16; ++k) acc += xa[(i + 16 * (k / 4)) * 4 + k % 4] * xb[(n + 16 * (k / 4)) * 4 + k % 4];
+    d[j] = acc;
+  }
+  bk->wbar[w]->arrive_and_wait();
+  return d;
+}
+#define __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, cbsz, abid, blgp) hipemu_mfma16(a, b, c)
+#endif
 #define __builtin_amdgcn_fence(...) ((void)0)
 inline void __threadfence() { std::atomic_thread_fence(std::memory_order_seq_cst); }
 
 inline int atomicAdd(int *p, int v) { return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST); }
+#ifdef __clang__
+// clang provides the scoped atomic builtins on the host as well
+#ifndef __HIP_MEMORY_SCOPE_AGENT
+#define __HIP_MEMORY_SCOPE_AGENT 3
+#endif
+#else
 #define __HIP_MEMORY_SCOPE_AGENT 3
 template <class T> inline T __hip_atomic_load(T *p, int, int) { return __atomic_load_n(p, __ATOMIC_SEQ_CST); }
 template <class T> inline void __hip_atomic_store(T *p, T v, int, int) { __atomic_store_n(p, v, __ATOMIC_SEQ_CST); }
 template <class T> inline bool __hip_atomic_compare_exchange_strong(T *p, T *e, T d, int, int, int) {
   return __atomic_compare_exchange_n(p, e, d, false, __ATOMIC_SEQ_CST, __ATOMIC_SEQ_CST);
 }
+#endif
 inline int atomicMax(int *p, int v) {
   int old = __atomic_load_n(p, __ATOMIC_SEQ_CST);
   while (old < v && !__atomic_compare_exchange_n(p, &old, v, false, __ATOMIC_SEQ_CST, __ATOMIC_SEQ_CST)) {
@@ -167,6 +203,8 @@ void launch(F f, dim3 grid, dim3 block, size_t, hipStream_t, A... args) {
         unsigned nw = (nt + 63) / 64;
         for (unsigned w = 0; w < nw; ++w) B.wbar.push_back(std::make_unique<std::barrier<>>(std::min(64u, nt - 64 * w)));
         B.xch.assign(nw * 64, 0.0);
+        B.mxa.assign(nw * 64 * 4, 0.f);
+        B.mxb.assign(nw * 64 * 4, 0.f);
         cur_block() = &B;
         std::vector<std::thread> th;
         th.reserve(nt);
