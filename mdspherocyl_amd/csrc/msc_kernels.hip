@@ -1060,6 +1060,7 @@ __global__ void __launch_bounds__(256) k_recip_tables(State s, Cpx<R> *__restric
 template <typename R, int KZ>
 __global__ void __launch_bounds__(256) k_recip_struct(State s, int rt0, const Cpx<R> *__restrict__ ph,
                                                       const R *__restrict__ amp, R *__restrict__ part) {
+  static_assert(kChunk == 64, "the staging loop maps lanes to 64 particles");
   __shared__ Cpx<R> sh[kChunk * 3 * KZ];
   __shared__ R sa[kChunk * 3];
   const int rtid = rt0 + blockIdx.x;
@@ -1081,9 +1082,10 @@ __global__ void __launch_bounds__(256) k_recip_struct(State s, int rt0, const Cp
   for (int p0 = 0; p0 < rt.n; p0 += kChunk) {
     const int np = min(kChunk, rt.n - p0);
     __syncthreads();
-    for (int q = threadIdx.x; q < np * 3 * KZ; q += blockDim.x) {
-      int p = q / (3 * KZ), r = q % (3 * KZ), a = r / KZ, n = r % KZ;
-      sh[q] = ph[((size_t)a * kMaxKZ + n) * N + rt.g0 + p0 + p];
+    // consecutive lanes read consecutive particles of one table row
+    for (int q = threadIdx.x; q < 64 * 3 * KZ; q += blockDim.x) {
+      const int p = q & 63, r = q >> 6, a = r / KZ, n = r % KZ;
+      if (p < np) sh[p * 3 * KZ + r] = ph[((size_t)a * kMaxKZ + n) * N + rt.g0 + p0 + p];
     }
     for (int q = threadIdx.x; q < np * 3; q += blockDim.x) {
       int p = q / 3, a = q % 3;
@@ -1166,10 +1168,11 @@ __device__ __forceinline__ void wave_rows_sum(const double *__restrict__ rows, i
 // Operand layout of the instruction (checked on gfx950): A[i][k] in lane
 // i + 16 (k/4), element k % 4; B[k][n] in lane n + 16 (k/4), element k % 4;
 // D[i][n] in lane n + 16 (i/4), element i % 4.
-// Block: 4 waves; wave w owns M-tiles 8 blockIdx.y + 2w + {0, 1} and every
-// N-tile.  Per 64 particles the phase tables are staged in LDS, the B
-// operands of all four 16-particle steps are built once for the block (wave
-// w builds N-tiles w, w + 4, ...) and read by all waves.
+// Block: 4 waves over one chunk task; wave w owns M-tiles
+// 8 blockIdx.y + 2w + {0, 1} and every N-tile.  Per 64 particles the
+// phase tables are staged in LDS, the B operands of all four 16-particle
+// steps are built once for the block (wave w builds N-tiles w, w + 4, ...)
+// and read by all waves.
 typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
 typedef float float4_t __attribute__((ext_vector_type(4)));
 
@@ -1203,16 +1206,17 @@ __global__ void __launch_bounds__(256) k_recip_struct_mfma(State s, int rt0, con
   // this lane's A rows (one per owned M-tile) and B column
   int mrow[MW];
 #pragma unroll
-  for (int u = 0; u < MW; ++u) mrow[u] = (8 * blockIdx.y + 2 * wv + u) * 16 + li;
+  for (int u = 0; u < MW; ++u) mrow[u] = (4 * MW * blockIdx.y + MW * wv + u) * 16 + li;
   for (int p0 = 0; p0 < rt.n; p0 += 64) {
     const int np = min(64, rt.n - p0);
     __syncthreads();
+    // consecutive lanes read consecutive particles of one table row
     for (int q = threadIdx.x; q < 64 * 3 * KZ; q += 256) {
-      const int p = q / (3 * KZ), r = q % (3 * KZ), a = r / KZ, n = r % KZ;
+      const int p = q & 63, r = q >> 6, a = r / KZ, n = r % KZ;
       Cpx<float> v;
       v.re = v.im = 0.f;
       if (p < np) v = ph[((size_t)a * kMaxKZ + n) * N + rt.g0 + p0 + p];
-      sh[q] = v;
+      sh[p * 3 * KZ + r] = v;
     }
     for (int q = threadIdx.x; q < 64 * 3; q += 256) {
       const int p = q / 3, a = q % 3;
@@ -1315,7 +1319,7 @@ __global__ void __launch_bounds__(256) k_recip_struct_mfma(State s, int rt0, con
     for (int t = 0; t < NT; ++t)
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        const int m = (8 * blockIdx.y + 2 * wv + u) * 16 + 4 * lq + j, n = t * 16 + li;
+        const int m = (4 * MW * blockIdx.y + MW * wv + u) * 16 + 4 * lq + j, n = t * 16 + li;
         if (m < MTOT && n < NTOT) {
           const int nyi = m / KZ, nz = m % KZ, a = n / side, nxi = n % side;
           float *o = dst + ((long)(nyi * side + nxi) * KZ + nz) * 6 + 2 * a;
@@ -2239,7 +2243,7 @@ static void recip_struct_kz(const State &s, int rt0, int nrt, int maxcol, const 
                             void *part, hipStream_t st) {
   if constexpr (sizeof(R) == sizeof(float)) {
     constexpr int MT = ((2 * KZ - 1) * KZ + 15) / 16;
-    MSC_LAUNCH((k_recip_struct_mfma<KZ>), dim3(nrt, (MT + 7) / 8), dim3(256), 0, st, s, rt0,
+    MSC_LAUNCH((k_recip_struct_mfma<KZ>), dim3(nrt, (MT + 4 * 2 - 1) / (4 * 2)), dim3(256), 0, st, s, rt0,
                (const Cpx<float> *)ph, (const float *)amp, (float *)part);
   } else {
     MSC_LAUNCH((k_recip_struct<R, KZ>), dim3(nrt, nblk(maxcol, 256)), dim3(256), 0, st, s, rt0,
