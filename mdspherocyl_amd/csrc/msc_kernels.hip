@@ -494,37 +494,90 @@ __global__ void __launch_bounds__(256) k_build_lists(State s, const int *__restr
 // replicas whose contact skin is used up while the reference list is still
 // valid.
 // one wave per particle, the lanes over its reference row (kept in order)
-__device__ void short_build_wave(const State &s, int g, const int *__restrict__ srebuild, int *__restrict__ flags) {
+// The contact row of particle g from its reference row, in the row's order,
+// by one wave (Q: the wave's 128-int LDS queue).  An exact lower bound (the
+// distance of each centre to the other segment, less half the other
+// segment's length) drops most entries; the rest are queued and their
+// segment distances taken 64 at a time with all lanes busy.
+__device__ void short_build_wave(const State &s, int g, const int *__restrict__ srebuild, int *__restrict__ flags,
+                                 int *Q) {
   const int lane = threadIdx.x & 63;
   const int rp = __builtin_amdgcn_readfirstlane(s.rep_of[g]);
   if (!srebuild[rp]) return;
   const RepDev &R = s.reps[rp];
+  const int N = s.Ntot;
   const double rs2 = R.r_short * R.r_short;
+  const double rg[3] = {s.x[g], s.x[N + g], s.x[2 * N + g]};
+  const double ug[3] = {s.ori[6 * N + g], s.ori[7 * N + g], s.ori[8 * N + g]};
+  const double hg = 0.5 * s.len[g];
   const int n = s.nl_cnt[g];
-  int ns = 0;
-  for (int q0 = 0; q0 < n; q0 += 64) {
-    const int q = q0 + lane;
-    const bool act = q < n;
-    const int j = act ? s.nl_idx[row_at(R.nl_base, R.N, R.n0, q, g)] : g;
+  int ns = 0, qn = 0;
+  auto flush = [&](int count) {
+    const bool act = lane < count;
+    const int j = act ? Q[lane] : g;
     const bool in = act && pair_segdist(s, R, g, j) < rs2;
     const unsigned long long m = __ballot(in);
     const int pos = ns + lane_prefix(m, lane);
     if (in && pos < R.sl_w) s.sl_idx[row_at(R.sl_base, R.N, R.n0, pos, g)] = j;
     ns += __popcll(m);
+  };
+  for (int q0 = 0; q0 < n; q0 += 64) {
+    const int q = q0 + lane;
+    const bool act = q < n;
+    const int j = act ? s.nl_idx[row_at(R.nl_base, R.N, R.n0, q, g)] : g;
+    bool pass = act;
+    {
+      double rgj[3];
+      for (int a = 0; a < 3; ++a) rgj[a] = wrapd(rg[a] - s.x[a * N + j], R.L[a]);
+      const double uj[3] = {s.ori[6 * N + j], s.ori[7 * N + j], s.ori[8 * N + j]};
+      const double hj = 0.5 * s.len[j];
+      double t1 = -dot3(rgj, ug), t2 = dot3(rgj, uj);
+      t1 = fmin(fmax(t1, -hg), hg);
+      t2 = fmin(fmax(t2, -hj), hj);
+      double d1 = 0.0, d2 = 0.0;
+      for (int a = 0; a < 3; ++a) {
+        const double w1 = -rgj[a] - t1 * ug[a], w2 = rgj[a] - t2 * uj[a];
+        d1 += w1 * w1;
+        d2 += w2 * w2;
+      }
+      const double R1 = R.r_short + hj + 1e-9, R2 = R.r_short + hg + 1e-9;
+      pass = pass && d1 < R1 * R1 * (1 + 1e-12) && d2 < R2 * R2 * (1 + 1e-12);
+    }
+    const unsigned long long m = __ballot(pass);
+    if (pass) Q[qn + lane_prefix(m, lane)] = j;
+    qn += __popcll(m);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (qn >= 64) {
+      flush(64);
+      const int rest = qn - 64;
+      const int keep = lane < rest ? Q[64 + lane] : 0;
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      if (lane < rest) Q[lane] = keep;
+      qn = rest;
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
   }
+  if (qn > 0) flush(qn);
   if (lane == 0) {
     s.sl_cnt[g] = min(ns, R.sl_w);
     if (ns > R.sl_w) atomicMax(&flags[rp * 4 + 1], ns);
   }
-  const int N = s.Ntot;
   if (lane < 3) {
     s.snap_e[lane * N + g] = s.x[lane * N + g];
     s.snap_e[(3 + lane) * N + g] = s.ori[(6 + lane) * N + g];
   }
 }
 __global__ void __launch_bounds__(256) k_short_build(State s, const int *__restrict__ srebuild, int *__restrict__ flags) {
-  const int g = s.g_begin + __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
-  if (g < s.g_end) short_build_wave(s, g, srebuild, flags);
+  __shared__ int queue[4][128];
+  const int wv = threadIdx.x >> 6;
+  const int g = s.g_begin + __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + wv);
+  if (g < s.g_end) short_build_wave(s, g, srebuild, flags, queue[wv]);
 }
 
 
