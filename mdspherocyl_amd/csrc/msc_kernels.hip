@@ -1166,14 +1166,19 @@ __global__ void __launch_bounds__(256) k_recip_struct(State s, int rt0, const Cp
     }
   }
   if (!active) return;
-  // partials: part[part0 + ((chunk within replica) * nslot + slot) * 6]
   const long nslot = (long)Rp.ncol * KZ;
-  R *dst = part + Rp.part0 + (((long)(rtid - Rp.rtask0)) * nslot + (long)col * KZ) * 6;
+  // partials of the chunk, rows m = (ny, nz) of 3 side (Re, Im) pairs in the
+  // order (component a, nx): neighbouring columns write neighbouring pairs
+  R *dst = part + Rp.part0 + ((long)(rtid - Rp.rtask0)) * nslot * 6;
+  const int nyi = ny + kmax, nxi = nx + kmax;
 #pragma unroll
   for (int z = 0; z < KZ; ++z) {
-    dst[6 * z + 0] = Qr[z][0]; dst[6 * z + 1] = Qi[z][0];
-    dst[6 * z + 2] = Qr[z][1]; dst[6 * z + 3] = Qi[z][1];
-    dst[6 * z + 4] = Qr[z][2]; dst[6 * z + 5] = Qi[z][2];
+    R *o = dst + (long)(nyi * KZ + z) * (6 * side) + 2 * nxi;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      o[2 * side * a + 0] = Qr[z][a];
+      o[2 * side * a + 1] = Qi[z][a];
+    }
   }
 }
 
@@ -1243,8 +1248,9 @@ __global__ void __launch_bounds__(256) k_recip_struct_mfma(State s, int rt0, con
   constexpr int MW = 2;  // M-tiles per wave
   __shared__ Cpx<float> sh[64 * 3 * KZ];
   __shared__ float sa[64 * 3];
-  // B operands of the 4 steps of a 64-particle stage: [step][N-tile][6][lane]
-  __shared__ half4_t sb[4][NT][6][64];
+  // B operands of the 4 steps of a 64-particle stage: [step][N-tile][4][lane]
+  // (Re hi, Re lo, Im hi, Im lo)
+  __shared__ half4_t sb[4][NT][4][64];
   const int rtid = rt0 + blockIdx.x;
   const Task rt = s.rtasks[rtid];
   const RepDev &Rp = s.reps[rt.rep];
@@ -1285,7 +1291,7 @@ __global__ void __launch_bounds__(256) k_recip_struct_mfma(State s, int rt0, con
       const float sx = nx < 0 ? -1.f : 1.f;
 #pragma unroll
       for (int step = 0; step < 4; ++step) {
-        half4_t rh, rl, ih, il, nih, nil;
+        half4_t rh, rl, ih, il;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
           const int p = step * 16 + 4 * lq + j;
@@ -1299,21 +1305,23 @@ __global__ void __launch_bounds__(256) k_recip_struct_mfma(State s, int rt0, con
           split_f16(bi, h, l);
           ih[j] = h;
           il[j] = l;
-          nih[j] = -h;
-          nil[j] = -l;
         }
         sb[step][t][0][lane] = rh;
         sb[step][t][1][lane] = rl;
         sb[step][t][2][lane] = ih;
         sb[step][t][3][lane] = il;
-        sb[step][t][4][lane] = nih;
-        sb[step][t][5][lane] = nil;
       }
     }
     __syncthreads();
 #pragma unroll
     for (int step = 0; step < 4; ++step) {
       if (step * 16 >= np) break;  // block-uniform
+      // the step's B operands, read once and used for every owned M-tile
+      half4_t bop[NT][4];
+#pragma unroll
+      for (int t = 0; t < NT; ++t)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) bop[t][c] = sb[step][t][c][lane];
 #pragma unroll
       for (int u = 0; u < MW; ++u) {
         const int m = mrow[u];
@@ -1321,7 +1329,7 @@ __global__ void __launch_bounds__(256) k_recip_struct_mfma(State s, int rt0, con
         const int nyi = ok ? m / KZ : kmax, nz = ok ? m % KZ : 0;
         const int ny = nyi - kmax, ay = ny < 0 ? -ny : ny;
         const float sy = ny < 0 ? -1.f : 1.f;
-        half4_t arh, arl, aih, ail;
+        half4_t arh, arl, aih, ail, naih, nail;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
           const int p = step * 16 + 4 * lq + j;
@@ -1338,19 +1346,19 @@ __global__ void __launch_bounds__(256) k_recip_struct_mfma(State s, int rt0, con
           split_f16(ai, h, l);
           aih[j] = h;
           ail[j] = l;
+          naih[j] = -h;
+          nail[j] = -l;
         }
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
-          const half4_t brh = sb[step][t][0][lane], brl = sb[step][t][1][lane];
-          const half4_t bih = sb[step][t][2][lane], bil = sb[step][t][3][lane];
-          const half4_t nbih = sb[step][t][4][lane], nbil = sb[step][t][5][lane];
+          const half4_t brh = bop[t][0], brl = bop[t][1], bih = bop[t][2], bil = bop[t][3];
           float4_t r = cr[u][t], i = ci[u][t];
           r = __builtin_amdgcn_mfma_f32_16x16x16f16(arh, brh, r, 0, 0, 0);
           r = __builtin_amdgcn_mfma_f32_16x16x16f16(arh, brl, r, 0, 0, 0);
           r = __builtin_amdgcn_mfma_f32_16x16x16f16(arl, brh, r, 0, 0, 0);
-          r = __builtin_amdgcn_mfma_f32_16x16x16f16(aih, nbih, r, 0, 0, 0);
-          r = __builtin_amdgcn_mfma_f32_16x16x16f16(aih, nbil, r, 0, 0, 0);
-          r = __builtin_amdgcn_mfma_f32_16x16x16f16(ail, nbih, r, 0, 0, 0);
+          r = __builtin_amdgcn_mfma_f32_16x16x16f16(naih, bih, r, 0, 0, 0);
+          r = __builtin_amdgcn_mfma_f32_16x16x16f16(naih, bil, r, 0, 0, 0);
+          r = __builtin_amdgcn_mfma_f32_16x16x16f16(nail, bih, r, 0, 0, 0);
           i = __builtin_amdgcn_mfma_f32_16x16x16f16(arh, bih, i, 0, 0, 0);
           i = __builtin_amdgcn_mfma_f32_16x16x16f16(arh, bil, i, 0, 0, 0);
           i = __builtin_amdgcn_mfma_f32_16x16x16f16(arl, bih, i, 0, 0, 0);
@@ -1363,7 +1371,7 @@ __global__ void __launch_bounds__(256) k_recip_struct_mfma(State s, int rt0, con
       }
     }
   }
-  // partials in the layout of k_recip_struct: slot (col, nz), col = (ny, nx)
+  // partials, rows m = (ny, nz) of the (Re, Im) pairs of columns n = (a, nx)
   const long nslot = (long)Rp.ncol * KZ;
   float *dst = part + Rp.part0 + (long)(rtid - Rp.rtask0) * nslot * 6;
 #pragma unroll
@@ -1374,14 +1382,17 @@ __global__ void __launch_bounds__(256) k_recip_struct_mfma(State s, int rt0, con
       for (int j = 0; j < 4; ++j) {
         const int m = (4 * MW * blockIdx.y + MW * wv + u) * 16 + 4 * lq + j, n = t * 16 + li;
         if (m < MTOT && n < NTOT) {
-          const int nyi = m / KZ, nz = m % KZ, a = n / side, nxi = n % side;
-          float *o = dst + ((long)(nyi * side + nxi) * KZ + nz) * 6 + 2 * a;
+          float *o = dst + (long)m * (2 * NTOT) + 2 * n;
           o[0] = cr[u][t][j];
           o[1] = ci[u][t][j];
         }
       }
 }
 
+// Chunk partials hold rows m = (ny, nz) of (Re, Im) pairs in the order
+// (component a, nx).  The reduction walks entries e = m * side + nx (nx
+// fastest, so neighbouring lanes read neighbouring pairs); entry e is slot
+// (column (nx, ny), nz).
 // Per slot: Q = sum over the replica's chunks (FP64, fixed order), S = k.Q,
 // energy (val/2)(|S|^2 - k^T M2 k) and the pressure-tensor term of
 // force_dipK (v25/functions.cpp:1926-1952, with its a4 = 1/(4 alpha));
@@ -1404,9 +1415,13 @@ __global__ void __launch_bounds__(512) k_recip_reduce(State s, const R *__restri
   const RepDev &Rp = s.reps[rep];
   const int nslot = Rp.ncol * Rp.KZ;
   const int lane = threadIdx.x & 63, grp = threadIdx.x >> 6;
-  const int sl = blockIdx.x * 64 + lane;
+  const int kmax = Rp.kmax, side = 2 * kmax + 1;
+  const int e = blockIdx.x * 64 + lane;
   if (blockIdx.x * 64 >= nslot) return;  // whole block idle (uniform)
-  const bool act = sl < nslot;
+  const bool act = e < nslot;
+  const int em = act ? e / side : 0, enx = act ? e % side : 0;
+  // offset of entry e's component 0 pair within a chunk; component a at + 2 side a
+  const long eoff = (long)em * (6 * side) + 2 * enx;
   double Q[6] = {0, 0, 0, 0, 0, 0};
   if (act) {
     // four chunk partials in flight at a time, added in chunk order
@@ -1415,9 +1430,12 @@ __global__ void __launch_bounds__(512) k_recip_reduce(State s, const R *__restri
       R v[4][6];
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
-        const R *p = part + Rp.part0 + ((long)(c + u * kGroups) * nslot + sl) * 6;
+        const R *p = part + Rp.part0 + (long)(c + u * kGroups) * nslot * 6 + eoff;
 #pragma unroll
-        for (int q = 0; q < 6; ++q) v[u][q] = p[q];
+        for (int a = 0; a < 3; ++a) {
+          v[u][2 * a] = p[2 * side * a];
+          v[u][2 * a + 1] = p[2 * side * a + 1];
+        }
       }
 #pragma unroll
       for (int u = 0; u < 4; ++u)
@@ -1425,9 +1443,12 @@ __global__ void __launch_bounds__(512) k_recip_reduce(State s, const R *__restri
         for (int q = 0; q < 6; ++q) Q[q] += (double)v[u][q];
     }
     for (; c < Rp.nrtask; c += kGroups) {
-      const R *p = part + Rp.part0 + ((long)c * nslot + sl) * 6;
+      const R *p = part + Rp.part0 + (long)c * nslot * 6 + eoff;
 #pragma unroll
-      for (int q = 0; q < 6; ++q) Q[q] += (double)p[q];
+      for (int a = 0; a < 3; ++a) {
+        Q[2 * a] += (double)p[2 * side * a];
+        Q[2 * a + 1] += (double)p[2 * side * a + 1];
+      }
     }
   }
 #pragma unroll
@@ -1447,9 +1468,9 @@ __global__ void __launch_bounds__(512) k_recip_reduce(State s, const R *__restri
   } else {
     wave_rows_sum<6>(m2rows, Rp.task0, Rp.ntask, lane, m);
   }
-  const int kmax = Rp.kmax, side = 2 * kmax + 1;
-  const int col = (act ? sl : 0) / Rp.KZ, nz = (act ? sl : 0) % Rp.KZ;
-  const int nx = col % side - kmax, ny = col / side - kmax;
+  const int nyi = em / Rp.KZ, nz = em % Rp.KZ;
+  const int sl = (nyi * side + enx) * Rp.KZ + nz;  // the slot of entry e
+  const int nx = enx - kmax, ny = nyi - kmax;
   const double val = act ? s.kval[Rp.slot0 + sl] : 0.0;
   double o[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
   Cpx<R> A;
