@@ -101,6 +101,7 @@ struct msc_ctx {
          *d_snap_e = nullptr, *d_snap_u = nullptr;
   int *d_clu = nullptr, *d_clu_flag = nullptr;
   double *d_rows = nullptr, *d_kblk = nullptr;  // per-task row sets and k-space block sums of the step loop
+  int *d_step = nullptr;  // {step within the run, first absolute step of the run}
   int *d_cell_of = nullptr, *d_cell_cnt = nullptr, *d_cell_start = nullptr, *d_cell_fill = nullptr,
       *d_cell_list = nullptr;
   int *d_cell_tmp = nullptr;
@@ -124,8 +125,9 @@ struct msc_ctx {
   size_t save_cap = 0;
   void *pin = nullptr;  // pinned staging buffer of kPinBytes
   size_t run_gs_cap = 0, run_obs_cap = 0;
+  // the step captured as a graph; dropped when anything it holds by value
+  // (row regions, run buffers, replica parameters) changes
   hipGraphExec_t graph_exec = nullptr;
-  int graph_renorm = -1;
 };
 
 static std::string g_err;  // failures that have no context to hold them
@@ -353,7 +355,15 @@ void grid(msc_ctx *ctx, int rp) {
   R.ncell = R.nc[0] * R.nc[1] * R.nc[2];
 }
 
+// the captured step holds row regions, run buffers and launch choices
+static void drop_graph(msc_ctx *ctx) {
+  if (ctx->graph_exec) hipGraphExecDestroy(ctx->graph_exec);
+  ctx->graph_exec = nullptr;
+}
+
 int upload_reps(msc_ctx *ctx) {
+  // replica parameters decide which launches a step makes (field, modes)
+  drop_graph(ctx);
   return h2d(ctx, ctx->d_reps, ctx->reps.data(), sizeof(RepDev) * ctx->nrep);
 }
 
@@ -632,6 +642,7 @@ int msc_create(int device, int precision, int nrep, const msc_replica_desc *reps
   A(&ctx->d_clu_flag, 2);
   A(&ctx->d_rows, ctx->tasks.size() * 16 * kRowSets);
   A(&ctx->d_kblk, (size_t)nrep * ((ctx->maxslot + 63) / 64) * 10);
+  A(&ctx->d_step, 2);
   A(&ctx->d_cell_of, N); A(&ctx->d_cell_list, N);
   A(&ctx->d_cell_tmp, N);
   A(&ctx->d_cell_pack, (size_t)8 * N);
@@ -714,7 +725,7 @@ void msc_destroy(msc_ctx *ctx) {
                   ctx->d_cell_cnt, ctx->d_cell_start, ctx->d_cell_fill, ctx->d_cell_list, ctx->d_nl_cnt,
                   ctx->d_nl_idx, ctx->d_dl_cnt, ctx->d_dl_idx, ctx->d_kval, ctx->d_cell_tmp, ctx->d_sl_cnt,
                   ctx->d_sl_idx, ctx->d_snap_e, ctx->d_srebuild, ctx->d_cell_pack, 
-                  ctx->d_snap_u, ctx->d_clu, ctx->d_clu_flag, ctx->d_rows, ctx->d_kblk,
+                  ctx->d_snap_u, ctx->d_clu, ctx->d_clu_flag, ctx->d_rows, ctx->d_kblk, ctx->d_step,
                   ctx->d_tab, ctx->d_ph, ctx->d_amp, ctx->d_part, ctx->d_A, ctx->d_slot_out, ctx->d_task_out,
                   ctx->d_red, ctx->d_M2, ctx->d_Kin, ctx->d_gs, ctx->d_obs, ctx->d_counts, ctx->d_E,
                   ctx->d_th, ctx->d_flags, ctx->d_rebuild, ctx->d_one, ctx->d_run_gs, ctx->d_run_obs, ctx->d_save};
@@ -881,6 +892,7 @@ static bool tight_lists() {
 // an overflowing rebuild, 0 where nothing overflowed).
 static int lists_layout(msc_ctx *ctx, const std::vector<long> &need_nl = {},
                         const std::vector<long> &need_sl = {}, const std::vector<long> &need_dl = {}) {
+  drop_graph(ctx);
   const int nrep = ctx->nrep;
   int ncells = 0;
   std::vector<int> cell_rep;
@@ -1291,20 +1303,23 @@ int msc_init_run(msc_ctx *ctx) {
 int msc_set_graph(msc_ctx *ctx, int on) {
   if (!ctx) return MSC_E_ARG;
   ctx->graph_on = on != 0;
+  if (!ctx->graph_on) drop_graph(ctx);
   return MSC_OK;
 }
 
 // One step of every replica, enqueued on the context stream, in the order of
 // v25/main.cpp:559-708 (the fused kernels are described in msc_kernels.hip).
-static void enqueue_step(msc_ctx *ctx, int step, const double *d_gs_step, double *d_obs_step) {
+// The step reads its index from d_step, so the same launches serve every
+// step of a run and can be captured once as a graph.
+static void enqueue_step(msc_ctx *ctx) {
   State s = sel(ctx, MSC_ALL);
   hipStream_t st = ctx->stream;
   const long region = (long)ctx->tasks.size() * 16;
   double *rows = ctx->d_rows;
-  if (std::fmod((float)step, (float)100) == 0) launch_renorm(s, st);
+  const double *gs = ctx->d_run_gs;
   {
     Bracket b(ctx, "integrate");
-    launch_kick1(s, ctx->d_th, d_gs_step, rows + kRowKick1 * region, st);
+    launch_kick1(s, ctx->d_th, gs, ctx->d_step, rows + kRowKick1 * region, st);
     launch_drift(s, ctx->mixed, ctx->d_ph, ctx->d_amp, rows + kRowDrift * region, st);
   }
   {
@@ -1318,15 +1333,15 @@ static void enqueue_step(msc_ctx *ctx, int step, const double *d_gs_step, double
   if (field) launch_ext_field(s, rows + kRowExt * region, st);
   {
     Bracket b(ctx, "integrate");
-    launch_kick2(s, ctx->d_th, d_gs_step, rows + kRowKick1 * region, rows + kRowKick2 * region, st);
-    launch_end(s, ctx->d_th, d_gs_step, rows, region, field, ctx->d_kblk, ctx->maxslot, ctx->d_E, ctx->d_Kin,
-               ctx->d_counts, d_obs_step, ctx->d_rebuild, ctx->d_srebuild, st);
+    launch_kick2(s, ctx->d_th, gs, ctx->d_step, rows + kRowKick1 * region, rows + kRowKick2 * region, st);
+    launch_end(s, ctx->d_th, gs, ctx->d_step, rows, region, field, ctx->d_kblk, ctx->maxslot, ctx->d_E, ctx->d_Kin,
+               ctx->d_counts, ctx->d_run_obs, ctx->d_rebuild, ctx->d_srebuild, st);
   }
   // list maintenance (v25/main.cpp:679-701): decided in k_end; the launches
   // return at once for replicas that keep their rows
   {
     Bracket b(ctx, "nlist");
-    launch_rebuild(s, ctx->d_rebuild, ctx->mixed ? ctx->d_srebuild : nullptr, ctx->d_flags, st);
+    launch_rebuild(s, ctx->d_rebuild, ctx->mixed ? ctx->d_srebuild : nullptr, ctx->d_flags, st, ctx->d_step);
   }
 }
 
@@ -1365,6 +1380,30 @@ static int state_copy(msc_ctx *ctx, bool restore) {
 // the summation order of the rebuilt lists).
 static constexpr int kRunChunk = 50;
 
+// Capture one step (enqueue_step) as a graph on the context stream.
+static int capture_step(msc_ctx *ctx) {
+  drop_graph(ctx);
+  hipGraph_t g = nullptr;
+  HIPC(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+  enqueue_step(ctx);
+  const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+  if (e != hipSuccess) {
+    if (g) hipGraphDestroy(g);
+    ctx->err = std::string("capturing the step: ") + hipGetErrorString(e);
+    g_err = ctx->err;
+    return MSC_E_HIP;
+  }
+  const hipError_t e2 = hipGraphInstantiate(&ctx->graph_exec, g, nullptr, nullptr, 0);
+  hipGraphDestroy(g);
+  if (e2 != hipSuccess) {
+    ctx->graph_exec = nullptr;
+    ctx->err = std::string("instantiating the step graph: ") + hipGetErrorString(e2);
+    g_err = ctx->err;
+    return MSC_E_HIP;
+  }
+  return MSC_OK;
+}
+
 int msc_run(msc_ctx *ctx, int step0, int nsteps, double *obs) {
   if (!ctx || nsteps < 0) return MSC_E_ARG;
   if (!ctx->have_lists) return fail(ctx, MSC_E_STATE, "call msc_init_run first");
@@ -1384,10 +1423,12 @@ int msc_run(msc_ctx *ctx, int step0, int nsteps, double *obs) {
   // run buffers grow with the longest run and are kept by the context
   const size_t nobs = (size_t)nsteps * nrep * MSC_OBS_N;
   if (gs.size() > ctx->run_gs_cap) {
+    drop_graph(ctx);
     if (int rc_ = dalloc(ctx, &ctx->d_run_gs, gs.size())) return rc_;
     ctx->run_gs_cap = gs.size();
   }
   if (nobs > ctx->run_obs_cap) {
+    drop_graph(ctx);
     if (int rc_ = dalloc(ctx, &ctx->d_run_obs, nobs)) return rc_;
     ctx->run_obs_cap = nobs;
   }
@@ -1400,8 +1441,17 @@ int msc_run(msc_ctx *ctx, int step0, int nsteps, double *obs) {
     const int nk = std::min(kRunChunk, nsteps - k0);
     if (int rc_ = state_copy(ctx, false)) return rc_;
     launch_fill_int(ctx->d_flags, 4 * nrep, 0, ctx->stream);
-    for (int k = k0; k < k0 + nk; ++k)
-      enqueue_step(ctx, step0 + k, d_gs_all + (size_t)k * nrep * 4, d_obs_all + (size_t)k * nrep * MSC_OBS_N);
+    const int step_pos[2] = {k0, step0};
+    if (int rc_ = h2d(ctx, ctx->d_step, step_pos, sizeof step_pos)) return rc_;
+    // event brackets and per-launch synchronisation cannot sit in a replayed
+    // graph: timed or checked runs launch directly
+    const bool graph = ctx->graph_on && !ctx->timing && !launches_checked();
+    if (graph && !ctx->graph_exec)
+      if (int rc_ = capture_step(ctx)) return rc_;
+    for (int k = k0; k < k0 + nk; ++k) {
+      if (graph) HIPC(hipGraphLaunch(ctx->graph_exec, ctx->stream));
+      else enqueue_step(ctx);
+    }
     LAUNCHED("msc_run");
     if (int rc_ = d2h(ctx, fl.data(), ctx->d_flags, sizeof(int) * fl.size())) return rc_;
     // an overflowing rebuild empties the replica's rows, so an overlap later in

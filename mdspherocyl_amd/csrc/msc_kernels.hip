@@ -23,6 +23,7 @@ static int sync_check_mode() {
   }();
   return mode;
 }
+bool launches_checked() { return sync_check_mode() != 0; }
 static void sync_check(const char *kernel) {
   if (sync_check_mode() == 0) return;
   hipError_t e = hipGetLastError();
@@ -144,7 +145,10 @@ __device__ __forceinline__ void cells_assign_one(const State &s, int g, const in
   s.cell_of[g] = cell;
   atomicAdd(&s.cell_cnt[cell], 1);
 }
-__global__ void k_cells_assign(State s, const int *__restrict__ rebuild) {
+// step (may be null): the step counter of the step loop, advanced here,
+// after every kernel of the step that reads it
+__global__ void k_cells_assign(State s, const int *__restrict__ rebuild, int *__restrict__ step) {
+  if (step && blockIdx.x == 0 && threadIdx.x == 0) step[0] += 1;
   int g = s.g_begin + blockIdx.x * blockDim.x + threadIdx.x;
   if (g < s.g_end) cells_assign_one(s, g, rebuild);
 }
@@ -1916,9 +1920,7 @@ __global__ void k_moveX(State s) {
   if (g < s.g_end) drift_particle(s, s.reps[s.rep_of[g]], g);
 }
 
-__global__ void k_renorm(State s) {
-  int g = s.g_begin + blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= s.g_end) return;
+__device__ __forceinline__ void renorm_particle(const State &s, int g) {
   const int N = s.Ntot;
   double o[9];
   for (int k = 0; k < 9; ++k) o[k] = s.ori[k * N + g];
@@ -1935,6 +1937,10 @@ __global__ void k_renorm(State s) {
   o[1] = o[5] * o[6] - o[8] * o[3];
   o[2] = o[3] * o[7] - o[6] * o[4];
   for (int k = 0; k < 9; ++k) s.ori[k * N + g] = o[k];
+}
+__global__ void k_renorm(State s) {
+  int g = s.g_begin + blockIdx.x * blockDim.x + threadIdx.x;
+  if (g < s.g_end) renorm_particle(s, g);
 }
 
 // moveTher for every replica (one thread each).  gs[rep*4 + 2*i + 0/1] hold
@@ -2026,15 +2032,24 @@ __global__ void k_zero_FT(State s) {
 //   k_end    every per-replica sum, moveTher mode 2, energies, observables
 //            and the rebuild decision (one block per replica).
 // --------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_kick1(State s, const Thermo *__restrict__ th, const double *__restrict__ gs,
+// step[0] = k, the step within the run (it selects the step's deviates
+// gs_base + 4 nrep k and observable rows), step[1] = the run's first
+// absolute step; the step counter advances in the step's first list kernel.
+__global__ void __launch_bounds__(256) k_kick1(State s, const Thermo *__restrict__ th,
+                                               const double *__restrict__ gs_base, const int *__restrict__ step,
                                                double *__restrict__ rows) {
   const int tid = s.task_begin + wave_task_id();
   const int lane = threadIdx.x & 63;
   if (wave_task_id() >= s.ntask) return;
   const Task tk = s.tasks[tid];
   const RepDev &Rp = s.reps[tk.rep];
+  const int k = __builtin_amdgcn_readfirstlane(step[0]);
+  const double *gs = gs_base + (size_t)k * s.nrep * 4;
+  // renorm every 100 steps at the start of the step
+  const bool renorm = fmodf((float)(step[1] + k), 100.f) == 0.f;
   double acc[14] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
   if (lane < tk.n) {
+    if (renorm) renorm_particle(s, tk.g0 + lane);
     const Thermo T = th[tk.rep];
     const double ts[2] = {thermo_half(Rp, T.s[0], gs[tk.rep * 4 + 0]), thermo_half(Rp, T.s[1], gs[tk.rep * 4 + 2])};
     kick_particle<false>(s, Rp, tk.g0 + lane, ts, acc);
@@ -2098,11 +2113,13 @@ __global__ void __launch_bounds__(256) k_drift(State s, Cpx<R> *__restrict__ ph,
 
 // moveTher mode 1 from the first kick's energies, re-summed by each wave,
 // then the last moveP.  rows2: [0..13] as kick_particle<true>.
-__global__ void __launch_bounds__(256) k_kick2(State s, const Thermo *__restrict__ th, const double *__restrict__ gs,
+__global__ void __launch_bounds__(256) k_kick2(State s, const Thermo *__restrict__ th,
+                                               const double *__restrict__ gs_base, const int *__restrict__ step,
                                                const double *__restrict__ rows1, double *__restrict__ rows2) {
   const int tid = s.task_begin + wave_task_id();
   const int lane = threadIdx.x & 63;
   if (wave_task_id() >= s.ntask) return;
+  const double *gs = gs_base + (size_t)__builtin_amdgcn_readfirstlane(step[0]) * s.nrep * 4;
   const Task tk = s.tasks[tid];
   const RepDev &Rp = s.reps[tk.rep];
   double K1[2];
@@ -2121,12 +2138,15 @@ __global__ void __launch_bounds__(256) k_kick2(State s, const Thermo *__restrict
 // Kin, the pair counts, the observables of print_out and the list decision
 // of k_check_decide; zeroes the cell counters of a replica whose list is
 // rebuilt next.
-__global__ void __launch_bounds__(256) k_end(State s, Thermo *__restrict__ th, const double *__restrict__ gs,
-                                             const double *__restrict__ rows, long region, int has_field,
-                                             const double *__restrict__ kblk, int kblk_stride,
-                                             RepEnergy *__restrict__ E, double *__restrict__ Kin,
-                                             double *__restrict__ counts, double *__restrict__ obs,
+__global__ void __launch_bounds__(256) k_end(State s, Thermo *__restrict__ th, const double *__restrict__ gs_base,
+                                             const int *__restrict__ step, const double *__restrict__ rows,
+                                             long region, int has_field, const double *__restrict__ kblk,
+                                             int kblk_stride, RepEnergy *__restrict__ E, double *__restrict__ Kin,
+                                             double *__restrict__ counts, double *__restrict__ obs_base,
                                              int *__restrict__ rebuild, int *__restrict__ srebuild) {
+  const int kstep = step[0];
+  const double *gs = gs_base + (size_t)kstep * s.nrep * 4;
+  double *obs = obs_base + (size_t)kstep * s.nrep * 12;
   __shared__ double v1[2], v2[14], vsc[12], vr[11], vext[1], vchk[2], vk[10];
   __shared__ int rb_sh;
   const int rp = s.rep_begin + blockIdx.x;
@@ -2239,12 +2259,12 @@ static inline int nblk(int n, int b) { return (n + b - 1) / b; }
 void launch_dipoles(const State &s, hipStream_t st) {
   MSC_LAUNCH(k_dipoles, dim3(nblk(s.g_end - s.g_begin, 256)), dim3(256), 0, st, s);
 }
-void launch_cells(const State &s, const int *rebuild, bool zero, hipStream_t st) {
+void launch_cells(const State &s, const int *rebuild, bool zero, hipStream_t st, int *step) {
   if (zero) {
     MSC_LAUNCH(k_fill_int, dim3(nblk(s.ncells, 256)), dim3(256), 0, st, s.cell_cnt, s.ncells, 0);
     MSC_LAUNCH(k_fill_int, dim3(nblk(s.ncells, 256)), dim3(256), 0, st, s.cell_fill, s.ncells, 0);
   }
-  MSC_LAUNCH(k_cells_assign, dim3(nblk(s.g_end - s.g_begin, 256)), dim3(256), 0, st, s, rebuild);
+  MSC_LAUNCH(k_cells_assign, dim3(nblk(s.g_end - s.g_begin, 256)), dim3(256), 0, st, s, rebuild, step);
   MSC_LAUNCH(k_scan_cells, dim3(s.nrep), dim3(1024), 0, st, s.cell_cnt, s.cell_start, s.reps, rebuild);
   MSC_LAUNCH(k_cells_fill, dim3(nblk(s.g_end - s.g_begin, 256)), dim3(256), 0, st, s, rebuild);
   MSC_LAUNCH(k_cells_rank, dim3(nblk(s.g_end - s.g_begin, 256)), dim3(256), 0, st, s, rebuild);
@@ -2252,8 +2272,8 @@ void launch_cells(const State &s, const int *rebuild, bool zero, hipStream_t st)
 void launch_lists(const State &s, const int *rebuild, int *flags, hipStream_t st) {
   MSC_LAUNCH(k_build_lists, dim3(nblk(s.g_end - s.g_begin, 4)), dim3(256), 0, st, s, rebuild, flags);
 }
-void launch_rebuild(const State &s, const int *rebuild, const int *srebuild, int *flags, hipStream_t st) {
-  launch_cells(s, rebuild, false, st);
+void launch_rebuild(const State &s, const int *rebuild, const int *srebuild, int *flags, hipStream_t st, int *step) {
+  launch_cells(s, rebuild, false, st, step);
   launch_lists(s, rebuild, flags, st);
   if (srebuild) launch_short(s, srebuild, flags, st);
 }
@@ -2401,8 +2421,8 @@ void launch_clusters(const State &s, double min_s, double reach_max, int *par, i
   MSC_LAUNCH(k_clust_edges, grid, dim3(256), 0, st, s, min_s, reach_max, par, flags);
   MSC_LAUNCH(k_clust_final, grid, dim3(256), 0, st, s, par);
 }
-void launch_kick1(const State &s, const Thermo *th, const double *gs, double *rows, hipStream_t st) {
-  MSC_LAUNCH(k_kick1, dim3(nblk(s.ntask, 4)), dim3(256), 0, st, s, th, gs, rows);
+void launch_kick1(const State &s, const Thermo *th, const double *gs, const int *step, double *rows, hipStream_t st) {
+  MSC_LAUNCH(k_kick1, dim3(nblk(s.ntask, 4)), dim3(256), 0, st, s, th, gs, step, rows);
 }
 void launch_drift(const State &s, bool mixed, void *ph, void *amp, double *rows, hipStream_t st) {
   if (mixed)
@@ -2410,15 +2430,15 @@ void launch_drift(const State &s, bool mixed, void *ph, void *amp, double *rows,
   else
     MSC_LAUNCH(k_drift<double>, dim3(s.ntask), dim3(256), 0, st, s, (Cpx<double> *)ph, (double *)amp, rows);
 }
-void launch_kick2(const State &s, const Thermo *th, const double *gs, const double *rows1, double *rows2,
-                  hipStream_t st) {
-  MSC_LAUNCH(k_kick2, dim3(nblk(s.ntask, 4)), dim3(256), 0, st, s, th, gs, rows1, rows2);
+void launch_kick2(const State &s, const Thermo *th, const double *gs, const int *step, const double *rows1,
+                  double *rows2, hipStream_t st) {
+  MSC_LAUNCH(k_kick2, dim3(nblk(s.ntask, 4)), dim3(256), 0, st, s, th, gs, step, rows1, rows2);
 }
-void launch_end(const State &s, Thermo *th, const double *gs, const double *rows, long region, bool has_field,
-                const double *kblk, int maxslot, RepEnergy *E, double *Kin, double *counts, double *obs,
-                int *rebuild, int *srebuild, hipStream_t st) {
-  MSC_LAUNCH(k_end, dim3(s.rep_end - s.rep_begin), dim3(256), 0, st, s, th, gs, rows, region, has_field ? 1 : 0,
-             kblk, nblk(maxslot, 64), E, Kin, counts, obs, rebuild, srebuild);
+void launch_end(const State &s, Thermo *th, const double *gs, const int *step, const double *rows, long region,
+                bool has_field, const double *kblk, int maxslot, RepEnergy *E, double *Kin, double *counts,
+                double *obs, int *rebuild, int *srebuild, hipStream_t st) {
+  MSC_LAUNCH(k_end, dim3(s.rep_end - s.rep_begin), dim3(256), 0, st, s, th, gs, step, rows, region,
+             has_field ? 1 : 0, kblk, nblk(maxslot, 64), E, Kin, counts, obs, rebuild, srebuild);
 }
 void launch_fill_int(int *p, int n, int v, hipStream_t st) {
   MSC_LAUNCH(k_fill_int, dim3(nblk(n, 256)), dim3(256), 0, st, p, n, v);

@@ -42,17 +42,19 @@ struct State {
 // first kernel launch that failed since the last call (hipSuccess if none),
 // with the kernel's name; clears the record
 hipError_t take_launch_error(const char **kernel);
+// MSC_SYNC_CHECK is set: every launch is followed by a synchronisation
+bool launches_checked();
 
 void launch_dipoles(const State &s, hipStream_t st);
 // zero: clear every cell counter first (the step loop's k_end clears those
 // of the replicas it rebuilds)
-void launch_cells(const State &s, const int *rebuild, bool zero, hipStream_t st);
+void launch_cells(const State &s, const int *rebuild, bool zero, hipStream_t st, int *step = nullptr);
 void launch_lists(const State &s, const int *rebuild, int *flags, hipStream_t st);
 void launch_short(const State &s, const int *srebuild, int *flags, hipStream_t st);
 // the step's list maintenance: cell grid and rows of the replicas with
 // rebuild set (cell counters already zero), contact rows of those with
 // srebuild set (srebuild may be null)
-void launch_rebuild(const State &s, const int *rebuild, const int *srebuild, int *flags, hipStream_t st);
+void launch_rebuild(const State &s, const int *rebuild, const int *srebuild, int *flags, hipStream_t st, int *step);
 void launch_check(const State &s, double *task_out, int *rebuild, int *srebuild, double *obs_rebuild,
                   int obs_stride, bool standalone, hipStream_t st);
 void launch_force_sc(const State &s, bool strict, double *task_out, int *flags, hipStream_t st);
@@ -84,13 +86,15 @@ void launch_fill_int(int *p, int n, int v, hipStream_t st);
 // the step loop's fused kernels; rows + k*region is per-task row set k:
 // 0 kick1, 1 drift, 2 sc, 3 R, 4 ext, 5 kick2 (16 doubles per task)
 constexpr int kRowKick1 = 0, kRowDrift = 1, kRowSC = 2, kRowR = 3, kRowExt = 4, kRowKick2 = 5, kRowSets = 6;
-void launch_kick1(const State &s, const Thermo *th, const double *gs, double *rows, hipStream_t st);
+// step: device {k, first absolute step of the run}; gs and obs are the run's
+// deviate and observable buffers, indexed by k in the kernels
+void launch_kick1(const State &s, const Thermo *th, const double *gs, const int *step, double *rows, hipStream_t st);
 void launch_drift(const State &s, bool mixed, void *ph, void *amp, double *rows, hipStream_t st);
-void launch_kick2(const State &s, const Thermo *th, const double *gs, const double *rows1, double *rows2,
-                  hipStream_t st);
-void launch_end(const State &s, Thermo *th, const double *gs, const double *rows, long region, bool has_field,
-                const double *kblk, int maxslot, RepEnergy *E, double *Kin, double *counts, double *obs,
-                int *rebuild, int *srebuild, hipStream_t st);
+void launch_kick2(const State &s, const Thermo *th, const double *gs, const int *step, const double *rows1,
+                  double *rows2, hipStream_t st);
+void launch_end(const State &s, Thermo *th, const double *gs, const int *step, const double *rows, long region,
+                bool has_field, const double *kblk, int maxslot, RepEnergy *E, double *Kin, double *counts,
+                double *obs, int *rebuild, int *srebuild, hipStream_t st);
 // contact-graph components of the selected replica: par[g] = smallest
 // particle of g's component (global index); flags[0] counts particles that
 // moved too far for the reference rows to hold every contact, flags[1] pairs

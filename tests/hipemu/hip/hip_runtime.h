@@ -149,7 +149,16 @@ inline int atomicCAS(int *p, int cmp, int val) {
 typedef int hipError_t;
 typedef void *hipStream_t;
 typedef void *hipEvent_t;
-typedef void *hipGraphExec_t;
+// graphs: launches made during a capture are recorded and replayed in order
+struct hipemu_graph {
+  std::vector<std::function<void()>> ops;
+};
+typedef hipemu_graph *hipGraph_t;
+typedef hipemu_graph *hipGraphExec_t;
+enum hipStreamCaptureMode { hipStreamCaptureModeGlobal, hipStreamCaptureModeThreadLocal, hipStreamCaptureModeRelaxed };
+namespace hipemu {
+inline hipemu_graph *&capturing() { static hipemu_graph *g = nullptr; return g; }
+}  // namespace hipemu
 enum { hipSuccess = 0, hipErrorInvalidValue = 1, hipErrorOutOfMemory = 2 };
 enum hipMemcpyKind { hipMemcpyHostToDevice, hipMemcpyDeviceToHost, hipMemcpyDeviceToDevice, hipMemcpyHostToHost };
 
@@ -158,7 +167,31 @@ inline hipError_t hipSetDevice(int) { return hipSuccess; }
 inline hipError_t hipStreamCreate(hipStream_t *s) { *s = (void *)0x1; return hipSuccess; }
 inline hipError_t hipStreamDestroy(hipStream_t) { return hipSuccess; }
 inline hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
-inline hipError_t hipGraphExecDestroy(hipGraphExec_t) { return hipSuccess; }
+inline hipError_t hipGraphExecDestroy(hipGraphExec_t e) {
+  delete e;
+  return hipSuccess;
+}
+inline hipError_t hipStreamBeginCapture(hipStream_t, hipStreamCaptureMode) {
+  hipemu::capturing() = new hipemu_graph();
+  return hipSuccess;
+}
+inline hipError_t hipStreamEndCapture(hipStream_t, hipGraph_t *g) {
+  *g = hipemu::capturing();
+  hipemu::capturing() = nullptr;
+  return hipSuccess;
+}
+inline hipError_t hipGraphInstantiate(hipGraphExec_t *e, hipGraph_t g, void *, char *, size_t) {
+  *e = new hipemu_graph(*g);
+  return hipSuccess;
+}
+inline hipError_t hipGraphDestroy(hipGraph_t g) {
+  delete g;
+  return hipSuccess;
+}
+inline hipError_t hipGraphLaunch(hipGraphExec_t e, hipStream_t) {
+  for (auto &op : e->ops) op();
+  return hipSuccess;
+}
 inline hipError_t hipMalloc(void **p, size_t n) {
   *p = std::malloc(n ? n : 1);
   return *p ? hipSuccess : hipErrorOutOfMemory;
@@ -185,7 +218,14 @@ inline hipError_t hipDeviceSynchronize() { return hipSuccess; }
 
 namespace hipemu {
 template <typename F, typename... A>
-void launch(F f, dim3 grid, dim3 block, size_t, hipStream_t, A... args) {
+void launch(F f, dim3 grid, dim3 block, size_t shm, hipStream_t st, A... args) {
+  if (capturing()) {
+    hipemu_graph *g = capturing();
+    capturing() = nullptr;  // the recorded launch runs when the graph is replayed
+    g->ops.push_back([=]() { launch(f, grid, block, shm, st, args...); });
+    capturing() = g;
+    return;
+  }
   unsigned nt = block.x * block.y * block.z;
   if (nt == 0 || grid.x == 0 || grid.y == 0 || grid.z == 0) {
     std::fprintf(stderr, "hipemu: empty launch\n");
