@@ -176,6 +176,25 @@ int msc_chunk_repeats(msc_ctx *ctx, long *n);
 int msc_cluster_labels(msc_ctx *ctx, int rep, double min_s, unsigned int *mt_state,
                        int *mt_index, int *labels, int *nclust, long *n_asym);
 
+/* ---- conf.dat (print_conf / chrg_conf, v25/functions.cpp:1181-1296) ----
+ * Host-only reader and writer of the reference's configuration file, so
+ * that states pass between the reference and this library.  Arrays are
+ * particle-major: r, p, l [3N], ori [9N] (rows = body axes), mass [N],
+ * dipole lab position and orientation dpos, dori [3 N nd], moment dmu
+ * [N nd], cluster label clust [N] (NULL writes 0), thermo = {s0, strn0,
+ * s1, strn1} of the two thermostats.  The writer formats as the reference's
+ * stream does (first line default 6 digits, then fixed with 12 decimals), so
+ * a file read and written back is byte-identical.  Readers may pass NULL for
+ * arrays they do not need; MSC_E_STATE reports a file that does not hold N
+ * particles of nd dipoles. */
+int msc_write_conf(const char *path, int N, int nd, const double L[3], const double *r,
+                   const double *p, const double *ori, const double *l, const double *mass,
+                   const double *dpos, const double *dori, const double *dmu, const int *clust,
+                   const double thermo[4]);
+int msc_read_conf(const char *path, int N, int nd, double L[3], double *r, double *p,
+                  double *ori, double *l, double *mass, double *dpos, double *dori, double *dmu,
+                  int *clust, double thermo[4]);
+
 /* ---- measurement -------------------------------------------------------
  * With timing on, the library brackets each launch of the named kernels with
  * HIP events on its stream; msc_kernel_time returns the summed milliseconds

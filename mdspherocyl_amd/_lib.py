@@ -23,7 +23,8 @@ EXPORTS = [
     "msc_get_neigh_list", "msc_check_list", "msc_force_sc", "msc_force_dipR", "msc_force_dipK",
     "msc_ext_field", "msc_self_ener", "msc_zero_forces", "msc_moveX", "msc_moveP", "msc_renorm",
     "msc_init_run", "msc_run", "msc_set_graph", "msc_cluster_labels", "msc_set_timing",
-    "msc_kernel_time", "msc_reset_timing", "msc_work_counts", "msc_chunk_repeats",
+    "msc_kernel_time", "msc_reset_timing", "msc_work_counts", "msc_chunk_repeats", "msc_write_conf",
+    "msc_read_conf",
 ]
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -92,6 +93,8 @@ def load():
         "msc_reset_timing": ([vp], ctypes.c_int),
         "msc_work_counts": ([vp, ctypes.c_int, ctypes.POINTER(ctypes.c_long)], ctypes.c_int),
         "msc_chunk_repeats": ([vp, ctypes.POINTER(ctypes.c_long)], ctypes.c_int),
+        "msc_write_conf": ([ctypes.c_char_p, ctypes.c_int, ctypes.c_int] + [_dp] * 9 + [_ip, _dp], ctypes.c_int),
+        "msc_read_conf": ([ctypes.c_char_p, ctypes.c_int, ctypes.c_int] + [_dp] * 9 + [_ip, _dp], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)

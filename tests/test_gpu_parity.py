@@ -554,3 +554,38 @@ def test_graph_replay_matches_direct_launches(msc, precision):
     assert np.array_equal(oa, ob)
     for x, y in zip(sa0 + sa1, sb0 + sb1):
         assert np.array_equal(x, y)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_conf_dat_round_trip_through_device(msc, precision, tmp_path):
+    """conf.dat of v25/v251/run1 loaded into a replica built from its
+    input.txt (chrg_conf's rules: box line of the file, dipoles rebuilt from
+    the body frame), read back from the device and written by
+    msc_write_conf: byte-identical to the reference's file.  After 20 steps
+    of msc_run the written file reads back to the device state to the 12
+    decimals print_conf keeps."""
+    from mdspherocyl_amd import refio
+    gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    inp = refio.read_input(os.path.join(gold, "v251_run1_input.txt"))
+    conf = os.path.join(gold, "v251_run1_conf.txt")
+    c = refio.read_conf(conf, 500, 1)
+    _say("case: conf.dat round trip v251 %s" % precision)
+    sysm = msc.MdSystem([refio.replica_desc_from_input(inp, box=c["L"])], precision=precision)
+    try:
+        refio.load_conf(sysm, 0, c, inp)
+        out = tmp_path / "conf.dat"
+        refio.write_conf(out, refio.conf_from_system(sysm, 0, inp, c["mass"], c["clust"]))
+        assert out.read_bytes() == open(conf, "rb").read()
+        sysm.init_run()
+        sysm.run(20, observe=False)
+        refio.write_conf(out, refio.conf_from_system(sysm, 0, inp, c["mass"]))
+        c2 = refio.read_conf(out, 500, 1)
+        r, p, l, ori = sysm.get_state(0)
+        assert not np.array_equal(r, c["r"])
+        th = sysm.thermostat(0)
+        th = np.array([th[0][0], th[1][0], th[0][1], th[1][1]])
+        for a, b in ((r, c2["r"]), (p, c2["p"]), (l, c2["l"]), (ori, c2["ori"]), (th, c2["thermo"])):
+            # half a unit of the 12th decimal, plus the parse's own rounding
+            assert np.all(np.abs(a - b) <= 5e-13 + 2 * np.spacing(np.abs(a)))
+    finally:
+        sysm.close()
