@@ -106,6 +106,21 @@ def kernel_table(sysm, names):
     return out
 
 
+def cpu_correl_rate(desc, st, n=4096, bar_w=0.01):
+    """Pairs per second of the oracle's acc_correl (serial) over the first n
+    particles of the workload's state, bars as on the GPU."""
+    from oracle import oracle as O
+    from mdspherocyl_amd import refio
+    S = O.OSys(n, 1, L=tuple(desc.L))
+    S.r[:] = st["r"][:n]
+    S.ori[:] = st["ori"][:n]
+    nb = refio.correl_bars(desc.L[:3], bar_w)
+    h = np.zeros((1, 4, nb))
+    t0 = time.perf_counter()
+    O.lib().o_acc_correl(S.c(), bar_w, nb, O.dptr(h))
+    return n * (n - 1) / 2 / (time.perf_counter() - t0)
+
+
 def cpu_baseline(desc, st, off, nbr, budget_s):
     """The oracle (serial C, FP64) on one force evaluation of the same
     workload: WCA over the same neighbour list, real space over all pairs
@@ -235,6 +250,27 @@ def main():
     log("bench: cluster labels %.3f ms per call (device %.3f ms), %d clusters, largest %d"
         % (1e3 * t_cl, clusters["device_ms"], nclust, clusters["largest"]))
 
+    # pair correlations (acc_correl) of the final state, bars of 0.01 as the
+    # reference's runs use, n_bars = floor(sqrt(3) Lmin / (2 bar_w))
+    from mdspherocyl_amd import refio
+    nb = refio.correl_bars(desc.L[:3], 0.01)
+    mix.pair_correl(0, 0.01, nb)
+    mix.reset_timing()
+    mix.set_timing(True)
+    t0 = time.perf_counter()
+    ncr = 3
+    for _ in range(ncr):
+        hist, dropped = mix.pair_correl(0, 0.01, nb)
+    t_cr = (time.perf_counter() - t0) / ncr
+    mix.set_timing(False)
+    cr_dev = mix.kernel_time("pair_correl")
+    npairs = desc.N * (desc.N - 1) // 2
+    correl = {"ms_per_call": round(1e3 * t_cr, 3), "device_ms": round(cr_dev[0] / max(cr_dev[1], 1), 4),
+              "n_bars": nb, "bar_w": 0.01, "pairs": npairs, "dropped": dropped,
+              "device_pairs_per_s": round(npairs / (cr_dev[0] / max(cr_dev[1], 1) * 1e-3), 1)}
+    log("bench: pair correlations %.3f ms per call (device %.3f ms, %.3e pairs/s), %d bars"
+        % (1e3 * t_cr, correl["device_ms"], correl["device_pairs_per_s"], nb))
+
     k64 = args.fp64_steps or max(args.steps // 2, 10)
     log("bench: FP64, %d warmup + %d timed steps" % (min(args.warmup, 10), k64))
     t_d, _ = timed_run(f64, k64, min(args.warmup, 10), False)
@@ -294,6 +330,8 @@ def main():
                       "fp64": {"F": rel_l2(Fd, Fo), "T": rel_l2(Td, To), "U": rel_abs(Ud, Uo)}}
         log("bench: oracle %.1f s per force evaluation; error vs oracle: mixed F %.2e T %.2e U %.2e, "
             "FP64 F %.2e T %.2e U %.2e" % (t_cpu, *err_oracle["mixed"].values(), *err_oracle["fp64"].values()))
+        correl["oracle_pairs_per_s"] = round(cpu_correl_rate(desc, st), 1)
+        log("bench: oracle acc_correl %.3e pairs/s (serial, first 4096 particles)" % correl["oracle_pairs_per_s"])
 
     line = {
         "metric": "steps/s @N=32768 mixed vs FP64 GPU (force err≤1e-5); batched replica-steps/s",
@@ -318,6 +356,7 @@ def main():
         "err_vs_oracle": err_oracle,
         "batched": batched,
         "cluster_labels": clusters,
+        "pair_correl": correl,
         "kernels_ms_per_launch": {n: round(v[0] / v[1], 4) for n, v in ktab.items()},
         "roofline": roofline,
         "cpu_baseline": cpu,

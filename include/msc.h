@@ -176,6 +176,22 @@ int msc_chunk_repeats(msc_ctx *ctx, long *n);
 int msc_cluster_labels(msc_ctx *ctx, int rep, double min_s, unsigned int *mt_state,
                        int *mt_index, int *labels, int *nclust, long *n_asym);
 
+/* acc_correl (v25/functions.cpp:2399-2466): orientational pair histograms
+ * 000, 110, 112 and 220 of replica `rep` over every pair f < g, binned by the
+ * minimum-image centre distance into bars of width bar_w, ADDED into
+ * hist[(set*4 + channel)*n_bars + bar] (host memory) as the reference adds
+ * into Hista/Histe.  Particles form species of counts[0..nspec-1] in order
+ * (sum = N, nspec <= 8); sets: one per species (pairs within it; weights 2,
+ * 2 dot, 2(3 d_i d_j - dot), 3 dot^2 - 1), then one per species pair m < n in
+ * row-major order (weights 1, dot, 3 d_i d_j - dot, (3 dot^2 - 1)/2).  The
+ * reference sizes n_bars = floor(sqrt(3) Lmin / (2 bar_w)) (v25/main.cpp:251).
+ * *dropped (may be NULL) = pairs at bar n_bars or beyond, not binned (the
+ * reference writes past its array there).  FP64 in every precision mode: the
+ * bars and the counts channel equal a serial evaluation exactly.  nor_correl's
+ * normalisation is host work (mdspherocyl_amd.refio.correlation_text). */
+int msc_pair_correl(msc_ctx *ctx, int rep, int nspec, const int *counts, double bar_w, int n_bars,
+                    double *hist, long *dropped);
+
 /* ---- conf.dat (print_conf / chrg_conf, v25/functions.cpp:1181-1296) ----
  * Host-only reader and writer of the reference's configuration file, so
  * that states pass between the reference and this library.  Arrays are

@@ -100,6 +100,9 @@ struct msc_ctx {
   double *d_dpc = nullptr, *d_doc = nullptr, *d_dmu = nullptr, *d_dpos = nullptr, *d_dori = nullptr, *d_snap = nullptr,
          *d_snap_e = nullptr, *d_snap_u = nullptr;
   int *d_clu = nullptr, *d_clu_flag = nullptr;
+  double *d_corr = nullptr;  // pair-correlation histograms (grown on demand)
+  size_t corr_cap = 0;
+  unsigned long long *d_corr_drop = nullptr;
   double *d_rows = nullptr, *d_kblk = nullptr;  // per-task row sets and k-space block sums of the step loop
   int *d_step = nullptr;  // {step within the run, first absolute step of the run}
   int *d_cell_of = nullptr, *d_cell_cnt = nullptr, *d_cell_start = nullptr, *d_cell_fill = nullptr,
@@ -725,7 +728,7 @@ void msc_destroy(msc_ctx *ctx) {
                   ctx->d_cell_cnt, ctx->d_cell_start, ctx->d_cell_fill, ctx->d_cell_list, ctx->d_nl_cnt,
                   ctx->d_nl_idx, ctx->d_dl_cnt, ctx->d_dl_idx, ctx->d_kval, ctx->d_cell_tmp, ctx->d_sl_cnt,
                   ctx->d_sl_idx, ctx->d_snap_e, ctx->d_srebuild, ctx->d_cell_pack, 
-                  ctx->d_snap_u, ctx->d_clu, ctx->d_clu_flag, ctx->d_rows, ctx->d_kblk, ctx->d_step,
+                  ctx->d_snap_u, ctx->d_clu, ctx->d_clu_flag, ctx->d_corr, ctx->d_corr_drop, ctx->d_rows, ctx->d_kblk, ctx->d_step,
                   ctx->d_tab, ctx->d_ph, ctx->d_amp, ctx->d_part, ctx->d_A, ctx->d_slot_out, ctx->d_task_out,
                   ctx->d_red, ctx->d_M2, ctx->d_Kin, ctx->d_gs, ctx->d_obs, ctx->d_counts, ctx->d_E,
                   ctx->d_th, ctx->d_flags, ctx->d_rebuild, ctx->d_one, ctx->d_run_gs, ctx->d_run_obs, ctx->d_save};
@@ -1547,6 +1550,52 @@ int msc_work_counts(msc_ctx *ctx, int rep, long counts[4]) {
 int msc_chunk_repeats(msc_ctx *ctx, long *n) {
   if (!ctx || !n) return MSC_E_ARG;
   *n = ctx->chunk_repeats;
+  return MSC_OK;
+}
+
+int msc_pair_correl(msc_ctx *ctx, int rep, int nspec, const int *counts, double bar_w, int n_bars, double *hist,
+                    long *dropped) {
+  if (!ctx || rep < 0 || rep >= ctx->nrep || !counts || !hist) return MSC_E_ARG;
+  if (nspec < 1 || nspec > kCorrelMaxSpec) return fail(ctx, MSC_E_ARG, "nspec outside [1, 8]");
+  if (!(bar_w > 0) || n_bars < 1) return fail(ctx, MSC_E_ARG, "bar_w and n_bars must be positive");
+  if (!ctx->have_state) return fail(ctx, MSC_E_STATE, "no state");
+  const RepDev &R = ctx->reps[rep];
+  CorrelArgs a{};
+  a.n0 = R.n0;
+  a.N = R.N;
+  a.nspec = nspec;
+  a.sets = nspec + nspec * (nspec - 1) / 2;
+  long sum = 0;
+  for (int m = 0; m < nspec; ++m) {
+    if (counts[m] < 0) return fail(ctx, MSC_E_ARG, "negative species count");
+    sum += counts[m];
+    a.spec_end[m] = (int)std::min(sum, (long)R.N);
+  }
+  if (sum != R.N) return fail(ctx, MSC_E_ARG, "species counts do not add up to the replica's particles");
+  for (int c = 0; c < 3; ++c) a.L[c] = R.L[c];
+  a.bar_w = bar_w;
+  a.n_bars = n_bars;
+  a.nbr = kCorrelLds / (4 * a.sets);
+  const size_t nh = (size_t)a.sets * 4 * n_bars;
+  if (nh > ctx->corr_cap) {
+    if (int rc_ = dalloc(ctx, &ctx->d_corr, nh)) return rc_;
+    ctx->corr_cap = nh;
+  }
+  if (!ctx->d_corr_drop)
+    if (int rc_ = dalloc(ctx, &ctx->d_corr_drop, 1)) return rc_;
+  HIPC(hipMemsetAsync(ctx->d_corr, 0, nh * sizeof(double), ctx->stream));
+  HIPC(hipMemsetAsync(ctx->d_corr_drop, 0, sizeof(unsigned long long), ctx->stream));
+  if (R.N >= 2) {
+    Bracket b(ctx, "pair_correl");
+    launch_pair_correl(ctx->d_x, ctx->d_ori, ctx->Ntot, a, ctx->d_corr, ctx->d_corr_drop, ctx->stream);
+  }
+  LAUNCHED("pair_correl");
+  std::vector<double> buf(nh);
+  unsigned long long dr = 0;
+  if (int rc_ = d2h(ctx, buf.data(), ctx->d_corr, sizeof(double) * nh)) return rc_;
+  if (int rc_ = d2h(ctx, &dr, ctx->d_corr_drop, sizeof dr)) return rc_;
+  for (size_t i = 0; i < nh; ++i) hist[i] += buf[i];
+  if (dropped) *dropped = (long)dr;
   return MSC_OK;
 }
 

@@ -1787,6 +1787,97 @@ __global__ void k_clust_final(State s, int *__restrict__ par) {
 }
 
 // --------------------------------------------------------------------------
+// pair correlations (acc_correl, v25/functions.cpp:2399-2466)
+// --------------------------------------------------------------------------
+// One block per (tile I <= tile J, bar range): thread t takes particle
+// f = I*kCorrelTile + t against every g of tile J (g > f on the diagonal), the
+// g loop uniform across the block so that g's data come in through scalar
+// loads.  The block's bars [b0, b0 + nb) of every histogram set live in LDS
+// (FP64 adds there), then go to the global histogram with one atomic add per
+// non-zero bar.  The distance, bin and weights are the reference's operations
+// in its order (this file is compiled without contraction), so bins and the
+// counts channel equal a serial evaluation exactly; the other channels differ
+// only in the order of their sums.
+__device__ __forceinline__ int correl_spec(const CorrelArgs &a, int i) {
+  int m = 0;
+  while (i >= a.spec_end[m]) ++m;
+  return m;
+}
+
+__global__ __launch_bounds__(kCorrelTile) void k_pair_correl(const double *__restrict__ x,
+                                                             const double *__restrict__ ori, int Ntot, CorrelArgs a,
+                                                             double *__restrict__ hist,
+                                                             unsigned long long *__restrict__ dropped) {
+  __shared__ double h[kCorrelLds];
+  const int b0 = blockIdx.y * a.nbr;
+  const int nb = min(a.nbr, a.n_bars - b0);
+  const int hn = a.sets * 4 * nb;
+  for (int i = threadIdx.x; i < hn; i += kCorrelTile) h[i] = 0.0;
+  // triangular block index k = J(J+1)/2 + I, 0 <= I <= J
+  const int k = blockIdx.x;
+  int J = (int)((sqrt(8.0 * k + 1.0) - 1.0) * 0.5);
+  while (J * (J + 1) / 2 > k) --J;
+  while ((J + 1) * (J + 2) / 2 <= k) ++J;
+  const int I = k - J * (J + 1) / 2;
+  __syncthreads();
+  const int f = I * kCorrelTile + threadIdx.x;
+  unsigned long long drop = 0;
+  if (f < a.N) {
+    const int gf = a.n0 + f;
+    const double ri0 = x[gf], ri1 = x[Ntot + gf], ri2 = x[2 * Ntot + gf];
+    const double oi0 = ori[6 * Ntot + gf], oi1 = ori[7 * Ntot + gf], oi2 = ori[8 * Ntot + gf];
+    const int m = a.nspec == 1 ? 0 : correl_spec(a, f);
+    const int g_lo = J * kCorrelTile, g_hi = min(a.N, g_lo + kCorrelTile);
+    for (int g = g_lo; g < g_hi; ++g) {
+      if (I == J && g <= f) continue;
+      const int gg = a.n0 + g;
+      double XIJ = ri0 - x[gg];
+      XIJ -= a.L[0] * floor(XIJ / a.L[0] + 0.5);
+      double YIJ = ri1 - x[Ntot + gg];
+      YIJ -= a.L[1] * floor(YIJ / a.L[1] + 0.5);
+      double ZIJ = ri2 - x[2 * Ntot + gg];
+      ZIJ -= a.L[2] * floor(ZIJ / a.L[2] + 0.5);
+      const double RIJ = sqrt(XIJ * XIJ + YIJ * YIJ + ZIJ * ZIJ);
+      const double oj0 = ori[6 * Ntot + gg], oj1 = ori[7 * Ntot + gg], oj2 = ori[8 * Ntot + gg];
+      const double dot = oi0 * oj0 + oi1 * oj1 + oi2 * oj2;
+      const double dotsq = dot * dot;
+      const double doti = (oi0 * XIJ + oi1 * YIJ + oi2 * ZIJ) / RIJ;
+      const double dotj = (oj0 * XIJ + oj1 * YIJ + oj2 * ZIJ) / RIJ;
+      const int bin = (int)floor(RIJ / a.bar_w);
+      if (bin >= a.n_bars) {
+        ++drop;
+        continue;
+      }
+      const int b = bin - b0;
+      if (b < 0 || b >= nb) continue;
+      const int n = a.nspec == 1 ? 0 : correl_spec(a, g);
+      double *hs;
+      if (m == n) {
+        hs = h + m * 4 * nb;
+        atomicAdd(&hs[b], 2.);
+        atomicAdd(&hs[nb + b], 2. * dot);
+        atomicAdd(&hs[2 * nb + b], 2. * (3. * doti * dotj - dot));
+        atomicAdd(&hs[3 * nb + b], 3. * dotsq - 1.);
+      } else {
+        int cc = n - m - 1;
+        for (int q = 0; q < m; ++q) cc += a.nspec - 1 - q;
+        hs = h + (a.nspec + cc) * 4 * nb;
+        atomicAdd(&hs[b], 1.);
+        atomicAdd(&hs[nb + b], dot);
+        atomicAdd(&hs[2 * nb + b], 3. * doti * dotj - dot);
+        atomicAdd(&hs[3 * nb + b], (3. * dotsq - 1.) / 2);
+      }
+    }
+  }
+  if (drop && blockIdx.y == 0) atomicAdd(dropped, drop);
+  __syncthreads();
+  for (int i = threadIdx.x; i < hn; i += kCorrelTile) {
+    const double v = h[i];
+    if (v != 0.0) atomicAdd(&hist[(size_t)(i / nb) * a.n_bars + b0 + i % nb], v);
+  }
+}
+
+// --------------------------------------------------------------------------
 // integrator (v25/functions.cpp:2071-2218, v25/classes.cpp:228-325)
 // --------------------------------------------------------------------------
 __device__ __forceinline__ void mm3(const double *A, const double *B, double *C) {
@@ -2413,6 +2504,12 @@ void launch_step_scalars(const State &s, int mode, const double *red, double *Ki
 }
 void launch_zero_FT(const State &s, hipStream_t st) {
   MSC_LAUNCH(k_zero_FT, dim3(nblk(s.g_end - s.g_begin, 256)), dim3(256), 0, st, s);
+}
+void launch_pair_correl(const double *x, const double *ori, int Ntot, const CorrelArgs &a, double *hist,
+                        unsigned long long *dropped, hipStream_t st) {
+  const int nt = (a.N + kCorrelTile - 1) / kCorrelTile;
+  const dim3 grid((unsigned)(nt * (nt + 1) / 2), (unsigned)((a.n_bars + a.nbr - 1) / a.nbr));
+  MSC_LAUNCH(k_pair_correl, grid, dim3(kCorrelTile), 0, st, x, ori, Ntot, a, hist, dropped);
 }
 void launch_clusters(const State &s, double min_s, double reach_max, int *par, int *flags, hipStream_t st) {
   const dim3 grid(nblk(s.g_end - s.g_begin, 256));

@@ -101,6 +101,23 @@ void launch_end(const State &s, Thermo *th, const double *gs, const int *step, c
 // whose contact test depends on the order of the two particles
 void launch_clusters(const State &s, double min_s, double reach_max, int *par, int *flags, hipStream_t st);
 
+// pair correlations of one replica (acc_correl): histograms of the sets
+// [sets][4][n_bars] added into hist, pairs beyond the last bar counted in
+// *dropped.  Bars go through LDS in ranges of nbr (kCorrelLds doubles per
+// block hold sets*4*nbr of them).
+constexpr int kCorrelTile = 512;          // particles per tile (threads per block)
+constexpr int kCorrelLds = 19456;         // LDS histogram, doubles (152 KiB)
+constexpr int kCorrelMaxSpec = 8;
+struct CorrelArgs {
+  int n0, N, nspec, sets;
+  int spec_end[kCorrelMaxSpec];  // cumulative species counts; spec_end[nspec-1] = N
+  double L[3];
+  double bar_w;
+  int n_bars, nbr;
+};
+void launch_pair_correl(const double *x, const double *ori, int Ntot, const CorrelArgs &a, double *hist,
+                        unsigned long long *dropped, hipStream_t st);
+
 constexpr int kChunk = 64;  // particles per reciprocal structure-factor chunk
 constexpr int kRedStride = 16;  // doubles per replica in the reduction scratch d_red
 constexpr int kRfGroups = 8;    // waves sharing one task in the reciprocal back projection

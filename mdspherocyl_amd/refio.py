@@ -199,3 +199,69 @@ def conf_from_system(sysm, rep, inp, mass, clust=None):
             "mass": np.asarray(mass, dtype=np.float64).reshape(N), "dpos": dpos, "dori": dori, "dmu": dmu,
             "clust": np.zeros(N, dtype=np.int32) if clust is None else np.asarray(clust, dtype=np.int32),
             "thermo": np.array([s[0], strn[0], s[1], strn[1]])}
+
+
+def correl_bars(L, bar_w):
+    """Number of histogram bars the reference allocates (v25/main.cpp:251):
+    floor(sqrt(3) * Lmin / (2 * bar_w))."""
+    return int(math.floor(math.sqrt(3.) * min(L) / (2. * bar_w)))
+
+
+def correl_sets(nspec):
+    """Histogram sets of ``nspec`` species: one per species (pairs within
+    it), then one per species pair m < n in row-major order."""
+    return nspec + nspec * (nspec - 1) // 2
+
+
+def _fmt_line_cor(first, v):
+    # the first value goes out in the stream's default format; setprecision(8)
+    # and fixed stay set for every later line (v25/functions.cpp:2491-2495)
+    return ("%g" % v) if first else ("%.8f" % v)
+
+
+def correlation_text(hist, n_part, Lmin, bar_w, cor_cont):
+    """correlation.dat as nor_correl writes it (v25/functions.cpp:2468-2538)
+    from accumulated histograms ``hist`` [sets][4][n_bars] (sets as
+    correl_sets orders them), the species counts, the smallest box edge, the
+    bar width and cor_cont (the sum of 1/V over accumulated frames)."""
+    NC = len(n_part)
+    hist = np.asarray(hist, dtype=np.float64).reshape(correl_sets(NC), 4, -1)
+    n_bars = hist.shape[2]
+    pd = 4. * math.pi
+    bar3 = bar_w * bar_w * bar_w / 12.
+    sq2 = math.sqrt(2.)
+    out = []
+    for b in range(n_bars):
+        pos = (float(b) + 0.5) * bar_w
+        if pos >= Lmin / 2. and pos <= Lmin / sq2:
+            norm = pd * pos * pos * (3 * Lmin / (2. * pos) - 2.) * bar_w
+        else:
+            norm = pd * (pos * pos * bar_w + bar3)
+        nnorm = norm * cor_cont
+        s = [_fmt_line_cor(b == 0, cor_cont), " %.8f %.4f & " % (norm, pos)]
+        for n in range(NC):
+            NN = n_part[n]
+            NN1 = NN - 1
+            if NN1 > 0:
+                h = hist[n, :, b]
+                s.append("%.8f " % (1. * h[0] / (nnorm * NN * NN1)))
+                s.append("%.8f " % (3. * h[1] / (nnorm * NN * NN1)))
+                s.append("%.8f " % ((3. / 2) * h[2] / (nnorm * NN * NN1)))
+                s.append("%.8f " % (5. * h[3] / (nnorm * NN * NN1)))
+            else:
+                s.append("0 ")
+            s.append("| ")
+        cc = NC
+        for m in range(NC - 1):
+            MM = n_part[m]
+            for n in range(m + 1, NC):
+                NN = n_part[n]
+                h = hist[cc, :, b]
+                s.append("%.8f " % (1. * h[0] / (nnorm * NN * MM)))
+                s.append("%.8f " % (3. * h[1] / (nnorm * NN * MM)))
+                s.append("%.8f " % ((3. / 2) * h[2] / (nnorm * NN * MM)))
+                s.append("%.8f " % (5. * h[3] / (nnorm * NN * MM)))
+                s.append("| ")
+                cc += 1
+        out.append("".join(s) + "\n")
+    return "".join(out)

@@ -205,6 +205,24 @@ class MdSystem:
         self._chk(self.L.msc_chunk_repeats(self._ctx, ctypes.byref(n)))
         return n.value
 
+    def pair_correl(self, rep, bar_w, n_bars, counts=None, hist=None):
+        """acc_correl (v25/functions.cpp:2399-2466): adds this replica's pair
+        histograms into ``hist`` ([sets][4][n_bars], new zeros when None)
+        and returns (hist, dropped); ``counts`` are the species sizes (one
+        species when None)."""
+        N = self.descs[rep].N
+        counts = np.ascontiguousarray([N] if counts is None else counts, dtype=np.int32)
+        nspec = len(counts)
+        sets = nspec + nspec * (nspec - 1) // 2
+        if hist is None:
+            hist = np.zeros((sets, 4, n_bars))
+        if hist.dtype != np.float64 or not hist.flags["C_CONTIGUOUS"] or hist.size != sets * 4 * n_bars:
+            raise ValueError("hist must be a C-contiguous float64 array of %d values" % (sets * 4 * n_bars))
+        dropped = ctypes.c_long(0)
+        self._chk(self.L.msc_pair_correl(self._ctx, rep, nspec, counts.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                         bar_w, n_bars, _d(hist)[1], ctypes.byref(dropped)))
+        return hist, dropped.value
+
     def cluster_labels(self, rep, min_s, mt_state, mt_index):
         """clust_conf (v25/functions.cpp:3270): per-particle cluster numbers.
 

@@ -1113,6 +1113,65 @@ int o_ini_mom(osys *S, double temp, omt *ran) {
 /* clusters: rnd_part / dist_test / clust_gen / clust_conf,                  */
 /* v25/functions.cpp:2950-3148, 3282-3324                                     */
 /* ------------------------------------------------------------------------ */
+/* acc_correl (v25/functions.cpp:2399-2466): orientational pair histograms
+ * 000/110/112/220 over every pair f < g, by minimum-image centre distance.
+ * hist[(set*4 + channel)*n_bars + bin]; sets: species n intra (n < nspec),
+ * then the species pairs m < n in row-major order (the reference's Vc[m].C,
+ * v25/main.cpp:120-142).  Added into hist.  A pair whose bin is n_bars or
+ * beyond is counted in the return value and not binned (the reference
+ * writes past its array there).  Serial, so the sums are in pair order. */
+long o_acc_correl(const osys *S, double bar_w, int n_bars, double *hist) {
+  const int N = S->N, NC = S->nspec;
+  int spec[OSYS_MAXSPEC + 1];
+  spec[0] = 0;
+  for (int n = 0; n < NC; ++n) spec[n + 1] = spec[n] + S->counts[n];
+  long dropped = 0;
+  for (int f = 0; f < N - 1; ++f) {
+    int m = 0;
+    while (f >= spec[m + 1]) ++m;
+    const double *ri = S->r + 3 * f, *oi = S->ori + 9 * f + 6;
+    for (int g = f + 1; g < N; ++g) {
+      int n = m;
+      while (g >= spec[n + 1]) ++n;
+      const double *rj = S->r + 3 * g, *oj = S->ori + 9 * g + 6;
+      double XIJ = ri[0] - rj[0];
+      XIJ -= S->L[0] * floor(XIJ / S->L[0] + 0.5);
+      double YIJ = ri[1] - rj[1];
+      YIJ -= S->L[1] * floor(YIJ / S->L[1] + 0.5);
+      double ZIJ = ri[2] - rj[2];
+      ZIJ -= S->L[2] * floor(ZIJ / S->L[2] + 0.5);
+      double RIJ = sqrt(XIJ * XIJ + YIJ * YIJ + ZIJ * ZIJ);
+      double dot = oi[0] * oj[0] + oi[1] * oj[1] + oi[2] * oj[2];
+      double dotsq = dot * dot;
+      double doti = (oi[0] * XIJ + oi[1] * YIJ + oi[2] * ZIJ) / RIJ;
+      double dotj = (oj[0] * XIJ + oj[1] * YIJ + oj[2] * ZIJ) / RIJ;
+      int bin = (int)floor(RIJ / bar_w);
+      if (bin >= n_bars) {
+        ++dropped;
+        continue;
+      }
+      double *h;
+      if (m == n) {
+        h = hist + (size_t)m * 4 * n_bars;
+        h[bin] += 2.;
+        h[n_bars + bin] += 2. * dot;
+        h[2 * n_bars + bin] += 2. * (3. * doti * dotj - dot);
+        h[3 * n_bars + bin] += 3. * dotsq - 1.;
+      } else {
+        int cc = 0;
+        for (int a = 0; a < m; ++a) cc += NC - 1 - a;
+        cc += n - m - 1;
+        h = hist + (size_t)(NC + cc) * 4 * n_bars;
+        h[bin] += 1.;
+        h[n_bars + bin] += dot;
+        h[2 * n_bars + bin] += 3. * doti * dotj - dot;
+        h[3 * n_bars + bin] += (3. * dotsq - 1.) / 2;
+      }
+    }
+  }
+  return dropped;
+}
+
 int o_dist_test(const osys *S, int A, int B, double min_s) {
   double sigma = (S->dia[A] + S->dia[B]) / 2 + min_s;
   double sig_sq = sigma * sigma, rij[3], st[2], dij[3];

@@ -107,6 +107,9 @@ int    o_ini_mom(osys *s, double temp, omt *ran);
 int    o_dist_test(const osys *s, int A, int B, double min_s);
 int    o_clust_conf(osys *s, double min_s, omt *ran);
 
+/* ---- pair correlations (v25/functions.cpp:2399-2466) ---- */
+long   o_acc_correl(const osys *s, double bar_w, int n_bars, double *hist);
+
 /* ---- whole run, restating v25/main.cpp:401-818 ---- */
 typedef struct orun_params {
   double dens, temp, dt, sr_cut, alpha, min_s;

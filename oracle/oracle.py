@@ -102,6 +102,8 @@ def lib():
         L.o_ini_mom.argtypes = [P(COsys), ctypes.c_double, P(COmt)]
         L.o_dist_test.argtypes = [P(COsys), ctypes.c_int, ctypes.c_int, ctypes.c_double]
         L.o_clust_conf.argtypes = [P(COsys), ctypes.c_double, P(COmt)]
+        L.o_acc_correl.argtypes = [P(COsys), ctypes.c_double, ctypes.c_int, _dp]
+        L.o_acc_correl.restype = ctypes.c_long
         L.o_run_init.argtypes = [P(COsys), P(CRunParams), P(CRun), ctypes.c_int]
         L.o_run_steps.argtypes = [P(COsys), P(CRunParams), P(CRun), ctypes.c_int, ctypes.c_int, _dp]
         L.o_run_free.argtypes = [P(CRun)]

@@ -10,11 +10,15 @@ nowhere else); the outputs are data files committed next to this script:
   v251_run1_iniPos.txt    ini_pos lattice (default stream precision)
   v251_run1_posvel.txt    pos_vel_<n>.dat, n = 0..50, concatenated with
                           "# file n" markers (25 particles every 2 steps)
+  v251_run1_correlation.txt.gz
+                          correlation.dat (pair histograms of steps 0, 10,
+                          ..., 100, normalised), gzip-compressed
   v25_run1_nve.txt        time, K and U columns of v25/run1/output.dat for the
                           two NVE segments used as drift statistics
 
 All of these are outputs the reference itself wrote; nothing here is source.
 """
+import gzip
 import os
 import sys
 
@@ -36,6 +40,10 @@ def main():
     for src, dst in [("conf.dat", "v251_run1_conf.txt"), ("compConf.dat", "v251_run1_compConf.txt"),
                      ("iniPos.dat", "v251_run1_iniPos.txt")]:
         open(os.path.join(HERE, dst), "w").write(open(os.path.join(run, src)).read())
+    with open(os.path.join(run, "correlation.dat"), "rb") as f:
+        data = f.read()
+    with gzip.GzipFile(os.path.join(HERE, "v251_run1_correlation.txt.gz"), "wb", mtime=0) as f:
+        f.write(data)
     with open(os.path.join(HERE, "v251_run1_posvel.txt"), "w") as f:
         for n in range(51):
             f.write("# file %d\n" % n)

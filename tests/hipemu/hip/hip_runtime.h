@@ -122,6 +122,17 @@ inline hipemu_float4 hipemu_mfma16(hipemu_half4 a, hipemu_half4 b, hipemu_float4
 inline void __threadfence() { std::atomic_thread_fence(std::memory_order_seq_cst); }
 
 inline int atomicAdd(int *p, int v) { return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST); }
+inline unsigned long long atomicAdd(unsigned long long *p, unsigned long long v) {
+  return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST);
+}
+inline double atomicAdd(double *p, double v) {
+  double o, n;
+  __atomic_load(p, &o, __ATOMIC_SEQ_CST);
+  do {
+    n = o + v;
+  } while (!__atomic_compare_exchange(p, &o, &n, false, __ATOMIC_SEQ_CST, __ATOMIC_SEQ_CST));
+  return o;
+}
 #ifdef __clang__
 // clang provides the scoped atomic builtins on the host as well
 #ifndef __HIP_MEMORY_SCOPE_AGENT

@@ -589,3 +589,91 @@ def test_conf_dat_round_trip_through_device(msc, precision, tmp_path):
             assert np.all(np.abs(a - b) <= 5e-13 + 2 * np.spacing(np.abs(a)))
     finally:
         sysm.close()
+
+
+def test_pair_correl_v251_correlation_dat(msc):
+    """acc_correl on the GPU at steps 0, 10, ..., 100 of v25/v251/run1 (the
+    oracle replays the run, the library gets each state): the counts channel
+    and the dropped count equal the oracle's exactly, the other channels
+    within 1e-12 of the sum of their terms' magnitudes (the GPU sums in
+    another order), and nor_correl's normalisation of the GPU histograms
+    reproduces the reference's correlation.dat to its 8 printed decimals."""
+    import gzip
+    from mdspherocyl_amd import refio
+    S, P = H.v251_state()
+    R = O.ORun(S, P, seed_init=True)
+    nb = refio.correl_bars(S.L, 0.01)
+    ho = np.zeros((1, 4, nb))
+    hg = np.zeros((1, 4, nb))
+    sysm = msc.MdSystem([H.v251_desc()], precision="fp64")
+    try:
+        for step in range(101):
+            R.run(1)
+            if step % 10:
+                continue
+            _say("case: pair correlations v251 after step %d" % step)
+            assert O.lib().o_acc_correl(S.c(), 0.01, nb, O.dptr(ho)) == 0
+            H.load_osys(sysm, 0, S)
+            _, dropped = sysm.pair_correl(0, 0.01, nb, hist=hg)
+            assert dropped == 0
+    finally:
+        sysm.close()
+        R.free()
+    assert np.array_equal(hg[:, 0], ho[:, 0])
+    npair = ho[0, 0] / 2
+    for c, wmax in ((1, 2.0), (2, 8.0), (3, 2.0)):
+        assert np.all(np.abs(hg[0, c] - ho[0, c]) <= 1e-12 * wmax * npair + 1e-300)
+    V = S.L[0] * S.L[1] * S.L[2]
+    cor = 0.0
+    for _ in range(11):
+        cor += 1. / V
+    mine = refio.correlation_text(hg, [500], S.L.min(), 0.01, cor).splitlines()
+    gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "v251_run1_correlation.txt.gz")
+    ref = gzip.open(gold, "rt").read().splitlines()
+    assert len(mine) == len(ref) == nb
+    same = 0
+    for a, b in zip(mine, ref):
+        assert a.split("&")[0] == b.split("&")[0]
+        va = [float(x) for x in a.split("&")[1].replace("|", " ").split()]
+        vb = [float(x) for x in b.split("&")[1].replace("|", " ").split()]
+        assert np.allclose(va, vb, rtol=0, atol=1.01e-8)
+        same += a == b
+    _say("  identical lines %d of %d" % (same, len(ref)))
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_pair_correl_three_species(msc, precision):
+    """Three species (900, 1100, 500 particles) in a non-cubic box with
+    unwrapped positions up to three boxes out: the six histogram sets, the
+    dropped pairs beyond the last bar and, with bars of a tenth the width,
+    the same histograms split over several LDS bar ranges, against the
+    oracle's acc_correl."""
+    from mdspherocyl_amd import refio, replica_desc
+    rng = np.random.default_rng(17)
+    counts = [900, 1100, 500]
+    N = sum(counts)
+    L = (20.0, 22.0, 18.0)
+    S = O.OSys(N, 1, L=L, counts=counts)
+    S.r[:] = rng.uniform(-3, 4, size=(N, 3)) * np.array(L)
+    for i in range(N):
+        q, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+        S.ori[i] = q
+    S.len[:] = 2.0
+    S.set_inertia()
+    sysm = msc.MdSystem([replica_desc(N, L, 0.5, 2, 5.0, rc=3.0)], precision=precision)
+    try:
+        H.load_osys(sysm, 0, S)
+        for bar_w in (0.05, 0.005):
+            nb = refio.correl_bars(L, bar_w)
+            _say("case: pair correlations 3 species %s bar_w %g n_bars %d" % (precision, bar_w, nb))
+            ho = np.zeros((6, 4, nb))
+            d_o = O.lib().o_acc_correl(S.c(), bar_w, nb, O.dptr(ho))
+            hg, d_g = sysm.pair_correl(0, bar_w, nb, counts=counts)
+            assert d_g == d_o and d_o > 0
+            assert np.array_equal(hg[:, 0], ho[:, 0])
+            # weight 2 per pair within a species, 1 across: every pair is binned or dropped
+            assert hg[:, 0].sum() == N * (N - 1) // 2 - d_o + hg[:3, 0].sum() / 2
+            for c, wmax in ((1, 2.0), (2, 8.0), (3, 2.0)):
+                assert np.all(np.abs(hg[:, c] - ho[:, c]) <= 1e-12 * wmax * np.maximum(ho[:, 0], 1))
+    finally:
+        sysm.close()

@@ -74,10 +74,16 @@ def v251_run():
     R = O.ORun(S, P, seed_init=True)
     posvel = {}
     obs = np.zeros((101, O.OBS_N))
+    nb = int(math.floor(math.sqrt(3.) * S.L.min() / (2. * 0.01)))
+    hist = np.zeros((1, 4, nb))
+    dropped = 0
     for step in range(101):
         obs[step] = R.run(1)[0]
         if step % 2 == 0:
             posvel[step // 2] = [posvel_line(S, i) for i in range(25)]
+        if step % 10 == 0:  # corrf = 10, after the step's clustering (v25/main.cpp:710-716)
+            dropped += O.lib().o_acc_correl(S.c(), 0.01, nb, O.dptr(hist))
+    S.correl = (hist, dropped)
     yield S, P, R, obs, posvel
     R.free()
 
@@ -151,3 +157,35 @@ def test_nve_drift_statistics_of_reference_run():
     E = seg1[:, 2] + seg1[:, 3]
     drift = np.max(np.abs(E - E[0]))
     assert 2e-3 < drift < 4e-3
+
+
+def test_correlation_dat(v251_run):
+    """acc_correl at steps 0, 10, ..., 100 of the replayed run, normalised
+    as nor_correl does: every line of the reference's correlation.dat
+    (6298 bars, 8 decimals), the first columns byte for byte and each
+    histogram value within one unit of its last printed digit (the
+    reference sums with OpenMP atomics in no fixed order)."""
+    import gzip
+    from mdspherocyl_amd import refio
+    S = v251_run[0]
+    hist, dropped = S.correl
+    assert dropped == 0
+    V = S.L[0] * S.L[1] * S.L[2]
+    cor = 0.0
+    for _ in range(11):
+        cor += 1. / V
+    mine = refio.correlation_text(hist, [500], S.L.min(), 0.01, cor).splitlines()
+    ref = gzip.open(os.path.join(GOLD, "v251_run1_correlation.txt.gz"), "rt").read().splitlines()
+    assert len(mine) == len(ref) == 6298
+    same = 0
+    for a, b in zip(mine, ref):
+        ha, ta = a.split("&")
+        hb, tb = b.split("&")
+        assert ha == hb
+        va = [float(x) for x in ta.replace("|", " ").split()]
+        vb = [float(x) for x in tb.replace("|", " ").split()]
+        assert np.allclose(va, vb, rtol=0, atol=1.01e-8), a + "\n" + b
+        same += a == b
+    assert same >= 6290
+    # the counts channel: exact pair counts, 2 per pair and frame
+    assert hist[0, 0].sum() == 2 * 11 * (500 * 499 // 2)
