@@ -1572,8 +1572,12 @@ int msc_pair_correl(msc_ctx *ctx, int rep, int nspec, const int *counts, double 
     a.spec_end[m] = (int)std::min(sum, (long)R.N);
   }
   if (sum != R.N) return fail(ctx, MSC_E_ARG, "species counts do not add up to the replica's particles");
-  for (int c = 0; c < 3; ++c) a.L[c] = R.L[c];
+  for (int c = 0; c < 3; ++c) {
+    a.L[c] = R.L[c];
+    a.invL[c] = 1.0 / R.L[c];
+  }
   a.bar_w = bar_w;
+  a.inv_bw = 1.0 / bar_w;
   a.n_bars = n_bars;
   a.nbr = kCorrelLds / (4 * a.sets);
   const size_t nh = (size_t)a.sets * 4 * n_bars;

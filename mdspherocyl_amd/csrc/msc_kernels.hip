@@ -1795,9 +1795,29 @@ __global__ void k_clust_final(State s, int *__restrict__ par) {
 // loads.  The block's bars [b0, b0 + nb) of every histogram set live in LDS
 // (FP64 adds there), then go to the global histogram with one atomic add per
 // non-zero bar.  The distance, bin and weights are the reference's operations
-// in its order (this file is compiled without contraction), so bins and the
+// in its order (this file is compiled without contraction; the roundings of
+// the minimum image and the bin take a product first and the reference's
+// division where the product could decide differently), so bins and the
 // counts channel equal a serial evaluation exactly; the other channels differ
 // only in the order of their sums.
+// floor(x / d + 0.5) as the reference rounds it, through the product with
+// inv = 1/d: the product and the quotient differ by at most 3 units of the
+// last place of x/d, so unless q + 0.5 lies within 1e-13 (|q| + 1) of an
+// integer both give the same floor, and otherwise the division decides.
+__device__ __forceinline__ double correl_round(double x, double d, double inv) {
+  const double t = x * inv + 0.5;
+  const double fl = floor(t);
+  if (fmin(t - fl, fl + 1.0 - t) > 1e-13 * (fabs(t) + 1.0)) return fl;
+  return floor(x / d + 0.5);
+}
+// floor(x / d) likewise
+__device__ __forceinline__ double correl_floor(double x, double d, double inv) {
+  const double t = x * inv;
+  const double fl = floor(t);
+  if (fmin(t - fl, fl + 1.0 - t) > 1e-13 * (fabs(t) + 1.0)) return fl;
+  return floor(x / d);
+}
+
 __device__ __forceinline__ int correl_spec(const CorrelArgs &a, int i) {
   int m = 0;
   while (i >= a.spec_end[m]) ++m;
@@ -1832,24 +1852,24 @@ __global__ __launch_bounds__(kCorrelTile) void k_pair_correl(const double *__res
       if (I == J && g <= f) continue;
       const int gg = a.n0 + g;
       double XIJ = ri0 - x[gg];
-      XIJ -= a.L[0] * floor(XIJ / a.L[0] + 0.5);
+      XIJ -= a.L[0] * correl_round(XIJ, a.L[0], a.invL[0]);
       double YIJ = ri1 - x[Ntot + gg];
-      YIJ -= a.L[1] * floor(YIJ / a.L[1] + 0.5);
+      YIJ -= a.L[1] * correl_round(YIJ, a.L[1], a.invL[1]);
       double ZIJ = ri2 - x[2 * Ntot + gg];
-      ZIJ -= a.L[2] * floor(ZIJ / a.L[2] + 0.5);
+      ZIJ -= a.L[2] * correl_round(ZIJ, a.L[2], a.invL[2]);
       const double RIJ = sqrt(XIJ * XIJ + YIJ * YIJ + ZIJ * ZIJ);
-      const double oj0 = ori[6 * Ntot + gg], oj1 = ori[7 * Ntot + gg], oj2 = ori[8 * Ntot + gg];
-      const double dot = oi0 * oj0 + oi1 * oj1 + oi2 * oj2;
-      const double dotsq = dot * dot;
-      const double doti = (oi0 * XIJ + oi1 * YIJ + oi2 * ZIJ) / RIJ;
-      const double dotj = (oj0 * XIJ + oj1 * YIJ + oj2 * ZIJ) / RIJ;
-      const int bin = (int)floor(RIJ / a.bar_w);
+      const int bin = (int)correl_floor(RIJ, a.bar_w, a.inv_bw);
       if (bin >= a.n_bars) {
         ++drop;
         continue;
       }
       const int b = bin - b0;
       if (b < 0 || b >= nb) continue;
+      const double oj0 = ori[6 * Ntot + gg], oj1 = ori[7 * Ntot + gg], oj2 = ori[8 * Ntot + gg];
+      const double dot = oi0 * oj0 + oi1 * oj1 + oi2 * oj2;
+      const double dotsq = dot * dot;
+      const double doti = (oi0 * XIJ + oi1 * YIJ + oi2 * ZIJ) / RIJ;
+      const double dotj = (oj0 * XIJ + oj1 * YIJ + oj2 * ZIJ) / RIJ;
       const int n = a.nspec == 1 ? 0 : correl_spec(a, g);
       double *hs;
       if (m == n) {

@@ -111,8 +111,8 @@ constexpr int kCorrelMaxSpec = 8;
 struct CorrelArgs {
   int n0, N, nspec, sets;
   int spec_end[kCorrelMaxSpec];  // cumulative species counts; spec_end[nspec-1] = N
-  double L[3];
-  double bar_w;
+  double L[3], invL[3];  // box and 1 / box
+  double bar_w, inv_bw;
   int n_bars, nbr;
 };
 void launch_pair_correl(const double *x, const double *ori, int Ntot, const CorrelArgs &a, double *hist,

@@ -677,3 +677,45 @@ def test_pair_correl_three_species(msc, precision):
                 assert np.all(np.abs(hg[:, c] - ho[:, c]) <= 1e-12 * wmax * np.maximum(ho[:, 0], 1))
     finally:
         sysm.close()
+
+
+def test_pair_correl_lattice_ties(msc):
+    """A simple cubic lattice (spacing 0.3, 12 per edge, box 12 * 0.3):
+    minimum-image quotients on exact halves and distances whose quotient by
+    the bar width falls on or next to an integer.  Here a product with the
+    reciprocal in place of the reference's divisions picks another image for
+    84816 pair components and another bar for 17564 pairs (bars of 0.1) and
+    4512 pairs (bars of 0.01), counted on the host below.  Counts channel and
+    dropped pairs equal the oracle's."""
+    from mdspherocyl_amd import refio, replica_desc
+    n, sp = 12, 0.3
+    N = n ** 3
+    Lb = n * sp
+    L = (Lb, Lb, Lb)
+    S = O.OSys(N, 1, L=L)
+    k = np.arange(N)
+    S.r[:] = np.stack([(k % n) * sp, (k // n % n) * sp, (k // (n * n)) * sp], axis=1)
+    S.len[:] = 0.1
+    S.dia[:] = 0.1
+    S.set_inertia()
+    # the ties are there: the naive product disagrees with the division
+    d = S.r[:, None, :] - S.r[None, :, :]
+    assert (np.floor(d / Lb + 0.5) != np.floor(d * (1.0 / Lb) + 0.5)).sum() > 0
+    d -= Lb * np.floor(d / Lb + 0.5)
+    R = np.sqrt((d * d).sum(-1))
+    for bar_w in (0.1, 0.01):
+        assert (np.floor(R / bar_w) != np.floor(R * (1.0 / bar_w))).sum() > 0
+    sysm = msc.MdSystem([replica_desc(N, L, 0.5, 2, 1.0, rc=0.5)], precision="fp64")
+    try:
+        H.load_osys(sysm, 0, S)
+        for bar_w in (0.1, 0.01):
+            nb = refio.correl_bars(L, bar_w)
+            _say("case: pair correlations lattice bar_w %g n_bars %d" % (bar_w, nb))
+            ho = np.zeros((1, 4, nb))
+            d_o = O.lib().o_acc_correl(S.c(), bar_w, nb, O.dptr(ho))
+            hg, d_g = sysm.pair_correl(0, bar_w, nb)
+            assert d_g == d_o
+            assert np.array_equal(hg[:, 0], ho[:, 0])
+            assert np.all(np.abs(hg[0, 1:] - ho[0, 1:]) <= 1e-12 * 8 * np.maximum(ho[0, 0], 1))
+    finally:
+        sysm.close()
