@@ -1848,14 +1848,27 @@ __global__ __launch_bounds__(kCorrelTile) void k_pair_correl(const double *__res
     const double oi0 = ori[6 * Ntot + gf], oi1 = ori[7 * Ntot + gf], oi2 = ori[8 * Ntot + gf];
     const int m = a.nspec == 1 ? 0 : correl_spec(a, f);
     const int g_lo = J * kCorrelTile, g_hi = min(a.N, g_lo + kCorrelTile);
+    // g's centre and axis are loaded one iteration ahead
+    int gg = a.n0 + g_lo;
+    double n0 = x[gg], n1 = x[Ntot + gg], n2 = x[2 * Ntot + gg];
+    double m0 = ori[6 * Ntot + gg], m1 = ori[7 * Ntot + gg], m2 = ori[8 * Ntot + gg];
     for (int g = g_lo; g < g_hi; ++g) {
+      const double rj0 = n0, rj1 = n1, rj2 = n2, oj0 = m0, oj1 = m1, oj2 = m2;
+      if (g + 1 < g_hi) {
+        gg = a.n0 + g + 1;
+        n0 = x[gg];
+        n1 = x[Ntot + gg];
+        n2 = x[2 * Ntot + gg];
+        m0 = ori[6 * Ntot + gg];
+        m1 = ori[7 * Ntot + gg];
+        m2 = ori[8 * Ntot + gg];
+      }
       if (I == J && g <= f) continue;
-      const int gg = a.n0 + g;
-      double XIJ = ri0 - x[gg];
+      double XIJ = ri0 - rj0;
       XIJ -= a.L[0] * correl_round(XIJ, a.L[0], a.invL[0]);
-      double YIJ = ri1 - x[Ntot + gg];
+      double YIJ = ri1 - rj1;
       YIJ -= a.L[1] * correl_round(YIJ, a.L[1], a.invL[1]);
-      double ZIJ = ri2 - x[2 * Ntot + gg];
+      double ZIJ = ri2 - rj2;
       ZIJ -= a.L[2] * correl_round(ZIJ, a.L[2], a.invL[2]);
       const double RIJ = sqrt(XIJ * XIJ + YIJ * YIJ + ZIJ * ZIJ);
       const int bin = (int)correl_floor(RIJ, a.bar_w, a.inv_bw);
@@ -1865,7 +1878,6 @@ __global__ __launch_bounds__(kCorrelTile) void k_pair_correl(const double *__res
       }
       const int b = bin - b0;
       if (b < 0 || b >= nb) continue;
-      const double oj0 = ori[6 * Ntot + gg], oj1 = ori[7 * Ntot + gg], oj2 = ori[8 * Ntot + gg];
       const double dot = oi0 * oj0 + oi1 * oj1 + oi2 * oj2;
       const double dotsq = dot * dot;
       const double doti = (oi0 * XIJ + oi1 * YIJ + oi2 * ZIJ) / RIJ;
