@@ -271,6 +271,21 @@ def main():
     log("bench: pair correlations %.3f ms per call (device %.3f ms, %.3e pairs/s), %d bars"
         % (1e3 * t_cr, correl["device_ms"], correl["device_pairs_per_s"], nb))
 
+    # orientational order and total moment (oo_params)
+    mix.order_params(0)
+    mix.reset_timing()
+    mix.set_timing(True)
+    t0 = time.perf_counter()
+    for _ in range(10):
+        G1, G2, Mtot = mix.order_params(0)
+    t_oo = (time.perf_counter() - t0) / 10
+    mix.set_timing(False)
+    oo_dev = mix.kernel_time("order_params")
+    order = {"ms_per_call": round(1e3 * t_oo, 4), "device_ms": round(oo_dev[0] / max(oo_dev[1], 1), 4),
+             "G1": G1, "G2": G2, "M": [float(m) for m in Mtot]}
+    log("bench: order parameters %.4f ms per call (device %.4f ms), G1 %.4f G2 %.4f"
+        % (1e3 * t_oo, order["device_ms"], G1, G2))
+
     k64 = args.fp64_steps or max(args.steps // 2, 10)
     log("bench: FP64, %d warmup + %d timed steps" % (min(args.warmup, 10), k64))
     t_d, _ = timed_run(f64, k64, min(args.warmup, 10), False)
@@ -357,6 +372,7 @@ def main():
         "batched": batched,
         "cluster_labels": clusters,
         "pair_correl": correl,
+        "order_params": order,
         "kernels_ms_per_launch": {n: round(v[0] / v[1], 4) for n, v in ktab.items()},
         "roofline": roofline,
         "cpu_baseline": cpu,

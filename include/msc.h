@@ -176,6 +176,16 @@ int msc_chunk_repeats(msc_ctx *ctx, long *n);
 int msc_cluster_labels(msc_ctx *ctx, int rep, double min_s, unsigned int *mt_state,
                        int *mt_index, int *labels, int *nclust, long *n_asym);
 
+/* oo_params (v25/functions.cpp:2702-2760) with Qmat_3d (3426-3453) and
+ * jacobi (v25/jacobi.c): of replica `rep`, G2 = the largest |eigenvalue| of
+ * Q = sum_i (3 u_i u_i - I) / (2N), G1 = |sum_i Ve . u_i| / N along its unit
+ * eigenvector Ve, and M = the total dipole moment (sum of moment times the
+ * lab-frame dipole axis; overwritten, as the reference's caller zeroes it
+ * before each call).  Sums in FP64 on the device, Q and its eigenvectors in
+ * FP64 on the host: G1 and G2 differ from the reference's single-precision Q
+ * by its rounding (about 1e-6 at N = 500). */
+int msc_order_params(msc_ctx *ctx, int rep, double *G1, double *G2, double M[3]);
+
 /* acc_correl (v25/functions.cpp:2399-2466): orientational pair histograms
  * 000, 110, 112 and 220 of replica `rep` over every pair f < g, binned by the
  * minimum-image centre distance into bars of width bar_w, ADDED into

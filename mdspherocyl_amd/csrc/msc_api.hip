@@ -103,6 +103,7 @@ struct msc_ctx {
   double *d_corr = nullptr;  // pair-correlation histograms (grown on demand)
   size_t corr_cap = 0;
   unsigned long long *d_corr_drop = nullptr;
+  double *d_order = nullptr;  // [kOrderSums] order-parameter sums
   double *d_rows = nullptr, *d_kblk = nullptr;  // per-task row sets and k-space block sums of the step loop
   int *d_step = nullptr;  // {step within the run, first absolute step of the run}
   int *d_cell_of = nullptr, *d_cell_cnt = nullptr, *d_cell_start = nullptr, *d_cell_fill = nullptr,
@@ -728,7 +729,7 @@ void msc_destroy(msc_ctx *ctx) {
                   ctx->d_cell_cnt, ctx->d_cell_start, ctx->d_cell_fill, ctx->d_cell_list, ctx->d_nl_cnt,
                   ctx->d_nl_idx, ctx->d_dl_cnt, ctx->d_dl_idx, ctx->d_kval, ctx->d_cell_tmp, ctx->d_sl_cnt,
                   ctx->d_sl_idx, ctx->d_snap_e, ctx->d_srebuild, ctx->d_cell_pack, 
-                  ctx->d_snap_u, ctx->d_clu, ctx->d_clu_flag, ctx->d_corr, ctx->d_corr_drop, ctx->d_rows, ctx->d_kblk, ctx->d_step,
+                  ctx->d_snap_u, ctx->d_clu, ctx->d_clu_flag, ctx->d_corr, ctx->d_corr_drop, ctx->d_order, ctx->d_rows, ctx->d_kblk, ctx->d_step,
                   ctx->d_tab, ctx->d_ph, ctx->d_amp, ctx->d_part, ctx->d_A, ctx->d_slot_out, ctx->d_task_out,
                   ctx->d_red, ctx->d_M2, ctx->d_Kin, ctx->d_gs, ctx->d_obs, ctx->d_counts, ctx->d_E,
                   ctx->d_th, ctx->d_flags, ctx->d_rebuild, ctx->d_one, ctx->d_run_gs, ctx->d_run_obs, ctx->d_save};
@@ -1550,6 +1551,93 @@ int msc_work_counts(msc_ctx *ctx, int rep, long counts[4]) {
 int msc_chunk_repeats(msc_ctx *ctx, long *n) {
   if (!ctx || !n) return MSC_E_ARG;
   *n = ctx->chunk_repeats;
+  return MSC_OK;
+}
+
+// Eigen-decomposition of a symmetric 3x3 matrix by cyclic Jacobi rotations
+// in double precision (the method of the reference's jacobi routine,
+// v25/jacobi.c:12-80, without its single-precision storage): a is
+// destroyed above the diagonal, d gets the eigenvalues, the columns of v the
+// eigenvectors.
+static void jacobi3_sym(double a[3][3], double d[3], double v[3][3]) {
+  double b[3], z[3];
+  for (int p = 0; p < 3; ++p) {
+    for (int q = 0; q < 3; ++q) v[p][q] = p == q ? 1.0 : 0.0;
+    b[p] = d[p] = a[p][p];
+    z[p] = 0.0;
+  }
+  for (int sweep = 1; sweep <= 50; ++sweep) {
+    const double off = std::fabs(a[0][1]) + std::fabs(a[0][2]) + std::fabs(a[1][2]);
+    if (off == 0.0) return;
+    const double tresh = sweep < 4 ? 0.2 * off / 9 : 0.0;
+    for (int p = 0; p < 2; ++p)
+      for (int q = p + 1; q < 3; ++q) {
+        const double g = 100.0 * std::fabs(a[p][q]);
+        if (sweep > 4 && std::fabs(d[p]) + g == std::fabs(d[p]) && std::fabs(d[q]) + g == std::fabs(d[q])) {
+          a[p][q] = 0.0;
+          continue;
+        }
+        if (!(std::fabs(a[p][q]) > tresh)) continue;
+        const double h = d[q] - d[p];
+        double t;
+        if (std::fabs(h) + g == std::fabs(h)) {
+          t = a[p][q] / h;
+        } else {
+          const double theta = 0.5 * h / a[p][q];
+          t = 1.0 / (std::fabs(theta) + std::sqrt(1.0 + theta * theta));
+          if (theta < 0.0) t = -t;
+        }
+        const double c = 1.0 / std::sqrt(1.0 + t * t), sn = t * c, tau = sn / (1.0 + c), ht = t * a[p][q];
+        z[p] -= ht;
+        z[q] += ht;
+        d[p] -= ht;
+        d[q] += ht;
+        a[p][q] = 0.0;
+        auto rot = [&](double &x, double &y) {
+          const double gx = x, hy = y;
+          x = gx - sn * (hy + gx * tau);
+          y = hy + sn * (gx - hy * tau);
+        };
+        for (int j = 0; j < p; ++j) rot(a[j][p], a[j][q]);
+        for (int j = p + 1; j < q; ++j) rot(a[p][j], a[j][q]);
+        for (int j = q + 1; j < 3; ++j) rot(a[p][j], a[q][j]);
+        for (int j = 0; j < 3; ++j) rot(v[j][p], v[j][q]);
+      }
+    for (int p = 0; p < 3; ++p) {
+      b[p] += z[p];
+      d[p] = b[p];
+      z[p] = 0.0;
+    }
+  }
+}
+
+int msc_order_params(msc_ctx *ctx, int rep, double *G1, double *G2, double M[3]) {
+  if (!ctx || rep < 0 || rep >= ctx->nrep || !G1 || !G2 || !M) return MSC_E_ARG;
+  if (!ctx->have_state) return fail(ctx, MSC_E_STATE, "no state");
+  if (!ctx->d_order)
+    if (int rc_ = dalloc(ctx, &ctx->d_order, kOrderSums)) return rc_;
+  {
+    Bracket b(ctx, "order_params");
+    launch_order_sums(sel(ctx, rep), rep, ctx->d_order, ctx->stream);
+  }
+  LAUNCHED("order_params");
+  double o[kOrderSums];
+  if (int rc_ = d2h(ctx, o, ctx->d_order, sizeof o)) return rc_;
+  const double N = ctx->reps[rep].N;
+  // Q = sum_i (3 u_i u_i - I) / (2N) (Qmat_3d)
+  const double uu[3][3] = {{o[0], o[1], o[2]}, {o[1], o[3], o[4]}, {o[2], o[4], o[5]}};
+  double Q[3][3], d[3], V[3][3];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) Q[i][j] = (3.0 * uu[i][j] - (i == j ? N : 0.0)) / (2.0 * N);
+  jacobi3_sym(Q, d, V);
+  // the reference's choice among equal magnitudes (v25/functions.cpp:2738-2748)
+  const double e1 = std::fabs(d[0]), e2 = std::fabs(d[1]), e3 = std::fabs(d[2]);
+  const int col = (e1 >= e2 && e1 >= e3) ? 0 : (e2 > e1 && e2 >= e3) ? 1 : 2;
+  *G2 = col == 0 ? e1 : col == 1 ? e2 : e3;
+  const double ve = std::sqrt(V[0][col] * V[0][col] + V[1][col] * V[1][col] + V[2][col] * V[2][col]);
+  // sum_i Ve . u_i = Ve . sum_i u_i
+  *G1 = std::fabs((V[0][col] * o[6] + V[1][col] * o[7] + V[2][col] * o[8]) / ve) / N;
+  for (int a = 0; a < 3; ++a) M[a] = o[9 + a];
   return MSC_OK;
 }
 

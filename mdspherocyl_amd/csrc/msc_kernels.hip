@@ -1787,6 +1787,45 @@ __global__ void k_clust_final(State s, int *__restrict__ par) {
 }
 
 // --------------------------------------------------------------------------
+// order-parameter sums (oo_params + Qmat_3d, v25/functions.cpp:2702-2760,
+// 3426-3453): per replica sum u_a u_b (xx xy xz yy yz zz), sum u_a and the
+// total dipole moment sum mu w_a over the lab-frame dipole axes, one block
+// per replica, a fixed-order tree over its threads.
+// --------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_order_sums(State s, int rep, double *__restrict__ out) {
+  __shared__ double sh[kOrderSums][256];
+  const RepDev &R = s.reps[rep];
+  const int N = s.Ntot;
+  double acc[kOrderSums];
+  for (int k = 0; k < kOrderSums; ++k) acc[k] = 0.0;
+  for (int i = threadIdx.x; i < R.N; i += 256) {
+    const int g = R.n0 + i;
+    const double u0 = s.ori[6 * N + g], u1 = s.ori[7 * N + g], u2 = s.ori[8 * N + g];
+    acc[0] += u0 * u0;
+    acc[1] += u0 * u1;
+    acc[2] += u0 * u2;
+    acc[3] += u1 * u1;
+    acc[4] += u1 * u2;
+    acc[5] += u2 * u2;
+    acc[6] += u0;
+    acc[7] += u1;
+    acc[8] += u2;
+    for (int w = 0; w < s.nd; ++w) {
+      const double mu = s.dmu[w * N + g];
+      for (int a = 0; a < 3; ++a) acc[9 + a] += mu * s.dori[(3 * w + a) * N + g];
+    }
+  }
+  for (int k = 0; k < kOrderSums; ++k) sh[k][threadIdx.x] = acc[k];
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h)
+      for (int k = 0; k < kOrderSums; ++k) sh[k][threadIdx.x] += sh[k][threadIdx.x + h];
+    __syncthreads();
+  }
+  if ((int)threadIdx.x < kOrderSums) out[threadIdx.x] = sh[threadIdx.x][0];
+}
+
+// --------------------------------------------------------------------------
 // pair correlations (acc_correl, v25/functions.cpp:2399-2466)
 // --------------------------------------------------------------------------
 // One block per (tile I <= tile J, bar range): thread t takes particle
@@ -2536,6 +2575,9 @@ void launch_step_scalars(const State &s, int mode, const double *red, double *Ki
 }
 void launch_zero_FT(const State &s, hipStream_t st) {
   MSC_LAUNCH(k_zero_FT, dim3(nblk(s.g_end - s.g_begin, 256)), dim3(256), 0, st, s);
+}
+void launch_order_sums(const State &s, int rep, double *out, hipStream_t st) {
+  MSC_LAUNCH(k_order_sums, dim3(1), dim3(256), 0, st, s, rep, out);
 }
 void launch_pair_correl(const double *x, const double *ori, int Ntot, const CorrelArgs &a, double *hist,
                         unsigned long long *dropped, hipStream_t st) {

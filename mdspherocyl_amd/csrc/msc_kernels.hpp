@@ -101,6 +101,11 @@ void launch_end(const State &s, Thermo *th, const double *gs, const int *step, c
 // whose contact test depends on the order of the two particles
 void launch_clusters(const State &s, double min_s, double reach_max, int *par, int *flags, hipStream_t st);
 
+// order-parameter sums of replica rep (oo_params): out[0..5] = sum u_a u_b
+// (xx xy xz yy yz zz), out[6..8] = sum u, out[9..11] = total dipole moment
+constexpr int kOrderSums = 12;
+void launch_order_sums(const State &s, int rep, double *out, hipStream_t st);
+
 // pair correlations of one replica (acc_correl): histograms of the sets
 // [sets][4][n_bars] added into hist, pairs beyond the last bar counted in
 // *dropped.  Bars go through LDS in ranges of nbr (kCorrelLds doubles per

@@ -1172,6 +1172,125 @@ long o_acc_correl(const osys *S, double bar_w, int n_bars, double *hist) {
   return dropped;
 }
 
+/* Jacobi eigenvalue iteration of a symmetric 3x3 matrix: the cyclic method
+ * with the threshold and skip rules of Numerical Recipes in C (2nd ed.,
+ * section 11.1, routine jacobi), as v25/jacobi.c:12-80 runs it in single
+ * precision when compiled as C++ (v25/Makefile:10): fabs and sqrt of a float
+ * stay float, the double constants 0.2, 100.0, 0.5 and 1.0 promote their
+ * expression to double before the float store.  a is overwritten above the
+ * diagonal; d gets the eigenvalues, the columns of v the eigenvectors.
+ * Returns the rotations made, -1 after 50 sweeps. */
+int o_jacobi3(float a[3][3], float d[3], float v[3][3]) {
+  float b[3], z[3];
+  int nrot = 0;
+  for (int p = 0; p < 3; ++p) {
+    for (int q = 0; q < 3; ++q) v[p][q] = 0.0f;
+    v[p][p] = 1.0f;
+    b[p] = d[p] = a[p][p];
+    z[p] = 0.0f;
+  }
+  for (int sweep = 1; sweep <= 50; ++sweep) {
+    float sm = 0.0f;
+    for (int p = 0; p < 2; ++p)
+      for (int q = p + 1; q < 3; ++q) sm += fabsf(a[p][q]);
+    if (sm == 0.0f) return nrot;
+    const float tresh = sweep < 4 ? (float)(0.2 * sm / (3 * 3)) : 0.0f;
+    for (int p = 0; p < 2; ++p) {
+      for (int q = p + 1; q < 3; ++q) {
+        const float g = (float)(100.0 * fabsf(a[p][q]));
+        if (sweep > 4 && (float)(fabsf(d[p]) + g) == fabsf(d[p]) && (float)(fabsf(d[q]) + g) == fabsf(d[q])) {
+          a[p][q] = 0.0f;
+        } else if (fabsf(a[p][q]) > tresh) {
+          float h = d[q] - d[p], t;
+          if ((float)(fabsf(h) + g) == fabsf(h)) {
+            t = a[p][q] / h;
+          } else {
+            const float theta = (float)(0.5 * h / a[p][q]);
+            t = (float)(1.0 / (fabsf(theta) + sqrt(1.0 + theta * theta)));
+            if (theta < 0.0) t = -t;
+          }
+          const float c = (float)(1.0 / sqrtf(1 + t * t));
+          const float s = t * c;
+          const float tau = (float)(s / (1.0 + c));
+          h = t * a[p][q];
+          z[p] -= h;
+          z[q] += h;
+          d[p] -= h;
+          d[q] += h;
+          a[p][q] = 0.0f;
+          /* plane rotation of the pair (x, y): x - s(y + x tau), y + s(x - y tau) */
+#define O_ROT(X, Y)                      \
+  do {                                   \
+    const float gx = (X), hy = (Y);      \
+    (X) = gx - s * (hy + gx * tau);      \
+    (Y) = hy + s * (gx - hy * tau);      \
+  } while (0)
+          for (int j = 0; j < p; ++j) O_ROT(a[j][p], a[j][q]);
+          for (int j = p + 1; j < q; ++j) O_ROT(a[p][j], a[j][q]);
+          for (int j = q + 1; j < 3; ++j) O_ROT(a[p][j], a[q][j]);
+          for (int j = 0; j < 3; ++j) O_ROT(v[j][p], v[j][q]);
+#undef O_ROT
+          ++nrot;
+        }
+      }
+    }
+    for (int p = 0; p < 3; ++p) {
+      b[p] += z[p];
+      d[p] = b[p];
+      z[p] = 0.0f;
+    }
+  }
+  return -1;
+}
+
+/* oo_params + Qmat_3d (v25/functions.cpp:2702-2760, 3426-3453): total
+ * dipole moment M (sum of moment times the stored lab-frame dipole axis),
+ * the nematic order G2 = the largest |eigenvalue| of the single-precision
+ * Q tensor sum_i (3 u_i u_i - I) / (2N), and the polar order G1 = |sum_i
+ * Ve . u_i| / N along its unit eigenvector Ve.  M is overwritten (the
+ * reference's caller zeroes it before every call, v25/main.cpp:723). */
+int o_oo_params(const osys *S, double *G1, double *G2, double M[3]) {
+  const int N = S->N;
+  M[0] = M[1] = M[2] = 0.;
+  for (int i = 0; i < N; ++i)
+    for (int w = 0; w < S->nd; ++w) {
+      const size_t q = (size_t)i * S->nd + w;
+      for (int l = 0; l < 3; ++l) M[l] += S->dmu[q] * S->du[3 * q + l];
+    }
+  float Q[3][3] = {{0}}, d[3], V[3][3];
+  for (int k = 0; k < N; ++k) {
+    const double *o = S->ori + 9 * k + 6;
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j)
+        Q[i][j] += ((float)(3 * o[i] * o[j]) - (float)(i == j ? 1 : 0)) / (2 * N);
+  }
+  const int nrot = o_jacobi3(Q, d, V);
+  const float e1 = fabsf(d[0]), e2 = fabsf(d[1]), e3 = fabsf(d[2]);
+  int col;
+  if (e1 >= e2 && e1 >= e3) {
+    *G2 = (double)e1;
+    col = 0;
+  } else if (e2 > e1 && e2 >= e3) {
+    *G2 = (double)e2;
+    col = 1;
+  } else {
+    *G2 = (double)e3;
+    col = 2;
+  }
+  float Ve[3] = {V[0][col], V[1][col], V[2][col]};
+  const float Vesq = sqrtf(Ve[0] * Ve[0] + Ve[1] * Ve[1] + Ve[2] * Ve[2]);
+  Ve[0] /= Vesq;
+  Ve[1] /= Vesq;
+  Ve[2] /= Vesq;
+  double g1 = 0.;
+  for (int k = 0; k < N; ++k) {
+    const double *o = S->ori + 9 * k + 6;
+    g1 += Ve[0] * o[0] + Ve[1] * o[1] + Ve[2] * o[2];
+  }
+  *G1 = fabs(g1) / N;
+  return nrot;
+}
+
 int o_dist_test(const osys *S, int A, int B, double min_s) {
   double sigma = (S->dia[A] + S->dia[B]) / 2 + min_s;
   double sig_sq = sigma * sigma, rij[3], st[2], dij[3];

@@ -107,6 +107,10 @@ int    o_ini_mom(osys *s, double temp, omt *ran);
 int    o_dist_test(const osys *s, int A, int B, double min_s);
 int    o_clust_conf(osys *s, double min_s, omt *ran);
 
+/* ---- order parameters (v25/functions.cpp:2702-2760, v25/jacobi.c) ---- */
+int    o_jacobi3(float a[3][3], float d[3], float v[3][3]);
+int    o_oo_params(const osys *s, double *G1, double *G2, double M[3]);
+
 /* ---- pair correlations (v25/functions.cpp:2399-2466) ---- */
 long   o_acc_correl(const osys *s, double bar_w, int n_bars, double *hist);
 

@@ -104,6 +104,9 @@ def lib():
         L.o_clust_conf.argtypes = [P(COsys), ctypes.c_double, P(COmt)]
         L.o_acc_correl.argtypes = [P(COsys), ctypes.c_double, ctypes.c_int, _dp]
         L.o_acc_correl.restype = ctypes.c_long
+        _fp = ctypes.POINTER(ctypes.c_float)
+        L.o_jacobi3.argtypes = [_fp, _fp, _fp]
+        L.o_oo_params.argtypes = [P(COsys), _dp, _dp, _dp]
         L.o_run_init.argtypes = [P(COsys), P(CRunParams), P(CRun), ctypes.c_int]
         L.o_run_steps.argtypes = [P(COsys), P(CRunParams), P(CRun), ctypes.c_int, ctypes.c_int, _dp]
         L.o_run_free.argtypes = [P(CRun)]

@@ -13,6 +13,8 @@ nowhere else); the outputs are data files committed next to this script:
   v251_run1_correlation.txt.gz
                           correlation.dat (pair histograms of steps 0, 10,
                           ..., 100, normalised), gzip-compressed
+  v251_run1_magmom.txt    magmom.dat (total dipole moment after every step)
+  v251_run1_counters.txt  counters.dat after step 100 (sums over the steps)
   v25_run1_nve.txt        time, K and U columns of v25/run1/output.dat for the
                           two NVE segments used as drift statistics
 
@@ -38,7 +40,8 @@ def main():
            if l.count("|") == 1 and l[:1].isdigit() and ". " in l][:11]
     open(os.path.join(HERE, "v251_run1_output.txt"), "w").writelines(out)
     for src, dst in [("conf.dat", "v251_run1_conf.txt"), ("compConf.dat", "v251_run1_compConf.txt"),
-                     ("iniPos.dat", "v251_run1_iniPos.txt")]:
+                     ("iniPos.dat", "v251_run1_iniPos.txt"), ("magmom.dat", "v251_run1_magmom.txt"),
+                     ("counters.dat", "v251_run1_counters.txt")]:
         open(os.path.join(HERE, dst), "w").write(open(os.path.join(run, src)).read())
     with open(os.path.join(run, "correlation.dat"), "rb") as f:
         data = f.read()

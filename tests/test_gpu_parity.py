@@ -719,3 +719,87 @@ def test_pair_correl_lattice_ties(msc):
             assert np.all(np.abs(hg[0, 1:] - ho[0, 1:]) <= 1e-12 * 8 * np.maximum(ho[0, 0], 1))
     finally:
         sysm.close()
+
+
+def _order_params_fp64(S):
+    """G1, G2 and M in FP64 with numpy's symmetric eigensolver, choosing
+    among equal magnitudes as oo_params does."""
+    u = S.ori[:, 2, :]
+    N = len(u)
+    Q = (3.0 * u.T @ u - N * np.eye(3)) / (2.0 * N)
+    w, v = np.linalg.eigh(Q)
+    k = int(np.argmax(np.abs(w)))
+    G1 = abs(v[:, k] @ u.sum(0)) / N
+    M = (S.dmu[:, :, None] * S.du).sum((0, 1))
+    return G1, abs(w[k]), M
+
+
+def test_order_params_v251(msc):
+    """oo_params on the GPU at steps 0, 10, ..., 100 of v25/v251/run1: the
+    total moment equals the oracle's within 1e-12 relative; G1 and G2 equal
+    an FP64 eigen-decomposition (numpy) within 1e-12 and the oracle's
+    single-precision Q tensor and Jacobi rotations (the reference's
+    arithmetic) within N * 2^-24, the bound of a serial single-precision sum
+    of N terms of size 1/N (the reference's Q is about 6e-6 off here)."""
+    import ctypes
+    S, P = H.v251_state()
+    R = O.ORun(S, P, seed_init=True)
+    sysm = msc.MdSystem([H.v251_desc()], precision="mixed")
+    try:
+        for step in range(101):
+            R.run(1)
+            if step % 10:
+                continue
+            _say("case: order parameters v251 after step %d" % step)
+            H.load_osys(sysm, 0, S)
+            G1, G2, M = sysm.order_params(0)
+            g1, g2, Mo = ctypes.c_double(), ctypes.c_double(), np.zeros(3)
+            O.lib().o_oo_params(S.c(), ctypes.byref(g1), ctypes.byref(g2), O.dptr(Mo))
+            assert np.abs(M - Mo).max() <= 1e-12 * np.abs(Mo).max()
+            f1, f2, _ = _order_params_fp64(S)
+            assert abs(G1 - f1) <= 1e-12 and abs(G2 - f2) <= 1e-12
+            assert abs(G1 - g1.value) <= 500 * 2.0 ** -24 and abs(G2 - g2.value) <= 500 * 2.0 ** -24
+    finally:
+        sysm.close()
+        R.free()
+
+
+def test_order_params_two_dipoles_nematic(msc):
+    """N=4096 with two dipoles per particle (moments 2 and -0.5, one off
+    the axis) and axes spread about a director (G2 near 0.6): G1, G2 and M
+    against the FP64 eigen-decomposition (1e-12) and the oracle (M 1e-12
+    relative, G1 and G2 within N * 2^-24, the single-precision Q's bound)."""
+    import ctypes
+    from mdspherocyl_amd import replica_desc
+    rng = np.random.default_rng(23)
+    N = 4096
+    L = (40.0, 40.0, 40.0)
+    S = O.OSys(N, 2, L=L)
+    S.r[:] = rng.uniform(0, 40, size=(N, 3))
+    n0 = np.array([1.0, 2.0, 2.0]) / 3.0
+    for i in range(N):
+        u = n0 * np.sign(rng.uniform(-0.3, 1)) + 0.5 * rng.normal(size=3)
+        u /= np.linalg.norm(u)
+        a = np.cross(u, [0.0, 0.0, 1.0] if abs(u[2]) < 0.9 else [1.0, 0.0, 0.0])
+        a /= np.linalg.norm(a)
+        S.ori[i] = np.stack([a, np.cross(u, a), u])
+    S.len[:] = 2.0
+    S.dpc[:, 0], S.dpc[:, 1] = 0.5, -0.5
+    S.doc[:, 1] = (0.6, 0.0, 0.8)
+    S.dmu[:, 0], S.dmu[:, 1] = 2.0, -0.5
+    S.set_inertia()
+    S.set_dip()
+    sysm = msc.MdSystem([replica_desc(N, L, 0.5, 2, 5.0, rc=3.0, nd=2)], precision="fp64")
+    try:
+        _say("case: order parameters N=4096 nd=2")
+        H.load_osys(sysm, 0, S)
+        G1, G2, M = sysm.order_params(0)
+    finally:
+        sysm.close()
+    g1, g2, Mo = ctypes.c_double(), ctypes.c_double(), np.zeros(3)
+    O.lib().o_oo_params(S.c(), ctypes.byref(g1), ctypes.byref(g2), O.dptr(Mo))
+    f1, f2, Mf = _order_params_fp64(S)
+    assert 0.3 < G2 < 0.9 and 0.1 < G1 < 0.9
+    assert abs(G1 - f1) <= 1e-12 and abs(G2 - f2) <= 1e-12
+    assert np.abs(M - Mo).max() <= 1e-12 * np.abs(Mo).max() and np.abs(M - Mf).max() <= 1e-12 * np.abs(Mf).max()
+    assert abs(G1 - g1.value) <= N * 2.0 ** -24 and abs(G2 - g2.value) <= N * 2.0 ** -24

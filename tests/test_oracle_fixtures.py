@@ -77,13 +77,18 @@ def v251_run():
     nb = int(math.floor(math.sqrt(3.) * S.L.min() / (2. * 0.01)))
     hist = np.zeros((1, 4, nb))
     dropped = 0
+    oo = []
     for step in range(101):
         obs[step] = R.run(1)[0]
         if step % 2 == 0:
             posvel[step // 2] = [posvel_line(S, i) for i in range(25)]
         if step % 10 == 0:  # corrf = 10, after the step's clustering (v25/main.cpp:710-716)
             dropped += O.lib().o_acc_correl(S.c(), 0.01, nb, O.dptr(hist))
+        g1, g2, M = ctypes.c_double(), ctypes.c_double(), np.zeros(3)
+        O.lib().o_oo_params(S.c(), ctypes.byref(g1), ctypes.byref(g2), O.dptr(M))
+        oo.append((g1.value, g2.value, M))
     S.correl = (hist, dropped)
+    S.oo = oo
     yield S, P, R, obs, posvel
     R.free()
 
@@ -189,3 +194,60 @@ def test_correlation_dat(v251_run):
     assert same >= 6290
     # the counts channel: exact pair counts, 2 per pair and frame
     assert hist[0, 0].sum() == 2 * 11 * (500 * 499 // 2)
+
+
+def test_magmom_dat(v251_run):
+    """oo_params' total dipole moment after each of the 101 steps equals
+    magmom.dat (tcfv = 1) to its 12 decimals.  M_z is about 2240, where the
+    12th decimal is two units of the last place of a double: there a value
+    may sit one printed unit off (one step of the 101 does)."""
+    S = v251_run[0]
+    lines = open(os.path.join(GOLD, "v251_run1_magmom.txt")).read().splitlines()
+    assert len(lines) == 101 == len(S.oo)
+    exact = 0
+    for k, line in enumerate(lines):
+        ref = line.split()[1:]
+        mine = ["%.12f" % v for v in S.oo[k][2]]
+        exact += mine == ref
+        assert np.all(np.abs(np.array(S.oo[k][2]) - np.array([float(x) for x in ref])) <= 1.01e-12), k
+    assert exact >= 100
+
+
+def test_counters_order_parameters(v251_run):
+    """The sums of G1, G1^2, G2 and G2^2 over the 101 steps (single-precision
+    Q tensor and Jacobi rotations as the reference's build runs them) equal
+    the four order-parameter fields of counters.dat to 12 decimals."""
+    S = v251_run[0]
+    tok = open(os.path.join(GOLD, "v251_run1_counters.txt")).read().split()
+    sums = [0.0] * 4
+    for g1, g2, _ in S.oo:
+        sums[0] += g1
+        sums[1] += g1 * g1
+        sums[2] += g2
+        sums[3] += g2 * g2
+    assert ["%.12f" % v for v in sums] == tok[115:119]
+
+
+def test_jacobi3_closed_forms():
+    """The Jacobi routine on matrices with known spectra: a rotated diagonal
+    matrix gives its eigenvalues and (up to sign) the rotation's columns; a
+    diagonal one needs no rotation; the eigenvectors are orthonormal."""
+    lib = O.lib()
+    fp = ctypes.POINTER(ctypes.c_float)
+    rng = np.random.default_rng(5)
+    for trial in range(20):
+        q, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+        lam = np.array([0.9, -0.35, -0.55]) if trial % 2 else rng.normal(size=3)
+        A = (q * lam) @ q.T
+        a = np.ascontiguousarray(A, dtype=np.float32)
+        d, v = np.zeros(3, np.float32), np.zeros((3, 3), np.float32)
+        nrot = lib.o_jacobi3(a.ctypes.data_as(fp), d.ctypes.data_as(fp), v.ctypes.data_as(fp))
+        assert nrot > 0
+        assert np.allclose(np.sort(d), np.sort(lam), atol=2e-6)
+        assert np.allclose(v.T @ v, np.eye(3), atol=2e-6)
+        for k in range(3):
+            assert np.allclose(A.astype(np.float32) @ v[:, k], d[k] * v[:, k], atol=5e-6)
+    a = np.diag([0.5, -0.25, -0.25]).astype(np.float32)
+    d, v = np.zeros(3, np.float32), np.zeros((3, 3), np.float32)
+    assert lib.o_jacobi3(a.ctypes.data_as(fp), d.ctypes.data_as(fp), v.ctypes.data_as(fp)) == 0
+    assert list(d) == [0.5, -0.25, -0.25] and np.array_equal(v, np.eye(3, dtype=np.float32))
