@@ -176,6 +176,25 @@ int msc_chunk_repeats(msc_ctx *ctx, long *n);
 int msc_cluster_labels(msc_ctx *ctx, int rep, double min_s, unsigned int *mt_state,
                        int *mt_index, int *labels, int *nclust, long *n_asym);
 
+/* The reference's Signal=0 initial state (v25/main.cpp:215-260), host only:
+ * ini_pos's stacked triangular lattice with every axis along z
+ * (v25/functions.cpp:491-676, including its overlap test), nmix GCAmix
+ * pivot-reflection cluster moves (v25/functions.cpp:3328-3409; the
+ * reference makes 1000) and ini_mom's Maxwell momenta with zeroed totals
+ * (v25/functions.cpp:677-1146), all drawn from one MT19937 generator given
+ * as its 624 words and position (the reference's `ran`, gsl_rng_mt19937
+ * seeded 1) and advanced in place.  Inputs: N particles in nspec species of
+ * counts[] (species-major), box L, per-particle len, dia, mass, temperature.
+ * Outputs r[3N], p[3N], l[3N], ori[9N] (row-major, third row the axis), the
+ * arrays msc_set_state takes.  One species only (ini_mom indexes its
+ * per-species scratch by the global particle counter, defined only then):
+ * nspec > 1 is MSC_E_ARG.  MSC_E_OVERLAP: the lattice does not fit the box
+ * or its sites overlap, where the reference stops.  O(N^2) segment distances
+ * per move: meant for the reference's system sizes. */
+int msc_ini_state(int N, int nspec, const int *counts, const double L[3], const double *len,
+                  const double *dia, const double *mass, double temp, int nmix, unsigned int *mt_state,
+                  int *mt_index, double *r, double *p, double *l, double *ori);
+
 /* oo_params (v25/functions.cpp:2702-2760) with Qmat_3d (3426-3453) and
  * jacobi (v25/jacobi.c): of replica `rep`, G2 = the largest |eigenvalue| of
  * Q = sum_i (3 u_i u_i - I) / (2N), G1 = |sum_i Ve . u_i| / N along its unit

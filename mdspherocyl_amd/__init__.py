@@ -6,4 +6,4 @@ routines over that ABI and ``configs`` builds the benchmark systems.
 """
 from ._lib import (EXPORTS, MSC_ALL, MSC_FP64, MSC_MIXED, OBS_NAMES, MscError, ReplicaDesc,  # noqa: F401
                    LIBPATH, load)
-from .system import MdSystem, Overlap, replica_desc  # noqa: F401
+from .system import MdSystem, Overlap, ini_state, mt19937_state, replica_desc  # noqa: F401

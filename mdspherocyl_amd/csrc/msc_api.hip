@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "../../include/msc.h"
+#include "msc_geom.hpp"
 #include "msc_kernels.hpp"
 
 using namespace msc;
@@ -454,6 +455,17 @@ void build_table(const RepDev &R, std::vector<double> &T) {
 // MT19937 of Matsumoto and Nishimura (ACM TOMACS 8, 1998) with GSL's state
 // convention: words mt[] and the position mti of the next word (624: the
 // block is used up); gsl_rng_uniform is the tempered word / 2^32.
+// principal moments of inertia of a spherocylinder, Spherocyl::set_inertia
+// (v25/classes.cpp:175-185), the reference's formula as written
+static void inertia(double len, double dia, double mass, double I[3]) {
+  const double Rr = dia / 2, Rsq = Rr * Rr, Rft = Rsq * Rsq, Lq = len, Lsq = Lq * Lq;
+  const double vol = M_PI * Rsq * (Lq + 3. * Rr / 4);
+  const double mdens = mass / vol;
+  I[0] = mdens * M_PI * (Lsq * Lq * Rsq / 12 + Lsq * Rsq * Rr / 3 + Lq * Rft / 4 + 8. * Rft * Rr / 15);
+  I[1] = I[0];
+  I[2] = mdens * M_PI * (Lq * Rft / 2 + 8. * Rft * Rr / 15);
+}
+
 struct Mt19937 {
   static constexpr int kN = 624, kM = 397;
   uint32_t mt[kN];
@@ -475,6 +487,180 @@ struct Mt19937 {
   }
   double uniform() { return next() / 4294967296.0; }
 };
+
+// --------------------------------------------------------------------------
+// Signal=0 initial state (v25/main.cpp:215-260): lattice, mixing moves,
+// momenta.  Host code: every step depends on the generator's previous draw.
+// --------------------------------------------------------------------------
+namespace init {
+
+struct Parts {
+  int N, nspec;
+  const int *counts;
+  const double *L, *len, *dia;
+  double *r, *ori;
+  int start(int n) const {
+    int b = 0;
+    for (int m = 0; m < n; ++m) b += counts[m];
+    return b;
+  }
+  double dist2(int A, const double rA[3], int B) const {
+    double rAB[3], dAB[3], st[2];
+    for (int a = 0; a < 3; ++a) rAB[a] = wrapd(rA[a] - r[3 * B + a], L[a]);
+    return segdist(rA, ori + 9 * A + 6, len[A], r + 3 * B, ori + 9 * B + 6, len[B], L, rAB, dAB, st);
+  }
+};
+
+// ini_pos (v25/functions.cpp:491-676): species by species, a stacked
+// triangular lattice of layers one diameter-and-length apart, every axis
+// along z, sites of later species skipped where they touch an earlier one;
+// then the reference's overlap test of the whole configuration.
+static int ini_pos(Parts &P) {
+  const double sq = std::sqrt(3.) / 2., sqt = 1. / (2 * std::sqrt(3.)), sqtt = std::sqrt(6.) / 3;
+  const double *L = P.L;
+  for (int n = 0; n < P.nspec; ++n) {
+    const int b0 = P.start(n), NN = P.counts[n];
+    const double dia = P.dia[b0], len = P.len[b0];
+    const double m1 = std::floor(L[0] / dia), m2 = std::floor(L[1] / (dia * sq)),
+                 m3 = std::floor(L[2] / (dia * sqtt + len));
+    const int num1 = (int)(m1 < 1. ? 1. : m1), num2 = (int)(m2 < 1. ? 1. : m2), num3 = (int)(m3 < 1. ? 1. : m3);
+    int cont = 0;
+    for (int pass = 0; cont < NN; ++pass) {
+      if (pass == 2) return 1;  // the lattice cannot hold the species
+      for (int nz = 0; nz < num3; ++nz)
+        for (int ny = 0; ny < num2; ++ny)
+          for (int nx = 0; nx < num1 && cont < NN; ++nx) {
+            double q[3];
+            const bool even_z = std::fmod((double)nz, 2.) == 0, even_y = std::fmod((double)ny, 2.) == 0;
+            const bool shift = even_z ? !even_y : even_y;
+            q[0] = shift ? dia * ((double)nx + 0.5) * 1.0001 : dia * (double)nx * 1.0001;
+            q[1] = even_z ? dia * (double)ny * sq * 1.0001 : dia * ((double)ny * sq + sqt) * 1.0001;
+            q[2] = (dia * sqtt + len) * (double)nz * 1.0001;
+            for (int a = 0; a < 3; ++a) q[a] -= L[a] * std::floor(q[a] / L[a] + 0.5);
+            const int i = b0 + cont;
+            for (int a = 0; a < 3; ++a) P.r[3 * i + a] = q[a];
+            for (int k = 0; k < 9; ++k) P.ori[9 * i + k] = (k % 4 == 0) ? 1.0 : 0.0;
+            bool clash = false;
+            for (int m = 0; m < n && !clash; ++m) {
+              const int c0 = P.start(m);
+              const double dd = (dia + P.dia[c0]) / 2, rsq = dd * dd;
+              for (int j = c0; j < c0 + P.counts[m] && !clash; ++j) clash = P.dist2(i, q, j) < rsq;
+            }
+            if (!clash) ++cont;
+          }
+    }
+  }
+  // overlap test (v25/functions.cpp:611-673): within and across species
+  for (int m = 0; m < P.nspec; ++m)
+    for (int n = m; n < P.nspec; ++n) {
+      const int a0 = P.start(m), b0 = P.start(n);
+      const double dd = (P.dia[a0] + P.dia[b0]) / 2, rsq = dd * dd;
+      for (int i = a0; i < a0 + P.counts[m]; ++i)
+        for (int j = (m == n ? i + 1 : b0); j < b0 + P.counts[n]; ++j)
+          if (P.dist2(i, P.r + 3 * i, j) < rsq) return 1;
+    }
+  return 0;
+}
+
+// GCAmix (v25/functions.cpp:3328-3409): a random pivot; a random seed
+// particle and, breadth first, every particle that touches one already moved
+// are reflected through the pivot (minimum image, wrapped into the box).
+static void gca_move(Parts &P, Mt19937 &mt, std::vector<char> &added, std::vector<int> &queue) {
+  const double *L = P.L;
+  double pivot[3];
+  for (int a = 0; a < 3; ++a) pivot[a] = (2 * mt.uniform() - 1) * L[a] / 2;
+  const int seed = (int)std::floor(mt.uniform() * P.N);  // species-major order is global order
+  std::fill(added.begin(), added.end(), 0);
+  int head = 0, tail = 0;
+  added[seed] = 1;
+  queue[tail++] = seed;
+  while (head < tail) {
+    const int q = queue[head++];
+    double ri[3];
+    for (int a = 0; a < 3; ++a) {
+      double d = pivot[a] - P.r[3 * q + a];
+      d -= L[a] * std::floor(d / L[a] + 0.5);
+      ri[a] = P.r[3 * q + a] + 2 * d;
+      ri[a] -= L[a] * std::floor(ri[a] / L[a] + 0.5);
+      P.r[3 * q + a] = ri[a];
+    }
+    int qs = 0;
+    for (int n = 0, b = 0; n < P.nspec; b += P.counts[n], ++n)
+      if (q >= b) qs = b;
+    for (int n = 0, b = 0; n < P.nspec; b += P.counts[n], ++n) {
+      const double dd = (P.dia[qs] + P.dia[b]) / 2, rsq = dd * dd;
+      for (int i = b; i < b + P.counts[n]; ++i)
+        if (!added[i] && P.dist2(q, ri, i) < rsq) {
+          added[i] = 1;
+          queue[tail++] = i;
+        }
+    }
+  }
+}
+
+// Box-Muller pair (gauss, v25/functions.cpp:1149-1157) with the given variance
+static void gauss_pair(double variance, double &u1, double &u2) {
+  const double a = std::sqrt(-2 * std::log(u1)) * std::cos(2 * M_PI * u2);
+  const double b = std::sqrt(-2 * std::log(u1)) * std::sin(2 * M_PI * u2);
+  const double sv = std::sqrt(variance);
+  u1 = a * sv;
+  u2 = b * sv;
+}
+
+// The reference's zeroing of a momentum component (v25/functions.cpp:
+// 728-770): flip signs, walking cyclically over all but the last particle,
+// until |sum| <= pr, then let the last particle cancel the rest.
+static void zero_sum(std::vector<double> &v, int n, double pr) {
+  double S = 0.;
+  for (int i = 0; i < n; ++i) S += v[i];
+  int k = 0;
+  for (long guard = 0; std::fabs(S) > pr && guard <= 100000000L; ++guard) {
+    while (S > pr) {
+      if (v[k] > 0) {
+        v[k] = -v[k];
+        S += 2 * v[k];
+      }
+      if (++k == n - 1) k = 0;
+    }
+    while (S < -pr) {
+      if (v[k] < 0) {
+        v[k] = -v[k];
+        S += 2 * v[k];
+      }
+      if (++k == n - 1) k = 0;
+    }
+  }
+  S -= v[n - 1];
+  v[n - 1] = -S;
+}
+
+// ini_mom (v25/functions.cpp:677-1146), one species: uniforms for every
+// component, Gaussian pairs of variance kT m (kT I_a for the angular
+// momenta; an odd last particle draws one more uniform), zero totals.
+static void ini_mom(int N, double temp, double mass, const double I[3], Mt19937 &mt, double *p, double *l) {
+  std::vector<double> c[3];
+  for (int pass = 0; pass < 2; ++pass) {
+    for (auto &v : c) v.assign((size_t)N + 1, 0.0);
+    for (int i = 0; i < N; ++i)
+      for (int j = 0; j < 3; ++j) c[j][i] = mt.uniform();
+    for (int i = 0; i < N; i += 2)
+      for (int j = 0; j < 3; ++j) {
+        const double var = pass == 0 ? temp * mass : temp * I[j];
+        if ((std::fmod((float)N, 2.f) != 0.) && i == N - 1) {
+          double u = mt.uniform();
+          gauss_pair(var, u, c[j][i]);
+        } else {
+          gauss_pair(var, c[j][i], c[j][i + 1]);
+        }
+      }
+    for (int j = 0; j < 3; ++j) zero_sum(c[j], N, 0.1);
+    double *out = pass == 0 ? p : l;
+    for (int i = 0; i < N; ++i)
+      for (int j = 0; j < 3; ++j) out[3 * i + j] = c[j][i];
+  }
+}
+
+}  // namespace init
 
 // Cluster numbering of clust_conf (v25/functions.cpp:3270-3301): while
 // particles are left, rnd_part (v25/functions.cpp:2950-2970) draws
@@ -769,16 +955,8 @@ int msc_set_state(msc_ctx *ctx, int rep, const double *r, const double *p, const
   // dipole constraints: host layout [N][nd], [N][nd][3] -> [nd][N], [3nd][N]
   if ((rc = put(ctx->d_dpc, dpc, nd)) || (rc = put(ctx->d_dmu, dmu, nd)) || (rc = put(ctx->d_doc, doc, 3 * nd)))
     return rc;
-  // inertia, Spherocyl::set_inertia (v25/classes.cpp:175-185)
   std::vector<double> I((size_t)3 * n);
-  for (int i = 0; i < n; ++i) {
-    double Rr = dia[i] / 2, Rsq = Rr * Rr, Rft = Rsq * Rsq, Lq = len[i], Lsq = Lq * Lq;
-    double vol = M_PI * Rsq * (Lq + 3. * Rr / 4);
-    double mdens = mass[i] / vol;
-    I[3 * i + 0] = mdens * M_PI * (Lsq * Lq * Rsq / 12 + Lsq * Rsq * Rr / 3 + Lq * Rft / 4 + 8. * Rft * Rr / 15);
-    I[3 * i + 1] = I[3 * i + 0];
-    I[3 * i + 2] = mdens * M_PI * (Lq * Rft / 2 + 8. * Rft * Rr / 15);
-  }
+  for (int i = 0; i < n; ++i) inertia(len[i], dia[i], mass[i], &I[3 * (size_t)i]);
   if ((rc = put(ctx->d_inert, I.data(), 3))) return rc;
   derive(ctx, rep, len, dia, dpc);
   upload_reps(ctx);
@@ -1638,6 +1816,38 @@ int msc_order_params(msc_ctx *ctx, int rep, double *G1, double *G2, double M[3])
   // sum_i Ve . u_i = Ve . sum_i u_i
   *G1 = std::fabs((V[0][col] * o[6] + V[1][col] * o[7] + V[2][col] * o[8]) / ve) / N;
   for (int a = 0; a < 3; ++a) M[a] = o[9 + a];
+  return MSC_OK;
+}
+
+int msc_ini_state(int N, int nspec, const int *counts, const double L[3], const double *len, const double *dia,
+                  const double *mass, double temp, int nmix, unsigned int *mt_state, int *mt_index, double *r,
+                  double *p, double *l, double *ori) {
+  if (N < 1 || nspec < 1 || nspec > 8 || !counts || !L || !len || !dia || !mass || nmix < 0 || !mt_state ||
+      !mt_index || !r || !p || !l || !ori)
+    return MSC_E_ARG;
+  if (*mt_index < 0 || *mt_index > Mt19937::kN) return MSC_E_ARG;
+  long sum = 0;
+  for (int n = 0; n < nspec; ++n) {
+    if (counts[n] < 1) return MSC_E_ARG;
+    sum += counts[n];
+  }
+  if (sum != N || !(L[0] > 0 && L[1] > 0 && L[2] > 0) || !(temp >= 0)) return MSC_E_ARG;
+  // ini_mom's scratch is per species but indexed by the global particle
+  // counter (v25/functions.cpp:721, 954): only one species is defined
+  if (nspec != 1) return MSC_E_ARG;
+  Mt19937 mt;
+  std::copy(mt_state, mt_state + Mt19937::kN, mt.mt);
+  mt.mti = *mt_index;
+  init::Parts P{N, nspec, counts, L, len, dia, r, ori};
+  if (init::ini_pos(P)) return MSC_E_OVERLAP;
+  std::vector<char> added(N);
+  std::vector<int> queue(N);
+  for (int k = 0; k < nmix; ++k) init::gca_move(P, mt, added, queue);
+  double I[3];
+  inertia(len[0], dia[0], mass[0], I);  // the first particle's, as ini_mom reads it
+  init::ini_mom(N, temp, mass[0], I, mt, p, l);
+  std::copy(mt.mt, mt.mt + Mt19937::kN, mt_state);
+  *mt_index = mt.mti;
   return MSC_OK;
 }
 

@@ -254,3 +254,39 @@ class MdSystem:
         c = (ctypes.c_long * 4)()
         self._chk(self.L.msc_work_counts(self._ctx, rep, c))
         return list(c)
+
+
+def mt19937_state(seed):
+    """MT19937 words after init_genrand(seed) and position 624, the state
+    gsl_rng_set(ran, seed) leaves (the reference seeds ``ran`` with 1)."""
+    mt = np.zeros(624, dtype=np.uint32)
+    mt[0] = seed
+    for i in range(1, 624):
+        prev = int(mt[i - 1])
+        mt[i] = (1812433253 * (prev ^ (prev >> 30)) + i) & 0xffffffff
+    return mt, 624
+
+
+def ini_state(N, L, length, dia, mass, temp, nmix=1000, mt=None, mt_index=624):
+    """The reference's Signal=0 initial state (ini_pos, ``nmix`` GCAmix moves,
+    ini_mom; v25/main.cpp:215-260) of N particles of one species, drawn from
+    the generator state ``mt``/``mt_index`` (default: seeded 1, as the
+    reference's ``ran``).  Returns a dict of r, p, l, ori (as set_state takes
+    them) and the advanced generator (``mt``, ``mt_index``)."""
+    lib = _lib.load()
+    if mt is None:
+        mt, mt_index = mt19937_state(1)
+    mt = np.ascontiguousarray(mt, dtype=np.uint32).copy()
+    idx = ctypes.c_int(mt_index)
+    counts = np.array([N], dtype=np.int32)
+    Lb = np.ascontiguousarray(L, dtype=np.float64)
+    arr = [np.ascontiguousarray(np.broadcast_to(v, (N,)), dtype=np.float64) for v in (length, dia, mass)]
+    r, p, l, ori = np.zeros((N, 3)), np.zeros((N, 3)), np.zeros((N, 3)), np.zeros((N, 3, 3))
+    rc = lib.msc_ini_state(N, 1, counts.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), _d(Lb)[1],
+                           *[_d(a)[1] for a in arr], temp, nmix, mt.ctypes.data_as(ctypes.POINTER(ctypes.c_uint)),
+                           ctypes.byref(idx), *[_d(a)[1] for a in (r, p, l, ori)])
+    if rc == MSC_E_OVERLAP:
+        raise Overlap(rc, "msc_ini_state: the lattice does not fit the box or its sites overlap")
+    if rc:
+        raise MscError(rc, "msc_ini_state failed")
+    return {"r": r, "p": p, "l": l, "ori": ori, "mt": mt, "mt_index": idx.value}

@@ -803,3 +803,29 @@ def test_order_params_two_dipoles_nematic(msc):
     assert abs(G1 - f1) <= 1e-12 and abs(G2 - f2) <= 1e-12
     assert np.abs(M - Mo).max() <= 1e-12 * np.abs(Mo).max() and np.abs(M - Mf).max() <= 1e-12 * np.abs(Mf).max()
     assert abs(G1 - g1.value) <= N * 2.0 ** -24 and abs(G2 - g2.value) <= N * 2.0 ** -24
+
+
+def test_reference_run_from_library_initial_state(msc):
+    """v25/v251/run1 end to end without the oracle: the library's Signal=0
+    initial state (msc_ini_state: lattice, 1000 GCAmix moves, ini_mom from
+    the generator seeded 1), loaded with the run's dipoles, then 101 steps of
+    msc_run (FP64): every printed observable of the reference's output.dat
+    within 1e-9 relative."""
+    import math
+    L = math.pow(500.0 / ((1.0 * 1.0) * 0.0013), 1.0 / 3)
+    st = msc.ini_state(500, (L, L, L), 9.0, 1.0, 1.0, 1.0)
+    _say("case: v251 run from msc_ini_state, 101 steps fp64")
+    sysm = msc.MdSystem([H.v251_desc()], precision="fp64")
+    try:
+        N = 500
+        sysm.set_state(0, st["r"], st["p"], st["l"], st["ori"], np.full(N, 9.0), np.ones(N), np.ones(N),
+                       np.zeros((N, 1)), np.tile([0.0, 0.0, 1.0], (N, 1, 1)), np.full((N, 1), 4.481))
+        sysm.init_run()
+        obs = sysm.run(101)
+    finally:
+        sysm.close()
+    ref = _fixture_lines()
+    for k, row in enumerate(ref):
+        mine = obs[10 * k, 0, :11]
+        for j in range(11):
+            assert mine[j] == pytest.approx(row[j], rel=1e-9, abs=2e-12), "step %d column %d" % (10 * k, j)
