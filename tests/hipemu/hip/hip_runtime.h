@@ -122,6 +122,7 @@ inline hipemu_float4 hipemu_mfma16(hipemu_half4 a, hipemu_half4 b, hipemu_float4
 inline void __threadfence() { std::atomic_thread_fence(std::memory_order_seq_cst); }
 
 inline int atomicAdd(int *p, int v) { return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST); }
+inline unsigned atomicAdd(unsigned *p, unsigned v) { return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST); }
 inline unsigned long long atomicAdd(unsigned long long *p, unsigned long long v) {
   return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST);
 }
