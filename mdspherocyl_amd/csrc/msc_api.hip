@@ -1877,14 +1877,7 @@ int msc_pair_correl(msc_ctx *ctx, int rep, int nspec, const int *counts, double 
   a.bar_w = bar_w;
   a.inv_bw = 1.0 / bar_w;
   a.n_bars = n_bars;
-  // bars per LDS window: three doubles and a 32-bit count per bar and set
-  a.nbr = (int)((sizeof(double) * kCorrelLds) / ((3 * sizeof(double) + sizeof(unsigned)) * a.sets));
-  // one window suffices when the bars past it lie beyond 0.7 of the smallest
-  // box edge: only the pairs near the box corners (a few percent) are
-  // further, and those go to the global histogram directly
-  const double Lmin = std::min(R.L[0], std::min(R.L[1], R.L[2]));
-  a.overflow = (a.nbr < n_bars && a.nbr * bar_w >= 0.7 * Lmin) ? 1 : 0;
-  if (a.overflow) a.nbr = std::min(a.nbr, n_bars);
+  a.nbr = kCorrelLds / (4 * a.sets);
   const size_t nh = (size_t)a.sets * 4 * n_bars;
   if (nh > ctx->corr_cap) {
     if (int rc_ = dalloc(ctx, &ctx->d_corr, nh)) return rc_;

@@ -1867,15 +1867,11 @@ __global__ __launch_bounds__(kCorrelTile) void k_pair_correl(const double *__res
                                                              const double *__restrict__ ori, int Ntot, CorrelArgs a,
                                                              double *__restrict__ hist,
                                                              unsigned long long *__restrict__ dropped) {
-  // LDS: channels 110, 112, 220 as doubles [set][3][nb], then the 000 counts
-  // as 32-bit integers [set][nb] (at most 2 * kCorrelTile^2 per bar)
   __shared__ double h[kCorrelLds];
   const int b0 = blockIdx.y * a.nbr;
   const int nb = min(a.nbr, a.n_bars - b0);
-  const int hd = a.sets * 3 * nb, hc = a.sets * nb;
-  unsigned *cnt = reinterpret_cast<unsigned *>(h + hd);
-  for (int i = threadIdx.x; i < hd; i += kCorrelTile) h[i] = 0.0;
-  for (int i = threadIdx.x; i < hc; i += kCorrelTile) cnt[i] = 0u;
+  const int hn = a.sets * 4 * nb;
+  for (int i = threadIdx.x; i < hn; i += kCorrelTile) h[i] = 0.0;
   // triangular block index k = J(J+1)/2 + I, 0 <= I <= J
   const int k = blockIdx.x;
   int J = (int)((sqrt(8.0 * k + 1.0) - 1.0) * 0.5);
@@ -1920,58 +1916,35 @@ __global__ __launch_bounds__(kCorrelTile) void k_pair_correl(const double *__res
         continue;
       }
       const int b = bin - b0;
-      const bool in_lds = b >= 0 && b < nb;
-      if (!in_lds && !a.overflow) continue;
+      if (b < 0 || b >= nb) continue;
       const double dot = oi0 * oj0 + oi1 * oj1 + oi2 * oj2;
       const double dotsq = dot * dot;
       const double doti = (oi0 * XIJ + oi1 * YIJ + oi2 * ZIJ) / RIJ;
       const double dotj = (oj0 * XIJ + oj1 * YIJ + oj2 * ZIJ) / RIJ;
       const int n = a.nspec == 1 ? 0 : correl_spec(a, g);
-      int set;
-      unsigned w0;
-      double w1, w2, w3;
+      double *hs;
       if (m == n) {
-        set = m;
-        w0 = 2u;
-        w1 = 2. * dot;
-        w2 = 2. * (3. * doti * dotj - dot);
-        w3 = 3. * dotsq - 1.;
+        hs = h + m * 4 * nb;
+        atomicAdd(&hs[b], 2.);
+        atomicAdd(&hs[nb + b], 2. * dot);
+        atomicAdd(&hs[2 * nb + b], 2. * (3. * doti * dotj - dot));
+        atomicAdd(&hs[3 * nb + b], 3. * dotsq - 1.);
       } else {
         int cc = n - m - 1;
         for (int q = 0; q < m; ++q) cc += a.nspec - 1 - q;
-        set = a.nspec + cc;
-        w0 = 1u;
-        w1 = dot;
-        w2 = 3. * doti * dotj - dot;
-        w3 = (3. * dotsq - 1.) / 2;
-      }
-      if (in_lds) {
-        atomicAdd(&cnt[set * nb + b], w0);
-        double *hs = h + set * 3 * nb + b;
-        atomicAdd(&hs[0], w1);
-        atomicAdd(&hs[nb], w2);
-        atomicAdd(&hs[2 * nb], w3);
-      } else {
-        // a bar beyond the block's LDS window (the few pairs near the box
-        // corners): straight to the global histogram
-        double *hg = hist + (size_t)set * 4 * a.n_bars + bin;
-        atomicAdd(&hg[0], (double)w0);
-        atomicAdd(&hg[a.n_bars], w1);
-        atomicAdd(&hg[2 * (size_t)a.n_bars], w2);
-        atomicAdd(&hg[3 * (size_t)a.n_bars], w3);
+        hs = h + (a.nspec + cc) * 4 * nb;
+        atomicAdd(&hs[b], 1.);
+        atomicAdd(&hs[nb + b], dot);
+        atomicAdd(&hs[2 * nb + b], 3. * doti * dotj - dot);
+        atomicAdd(&hs[3 * nb + b], (3. * dotsq - 1.) / 2);
       }
     }
   }
   if (drop && blockIdx.y == 0) atomicAdd(dropped, drop);
   __syncthreads();
-  for (int i = threadIdx.x; i < hd; i += kCorrelTile) {
+  for (int i = threadIdx.x; i < hn; i += kCorrelTile) {
     const double v = h[i];
-    const int set = i / (3 * nb), ch = 1 + (i / nb) % 3;
-    if (v != 0.0) atomicAdd(&hist[(size_t)(set * 4 + ch) * a.n_bars + b0 + i % nb], v);
-  }
-  for (int i = threadIdx.x; i < hc; i += kCorrelTile) {
-    const unsigned c = cnt[i];
-    if (c) atomicAdd(&hist[(size_t)(i / nb) * 4 * a.n_bars + b0 + i % nb], (double)c);
+    if (v != 0.0) atomicAdd(&hist[(size_t)(i / nb) * a.n_bars + b0 + i % nb], v);
   }
 }
 
@@ -2609,8 +2582,7 @@ void launch_order_sums(const State &s, int rep, double *out, hipStream_t st) {
 void launch_pair_correl(const double *x, const double *ori, int Ntot, const CorrelArgs &a, double *hist,
                         unsigned long long *dropped, hipStream_t st) {
   const int nt = (a.N + kCorrelTile - 1) / kCorrelTile;
-  const int ranges = a.overflow ? 1 : (a.n_bars + a.nbr - 1) / a.nbr;
-  const dim3 grid((unsigned)(nt * (nt + 1) / 2), (unsigned)ranges);
+  const dim3 grid((unsigned)(nt * (nt + 1) / 2), (unsigned)((a.n_bars + a.nbr - 1) / a.nbr));
   MSC_LAUNCH(k_pair_correl, grid, dim3(kCorrelTile), 0, st, x, ori, Ntot, a, hist, dropped);
 }
 void launch_clusters(const State &s, double min_s, double reach_max, int *par, int *flags, hipStream_t st) {

@@ -109,8 +109,7 @@ void launch_order_sums(const State &s, int rep, double *out, hipStream_t st);
 // pair correlations of one replica (acc_correl): histograms of the sets
 // [sets][4][n_bars] added into hist, pairs beyond the last bar counted in
 // *dropped.  Bars go through LDS in ranges of nbr (kCorrelLds doubles per
-// block hold sets*nbr bars of three doubles and a 32-bit count); with
-// overflow set there is one range and bars past it go to hist directly.
+// block hold sets*4*nbr of them).
 constexpr int kCorrelTile = 512;          // particles per tile (threads per block)
 constexpr int kCorrelLds = 19456;         // LDS histogram, doubles (152 KiB)
 constexpr int kCorrelMaxSpec = 8;
@@ -120,7 +119,6 @@ struct CorrelArgs {
   double L[3], invL[3];  // box and 1 / box
   double bar_w, inv_bw;
   int n_bars, nbr;
-  int overflow;  // bars at nbr and beyond go straight to the global histogram
 };
 void launch_pair_correl(const double *x, const double *ori, int Ntot, const CorrelArgs &a, double *hist,
                         unsigned long long *dropped, hipStream_t st);
