@@ -43,3 +43,30 @@ def test_bad_arguments_rejected_without_a_device():
     arr = (m.ReplicaDesc * 1)(d)
     assert L.msc_create(0, 0, 1, arr, ctypes.byref(ctx)) == m._lib.MSC_E_ARG
     assert L.msc_last_error(None) == b"no context"
+
+
+def test_new_entry_points_reject_bad_arguments_without_a_device(tmp_path):
+    """The correlation, order-parameter, conf.dat and initial-state entry
+    points refuse missing contexts, pointers and sizes before any device
+    work; the conf.dat reader reports a missing file."""
+    import mdspherocyl_amd as m
+    if not os.path.exists(m.LIBPATH):
+        pytest.skip("library not built")
+    L = m.load()
+    E = m._lib.MSC_E_ARG
+    h = (ctypes.c_double * 8)()
+    cnt = (ctypes.c_int * 1)(10)
+    assert L.msc_pair_correl(None, 0, 1, cnt, 0.1, 2, h, None) == E
+    g1, g2, M = ctypes.c_double(), ctypes.c_double(), (ctypes.c_double * 3)()
+    assert L.msc_order_params(None, 0, ctypes.byref(g1), ctypes.byref(g2), M) == E
+    z = (ctypes.c_double * 64)()
+    assert L.msc_read_conf(str(tmp_path / "missing.dat").encode(), 1, 1, z, z, z, z, z, z, z, z, z, None, z) == E
+    assert L.msc_write_conf(None, 1, 1, z, z, z, z, z, z, z, z, z, None, z) == E
+    mt = (ctypes.c_uint * 624)()
+    idx = ctypes.c_int(624)
+    one = (ctypes.c_double * 3)(1.0, 1.0, 1.0)
+    # N does not match the species counts
+    assert L.msc_ini_state(2, 1, cnt, one, one, one, one, 1.0, 0, mt, ctypes.byref(idx), z, z, z, z) == E
+    # generator position outside [0, 624]
+    idx.value = 700
+    assert L.msc_ini_state(10, 1, cnt, one, z, z, z, 1.0, 0, mt, ctypes.byref(idx), z, z, z, z) == E
