@@ -1878,9 +1878,6 @@ __global__ __launch_bounds__(kCorrelTile) void k_pair_correl(const double *__res
   while (J * (J + 1) / 2 > k) --J;
   while ((J + 1) * (J + 2) / 2 <= k) ++J;
   const int I = k - J * (J + 1) / 2;
-  // (b0 bar_w (1 - 1e-9))^2: below it sqrt, the quotient by bar_w and its
-  // floor all stay below bar b0, whatever their rounding
-  const double lo = (double)b0 * a.bar_w * (1.0 - 1e-9), lo2 = lo * lo;
   __syncthreads();
   const int f = I * kCorrelTile + threadIdx.x;
   unsigned long long drop = 0;
@@ -1912,11 +1909,7 @@ __global__ __launch_bounds__(kCorrelTile) void k_pair_correl(const double *__res
       YIJ -= a.L[1] * correl_round(YIJ, a.L[1], a.invL[1]);
       double ZIJ = ri2 - rj2;
       ZIJ -= a.L[2] * correl_round(ZIJ, a.L[2], a.invL[2]);
-      const double R2 = XIJ * XIJ + YIJ * YIJ + ZIJ * ZIJ;
-      // a range after the first skips, by the square alone, the pairs that
-      // fall clearly below its first bar (it counts no dropped pairs)
-      if (R2 < lo2) continue;
-      const double RIJ = sqrt(R2);
+      const double RIJ = sqrt(XIJ * XIJ + YIJ * YIJ + ZIJ * ZIJ);
       const int bin = (int)correl_floor(RIJ, a.bar_w, a.inv_bw);
       if (bin >= a.n_bars) {
         ++drop;
