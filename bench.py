@@ -291,6 +291,12 @@ def main():
     t_d, _ = timed_run(f64, k64, min(args.warmup, 10), False)
     sps_d = k64 / t_d
     log("bench: FP64 %.2f steps/s; mixed/FP64 = %.2f" % (sps_d, sps_mix / sps_d))
+    # FP64 with the WCA over the contact rows, as the mixed path walks them
+    f64.set_contact_rows(True)
+    t_dc, _ = timed_run(f64, k64, min(args.warmup, 10), False)
+    f64.set_contact_rows(False)
+    sps_dc = k64 / t_dc
+    log("bench: FP64 on contact rows %.2f steps/s; mixed/that = %.2f" % (sps_dc, sps_mix / sps_dc))
     mix.close()
     f64.close()
 
@@ -367,6 +373,8 @@ def main():
                    "k_pairs": nk, "wca_pairs": counts[0], "real_pairs": counts[1], "list_pairs": counts[3]},
         "fp64_steps_per_s": round(sps_d, 3),
         "mixed_over_fp64": round(sps_mix / sps_d, 3),
+        "fp64_contact_rows_steps_per_s": round(sps_dc, 3),
+        "mixed_over_fp64_contact_rows": round(sps_mix / sps_dc, 3),
         "err_mixed_vs_fp64": err_mix_vs_fp64,
         "err_vs_oracle": err_oracle,
         "batched": batched,

@@ -176,6 +176,17 @@ int msc_chunk_repeats(msc_ctx *ctx, long *n);
 int msc_cluster_labels(msc_ctx *ctx, int rep, double min_s, unsigned int *mt_state,
                        int *mt_index, int *labels, int *nclust, long *n_asym);
 
+/* Which neighbour rows force_sc (and the WCA launch of msc_run) walks.
+ * on = 0: every pair of the reference rows, as force_sc does
+ * (v25/functions.cpp:1397-1553); the default of MSC_FP64 contexts.  on = 1:
+ * the contact rows (pairs within 2^(1/6) D_max plus a skin of D_min / 2,
+ * refreshed from the reference rows when the skin is used up); the default
+ * of MSC_MIXED contexts.  Every pair that interacts or overlaps lies in the
+ * contact rows, which keep the reference rows' order, so forces, energies
+ * and overlap reports are the same; only the work differs.  No
+ * counterpart in the reference. */
+int msc_set_contact_rows(msc_ctx *ctx, int on);
+
 /* The reference's Signal=0 initial state (v25/main.cpp:215-260), host only:
  * ini_pos's stacked triangular lattice with every axis along z
  * (v25/functions.cpp:491-676, including its overlap test), nmix GCAmix

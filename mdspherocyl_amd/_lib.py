@@ -24,7 +24,7 @@ EXPORTS = [
     "msc_ext_field", "msc_self_ener", "msc_zero_forces", "msc_moveX", "msc_moveP", "msc_renorm",
     "msc_init_run", "msc_run", "msc_set_graph", "msc_cluster_labels", "msc_set_timing",
     "msc_kernel_time", "msc_reset_timing", "msc_work_counts", "msc_chunk_repeats", "msc_write_conf",
-    "msc_read_conf", "msc_pair_correl", "msc_order_params", "msc_ini_state",
+    "msc_read_conf", "msc_pair_correl", "msc_order_params", "msc_ini_state", "msc_set_contact_rows",
 ]
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -88,6 +88,7 @@ def load():
         "msc_set_graph": ([vp, ctypes.c_int], ctypes.c_int),
         "msc_cluster_labels": ([vp, ctypes.c_int, ctypes.c_double, ctypes.POINTER(ctypes.c_uint),
                                 _ip, _ip, _ip, ctypes.POINTER(ctypes.c_long)], ctypes.c_int),
+        "msc_set_contact_rows": ([vp, ctypes.c_int], ctypes.c_int),
         "msc_ini_state": ([ctypes.c_int, ctypes.c_int, _ip, _dp, _dp, _dp, _dp, ctypes.c_double, ctypes.c_int,
                            ctypes.POINTER(ctypes.c_uint), _ip, _dp, _dp, _dp, _dp], ctypes.c_int),
         "msc_order_params": ([vp, ctypes.c_int, _dp, _dp, _dp], ctypes.c_int),

@@ -78,6 +78,7 @@ struct TimerSet {
 struct msc_ctx {
   int device = 0, precision = MSC_FP64;
   bool mixed = false;
+  bool contact = false;  // force_sc walks the contact rows (else every reference-row pair)
   hipStream_t stream = nullptr;
   int nrep = 0, Ntot = 0, nd = 1;
   int rchunk = 128;           // particles per structure-factor chunk task
@@ -730,6 +731,7 @@ int msc_create(int device, int precision, int nrep, const msc_replica_desc *reps
   ctx->device = device;
   ctx->precision = precision;
   ctx->mixed = precision == MSC_MIXED;
+  ctx->contact = ctx->mixed;
   ctx->nrep = nrep;
   ctx->desc.assign(reps, reps + nrep);
   int nd = reps[0].nd;
@@ -1216,7 +1218,7 @@ static int read_energy(msc_ctx *ctx, int rep, RepEnergy *e) {
 
 static void do_force_sc(msc_ctx *ctx, int rep) {
   State s = sel(ctx, rep);
-  if (ctx->mixed && rep != MSC_ALL) {
+  if (ctx->contact && rep != MSC_ALL) {
     // a force call outside the step loop: refresh contact rows whose skin
     // the particles have used up since they were built
     launch_check(s, ctx->d_task_out, ctx->d_rebuild, ctx->d_srebuild, nullptr, 0, true, ctx->stream);
@@ -1224,7 +1226,7 @@ static void do_force_sc(msc_ctx *ctx, int rep) {
   }
   {
     Bracket b(ctx, "wca");
-    launch_force_sc(s, !ctx->mixed, ctx->d_task_out, ctx->d_flags, ctx->stream);
+    launch_force_sc(s, !ctx->contact, ctx->d_task_out, ctx->d_flags, ctx->stream);
   }
   launch_reduce(s, ctx->d_task_out, 16, 11, 0, (double *)ctx->d_E, kEstride, kEoff_sc, ctx->stream, 1, ctx->d_counts,
                 4, 0);
@@ -1506,7 +1508,7 @@ static void enqueue_step(msc_ctx *ctx) {
   }
   {
     Bracket b(ctx, "wca");
-    launch_force_sc(s, !ctx->mixed, rows + kRowSC * region, ctx->d_flags, st);
+    launch_force_sc(s, !ctx->contact, rows + kRowSC * region, ctx->d_flags, st);
   }
   dipR_launches(ctx, MSC_ALL, rows + kRowR * region);
   dipK_launches(ctx, MSC_ALL, rows + kRowDrift * region + 2, ctx->d_kblk);
@@ -1523,7 +1525,7 @@ static void enqueue_step(msc_ctx *ctx) {
   // return at once for replicas that keep their rows
   {
     Bracket b(ctx, "nlist");
-    launch_rebuild(s, ctx->d_rebuild, ctx->mixed ? ctx->d_srebuild : nullptr, ctx->d_flags, st, ctx->d_step);
+    launch_rebuild(s, ctx->d_rebuild, ctx->contact ? ctx->d_srebuild : nullptr, ctx->d_flags, st, ctx->d_step);
   }
 }
 
@@ -1848,6 +1850,25 @@ int msc_ini_state(int N, int nspec, const int *counts, const double L[3], const 
   init::ini_mom(N, temp, mass[0], I, mt, p, l);
   std::copy(mt.mt, mt.mt + Mt19937::kN, mt_state);
   *mt_index = mt.mti;
+  return MSC_OK;
+}
+
+int msc_set_contact_rows(msc_ctx *ctx, int on) {
+  if (!ctx || (on != 0 && on != 1)) return MSC_E_ARG;
+  if (ctx->contact != (on == 1)) {
+    ctx->contact = on == 1;
+    drop_graph(ctx);  // the captured step launches the other WCA kernel
+    if (ctx->contact && ctx->have_lists) {
+      // the contact rows were last refreshed at a list build: refresh those
+      // whose skin the particles have used up since, before any force
+      for (int r = 0; r < ctx->nrep; ++r) {
+        const State s = sel(ctx, r);
+        launch_check(s, ctx->d_task_out, ctx->d_rebuild, ctx->d_srebuild, nullptr, 0, true, ctx->stream);
+        launch_short(s, ctx->d_srebuild, ctx->d_flags, ctx->stream);
+      }
+      LAUNCHED("contact rows");
+    }
+  }
   return MSC_OK;
 }
 

@@ -185,6 +185,11 @@ class MdSystem:
         self._chk(self.L.msc_run(self._ctx, step0, nsteps, _d(obs)[1] if observe else None))
         return obs[:nsteps] if observe else None
 
+    def set_contact_rows(self, on):
+        """WCA over the contact rows (True; mixed contexts' default) or over
+        every reference-row pair (False; FP64 contexts' default)."""
+        self._chk(self.L.msc_set_contact_rows(self._ctx, 1 if on else 0))
+
     def set_graph(self, on):
         self._chk(self.L.msc_set_graph(self._ctx, int(bool(on))))
 

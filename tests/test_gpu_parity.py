@@ -829,3 +829,53 @@ def test_reference_run_from_library_initial_state(msc):
         mine = obs[10 * k, 0, :11]
         for j in range(11):
             assert mine[j] == pytest.approx(row[j], rel=1e-9, abs=2e-12), "step %d column %d" % (10 * k, j)
+
+
+def test_fp64_contact_rows_match_reference_rows(msc):
+    """An FP64 context with msc_set_contact_rows(1) (WCA over the contact
+    rows, refreshed as the particles move) against the default (every
+    reference-row pair): 101 steps of v25/v251/run1 give the same printed
+    observables to 1e-12 and the reference's output.dat to 1e-9; and a
+    denser system (N=1500, packing fraction 0.25) after 200 steps has the
+    same state to 1e-12."""
+    from mdspherocyl_amd import configs
+    import math
+    L = math.pow(500.0 / ((1.0 * 1.0) * 0.0013), 1.0 / 3)
+    st = msc.ini_state(500, (L, L, L), 9.0, 1.0, 1.0, 1.0)
+    N = 500
+    args = (st["r"], st["p"], st["l"], st["ori"], np.full(N, 9.0), np.ones(N), np.ones(N), np.zeros((N, 1)),
+            np.tile([0.0, 0.0, 1.0], (N, 1, 1)), np.full((N, 1), 4.481))
+    obs = []
+    for contact in (False, True):
+        _say("case: v251 fp64 contact rows %s, 101 steps" % contact)
+        sysm = msc.MdSystem([H.v251_desc()], precision="fp64")
+        try:
+            sysm.set_contact_rows(contact)
+            sysm.set_state(0, *args)
+            sysm.init_run()
+            obs.append(sysm.run(101))
+        finally:
+            sysm.close()
+    assert np.allclose(obs[1][:, 0, :11], obs[0][:, 0, :11], rtol=1e-12, atol=1e-14)
+    ref = _fixture_lines()
+    for k, row in enumerate(ref):
+        for j in range(11):
+            assert obs[1][10 * k, 0, j] == pytest.approx(row[j], rel=1e-9, abs=2e-12)
+    rng = np.random.default_rng(41)
+    n = 1500
+    d, s0 = configs.make_replica(rng, n, configs.truncated_normal(rng, n, 3.0, 0.3), 0.25, 1.5, 5,
+                                 lambda b: 3.0 / 3.5, 3.5)
+    states = []
+    for contact in (False, True):
+        _say("case: N=1500 phi=0.25 fp64 contact rows %s, 200 steps" % contact)
+        sysm = msc.MdSystem([d], precision="fp64")
+        try:
+            sysm.set_contact_rows(contact)
+            configs.load_into(sysm, [(d, s0)])
+            sysm.init_run()
+            sysm.run(200, observe=False)
+            states.append(sysm.get_state(0))
+        finally:
+            sysm.close()
+    for a, b in zip(*states):
+        assert np.abs(a - b).max() <= 1e-12 * max(1.0, np.abs(a).max())
