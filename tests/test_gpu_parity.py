@@ -879,3 +879,31 @@ def test_fp64_contact_rows_match_reference_rows(msc):
             sysm.close()
     for a, b in zip(*states):
         assert np.abs(a - b).max() <= 1e-12 * max(1.0, np.abs(a).max())
+
+
+def test_fp64_contact_rows_switched_on_mid_run(msc):
+    """Contact rows switched on between two msc_run calls of an FP64
+    context (they are refreshed where the particles have used up their skin
+    since the last list build): 100 + 100 steps equal 200 steps on the
+    reference rows to 1e-12."""
+    from mdspherocyl_amd import configs
+    rng = np.random.default_rng(43)
+    n = 1500
+    d, s0 = configs.make_replica(rng, n, configs.truncated_normal(rng, n, 3.0, 0.3), 0.25, 1.5, 5,
+                                 lambda b: 3.0 / 3.5, 3.5)
+    states = []
+    for switch in (False, True):
+        _say("case: N=1500 fp64, contact rows switched on at step 100: %s" % switch)
+        sysm = msc.MdSystem([d], precision="fp64")
+        try:
+            configs.load_into(sysm, [(d, s0)])
+            sysm.init_run()
+            sysm.run(100, observe=False)
+            if switch:
+                sysm.set_contact_rows(True)
+            sysm.run(100, step0=100, observe=False)
+            states.append(sysm.get_state(0))
+        finally:
+            sysm.close()
+    for a, b in zip(*states):
+        assert np.abs(a - b).max() <= 1e-12 * max(1.0, np.abs(a).max())
