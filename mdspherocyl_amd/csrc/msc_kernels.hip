@@ -1735,14 +1735,17 @@ __global__ void k_clust_init(State s, int *__restrict__ par) {
   if (g < s.g_end) par[g] = g;
 }
 
-// One thread per particle g over its reference row, each pair once (j > g).
-// The row holds every pair closer than r_list_max when the list was built;
-// a pair in contact now was closer than reach_max + de_g + de_j then, with
-// de the largest displacement of a point of the segment since, so a particle
-// with 2 de >= r_list_max - reach_max is counted in flags[0].
+// One wave per particle g, its lanes over g's reference row, each pair once
+// (j > g).  The row holds every pair closer than r_list_max when the list was
+// built; a pair in contact now was closer than reach_max + de_g + de_j then,
+// with de the largest displacement of a point of the segment since, so a
+// particle with 2 de >= r_list_max - reach_max is counted in flags[0].
+// Unions from concurrent lanes are safe (uf_union retries its exchange); the
+// labels depend only on the components, not on the roots the unions pick.
 __global__ void __launch_bounds__(256) k_clust_edges(State s, double min_s, double reach_max, int *__restrict__ par,
                                                      int *__restrict__ flags) {
-  const int g = s.g_begin + blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const int g = s.g_begin + __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
   if (g >= s.g_end) return;
   const RepDev &R = s.reps[s.rep_of[g]];
   const int N = s.Ntot;
@@ -1758,9 +1761,9 @@ __global__ void __launch_bounds__(256) k_clust_edges(State s, double min_s, doub
   const double Lg = s.len[g], Dg = s.dia[g];
   // the 1e-9 covers the 12-decimal rounding of the reference positions
   const double de = sqrt(dr2) + 0.5 * Lg * sqrt(du2) + 1e-9;
-  if (2.0 * de >= R.r_list_max - reach_max) atomicAdd(&flags[0], 1);
+  if (lane == 0 && 2.0 * de >= R.r_list_max - reach_max) atomicAdd(&flags[0], 1);
   const int n = s.nl_cnt[g];
-  for (int q = 0; q < n; ++q) {
+  for (int q = lane; q < n; q += 64) {
     const int j = s.nl_idx[row_at(R.nl_base, R.N, R.n0, q, g)];
     if (j <= g) continue;
     // dist_test, v25/functions.cpp:2987-3013
@@ -2589,7 +2592,7 @@ void launch_clusters(const State &s, double min_s, double reach_max, int *par, i
   const dim3 grid(nblk(s.g_end - s.g_begin, 256));
   launch_fill_int(flags, 2, 0, st);
   MSC_LAUNCH(k_clust_init, grid, dim3(256), 0, st, s, par);
-  MSC_LAUNCH(k_clust_edges, grid, dim3(256), 0, st, s, min_s, reach_max, par, flags);
+  MSC_LAUNCH(k_clust_edges, dim3(nblk(s.g_end - s.g_begin, 4)), dim3(256), 0, st, s, min_s, reach_max, par, flags);
   MSC_LAUNCH(k_clust_final, grid, dim3(256), 0, st, s, par);
 }
 void launch_kick1(const State &s, const Thermo *th, const double *gs, const int *step, double *rows, hipStream_t st) {
