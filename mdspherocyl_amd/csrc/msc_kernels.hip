@@ -1793,15 +1793,15 @@ __global__ void k_clust_final(State s, int *__restrict__ par) {
 // order-parameter sums (oo_params + Qmat_3d, v25/functions.cpp:2702-2760,
 // 3426-3453): per replica sum u_a u_b (xx xy xz yy yz zz), sum u_a and the
 // total dipole moment sum mu w_a over the lab-frame dipole axes, one block
-// per replica, a fixed-order tree over its threads.
+// of kOrderThreads per replica, a fixed-order tree over its threads.
 // --------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_order_sums(State s, int rep, double *__restrict__ out) {
-  __shared__ double sh[kOrderSums][256];
+__global__ __launch_bounds__(kOrderThreads) void k_order_sums(State s, int rep, double *__restrict__ out) {
+  __shared__ double sh[kOrderSums][kOrderThreads];
   const RepDev &R = s.reps[rep];
   const int N = s.Ntot;
   double acc[kOrderSums];
   for (int k = 0; k < kOrderSums; ++k) acc[k] = 0.0;
-  for (int i = threadIdx.x; i < R.N; i += 256) {
+  for (int i = threadIdx.x; i < R.N; i += kOrderThreads) {
     const int g = R.n0 + i;
     const double u0 = s.ori[6 * N + g], u1 = s.ori[7 * N + g], u2 = s.ori[8 * N + g];
     acc[0] += u0 * u0;
@@ -1820,7 +1820,7 @@ __global__ __launch_bounds__(256) void k_order_sums(State s, int rep, double *__
   }
   for (int k = 0; k < kOrderSums; ++k) sh[k][threadIdx.x] = acc[k];
   __syncthreads();
-  for (int h = 128; h > 0; h >>= 1) {
+  for (int h = kOrderThreads / 2; h > 0; h >>= 1) {
     if ((int)threadIdx.x < h)
       for (int k = 0; k < kOrderSums; ++k) sh[k][threadIdx.x] += sh[k][threadIdx.x + h];
     __syncthreads();
@@ -2580,7 +2580,7 @@ void launch_zero_FT(const State &s, hipStream_t st) {
   MSC_LAUNCH(k_zero_FT, dim3(nblk(s.g_end - s.g_begin, 256)), dim3(256), 0, st, s);
 }
 void launch_order_sums(const State &s, int rep, double *out, hipStream_t st) {
-  MSC_LAUNCH(k_order_sums, dim3(1), dim3(256), 0, st, s, rep, out);
+  MSC_LAUNCH(k_order_sums, dim3(1), dim3(kOrderThreads), 0, st, s, rep, out);
 }
 void launch_pair_correl(const double *x, const double *ori, int Ntot, const CorrelArgs &a, double *hist,
                         unsigned long long *dropped, hipStream_t st) {

@@ -104,6 +104,7 @@ void launch_clusters(const State &s, double min_s, double reach_max, int *par, i
 // order-parameter sums of replica rep (oo_params): out[0..5] = sum u_a u_b
 // (xx xy xz yy yz zz), out[6..8] = sum u, out[9..11] = total dipole moment
 constexpr int kOrderSums = 12;
+constexpr int kOrderThreads = 1024;  // one block; its LDS tree holds 12 x 1024 doubles
 void launch_order_sums(const State &s, int rep, double *out, hipStream_t st);
 
 // pair correlations of one replica (acc_correl): histograms of the sets
