@@ -66,11 +66,6 @@ def force_split(sysm, rep, sr_cut):
     return F, T, Usc + UR + UK
 
 
-def rel_l2(a, b):
-    d = np.linalg.norm(b)
-    return float(np.linalg.norm(a - b) / d) if d > 0 else float(np.linalg.norm(a - b))
-
-
 def timed_run(sysm, steps, warmup, timing, step0=0):
     if warmup:
         sysm.run(warmup, step0=step0, observe=False)
@@ -91,10 +86,6 @@ def mt19937_state(seed):
         prev = int(mt[i - 1])
         mt[i] = (1812433253 * (prev ^ (prev >> 30)) + i) & 0xffffffff
     return mt
-
-
-def rel_abs(a, b):
-    return abs(a - b) / abs(b) if b != 0 else float("nan")
 
 
 def kernel_table(sysm, names):
@@ -181,7 +172,7 @@ def main():
         sys.exit(2)
 
     import mdspherocyl_amd as m
-    from mdspherocyl_amd import configs
+    from mdspherocyl_amd import accuracy, configs
 
     m.load()
     if args.copy_mib > 0:
@@ -212,9 +203,21 @@ def main():
     Fd, Td, Ud = force_split(f64, 0, desc.sr_cut)
     off, nbr = f64.nblist(0)
     counts = f64.work_counts(0)
-    err_mix_vs_fp64 = {"F": rel_l2(Fm, Fd), "T": rel_l2(Tm, Td), "U": rel_abs(Um, Ud)}
-    log("bench: mixed vs FP64 GPU at the initial state: rel L2 F %.2e T %.2e, U %.2e"
-        % (err_mix_vs_fp64["F"], err_mix_vs_fp64["T"], err_mix_vs_fp64["U"]))
+    # a dead evaluation (all-zero or non-finite forces) stops the bench: it
+    # would otherwise report perfect agreement
+    try:
+        err_mix_vs_fp64 = accuracy.force_errors(Fm, Tm, Um, Fd, Td, Ud)
+        accuracy.check_reference("mixed F", Fm)
+        accuracy.check_reference("mixed T", Tm)
+    except accuracy.DegenerateReference as e:
+        log("bench: FAILED, degenerate force evaluation: %s" % e)
+        sys.exit(3)
+    fp64_norms = {"F": float(np.linalg.norm(Fd)), "T": float(np.linalg.norm(Td)), "U": Ud}
+    log("bench: mixed vs FP64 GPU at the initial state: F rel L2 %.2e max/rms %.2e max/own %.2e; "
+        "T rel L2 %.2e max/rms %.2e max/own %.2e; U %.2e"
+        % (err_mix_vs_fp64["F"]["rel_l2"], err_mix_vs_fp64["F"]["max_rel_rms"], err_mix_vs_fp64["F"]["max_rel_own"],
+           err_mix_vs_fp64["T"]["rel_l2"], err_mix_vs_fp64["T"]["max_rel_rms"], err_mix_vs_fp64["T"]["max_rel_own"],
+           err_mix_vs_fp64["U"]))
 
     names = ["recip_struct", "recip_force", "real_pairs", "wca", "nlist", "integrate"]
     log("bench: mixed, %d warmup + %d timed steps" % (args.warmup, args.steps))
@@ -347,10 +350,13 @@ def main():
                "sample": "one force evaluation (WCA on the neighbour list, real space within r_c=3.5, reciprocal "
                          "k_max=10) of the N=32768 workload, serial C oracle, %.1f s; the integrator "
                          "(about 1%% of a reference step) is not timed" % t_cpu}
-        err_oracle = {"mixed": {"F": rel_l2(Fm, Fo), "T": rel_l2(Tm, To), "U": rel_abs(Um, Uo)},
-                      "fp64": {"F": rel_l2(Fd, Fo), "T": rel_l2(Td, To), "U": rel_abs(Ud, Uo)}}
-        log("bench: oracle %.1f s per force evaluation; error vs oracle: mixed F %.2e T %.2e U %.2e, "
-            "FP64 F %.2e T %.2e U %.2e" % (t_cpu, *err_oracle["mixed"].values(), *err_oracle["fp64"].values()))
+        err_oracle = {"mixed": accuracy.force_errors(Fm, Tm, Um, Fo, To, Uo),
+                      "fp64": accuracy.force_errors(Fd, Td, Ud, Fo, To, Uo)}
+        for k in ("mixed", "fp64"):
+            e = err_oracle[k]
+            log("bench: %s vs oracle: F rel L2 %.2e max/rms %.2e max/own %.2e; T rel L2 %.2e max/rms %.2e "
+                "max/own %.2e; U %.2e" % (k, e["F"]["rel_l2"], e["F"]["max_rel_rms"], e["F"]["max_rel_own"],
+                                          e["T"]["rel_l2"], e["T"]["max_rel_rms"], e["T"]["max_rel_own"], e["U"]))
         correl["oracle_pairs_per_s"] = round(cpu_correl_rate(desc, st), 1)
         log("bench: oracle acc_correl %.3e pairs/s (serial, first 4096 particles)" % correl["oracle_pairs_per_s"])
 
@@ -376,6 +382,7 @@ def main():
         "fp64_contact_rows_steps_per_s": round(sps_dc, 3),
         "mixed_over_fp64_contact_rows": round(sps_mix / sps_dc, 3),
         "err_mixed_vs_fp64": err_mix_vs_fp64,
+        "fp64_force_norms": fp64_norms,
         "err_vs_oracle": err_oracle,
         "batched": batched,
         "cluster_labels": clusters,

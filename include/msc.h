@@ -32,7 +32,9 @@ extern "C" {
 #define MSC_E_OVERLAP   1   /* force_sc: segment distance^2 < 0.49 sigma^2  */
 #define MSC_E_DISTANCE  2   /* force_sc: segment distance^2 > 1.00001 r^2   */
 #define MSC_E_ARG       3   /* bad argument                                  */
-#define MSC_E_HIP       4   /* HIP runtime error                             */
+#define MSC_E_HIP       4   /* HIP runtime error, or kernels that did not
+                               execute (each call that enqueues kernels reads
+                               back a completion stamp written after them)  */
 #define MSC_E_CAPACITY  5   /* a device buffer bound was exceeded            */
 #define MSC_E_STATE     6   /* call order (e.g. force before neighbour list) */
 
@@ -183,8 +185,11 @@ int msc_cluster_labels(msc_ctx *ctx, int rep, double min_s, unsigned int *mt_sta
  * refreshed from the reference rows when the skin is used up); the default
  * of MSC_MIXED contexts.  Every pair that interacts or overlaps lies in the
  * contact rows, which keep the reference rows' order, so forces, energies
- * and overlap reports are the same; only the work differs.  No
- * counterpart in the reference. */
+ * and MSC_E_OVERLAP reports are the same; only the work differs.  One
+ * difference: the reference's second abort test, d^2 > 1.00001 r^2 on every
+ * list pair (v25/functions.cpp:1466, MSC_E_DISTANCE), runs on the contact
+ * pairs only, so an inconsistency of a distant list pair is not reported.
+ * No counterpart in the reference. */
 int msc_set_contact_rows(msc_ctx *ctx, int on);
 
 /* The reference's Signal=0 initial state (v25/main.cpp:215-260), host only:

@@ -134,7 +134,19 @@ struct msc_ctx {
   // the step captured as a graph; dropped when anything it holds by value
   // (row regions, run buffers, replica parameters) changes
   hipGraphExec_t graph_exec = nullptr;
+  LaunchRecord launch;      // failed launches of this context's entry points
+  double *d_stamp = nullptr;  // completion stamp (finish)
+  double seq = 0;             // last stamp value enqueued
 };
+
+// Makes the context's launch record current on the calling thread for the
+// duration of an entry point (launch failures are kept per context).
+struct LaunchScope {
+  LaunchRecord *prev;
+  explicit LaunchScope(msc_ctx *c) : prev(set_launch_record(c ? &c->launch : nullptr)) {}
+  ~LaunchScope() { set_launch_record(prev); }
+};
+#define SCOPE(ctx) LaunchScope launch_scope_(ctx)
 
 static std::string g_err;  // failures that have no context to hold them
 static constexpr size_t kPinBytes = (size_t)4 << 20;
@@ -218,6 +230,30 @@ static int h2d(msc_ctx *ctx, void *dst, const void *src, size_t bytes) {
   HIPC(hipStreamSynchronize(ctx->stream));
   return MSC_OK;
 }
+
+// Proof that the kernels an entry point enqueued have executed: a one-lane
+// kernel enqueued after them writes a fresh sequence number, which is read
+// back and compared.  A launch the runtime accepted but never ran (results
+// would stay at their previous or zero values) makes the call fail.
+static int finish(msc_ctx *ctx, const char *what) {
+  LAUNCHED(what);
+  const double want = ++ctx->seq;
+  launch_stamp(ctx->d_stamp, want, ctx->stream);
+  LAUNCHED(what);
+  double got = -1;
+  if (int rc_ = d2h(ctx, &got, ctx->d_stamp, sizeof got)) return rc_;
+  if (got != want) {
+    ctx->err = std::string(what) + ": the device did not execute the library's kernels (completion stamp " +
+               std::to_string(got) + ", expected " + std::to_string(want) + ")";
+    g_err = ctx->err;
+    return MSC_E_HIP;
+  }
+  return MSC_OK;
+}
+#define FINISH(what)                                        \
+  do {                                                      \
+    if (int rc_f_ = finish(ctx, (what))) return rc_f_;      \
+  } while (0)
 
 namespace {
 
@@ -854,6 +890,7 @@ int msc_create(int device, int precision, int nrep, const msc_replica_desc *reps
   A(&ctx->d_rebuild, nrep);
   A(&ctx->d_srebuild, nrep);
   A(&ctx->d_one, nrep);
+  A(&ctx->d_stamp, 1);
   if (rc == MSC_OK) {
     unsigned char **raw[] = {(unsigned char **)&ctx->d_tab, (unsigned char **)&ctx->d_ph,
                              (unsigned char **)&ctx->d_amp, (unsigned char **)&ctx->d_part,
@@ -920,7 +957,8 @@ void msc_destroy(msc_ctx *ctx) {
                   ctx->d_snap_u, ctx->d_clu, ctx->d_clu_flag, ctx->d_corr, ctx->d_corr_drop, ctx->d_order, ctx->d_rows, ctx->d_kblk, ctx->d_step,
                   ctx->d_tab, ctx->d_ph, ctx->d_amp, ctx->d_part, ctx->d_A, ctx->d_slot_out, ctx->d_task_out,
                   ctx->d_red, ctx->d_M2, ctx->d_Kin, ctx->d_gs, ctx->d_obs, ctx->d_counts, ctx->d_E,
-                  ctx->d_th, ctx->d_flags, ctx->d_rebuild, ctx->d_one, ctx->d_run_gs, ctx->d_run_obs, ctx->d_save};
+                  ctx->d_th, ctx->d_flags, ctx->d_rebuild, ctx->d_one, ctx->d_run_gs, ctx->d_run_obs, ctx->d_save,
+                  ctx->d_stamp};
   for (void *b : bufs)
     if (b) hipFree(b);
   for (auto &kv : ctx->timers)
@@ -936,6 +974,7 @@ void msc_destroy(msc_ctx *ctx) {
 int msc_set_state(msc_ctx *ctx, int rep, const double *r, const double *p, const double *l, const double *ori,
                   const double *len, const double *dia, const double *mass, const double *dpc, const double *doc,
                   const double *dmu) {
+  SCOPE(ctx);
   if (!ctx || rep < 0 || rep >= ctx->nrep || !r || !p || !l || !ori || !len || !dia || !mass || !dpc || !doc || !dmu)
     return MSC_E_ARG;
   RepDev &R = ctx->reps[rep];
@@ -963,12 +1002,13 @@ int msc_set_state(msc_ctx *ctx, int rep, const double *r, const double *p, const
   derive(ctx, rep, len, dia, dpc);
   upload_reps(ctx);
   launch_dipoles(sel(ctx, rep), ctx->stream);
-  HIPC(hipStreamSynchronize(ctx->stream));
+  FINISH("msc_set_state");
   ctx->have_state = true;
   return MSC_OK;
 }
 
 int msc_get_state(msc_ctx *ctx, int rep, double *r, double *p, double *l, double *ori) {
+  SCOPE(ctx);
   if (!ctx || rep < 0 || rep >= ctx->nrep) return MSC_E_ARG;
   const RepDev &R = ctx->reps[rep];
   const int n = R.N, N = ctx->Ntot;
@@ -990,6 +1030,7 @@ int msc_get_state(msc_ctx *ctx, int rep, double *r, double *p, double *l, double
 }
 
 int msc_get_forces(msc_ctx *ctx, int rep, double *F, double *T) {
+  SCOPE(ctx);
   if (!ctx || rep < 0 || rep >= ctx->nrep) return MSC_E_ARG;
   const RepDev &R = ctx->reps[rep];
   const int n = R.N, N = ctx->Ntot;
@@ -1009,6 +1050,7 @@ int msc_get_forces(msc_ctx *ctx, int rep, double *F, double *T) {
 }
 
 int msc_set_forces(msc_ctx *ctx, int rep, const double *F, const double *T) {
+  SCOPE(ctx);
   if (!ctx || rep < 0 || rep >= ctx->nrep || !F || !T) return MSC_E_ARG;
   const RepDev &R = ctx->reps[rep];
   const int n = R.N, N = ctx->Ntot;
@@ -1023,6 +1065,7 @@ int msc_set_forces(msc_ctx *ctx, int rep, const double *F, const double *T) {
 }
 
 int msc_get_thermostat(msc_ctx *ctx, int rep, double s[2], double strn[2]) {
+  SCOPE(ctx);
   if (!ctx || rep < 0 || rep >= ctx->nrep) return MSC_E_ARG;
   Thermo t;
   HIPC(hipStreamSynchronize(ctx->stream));
@@ -1033,6 +1076,7 @@ int msc_get_thermostat(msc_ctx *ctx, int rep, double s[2], double strn[2]) {
 }
 
 int msc_set_thermostat(msc_ctx *ctx, int rep, const double s[2], const double strn[2]) {
+  SCOPE(ctx);
   if (!ctx || rep < 0 || rep >= ctx->nrep || !s || !strn) return MSC_E_ARG;
   Thermo t;
   t.s[0] = s[0]; t.s[1] = s[1]; t.strn[0] = strn[0]; t.strn[1] = strn[1];
@@ -1151,14 +1195,18 @@ static int lists_layout(msc_ctx *ctx, const std::vector<long> &need_nl = {},
 }
 
 int msc_neigh_list(msc_ctx *ctx, int rep, double r_list) {
+  SCOPE(ctx);
   if (!ctx || !rep_ok(ctx, rep) || !(r_list > 0)) return MSC_E_ARG;
   if (!ctx->have_state) return fail(ctx, MSC_E_STATE, "no state");
+  // the rows are valid again only once this build has fitted its regions
+  ctx->have_lists = false;
   for (int r = 0; r < ctx->nrep; ++r)
     if (rep == MSC_ALL || r == rep) ctx->reps[r].r_list_max = r_list;
   int rc = lists_layout(ctx);
   if (rc) return rc;
   launch_fill_int(ctx->d_flags, 4 * ctx->nrep, 0, ctx->stream);
   if ((rc = lists_build(ctx, ctx->d_one))) return rc;
+  FINISH("msc_neigh_list");
   std::vector<int> fl(4 * ctx->nrep);
   if (int rc_ = d2h(ctx, fl.data(), ctx->d_flags, sizeof(int) * fl.size())) return rc_;
   for (int r = 0; r < ctx->nrep; ++r)
@@ -1169,6 +1217,7 @@ int msc_neigh_list(msc_ctx *ctx, int rep, double r_list) {
 }
 
 int msc_get_neigh_list(msc_ctx *ctx, int rep, int *off, int *nbr, long cap, long *total) {
+  SCOPE(ctx);
   if (!ctx || rep < 0 || rep >= ctx->nrep || !off || !total) return MSC_E_ARG;
   if (!ctx->have_lists) return fail(ctx, MSC_E_STATE, "no neighbour list");
   const RepDev &R = ctx->reps[rep];
@@ -1196,6 +1245,7 @@ int msc_get_neigh_list(msc_ctx *ctx, int rep, int *off, int *nbr, long cap, long
 }
 
 int msc_check_list(msc_ctx *ctx, int rep, double r_list_min, double r_list_max, int *rebuild) {
+  SCOPE(ctx);
   if (!ctx || rep < 0 || rep >= ctx->nrep || !rebuild) return MSC_E_ARG;
   if (!ctx->have_lists) return fail(ctx, MSC_E_STATE, "no neighbour list");
   ctx->reps[rep].r_list_min = r_list_min;
@@ -1204,6 +1254,7 @@ int msc_check_list(msc_ctx *ctx, int rep, double r_list_min, double r_list_max, 
   if ((rc = upload_reps(ctx))) return rc;
   State s = sel(ctx, rep);
   launch_check(s, ctx->d_task_out, ctx->d_rebuild, nullptr, nullptr, 0, false, ctx->stream);
+  FINISH("msc_check_list");
   if (int rc_ = d2h(ctx, rebuild, ctx->d_rebuild + rep, sizeof(int))) return rc_;
   return MSC_OK;
 }
@@ -1211,8 +1262,8 @@ int msc_check_list(msc_ctx *ctx, int rep, double r_list_min, double r_list_max, 
 // --------------------------------------------------------------------------
 // forces
 // --------------------------------------------------------------------------
+// energies of the last force evaluation (after finish())
 static int read_energy(msc_ctx *ctx, int rep, RepEnergy *e) {
-  LAUNCHED("force routine");
   return d2h(ctx, e, ctx->d_E + rep, sizeof(RepEnergy));
 }
 
@@ -1233,6 +1284,7 @@ static void do_force_sc(msc_ctx *ctx, int rep) {
 }
 
 int msc_force_sc(msc_ctx *ctx, int rep, double r_cut, double *U, double *W, double PT[9]) {
+  SCOPE(ctx);
   if (!ctx || rep < 0 || rep >= ctx->nrep) return MSC_E_ARG;
   if (!ctx->have_lists) return fail(ctx, MSC_E_STATE, "no neighbour list");
   if (r_cut != ctx->reps[rep].sr_cut) {
@@ -1242,6 +1294,7 @@ int msc_force_sc(msc_ctx *ctx, int rep, double r_cut, double *U, double *W, doub
   }
   HIPC(hipMemsetAsync(ctx->d_flags + 4 * rep, 0, sizeof(int) * 4, ctx->stream));
   do_force_sc(ctx, rep);
+  FINISH("msc_force_sc");
   int fl[4];
   if (int rc_ = d2h(ctx, fl, ctx->d_flags + 4 * rep, sizeof fl)) return rc_;
   if (fl[0]) return fail(ctx, fl[0] == 1 ? MSC_E_OVERLAP : MSC_E_DISTANCE, "force_sc abort");
@@ -1282,9 +1335,11 @@ static void do_dipR(msc_ctx *ctx, int rep) {
 }
 
 int msc_force_dipR(msc_ctx *ctx, int rep, double *U, double PT[9]) {
+  SCOPE(ctx);
   if (!ctx || rep < 0 || rep >= ctx->nrep) return MSC_E_ARG;
   if (ctx->reps[rep].rc > 0 && !ctx->have_lists) return fail(ctx, MSC_E_STATE, "no pair list");
   do_dipR(ctx, rep);
+  FINISH("msc_force_dipR");
   RepEnergy e;
   int rc = read_energy(ctx, rep, &e);
   if (rc) return rc;
@@ -1353,10 +1408,12 @@ static int dipK_launches(msc_ctx *ctx, int rep, const double *m2rows, double *kb
 static int do_dipK(msc_ctx *ctx, int rep) { return dipK_launches(ctx, rep, nullptr, nullptr); }
 
 int msc_force_dipK(msc_ctx *ctx, int rep, double *U, double PT[9]) {
+  SCOPE(ctx);
   if (!ctx || rep < 0 || rep >= ctx->nrep) return MSC_E_ARG;
   if (ctx->nd != 1) return fail(ctx, MSC_E_ARG, "the reciprocal kernels take one dipole slot per particle");
   int rc = do_dipK(ctx, rep);
   if (rc) return rc;
+  FINISH("msc_force_dipK");
   RepEnergy e;
   if ((rc = read_energy(ctx, rep, &e))) return rc;
   if (U) *U = e.UK;
@@ -1365,10 +1422,12 @@ int msc_force_dipK(msc_ctx *ctx, int rep, double *U, double PT[9]) {
 }
 
 int msc_ext_field(msc_ctx *ctx, int rep, double *U) {
+  SCOPE(ctx);
   if (!ctx || rep < 0 || rep >= ctx->nrep) return MSC_E_ARG;
   State s = sel(ctx, rep);
   launch_ext_field(s, ctx->d_task_out, ctx->stream);
   launch_reduce(s, ctx->d_task_out, 16, 1, 0, (double *)ctx->d_E, kEstride, kEoff_ext, ctx->stream);
+  FINISH("msc_ext_field");
   RepEnergy e;
   int rc = read_energy(ctx, rep, &e);
   if (rc) return rc;
@@ -1377,10 +1436,12 @@ int msc_ext_field(msc_ctx *ctx, int rep, double *U) {
 }
 
 int msc_self_ener(msc_ctx *ctx, int rep, double *U) {
+  SCOPE(ctx);
   if (!ctx || rep < 0 || rep >= ctx->nrep || !U) return MSC_E_ARG;
   State s = sel(ctx, rep);
   launch_self_ener(s, ctx->d_task_out, ctx->stream);
   launch_reduce(s, ctx->d_task_out, 16, 1, 0, ctx->d_red, kRedStride, 15, ctx->stream);
+  FINISH("msc_self_ener");
   double v[16];
   if (int rc_ = d2h(ctx, v, ctx->d_red + kRedStride * rep, sizeof v)) return rc_;
   *U = v[15];
@@ -1388,9 +1449,10 @@ int msc_self_ener(msc_ctx *ctx, int rep, double *U) {
 }
 
 int msc_zero_forces(msc_ctx *ctx, int rep) {
+  SCOPE(ctx);
   if (!ctx || !rep_ok(ctx, rep)) return MSC_E_ARG;
   launch_zero_FT(sel(ctx, rep), ctx->stream);
-  HIPC(hipStreamSynchronize(ctx->stream));
+  FINISH("msc_zero_forces");
   return MSC_OK;
 }
 
@@ -1398,6 +1460,7 @@ int msc_zero_forces(msc_ctx *ctx, int rep) {
 // integrator
 // --------------------------------------------------------------------------
 int msc_moveX(msc_ctx *ctx, int rep, double dt) {
+  SCOPE(ctx);
   if (!ctx || !rep_ok(ctx, rep)) return MSC_E_ARG;
   for (int r = 0; r < ctx->nrep; ++r)
     if (rep == MSC_ALL || r == rep) ctx->reps[r].dt = dt;
@@ -1406,11 +1469,12 @@ int msc_moveX(msc_ctx *ctx, int rep, double dt) {
   State s = sel(ctx, rep);
   launch_moveX(s, ctx->stream);
   launch_dipoles(s, ctx->stream);
-  HIPC(hipStreamSynchronize(ctx->stream));
+  FINISH("msc_moveX");
   return MSC_OK;
 }
 
 int msc_moveP(msc_ctx *ctx, int rep, double dt, int last, double K[2], double PT[9]) {
+  SCOPE(ctx);
   if (!ctx || rep < 0 || rep >= ctx->nrep) return MSC_E_ARG;
   ctx->reps[rep].dt = dt;
   int rc = upload_reps(ctx);
@@ -1418,6 +1482,7 @@ int msc_moveP(msc_ctx *ctx, int rep, double dt, int last, double K[2], double PT
   State s = sel(ctx, rep);
   launch_moveP(s, last != 0, ctx->d_th, ctx->d_task_out, ctx->stream);
   launch_reduce(s, ctx->d_task_out, 16, 8, 0, ctx->d_red, kRedStride, 0, ctx->stream);
+  FINISH("msc_moveP");
   double v[16];
   if (int rc_ = d2h(ctx, v, ctx->d_red + kRedStride * rep, sizeof v)) return rc_;
   if (K) { K[0] = v[0] / 2; K[1] = v[1] / 2; }
@@ -1430,9 +1495,10 @@ int msc_moveP(msc_ctx *ctx, int rep, double dt, int last, double K[2], double PT
 }
 
 int msc_renorm(msc_ctx *ctx, int rep) {
+  SCOPE(ctx);
   if (!ctx || !rep_ok(ctx, rep)) return MSC_E_ARG;
   launch_renorm(sel(ctx, rep), ctx->stream);
-  HIPC(hipStreamSynchronize(ctx->stream));
+  FINISH("msc_renorm");
   return MSC_OK;
 }
 
@@ -1460,17 +1526,19 @@ static void forces_all(msc_ctx *ctx) {
 }
 
 int msc_init_run(msc_ctx *ctx) {
+  SCOPE(ctx);
   if (!ctx || !ctx->have_state) return MSC_E_ARG;
   if (ctx->nd != 1) return fail(ctx, MSC_E_ARG, "the step loop takes one dipole slot per particle");
+  ctx->have_lists = false;
   int rc = lists_layout(ctx);
   if (rc) return rc;
   launch_fill_int(ctx->d_flags, 4 * ctx->nrep, 0, ctx->stream);
   if ((rc = lists_build(ctx, ctx->d_one))) return rc;
-  ctx->have_lists = true;
   forces_all(ctx);
   State s = sel(ctx, MSC_ALL);
   launch_self_ener(s, ctx->d_task_out, ctx->stream);
   launch_reduce(s, ctx->d_task_out, 16, 1, 0, ctx->d_red, kRedStride, 15, ctx->stream);
+  FINISH("msc_init_run");
   std::vector<double> red(kRedStride * ctx->nrep);
   std::vector<int> fl(4 * ctx->nrep);
   if (int rc_ = d2h(ctx, red.data(), ctx->d_red, sizeof(double) * red.size())) return rc_;
@@ -1481,10 +1549,13 @@ int msc_init_run(msc_ctx *ctx) {
       return fail(ctx, MSC_E_CAPACITY, "neighbour rows longer than measured");
     if (fl[4 * r + 0]) return fail(ctx, fl[4 * r] == 1 ? MSC_E_OVERLAP : MSC_E_DISTANCE, "initial overlap");
   }
+  // rows fitted their regions and no pair overlaps: the lists hold every pair
+  ctx->have_lists = true;
   return upload_reps(ctx);
 }
 
 int msc_set_graph(msc_ctx *ctx, int on) {
+  SCOPE(ctx);
   if (!ctx) return MSC_E_ARG;
   ctx->graph_on = on != 0;
   if (!ctx->graph_on) drop_graph(ctx);
@@ -1589,6 +1660,7 @@ static int capture_step(msc_ctx *ctx) {
 }
 
 int msc_run(msc_ctx *ctx, int step0, int nsteps, double *obs) {
+  SCOPE(ctx);
   if (!ctx || nsteps < 0) return MSC_E_ARG;
   if (!ctx->have_lists) return fail(ctx, MSC_E_STATE, "call msc_init_run first");
   const int nrep = ctx->nrep;
@@ -1638,6 +1710,15 @@ int msc_run(msc_ctx *ctx, int step0, int nsteps, double *obs) {
     }
     LAUNCHED("msc_run");
     if (int rc_ = d2h(ctx, fl.data(), ctx->d_flags, sizeof(int) * fl.size())) return rc_;
+    // every step advances the device step counter (k_cells_assign): a chunk
+    // whose kernels did not all execute leaves it behind
+    int pos[2] = {-1, -1};
+    if (int rc_ = d2h(ctx, pos, ctx->d_step, sizeof pos)) return rc_;
+    if (pos[0] != k0 + nk) {
+      ctx->have_lists = false;
+      return fail(ctx, MSC_E_HIP, "msc_run: the device did not execute the step kernels (step counter " +
+                                      std::to_string(pos[0]) + ", expected " + std::to_string(k0 + nk) + ")");
+    }
     // an overflowing rebuild empties the replica's rows, so an overlap later in
     // the same chunk may be its consequence: overflow is handled first
     bool overflow = false;
@@ -1650,8 +1731,11 @@ int msc_run(msc_ctx *ctx, int step0, int nsteps, double *obs) {
           if (!code) code = fl[4 * r] == 1 ? MSC_E_OVERLAP : MSC_E_DISTANCE;
           m += " replica " + std::to_string(r) + (fl[4 * r] == 1 ? " (overlap)" : " (distance)");
         }
-      if (code) return fail(ctx, code, "force_sc abort during steps " + std::to_string(step0 + k0) + ".." +
-                                           std::to_string(step0 + k0 + nk - 1) + ":" + m);
+      if (code) {
+        ctx->have_lists = false;  // the caller restarts from msc_init_run
+        return fail(ctx, code, "force_sc abort during steps " + std::to_string(step0 + k0) + ".." +
+                                   std::to_string(step0 + k0 + nk - 1) + ":" + m);
+      }
     }
     if (overflow) {
       if (++retries > 4) {
@@ -1662,6 +1746,7 @@ int msc_run(msc_ctx *ctx, int step0, int nsteps, double *obs) {
                  std::to_string(fl[4 * r + 1]) + "/" + std::to_string(fl[4 * r + 3]) + " of " +
                  std::to_string(ctx->reps[r].nl_w) + "/" + std::to_string(ctx->reps[r].sl_w) + "/" +
                  std::to_string(ctx->reps[r].dl_w) + ";";
+        ctx->have_lists = false;  // the rows of this chunk are truncated
         return fail(ctx, MSC_E_CAPACITY, m);
       }
       ++ctx->chunk_repeats;
@@ -1682,6 +1767,7 @@ int msc_run(msc_ctx *ctx, int step0, int nsteps, double *obs) {
     }
     k0 += nk;
   }
+  FINISH("msc_run");
   if (obs && nobs)
     if (int rc_ = d2h(ctx, obs, d_obs_all, sizeof(double) * nobs)) return rc_;
   if (ctx->timing) settle_timers(ctx);
@@ -1692,17 +1778,20 @@ int msc_run(msc_ctx *ctx, int step0, int nsteps, double *obs) {
 // measurement
 // --------------------------------------------------------------------------
 int msc_set_timing(msc_ctx *ctx, int on) {
+  SCOPE(ctx);
   if (!ctx) return MSC_E_ARG;
   ctx->timing = on != 0;
   return MSC_OK;
 }
 int msc_reset_timing(msc_ctx *ctx) {
+  SCOPE(ctx);
   if (!ctx) return MSC_E_ARG;
   settle_timers(ctx);
   ctx->timers.clear();
   return MSC_OK;
 }
 int msc_kernel_time(msc_ctx *ctx, const char *name, double *ms, long *launches) {
+  SCOPE(ctx);
   if (!ctx || !name) return MSC_E_ARG;
   settle_timers(ctx);
   auto it = ctx->timers.find(name);
@@ -1711,6 +1800,7 @@ int msc_kernel_time(msc_ctx *ctx, const char *name, double *ms, long *launches) 
   return MSC_OK;
 }
 int msc_work_counts(msc_ctx *ctx, int rep, long counts[4]) {
+  SCOPE(ctx);
   if (!ctx || rep < 0 || rep >= ctx->nrep || !counts) return MSC_E_ARG;
   double c[4];
   HIPC(hipStreamSynchronize(ctx->stream));
@@ -1729,6 +1819,7 @@ int msc_work_counts(msc_ctx *ctx, int rep, long counts[4]) {
 }
 
 int msc_chunk_repeats(msc_ctx *ctx, long *n) {
+  SCOPE(ctx);
   if (!ctx || !n) return MSC_E_ARG;
   *n = ctx->chunk_repeats;
   return MSC_OK;
@@ -1792,6 +1883,7 @@ static void jacobi3_sym(double a[3][3], double d[3], double v[3][3]) {
 }
 
 int msc_order_params(msc_ctx *ctx, int rep, double *G1, double *G2, double M[3]) {
+  SCOPE(ctx);
   if (!ctx || rep < 0 || rep >= ctx->nrep || !G1 || !G2 || !M) return MSC_E_ARG;
   if (!ctx->have_state) return fail(ctx, MSC_E_STATE, "no state");
   if (!ctx->d_order)
@@ -1800,7 +1892,7 @@ int msc_order_params(msc_ctx *ctx, int rep, double *G1, double *G2, double M[3])
     Bracket b(ctx, "order_params");
     launch_order_sums(sel(ctx, rep), rep, ctx->d_order, ctx->stream);
   }
-  LAUNCHED("order_params");
+  FINISH("msc_order_params");
   double o[kOrderSums];
   if (int rc_ = d2h(ctx, o, ctx->d_order, sizeof o)) return rc_;
   const double N = ctx->reps[rep].N;
@@ -1854,6 +1946,7 @@ int msc_ini_state(int N, int nspec, const int *counts, const double L[3], const 
 }
 
 int msc_set_contact_rows(msc_ctx *ctx, int on) {
+  SCOPE(ctx);
   if (!ctx || (on != 0 && on != 1)) return MSC_E_ARG;
   if (ctx->contact != (on == 1)) {
     ctx->contact = on == 1;
@@ -1866,7 +1959,7 @@ int msc_set_contact_rows(msc_ctx *ctx, int on) {
         launch_check(s, ctx->d_task_out, ctx->d_rebuild, ctx->d_srebuild, nullptr, 0, true, ctx->stream);
         launch_short(s, ctx->d_srebuild, ctx->d_flags, ctx->stream);
       }
-      LAUNCHED("contact rows");
+      FINISH("msc_set_contact_rows");
     }
   }
   return MSC_OK;
@@ -1874,6 +1967,7 @@ int msc_set_contact_rows(msc_ctx *ctx, int on) {
 
 int msc_pair_correl(msc_ctx *ctx, int rep, int nspec, const int *counts, double bar_w, int n_bars, double *hist,
                     long *dropped) {
+  SCOPE(ctx);
   if (!ctx || rep < 0 || rep >= ctx->nrep || !counts || !hist) return MSC_E_ARG;
   if (nspec < 1 || nspec > kCorrelMaxSpec) return fail(ctx, MSC_E_ARG, "nspec outside [1, 8]");
   if (!(bar_w > 0) || n_bars < 1) return fail(ctx, MSC_E_ARG, "bar_w and n_bars must be positive");
@@ -1912,7 +2006,7 @@ int msc_pair_correl(msc_ctx *ctx, int rep, int nspec, const int *counts, double 
     Bracket b(ctx, "pair_correl");
     launch_pair_correl(ctx->d_x, ctx->d_ori, ctx->Ntot, a, ctx->d_corr, ctx->d_corr_drop, ctx->stream);
   }
-  LAUNCHED("pair_correl");
+  FINISH("msc_pair_correl");
   std::vector<double> buf(nh);
   unsigned long long dr = 0;
   if (int rc_ = d2h(ctx, buf.data(), ctx->d_corr, sizeof(double) * nh)) return rc_;
@@ -1924,6 +2018,7 @@ int msc_pair_correl(msc_ctx *ctx, int rep, int nspec, const int *counts, double 
 
 int msc_cluster_labels(msc_ctx *ctx, int rep, double min_s, unsigned int *mt_state, int *mt_index, int *labels,
                        int *nclust, long *n_asym) {
+  SCOPE(ctx);
   if (!ctx || rep < 0 || rep >= ctx->nrep || !(min_s >= 0) || !mt_state || !mt_index || !labels || !nclust)
     return MSC_E_ARG;
   if (*mt_index < 0 || *mt_index > Mt19937::kN) return fail(ctx, MSC_E_ARG, "mt_index outside [0, 624]");
@@ -1938,7 +2033,7 @@ int msc_cluster_labels(msc_ctx *ctx, int rep, double min_s, unsigned int *mt_sta
     Bracket b(ctx, "clusters");
     launch_clusters(s, min_s, reach_max, ctx->d_clu, ctx->d_clu_flag, ctx->stream);
   }
-  LAUNCHED("clusters");
+  FINISH("msc_cluster_labels");
   std::vector<int> root(R.N);
   int fl[2];
   if (int rc_ = d2h(ctx, root.data(), ctx->d_clu + R.n0, sizeof(int) * R.N)) return rc_;

@@ -37,26 +37,46 @@ static void sync_announce(const char *kernel) {
   std::fflush(stderr);
 }
 // The runtime's last-error slot is overwritten by every later call, so each
-// launch's own status is kept here: the first failure since the last
-// take_launch_error(), with the kernel's name.
-static std::mutex g_launch_mu;
-static hipError_t g_launch_err = hipSuccess;
-static const char *g_launch_kernel = "";
+// launch's own status is kept in a launch record: the first failure since the
+// last take_launch_error(), with the kernel's name.  Each context owns a
+// record; its entry points make it the current record of the calling host
+// thread for their duration, so a failure is reported by the context whose
+// launch failed.  Launches outside any entry point share one record.
+static std::mutex g_shared_mu;
+static LaunchRecord g_shared_record;
+static thread_local LaunchRecord *t_record = nullptr;
+LaunchRecord *set_launch_record(LaunchRecord *r) {
+  LaunchRecord *prev = t_record;
+  t_record = r;
+  return prev;
+}
 static void note_launch(const char *kernel) {
   const hipError_t e = hipGetLastError();
   if (e == hipSuccess) return;
-  std::lock_guard<std::mutex> lk(g_launch_mu);
-  if (g_launch_err == hipSuccess) {
-    g_launch_err = e;
-    g_launch_kernel = kernel;
+  if (t_record) {
+    if (t_record->err == hipSuccess) {
+      t_record->err = e;
+      t_record->kernel = kernel;
+    }
+    return;
+  }
+  std::lock_guard<std::mutex> lk(g_shared_mu);
+  if (g_shared_record.err == hipSuccess) {
+    g_shared_record.err = e;
+    g_shared_record.kernel = kernel;
   }
 }
 hipError_t take_launch_error(const char **kernel) {
-  std::lock_guard<std::mutex> lk(g_launch_mu);
-  const hipError_t e = g_launch_err;
-  *kernel = g_launch_kernel;
-  g_launch_err = hipSuccess;
-  g_launch_kernel = "";
+  LaunchRecord *r = t_record;
+  std::unique_lock<std::mutex> lk(g_shared_mu, std::defer_lock);
+  if (!r) {
+    lk.lock();
+    r = &g_shared_record;
+  }
+  const hipError_t e = r->err;
+  *kernel = r->kernel;
+  r->err = hipSuccess;
+  r->kernel = "";
   return e;
 }
 #define MSC_LAUNCH(K, ...)          \
@@ -2410,6 +2430,12 @@ __global__ void __launch_bounds__(256) k_end(State s, Thermo *__restrict__ th, c
     }
 }
 
+// completion stamp: written after an entry point's kernels, read back by the
+// host to prove they executed
+__global__ void k_stamp(double *__restrict__ p, double v) {
+  if (threadIdx.x == 0) p[0] = v;
+}
+
 // fill kernel (graph-safe replacement for memset nodes)
 __global__ void k_fill_int(int *__restrict__ p, int n, int v) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -2614,6 +2640,7 @@ void launch_end(const State &s, Thermo *th, const double *gs, const int *step, c
   MSC_LAUNCH(k_end, dim3(s.rep_end - s.rep_begin), dim3(256), 0, st, s, th, gs, step, rows, region,
              has_field ? 1 : 0, kblk, nblk(maxslot, 64), E, Kin, counts, obs, rebuild, srebuild);
 }
+void launch_stamp(double *p, double v, hipStream_t st) { MSC_LAUNCH(k_stamp, dim3(1), dim3(64), 0, st, p, v); }
 void launch_fill_int(int *p, int n, int v, hipStream_t st) {
   MSC_LAUNCH(k_fill_int, dim3(nblk(n, 256)), dim3(256), 0, st, p, n, v);
 }

@@ -39,8 +39,16 @@ struct State {
   const double *kval;
 };
 
+// First failed kernel launch since the record was last taken.
+struct LaunchRecord {
+  hipError_t err = hipSuccess;
+  const char *kernel = "";
+};
+// Makes r the record that launches from the calling host thread report into
+// (null: the shared record); returns the previous one.
+LaunchRecord *set_launch_record(LaunchRecord *r);
 // first kernel launch that failed since the last call (hipSuccess if none),
-// with the kernel's name; clears the record
+// with the kernel's name, from the calling thread's current record; clears it
 hipError_t take_launch_error(const char **kernel);
 // MSC_SYNC_CHECK is set: every launch is followed by a synchronisation
 bool launches_checked();
@@ -83,6 +91,8 @@ void launch_step_scalars(const State &s, int mode, const double *red, double *Ki
                          double *obs, hipStream_t st);
 void launch_zero_FT(const State &s, hipStream_t st);
 void launch_fill_int(int *p, int n, int v, hipStream_t st);
+// one-lane kernel writing v to p[0] (completion stamp)
+void launch_stamp(double *p, double v, hipStream_t st);
 // the step loop's fused kernels; rows + k*region is per-task row set k:
 // 0 kick1, 1 drift, 2 sc, 3 R, 4 ext, 5 kick2 (16 doubles per task)
 constexpr int kRowKick1 = 0, kRowDrift = 1, kRowSC = 2, kRowR = 3, kRowExt = 4, kRowKick2 = 5, kRowSets = 6;
