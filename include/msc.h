@@ -237,6 +237,19 @@ int msc_order_params(msc_ctx *ctx, int rep, double *G1, double *G2, double M[3])
 int msc_pair_correl(msc_ctx *ctx, int rep, int nspec, const int *counts, double bar_w, int n_bars,
                     double *hist, long *dropped);
 
+/* msc_pair_correl without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, checks and results as msc_pair_correl, but the histogram
+ * scratch is a fixed device array of MSC_NOALLOC_CORREL_MAX doubles (2 MiB)
+ * that the library holds on each device it runs on until the process ends;
+ * sets * 4 * n_bars above that limit is MSC_E_CAPACITY.  The array is shared
+ * by every call on one device and keeps the last call's contents (each call
+ * clears what it uses first): calls that overlap in time on one device race. */
+#define MSC_NOALLOC_CORREL_MAX 262144
+int msc_pair_correl_noalloc(msc_ctx *ctx, int rep, int nspec, const int *counts, double bar_w, int n_bars,
+                            double *hist, long *dropped);
+
 /* ---- conf.dat (print_conf / chrg_conf, v25/functions.cpp:1181-1296) ----
  * Host-only reader and writer of the reference's configuration file, so
  * that states pass between the reference and this library.  Arrays are

@@ -25,6 +25,7 @@ EXPORTS = [
     "msc_init_run", "msc_run", "msc_set_graph", "msc_cluster_labels", "msc_set_timing",
     "msc_kernel_time", "msc_reset_timing", "msc_work_counts", "msc_chunk_repeats", "msc_write_conf",
     "msc_read_conf", "msc_pair_correl", "msc_order_params", "msc_ini_state", "msc_set_contact_rows",
+    "msc_pair_correl_noalloc",
 ]
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -94,6 +95,8 @@ def load():
         "msc_order_params": ([vp, ctypes.c_int, _dp, _dp, _dp], ctypes.c_int),
         "msc_pair_correl": ([vp, ctypes.c_int, ctypes.c_int, _ip, ctypes.c_double, ctypes.c_int, _dp,
                              ctypes.POINTER(ctypes.c_long)], ctypes.c_int),
+        "msc_pair_correl_noalloc": ([vp, ctypes.c_int, ctypes.c_int, _ip, ctypes.c_double, ctypes.c_int, _dp,
+                                     ctypes.POINTER(ctypes.c_long)], ctypes.c_int),
         "msc_set_timing": ([vp, ctypes.c_int], ctypes.c_int),
         "msc_kernel_time": ([vp, ctypes.c_char_p, _dp, ctypes.POINTER(ctypes.c_long)], ctypes.c_int),
         "msc_reset_timing": ([vp], ctypes.c_int),

@@ -1965,10 +1965,11 @@ int msc_set_contact_rows(msc_ctx *ctx, int on) {
   return MSC_OK;
 }
 
-int msc_pair_correl(msc_ctx *ctx, int rep, int nspec, const int *counts, double bar_w, int n_bars, double *hist,
-                    long *dropped) {
-  SCOPE(ctx);
-  if (!ctx || rep < 0 || rep >= ctx->nrep || !counts || !hist) return MSC_E_ARG;
+}  // extern "C"
+
+// Arguments of k_pair_correl for replica rep (checks of msc_pair_correl).
+static int correl_args(msc_ctx *ctx, int rep, int nspec, const int *counts, double bar_w, int n_bars,
+                       CorrelArgs *out) {
   if (nspec < 1 || nspec > kCorrelMaxSpec) return fail(ctx, MSC_E_ARG, "nspec outside [1, 8]");
   if (!(bar_w > 0) || n_bars < 1) return fail(ctx, MSC_E_ARG, "bar_w and n_bars must be positive");
   if (!ctx->have_state) return fail(ctx, MSC_E_STATE, "no state");
@@ -1993,6 +1994,60 @@ int msc_pair_correl(msc_ctx *ctx, int rep, int nspec, const int *counts, double 
   a.inv_bw = 1.0 / bar_w;
   a.n_bars = n_bars;
   a.nbr = kCorrelLds / (4 * a.sets);
+  *out = a;
+  return MSC_OK;
+}
+
+// dst[i] += device src[i], staged through the pinned buffer (no allocation)
+static int d2h_add(msc_ctx *ctx, double *dst, const double *src, size_t n) {
+  LAUNCHED("a device read-back");
+  const size_t per = kPinBytes / sizeof(double);
+  const double *pin = (const double *)ctx->pin;
+  for (size_t off = 0; off < n; off += per) {
+    const size_t m = std::min(per, n - off);
+    HIPC(hipMemcpyAsync(ctx->pin, src + off, m * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPC(hipStreamSynchronize(ctx->stream));
+    for (size_t i = 0; i < m; ++i) dst[off + i] += pin[i];
+  }
+  return MSC_OK;
+}
+
+// Scratch of msc_pair_correl_noalloc: element 0 holds the dropped-pair
+// counter (a 64-bit integer), the histograms start at element
+// kNoAllocHistOff (256 bytes in).  One copy per device, shared by every call
+// on that device; its address is resolved once per device (noalloc_scratch).
+static constexpr size_t kNoAllocHistOff = 32;
+static __device__ __align__(256) double scratch[kNoAllocHistOff + MSC_NOALLOC_CORREL_MAX];
+
+// Device address of `scratch` on the current device, resolved by
+// hipGetSymbolAddress at the first call on each device and kept.  Host code
+// uses only this address: the array's own name is a host-side stand-in.
+static int noalloc_scratch(msc_ctx *ctx, double **out) {
+  static std::mutex mu;
+  static std::map<int, double *> addr;
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = addr.find(ctx->device);
+  if (it != addr.end()) {
+    *out = it->second;
+    return MSC_OK;
+  }
+  HIPC(hipSetDevice(ctx->device));
+  void *p = nullptr;
+  HIPC(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  if (!p) return fail(ctx, MSC_E_HIP, "hipGetSymbolAddress returned no address for the scratch array");
+  addr[ctx->device] = (double *)p;
+  *out = (double *)p;
+  return MSC_OK;
+}
+
+extern "C" {
+
+int msc_pair_correl(msc_ctx *ctx, int rep, int nspec, const int *counts, double bar_w, int n_bars, double *hist,
+                    long *dropped) {
+  SCOPE(ctx);
+  if (!ctx || rep < 0 || rep >= ctx->nrep || !counts || !hist) return MSC_E_ARG;
+  CorrelArgs a;
+  if (int rc_ = correl_args(ctx, rep, nspec, counts, bar_w, n_bars, &a)) return rc_;
   const size_t nh = (size_t)a.sets * 4 * n_bars;
   if (nh > ctx->corr_cap) {
     if (int rc_ = dalloc(ctx, &ctx->d_corr, nh)) return rc_;
@@ -2002,16 +2057,41 @@ int msc_pair_correl(msc_ctx *ctx, int rep, int nspec, const int *counts, double 
     if (int rc_ = dalloc(ctx, &ctx->d_corr_drop, 1)) return rc_;
   HIPC(hipMemsetAsync(ctx->d_corr, 0, nh * sizeof(double), ctx->stream));
   HIPC(hipMemsetAsync(ctx->d_corr_drop, 0, sizeof(unsigned long long), ctx->stream));
-  if (R.N >= 2) {
+  if (a.N >= 2) {
     Bracket b(ctx, "pair_correl");
     launch_pair_correl(ctx->d_x, ctx->d_ori, ctx->Ntot, a, ctx->d_corr, ctx->d_corr_drop, ctx->stream);
   }
   FINISH("msc_pair_correl");
-  std::vector<double> buf(nh);
   unsigned long long dr = 0;
-  if (int rc_ = d2h(ctx, buf.data(), ctx->d_corr, sizeof(double) * nh)) return rc_;
+  if (int rc_ = d2h_add(ctx, hist, ctx->d_corr, nh)) return rc_;
   if (int rc_ = d2h(ctx, &dr, ctx->d_corr_drop, sizeof dr)) return rc_;
-  for (size_t i = 0; i < nh; ++i) hist[i] += buf[i];
+  if (dropped) *dropped = (long)dr;
+  return MSC_OK;
+}
+
+int msc_pair_correl_noalloc(msc_ctx *ctx, int rep, int nspec, const int *counts, double bar_w, int n_bars,
+                            double *hist, long *dropped) {
+  SCOPE(ctx);
+  if (!ctx || rep < 0 || rep >= ctx->nrep || !counts || !hist) return MSC_E_ARG;
+  CorrelArgs a;
+  if (int rc_ = correl_args(ctx, rep, nspec, counts, bar_w, n_bars, &a)) return rc_;
+  const size_t nh = (size_t)a.sets * 4 * n_bars;
+  if (nh > (size_t)MSC_NOALLOC_CORREL_MAX)
+    return fail(ctx, MSC_E_CAPACITY, "msc_pair_correl_noalloc: sets * 4 * n_bars = " + std::to_string(nh) +
+                                         " exceeds MSC_NOALLOC_CORREL_MAX = " +
+                                         std::to_string(MSC_NOALLOC_CORREL_MAX));
+  double *base = nullptr;
+  if (int rc_ = noalloc_scratch(ctx, &base)) return rc_;
+  // the previous call's contents stay in the array: clear the counter and the
+  // histograms this call uses with a fill kernel
+  launch_fill_int((int *)base, (int)(2 * (kNoAllocHistOff + nh)), 0, ctx->stream);
+  unsigned long long *d_drop = (unsigned long long *)base;
+  double *d_hist = base + kNoAllocHistOff;
+  if (a.N >= 2) launch_pair_correl(ctx->d_x, ctx->d_ori, ctx->Ntot, a, d_hist, d_drop, ctx->stream);
+  FINISH("msc_pair_correl_noalloc");
+  unsigned long long dr = 0;
+  if (int rc_ = d2h_add(ctx, hist, d_hist, nh)) return rc_;
+  if (int rc_ = d2h(ctx, &dr, d_drop, sizeof dr)) return rc_;
   if (dropped) *dropped = (long)dr;
   return MSC_OK;
 }
