@@ -1071,41 +1071,98 @@ static void zero_sum(double *pp, int Ntot, double pr) {
   pp[Ntot - 1] = -P;
 }
 
+/* ini_mom's per-species scratch (v25/functions.cpp:683-689, 915-920): for
+ * each species n and component j an array of N_n + 1 doubles, allocated
+ * species-major (n outer, j inner).  The copy into the global list reads
+ * rnd[j][n][cnt] with the GLOBAL counter cnt (v25/functions.cpp:717-726,
+ * 945-954), i.e. past the end of species n's array for every n > 0.  In the
+ * reference's build those arrays are consecutive heap chunks of
+ * 8 (N_n + 1) + 8 bytes rounded up to 16, each chunk's size word (size | 1,
+ * "previous chunk in use") sitting in the 8 bytes after the array, and the
+ * memory behind the last array is untouched (zero).  The reads therefore
+ * land on the size word and on the following arrays' values; this image
+ * lays the arrays out that way so that the same reads return the same
+ * values.  The word after the last array (the heap's free-space size) is
+ * taken as zero: it is read as a denormal of magnitude below 1e-300, which
+ * no printed value can see.  Pinned by v25/output.dat lines 1220-1880
+ * (two species of 50, every value equal at 12 decimals). */
+typedef struct heap_image {
+  double *img;           /* arrays and size words, then Ntot zeros */
+  long start[OSYS_MAXSPEC][3];
+} heap_image;
+
+static void heap_image_make(heap_image *H, const osys *S) {
+  long tot = 0;
+  for (int n = 0; n < S->nspec; ++n) tot += 3L * (S->counts[n] + 2);
+  H->img = (double *)calloc((size_t)(tot + S->N + 1), sizeof(double));
+  long at = 0;
+  for (int n = 0; n < S->nspec; ++n)
+    for (int j = 0; j < 3; ++j) {
+      H->start[n][j] = at;
+      at += S->counts[n] + 1;
+      const int last = (n == S->nspec - 1 && j == 2);
+      const uint64_t chunk = ((uint64_t)(8 * (S->counts[n] + 1) + 8 + 15) & ~(uint64_t)15) | 1u;
+      double word = 0.0;
+      if (!last) memcpy(&word, &chunk, sizeof word);
+      H->img[at++] = word;
+    }
+}
+
+/* Maxwell draws of one kind (momenta or angular momenta) into the image,
+ * species by species as v25/functions.cpp:690-710 and 921-941 */
+static void ini_mom_draw(heap_image *H, const osys *S, omt *ran, const double *var /* [nspec][3] */) {
+  for (int n = 0; n < S->nspec; ++n)
+    for (int i = 0; i < S->counts[n]; ++i)
+      for (int j = 0; j < 3; ++j) H->img[H->start[n][j] + i] = omt_uniform(ran);
+  for (int n = 0; n < S->nspec; ++n) {
+    const int NN = S->counts[n];
+    for (int i = 0; i < NN; i += 2) {
+      if ((fmodf((float)NN, 2) != 0.) && (i == NN - 1)) {
+        for (int j = 0; j < 3; j++) {
+          double rr = omt_uniform(ran);
+          o_gauss(var[3 * n + j], &rr, &H->img[H->start[n][j] + i]);
+        }
+      } else {
+        for (int j = 0; j < 3; j++) o_gauss(var[3 * n + j], &H->img[H->start[n][j] + i], &H->img[H->start[n][j] + i + 1]);
+      }
+    }
+  }
+}
+
 int o_ini_mom(osys *S, double temp, omt *ran) {
-  if (S->nspec != 1) return -1;
-  int NN = S->N;
-  double *rnd[3], *ond[3];
+  const int Ntot = S->N;
+  double var[3 * OSYS_MAXSPEC];
+  double *pp[3], *tt[3];
   for (int j = 0; j < 3; ++j) {
-    rnd[j] = (double *)calloc((size_t)NN + 1, sizeof(double));
-    ond[j] = (double *)calloc((size_t)NN + 1, sizeof(double));
+    pp[j] = (double *)calloc((size_t)Ntot + 1, sizeof(double));
+    tt[j] = (double *)calloc((size_t)Ntot + 1, sizeof(double));
   }
-  for (int i = 0; i < NN; ++i)
-    for (int j = 0; j < 3; ++j) rnd[j][i] = omt_uniform(ran);
-  double mtemp = temp * S->mass[0];
-  for (int i = 0; i < NN; i += 2) {
-    if ((fmodf((float)NN, 2) != 0.) && (i == NN - 1)) {
-      for (int j = 0; j < 3; j++) { double rr = omt_uniform(ran); o_gauss(mtemp, &rr, &rnd[j][i]); }
-    } else {
-      for (int j = 0; j < 3; j++) o_gauss(mtemp, &rnd[j][i], &rnd[j][i + 1]);
-    }
-  }
-  for (int j = 0; j < 3; ++j) zero_sum(rnd[j], NN, 0.1);
-  for (int i = 0; i < NN; ++i)
-    for (int j = 0; j < 3; ++j) S->p[3 * i + j] = rnd[j][i];
-  for (int i = 0; i < NN; ++i)
-    for (int j = 0; j < 3; ++j) ond[j][i] = omt_uniform(ran);
-  double mt3[3] = {temp * S->inert[0], temp * S->inert[1], temp * S->inert[2]};
-  for (int i = 0; i < NN; i += 2) {
-    if ((fmodf((float)NN, 2) != 0.) && (i == NN - 1)) {
-      for (int j = 0; j < 3; j++) { double rr = omt_uniform(ran); o_gauss(mt3[j], &rr, &ond[j][i]); }
-    } else {
-      for (int j = 0; j < 3; j++) o_gauss(mt3[j], &ond[j][i], &ond[j][i + 1]);
-    }
-  }
-  for (int j = 0; j < 3; ++j) zero_sum(ond[j], NN, 0.1);
-  for (int i = 0; i < NN; ++i)
-    for (int j = 0; j < 3; ++j) S->l[3 * i + j] = ond[j][i];
-  for (int j = 0; j < 3; ++j) { free(rnd[j]); free(ond[j]); }
+  /* translation: variance temp * mass of the species' first particle */
+  heap_image H;
+  heap_image_make(&H, S);
+  for (int n = 0, b = 0; n < S->nspec; b += S->counts[n], ++n)
+    for (int j = 0; j < 3; ++j) var[3 * n + j] = temp * S->mass[b];
+  ini_mom_draw(&H, S, ran, var);
+  for (int n = 0, cnt = 0; n < S->nspec; ++n)
+    for (int i = 0; i < S->counts[n]; ++i, ++cnt)
+      for (int j = 0; j < 3; ++j) pp[j][cnt] = H.img[H.start[n][j] + cnt];
+  free(H.img);
+  for (int j = 0; j < 3; ++j) zero_sum(pp[j], Ntot, 0.1);
+  for (int i = 0; i < Ntot; ++i)
+    for (int j = 0; j < 3; ++j) S->p[3 * i + j] = pp[j][i];
+  /* rotation: variance temp * I_j of the species' first particle */
+  heap_image_make(&H, S);
+  for (int n = 0, b = 0; n < S->nspec; b += S->counts[n], ++n)
+    for (int j = 0; j < 3; ++j) var[3 * n + j] = temp * S->inert[3 * b + j];
+  ini_mom_draw(&H, S, ran, var);
+  for (int n = 0, cnt = 0; n < S->nspec; ++n)
+    for (int i = 0; i < S->counts[n]; ++i, ++cnt)
+      for (int j = 0; j < 3; ++j) tt[j][cnt] = H.img[H.start[n][j] + cnt];
+  free(H.img);
+  for (int j = 0; j < 3; ++j) zero_sum(tt[j], Ntot, 0.1);
+  for (int i = 0; i < Ntot; ++i)
+    for (int j = 0; j < 3; ++j) S->l[3 * i + j] = tt[j][i];
+  for (int j = 0; j < 3; ++j) { free(pp[j]); free(tt[j]); }
   return 0;
 }
 
@@ -1453,6 +1510,11 @@ int o_run_steps(osys *S, const orun_params *P, orun *R, int step0, int nsteps, d
     double Pss = Wss / (3 * V), PdipR = WdipR / (3 * V), PdipK = WdipK / (3 * V);
     double ctemp = K / 3;
     double Pr = ctemp * P->dens + Pss + PdipR + PdipK;
+    /* each part divided by V, then summed in this order (v25/v251/main.cpp:635-642) */
+    for (int q = 0; q < 9; ++q) {
+      double kin = PTkin[q] / V, ss = e[5 + q] / V, dR = e[14 + q] / V, dK = e[23 + q] / V;
+      R->PT[q] = kin + ss + dR + dK;
+    }
     int reb = o_check_list(S, R->snap, R->r_list_min, R->r_list_max);
     if (reb) list_build(S, R);
     if (P->cl_fr > 0 && fmod((double)step, P->cl_fr) == 0) o_clust_conf(S, P->min_s, &R->ran);

@@ -140,6 +140,9 @@ typedef struct orun {
   double dmax, dmin;
   otherm th[2];
   omt ran;
+  double PT[9];             /* last step's total pressure tensor
+                               (PTkin + PTss + PTdipR + PTdipK) / V,
+                               v25/v251/main.cpp:635-642 */
 } orun;
 
 int  o_run_init(osys *s, const orun_params *P, orun *R, int seed_init);

@@ -53,7 +53,7 @@ class CRun(ctypes.Structure):
                 ("off", _ip), ("nbr", _ip), ("cap", ctypes.c_long), ("snap", _dp),
                 ("r_list_min", ctypes.c_double), ("r_list_max", ctypes.c_double),
                 ("Uself", ctypes.c_double), ("dmax", ctypes.c_double), ("dmin", ctypes.c_double),
-                ("th", COtherm * 2), ("ran", COmt)]
+                ("th", COtherm * 2), ("ran", COmt), ("PT", ctypes.c_double * 9)]
 
 
 _lib = None

@@ -17,6 +17,14 @@ nowhere else); the outputs are data files committed next to this script:
   v251_run1_counters.txt  counters.dat after step 100 (sums over the steps)
   v25_run1_nve.txt        time, K and U columns of v25/run1/output.dat for the
                           two NVE segments used as drift statistics
+  v251_run1_ptens.txt     ptens.dat of v25/v251/run1 (time, Helfand moment
+                          sum r_a p_b, total pressure tensor; every step)
+  v25_input.txt           v25/input.txt (two species of 50 short rods, two
+                          off-centre dipoles each, K=10, Signal=0)
+  v25_output_seg.txt      v25/output.dat lines 1220-1880: the 661 printed
+                          steps of one uninterrupted run of v25/input.txt
+                          (time and the 11 observables; the two timing
+                          columns are dropped)
 
 All of these are outputs the reference itself wrote; nothing here is source.
 """
@@ -41,7 +49,7 @@ def main():
     open(os.path.join(HERE, "v251_run1_output.txt"), "w").writelines(out)
     for src, dst in [("conf.dat", "v251_run1_conf.txt"), ("compConf.dat", "v251_run1_compConf.txt"),
                      ("iniPos.dat", "v251_run1_iniPos.txt"), ("magmom.dat", "v251_run1_magmom.txt"),
-                     ("counters.dat", "v251_run1_counters.txt")]:
+                     ("counters.dat", "v251_run1_counters.txt"), ("ptens.dat", "v251_run1_ptens.txt")]:
         open(os.path.join(HERE, dst), "w").write(open(os.path.join(run, src)).read())
     with open(os.path.join(run, "correlation.dat"), "rb") as f:
         data = f.read()
@@ -51,6 +59,12 @@ def main():
         for n in range(51):
             f.write("# file %d\n" % n)
             f.write(open(os.path.join(run, "pos_vel", "pos_vel_%d.dat" % n)).read())
+    open(os.path.join(HERE, "v25_input.txt"), "w").write(open(os.path.join(REF, "input.txt")).read())
+    lines = open(os.path.join(REF, "output.dat")).read().splitlines()[1219:1880]
+    with open(os.path.join(HERE, "v25_output_seg.txt"), "w") as f:
+        for line in lines:
+            head, tail = line.split("|")
+            f.write("%s | %s\n" % (head.split()[0], " ".join(tail.split())))
     # v25/run1: N=100 dipolar spheres, Q=1e12, gamma=0 (NVE).  The segments
     # are the runs bounded by restarts of the time column.
     rows = []

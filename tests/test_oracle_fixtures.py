@@ -68,6 +68,18 @@ def posvel_line(S, i):
     return " ".join(f12(v) for v in vals) + " | " + " ".join(f12(v) for v in vals2)
 
 
+def helfand(S):
+    """print_ptens's Helfand sum (v25/v251/functions.cpp:2646-2665):
+    sum over particles of r_a p_b, summed in particle order."""
+    h = [[0.0] * 3 for _ in range(3)]
+    for i in range(S.N):
+        r, p = S.r[i].tolist(), S.p[i].tolist()
+        for a in range(3):
+            for b in range(3):
+                h[a][b] += r[a] * p[b]
+    return [v for row in h for v in row]
+
+
 @pytest.fixture(scope="module")
 def v251_run():
     S, P = v251_system()
@@ -77,7 +89,7 @@ def v251_run():
     nb = int(math.floor(math.sqrt(3.) * S.L.min() / (2. * 0.01)))
     hist = np.zeros((1, 4, nb))
     dropped = 0
-    oo = []
+    oo, pt = [], []
     for step in range(101):
         obs[step] = R.run(1)[0]
         if step % 2 == 0:
@@ -87,8 +99,10 @@ def v251_run():
         g1, g2, M = ctypes.c_double(), ctypes.c_double(), np.zeros(3)
         O.lib().o_oo_params(S.c(), ctypes.byref(g1), ctypes.byref(g2), O.dptr(M))
         oo.append((g1.value, g2.value, M))
+        pt.append((helfand(S), list(R.R.PT)))
     S.correl = (hist, dropped)
     S.oo = oo
+    S.pt = pt
     yield S, P, R, obs, posvel
     R.free()
 
@@ -251,3 +265,74 @@ def test_jacobi3_closed_forms():
     d, v = np.zeros(3, np.float32), np.zeros((3, 3), np.float32)
     assert lib.o_jacobi3(a.ctypes.data_as(fp), d.ctypes.data_as(fp), v.ctypes.data_as(fp)) == 0
     assert list(d) == [0.5, -0.25, -0.25] and np.array_equal(v, np.eye(3, dtype=np.float32))
+
+
+def test_ptens_dat(v251_run):
+    """ptens.dat (tcfv = 1: every step): the Helfand sum r_a p_b and the total
+    pressure tensor (PTkin + PTss + PTdipR + PTdipK) / V of
+    v25/v251/main.cpp:635-642, including the reciprocal part's
+    a4 = 1/(4 alpha) (v25/functions.cpp:1715, 1738), equal the file's 18
+    values at its 12 decimals.  The Helfand sums reach 764, where the 12th
+    decimal is eight units of the last place of a double: a value may sit one
+    printed unit off there."""
+    S = v251_run[0]
+    lines = open(os.path.join(GOLD, "v251_run1_ptens.txt")).read().splitlines()
+    assert len(lines) == 101 == len(S.pt)
+    exact = 0
+    for k, line in enumerate(lines):
+        ref = line.split()[1:]
+        assert len(ref) == 18
+        hf, pt = S.pt[k]
+        mine = [f12(v) for v in hf + pt]
+        exact += mine == ref
+        # the tensor part exactly; the Helfand part within one printed unit
+        assert mine[9:] == ref[9:], "pressure tensor of step %d" % k
+        assert np.all(np.abs(np.array(hf) - np.array([float(x) for x in ref[:9]])) <= 1.01e-12), k
+    assert exact >= 99
+
+
+def test_v25_two_species_two_dipole_trajectory():
+    """v25/input.txt from its Signal=0 start: two species of 50 rods (L = D =
+    1), each with two off-centre dipoles (at -0.5 and +0.5 along the axis,
+    tilted in the body frame, moments 1.5 and 2), K = 10, alpha = 0.75,
+    NHL thermostat.  All 661 steps of the run printed at v25/output.dat
+    lines 1220-1880 (K U P Uss Pss UdipR PdipR UdipK PdipK |P|^2 |L|^2)
+    equal the oracle's at 12 decimals.  This pins the lever-arm terms of the
+    real-space and reciprocal sums, the reciprocal intra-particle exclusion
+    over two dipole slots, the per-species inertia and ini_pos / GCAmix /
+    ini_mom for two species (including ini_mom's reads past its per-species
+    arrays, v25/functions.cpp:717-726)."""
+    from mdspherocyl_amd import refio
+    inp = refio.read_input(os.path.join(GOLD, "v25_input.txt"))
+    assert inp["signal"] == 0 and inp["n_part"] == [50, 50]
+    N = sum(inp["n_part"])
+    Lb = math.pow(N / (inp["ArLx"] * inp["ArLy"] * inp["dens"]), 1.0 / 3)
+    nd = len(inp["dip"][0])
+    S = O.OSys(N, nd, L=(inp["ArLx"] * Lb, inp["ArLy"] * Lb, Lb), counts=inp["n_part"])
+    b = 0
+    for n, NN in enumerate(inp["n_part"]):
+        S.len[b:b + NN] = inp["length"][n]
+        S.dia[b:b + NN] = inp["diameter"][n]
+        S.mass[b:b + NN] = inp["mass"][n]
+        for w, (pos, ori, mu) in enumerate(inp["dip"][n]):
+            S.dpc[b:b + NN, w] = pos
+            S.doc[b:b + NN, w] = ori
+            S.dmu[b:b + NN, w] = mu
+        b += NN
+    S.set_inertia()
+    P = O.run_params(dens=inp["dens"], temp=inp["temp"], dt=inp["Dt"], sr_cut=inp["sr_cut"], alpha=inp["alpha"],
+                     min_s=inp["min_s"], Q=inp["Q"], gamma=inp["gamma"], s0=0.0, B=tuple(inp["B"]),
+                     K=inp["K_max"], cl_fr=inp["cl_fr"])
+    R = O.ORun(S, P, seed_init=True)
+    try:
+        obs = R.run(661)
+    finally:
+        R.free()
+    lines = open(os.path.join(GOLD, "v25_output_seg.txt")).read().splitlines()
+    assert len(lines) == 661
+    for k, line in enumerate(lines):
+        head, tail = line.split("|")
+        # the time column restarts after the pre_steps equilibration steps
+        t = (k + 1 if k < inp["pre_steps"] else k + 1 - inp["pre_steps"]) * inp["Dt"]
+        assert abs(float(head.split()[0].rstrip(".")) - t) < 1e-12
+        assert [f12(v) for v in obs[k, :11]] == tail.split(), "step %d" % (k + 1)
