@@ -177,6 +177,15 @@ int msc_chunk_repeats(msc_ctx *ctx, long *n);
  * rows to hold every contact. */
 int msc_cluster_labels(msc_ctx *ctx, int rep, double min_s, unsigned int *mt_state,
                        int *mt_index, int *labels, int *nclust, long *n_asym);
+/* msc_cluster_labels with a dipolar contact criterion (no reference
+ * counterpart; the reference's test is purely geometric): a pair is in
+ * contact when dist_test holds AND its bare dipole-dipole energy, summed
+ * over the dipole slots of both particles at the minimum image of each site
+ * separation, mu_a mu_b (u_a.u_b r^2 - 3 (u_a.r)(u_b.r)) / r^5, is below
+ * u_cut (e.g. -1.0 times the temperature for "below -1 kT").  u_cut = +inf
+ * is msc_cluster_labels; NaN or -inf is MSC_E_ARG. */
+int msc_cluster_labels_dd(msc_ctx *ctx, int rep, double min_s, double u_cut, unsigned int *mt_state,
+                          int *mt_index, int *labels, int *nclust, long *n_asym);
 
 /* Which neighbour rows force_sc (and the WCA launch of msc_run) walks.
  * on = 0: every pair of the reference rows, as force_sc does
@@ -268,6 +277,15 @@ int msc_write_conf(const char *path, int N, int nd, const double L[3], const dou
 int msc_read_conf(const char *path, int N, int nd, double L[3], double *r, double *p,
                   double *ori, double *l, double *mass, double *dpos, double *dori, double *dmu,
                   int *clust, double thermo[4]);
+
+/* ---- synthetic initial states (no reference counterpart) --------------
+ * Random sequential addition of N rods of lengths len[] into a periodic
+ * cubic box of edge `box` centred on the origin: r[3N] centres, u[3N] unit
+ * axes, every pair at segment distance >= dmin.  Deterministic for a given
+ * seed (xoshiro256** via splitmix64).  Host only.  MSC_E_CAPACITY: the box
+ * could not be filled. */
+int msc_place_rods(int N, double box, const double *len, double dmin, unsigned long long seed, double *r,
+                   double *u);
 
 /* ---- measurement -------------------------------------------------------
  * With timing on, the library brackets each launch of the named kernels with

@@ -25,7 +25,7 @@ EXPORTS = [
     "msc_init_run", "msc_run", "msc_set_graph", "msc_cluster_labels", "msc_set_timing",
     "msc_kernel_time", "msc_reset_timing", "msc_work_counts", "msc_chunk_repeats", "msc_write_conf",
     "msc_read_conf", "msc_pair_correl", "msc_order_params", "msc_ini_state", "msc_set_contact_rows",
-    "msc_pair_correl_noalloc",
+    "msc_pair_correl_noalloc", "msc_place_rods", "msc_cluster_labels_dd",
 ]
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -89,6 +89,8 @@ def load():
         "msc_set_graph": ([vp, ctypes.c_int], ctypes.c_int),
         "msc_cluster_labels": ([vp, ctypes.c_int, ctypes.c_double, ctypes.POINTER(ctypes.c_uint),
                                 _ip, _ip, _ip, ctypes.POINTER(ctypes.c_long)], ctypes.c_int),
+        "msc_cluster_labels_dd": ([vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.POINTER(ctypes.c_uint),
+                                   _ip, _ip, _ip, ctypes.POINTER(ctypes.c_long)], ctypes.c_int),
         "msc_set_contact_rows": ([vp, ctypes.c_int], ctypes.c_int),
         "msc_ini_state": ([ctypes.c_int, ctypes.c_int, _ip, _dp, _dp, _dp, _dp, ctypes.c_double, ctypes.c_int,
                            ctypes.POINTER(ctypes.c_uint), _ip, _dp, _dp, _dp, _dp], ctypes.c_int),
@@ -97,6 +99,8 @@ def load():
                              ctypes.POINTER(ctypes.c_long)], ctypes.c_int),
         "msc_pair_correl_noalloc": ([vp, ctypes.c_int, ctypes.c_int, _ip, ctypes.c_double, ctypes.c_int, _dp,
                                      ctypes.POINTER(ctypes.c_long)], ctypes.c_int),
+        "msc_place_rods": ([ctypes.c_int, ctypes.c_double, _dp, ctypes.c_double, ctypes.c_ulonglong, _dp, _dp],
+                           ctypes.c_int),
         "msc_set_timing": ([vp, ctypes.c_int], ctypes.c_int),
         "msc_kernel_time": ([vp, ctypes.c_char_p, _dp, ctypes.POINTER(ctypes.c_long)], ctypes.c_int),
         "msc_reset_timing": ([vp], ctypes.c_int),

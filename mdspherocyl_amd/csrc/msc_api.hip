@@ -2098,7 +2098,13 @@ int msc_pair_correl_noalloc(msc_ctx *ctx, int rep, int nspec, const int *counts,
 
 int msc_cluster_labels(msc_ctx *ctx, int rep, double min_s, unsigned int *mt_state, int *mt_index, int *labels,
                        int *nclust, long *n_asym) {
+  return msc_cluster_labels_dd(ctx, rep, min_s, INFINITY, mt_state, mt_index, labels, nclust, n_asym);
+}
+
+int msc_cluster_labels_dd(msc_ctx *ctx, int rep, double min_s, double u_cut, unsigned int *mt_state, int *mt_index,
+                          int *labels, int *nclust, long *n_asym) {
   SCOPE(ctx);
+  if (std::isnan(u_cut) || u_cut == -INFINITY) return ctx ? fail(ctx, MSC_E_ARG, "u_cut must be a number or +inf") : MSC_E_ARG;
   if (!ctx || rep < 0 || rep >= ctx->nrep || !(min_s >= 0) || !mt_state || !mt_index || !labels || !nclust)
     return MSC_E_ARG;
   if (*mt_index < 0 || *mt_index > Mt19937::kN) return fail(ctx, MSC_E_ARG, "mt_index outside [0, 624]");
@@ -2111,7 +2117,7 @@ int msc_cluster_labels(msc_ctx *ctx, int rep, double min_s, unsigned int *mt_sta
   const State s = sel(ctx, rep);
   {
     Bracket b(ctx, "clusters");
-    launch_clusters(s, min_s, reach_max, ctx->d_clu, ctx->d_clu_flag, ctx->stream);
+    launch_clusters(s, min_s, reach_max, u_cut, ctx->d_clu, ctx->d_clu_flag, ctx->stream);
   }
   FINISH("msc_cluster_labels");
   std::vector<int> root(R.N);

@@ -1258,11 +1258,19 @@ __device__ __forceinline__ void wave_rows_sum(const double *__restrict__ rows, i
 typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
 typedef float float4_t __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ void split_f16(float x, _Float16 &h, _Float16 &l) {
-  h = (_Float16)x;
-  l = (_Float16)(x - (float)h);
+// x = h + l with h = fp16(x) and l = fp16(x - h), x - h taken in FP64 (x is
+// an FP64 operand, see below); x - (h + l) is below 2^-22 |x| (2^-25
+// absolute where the pieces fall into fp16's subnormal range).
+__device__ __forceinline__ void split_f16(double x, _Float16 &h, _Float16 &l) {
+  h = (_Float16)(float)x;
+  l = (_Float16)(float)(x - (double)(float)h);
 }
 
+// Operands are formed in FP64 from the FP32 tables (a product of two FP32
+// values is exact in FP64) and split from there.  Splitting the operands
+// after rounding them to FP32 left force errors 3 to 80 times larger
+// (N = 32768: relative L2 2e-7 against 6e-8; replicas of 3000 to 8000
+// rods: reciprocal torques 5.6e-5 against 1e-6 of their rms).
 template <int KZ>
 __global__ void __launch_bounds__(256) k_recip_struct_mfma(State s, int rt0, const Cpx<float> *__restrict__ ph,
                                                            const float *__restrict__ amp, float *__restrict__ part) {
@@ -1320,13 +1328,12 @@ __global__ void __launch_bounds__(256) k_recip_struct_mfma(State s, int rt0, con
         for (int j = 0; j < 4; ++j) {
           const int p = step * 16 + 4 * lq + j;
           const Cpx<float> ex = sh[p * 3 * KZ + ax];
-          const float c = ok ? sa[3 * p + a] : 0.f;
-          const float br = c * ex.re, bi = c * (sx * ex.im);
+          const double c = ok ? (double)sa[3 * p + a] : 0.0;
           _Float16 h, l;
-          split_f16(br, h, l);
+          split_f16(c * (double)ex.re, h, l);
           rh[j] = h;
           rl[j] = l;
-          split_f16(bi, h, l);
+          split_f16(c * (double)(sx * ex.im), h, l);
           ih[j] = h;
           il[j] = l;
         }
@@ -1360,9 +1367,9 @@ __global__ void __launch_bounds__(256) k_recip_struct_mfma(State s, int rt0, con
           const Cpx<float> *tp = sh + p * 3 * KZ;
           Cpx<float> ey = tp[KZ + ay], ez = tp[2 * KZ + nz];
           ey.im *= sy;
-          float ar = fmaf(ey.re, ez.re, -ey.im * ez.im);
-          float ai = fmaf(ey.re, ez.im, ey.im * ez.re);
-          if (!ok) ar = ai = 0.f;
+          double ar = (double)ey.re * ez.re - (double)ey.im * ez.im;
+          double ai = (double)ey.re * ez.im + (double)ey.im * ez.re;
+          if (!ok) ar = ai = 0.0;
           _Float16 h, l;
           split_f16(ar, h, l);
           arh[j] = h;
@@ -1762,8 +1769,39 @@ __global__ void k_clust_init(State s, int *__restrict__ par) {
 // particle with 2 de >= r_list_max - reach_max is counted in flags[0].
 // Unions from concurrent lanes are safe (uf_union retries its exchange); the
 // labels depend only on the components, not on the roots the unions pick.
-__global__ void __launch_bounds__(256) k_clust_edges(State s, double min_s, double reach_max, int *__restrict__ par,
-                                                     int *__restrict__ flags) {
+// Bare dipole-dipole energy of particles A < B over their dipole slots,
+// minimum image of the site separation, in the operation order of the
+// oracle's o_udd (slots of A outer, of B inner; no fused products).
+__device__ double pair_udd(const State &s, const RepDev &R, int A, int B) {
+  const int N = s.Ntot;
+  double U = 0.0;
+  for (int v = 0; v < s.nd; ++v) {
+    const double ma = s.dmu[v * N + A];
+    if (ma == 0.0) continue;
+    for (int w = 0; w < s.nd; ++w) {
+      const double mb = s.dmu[w * N + B];
+      if (mb == 0.0) continue;
+      double r[3], ua[3], ub[3];
+      for (int a = 0; a < 3; ++a) {
+        r[a] = s.dpos[(3 * v + a) * N + A] - s.dpos[(3 * w + a) * N + B];
+        r[a] -= R.L[a] * floor(r[a] / R.L[a] + 0.5);
+        ua[a] = s.dori[(3 * v + a) * N + A];
+        ub[a] = s.dori[(3 * w + a) * N + B];
+      }
+      const double r2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+      const double uar = ua[0] * r[0] + ua[1] * r[1] + ua[2] * r[2];
+      const double ubr = ub[0] * r[0] + ub[1] * r[1] + ub[2] * r[2];
+      const double uab = ua[0] * ub[0] + ua[1] * ub[1] + ua[2] * ub[2];
+      U += ma * mb * (uab * r2 - 3.0 * uar * ubr) / (r2 * r2 * sqrt(r2));
+    }
+  }
+  return U;
+}
+
+// u_cut: the contact test also needs the dipolar pair energy below u_cut
+// (+inf: the reference's purely geometric test).
+__global__ void __launch_bounds__(256) k_clust_edges(State s, double min_s, double reach_max, double u_cut,
+                                                     int *__restrict__ par, int *__restrict__ flags) {
   const int lane = threadIdx.x & 63;
   const int g = s.g_begin + __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
   if (g >= s.g_end) return;
@@ -1800,7 +1838,9 @@ __global__ void __launch_bounds__(256) k_clust_edges(State s, double min_s, doub
     const bool t1 = dir_segdist(s, R, g, j) < sig_sq;
     const bool t2 = dir_segdist(s, R, j, g) < sig_sq;
     if (t1 != t2) atomicAdd(&flags[1], 1);
-    if (t1 || t2) uf_union(par, g, j);
+    if (!(t1 || t2)) continue;
+    if (!isinf(u_cut) && !(pair_udd(s, R, g, j) < u_cut)) continue;
+    uf_union(par, g, j);
   }
 }
 
@@ -2614,11 +2654,13 @@ void launch_pair_correl(const double *x, const double *ori, int Ntot, const Corr
   const dim3 grid((unsigned)(nt * (nt + 1) / 2), (unsigned)((a.n_bars + a.nbr - 1) / a.nbr));
   MSC_LAUNCH(k_pair_correl, grid, dim3(kCorrelTile), 0, st, x, ori, Ntot, a, hist, dropped);
 }
-void launch_clusters(const State &s, double min_s, double reach_max, int *par, int *flags, hipStream_t st) {
+void launch_clusters(const State &s, double min_s, double reach_max, double u_cut, int *par, int *flags,
+                     hipStream_t st) {
   const dim3 grid(nblk(s.g_end - s.g_begin, 256));
   launch_fill_int(flags, 2, 0, st);
   MSC_LAUNCH(k_clust_init, grid, dim3(256), 0, st, s, par);
-  MSC_LAUNCH(k_clust_edges, dim3(nblk(s.g_end - s.g_begin, 4)), dim3(256), 0, st, s, min_s, reach_max, par, flags);
+  MSC_LAUNCH(k_clust_edges, dim3(nblk(s.g_end - s.g_begin, 4)), dim3(256), 0, st, s, min_s, reach_max, u_cut, par,
+             flags);
   MSC_LAUNCH(k_clust_final, grid, dim3(256), 0, st, s, par);
 }
 void launch_kick1(const State &s, const Thermo *th, const double *gs, const int *step, double *rows, hipStream_t st) {

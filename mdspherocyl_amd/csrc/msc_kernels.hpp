@@ -109,7 +109,9 @@ void launch_end(const State &s, Thermo *th, const double *gs, const int *step, c
 // particle of g's component (global index); flags[0] counts particles that
 // moved too far for the reference rows to hold every contact, flags[1] pairs
 // whose contact test depends on the order of the two particles
-void launch_clusters(const State &s, double min_s, double reach_max, int *par, int *flags, hipStream_t st);
+// u_cut: contacts also need the bare dipolar pair energy below u_cut (+inf: off)
+void launch_clusters(const State &s, double min_s, double reach_max, double u_cut, int *par, int *flags,
+                     hipStream_t st);
 
 // order-parameter sums of replica rep (oo_params): out[0..5] = sum u_a u_b
 // (xx xy xz yy yz zz), out[6..8] = sum u, out[9..11] = total dipole moment

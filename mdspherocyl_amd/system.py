@@ -235,12 +235,14 @@ class MdSystem:
                                          bar_w, n_bars, _d(hist)[1], ctypes.byref(dropped)))
         return hist, dropped.value
 
-    def cluster_labels(self, rep, min_s, mt_state, mt_index):
+    def cluster_labels(self, rep, min_s, mt_state, mt_index, u_cut=None):
         """clust_conf (v25/functions.cpp:3270): per-particle cluster numbers.
 
         mt_state (624 words) and mt_index are the MT19937 state of the
         reference's `ran` generator; returns (labels, nclust, mt_state,
         mt_index, n_asym) with the state advanced by the draws made.
+        ``u_cut`` (None: the reference's geometric test) adds the dipolar
+        clause: a contact also needs a bare dipolar pair energy below u_cut.
         """
         st = np.ascontiguousarray(mt_state, dtype=np.uint32).copy()
         if st.shape != (624,):
@@ -249,10 +251,11 @@ class MdSystem:
         n = self.descs[rep].N
         labels = np.zeros(n, dtype=np.int32)
         nc, na = ctypes.c_int(), ctypes.c_long()
-        self._chk(self.L.msc_cluster_labels(self._ctx, rep, float(min_s),
-                                            st.ctypes.data_as(ctypes.POINTER(ctypes.c_uint)), ctypes.byref(idx),
-                                            labels.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), ctypes.byref(nc),
-                                            ctypes.byref(na)))
+        self._chk(self.L.msc_cluster_labels_dd(self._ctx, rep, float(min_s),
+                                               float("inf") if u_cut is None else float(u_cut),
+                                               st.ctypes.data_as(ctypes.POINTER(ctypes.c_uint)), ctypes.byref(idx),
+                                               labels.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                               ctypes.byref(nc), ctypes.byref(na)))
         return labels, nc.value, st, idx.value, na.value
 
     def work_counts(self, rep):

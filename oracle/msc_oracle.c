@@ -1355,7 +1355,64 @@ int o_dist_test(const osys *S, int A, int B, double min_s) {
   return o_spherocyl_dist(S, A, B, rij, dij, st) < sig_sq;
 }
 
-int o_clust_conf(osys *S, double min_s, omt *ran) {
+/* Bare dipole-dipole energy of particles A and B summed over their dipole
+ * slots (minimum image of the site separation):
+ *   mu_a mu_b (u_a.u_b r^2 - 3 (u_a.r)(u_b.r)) / (r^4 r).
+ * No reference counterpart: the energy clause of the workloads' dipolar
+ * contact criterion.  Callers pass the lower index as A, so the operation
+ * order (slots of A outer, of B inner) does not depend on which particle a
+ * search meets first. */
+double o_udd(const osys *S, int A, int B) {
+  double U = 0.;
+  for (int v = 0; v < S->nd; ++v) {
+    const int da = A * S->nd + v;
+    const double ma = S->dmu[da];
+    if (ma == 0.) continue;
+    for (int w = 0; w < S->nd; ++w) {
+      const int db = B * S->nd + w;
+      const double mb = S->dmu[db];
+      if (mb == 0.) continue;
+      double r[3];
+      for (int a = 0; a < 3; ++a) {
+        r[a] = S->dr[3 * da + a] - S->dr[3 * db + a];
+        r[a] -= S->L[a] * floor(r[a] / S->L[a] + 0.5);
+      }
+      const double *ua = S->du + 3 * da, *ub = S->du + 3 * db;
+      const double r2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+      const double uar = ua[0] * r[0] + ua[1] * r[1] + ua[2] * r[2];
+      const double ubr = ub[0] * r[0] + ub[1] * r[1] + ub[2] * r[2];
+      const double uab = ua[0] * ub[0] + ua[1] * ub[1] + ua[2] * ub[2];
+      U += ma * mb * (uab * r2 - 3.0 * uar * ubr) / (r2 * r2 * sqrt(r2));
+    }
+  }
+  return U;
+}
+
+/* The contact test of the cluster search: dist_test(A, B) and, when u_cut
+ * is finite, the dipolar pair energy below u_cut. */
+static int contact_test(const osys *S, int A, int B, double min_s, double u_cut) {
+  if (!o_dist_test(S, A, B, min_s)) return 0;
+  if (isinf(u_cut)) return 1;
+  return (A < B ? o_udd(S, A, B) : o_udd(S, B, A)) < u_cut;
+}
+
+long o_contact_pairs(const osys *S, long n, const int *A, const int *B, double min_s, double u_cut,
+                     unsigned char *out) {
+  long hits = 0;
+  for (long k = 0; k < n; ++k) {
+    const int a = A[k], b = B[k];
+    out[k] = (unsigned char)(contact_test(S, a, b, min_s, u_cut) || contact_test(S, b, a, min_s, u_cut));
+    hits += out[k];
+  }
+  return hits;
+}
+
+int o_clust_conf(osys *S, double min_s, omt *ran) { return o_clust_conf_dd(S, min_s, INFINITY, ran); }
+
+/* clust_conf (v25/functions.cpp:3282-3324) with rnd_part (2950-2970) and
+ * clust_gen's breadth-first search (3018-3148); u_cut = +inf is the
+ * reference's purely geometric criterion. */
+int o_clust_conf_dd(osys *S, double min_s, double u_cut, omt *ran) {
   int N = S->N, NC = S->nspec;
   int *NI[OSYS_MAXSPEC], nI[OSYS_MAXSPEC], base[OSYS_MAXSPEC];
   for (int n = 0, b = 0; n < NC; b += S->counts[n], ++n) {
@@ -1391,7 +1448,7 @@ int o_clust_conf(osys *S, double min_s, omt *ran) {
         for (int jj = 0; jj < S->counts[o]; ++jj) {
           int II = base[o] + jj;
           if (locadd[II]) continue;
-          if (!o_dist_test(S, q, II, min_s)) continue;
+          if (!contact_test(S, q, II, min_s, u_cut)) continue;
           if (added[II]) continue; /* the reference logs "classifies for 2 clusters" */
           queue[tail++] = II;
           int l2 = 0;

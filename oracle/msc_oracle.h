@@ -106,6 +106,12 @@ int    o_ini_mom(osys *s, double temp, omt *ran);
 /* ---- clusters (v25/functions.cpp:2950-3324) ---- */
 int    o_dist_test(const osys *s, int A, int B, double min_s);
 int    o_clust_conf(osys *s, double min_s, omt *ran);
+/* with the opt-in dipolar energy clause: contacts also need o_udd < u_cut */
+int    o_clust_conf_dd(osys *s, double min_s, double u_cut, omt *ran);
+double o_udd(const osys *s, int A, int B);
+/* out[k] = contact of pair (A[k], B[k]) in either order; returns the count */
+long   o_contact_pairs(const osys *s, long n, const int *A, const int *B, double min_s,
+                       double u_cut, unsigned char *out);
 
 /* ---- order parameters (v25/functions.cpp:2702-2760, v25/jacobi.c) ---- */
 int    o_jacobi3(float a[3][3], float d[3], float v[3][3]);

@@ -102,6 +102,12 @@ def lib():
         L.o_ini_mom.argtypes = [P(COsys), ctypes.c_double, P(COmt)]
         L.o_dist_test.argtypes = [P(COsys), ctypes.c_int, ctypes.c_int, ctypes.c_double]
         L.o_clust_conf.argtypes = [P(COsys), ctypes.c_double, P(COmt)]
+        L.o_clust_conf_dd.argtypes = [P(COsys), ctypes.c_double, ctypes.c_double, P(COmt)]
+        L.o_udd.restype = ctypes.c_double
+        L.o_udd.argtypes = [P(COsys), ctypes.c_int, ctypes.c_int]
+        L.o_contact_pairs.restype = ctypes.c_long
+        L.o_contact_pairs.argtypes = [P(COsys), ctypes.c_long, _ip, _ip, ctypes.c_double, ctypes.c_double,
+                                      ctypes.POINTER(ctypes.c_ubyte)]
         L.o_acc_correl.argtypes = [P(COsys), ctypes.c_double, ctypes.c_int, _dp]
         L.o_acc_correl.restype = ctypes.c_long
         _fp = ctypes.POINTER(ctypes.c_float)

@@ -537,3 +537,69 @@ def test_cutoff_mode_is_the_sum_of_isolated_slot_pairs():
     assert U == pytest.approx(Ub, rel=1e-12)
     for a, b in ((F, Fb), (T, Tb), (PT, PTb)):
         assert np.allclose(a, b, rtol=1e-11, atol=1e-12 * np.abs(b).max())
+
+
+# --------------------------------------------------------------------------
+# dipolar energy clause of the cluster criterion (no reference counterpart)
+# --------------------------------------------------------------------------
+def test_udd_is_the_bare_limit_of_the_real_space_sum():
+    """o_udd of two particles with two off-centre tilted slots each equals
+    the real-space Ewald energy of the same pair at alpha -> 0 (all slot
+    pairs, minimum image), a separate evaluation through the tabulated
+    kernel; and o_udd is symmetric in its arguments."""
+    rng = np.random.default_rng(41)
+    for _ in range(5):
+        S = two_slot_system(rng, 2, 60.0)
+        S.r[1] = S.r[0] + unit(rng.normal(size=3)) * rng.uniform(3.0, 5.0)  # the table is fine there
+        S.set_dip()
+        Ud = L.o_udd(S.c(), 0, 1)
+        tab = np.zeros(32768 * 3)
+        L.o_tabul(O.dptr(tab), 1e-7)
+        UR = dipR_eval(S, tab, 0.0)[0]
+        assert Ud == pytest.approx(UR, rel=2e-5)
+        assert Ud == pytest.approx(L.o_udd(S.c(), 1, 0), rel=1e-14)
+
+
+def test_energy_clause_refines_the_geometric_clusters():
+    """With u_cut = +inf the clause changes nothing; with a finite cut every
+    cluster lies inside one geometric cluster, and its clusters are the
+    components of the geometric contacts whose dipolar energy is below
+    u_cut."""
+    rng = np.random.default_rng(42)
+    box, N, min_s = 15.0, 120, 0.8
+    r, u = random_rods(N, box, rng, lens=2.0, mind=1.0)
+    S = make_sys(r, u, 2.0, (box, box, box), mu=rng.uniform(3.0, 4.0, size=N))
+    geo = []
+    for cut in (math.inf, -1.0):
+        g = O.COmt()
+        L.omt_seed(ctypes.byref(g), 3)
+        n = L.o_clust_conf_dd(S.c(), min_s, cut, ctypes.byref(g))
+        geo.append((n, S.clust.copy()))
+    g = O.COmt()
+    L.omt_seed(ctypes.byref(g), 3)
+    n0 = L.o_clust_conf(S.c(), min_s, ctypes.byref(g))
+    assert n0 == geo[0][0] and np.array_equal(S.clust, geo[0][1])
+    (ng, lg), (nd, ld) = geo
+    assert nd > ng
+    for c in range(nd):
+        assert len(set(lg[ld == c])) == 1
+    parent = list(range(N))
+
+    def find(a):
+        while parent[a] != a:
+            parent[a] = parent[parent[a]]
+            a = parent[a]
+        return a
+    kept = contacts = 0
+    for i in range(N):
+        for j in range(i + 1, N):
+            if L.o_dist_test(S.c(), i, j, min_s):
+                contacts += 1
+                if L.o_udd(S.c(), i, j) < -1.0:
+                    parent[find(i)] = find(j)
+                    kept += 1
+    assert 10 < kept < contacts - 50   # the clause keeps some contacts and drops many
+    comp = {}
+    for i in range(N):
+        comp.setdefault(find(i), set()).add(i)
+    assert sorted(map(sorted, comp.values())) == sorted(sorted(np.nonzero(ld == c)[0].tolist()) for c in range(nd))
