@@ -118,6 +118,10 @@ struct msc_ctx {
   int ncells = 0;
   double *d_kval = nullptr;
   void *d_tab = nullptr, *d_ph = nullptr, *d_amp = nullptr, *d_part = nullptr, *d_A = nullptr;
+  void *d_bop = nullptr;  // matrix-core back-projection operands (mixed contexts)
+  long bop_elems = 0;     // their count, in groups of 4 fp16 values
+  int *d_bexp = nullptr;  // their per-column scale exponents
+  std::vector<double> kval_host;  // reciprocal weights of every slot (build_kspace)
   double *d_slot_out = nullptr, *d_task_out = nullptr, *d_red = nullptr, *d_M2 = nullptr, *d_Kin = nullptr,
          *d_gs = nullptr, *d_obs = nullptr, *d_counts = nullptr;
   RepEnergy *d_E = nullptr;
@@ -484,6 +488,39 @@ void build_table(const RepDev &R, std::vector<double> &T) {
   }
 }
 
+// Scale exponents of the matrix-core back-projection operands of replica
+// rp: the operands of column tile ct (16 columns (nx, ny)) and weight l
+// (nz^l A) are multiplied by 2^e before their fp16 split, with e chosen so
+// that the hard bound
+//   |nz^l A(k)| <= nz^l val(k) |k| sum_i |mu_i|  (|S(k)| <= |k| sum |c_i|)
+// maps to at most 2^15 over the tile: nothing can overflow fp16, and the
+// entries that carry the sums stay in fp16's normal range, where the split
+// keeps 22 bits.  The kernel multiplies its column sums by 2^-e.
+void build_bop_exp(const msc_ctx *ctx, int rp, double musum, std::vector<int> &out) {
+  const RepDev &R = ctx->reps[rp];
+  const int kmax = R.kmax, side = 2 * kmax + 1, nct = (R.ncol + 15) / 16;
+  out.assign((size_t)3 * nct, 0);
+  for (int ct = 0; ct < nct; ++ct) {
+    double bmax[3] = {0, 0, 0};
+    for (int col = 16 * ct; col < std::min(R.ncol, 16 * ct + 16); ++col) {
+      const int nx = col % side - kmax, ny = col / side - kmax;
+      for (int nz = 0; nz < R.KZ; ++nz) {
+        const double val = ctx->kval_host[R.slot0 + col * R.KZ + nz];
+        if (val == 0.0) continue;
+        const double k[3] = {R.P2[0] * nx, R.P2[1] * ny, R.P2[2] * nz};
+        const double b = val * std::sqrt(k[0] * k[0] + k[1] * k[1] + k[2] * k[2]) * musum;
+        double w = 1.0;
+        for (int l = 0; l < 3; ++l, w *= nz) bmax[l] = std::max(bmax[l], w * b);
+      }
+    }
+    for (int l = 0; l < 3; ++l) {
+      int e = 0;
+      if (bmax[l] > 0) e = (int)std::floor(std::log2(32768.0 / bmax[l]));
+      out[(size_t)3 * ct + l] = std::min(100, std::max(-100, e));
+    }
+  }
+}
+
 }  // namespace
 
 // ==========================================================================
@@ -797,7 +834,7 @@ int msc_create(int device, int precision, int nrep, const msc_replica_desc *reps
     }
   }
   // replica layout: particles, tasks, chunk tasks, k slots, tables, partials
-  int n0 = 0, slot0 = 0;
+  int n0 = 0, slot0 = 0, bexp_cols = 0;
   long part0 = 0;
   ctx->reps.resize(nrep);
   for (int r = 0; r < nrep; ++r) {
@@ -836,6 +873,10 @@ int msc_create(int device, int precision, int nrep, const msc_replica_desc *reps
     R.nrtask = (int)ctx->rtasks.size() - R.rtask0;
     R.part0 = part0;
     part0 += (long)R.nrtask * R.ncol * R.KZ * 6;
+    R.bop0 = ctx->bop_elems;
+    ctx->bop_elems += (long)((R.ncol + 15) / 16) * kBopPerTile;
+    R.bexp0 = 3 * bexp_cols;
+    bexp_cols += (R.ncol + 15) / 16;
     slot0 += R.ncol * R.KZ;
     ctx->maxslot = std::max(ctx->maxslot, R.ncol * R.KZ);
     ctx->maxcol = std::max(ctx->maxcol, R.ncol);
@@ -891,6 +932,9 @@ int msc_create(int device, int precision, int nrep, const msc_replica_desc *reps
   A(&ctx->d_srebuild, nrep);
   A(&ctx->d_one, nrep);
   A(&ctx->d_stamp, 1);
+  // zeroed here once: entries for nz >= K and columns past the last stay 0
+  if (ctx->mixed && rc == MSC_OK) rc = dalloc(ctx, (unsigned long long **)&ctx->d_bop, ctx->bop_elems);
+  if (ctx->mixed && rc == MSC_OK) rc = dalloc(ctx, &ctx->d_bexp, (size_t)3 * bexp_cols);
   if (rc == MSC_OK) {
     unsigned char **raw[] = {(unsigned char **)&ctx->d_tab, (unsigned char **)&ctx->d_ph,
                              (unsigned char **)&ctx->d_amp, (unsigned char **)&ctx->d_part,
@@ -913,7 +957,7 @@ int msc_create(int device, int precision, int nrep, const msc_replica_desc *reps
     th[r].s[0] = th[r].s[1] = reps[r].s0;
     th[r].strn[0] = th[r].strn[1] = 0;
   }
-  std::vector<double> kval;
+  std::vector<double> &kval = ctx->kval_host;
   build_kspace(ctx, kval);
   std::vector<double> tab((size_t)nrep * kTabN * 3);
   for (int r = 0; r < nrep; ++r) build_table(ctx->reps[r], tab);
@@ -958,7 +1002,7 @@ void msc_destroy(msc_ctx *ctx) {
                   ctx->d_tab, ctx->d_ph, ctx->d_amp, ctx->d_part, ctx->d_A, ctx->d_slot_out, ctx->d_task_out,
                   ctx->d_red, ctx->d_M2, ctx->d_Kin, ctx->d_gs, ctx->d_obs, ctx->d_counts, ctx->d_E,
                   ctx->d_th, ctx->d_flags, ctx->d_rebuild, ctx->d_one, ctx->d_run_gs, ctx->d_run_obs, ctx->d_save,
-                  ctx->d_stamp};
+                  ctx->d_stamp, ctx->d_bop, ctx->d_bexp};
   for (void *b : bufs)
     if (b) hipFree(b);
   for (auto &kv : ctx->timers)
@@ -1001,6 +1045,13 @@ int msc_set_state(msc_ctx *ctx, int rep, const double *r, const double *p, const
   if ((rc = put(ctx->d_inert, I.data(), 3))) return rc;
   derive(ctx, rep, len, dia, dpc);
   upload_reps(ctx);
+  if (ctx->mixed) {
+    double musum = 0;
+    for (int i = 0; i < n * nd; ++i) musum += std::fabs(dmu[i]);
+    std::vector<int> ex;
+    build_bop_exp(ctx, rep, musum, ex);
+    if ((rc = h2d(ctx, ctx->d_bexp + R.bexp0, ex.data(), sizeof(int) * ex.size()))) return rc;
+  }
   launch_dipoles(sel(ctx, rep), ctx->stream);
   FINISH("msc_set_state");
   ctx->have_state = true;
@@ -1383,7 +1434,7 @@ static int dipK_launches(msc_ctx *ctx, int rep, const double *m2rows, double *kb
     }
   }
   launch_recip_reduce(s, ctx->mixed, ctx->d_part, m2rows ? nullptr : ctx->d_M2, m2rows, ctx->d_A, ctx->d_slot_out,
-                      kblk, ctx->maxslot, ctx->stream);
+                      kblk, ctx->maxslot, ctx->d_bop, ctx->d_bexp, ctx->stream);
   if (!kblk) launch_reduce(s, ctx->d_slot_out, 10, 10, 1, (double *)ctx->d_E, kEstride, kEoff_K, ctx->stream);
   for (int kz : ctx->kz_groups) {
     for (int r = 0; r < ctx->nrep; ++r) {
@@ -1397,7 +1448,8 @@ static int dipK_launches(msc_ctx *ctx, int rep, const double *m2rows, double *kb
       s1.rep_end = r1 + 1;
       {
         Bracket b(ctx, "recip_force");
-        launch_recip_force(s1, ctx->mixed, kz, ctx->d_ph, ctx->d_amp, ctx->d_A, ctx->stream);
+        launch_recip_force(s1, ctx->mixed, kz, ctx->d_ph, ctx->d_amp, ctx->d_A, ctx->d_bop, ctx->d_bexp,
+                           ctx->stream);
       }
       r = r1;
     }

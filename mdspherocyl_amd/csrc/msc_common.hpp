@@ -35,6 +35,8 @@ struct RepDev {
   int pad_w;
   long nl_base, sl_base, dl_base;
   long part0;           // first element of this replica's structure-factor partials
+  long bop0;            // first element (4 fp16 values) of this replica's back-projection operands
+  int bexp0;            // first of this replica's 3 operand scale exponents per column tile
   int task0, ntask;     // wave tasks (64 particles) of this replica
   int rtask0, nrtask;   // reciprocal chunk tasks of this replica
   double L[3], V, P2[3];

@@ -1435,11 +1435,19 @@ __global__ void __launch_bounds__(256) k_recip_struct_mfma(State s, int rt0, con
 // from the per-task rows m2rows (6 values each).  With kblk, wave 0 also
 // stores the sum of its 64 slots' 10 values as row blockIdx.x of the
 // replica (kblk_stride rows per replica).
+// With bop (mixed contexts) the kernel also writes the operands of the
+// matrix-core back projection, B_l(col, nz) = 2^e nz^l A(col, nz) for
+// l = 0, 1, 2 with the column tile's scale exponent e = bexp[3 (col / 16) + l], formed
+// in FP64 and split into fp16 (hi, lo) pieces, in the lane layout of a
+// 16x16x16 f16 matrix operand: column tile ct = col / 16, piece
+// q = (re hi, re lo, im hi, im lo), lane (col % 16) + 16 (nz / 4), element
+// nz % 4 (see k_recip_force_mfma).
 template <typename R>
 __global__ void __launch_bounds__(512) k_recip_reduce(State s, const R *__restrict__ part, const double *__restrict__ M2,
                                                        const double *__restrict__ m2rows, Cpx<R> *__restrict__ Aout,
                                                        double *__restrict__ slot_out, double *__restrict__ kblk,
-                                                       int kblk_stride) {
+                                                       int kblk_stride, _Float16 *__restrict__ bop,
+                                                       const int *__restrict__ bexp) {
   constexpr int kGroups = 8;
   __shared__ double sh[kGroups][6][64];
   const int rep = s.rep_begin + blockIdx.y;
@@ -1506,6 +1514,7 @@ __global__ void __launch_bounds__(512) k_recip_reduce(State s, const R *__restri
   double o[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
   Cpx<R> A;
   A.re = A.im = (R)0;
+  double Ar = 0.0, Ai = 0.0;
   if (val != 0.0) {
     double k[3] = {Rp.P2[0] * nx, Rp.P2[1] * ny, Rp.P2[2] * nz};
     double Sr = k[0] * Q[0] + k[1] * Q[2] + k[2] * Q[4];
@@ -1525,14 +1534,35 @@ __global__ void __launch_bounds__(512) k_recip_reduce(State s, const R *__restri
         o[1 + 3 * j + c] = val * 0.5 * (((j == c) ? 1.0 : 0.0) - pval * k[j] * k[c]) * nrm +
                            val * 0.5 * 2 * (re - Mk[j]) * k[c];
       }
-    A.re = (R)(val * Sr);
-    A.im = (R)(-val * Si);
+    Ar = val * Sr;
+    Ai = -val * Si;
+    A.re = (R)Ar;
+    A.im = (R)Ai;
   }
   if (act) {
     double *os = slot_out + (size_t)(Rp.slot0 + sl) * 10;
 #pragma unroll
     for (int q = 0; q < 10; ++q) os[q] = o[q];
     Aout[Rp.slot0 + sl] = A;
+    if (bop) {
+      const int col = nyi * side + enx, ct = col >> 4;
+      const int bl = (col & 15) + 16 * (nz >> 2), be = nz & 3;
+      double w = 1.0;
+#pragma unroll
+      for (int l = 0; l < 3; ++l) {
+        const int e = bexp[Rp.bexp0 + 3 * ct + l];
+        const double v[2] = {ldexp(w * Ar, e), ldexp(w * Ai, e)};
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+          const _Float16 h = (_Float16)(float)v[c];
+          const _Float16 lo = (_Float16)(float)(v[c] - (double)(float)h);
+          _Float16 *d = bop + 4 * (Rp.bop0 + ((long)ct * 12 + l * 4 + 2 * c) * 64 + bl) + be;
+          d[0] = h;
+          d[4 * 64] = lo;
+        }
+        w *= (double)nz;
+      }
+    }
   }
   if (kblk) {
     double *kb = kblk + ((size_t)rep * kblk_stride + blockIdx.x) * 10;
@@ -1665,6 +1695,221 @@ __global__ void __launch_bounds__(64 * kRfGroups) k_recip_force(State s, const C
   for (int a = 0; a < 3; ++a) {
     s.F[a * N + g] += F[a];
     s.T[a * N + g] += t[a];
+  }
+}
+
+// Back projection on the matrix cores (mixed contexts).  The nz sums of
+// k_recip_force, S_l(col) = sum_nz nz^l e_z(nz) A(col, nz), l = 0, 1, 2, are
+// complex matrix products with the particles as rows, nz as the inner
+// dimension (one 16-deep tile) and the columns (nx, ny) as columns:
+// A operand e_z of the particle (split fp16, built once), B operand
+// nz^l A (split fp16, written by k_recip_reduce).  Each product is
+// hi*hi + hi*lo + lo*hi in v_mfma_f32_16x16x16f16 with FP32 accumulation.
+// The rest is k_recip_force's per-column combination on the vector ALUs:
+// x = e_x e_y, and the force and field sums from x S_l, in FP32 over the
+// column tiles a lane visits and FP64 from there on.  The operands of column
+// tile ct and weight l carry a factor 2^bexp[3 ct + l] (k_recip_reduce),
+// removed through the per-tile coefficients here.
+// One block per wave task: 8 waves, two groups of 4; wave w of a group
+// takes the task's particles 16 w .. 16 w + 15, group g the column tiles
+// ct = g, g + 2, ...  A group's B operands of a column tile are staged in
+// LDS for its 4 waves, the next tile's loaded while this one is used; the
+// two groups' sums meet in LDS in a fixed order.
+constexpr int kBpGroups = 2;
+template <int KZ>
+__global__ void __launch_bounds__(256 * kBpGroups) k_recip_force_mfma(State s, const Cpx<float> *__restrict__ ph,
+                                                                      const float *__restrict__ amp,
+                                                                      const half4_t *__restrict__ bop,
+                                                                      const int *__restrict__ bexp) {
+  constexpr int kmax = KZ - 1, side = 2 * kmax + 1, ncol = side * side, NCT = (ncol + 15) / 16;
+  constexpr int NSTEP = (NCT + kBpGroups - 1) / kBpGroups;
+  __shared__ Cpx<float> tx[4][16][KZ], ty[4][16][KZ];
+  __shared__ half4_t sB[kBpGroups][2][12][64];
+  __shared__ float scl[NCT][3];
+  __shared__ double red[kBpGroups - 1][4][4][4][6];  // [group-1][wave][lq][j][value]
+  if ((int)blockIdx.x >= s.ntask) return;  // block-uniform
+  const int tid = s.task_begin + blockIdx.x;
+  const Task tk = s.tasks[tid];
+  const RepDev &Rp = s.reps[tk.rep];
+  const int N = s.Ntot;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int grp = wave >> 2, wv = wave & 3;
+  const int gt = threadIdx.x & 255;  // thread index within the group
+  const int li = lane & 15, lq = lane >> 4;
+  const int pw = wv * 16;  // the wave's first particle within the task
+  const half4_t *B = bop + Rp.bop0;
+  // each group's first column tile, every tile's scale factors
+  if (grp < NCT)
+    for (int q = gt; q < 12 * 64; q += 256) sB[grp][0][q >> 6][q & 63] = B[(size_t)grp * 12 * 64 + q];
+  for (int q = threadIdx.x; q < 3 * NCT; q += 256 * kBpGroups) scl[q / 3][q % 3] = ldexpf(1.f, -bexp[Rp.bexp0 + q]);
+  // e_x and e_y of the wave's 16 particles (group 0 loads them)
+  if (grp == 0)
+    for (int q = lane; q < 16 * KZ; q += 64) {
+      const int p = q & 15, n = q >> 4;
+      Cpx<float> vx, vy;
+      vx.re = vx.im = vy.re = vy.im = 0.f;
+      if (pw + p < tk.n) {
+        const int g = tk.g0 + pw + p;
+        vx = ph[((size_t)0 * kMaxKZ + n) * N + g];
+        vy = ph[((size_t)1 * kMaxKZ + n) * N + g];
+      }
+      tx[wv][p][n] = vx;
+      ty[wv][p][n] = vy;
+    }
+  // A operand: lane (li, lq) holds e_z(nz = 4 lq + j) of the wave's particle li
+  half4_t arh, arl, aih, ail, naih, nail;
+  {
+    const bool ok = pw + li < tk.n;
+    const int g = tk.g0 + pw + (ok ? li : 0);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int nz = 4 * lq + j;
+      Cpx<float> ez;
+      ez.re = ez.im = 0.f;
+      if (ok && nz < KZ) ez = ph[((size_t)2 * kMaxKZ + nz) * N + g];
+      _Float16 h, l;
+      split_f16((double)ez.re, h, l);
+      arh[j] = h;
+      arl[j] = l;
+      split_f16((double)ez.im, h, l);
+      aih[j] = h;
+      ail[j] = l;
+      naih[j] = -h;
+      nail[j] = -l;
+    }
+  }
+  // the lane's output particles: 4 lq + j of the wave
+  float c0[4], c1[4], c2z[4];
+  const float P2x = (float)Rp.P2[0], P2y = (float)Rp.P2[1], P2z = (float)Rp.P2[2];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int p = pw + 4 * lq + j;
+    const int g = tk.g0 + (p < tk.n ? p : 0);
+    c0[j] = amp[g];
+    c1[j] = amp[N + g];
+    c2z[j] = amp[2 * N + g] * P2z;
+  }
+  float part[4][6];
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+#pragma unroll
+    for (int q = 0; q < 6; ++q) part[j][q] = 0.f;
+  for (int st = 0; st < NSTEP; ++st) {
+    __syncthreads();  // this step's tiles staged; the other buffers free
+    const int ct = st * kBpGroups + grp;
+    const int cn = ct + kBpGroups;  // the group's next tile
+    const bool have = ct < NCT, more = cn < NCT;
+    half4_t nxt[3];
+    if (more) {
+#pragma unroll
+      for (int u = 0; u < 3; ++u) nxt[u] = B[(size_t)cn * 12 * 64 + gt + 256 * u];
+    }
+    const int buf = st & 1;
+    if (have) {
+      float4_t Sr[3], Si[3];
+#pragma unroll
+      for (int l = 0; l < 3; ++l) {
+        const half4_t brh = sB[grp][buf][4 * l + 0][lane], brl = sB[grp][buf][4 * l + 1][lane];
+        const half4_t bih = sB[grp][buf][4 * l + 2][lane], bil = sB[grp][buf][4 * l + 3][lane];
+        float4_t r = {0.f, 0.f, 0.f, 0.f}, i = {0.f, 0.f, 0.f, 0.f};
+        r = __builtin_amdgcn_mfma_f32_16x16x16f16(arh, brl, r, 0, 0, 0);
+        r = __builtin_amdgcn_mfma_f32_16x16x16f16(arl, brh, r, 0, 0, 0);
+        r = __builtin_amdgcn_mfma_f32_16x16x16f16(naih, bil, r, 0, 0, 0);
+        r = __builtin_amdgcn_mfma_f32_16x16x16f16(nail, bih, r, 0, 0, 0);
+        r = __builtin_amdgcn_mfma_f32_16x16x16f16(arh, brh, r, 0, 0, 0);
+        r = __builtin_amdgcn_mfma_f32_16x16x16f16(naih, bih, r, 0, 0, 0);
+        i = __builtin_amdgcn_mfma_f32_16x16x16f16(arh, bil, i, 0, 0, 0);
+        i = __builtin_amdgcn_mfma_f32_16x16x16f16(arl, bih, i, 0, 0, 0);
+        i = __builtin_amdgcn_mfma_f32_16x16x16f16(aih, brl, i, 0, 0, 0);
+        i = __builtin_amdgcn_mfma_f32_16x16x16f16(ail, brh, i, 0, 0, 0);
+        i = __builtin_amdgcn_mfma_f32_16x16x16f16(arh, bih, i, 0, 0, 0);
+        i = __builtin_amdgcn_mfma_f32_16x16x16f16(aih, brh, i, 0, 0, 0);
+        Sr[l] = r;
+        Si[l] = i;
+      }
+      // the tile's scale factors (uniform), folded into the coefficients
+      const float sc0 = scl[ct][0], sc1 = scl[ct][1], sc2 = scl[ct][2];
+      // lane: column col, particles 4 lq + j
+      const int col = ct * 16 + li;
+      const int cc = col < ncol ? col : 0;
+      const int iy = cc / side, ix = cc - iy * side;
+      const int nx = ix - kmax, ny = iy - kmax;
+      const int ax = nx < 0 ? -nx : nx, ay = ny < 0 ? -ny : ny;
+      const float kx = P2x * (float)nx, ky = P2y * (float)ny;
+      const float kx0 = kx * sc0, ky0 = ky * sc0, kz1 = P2z * sc1;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int p = 4 * lq + j;
+        Cpx<float> exv = tx[wv][p][ax], ey = ty[wv][p][ay];
+        if (nx < 0) exv.im = -exv.im;
+        if (ny < 0) ey.im = -ey.im;
+        const float xr = fmaf(exv.re, ey.re, -exv.im * ey.im);
+        const float xi = fmaf(exv.re, ey.im, exv.im * ey.re);
+        // x S_l of the scaled sums
+        const float r0 = fmaf(xr, Sr[0][j], -xi * Si[0][j]), i0 = fmaf(xr, Si[0][j], xi * Sr[0][j]);
+        const float r1 = fmaf(xr, Sr[1][j], -xi * Si[1][j]), i1 = fmaf(xr, Si[1][j], xi * Sr[1][j]);
+        const float i2 = fmaf(xr, Si[2][j], xi * Sr[2][j]);
+        const float pcol = fmaf(c0[j], kx, c1[j] * ky);
+        const float Gc = fmaf(c2z[j] * sc1, i1, (pcol * sc0) * i0);
+        const float Gz = P2z * fmaf(c2z[j] * sc2, i2, (pcol * sc1) * i1);
+        part[j][0] = fmaf(Gc, kx, part[j][0]);
+        part[j][1] = fmaf(Gc, ky, part[j][1]);
+        part[j][2] += Gz;
+        part[j][3] = fmaf(r0, kx0, part[j][3]);
+        part[j][4] = fmaf(r0, ky0, part[j][4]);
+        part[j][5] = fmaf(r1, kz1, part[j][5]);
+      }
+    }
+    if (more) {
+#pragma unroll
+      for (int u = 0; u < 3; ++u) {
+        const int q = gt + 256 * u;
+        sB[grp][buf ^ 1][q >> 6][q & 63] = nxt[u];
+      }
+    }
+  }
+  // sums over the 16 columns a lane group holds (fixed butterfly order, FP64)
+  double acc[4][6];
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      double v = (double)part[j][q];
+      for (int o = 1; o < 16; o <<= 1) v += __shfl_xor(v, o);
+      acc[j][q] = v;
+    }
+  // the groups' sums meet in group 0, in group order
+  if (grp > 0 && li == 0)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int q = 0; q < 6; ++q) red[grp - 1][wv][lq][j][q] = acc[j][q];
+  __syncthreads();
+  if (grp != 0 || li != 0) return;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int p = pw + 4 * lq + j;
+    if (p >= tk.n) continue;
+#pragma unroll
+    for (int q = 0; q < 6; ++q)
+      for (int g2 = 1; g2 < kBpGroups; ++g2) acc[j][q] += red[g2 - 1][wv][lq][j][q];
+    const int g = tk.g0 + p;
+    double F[3] = {-acc[j][0], -acc[j][1], -acc[j][2]};
+    double H[3] = {acc[j][3], acc[j][4], acc[j][5]};
+    double mu = s.dmu[g];
+    double u[3] = {s.dori[g], s.dori[N + g], s.dori[2 * N + g]};
+    double sd = s.dpc[g];
+    const double *G = Rp.G;
+    double Gu[3] = {G[0] * u[0] + G[1] * u[1] + G[2] * u[2], G[3] * u[0] + G[4] * u[1] + G[5] * u[2],
+                    G[6] * u[0] + G[7] * u[1] + G[8] * u[2]};
+    double w[3];
+    for (int a = 0; a < 3; ++a) w[a] = -mu * H[a] + mu * mu * Gu[a] + sd * F[a];
+    double t[3];
+    cross3(u, w, t);
+    for (int a = 0; a < 3; ++a) {
+      s.F[a * N + g] += F[a];
+      s.T[a * N + g] += t[a];
+    }
   }
 }
 
@@ -2576,9 +2821,15 @@ static void recip_struct_kz(const State &s, int rt0, int nrt, int maxcol, const 
   }
 }
 template <typename R, int KZ>
-static void recip_force_kz(const State &s, const void *ph, const void *amp, const void *A, hipStream_t st) {
-  MSC_LAUNCH((k_recip_force<R, KZ>), dim3(s.ntask), dim3(64 * kRfGroups), 0, st, s, (const Cpx<R> *)ph,
-             (const R *)amp, (const Cpx<R> *)A);
+static void recip_force_kz(const State &s, const void *ph, const void *amp, const void *A, const void *bop,
+                           const int *bexp, hipStream_t st) {
+  if constexpr (sizeof(R) == sizeof(float)) {
+    MSC_LAUNCH((k_recip_force_mfma<KZ>), dim3(s.ntask), dim3(256 * kBpGroups), 0, st, s, (const Cpx<float> *)ph,
+               (const float *)amp, (const half4_t *)bop, bexp);
+  } else {
+    MSC_LAUNCH((k_recip_force<R, KZ>), dim3(s.ntask), dim3(64 * kRfGroups), 0, st, s, (const Cpx<R> *)ph,
+               (const R *)amp, (const Cpx<R> *)A);
+  }
 }
 
 #define MSC_KZ_CASES(X) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16)
@@ -2598,12 +2849,12 @@ int launch_recip_struct(const State &s, bool mixed, int KZ, int rt0, int nrt, in
   }
 }
 int launch_recip_force(const State &s, bool mixed, int KZ, const void *ph, const void *amp, const void *A,
-                       hipStream_t st) {
+                       const void *bop, const int *bexp, hipStream_t st) {
   switch (KZ) {
 #define CASE(k)                                                        \
   case k:                                                              \
-    if (mixed) recip_force_kz<float, k>(s, ph, amp, A, st);            \
-    else recip_force_kz<double, k>(s, ph, amp, A, st);                 \
+    if (mixed) recip_force_kz<float, k>(s, ph, amp, A, bop, bexp, st); \
+    else recip_force_kz<double, k>(s, ph, amp, A, bop, bexp, st);      \
     return 0;
     MSC_KZ_CASES(CASE)
 #undef CASE
@@ -2612,14 +2863,15 @@ int launch_recip_force(const State &s, bool mixed, int KZ, const void *ph, const
   }
 }
 void launch_recip_reduce(const State &s, bool mixed, const void *part, const double *M2, const double *m2rows,
-                         void *A, double *slot_out, double *kblk, int maxslot, hipStream_t st) {
+                         void *A, double *slot_out, double *kblk, int maxslot, void *bop, const int *bexp,
+                         hipStream_t st) {
   const dim3 grid(nblk(maxslot, 64), s.rep_end - s.rep_begin);
   if (mixed)
     MSC_LAUNCH(k_recip_reduce<float>, grid, dim3(512), 0, st, s, (const float *)part, M2, m2rows,
-               (Cpx<float> *)A, slot_out, kblk, nblk(maxslot, 64));
+               (Cpx<float> *)A, slot_out, kblk, nblk(maxslot, 64), (_Float16 *)bop, bexp);
   else
     MSC_LAUNCH(k_recip_reduce<double>, grid, dim3(512), 0, st, s, (const double *)part, M2, m2rows,
-               (Cpx<double> *)A, slot_out, kblk, nblk(maxslot, 64));
+               (Cpx<double> *)A, slot_out, kblk, nblk(maxslot, 64), (_Float16 *)nullptr, (const int *)nullptr);
 }
 
 void launch_moveP(const State &s, bool last, const Thermo *th, double *task_out, hipStream_t st) {

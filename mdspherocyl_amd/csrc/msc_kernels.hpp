@@ -79,10 +79,15 @@ int launch_recip_struct(const State &s, bool mixed, int KZ, int rt0, int nrt, in
 // M2: per-replica sums of c c^T, or null to sum them from the per-task rows
 // m2rows; kblk (may be null): per-block sums of the 10 slot values,
 // nblk(maxslot, 64) rows per replica
+// bop (mixed contexts): the operands of the matrix-core back projection,
+// kBopPerTile groups of 4 fp16 values per column tile of 16 (RepDev::bop0),
+// scaled by 2^bexp[3 col + l] (RepDev::bexp0)
 void launch_recip_reduce(const State &s, bool mixed, const void *part, const double *M2, const double *m2rows,
-                         void *A, double *slot_out, double *kblk, int maxslot, hipStream_t st);
+                         void *A, double *slot_out, double *kblk, int maxslot, void *bop, const int *bexp,
+                         hipStream_t st);
+constexpr int kBopPerTile = 12 * 64;  // 3 weights x 4 pieces x 64 lanes
 int launch_recip_force(const State &s, bool mixed, int KZ, const void *ph, const void *amp, const void *A,
-                       hipStream_t st);
+                       const void *bop, const int *bexp, hipStream_t st);
 void launch_moveP(const State &s, bool last, const Thermo *th, double *task_out, hipStream_t st);
 void launch_moveX(const State &s, hipStream_t st);
 void launch_renorm(const State &s, hipStream_t st);
