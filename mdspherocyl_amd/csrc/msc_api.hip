@@ -127,6 +127,8 @@ struct msc_ctx {
   RepEnergy *d_E = nullptr;
   Thermo *d_th = nullptr;
   int *d_flags = nullptr, *d_rebuild = nullptr, *d_srebuild = nullptr, *d_one = nullptr;
+  int *d_lctr = nullptr;  // [nrep] arrival counters of the step loop's cell scan (zero between launches)
+  unsigned long long *d_dmax = nullptr;  // [2][nrep][kDmaxBuckets][2] displacement maxima of the step loop
   std::map<std::string, TimerSet> timers;
   std::vector<hipEvent_t> event_pool;
   double *d_run_gs = nullptr, *d_run_obs = nullptr;
@@ -930,6 +932,8 @@ int msc_create(int device, int precision, int nrep, const msc_replica_desc *reps
   A(&ctx->d_flags, (size_t)nrep * 4);
   A(&ctx->d_rebuild, nrep);
   A(&ctx->d_srebuild, nrep);
+  A(&ctx->d_lctr, nrep);
+  A(&ctx->d_dmax, (size_t)4 * kDmaxBuckets * nrep);
   A(&ctx->d_one, nrep);
   A(&ctx->d_stamp, 1);
   // zeroed here once: entries for nz >= K and columns past the last stay 0
@@ -1002,7 +1006,7 @@ void msc_destroy(msc_ctx *ctx) {
                   ctx->d_tab, ctx->d_ph, ctx->d_amp, ctx->d_part, ctx->d_A, ctx->d_slot_out, ctx->d_task_out,
                   ctx->d_red, ctx->d_M2, ctx->d_Kin, ctx->d_gs, ctx->d_obs, ctx->d_counts, ctx->d_E,
                   ctx->d_th, ctx->d_flags, ctx->d_rebuild, ctx->d_one, ctx->d_run_gs, ctx->d_run_obs, ctx->d_save,
-                  ctx->d_stamp, ctx->d_bop, ctx->d_bexp};
+                  ctx->d_stamp, ctx->d_bop, ctx->d_bexp, ctx->d_lctr, ctx->d_dmax};
   for (void *b : bufs)
     if (b) hipFree(b);
   for (auto &kv : ctx->timers)
@@ -1624,14 +1628,20 @@ static void enqueue_step(msc_ctx *ctx) {
   const long region = (long)ctx->tasks.size() * 16;
   double *rows = ctx->d_rows;
   const double *gs = ctx->d_run_gs;
+  // list maintenance (v25/main.cpp:679-701): displacements reduced by
+  // k_drift, decision and cell grid by extra blocks of k_force_sc, k_kick2
+  // and k_end, the rows by k_lists_step; for replicas that keep their rows
+  // these return at once
+  const ListStep ls = {ctx->d_rebuild, ctx->contact ? ctx->d_srebuild : nullptr, ctx->d_lctr, ctx->d_dmax,
+                       ctx->d_run_obs, ctx->d_step};
   {
     Bracket b(ctx, "integrate");
-    launch_kick1(s, ctx->d_th, gs, ctx->d_step, rows + kRowKick1 * region, st);
-    launch_drift(s, ctx->mixed, ctx->d_ph, ctx->d_amp, rows + kRowDrift * region, st);
+    launch_drift(s, ctx->mixed, ctx->d_th, gs, ctx->d_step, rows + kRowKick1 * region, ctx->d_ph, ctx->d_amp,
+                 rows + kRowDrift * region, ls, st);
   }
   {
     Bracket b(ctx, "wca");
-    launch_force_sc(s, !ctx->contact, rows + kRowSC * region, ctx->d_flags, st);
+    launch_force_sc(s, !ctx->contact, rows + kRowSC * region, ctx->d_flags, st, &ls);
   }
   dipR_launches(ctx, MSC_ALL, rows + kRowR * region);
   dipK_launches(ctx, MSC_ALL, rows + kRowDrift * region + 2, ctx->d_kblk);
@@ -1640,15 +1650,13 @@ static void enqueue_step(msc_ctx *ctx) {
   if (field) launch_ext_field(s, rows + kRowExt * region, st);
   {
     Bracket b(ctx, "integrate");
-    launch_kick2(s, ctx->d_th, gs, ctx->d_step, rows + kRowKick1 * region, rows + kRowKick2 * region, st);
+    launch_kick2(s, ctx->d_th, gs, ctx->d_step, rows + kRowKick1 * region, rows + kRowKick2 * region, ls, st);
     launch_end(s, ctx->d_th, gs, ctx->d_step, rows, region, field, ctx->d_kblk, ctx->maxslot, ctx->d_E, ctx->d_Kin,
-               ctx->d_counts, ctx->d_run_obs, ctx->d_rebuild, ctx->d_srebuild, st);
+               ctx->d_counts, ctx->d_run_obs, ls, st);
   }
-  // list maintenance (v25/main.cpp:679-701): decided in k_end; the launches
-  // return at once for replicas that keep their rows
   {
     Bracket b(ctx, "nlist");
-    launch_rebuild(s, ctx->d_rebuild, ctx->contact ? ctx->d_srebuild : nullptr, ctx->d_flags, st, ctx->d_step);
+    launch_lists_step(s, ls, ctx->d_flags, ctx->d_step, st);
   }
 }
 
@@ -1749,6 +1757,9 @@ int msc_run(msc_ctx *ctx, int step0, int nsteps, double *obs) {
     const int nk = std::min(kRunChunk, nsteps - k0);
     if (int rc_ = state_copy(ctx, false)) return rc_;
     launch_fill_int(ctx->d_flags, 4 * nrep, 0, ctx->stream);
+    // both parities of the displacement maxima start at zero (k_drift then
+    // clears the next step's before it is used)
+    launch_fill_int(reinterpret_cast<int *>(ctx->d_dmax), 8 * kDmaxBuckets * nrep, 0, ctx->stream);
     const int step_pos[2] = {k0, step0};
     if (int rc_ = h2d(ctx, ctx->d_step, step_pos, sizeof step_pos)) return rc_;
     // event brackets and per-launch synchronisation cannot sit in a replayed
@@ -1762,7 +1773,7 @@ int msc_run(msc_ctx *ctx, int step0, int nsteps, double *obs) {
     }
     LAUNCHED("msc_run");
     if (int rc_ = d2h(ctx, fl.data(), ctx->d_flags, sizeof(int) * fl.size())) return rc_;
-    // every step advances the device step counter (k_cells_assign): a chunk
+    // every step advances the device step counter (k_lists_step): a chunk
     // whose kernels did not all execute leaves it behind
     int pos[2] = {-1, -1};
     if (int rc_ = d2h(ctx, pos, ctx->d_step, sizeof pos)) return rc_;

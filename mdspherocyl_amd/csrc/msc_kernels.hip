@@ -114,6 +114,22 @@ __device__ __forceinline__ int wave_task_id() {
   return __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave);
 }
 
+// True in every thread of the block that arrives last among the `total`
+// blocks counting on *ctr, which it resets for the next launch.  The block's
+// global writes before the call are visible to the last block after it.
+__device__ bool last_block_arrival(int *ctr, int total) {
+  __shared__ int last;
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    last = atomicAdd(ctr, 1) == total - 1;
+    if (last) *ctr = 0;
+  }
+  __syncthreads();
+  if (last) __threadfence();
+  return last;
+}
+
 template <typename R>
 __device__ __forceinline__ R fmaR(R a, R b, R c) { return fma(a, b, c); }
 
@@ -149,10 +165,9 @@ __global__ void k_dipoles(State s) {
 // --------------------------------------------------------------------------
 // cell grid (integer binning of wrapped centres) and neighbour lists
 // --------------------------------------------------------------------------
-__device__ __forceinline__ void cells_assign_one(const State &s, int g, const int *__restrict__ rebuild) {
-  int rp = s.rep_of[g];
-  if (!rebuild[rp]) return;
-  const RepDev &R = s.reps[rp];
+// Cell counters are zero outside a rebuild: the scan that consumes them
+// clears them, and the fill cursors are set by the scan.
+__device__ __forceinline__ void cells_assign_do(const State &s, const RepDev &R, int g) {
   int ci[3];
   for (int a = 0; a < 3; ++a) {
     double x = s.x[a * s.Ntot + g];
@@ -165,20 +180,26 @@ __device__ __forceinline__ void cells_assign_one(const State &s, int g, const in
   s.cell_of[g] = cell;
   atomicAdd(&s.cell_cnt[cell], 1);
 }
-// step (may be null): the step counter of the step loop, advanced here,
-// after every kernel of the step that reads it
-__global__ void k_cells_assign(State s, const int *__restrict__ rebuild, int *__restrict__ step) {
-  if (step && blockIdx.x == 0 && threadIdx.x == 0) step[0] += 1;
+__device__ __forceinline__ void cells_assign_one(const State &s, int g, const int *__restrict__ rebuild) {
+  const int rp = s.rep_of[g];
+  if (rebuild[rp]) cells_assign_do(s, s.reps[rp], g);
+}
+// one past the last cell-order position of global cell c of replica R
+__device__ __forceinline__ int cell_end(const State &s, const RepDev &R, int c) {
+  return c + 1 < R.cell0 + R.ncell ? s.cell_start[c + 1] : R.n0 + R.N;
+}
+__global__ void k_cells_assign(State s, const int *__restrict__ rebuild) {
   int g = s.g_begin + blockIdx.x * blockDim.x + threadIdx.x;
   if (g < s.g_end) cells_assign_one(s, g, rebuild);
 }
 
-// Exclusive scan of the cell counts of replica rp into cell starts by one
-// block (each of its nt threads owns a contiguous run of cells; sh holds nt
-// ints); the replica's particles occupy [n0, n0 + N) of the cell order.
+// Exclusive scan of the cell counts of replica rp into cell starts (out,
+// and the fill cursors cur) by one block (each of its nt threads owns a
+// contiguous run of cells; sh holds nt ints); clears the counts.  The
+// replica's particles occupy [n0, n0 + N) of the cell order.
 // Block-uniform: every thread of the block calls it with the same rp.
-__device__ void scan_cells_block(const int *__restrict__ cnt, int *__restrict__ out, const RepDev &R, int *sh,
-                                 int nt) {
+__device__ void scan_cells_block(int *__restrict__ cnt, int *__restrict__ out, int *__restrict__ cur,
+                                 const RepDev &R, int *sh, int nt) {
   const int b0 = R.cell0, n = R.ncell, base = R.n0;
   const int t = threadIdx.x;
   const int seg = (n + nt - 1) / nt;
@@ -197,24 +218,69 @@ __device__ void scan_cells_block(const int *__restrict__ cnt, int *__restrict__ 
   for (int i = b; i < e; ++i) {
     int v = cnt[b0 + i];
     out[b0 + i] = run;
+    cur[b0 + i] = run;
+    cnt[b0 + i] = 0;
     run += v;
   }
   __syncthreads();
 }
 __global__ void __launch_bounds__(1024) k_scan_cells(int *__restrict__ cnt, int *__restrict__ out,
-                                                     const RepDev *__restrict__ reps,
+                                                     int *__restrict__ cur, const RepDev *__restrict__ reps,
                                                      const int *__restrict__ rebuild) {
   __shared__ int sh[1024];
   const int rp = blockIdx.x;
   if (!rebuild[rp]) return;
-  scan_cells_block(cnt, out, reps[rp], sh, 1024);
+  scan_cells_block(cnt, out, cur, reps[rp], sh, 1024);
 }
 
-// particles into their cells in arrival order ...
+// The list decision of the step loop (v25/main.cpp:679-701, as
+// k_check_decide) for replica rp from the largest displacements k_drift
+// reduced into ls.dmax: rebuild when 2 sqrt(3) max|dx| exceeds the skin;
+// else refresh the contact rows (srb) when the largest segment-point
+// displacement since they were built exceeds half the contact skin.
+// Wave-uniform (each lane reads one bucket).
+__device__ __forceinline__ bool list_decision(const State &s, const RepDev &R, int rp, const ListStep &ls, int k,
+                                              bool &srb) {
+  static_assert(kDmaxBuckets <= 64, "one bucket per lane");
+  const int lane = threadIdx.x & 63;
+  const unsigned long long *d = ls.dmax + (((size_t)(k & 1) * s.nrep + rp) * kDmaxBuckets + lane % kDmaxBuckets) * 2;
+  double dm = wave_max(__longlong_as_double((long long)d[0]));
+  const double me = wave_max(__longlong_as_double((long long)d[1]));
+  dm = 2. * sqrt(3. * dm * dm);
+  const bool rb = dm > R.r_list_max - R.r_list_min;
+  srb = !rb && 2. * me > R.skin_short;
+  return rb;
+}
+
+// Extra block t of k_force_sc in the step loop: the list decision of task
+// t's replica (recorded by the replica's first task), then on a rebuild the
+// cell assignment of the task's particles (one per thread); the last block
+// of the replica to finish scans the counts (k_scan_cells).
+__device__ void cells_assign_task(const State &s, int t, const ListStep &ls) {
+  __shared__ int sh[64 * kPairSplit];
+  const int tid = s.task_begin + t;
+  const Task tk = s.tasks[tid];
+  const RepDev &R = s.reps[tk.rep];
+  const int k = __builtin_amdgcn_readfirstlane(ls.step[0]);
+  bool srb;
+  const bool rb = list_decision(s, R, tk.rep, ls, k, srb);
+  if (tid == R.task0 && threadIdx.x == 0) {
+    ls.rebuild[tk.rep] = rb;
+    if (ls.srebuild) ls.srebuild[tk.rep] = srb;
+    if (ls.obs) ls.obs[((size_t)k * s.nrep + tk.rep) * 12 + 11] = rb;
+  }
+  if (!rb) return;
+  if ((int)threadIdx.x < tk.n) cells_assign_do(s, R, tk.g0 + threadIdx.x);
+  if (last_block_arrival(ls.ctr + tk.rep, R.ntask))
+    scan_cells_block(s.cell_cnt, s.cell_start, s.cell_fill, R, sh, blockDim.x);
+}
+
+// particles into their cells in arrival order (cell_fill: the cursors the
+// scan set to the cell starts) ...
 __device__ __forceinline__ void cells_fill_one(const State &s, int g, const int *__restrict__ rebuild) {
   if (!rebuild[s.rep_of[g]]) return;
   int c = s.cell_of[g];
-  int pos = s.cell_start[c] + atomicAdd(&s.cell_fill[c], 1);
+  int pos = atomicAdd(&s.cell_fill[c], 1);
   s.cell_tmp[pos] = g;
 }
 __global__ void k_cells_fill(State s, const int *__restrict__ rebuild) {
@@ -228,9 +294,10 @@ __global__ void k_cells_fill(State s, const int *__restrict__ rebuild) {
 // its centre, axis, length and index at its cell-order position t, packed
 // into one 64-byte record so that a candidate costs one cache line.
 __device__ __forceinline__ void cells_rank_one(const State &s, int g, const int *__restrict__ rebuild) {
-  if (!rebuild[s.rep_of[g]]) return;
+  const int rp = s.rep_of[g];
+  if (!rebuild[rp]) return;
   int c = s.cell_of[g];
-  int b = s.cell_start[c], n = s.cell_cnt[c];
+  int b = s.cell_start[c], n = cell_end(s, s.reps[rp], c) - b;
   int rank = 0;
   for (int k = 0; k < n; ++k) rank += s.cell_tmp[b + k] < g;
   const int t = b + rank, N = s.Ntot;
@@ -400,7 +467,7 @@ __device__ void build_lists_wave(const State &s, int t, const int *__restrict__ 
     const int crow = R.cell0 + (cz * R.nc[1] + cy) * ncx;
     const int cl = crow + (ru ? runs[1][0] : runs[0][0]), ch = crow + (ru ? runs[1][1] : runs[0][1]);
     RB[lane] = s.cell_start[cl];
-    RE[lane] = s.cell_start[ch] + s.cell_cnt[ch];
+    RE[lane] = cell_end(s, R, ch);
   }
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -604,6 +671,31 @@ __global__ void __launch_bounds__(256) k_short_build(State s, const int *__restr
   if (g < s.g_end) short_build_wave(s, g, srebuild, flags, queue[wv]);
 }
 
+// The step loop's list launch, one wave per particle: the rows of every
+// replica whose list is rebuilt (its cell grid was rebuilt by the step's
+// earlier launches) and the contact rows of every replica that keeps its
+// reference rows but has used up its contact skin.  Each wave first reads
+// the replicas' flags (one load per lane), so that in a step without either
+// every wave ends after one load.  Advances the step counter: every kernel
+// of the step that reads it has finished.
+__global__ void __launch_bounds__(256) k_lists_step(State s, ListStep ls, int *__restrict__ flags,
+                                                    int *__restrict__ step) {
+  __shared__ int queue[4][128];
+  __shared__ int run_b[4][64], run_e[4][64];
+  if (blockIdx.x == 0 && threadIdx.x == 0) step[0] += 1;
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  bool any = false;
+  for (int r = s.rep_begin + lane; r < s.rep_end; r += 64)
+    any = any || ls.rebuild[r] || (ls.srebuild && ls.srebuild[r]);
+  if (__ballot(any) == 0) return;
+  const int g = s.g_begin + __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + wv);
+  if (g >= s.g_end) return;
+  const int rp = __builtin_amdgcn_readfirstlane(s.rep_of[g]);
+  if (ls.rebuild[rp])
+    build_lists_wave(s, g, ls.rebuild, flags, queue[wv], run_b[wv], run_e[wv]);
+  else if (ls.srebuild && ls.srebuild[rp])
+    short_build_wave(s, g, ls.srebuild, flags, queue[wv]);
+}
 
 // --------------------------------------------------------------------------
 // savec_list / check_list (v25/functions.cpp:1336-1394)
@@ -687,13 +779,19 @@ __global__ void __launch_bounds__(256) k_check_decide(State s, const double *__r
 // within r_short, which hold every pair inside 2^(1/6) sigma until the
 // contact skin is used up) are evaluated; the others contribute exactly zero.
 template <bool STRICT>
+// Extra blocks (ls.rebuild set; block ntask + t for task t): the cell
+// assignment of rebuilt replicas; the replica's last such block scans its
+// cell counts into cell starts.
 __global__ void __launch_bounds__(64 * kPairSplit) k_force_sc(State s, double *__restrict__ task_out,
-                                                               int *__restrict__ flags) {
+                                                               int *__restrict__ flags, ListStep ls) {
   // one block per wave task; wave sw of the block takes row entries
   // q = sw (mod kPairSplit); partial sums meet in LDS in a fixed order
   __shared__ double red[kPairSplit][18][kWave];
   const int lane = threadIdx.x & 63, sw = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  if ((int)blockIdx.x >= s.ntask) return;
+  if ((int)blockIdx.x >= s.ntask) {
+    if (ls.rebuild && (int)blockIdx.x < 2 * s.ntask) cells_assign_task(s, (int)blockIdx.x - s.ntask, ls);
+    return;
+  }
   const int tid = s.task_begin + blockIdx.x;
   const Task tk = s.tasks[tid];
   const RepDev &R = s.reps[tk.rep];
@@ -2494,57 +2592,62 @@ __global__ void k_zero_FT(State s) {
 // sums a kernel needs are re-summed from per-task rows by every wave that
 // needs them (wave_rows_sum: same rows, same order, same result), so no
 // kernel waits for a separate reduction launch:
-//   k_kick1  moveTher mode 0 + first moveP           -> rows kick1 [0..1]
-//   k_drift  moveX, dipoles, F = T = 0, the list check and the reciprocal
-//            phase tables                             -> rows drift [0..7]
+//   k_drift  moveTher mode 0 + first moveP, moveX, dipoles, F = T = 0, the
+//            list check and the reciprocal phase tables
+//                                      -> rows kick1 [0..1], drift [0..7]
+//            and the replica's largest displacements (atomic max)
 //   (force kernels)                                   -> rows sc, R, ext
 //   k_kick2  moveTher mode 1 (from rows kick1) + last moveP -> rows kick2
 //   k_end    every per-replica sum, moveTher mode 2, energies, observables
-//            and the rebuild decision (one block per replica).
+//            (one block per replica)
+//   k_lists_step  the rows of replicas whose list is rebuilt or whose
+//            contact rows are refreshed; advances the step counter.
+// The list decision and a replica's cell grid are computed by extra blocks
+// of the launches before k_lists_step (ListStep): decision, assign and
+// per-replica scan in k_force_sc, fill in k_kick2, rank in k_end.  None of
+// those launches' other blocks reads the cell grid, and the positions do not
+// change between k_drift and the next step's k_drift, so the grid and rows
+// are built from the positions the reference builds them from (after the
+// step's moveX).  Without a rebuild the extra blocks return at once, and the
+// list maintenance costs one launch.
 // --------------------------------------------------------------------------
 // step[0] = k, the step within the run (it selects the step's deviates
 // gs_base + 4 nrep k and observable rows), step[1] = the run's first
-// absolute step; the step counter advances in the step's first list kernel.
-__global__ void __launch_bounds__(256) k_kick1(State s, const Thermo *__restrict__ th,
-                                               const double *__restrict__ gs_base, const int *__restrict__ step,
-                                               double *__restrict__ rows) {
-  const int tid = s.task_begin + wave_task_id();
-  const int lane = threadIdx.x & 63;
-  if (wave_task_id() >= s.ntask) return;
-  const Task tk = s.tasks[tid];
-  const RepDev &Rp = s.reps[tk.rep];
-  const int k = __builtin_amdgcn_readfirstlane(step[0]);
-  const double *gs = gs_base + (size_t)k * s.nrep * 4;
-  // renorm every 100 steps at the start of the step
-  const bool renorm = fmodf((float)(step[1] + k), 100.f) == 0.f;
-  double acc[14] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-  if (lane < tk.n) {
-    if (renorm) renorm_particle(s, tk.g0 + lane);
-    const Thermo T = th[tk.rep];
-    const double ts[2] = {thermo_half(Rp, T.s[0], gs[tk.rep * 4 + 0]), thermo_half(Rp, T.s[1], gs[tk.rep * 4 + 2])};
-    kick_particle<false>(s, Rp, tk.g0 + lane, ts, acc);
-  }
-  wave_store_sums(rows + (size_t)tid * 16, acc, 2, lane);
-}
+// absolute step; the step counter advances in k_lists_step.
 
-// One block of 4 waves per task.  Wave 0 moves the task's particles and
-// writes rows: [0] largest centre-component displacement since savec_list,
-// [1] largest segment-point displacement since the contact rows were built
-// (as k_check_list), [2..7] sums of c c^T (as k_recip_tables); waves 0-2
-// then write the phase tables of axes 0-2 (one particle per lane).
+// One block of 4 waves per task.  Wave 0 renormalises every 100 steps,
+// kicks (moveTher mode 0 + moveP) and moves the task's particles and writes
+// rows1: [0..1] twice the kinetic energies after the kick, and rows: [0]
+// largest centre-component displacement since savec_list, [1] largest
+// segment-point displacement since the contact rows were built (as
+// k_check_list), [2..7] sums of c c^T (as k_recip_tables); waves 0-2 then
+// write the phase tables of axes 0-2 (one particle per lane).  In the step
+// loop (ls.dmax set) wave 0 also reduces the displacements into the
+// replica's maxima for the list decision.
 template <typename R>
-__global__ void __launch_bounds__(256) k_drift(State s, Cpx<R> *__restrict__ ph, R *__restrict__ amp,
-                                               double *__restrict__ rows) {
+__global__ void __launch_bounds__(256) k_drift(State s, const Thermo *__restrict__ th,
+                                               const double *__restrict__ gs_base, const int *__restrict__ step,
+                                               double *__restrict__ rows1, Cpx<R> *__restrict__ ph,
+                                               R *__restrict__ amp, double *__restrict__ rows, ListStep ls) {
   __shared__ double th_sh[3][64];
   const int tid = s.task_begin + __builtin_amdgcn_readfirstlane(blockIdx.x);
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const Task tk = s.tasks[tid];
   const RepDev &Rp = s.reps[tk.rep];
   const int N = s.Ntot;
+  const int k = __builtin_amdgcn_readfirstlane(step[0]);
   if (wv == 0) {
+    const double *gs = gs_base + (size_t)k * s.nrep * 4;
+    // renorm every 100 steps at the start of the step
+    const bool renorm = fmodf((float)(step[1] + k), 100.f) == 0.f;
+    double acc[14] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     double m = 0.0, me = 0.0, m2[6] = {0, 0, 0, 0, 0, 0};
     if (lane < tk.n) {
       const int g = tk.g0 + lane;
+      if (renorm) renorm_particle(s, g);
+      const Thermo T = th[tk.rep];
+      const double ts[2] = {thermo_half(Rp, T.s[0], gs[tk.rep * 4 + 0]), thermo_half(Rp, T.s[1], gs[tk.rep * 4 + 2])};
+      kick_particle<false>(s, Rp, g, ts, acc);
       drift_particle(s, Rp, g);
       dipoles_particle(s, g);
       double dr2 = 0, du2 = 0;
@@ -2565,16 +2668,31 @@ __global__ void __launch_bounds__(256) k_drift(State s, Cpx<R> *__restrict__ ph,
       me = sqrt(dr2) + 0.5 * s.len[g] * sqrt(du2);
       recip_amp_particle<R>(s, g, amp, m2);
     }
+    wave_store_sums(rows1 + (size_t)tid * 16, acc, 2, lane);
     m = wave_max(m);
     me = wave_max(me);
     double *o = rows + (size_t)tid * 16;
     if (lane == 0) {
       o[0] = m;
       o[1] = me;
+      if (ls.dmax) {
+        // replica maxima for the list decision in kDmaxBuckets buckets
+        // (fewer tasks per address; non-negative doubles order as their bit
+        // patterns)
+        unsigned long long *d = ls.dmax + (((size_t)(k & 1) * s.nrep + tk.rep) * kDmaxBuckets + (tid & (kDmaxBuckets - 1))) * 2;
+        atomicMax(d, (unsigned long long)__double_as_longlong(m));
+        atomicMax(d + 1, (unsigned long long)__double_as_longlong(me));
+      }
     }
-    for (int k = 0; k < 6; ++k) {
-      const double t = wave_sum(m2[k]);
-      if (lane == 0) o[2 + k] = t;
+    // the replica's first task clears the buckets of the next step
+    if (ls.dmax && tid == Rp.task0 && lane < kDmaxBuckets) {
+      unsigned long long *dn = ls.dmax + (((size_t)((k + 1) & 1) * s.nrep + tk.rep) * kDmaxBuckets + lane) * 2;
+      dn[0] = 0;
+      dn[1] = 0;
+    }
+    for (int q = 0; q < 6; ++q) {
+      const double t = wave_sum(m2[q]);
+      if (lane == 0) o[2 + q] = t;
     }
   }
   __syncthreads();
@@ -2583,9 +2701,18 @@ __global__ void __launch_bounds__(256) k_drift(State s, Cpx<R> *__restrict__ ph,
 
 // moveTher mode 1 from the first kick's energies, re-summed by each wave,
 // then the last moveP.  rows2: [0..13] as kick_particle<true>.
+// Extra blocks (ls.rebuild set, beyond the tasks' blocks): the cell fill of
+// rebuilt replicas, one particle per thread.
 __global__ void __launch_bounds__(256) k_kick2(State s, const Thermo *__restrict__ th,
                                                const double *__restrict__ gs_base, const int *__restrict__ step,
-                                               const double *__restrict__ rows1, double *__restrict__ rows2) {
+                                               const double *__restrict__ rows1, double *__restrict__ rows2,
+                                               ListStep ls) {
+  const int nmain = (s.ntask + 3) / 4;
+  if ((int)blockIdx.x >= nmain) {
+    const int g = s.g_begin + ((int)blockIdx.x - nmain) * 256 + (int)threadIdx.x;
+    if (ls.rebuild && g < s.g_end) cells_fill_one(s, g, ls.rebuild);
+    return;
+  }
   const int tid = s.task_begin + wave_task_id();
   const int lane = threadIdx.x & 63;
   if (wave_task_id() >= s.ntask) return;
@@ -2605,20 +2732,26 @@ __global__ void __launch_bounds__(256) k_kick2(State s, const Thermo *__restrict
 // One block (4 waves) per replica.  rows + k*region is row set k (kick1,
 // drift, sc, R, ext, kick2).  Writes the thermostat after modes 0, 1 and 2
 // (k_thermo0 / k_step_scalars), the energies in the layout of RepEnergy,
-// Kin, the pair counts, the observables of print_out and the list decision
-// of k_check_decide; zeroes the cell counters of a replica whose list is
-// rebuilt next.
+// Kin, the pair counts and the observables of print_out but the rebuild flag
+// (written with the decision in k_drift).  Extra blocks (ls.rebuild set,
+// beyond the replicas' blocks): the cell rank and packed records of rebuilt
+// replicas, one particle per thread.
 __global__ void __launch_bounds__(256) k_end(State s, Thermo *__restrict__ th, const double *__restrict__ gs_base,
                                              const int *__restrict__ step, const double *__restrict__ rows,
                                              long region, int has_field, const double *__restrict__ kblk,
                                              int kblk_stride, RepEnergy *__restrict__ E, double *__restrict__ Kin,
                                              double *__restrict__ counts, double *__restrict__ obs_base,
-                                             int *__restrict__ rebuild, int *__restrict__ srebuild) {
+                                             ListStep ls) {
+  const int nmain = s.rep_end - s.rep_begin;
+  if ((int)blockIdx.x >= nmain) {
+    const int g = s.g_begin + ((int)blockIdx.x - nmain) * 256 + (int)threadIdx.x;
+    if (ls.rebuild && g < s.g_end) cells_rank_one(s, g, ls.rebuild);
+    return;
+  }
   const int kstep = step[0];
   const double *gs = gs_base + (size_t)kstep * s.nrep * 4;
   double *obs = obs_base + (size_t)kstep * s.nrep * 12;
-  __shared__ double v1[2], v2[14], vsc[12], vr[11], vext[1], vchk[2], vk[10];
-  __shared__ int rb_sh;
+  __shared__ double v1[2], v2[14], vsc[12], vr[11], vext[1], vk[10];
   const int rp = s.rep_begin + blockIdx.x;
   const RepDev &R = s.reps[rp];
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -2638,19 +2771,9 @@ __global__ void __launch_bounds__(256) k_end(State s, Thermo *__restrict__ th, c
     double a[11], e[1] = {0.0};
     wave_rows_sum<11>(rows + 3 * region, R.task0, R.ntask, lane, a);
     if (has_field) wave_rows_sum<1>(rows + 4 * region, R.task0, R.ntask, lane, e);
-    double m = 0.0, me = 0.0;
-    const double *dr = rows + region;
-    for (int t = lane; t < R.ntask; t += 64) {
-      m = fmax(m, dr[(size_t)(R.task0 + t) * 16]);
-      me = fmax(me, dr[(size_t)(R.task0 + t) * 16 + 1]);
-    }
-    m = wave_max(m);
-    me = wave_max(me);
     if (lane == 0) {
       for (int k = 0; k < 11; ++k) vr[k] = a[k];
       vext[0] = e[0];
-      vchk[0] = m;
-      vchk[1] = me;
     }
   } else {
     const int nb = (R.ncol * R.KZ + 63) / 64;
@@ -2698,21 +2821,7 @@ __global__ void __launch_bounds__(256) k_end(State s, Thermo *__restrict__ th, c
     const double *mo = v2 + 8;
     o[9] = mo[0] * mo[0] + mo[1] * mo[1] + mo[2] * mo[2];
     o[10] = mo[3] * mo[3] + mo[4] * mo[4] + mo[5] * mo[5];
-    // list decision (as k_check_decide in the step loop)
-    double dm = vchk[0];
-    dm = 2. * sqrt(3. * dm * dm);
-    const int rb = dm > R.r_list_max - R.r_list_min;
-    rebuild[rp] = rb;
-    srebuild[rp] = !rb && 2. * vchk[1] > R.skin_short;
-    o[11] = rb;
-    rb_sh = rb;
   }
-  __syncthreads();
-  if (rb_sh)
-    for (int c = threadIdx.x; c < R.ncell; c += 256) {
-      s.cell_cnt[R.cell0 + c] = 0;
-      s.cell_fill[R.cell0 + c] = 0;
-    }
 }
 
 // completion stamp: written after an entry point's kernels, read back by the
@@ -2735,23 +2844,18 @@ static inline int nblk(int n, int b) { return (n + b - 1) / b; }
 void launch_dipoles(const State &s, hipStream_t st) {
   MSC_LAUNCH(k_dipoles, dim3(nblk(s.g_end - s.g_begin, 256)), dim3(256), 0, st, s);
 }
-void launch_cells(const State &s, const int *rebuild, bool zero, hipStream_t st, int *step) {
+void launch_cells(const State &s, const int *rebuild, bool zero, hipStream_t st) {
   if (zero) {
     MSC_LAUNCH(k_fill_int, dim3(nblk(s.ncells, 256)), dim3(256), 0, st, s.cell_cnt, s.ncells, 0);
     MSC_LAUNCH(k_fill_int, dim3(nblk(s.ncells, 256)), dim3(256), 0, st, s.cell_fill, s.ncells, 0);
   }
-  MSC_LAUNCH(k_cells_assign, dim3(nblk(s.g_end - s.g_begin, 256)), dim3(256), 0, st, s, rebuild, step);
-  MSC_LAUNCH(k_scan_cells, dim3(s.nrep), dim3(1024), 0, st, s.cell_cnt, s.cell_start, s.reps, rebuild);
+  MSC_LAUNCH(k_cells_assign, dim3(nblk(s.g_end - s.g_begin, 256)), dim3(256), 0, st, s, rebuild);
+  MSC_LAUNCH(k_scan_cells, dim3(s.nrep), dim3(1024), 0, st, s.cell_cnt, s.cell_start, s.cell_fill, s.reps, rebuild);
   MSC_LAUNCH(k_cells_fill, dim3(nblk(s.g_end - s.g_begin, 256)), dim3(256), 0, st, s, rebuild);
   MSC_LAUNCH(k_cells_rank, dim3(nblk(s.g_end - s.g_begin, 256)), dim3(256), 0, st, s, rebuild);
 }
 void launch_lists(const State &s, const int *rebuild, int *flags, hipStream_t st) {
   MSC_LAUNCH(k_build_lists, dim3(nblk(s.g_end - s.g_begin, 4)), dim3(256), 0, st, s, rebuild, flags);
-}
-void launch_rebuild(const State &s, const int *rebuild, const int *srebuild, int *flags, hipStream_t st, int *step) {
-  launch_cells(s, rebuild, false, st, step);
-  launch_lists(s, rebuild, flags, st);
-  if (srebuild) launch_short(s, srebuild, flags, st);
 }
 void launch_short(const State &s, const int *srebuild, int *flags, hipStream_t st) {
   MSC_LAUNCH(k_short_build, dim3(nblk(s.g_end - s.g_begin, 4)), dim3(256), 0, st, s, srebuild, flags);
@@ -2762,11 +2866,15 @@ void launch_check(const State &s, double *task_out, int *rebuild, int *srebuild,
   MSC_LAUNCH(k_check_decide, dim3(s.rep_end - s.rep_begin), dim3(256), 0, st, s, task_out, rebuild, srebuild,
              obs_rebuild, obs_stride, standalone ? 1 : 0);
 }
-void launch_force_sc(const State &s, bool strict, double *task_out, int *flags, hipStream_t st) {
+void launch_force_sc(const State &s, bool strict, double *task_out, int *flags, hipStream_t st,
+                     const ListStep *ls) {
+  const ListStep none = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  const ListStep l = ls ? *ls : none;
+  const dim3 grid(ls ? 2 * s.ntask : s.ntask);
   if (strict)
-    MSC_LAUNCH(k_force_sc<true>, dim3(s.ntask), dim3(64 * kPairSplit), 0, st, s, task_out, flags);
+    MSC_LAUNCH(k_force_sc<true>, grid, dim3(64 * kPairSplit), 0, st, s, task_out, flags, l);
   else
-    MSC_LAUNCH(k_force_sc<false>, dim3(s.ntask), dim3(64 * kPairSplit), 0, st, s, task_out, flags);
+    MSC_LAUNCH(k_force_sc<false>, grid, dim3(64 * kPairSplit), 0, st, s, task_out, flags, l);
 }
 void launch_dipR(const State &s, bool mixed, bool all_pairs, const void *tab, double *task_out, hipStream_t st) {
   if (mixed) {
@@ -2915,24 +3023,32 @@ void launch_clusters(const State &s, double min_s, double reach_max, double u_cu
              flags);
   MSC_LAUNCH(k_clust_final, grid, dim3(256), 0, st, s, par);
 }
-void launch_kick1(const State &s, const Thermo *th, const double *gs, const int *step, double *rows, hipStream_t st) {
-  MSC_LAUNCH(k_kick1, dim3(nblk(s.ntask, 4)), dim3(256), 0, st, s, th, gs, step, rows);
-}
-void launch_drift(const State &s, bool mixed, void *ph, void *amp, double *rows, hipStream_t st) {
+void launch_drift(const State &s, bool mixed, const Thermo *th, const double *gs, const int *step, double *rows1,
+                  void *ph, void *amp, double *rows, const ListStep &ls, hipStream_t st) {
   if (mixed)
-    MSC_LAUNCH(k_drift<float>, dim3(s.ntask), dim3(256), 0, st, s, (Cpx<float> *)ph, (float *)amp, rows);
+    MSC_LAUNCH(k_drift<float>, dim3(s.ntask), dim3(256), 0, st, s, th, gs, step, rows1, (Cpx<float> *)ph,
+               (float *)amp, rows, ls);
   else
-    MSC_LAUNCH(k_drift<double>, dim3(s.ntask), dim3(256), 0, st, s, (Cpx<double> *)ph, (double *)amp, rows);
+    MSC_LAUNCH(k_drift<double>, dim3(s.ntask), dim3(256), 0, st, s, th, gs, step, rows1, (Cpx<double> *)ph,
+               (double *)amp, rows, ls);
+}
+// extra blocks of the step loop's cell phases: one particle per thread
+static inline int cell_blocks(const State &s, const ListStep &ls) {
+  return ls.rebuild ? nblk(s.g_end - s.g_begin, 256) : 0;
 }
 void launch_kick2(const State &s, const Thermo *th, const double *gs, const int *step, const double *rows1,
-                  double *rows2, hipStream_t st) {
-  MSC_LAUNCH(k_kick2, dim3(nblk(s.ntask, 4)), dim3(256), 0, st, s, th, gs, step, rows1, rows2);
+                  double *rows2, const ListStep &ls, hipStream_t st) {
+  MSC_LAUNCH(k_kick2, dim3(nblk(s.ntask, 4) + cell_blocks(s, ls)), dim3(256), 0, st, s, th, gs, step, rows1, rows2,
+             ls);
 }
 void launch_end(const State &s, Thermo *th, const double *gs, const int *step, const double *rows, long region,
                 bool has_field, const double *kblk, int maxslot, RepEnergy *E, double *Kin, double *counts,
-                double *obs, int *rebuild, int *srebuild, hipStream_t st) {
-  MSC_LAUNCH(k_end, dim3(s.rep_end - s.rep_begin), dim3(256), 0, st, s, th, gs, step, rows, region,
-             has_field ? 1 : 0, kblk, nblk(maxslot, 64), E, Kin, counts, obs, rebuild, srebuild);
+                double *obs, const ListStep &ls, hipStream_t st) {
+  MSC_LAUNCH(k_end, dim3(s.rep_end - s.rep_begin + cell_blocks(s, ls)), dim3(256), 0, st, s, th, gs, step, rows,
+             region, has_field ? 1 : 0, kblk, nblk(maxslot, 64), E, Kin, counts, obs, ls);
+}
+void launch_lists_step(const State &s, const ListStep &ls, int *flags, int *step, hipStream_t st) {
+  MSC_LAUNCH(k_lists_step, dim3(nblk(s.g_end - s.g_begin, 4)), dim3(256), 0, st, s, ls, flags, step);
 }
 void launch_stamp(double *p, double v, hipStream_t st) { MSC_LAUNCH(k_stamp, dim3(1), dim3(64), 0, st, p, v); }
 void launch_fill_int(int *p, int n, int v, hipStream_t st) {

@@ -39,6 +39,22 @@ struct State {
   const double *kval;
 };
 
+// List maintenance of the step loop (v25/main.cpp:679-701).  k_drift
+// reduces the displacements into dmax; extra blocks of k_force_sc decide
+// (rebuild, srebuild, obs[.][11]) and, with those of the step's later
+// launches, rebuild the cell grid.
+constexpr int kDmaxBuckets = 64;  // power of two, at most 64
+struct ListStep {
+  int *rebuild;              // [nrep] rebuild the reference rows this step
+  int *srebuild;             // [nrep] refresh the contact rows (null: no contact rows)
+  int *ctr;                  // [nrep] arrival counters, zero between launches
+  unsigned long long *dmax;  // [2][nrep][kDmaxBuckets][2] by step parity: largest
+                             // centre-component and segment-point displacements
+                             // (double bits); zero before the step that fills them
+  double *obs;               // observables (12 per replica and step) or null
+  const int *step;           // the step counter
+};
+
 // First failed kernel launch since the record was last taken.
 struct LaunchRecord {
   hipError_t err = hipSuccess;
@@ -54,18 +70,17 @@ hipError_t take_launch_error(const char **kernel);
 bool launches_checked();
 
 void launch_dipoles(const State &s, hipStream_t st);
-// zero: clear every cell counter first (the step loop's k_end clears those
-// of the replicas it rebuilds)
-void launch_cells(const State &s, const int *rebuild, bool zero, hipStream_t st, int *step = nullptr);
+// zero: clear every cell counter first (they are zero outside a rebuild:
+// the scan clears those it consumes)
+void launch_cells(const State &s, const int *rebuild, bool zero, hipStream_t st);
 void launch_lists(const State &s, const int *rebuild, int *flags, hipStream_t st);
 void launch_short(const State &s, const int *srebuild, int *flags, hipStream_t st);
-// the step's list maintenance: cell grid and rows of the replicas with
-// rebuild set (cell counters already zero), contact rows of those with
-// srebuild set (srebuild may be null)
-void launch_rebuild(const State &s, const int *rebuild, const int *srebuild, int *flags, hipStream_t st, int *step);
 void launch_check(const State &s, double *task_out, int *rebuild, int *srebuild, double *obs_rebuild,
                   int obs_stride, bool standalone, hipStream_t st);
-void launch_force_sc(const State &s, bool strict, double *task_out, int *flags, hipStream_t st);
+// ls (step loop only): the cell assignment and scan of rebuilt replicas run
+// in extra blocks of the same launch
+void launch_force_sc(const State &s, bool strict, double *task_out, int *flags, hipStream_t st,
+                     const ListStep *ls = nullptr);
 void launch_dipR(const State &s, bool mixed, bool all_pairs, const void *tab, double *task_out, hipStream_t st);
 void launch_ext_field(const State &s, double *task_out, hipStream_t st);
 void launch_self_ener(const State &s, double *task_out, hipStream_t st);
@@ -103,13 +118,16 @@ void launch_stamp(double *p, double v, hipStream_t st);
 constexpr int kRowKick1 = 0, kRowDrift = 1, kRowSC = 2, kRowR = 3, kRowExt = 4, kRowKick2 = 5, kRowSets = 6;
 // step: device {k, first absolute step of the run}; gs and obs are the run's
 // deviate and observable buffers, indexed by k in the kernels
-void launch_kick1(const State &s, const Thermo *th, const double *gs, const int *step, double *rows, hipStream_t st);
-void launch_drift(const State &s, bool mixed, void *ph, void *amp, double *rows, hipStream_t st);
+void launch_drift(const State &s, bool mixed, const Thermo *th, const double *gs, const int *step, double *rows1,
+                  void *ph, void *amp, double *rows, const ListStep &ls, hipStream_t st);
 void launch_kick2(const State &s, const Thermo *th, const double *gs, const int *step, const double *rows1,
-                  double *rows2, hipStream_t st);
+                  double *rows2, const ListStep &ls, hipStream_t st);
 void launch_end(const State &s, Thermo *th, const double *gs, const int *step, const double *rows, long region,
                 bool has_field, const double *kblk, int maxslot, RepEnergy *E, double *Kin, double *counts,
-                double *obs, int *rebuild, int *srebuild, hipStream_t st);
+                double *obs, const ListStep &ls, hipStream_t st);
+// the step's last launch: rows of rebuilt replicas, contact rows of
+// refreshed ones; advances step[0]
+void launch_lists_step(const State &s, const ListStep &ls, int *flags, int *step, hipStream_t st);
 // contact-graph components of the selected replica: par[g] = smallest
 // particle of g's component (global index); flags[0] counts particles that
 // moved too far for the reference rows to hold every contact, flags[1] pairs
