@@ -43,8 +43,19 @@ if ROOT not in sys.path:
 #   WCA pair (d < 2^{1/6} sigma_ij): segment distance and force/torque    120
 #   dipolar real-space pair within r_c: table lerp, force, torques        70
 FLOPS = {"recip_struct": 18, "recip_force": 21, "wca": 120, "real_pairs": 70}
-PEAK_TFLOPS = {"fp32": 157.3, "fp64": 78.6}   # MI355X vector peaks
+# What bounds each kernel of the mixed step: the reciprocal kernels run their
+# products on the matrix cores (v_mfma_f32_16x16x16_f16 on operands split into
+# two fp16 pieces, three products per FP32-accurate product); the pair
+# kernels are gathers.  The reciprocal kernels' algorithmic flops are FP32
+# products, priced against MI355X's dense FP32 matrix peak (157.3 TFLOP/s,
+# equal to its FP32 vector peak).
+BOUND = {"recip_struct": "mfma", "recip_force": "mfma", "wca": "hbm", "real_pairs": "hbm"}
+PEAK_TFLOPS = {"fp32": 157.3, "fp64": 78.6}
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of a kernel from two rocprofv3 --pmc passes
+# (FETCH_SIZE, WRITE_SIZE) scaled by a copy kernel of known size, as
+# recorded in the profile summaries
+TRAFFIC_FILES = {"recip_force": "profiles/recip_force_mfma_n32768_counters.md"}
 
 
 def log(msg):
@@ -76,6 +87,92 @@ def timed_run(sysm, steps, warmup, timing, step0=0):
     dt = time.perf_counter() - t0
     sysm.set_timing(False)
     return dt, obs
+
+
+def traffic_from_profile(kernel):
+    """HBM bytes per launch of `kernel` recorded by a counter summary (the
+    line 'traffic_bytes_per_launch: B'), or None."""
+    rel = TRAFFIC_FILES.get(kernel)
+    if not rel or not os.path.exists(os.path.join(ROOT, rel)):
+        return None, None
+    for line in open(os.path.join(ROOT, rel)):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return int(float(line.split(":", 1)[1])), rel
+    return None, None
+
+
+def device_copy(nbytes):
+    """One device-to-device copy of nbytes (the runtime's copy kernel reads
+    and writes nbytes each), through the HIP runtime the library uses."""
+    path = "/opt/rocm/lib/libamdhip64.so"
+    hip = ctypes.CDLL(path if os.path.exists(path) else "libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    for p in (a, b):
+        if hip.hipMalloc(ctypes.byref(p), ctypes.c_size_t(nbytes)):
+            raise RuntimeError("hipMalloc failed")
+    try:
+        if hip.hipMemset(a, 1, ctypes.c_size_t(nbytes)) or \
+                hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3) or hip.hipDeviceSynchronize():
+            raise RuntimeError("device copy failed")
+    finally:
+        hip.hipFree(a)
+        hip.hipFree(b)
+
+
+def host_info():
+    """The host the CPU baseline ran on: CPU model (/proc/cpuinfo), the CPUs
+    this process may use, and the machine's CPU count."""
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return {"cpu": model, "usable_cpus": usable, "cpu_count": os.cpu_count(),
+            "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS")}
+
+
+def cpu_step_rate(L, reps, budget_s):
+    """Steps per second of the oracle's whole reference step (o_run_steps:
+    moveTher, moveP, moveX, force_sc + force_dipR + force_dipK, the list check
+    and its rebuilds; v25/main.cpp:559-701) over every replica of a workload,
+    each replica a run of its own, sampled for about budget_s after one
+    untimed step (the first step alone when it takes half the budget)."""
+    from oracle import oracle as O
+    runs = []
+    for d, st in reps:
+        S = O.osys_from_replica(d, st)
+        P = O.run_params_from_replica(d)
+        R = O.CRun()
+        if L.o_run_init(S.c(), ctypes.byref(P), ctypes.byref(R), 0):
+            raise RuntimeError("oracle run init failed")
+        runs.append((S, P, R))
+
+    rebuilds = [0]
+
+    def steps(k0, n):
+        obs = np.zeros((n, O.OBS_N))
+        t0 = time.perf_counter()
+        for S, P, R in runs:
+            if L.o_run_steps(S.c(), ctypes.byref(P), ctypes.byref(R), k0, n, O.dptr(obs)):
+                raise RuntimeError("oracle step failed")
+            rebuilds[0] += int(obs[:, O.OBS_NAMES.index("rebuild")].sum())
+        return time.perf_counter() - t0
+
+    t1 = steps(0, 1)
+    if t1 >= 0.5 * budget_s:
+        n, t = 1, t1  # one step already fills the sample
+    else:
+        rebuilds[0] = 0
+        n = max(1, min(200, int(budget_s / max(t1, 1e-6))))
+        t = steps(1, n)
+    for _, _, R in runs:
+        L.o_run_free(ctypes.byref(R))
+    return {"steps_per_s": round(n / t, 5), "steps": n, "seconds": round(t, 2), "list_rebuilds": rebuilds[0],
+            "replicas": len(reps), "N": sum(d.N for d, _ in reps)}
 
 
 def mt19937_state(seed):
@@ -163,7 +260,8 @@ def main():
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first copy this many MiB device to device (a kernel of known traffic, to scale "
                          "hardware byte counters)")
-    ap.add_argument("--cpu-budget", type=float, default=30.0)
+    ap.add_argument("--cpu-budget", type=float, default=30.0,
+                    help="seconds of timed oracle steps over the four workloads")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (replicas would shard across GPUs, "
@@ -176,13 +274,8 @@ def main():
 
     m.load()
     if args.copy_mib > 0:
-        import torch
-        src = torch.ones(args.copy_mib << 20, dtype=torch.uint8, device="cuda")
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
+        device_copy(args.copy_mib << 20)
         log("bench: copied %d MiB device to device" % args.copy_mib)
-        del src, dst
     t0 = time.perf_counter()
     desc, st = configs.build("c3")[0]
     log("bench: workload N=%d built in %.1f s (box %.3f, alpha %.4f, r_c %.2f, K %d)"
@@ -231,6 +324,11 @@ def main():
     log("bench: mixed %.2f steps/s (%.3f ms/step, %d list rebuilds)" % (sps_mix, 1e3 * t_mix / args.steps, rebuilds))
     t_kern, _ = timed_run(mix, args.steps, 0, True, step0=args.warmup + args.steps)
     ktab = kernel_table(mix, names)
+    # list rebuilds at their natural rate: a long window of the same run
+    n_am = max(1000, args.steps)
+    t_am, obs_am = timed_run(mix, n_am, 0, False, step0=args.warmup + 2 * args.steps)
+    reb_am = int(np.sum(obs_am[:, 0, m.OBS_NAMES.index("rebuild")]))
+    log("bench: mixed over %d steps: %.2f steps/s (%d list rebuilds)" % (n_am, n_am / t_am, reb_am))
     log("bench: per-kernel pass (event brackets on): %.3f ms/step" % (1e3 * t_kern / args.steps))
     for n, (ms, cnt) in sorted(ktab.items(), key=lambda kv: -kv[1][0]):
         log("  %-13s %8.3f ms/launch  (%d launches, %.1f%% of the headline step time)"
@@ -289,11 +387,18 @@ def main():
     log("bench: order parameters %.4f ms per call (device %.4f ms), G1 %.4f G2 %.4f"
         % (1e3 * t_oo, order["device_ms"], G1, G2))
 
-    k64 = args.fp64_steps or max(args.steps // 2, 10)
+    k64 = args.fp64_steps or args.steps
     log("bench: FP64, %d warmup + %d timed steps" % (min(args.warmup, 10), k64))
     t_d, _ = timed_run(f64, k64, min(args.warmup, 10), False)
     sps_d = k64 / t_d
     log("bench: FP64 %.2f steps/s; mixed/FP64 = %.2f" % (sps_d, sps_mix / sps_d))
+    t_dam, obs_dam = timed_run(f64, n_am, 0, False, step0=min(args.warmup, 10) + k64)
+    reb_dam = int(np.sum(obs_dam[:, 0, m.OBS_NAMES.index("rebuild")]))
+    log("bench: FP64 over %d steps: %.2f steps/s (%d list rebuilds); mixed/FP64 = %.2f"
+        % (n_am, n_am / t_dam, reb_dam, t_dam / t_am))
+    amortized = {"steps": n_am, "mixed_steps_per_s": round(n_am / t_am, 3), "mixed_list_rebuilds": reb_am,
+                 "fp64_steps_per_s": round(n_am / t_dam, 3), "fp64_list_rebuilds": reb_dam,
+                 "mixed_over_fp64": round(t_dam / t_am, 3)}
     # FP64 with the WCA over the contact rows, as the mixed path walks them
     f64.set_contact_rows(True)
     t_dc, _ = timed_run(f64, k64, min(args.warmup, 10), False)
@@ -310,17 +415,18 @@ def main():
     ms, cnt = ktab[dom]
     flop = FLOPS[dom] * units[dom]
     achieved = flop / (ms / cnt * 1e-3) / 1e12 if ms > 0 else 0.0
-    # the kernel runs on the vector ALUs; MI355X's published FP32 figure is the
-    # same 157.3 TFLOP/s for vector and matrix issue
-    roofline = {"kernel": dom, "bound": "mfma", "achieved": round(achieved, 3),
+    traffic, traffic_src = traffic_from_profile(dom)
+    roofline = {"kernel": dom, "bound": BOUND[dom], "achieved": round(achieved, 3),
                 "peak": PEAK_TFLOPS["fp32"], "unit": "TFLOP/s", "frac": round(achieved / PEAK_TFLOPS["fp32"], 4),
-                "traffic": None, "flop_per_launch": flop, "ms_per_launch": round(ms / cnt, 4)}
+                "traffic": traffic, "traffic_source": traffic_src, "flop_per_launch": flop,
+                "ms_per_launch": round(ms / cnt, 4)}
 
     # --- batched replicas ---------------------------------------------------------
     batched = None
+    reps_c4 = None
     if not args.no_batched:
         t0 = time.perf_counter()
-        reps = configs.build("c4")
+        reps = reps_c4 = configs.build("c4")
         ntot = sum(d.N for d, _ in reps)
         log("bench: batched workload, %d replicas, sum N = %d, built in %.1f s"
             % (len(reps), ntot, time.perf_counter() - t0))
@@ -344,12 +450,28 @@ def main():
     cpu = None
     err_oracle = None
     if not args.no_cpu:
+        from oracle import oracle as O
+        Lomp = O.lib_omp()
+        threads = int(Lomp.o_threads())
+        host = host_info()
+        log("bench: host %s, %d usable CPUs, oracle timing build on %d threads"
+            % (host["cpu"], host["usable_cpus"], threads))
+        per = {}
+        for name in ("c1", "c2", "c3", "c4"):
+            reps = [(desc, st)] if name == "c3" else (reps_c4 if name == "c4" and reps_c4 else configs.build(name))
+            per[name] = cpu_step_rate(Lomp, reps, args.cpu_budget / 4)
+            log("bench: oracle full step %s (%d replicas, sum N %d): %.4g steps/s over %d steps on %d threads"
+                % (name, per[name]["replicas"], per[name]["N"], per[name]["steps_per_s"], per[name]["steps"], threads))
         log("bench: oracle force evaluation on the host (serial FP64)")
         t_cpu, Fo, To, Uo = cpu_baseline(desc, st, off, nbr, args.cpu_budget)
-        cpu = {"value": round(1.0 / t_cpu, 5), "unit": "steps/s", "cores": 1, "kind": "port",
-               "sample": "one force evaluation (WCA on the neighbour list, real space within r_c=3.5, reciprocal "
-                         "k_max=10) of the N=32768 workload, serial C oracle, %.1f s; the integrator "
-                         "(about 1%% of a reference step) is not timed" % t_cpu}
+        cpu = {"value": per["c3"]["steps_per_s"], "unit": "steps/s", "cores": threads, "kind": "port",
+               "sample": "%d whole reference step(s) (integrator, WCA over the reference list, real space "
+                         "within r_c=3.5, reciprocal k_max=10, list check; %d list rebuilds) of the N=32768 "
+                         "workload, the oracle's OpenMP timing build on %d threads, %.1f s"
+                         % (per["c3"]["steps"], per["c3"]["list_rebuilds"], threads, per["c3"]["seconds"]),
+               "host": host, "workloads": per,
+               "serial_force_evals_per_s": round(1.0 / t_cpu, 5),
+               "serial_sample": "one force evaluation of the N=32768 workload by the serial oracle, %.1f s" % t_cpu}
         err_oracle = {"mixed": accuracy.force_errors(Fm, Tm, Um, Fo, To, Uo),
                       "fp64": accuracy.force_errors(Fd, Td, Ud, Fo, To, Uo)}
         for k in ("mixed", "fp64"):
@@ -381,6 +503,7 @@ def main():
         "mixed_over_fp64": round(sps_mix / sps_d, 3),
         "fp64_contact_rows_steps_per_s": round(sps_dc, 3),
         "mixed_over_fp64_contact_rows": round(sps_mix / sps_dc, 3),
+        "amortized": amortized,
         "err_mixed_vs_fp64": err_mix_vs_fp64,
         "fp64_force_norms": fp64_norms,
         "err_vs_oracle": err_oracle,

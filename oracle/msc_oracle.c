@@ -9,8 +9,18 @@
  * reference tree).  Expressions keep the reference's operand order and
  * C left-to-right association so that the results agree to the last bit with a
  * serial (non-OpenMP) reference build; compile with -ffp-contract=off.
+ *
+ * Built a second time with OpenMP (libmsc_oracle_omp.so) only to time the
+ * host baseline on every core: the neighbour list, force_sc, force_dipR and
+ * force_dipK loops then run on all threads, each with its own force arrays,
+ * so that build's sums are taken in another order and it is never the
+ * checker.
  */
 #include "msc_oracle.h"
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #include <math.h>
 #include <stdio.h>
@@ -290,6 +300,39 @@ double o_spherocyl_dist(const osys *S, int A, int B, const double rAB[3],
 long o_neigh_list(const osys *S, double r_list, int *off, int *nbr, long cap) {
   double rl_sq = r_list * r_list;
   long cnt = 0;
+#ifdef _OPENMP
+  /* timing build: rows found in parallel, then concatenated in row order
+   * (the same list as the serial loop) */
+  int **rows = (int **)calloc((size_t)S->N, sizeof(int *));
+  int *rn = (int *)calloc((size_t)S->N, sizeof(int));
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int f = 0; f < S->N - 1; ++f) {
+    int capr = 64, n = 0;
+    int *buf = (int *)malloc(sizeof(int) * (size_t)capr);
+    for (int g = f + 1; g < S->N; ++g) {
+      double rij[3], st[2], dij[3];
+      o_minimg(S->r + 3 * f, S->r + 3 * g, S->L, rij);
+      if (o_spherocyl_dist(S, f, g, rij, dij, st) < rl_sq) {
+        if (n == capr) buf = (int *)realloc(buf, sizeof(int) * (size_t)(capr *= 2));
+        buf[n++] = g;
+      }
+    }
+    rows[f] = buf;
+    rn[f] = n;
+  }
+  for (int f = 0; f < S->N; ++f) {
+    if (off) off[f] = (int)(cnt < 0x7fffffff ? cnt : 0x7fffffff);
+    for (int q = 0; q < rn[f]; ++q) {
+      if (nbr && cnt < cap) nbr[cnt] = rows[f][q];
+      ++cnt;
+    }
+    free(rows[f]);
+  }
+  free(rows);
+  free(rn);
+  if (off) off[S->N] = (int)(cnt < 0x7fffffff ? cnt : 0x7fffffff);
+  return cnt;
+#endif
   for (int f = 0; f < S->N; ++f) {
     if (off) off[f] = (int)(cnt < 0x7fffffff ? cnt : 0x7fffffff);
     if (f == S->N - 1) continue;
@@ -338,6 +381,68 @@ int o_check_list(const osys *S, const double *snap, double r_list_min,
 /* force_sc, v25/functions.cpp:1397-1553                                     */
 /* returns 0, or 1 on overlap (d^2 < 0.49 sigma^2), 2 on d^2 > 1.00001 r^2.   */
 /* ------------------------------------------------------------------------ */
+/* Row id of the list (the pairs (id, nbr[q])): forces and torques into F
+ * and T, energy and virial into *Uc, *Wc, the pressure tensor into P; the
+ * overlap tests set *abort_code as the reference's loop does. */
+static void sc_row(const osys *S, int id, const int *off, const int *nbr, double r_cut_sq, double srtsq,
+                   double *F, double *T, double *Uc, double *Wc, double P[9], int *abort_code) {
+  const double *ri = S->r + 3 * id, *oi = S->ori + 9 * id + 6;
+  for (int q = off[id]; q < off[id + 1]; ++q) {
+    int jd = nbr[q];
+    const double *rj = S->r + 3 * jd, *oj = S->ori + 9 * jd + 6;
+    double sigma = (S->dia[id] + S->dia[jd]) / 2;
+    double sig_sq = sigma * sigma;
+    double rsh = srtsq * sig_sq;
+    double rij[3], rsq = 0;
+    for (int a = 0; a < 3; ++a) {
+      rij[a] = ri[a] - rj[a];
+      rij[a] -= S->L[a] * floor(rij[a] / S->L[a] + 0.5);
+      rsq += rij[a] * rij[a];
+    }
+    double st[2], dij[3];
+    double dsq = o_spherocyl_dist(S, id, jd, rij, dij, st);
+    if (dsq < 0.49 * sig_sq) *abort_code = 1;
+    if (dsq > rsq * 1.00001 && !*abort_code) *abort_code = 2;
+    if (dsq < r_cut_sq) {
+      double SR2s = sig_sq / dsq;
+      double SR6s = SR2s * SR2s * SR2s;
+      double pot = 4 * SR6s * (SR6s - 1.);
+      double UIJ = 0., WIJ = 0.;
+      double ddr = dot3(rij, dij);
+      if (dsq < rsh) {
+        UIJ = pot + 1.;
+        WIJ = 6 * (pot + 4 * SR6s * SR6s) * ddr / dsq;
+      }
+      *Uc += UIJ;
+      *Wc += WIJ;
+      double FIJ = WIJ / ddr, f[3], t[3];
+      for (int a = 0; a < 3; ++a) f[a] = FIJ * dij[a];
+      for (int a = 0; a < 3; ++a) F[3 * id + a] += f[a];
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) P[3 * a + b] += rij[a] * f[b];
+      for (int a = 0; a < 3; ++a) f[a] *= -1;
+      for (int a = 0; a < 3; ++a) F[3 * jd + a] += f[a];
+      double icroj[3], icror[3], jcror[3];
+      cross3(oi, oj, icroj);
+      cross3(oi, rij, icror);
+      cross3(oj, rij, jcror);
+      double fact1 = (st[0] - 0.5) * S->len[id];
+      double fact2 = -(st[1] - 0.5) * S->len[jd];
+      if (st[0] == -1.e+6) fact1 = fact2 = 0.;
+      /* sentinel overrides; the reference tests -3e10 twice, so the (-/+)
+       * code -4e10 keeps the factors computed above (v25/functions.cpp:1526-1532) */
+      if (st[0] == -1.e+10) { fact1 = 0.5 * S->len[id]; fact2 = 0.5 * S->len[jd]; }
+      else if (st[0] == -2.e+10) { fact1 = -0.5 * S->len[id]; fact2 = -0.5 * S->len[jd]; }
+      else if (st[0] == -3.e+10) { fact1 = 0.5 * S->len[id]; fact2 = -0.5 * S->len[jd]; }
+      double fmul = fact1 * fact2;
+      for (int a = 0; a < 3; ++a) t[a] = FIJ * (fact1 * icror[a] + fmul * icroj[a]);
+      for (int a = 0; a < 3; ++a) T[3 * id + a] += t[a];
+      for (int a = 0; a < 3; ++a) t[a] = FIJ * (fact2 * jcror[a] - fmul * icroj[a]);
+      for (int a = 0; a < 3; ++a) T[3 * jd + a] += t[a];
+    }
+  }
+}
+
 int o_force_sc(osys *S, double r_cut, const int *off, const int *nbr,
                double *U, double *W, double PT[9]) {
   double r_cut_sq = r_cut * r_cut;
@@ -345,63 +450,28 @@ int o_force_sc(osys *S, double r_cut, const int *off, const int *nbr,
   double srt = pow(2., 1. / 6);
   double srtsq = srt * srt;
   int abort_code = 0;
-  for (int id = 0; id < S->N && !abort_code; ++id) {
-    const double *ri = S->r + 3 * id, *oi = S->ori + 9 * id + 6;
-    for (int q = off[id]; q < off[id + 1]; ++q) {
-      int jd = nbr[q];
-      const double *rj = S->r + 3 * jd, *oj = S->ori + 9 * jd + 6;
-      double sigma = (S->dia[id] + S->dia[jd]) / 2;
-      double sig_sq = sigma * sigma;
-      double rsh = srtsq * sig_sq;
-      double rij[3], rsq = 0;
-      for (int a = 0; a < 3; ++a) {
-        rij[a] = ri[a] - rj[a];
-        rij[a] -= S->L[a] * floor(rij[a] / S->L[a] + 0.5);
-        rsq += rij[a] * rij[a];
-      }
-      double st[2], dij[3];
-      double dsq = o_spherocyl_dist(S, id, jd, rij, dij, st);
-      if (dsq < 0.49 * sig_sq) abort_code = 1;
-      if (dsq > rsq * 1.00001 && !abort_code) abort_code = 2;
-      if (dsq < r_cut_sq) {
-        double SR2s = sig_sq / dsq;
-        double SR6s = SR2s * SR2s * SR2s;
-        double pot = 4 * SR6s * (SR6s - 1.);
-        double UIJ = 0., WIJ = 0.;
-        double ddr = dot3(rij, dij);
-        if (dsq < rsh) {
-          UIJ = pot + 1.;
-          WIJ = 6 * (pot + 4 * SR6s * SR6s) * ddr / dsq;
-        }
-        Uc += UIJ;
-        Wc += WIJ;
-        double FIJ = WIJ / ddr, f[3], t[3];
-        for (int a = 0; a < 3; ++a) f[a] = FIJ * dij[a];
-        for (int a = 0; a < 3; ++a) S->F[3 * id + a] += f[a];
-        for (int a = 0; a < 3; ++a)
-          for (int b = 0; b < 3; ++b) P[3 * a + b] += rij[a] * f[b];
-        for (int a = 0; a < 3; ++a) f[a] *= -1;
-        for (int a = 0; a < 3; ++a) S->F[3 * jd + a] += f[a];
-        double icroj[3], icror[3], jcror[3];
-        cross3(oi, oj, icroj);
-        cross3(oi, rij, icror);
-        cross3(oj, rij, jcror);
-        double fact1 = (st[0] - 0.5) * S->len[id];
-        double fact2 = -(st[1] - 0.5) * S->len[jd];
-        if (st[0] == -1.e+6) fact1 = fact2 = 0.;
-        /* sentinel overrides; the reference tests -3e10 twice, so the (-/+)
-         * code -4e10 keeps the factors computed above (v25/functions.cpp:1526-1532) */
-        if (st[0] == -1.e+10) { fact1 = 0.5 * S->len[id]; fact2 = 0.5 * S->len[jd]; }
-        else if (st[0] == -2.e+10) { fact1 = -0.5 * S->len[id]; fact2 = -0.5 * S->len[jd]; }
-        else if (st[0] == -3.e+10) { fact1 = 0.5 * S->len[id]; fact2 = -0.5 * S->len[jd]; }
-        double fmul = fact1 * fact2;
-        for (int a = 0; a < 3; ++a) t[a] = FIJ * (fact1 * icror[a] + fmul * icroj[a]);
-        for (int a = 0; a < 3; ++a) S->T[3 * id + a] += t[a];
-        for (int a = 0; a < 3; ++a) t[a] = FIJ * (fact2 * jcror[a] - fmul * icroj[a]);
-        for (int a = 0; a < 3; ++a) S->T[3 * jd + a] += t[a];
-      }
+#ifdef _OPENMP
+  /* timing build: rows split over threads, each with its own force arrays */
+#pragma omp parallel
+  {
+    double *Ft = (double *)calloc(6 * (size_t)S->N, sizeof(double)), *Tt = Ft + 3 * (size_t)S->N;
+    double Ut = 0., Wt = 0., Pt[9] = {0};
+    int at = 0;
+#pragma omp for schedule(dynamic, 64)
+    for (int id = 0; id < S->N; ++id) sc_row(S, id, off, nbr, r_cut_sq, srtsq, Ft, Tt, &Ut, &Wt, Pt, &at);
+#pragma omp critical
+    {
+      for (size_t q = 0; q < 3 * (size_t)S->N; ++q) { S->F[q] += Ft[q]; S->T[q] += Tt[q]; }
+      Uc += Ut; Wc += Wt;
+      for (int k = 0; k < 9; ++k) P[k] += Pt[k];
+      if (at && !abort_code) abort_code = at;
     }
+    free(Ft);
   }
+#else
+  for (int id = 0; id < S->N && !abort_code; ++id)
+    sc_row(S, id, off, nbr, r_cut_sq, srtsq, S->F, S->T, &Uc, &Wc, P, &abort_code);
+#endif
   if (abort_code) return abort_code;
   *U = Uc;
   *W = Wc;
@@ -435,88 +505,111 @@ void o_tabul(double *Tab, double alpha) {
 /* rc <= 0: every minimum-image pair of dipoles on distinct particles, as the
  * reference; rc > 0: only dipole pairs closer than rc (no reference
  * counterpart; used for the cut-off configurations). */
-void o_force_dipR(osys *S, const double *Tab, double rc, double *U, double PT[9]) {
-  double Uc = 0., P[9] = {0};
-  double Lh = 0.5 * (200 * 1.001) / 32768;
+/* Row k of the pair loop (the pairs (k, l > k)), accumulating into F, T,
+ * *Uc and P. */
+static void dipR_row(const osys *S, const double *Tab, double rc, double Lh, int k, double *F, double *T,
+                     double *Uc, double P[9]) {
   int nd = S->nd;
-  for (int k = 0; k < S->N - 1; ++k) {
-    for (int l = k + 1; l < S->N; ++l) {
-      for (int v = 0; v < nd; ++v) {
-        int dv = k * nd + v;
-        double di[3];
-        for (int a = 0; a < 3; ++a) di[a] = S->dpc[dv] * S->ori[9 * k + 6 + a];
-        const double *ri = S->dr + 3 * dv, *oi = S->du + 3 * dv;
-        double mui = S->dmu[dv];
-        if (mui == 0.) continue;
-        for (int w = 0; w < nd; ++w) {
-          int dw = l * nd + w;
-          double dj[3];
-          for (int a = 0; a < 3; ++a) dj[a] = S->dpc[dw] * S->ori[9 * l + 6 + a];
-          const double *rj = S->dr + 3 * dw, *oj = S->du + 3 * dw;
-          double muj = S->dmu[dw];
-          if (muj == 0.) continue;
-          double rij[3];
-          for (int a = 0; a < 3; ++a) {
-            rij[a] = ri[a] - rj[a];
-            rij[a] -= S->L[a] * floor(rij[a] / S->L[a] + 0.5);
-          }
-          double RIJSQ = dot3(rij, rij);
-          double uu = mui * muj;
-          double SR2 = 1. / RIJSQ;
-          double RIJ = sqrt(RIJSQ);
-          if (rc > 0 && !(RIJ < rc)) continue;
-          int ix = (int)floor(RIJ / Lh);
-          /* the reference reads Tab[-1] below Lh and past the end above
-           * 100.1; both are undefined there, clamped here */
-          if (ix < 1) ix = 1;
-          if (ix > 32767) ix = 32767;
-          double R_L0 = RIJ - Lh * ix;
-          double R_L1 = RIJ - Lh * (ix + 1);
-          const double *T1 = Tab + 3 * ix, *T0 = Tab + 3 * (ix - 1);
-          double A = (T1[0] * R_L0 - T0[0] * R_L1) / Lh;
-          double B = (T1[1] * R_L0 - T0[1] * R_L1) / Lh;
-          double C = (T1[2] * R_L0 - T0[2] * R_L1) / Lh;
-          double UIR = dot3(oi, rij), UJR = dot3(oj, rij);
-          double UIJR = UIR * UJR;
-          double UIUJ = dot3(oi, oj);
-          double UIJ = uu * (UIUJ * A - UIJR * B);
-          Uc += UIJ;
-          double fact = (UIUJ - 5. * UIJR * SR2) * B - UIJR * SR2 * C;
-          double f[3], t[3];
-          for (int a = 0; a < 3; ++a) f[a] = uu * (fact * rij[a] + (UIR * oj[a] + UJR * oi[a]) * B);
-          for (int a = 0; a < 3; ++a) S->F[3 * k + a] += f[a];
-          for (int a = 0; a < 3; ++a)
-            for (int b = 0; b < 3; ++b) P[3 * a + b] += rij[a] * f[b];
-          for (int a = 0; a < 3; ++a) f[a] *= -1;
-          for (int a = 0; a < 3; ++a) S->F[3 * l + a] += f[a];
-          double icror[3], dicror[3], dicroi[3], jcror[3], djcror[3], djcroj[3];
-          double dicroj[3], djcroi[3], icroj[3];
-          cross3(oi, rij, icror);
-          cross3(di, rij, dicror);
-          cross3(di, oi, dicroi);
-          cross3(oj, rij, jcror);
-          cross3(dj, rij, djcror);
-          cross3(dj, oj, djcroj);
-          cross3(di, oj, dicroj);
-          cross3(dj, oi, djcroi);
-          cross3(oi, oj, icroj);
-          for (int a = 0; a < 3; ++a) icroj[a] *= A;
-          fact = -UIR * UJR * (5 * B + C) * SR2 + UIUJ * B;
-          double fact1;
-          for (int a = 0; a < 3; ++a) {
-            fact1 = (UJR * (icror[a] + dicroi[a]) + UIR * dicroj[a]) * B;
-            t[a] = uu * (fact * dicror[a] + fact1 - icroj[a]);
-          }
-          for (int a = 0; a < 3; ++a) S->T[3 * k + a] += t[a];
-          for (int a = 0; a < 3; ++a) {
-            fact1 = (UIR * (jcror[a] - djcroj[a]) - UJR * djcroi[a]) * B;
-            t[a] = uu * (-fact * djcror[a] + fact1 + icroj[a]);
-          }
-          for (int a = 0; a < 3; ++a) S->T[3 * l + a] += t[a];
+  for (int l = k + 1; l < S->N; ++l) {
+    for (int v = 0; v < nd; ++v) {
+      int dv = k * nd + v;
+      double di[3];
+      for (int a = 0; a < 3; ++a) di[a] = S->dpc[dv] * S->ori[9 * k + 6 + a];
+      const double *ri = S->dr + 3 * dv, *oi = S->du + 3 * dv;
+      double mui = S->dmu[dv];
+      if (mui == 0.) continue;
+      for (int w = 0; w < nd; ++w) {
+        int dw = l * nd + w;
+        double dj[3];
+        for (int a = 0; a < 3; ++a) dj[a] = S->dpc[dw] * S->ori[9 * l + 6 + a];
+        const double *rj = S->dr + 3 * dw, *oj = S->du + 3 * dw;
+        double muj = S->dmu[dw];
+        if (muj == 0.) continue;
+        double rij[3];
+        for (int a = 0; a < 3; ++a) {
+          rij[a] = ri[a] - rj[a];
+          rij[a] -= S->L[a] * floor(rij[a] / S->L[a] + 0.5);
         }
+        double RIJSQ = dot3(rij, rij);
+        double uu = mui * muj;
+        double SR2 = 1. / RIJSQ;
+        double RIJ = sqrt(RIJSQ);
+        if (rc > 0 && !(RIJ < rc)) continue;
+        int ix = (int)floor(RIJ / Lh);
+        /* the reference reads Tab[-1] below Lh and past the end above
+         * 100.1; both are undefined there, clamped here */
+        if (ix < 1) ix = 1;
+        if (ix > 32767) ix = 32767;
+        double R_L0 = RIJ - Lh * ix;
+        double R_L1 = RIJ - Lh * (ix + 1);
+        const double *T1 = Tab + 3 * ix, *T0 = Tab + 3 * (ix - 1);
+        double A = (T1[0] * R_L0 - T0[0] * R_L1) / Lh;
+        double B = (T1[1] * R_L0 - T0[1] * R_L1) / Lh;
+        double C = (T1[2] * R_L0 - T0[2] * R_L1) / Lh;
+        double UIR = dot3(oi, rij), UJR = dot3(oj, rij);
+        double UIJR = UIR * UJR;
+        double UIUJ = dot3(oi, oj);
+        double UIJ = uu * (UIUJ * A - UIJR * B);
+        *Uc += UIJ;
+        double fact = (UIUJ - 5. * UIJR * SR2) * B - UIJR * SR2 * C;
+        double f[3], t[3];
+        for (int a = 0; a < 3; ++a) f[a] = uu * (fact * rij[a] + (UIR * oj[a] + UJR * oi[a]) * B);
+        for (int a = 0; a < 3; ++a) F[3 * k + a] += f[a];
+        for (int a = 0; a < 3; ++a)
+          for (int b = 0; b < 3; ++b) P[3 * a + b] += rij[a] * f[b];
+        for (int a = 0; a < 3; ++a) f[a] *= -1;
+        for (int a = 0; a < 3; ++a) F[3 * l + a] += f[a];
+        double icror[3], dicror[3], dicroi[3], jcror[3], djcror[3], djcroj[3];
+        double dicroj[3], djcroi[3], icroj[3];
+        cross3(oi, rij, icror);
+        cross3(di, rij, dicror);
+        cross3(di, oi, dicroi);
+        cross3(oj, rij, jcror);
+        cross3(dj, rij, djcror);
+        cross3(dj, oj, djcroj);
+        cross3(di, oj, dicroj);
+        cross3(dj, oi, djcroi);
+        cross3(oi, oj, icroj);
+        for (int a = 0; a < 3; ++a) icroj[a] *= A;
+        fact = -UIR * UJR * (5 * B + C) * SR2 + UIUJ * B;
+        double fact1;
+        for (int a = 0; a < 3; ++a) {
+          fact1 = (UJR * (icror[a] + dicroi[a]) + UIR * dicroj[a]) * B;
+          t[a] = uu * (fact * dicror[a] + fact1 - icroj[a]);
+        }
+        for (int a = 0; a < 3; ++a) T[3 * k + a] += t[a];
+        for (int a = 0; a < 3; ++a) {
+          fact1 = (UIR * (jcror[a] - djcroj[a]) - UJR * djcroi[a]) * B;
+          t[a] = uu * (-fact * djcror[a] + fact1 + icroj[a]);
+        }
+        for (int a = 0; a < 3; ++a) T[3 * l + a] += t[a];
       }
     }
   }
+}
+
+void o_force_dipR(osys *S, const double *Tab, double rc, double *U, double PT[9]) {
+  double Uc = 0., P[9] = {0};
+  double Lh = 0.5 * (200 * 1.001) / 32768;
+#ifdef _OPENMP
+  /* timing build: rows split over threads, each with its own force arrays */
+#pragma omp parallel
+  {
+    double *Ft = (double *)calloc(6 * (size_t)S->N, sizeof(double)), *Tt = Ft + 3 * (size_t)S->N;
+    double Ut = 0., Pt[9] = {0};
+#pragma omp for schedule(dynamic, 16)
+    for (int k = 0; k < S->N - 1; ++k) dipR_row(S, Tab, rc, Lh, k, Ft, Tt, &Ut, Pt);
+#pragma omp critical
+    {
+      for (size_t q = 0; q < 3 * (size_t)S->N; ++q) { S->F[q] += Ft[q]; S->T[q] += Tt[q]; }
+      Uc += Ut;
+      for (int q = 0; q < 9; ++q) P[q] += Pt[q];
+    }
+    free(Ft);
+  }
+#else
+  for (int k = 0; k < S->N - 1; ++k) dipR_row(S, Tab, rc, Lh, k, S->F, S->T, &Uc, P);
+#endif
   *U = Uc;
   for (int k = 0; k < 9; ++k) PT[k] = P[k];
 }
@@ -553,154 +646,182 @@ static inline cplx cconj(cplx a) { cplx r = {a.re, -a.im}; return r; }
 static inline cplx csub(cplx a, cplx b) { cplx r = {a.re - b.re, a.im - b.im}; return r; }
 static inline double cnorm(cplx a) { return a.re * a.re + a.im * a.im; }
 
-void o_force_dipK(osys *S, int K, double alpha, const double *Kvec, double *U,
-                  double PT[9]) {
+/* k-vector kn of force_dipK (its four sign variants): forces and torques
+ * into F and T, energy into *Uc, pressure tensor into PP; M is scratch of
+ * 4 N nd values. */
+static void dipK_kvec(const osys *S, int kn, int K, const double *Kvec, double a4, cplx *M, double *F,
+                      double *T, double *Uc, double PP[9]) {
   int NC = S->nspec, nd = S->nd, N = S->N;
   double V = S->L[0] * S->L[1] * S->L[2];
   double P2[3] = {2. * M_PI / S->L[0], 2. * M_PI / S->L[1], 2. * M_PI / S->L[2]};
-  int K2 = K * K, K3 = K2 * K, K1 = K + 1;
-  double a4 = 1. / (4. * alpha);
-  double Uc = 0., PP[9] = {0};
-  cplx *M = (cplx *)malloc(sizeof(cplx) * 4 * (size_t)N * nd);
+  int K2 = K * K, K1 = K + 1;
   const cplx cc = {0., 0.};
-  for (int kn = 0; kn < K3; ++kn) {
-    int nz = (int)floorf((float)kn / K2);
-    float lf = (float)(kn - K2 * nz);
-    int ny = (int)floorf(lf / K);
-    int nx = (int)lf - K * ny;
-    double Kz = P2[2] * nz, Ky = P2[1] * ny, Kx = P2[0] * nx;
-    double Ksq = Kx * Kx + Ky * Ky + Kz * Kz;
-    if (Ksq == 0) continue;
-    double KK = Kvec[(nx * K1 + ny) * K1 + nz];
-    double val = 2. * KK / V;
-    double pval = 2 * (a4 + 1. / Ksq);
-    double Kb[4][3] = {{Kx, Ky, Kz}, {-Kx, Ky, Kz}, {Kx, -Ky, Kz}, {-Kx, -Ky, Kz}};
-    cplx sum[4][OSYS_MAXSPEC], uve[4][3];
-    double self[4][OSYS_MAXSPEC], uvslf[4][3];
-    for (int b = 0; b < 4; ++b) {
-      for (int k = 0; k < 3; ++k) { uve[b][k] = cc; uvslf[b][k] = 0.; }
-      for (int n = 0; n < NC; ++n) { sum[b][n] = cc; self[b][n] = 0.; }
-    }
-    for (size_t q = 0; q < 4 * (size_t)N * nd; ++q) M[q] = cc;
-    /* generate values, v25/functions.cpp:1780-1827 */
-    int base = 0;
-    for (int n = 0; n < NC; ++n) {
-      for (int ii = 0; ii < S->counts[n]; ++ii) {
-        int i = base + ii;
-        cplx selfpart[4] = {cc, cc, cc, cc}, uvslfpart[4][3];
-        for (int b = 0; b < 4; ++b)
-          for (int k = 0; k < 3; ++k) uvslfpart[b][k] = cc;
-        for (int w = 0; w < nd; ++w) {
-          int dw = i * nd + w;
-          double ri[3], oi[3];
-          for (int k = 0; k < 3; ++k) {
-            ri[k] = S->dr[3 * dw + k];
-            ri[k] -= S->L[k] * floor(ri[k] / S->L[k] + 0.5);
-            oi[k] = S->du[3 * dw + k];
-          }
-          double mu = S->dmu[dw];
-          if (mu == 0.) continue;
-          double RIK[4], UIK[4];
-          cplx z[4];
-          for (int b = 0; b < 4; ++b) {
-            RIK[b] = dot3(ri, Kb[b]);
-            UIK[b] = dot3(oi, Kb[b]);
-            UIK[b] *= mu;
-            z[b].re = cos(RIK[b]);
-            z[b].im = sin(RIK[b]);
-          }
-          for (int b = 0; b < 4; ++b) {
-            cplx m = cconj(z[b]);
-            M[((size_t)b * N + i) * nd + w] = m;
-            cplx ukexp = cscale(m, UIK[b]);
-            sum[b][n] = cadd(sum[b][n], ukexp);
-            selfpart[b] = cadd(selfpart[b], ukexp);
-            for (int k = 0; k < 3; ++k) {
-              ukexp = cscale(m, mu * oi[k]);
-              uve[b][k] = cadd(uve[b][k], ukexp);
-              uvslfpart[b][k] = cadd(uvslfpart[b][k], ukexp);
-            }
-          }
+  int nz = (int)floorf((float)kn / K2);
+  float lf = (float)(kn - K2 * nz);
+  int ny = (int)floorf(lf / K);
+  int nx = (int)lf - K * ny;
+  double Kz = P2[2] * nz, Ky = P2[1] * ny, Kx = P2[0] * nx;
+  double Ksq = Kx * Kx + Ky * Ky + Kz * Kz;
+  if (Ksq == 0) return;
+  double KK = Kvec[(nx * K1 + ny) * K1 + nz];
+  double val = 2. * KK / V;
+  double pval = 2 * (a4 + 1. / Ksq);
+  double Kb[4][3] = {{Kx, Ky, Kz}, {-Kx, Ky, Kz}, {Kx, -Ky, Kz}, {-Kx, -Ky, Kz}};
+  cplx sum[4][OSYS_MAXSPEC], uve[4][3];
+  double self[4][OSYS_MAXSPEC], uvslf[4][3];
+  for (int b = 0; b < 4; ++b) {
+    for (int k = 0; k < 3; ++k) { uve[b][k] = cc; uvslf[b][k] = 0.; }
+    for (int n = 0; n < NC; ++n) { sum[b][n] = cc; self[b][n] = 0.; }
+  }
+  for (size_t q = 0; q < 4 * (size_t)N * nd; ++q) M[q] = cc;
+  /* generate values, v25/functions.cpp:1780-1827 */
+  int base = 0;
+  for (int n = 0; n < NC; ++n) {
+    for (int ii = 0; ii < S->counts[n]; ++ii) {
+      int i = base + ii;
+      cplx selfpart[4] = {cc, cc, cc, cc}, uvslfpart[4][3];
+      for (int b = 0; b < 4; ++b)
+        for (int k = 0; k < 3; ++k) uvslfpart[b][k] = cc;
+      for (int w = 0; w < nd; ++w) {
+        int dw = i * nd + w;
+        double ri[3], oi[3];
+        for (int k = 0; k < 3; ++k) {
+          ri[k] = S->dr[3 * dw + k];
+          ri[k] -= S->L[k] * floor(ri[k] / S->L[k] + 0.5);
+          oi[k] = S->du[3 * dw + k];
+        }
+        double mu = S->dmu[dw];
+        if (mu == 0.) continue;
+        double RIK[4], UIK[4];
+        cplx z[4];
+        for (int b = 0; b < 4; ++b) {
+          RIK[b] = dot3(ri, Kb[b]);
+          UIK[b] = dot3(oi, Kb[b]);
+          UIK[b] *= mu;
+          z[b].re = cos(RIK[b]);
+          z[b].im = sin(RIK[b]);
         }
         for (int b = 0; b < 4; ++b) {
-          self[b][n] += cnorm(selfpart[b]);
-          for (int k = 0; k < 3; ++k)
-            uvslf[b][k] += cmul(cconj(selfpart[b]), uvslfpart[b][k]).re;
-        }
-      }
-      base += S->counts[n];
-    }
-    cplx adds[4] = {cc, cc, cc, cc};
-    double addslf[4] = {0., 0., 0., 0.};
-    for (int b = 0; b < 4; ++b)
-      for (int n = 0; n < NC; ++n) { adds[b] = cadd(adds[b], sum[b][n]); addslf[b] += self[b][n]; }
-    double Upar = 0;
-    for (int b = 0; b < 4; ++b) Upar += val * 0.5 * (cnorm(adds[b]) - addslf[b]);
-    int quarter = (nx == 0 && ny == 0) || (nx == 0 && nz == 0) || (ny == 0 && nz == 0);
-    int half = !quarter && ((nz == 0 && nx != 0 && ny != 0) || (ny == 0 && nz != 0 && nx != 0) ||
-                            (nx == 0 && ny != 0 && nz != 0));
-    if (quarter) Upar /= 4.;
-    else if (half) Upar /= 2.;
-    Uc += Upar;
-    /* force and torque, v25/functions.cpp:1853-1925 */
-    for (int i = 0; i < N; ++i) {
-      cplx local[4];
-      double UIKw[4][8];
-      for (int b = 0; b < 4; ++b) {
-        local[b] = cc;
-        for (int w = 0; w < nd; ++w) {
-          int dw = i * nd + w;
-          double mu = S->dmu[dw];
-          if (mu != 0) {
-            UIKw[b][w] = dot3(S->du + 3 * dw, Kb[b]);
-            UIKw[b][w] *= mu;
-            local[b] = cadd(local[b], cscale(M[((size_t)b * N + i) * nd + w], UIKw[b][w]));
+          cplx m = cconj(z[b]);
+          M[((size_t)b * N + i) * nd + w] = m;
+          cplx ukexp = cscale(m, UIK[b]);
+          sum[b][n] = cadd(sum[b][n], ukexp);
+          selfpart[b] = cadd(selfpart[b], ukexp);
+          for (int k = 0; k < 3; ++k) {
+            ukexp = cscale(m, mu * oi[k]);
+            uve[b][k] = cadd(uve[b][k], ukexp);
+            uvslfpart[b][k] = cadd(uvslfpart[b][k], ukexp);
           }
         }
       }
+      for (int b = 0; b < 4; ++b) {
+        self[b][n] += cnorm(selfpart[b]);
+        for (int k = 0; k < 3; ++k)
+          uvslf[b][k] += cmul(cconj(selfpart[b]), uvslfpart[b][k]).re;
+      }
+    }
+    base += S->counts[n];
+  }
+  cplx adds[4] = {cc, cc, cc, cc};
+  double addslf[4] = {0., 0., 0., 0.};
+  for (int b = 0; b < 4; ++b)
+    for (int n = 0; n < NC; ++n) { adds[b] = cadd(adds[b], sum[b][n]); addslf[b] += self[b][n]; }
+  double Upar = 0;
+  for (int b = 0; b < 4; ++b) Upar += val * 0.5 * (cnorm(adds[b]) - addslf[b]);
+  int quarter = (nx == 0 && ny == 0) || (nx == 0 && nz == 0) || (ny == 0 && nz == 0);
+  int half = !quarter && ((nz == 0 && nx != 0 && ny != 0) || (ny == 0 && nz != 0 && nx != 0) ||
+                          (nx == 0 && ny != 0 && nz != 0));
+  if (quarter) Upar /= 4.;
+  else if (half) Upar /= 2.;
+  *Uc += Upar;
+  /* force and torque, v25/functions.cpp:1853-1925 */
+  for (int i = 0; i < N; ++i) {
+    cplx local[4];
+    double UIKw[4][8];
+    for (int b = 0; b < 4; ++b) {
+      local[b] = cc;
       for (int w = 0; w < nd; ++w) {
         int dw = i * nd + w;
         double mu = S->dmu[dw];
-        if (mu == 0) continue;
-        double di[3], oi[3];
-        for (int a = 0; a < 3; ++a) {
-          di[a] = S->dpc[dw] * S->ori[9 * i + 6 + a];
-          oi[a] = S->du[3 * dw + a];
+        if (mu != 0) {
+          UIKw[b][w] = dot3(S->du + 3 * dw, Kb[b]);
+          UIKw[b][w] *= mu;
+          local[b] = cadd(local[b], cscale(M[((size_t)b * N + i) * nd + w], UIKw[b][w]));
         }
-        double icrok[4][3], dicrok[4][3];
-        for (int b = 0; b < 4; ++b) {
-          cross3(di, Kb[b], dicrok[b]);
-          cross3(oi, Kb[b], icrok[b]);
-          for (int j = 0; j < 3; ++j) icrok[b][j] *= mu;
-        }
-        double f[3] = {0., 0., 0.}, t[3] = {0., 0., 0.};
-        for (int b = 0; b < 4; ++b) {
-          cplx m = M[((size_t)b * N + i) * nd + w];
-          cplx X = csub(cconj(adds[b]), cconj(local[b]));
-          double factF = -val * cmul(cscale(m, UIKw[b][w]), X).im;
-          for (int j = 0; j < 3; ++j) f[j] += factF * Kb[b][j];
-          double factT = -val * cmul(m, X).re;
-          for (int j = 0; j < 3; ++j) t[j] += factT * icrok[b][j] + factF * dicrok[b][j];
-        }
-        if (quarter) { for (int j = 0; j < 3; ++j) { f[j] /= 4.; t[j] /= 4.; } }
-        else if (half) { for (int j = 0; j < 3; ++j) { f[j] /= 2.; t[j] /= 2.; } }
-        for (int j = 0; j < 3; ++j) { S->F[3 * i + j] += f[j]; }
-        for (int j = 0; j < 3; ++j) { S->T[3 * i + j] += t[j]; }
       }
     }
-    /* pressure tensor, v25/functions.cpp:1926-1952 (a4 = 1/(4 alpha) as written) */
-    double Pk[9] = {0};
-    for (int b = 0; b < 4; ++b)
-      for (int j = 0; j < 3; ++j)
-        for (int k = 0; k < 3; ++k)
-          Pk[3 * j + k] += val * 0.5 *
-              (((double)(j == k) - pval * Kb[b][j] * Kb[b][k]) * (cnorm(adds[b]) - addslf[b]) +
-               2 * (cmul(cconj(adds[b]), uve[b][j]).re - uvslf[b][j]) * Kb[b][k]);
-    if (quarter) { for (int q = 0; q < 9; ++q) Pk[q] /= 4.; }
-    else if (half) { for (int q = 0; q < 9; ++q) Pk[q] /= 2.; }
-    for (int q = 0; q < 9; ++q) PP[q] += Pk[q];
+    for (int w = 0; w < nd; ++w) {
+      int dw = i * nd + w;
+      double mu = S->dmu[dw];
+      if (mu == 0) continue;
+      double di[3], oi[3];
+      for (int a = 0; a < 3; ++a) {
+        di[a] = S->dpc[dw] * S->ori[9 * i + 6 + a];
+        oi[a] = S->du[3 * dw + a];
+      }
+      double icrok[4][3], dicrok[4][3];
+      for (int b = 0; b < 4; ++b) {
+        cross3(di, Kb[b], dicrok[b]);
+        cross3(oi, Kb[b], icrok[b]);
+        for (int j = 0; j < 3; ++j) icrok[b][j] *= mu;
+      }
+      double f[3] = {0., 0., 0.}, t[3] = {0., 0., 0.};
+      for (int b = 0; b < 4; ++b) {
+        cplx m = M[((size_t)b * N + i) * nd + w];
+        cplx X = csub(cconj(adds[b]), cconj(local[b]));
+        double factF = -val * cmul(cscale(m, UIKw[b][w]), X).im;
+        for (int j = 0; j < 3; ++j) f[j] += factF * Kb[b][j];
+        double factT = -val * cmul(m, X).re;
+        for (int j = 0; j < 3; ++j) t[j] += factT * icrok[b][j] + factF * dicrok[b][j];
+      }
+      if (quarter) { for (int j = 0; j < 3; ++j) { f[j] /= 4.; t[j] /= 4.; } }
+      else if (half) { for (int j = 0; j < 3; ++j) { f[j] /= 2.; t[j] /= 2.; } }
+      for (int j = 0; j < 3; ++j) { F[3 * i + j] += f[j]; }
+      for (int j = 0; j < 3; ++j) { T[3 * i + j] += t[j]; }
+    }
   }
+  /* pressure tensor, v25/functions.cpp:1926-1952 (a4 = 1/(4 alpha) as written) */
+  double Pk[9] = {0};
+  for (int b = 0; b < 4; ++b)
+    for (int j = 0; j < 3; ++j)
+      for (int k = 0; k < 3; ++k)
+        Pk[3 * j + k] += val * 0.5 *
+            (((double)(j == k) - pval * Kb[b][j] * Kb[b][k]) * (cnorm(adds[b]) - addslf[b]) +
+             2 * (cmul(cconj(adds[b]), uve[b][j]).re - uvslf[b][j]) * Kb[b][k]);
+  if (quarter) { for (int q = 0; q < 9; ++q) Pk[q] /= 4.; }
+  else if (half) { for (int q = 0; q < 9; ++q) Pk[q] /= 2.; }
+  for (int q = 0; q < 9; ++q) PP[q] += Pk[q];
+}
+
+void o_force_dipK(osys *S, int K, double alpha, const double *Kvec, double *U,
+                  double PT[9]) {
+  int nd = S->nd, N = S->N;
+  int K3 = K * K * K;
+  double a4 = 1. / (4. * alpha);
+  double Uc = 0., PP[9] = {0};
+#ifdef _OPENMP
+  /* timing build: k-vectors split over threads, each with its own arrays */
+#pragma omp parallel
+  {
+    cplx *Mt = (cplx *)malloc(sizeof(cplx) * 4 * (size_t)N * nd);
+    double *Ft = (double *)calloc(6 * (size_t)N, sizeof(double)), *Tt = Ft + 3 * (size_t)N;
+    double Ut = 0., Pt[9] = {0};
+#pragma omp for schedule(dynamic)
+    for (int kn = 0; kn < K3; ++kn) dipK_kvec(S, kn, K, Kvec, a4, Mt, Ft, Tt, &Ut, Pt);
+#pragma omp critical
+    {
+      for (size_t q = 0; q < 3 * (size_t)N; ++q) { S->F[q] += Ft[q]; S->T[q] += Tt[q]; }
+      Uc += Ut;
+      for (int q = 0; q < 9; ++q) PP[q] += Pt[q];
+    }
+    free(Mt);
+    free(Ft);
+  }
+#else
+  cplx *M = (cplx *)malloc(sizeof(cplx) * 4 * (size_t)N * nd);
+  for (int kn = 0; kn < K3; ++kn) dipK_kvec(S, kn, K, Kvec, a4, M, S->F, S->T, &Uc, PP);
   free(M);
+#endif
   *U = Uc;
   for (int q = 0; q < 9; ++q) PT[q] = PP[q];
 }
@@ -1588,6 +1709,15 @@ int o_run_steps(osys *S, const orun_params *P, orun *R, int step0, int nsteps, d
     }
   }
   return 0;
+}
+
+/* threads the timing build's parallel loops use (1 in the serial build) */
+int o_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
 }
 
 void o_run_free(orun *R) {

@@ -155,6 +155,8 @@ int  o_run_init(osys *s, const orun_params *P, orun *R, int seed_init);
 int  o_run_steps(osys *s, const orun_params *P, orun *R, int step0, int nsteps,
                  double *obs);
 void o_run_free(orun *R);
+/* threads of the OpenMP timing build's loops (1 in the serial build) */
+int  o_threads(void);
 
 /* One force evaluation in the reference's step order (v25/main.cpp:582-614).
  * e[0..]: Uss Wss UdipR UdipK Uext, then PTss[9] PTdipR[9] PTdipK[9]. */

@@ -13,6 +13,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(HERE, "libmsc_oracle.so")
+LIBPATH_OMP = os.path.join(HERE, "libmsc_oracle_omp.so")
 MAXSPEC = 8
 OBS_NAMES = ["K", "U", "P", "Uss", "Pss", "UdipR", "PdipR", "UdipK", "PdipK",
              "Mom2", "Ang2", "rebuild"]
@@ -64,61 +65,82 @@ def build():
 
 
 def lib():
+    """The serial oracle: the checker."""
     global _lib
     if _lib is None:
         if not os.path.exists(LIBPATH):
             build()
-        L = ctypes.CDLL(LIBPATH)
-        P = ctypes.POINTER
-        L.o_segdist.restype = ctypes.c_double
-        L.o_segdist.argtypes = [_dp, _dp, ctypes.c_double, _dp, _dp, ctypes.c_double,
-                                _dp, _dp, _dp, _dp]
-        L.o_spherocyl_dist.restype = ctypes.c_double
-        L.o_neigh_list.restype = ctypes.c_long
-        L.o_neigh_list.argtypes = [P(COsys), ctypes.c_double, _ip, _ip, ctypes.c_long]
-        L.o_round12.restype = ctypes.c_double
-        L.o_round12.argtypes = [ctypes.c_double]
-        L.o_check_list.argtypes = [P(COsys), _dp, ctypes.c_double, ctypes.c_double]
-        L.o_force_sc.argtypes = [P(COsys), ctypes.c_double, _ip, _ip, _dp, _dp, _dp]
-        L.o_tabul.argtypes = [_dp, ctypes.c_double]
-        L.o_force_dipR.argtypes = [P(COsys), _dp, ctypes.c_double, _dp, _dp]
-        L.o_wave_v.argtypes = [_dp, _dp, ctypes.c_double, ctypes.c_int]
-        L.o_force_dipK.argtypes = [P(COsys), ctypes.c_int, ctypes.c_double, _dp, _dp, _dp]
-        L.o_ext_field.argtypes = [P(COsys), _dp, _dp]
-        L.o_self_ener.restype = ctypes.c_double
-        L.o_self_ener.argtypes = [P(COsys)]
-        L.o_set_inertia.argtypes = [P(COsys), ctypes.c_int]
-        L.o_set_dip.argtypes = [P(COsys), ctypes.c_int]
-        L.o_moveX.argtypes = [P(COsys), ctypes.c_double]
-        L.o_renorm.argtypes = [P(COsys), ctypes.c_int]
-        L.omt_seed.argtypes = [P(COmt), ctypes.c_ulong]
-        L.omt_u32.restype = ctypes.c_uint32
-        L.omt_u32.argtypes = [P(COmt)]
-        L.omt_uniform.restype = ctypes.c_double
-        L.omt_uniform.argtypes = [P(COmt)]
-        L.o_gauss.argtypes = [ctypes.c_double, _dp, _dp]
-        L.o_ini_pos.argtypes = [P(COsys)]
-        L.o_gcamix.argtypes = [P(COsys), P(COmt)]
-        L.o_ini_mom.argtypes = [P(COsys), ctypes.c_double, P(COmt)]
-        L.o_dist_test.argtypes = [P(COsys), ctypes.c_int, ctypes.c_int, ctypes.c_double]
-        L.o_clust_conf.argtypes = [P(COsys), ctypes.c_double, P(COmt)]
-        L.o_clust_conf_dd.argtypes = [P(COsys), ctypes.c_double, ctypes.c_double, P(COmt)]
-        L.o_udd.restype = ctypes.c_double
-        L.o_udd.argtypes = [P(COsys), ctypes.c_int, ctypes.c_int]
-        L.o_contact_pairs.restype = ctypes.c_long
-        L.o_contact_pairs.argtypes = [P(COsys), ctypes.c_long, _ip, _ip, ctypes.c_double, ctypes.c_double,
-                                      ctypes.POINTER(ctypes.c_ubyte)]
-        L.o_acc_correl.argtypes = [P(COsys), ctypes.c_double, ctypes.c_int, _dp]
-        L.o_acc_correl.restype = ctypes.c_long
-        _fp = ctypes.POINTER(ctypes.c_float)
-        L.o_jacobi3.argtypes = [_fp, _fp, _fp]
-        L.o_oo_params.argtypes = [P(COsys), _dp, _dp, _dp]
-        L.o_run_init.argtypes = [P(COsys), P(CRunParams), P(CRun), ctypes.c_int]
-        L.o_run_steps.argtypes = [P(COsys), P(CRunParams), P(CRun), ctypes.c_int, ctypes.c_int, _dp]
-        L.o_run_free.argtypes = [P(CRun)]
-        L.o_forces.argtypes = [P(COsys), P(CRunParams), P(CRun), _dp]
-        _lib = L
+        _lib = _bind(ctypes.CDLL(LIBPATH))
     return _lib
+
+
+_lib_omp = None
+
+
+def lib_omp():
+    """The same routines built with OpenMP (force loops split over threads):
+    only for timing the host baseline; its sums are taken in another order."""
+    global _lib_omp
+    if _lib_omp is None:
+        if not os.path.exists(LIBPATH_OMP):
+            build()
+        _lib_omp = _bind(ctypes.CDLL(LIBPATH_OMP))
+    return _lib_omp
+
+
+def _bind(L):
+    """Argument and result types of the exported routines."""
+    P = ctypes.POINTER
+    L.o_segdist.restype = ctypes.c_double
+    L.o_segdist.argtypes = [_dp, _dp, ctypes.c_double, _dp, _dp, ctypes.c_double,
+                            _dp, _dp, _dp, _dp]
+    L.o_spherocyl_dist.restype = ctypes.c_double
+    L.o_neigh_list.restype = ctypes.c_long
+    L.o_neigh_list.argtypes = [P(COsys), ctypes.c_double, _ip, _ip, ctypes.c_long]
+    L.o_round12.restype = ctypes.c_double
+    L.o_round12.argtypes = [ctypes.c_double]
+    L.o_check_list.argtypes = [P(COsys), _dp, ctypes.c_double, ctypes.c_double]
+    L.o_force_sc.argtypes = [P(COsys), ctypes.c_double, _ip, _ip, _dp, _dp, _dp]
+    L.o_tabul.argtypes = [_dp, ctypes.c_double]
+    L.o_force_dipR.argtypes = [P(COsys), _dp, ctypes.c_double, _dp, _dp]
+    L.o_wave_v.argtypes = [_dp, _dp, ctypes.c_double, ctypes.c_int]
+    L.o_force_dipK.argtypes = [P(COsys), ctypes.c_int, ctypes.c_double, _dp, _dp, _dp]
+    L.o_ext_field.argtypes = [P(COsys), _dp, _dp]
+    L.o_self_ener.restype = ctypes.c_double
+    L.o_self_ener.argtypes = [P(COsys)]
+    L.o_set_inertia.argtypes = [P(COsys), ctypes.c_int]
+    L.o_set_dip.argtypes = [P(COsys), ctypes.c_int]
+    L.o_moveX.argtypes = [P(COsys), ctypes.c_double]
+    L.o_renorm.argtypes = [P(COsys), ctypes.c_int]
+    L.omt_seed.argtypes = [P(COmt), ctypes.c_ulong]
+    L.omt_u32.restype = ctypes.c_uint32
+    L.omt_u32.argtypes = [P(COmt)]
+    L.omt_uniform.restype = ctypes.c_double
+    L.omt_uniform.argtypes = [P(COmt)]
+    L.o_gauss.argtypes = [ctypes.c_double, _dp, _dp]
+    L.o_ini_pos.argtypes = [P(COsys)]
+    L.o_gcamix.argtypes = [P(COsys), P(COmt)]
+    L.o_ini_mom.argtypes = [P(COsys), ctypes.c_double, P(COmt)]
+    L.o_dist_test.argtypes = [P(COsys), ctypes.c_int, ctypes.c_int, ctypes.c_double]
+    L.o_clust_conf.argtypes = [P(COsys), ctypes.c_double, P(COmt)]
+    L.o_clust_conf_dd.argtypes = [P(COsys), ctypes.c_double, ctypes.c_double, P(COmt)]
+    L.o_udd.restype = ctypes.c_double
+    L.o_udd.argtypes = [P(COsys), ctypes.c_int, ctypes.c_int]
+    L.o_contact_pairs.restype = ctypes.c_long
+    L.o_contact_pairs.argtypes = [P(COsys), ctypes.c_long, _ip, _ip, ctypes.c_double, ctypes.c_double,
+                                  ctypes.POINTER(ctypes.c_ubyte)]
+    L.o_acc_correl.argtypes = [P(COsys), ctypes.c_double, ctypes.c_int, _dp]
+    L.o_acc_correl.restype = ctypes.c_long
+    _fp = ctypes.POINTER(ctypes.c_float)
+    L.o_jacobi3.argtypes = [_fp, _fp, _fp]
+    L.o_oo_params.argtypes = [P(COsys), _dp, _dp, _dp]
+    L.o_run_init.argtypes = [P(COsys), P(CRunParams), P(CRun), ctypes.c_int]
+    L.o_run_steps.argtypes = [P(COsys), P(CRunParams), P(CRun), ctypes.c_int, ctypes.c_int, _dp]
+    L.o_run_free.argtypes = [P(CRun)]
+    L.o_forces.argtypes = [P(COsys), P(CRunParams), P(CRun), _dp]
+    L.o_threads.restype = ctypes.c_int
+    L.o_threads.argtypes = []
+    return L
 
 
 def dptr(a):
@@ -206,6 +228,24 @@ def run_params(**kw):
         else:
             setattr(P, k, v)
     return P
+
+
+def osys_from_replica(desc, st):
+    """Oracle copy of a replica (descriptor and state of
+    mdspherocyl_amd.configs)."""
+    S = OSys(desc.N, 1, L=tuple(desc.L))
+    for k, key in (("r", "r"), ("p", "p"), ("l", "l"), ("ori", "ori"), ("len", "length"), ("dia", "dia"),
+                   ("mass", "mass"), ("dpc", "dpc"), ("doc", "doc"), ("dmu", "dmu")):
+        getattr(S, k)[:] = st[key]
+    S.set_inertia()
+    S.set_dip()
+    return S
+
+
+def run_params_from_replica(desc):
+    """Run parameters of a replica descriptor (r_c > 0: the cut-off mode)."""
+    return run_params(dens=desc.dens, temp=desc.temp, dt=desc.dt, sr_cut=desc.sr_cut, alpha=desc.alpha,
+                      Q=desc.Q, gamma=desc.gamma, s0=desc.s0, B=tuple(desc.B), K=desc.K, rc_real=desc.rc)
 
 
 class ORun:
