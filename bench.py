@@ -89,6 +89,15 @@ def timed_run(sysm, steps, warmup, timing, step0=0):
     return dt, obs
 
 
+def autotune(sysm, name, steps=100):
+    best, tried = sysm.autotune(steps)
+    log("bench: autotune %s: %s" % (name, ", ".join(
+        "graph %d fused %d waves %d %.4f ms" % (c["graph"], c["fused_pairs"], c["pair_waves"], t) for c, t in tried)))
+    log("bench: autotune %s chose %s" % (name, best))
+    return {"choice": best, "steps": steps,
+            "tried": [dict(c, ms_per_step=round(t, 5)) for c, t in tried]}
+
+
 def traffic_from_profile(kernel):
     """HBM bytes per launch of `kernel` recorded by a counter summary (the
     line 'traffic_bytes_per_launch: B'), or None."""
@@ -288,6 +297,9 @@ def main():
     configs.load_into(mix, [(desc, st)])
     mix.init_run()
     Fm, Tm, Um = force_split(mix, 0, desc.sr_cut)
+    # the autotuner picks the launch choices (graph, fused pair launch, waves
+    # per task) by timing each from this state, then restores it
+    tune_c3 = autotune(mix, "N=32768 mixed")
 
     # --- FP64 context on the same state --------------------------------------
     f64 = m.MdSystem([desc], precision="fp64")
@@ -433,6 +445,7 @@ def main():
         bat = m.MdSystem([d for d, _ in reps], precision="mixed")
         configs.load_into(bat, reps)
         bat.init_run()
+        tune_c4 = autotune(bat, "64 replicas mixed", steps=50)
         kb = max(args.steps // 2, 10)
         t_b, _ = timed_run(bat, kb, min(args.warmup, 10), False)
         repeats = bat.chunk_repeats()
@@ -442,7 +455,8 @@ def main():
         pps_single = sps_mix * desc.N
         batched = {"replicas": len(reps), "sum_N": ntot, "steps": kb, "steps_per_s": round(sps_b, 3),
                    "replica_steps_per_s": round(sps_b * len(reps), 2), "particle_steps_per_s": round(pps_b, 1),
-                   "vs_single_particle_steps": round(pps_b / pps_single, 3), "chunk_repeats": repeats}
+                   "vs_single_particle_steps": round(pps_b / pps_single, 3), "chunk_repeats": repeats,
+                   "tuning": tune_c4}
         log("bench: batched %.2f steps/s = %.1f replica-steps/s; particle-steps/s %.3e = %.2f x single system"
             % (sps_b, sps_b * len(reps), pps_b, pps_b / pps_single))
 
@@ -504,6 +518,7 @@ def main():
         "fp64_contact_rows_steps_per_s": round(sps_dc, 3),
         "mixed_over_fp64_contact_rows": round(sps_mix / sps_dc, 3),
         "amortized": amortized,
+        "tuning": tune_c3,
         "err_mixed_vs_fp64": err_mix_vs_fp64,
         "fp64_force_norms": fp64_norms,
         "err_vs_oracle": err_oracle,

@@ -152,6 +152,33 @@ int msc_run(msc_ctx *ctx, int step0, int nsteps, double *obs);
  * MI355X at N=32768 the direct launches are about 1% faster (the step is
  * bound by the GPU, not by the host's launches). */
 int msc_set_graph(msc_ctx *ctx, int on);
+/* Launch choices of a context; no counterpart in the reference.  graph and
+ * fused_pairs give the same results bit for bit; pair_waves changes how the
+ * row entries of a 64-particle task are split over waves, so the rounding of
+ * the real-space and WCA sums (each choice is deterministic).  Defaults:
+ * graph 0, fused_pairs 1, pair_waves 4. */
+typedef struct msc_tuning {
+  int graph;        /* msc_run replays each step as one captured HIP graph */
+  int fused_pairs;  /* MSC_MIXED contexts with contact rows whose replicas all
+                       have a real-space cut-off run the WCA over the contact
+                       rows and the real-space dipole pairs in one launch,
+                       partners read as packed records */
+  int pair_waves;   /* waves per task of that launch: 2, 4 or 8 */
+} msc_tuning;
+int msc_get_tuning(msc_ctx *ctx, msc_tuning *t);
+int msc_set_tuning(msc_ctx *ctx, const msc_tuning *t);
+/* The autotuner: times `steps` steps of msc_run under each candidate tuning
+ * (graph off/on x, for contexts that can fuse, unfused or fused with 2, 4 or
+ * 8 waves per task), each from the state at the call, and keeps the
+ * fastest.  The state, thermostats and thermostat generators are restored
+ * after every trial and the lists rebuilt from the restored state (as after a
+ * chunk rollback of msc_run), so a run continues from the same state; only
+ * the summation order of the rebuilt lists can differ from an uninterrupted
+ * run.  *best (may be NULL) receives the choice; ms (may be NULL, room for
+ * nms values) the milliseconds per step of each candidate, in the order
+ * tried; *ntried (may be NULL) their number.  No counterpart in the
+ * reference. */
+int msc_autotune(msc_ctx *ctx, int steps, msc_tuning *best, double *ms, int nms, int *ntried);
 /* msc_run advances in chunks of 50 steps.  A chunk during which a list
  * rebuild outgrew its device region is rolled back, the regions are enlarged
  * and the lists rebuilt from the chunk's first state, and the chunk is run

@@ -25,11 +25,20 @@ EXPORTS = [
     "msc_init_run", "msc_run", "msc_set_graph", "msc_cluster_labels", "msc_set_timing",
     "msc_kernel_time", "msc_reset_timing", "msc_work_counts", "msc_chunk_repeats", "msc_write_conf",
     "msc_read_conf", "msc_pair_correl", "msc_order_params", "msc_ini_state", "msc_set_contact_rows",
-    "msc_pair_correl_noalloc", "msc_place_rods", "msc_cluster_labels_dd",
+    "msc_pair_correl_noalloc", "msc_place_rods", "msc_cluster_labels_dd", "msc_get_tuning", "msc_set_tuning",
+    "msc_autotune",
 ]
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
+
+
+class Tuning(ctypes.Structure):
+    """msc_tuning: launch choices of a context."""
+    _fields_ = [("graph", ctypes.c_int), ("fused_pairs", ctypes.c_int), ("pair_waves", ctypes.c_int)]
+
+    def as_dict(self):
+        return {"graph": self.graph, "fused_pairs": self.fused_pairs, "pair_waves": self.pair_waves}
 
 
 class ReplicaDesc(ctypes.Structure):
@@ -87,6 +96,9 @@ def load():
         "msc_init_run": ([vp], ctypes.c_int),
         "msc_run": ([vp, ctypes.c_int, ctypes.c_int, _dp], ctypes.c_int),
         "msc_set_graph": ([vp, ctypes.c_int], ctypes.c_int),
+        "msc_get_tuning": ([vp, ctypes.POINTER(Tuning)], ctypes.c_int),
+        "msc_set_tuning": ([vp, ctypes.POINTER(Tuning)], ctypes.c_int),
+        "msc_autotune": ([vp, ctypes.c_int, ctypes.POINTER(Tuning), _dp, ctypes.c_int, _ip], ctypes.c_int),
         "msc_cluster_labels": ([vp, ctypes.c_int, ctypes.c_double, ctypes.POINTER(ctypes.c_uint),
                                 _ip, _ip, _ip, ctypes.POINTER(ctypes.c_long)], ctypes.c_int),
         "msc_cluster_labels_dd": ([vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.POINTER(ctypes.c_uint),

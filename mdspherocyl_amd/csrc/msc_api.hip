@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -90,6 +91,9 @@ struct msc_ctx {
   std::string err;
   State S{};
   bool have_state = false, have_lists = false, graph_on = false, timing = false;
+  // launch choices (msc_tuning): the fused mixed pair launch and its waves per task
+  bool fused_pairs = true;
+  int pair_waves = 4;
   long nl_alloc = 0, sl_alloc = 0, dl_alloc = 0;  // entries allocated for the row arrays
   size_t part_elems = 0;
   int nslot_tot = 0, maxslot = 0, maxcol = 0;
@@ -112,6 +116,7 @@ struct msc_ctx {
       *d_cell_list = nullptr;
   int *d_cell_tmp = nullptr;
   double *d_cell_pack = nullptr;
+  double *d_rec_rod = nullptr, *d_rec_dip = nullptr;  // partner records (mixed contexts)
   int *d_nl_cnt = nullptr, *d_nl_idx = nullptr;
   int *d_sl_cnt = nullptr, *d_sl_idx = nullptr;
   int *d_dl_cnt = nullptr, *d_dl_idx = nullptr;
@@ -133,6 +138,8 @@ struct msc_ctx {
   std::vector<hipEvent_t> event_pool;
   double *d_run_gs = nullptr, *d_run_obs = nullptr;
   char *d_save = nullptr;  // chunk rollback copy of the step state
+  char *d_tune_save = nullptr;  // msc_autotune's copy of the step state
+  size_t tune_save_cap = 0;
   long chunk_repeats = 0;  // chunks of msc_run repeated after a list overflow
   size_t save_cap = 0;
   void *pin = nullptr;  // pinned staging buffer of kPinBytes
@@ -444,6 +451,8 @@ void refresh_state_ptrs(msc_ctx *ctx) {
   s.dl_cnt = ctx->d_dl_cnt;
   s.dl_idx = ctx->d_dl_idx;
   s.kval = ctx->d_kval;
+  s.rec_rod = ctx->d_rec_rod;
+  s.rec_dip = ctx->d_rec_dip;
 }
 
 // reciprocal weights val = 2 KK / V of wave_v/force_dipK (v25/functions.cpp:1733-1737,
@@ -854,6 +863,7 @@ int msc_create(int device, int precision, int nrep, const msc_replica_desc *reps
     R.tab0 = r * kTabN;
     for (int a = 0; a < 3; ++a) {
       R.L[a] = D.L[a];
+      R.iL[a] = 1.0 / D.L[a];
       R.P2[a] = 2. * M_PI / D.L[a];
       R.B[a] = D.B[a];
     }
@@ -939,6 +949,8 @@ int msc_create(int device, int precision, int nrep, const msc_replica_desc *reps
   // zeroed here once: entries for nz >= K and columns past the last stay 0
   if (ctx->mixed && rc == MSC_OK) rc = dalloc(ctx, (unsigned long long **)&ctx->d_bop, ctx->bop_elems);
   if (ctx->mixed && rc == MSC_OK) rc = dalloc(ctx, &ctx->d_bexp, (size_t)3 * bexp_cols);
+  if (ctx->mixed && rc == MSC_OK) rc = dalloc(ctx, &ctx->d_rec_rod, (size_t)8 * N);
+  if (ctx->mixed && rc == MSC_OK) rc = dalloc(ctx, &ctx->d_rec_dip, (size_t)8 * nd * N);
   if (rc == MSC_OK) {
     unsigned char **raw[] = {(unsigned char **)&ctx->d_tab, (unsigned char **)&ctx->d_ph,
                              (unsigned char **)&ctx->d_amp, (unsigned char **)&ctx->d_part,
@@ -1005,8 +1017,9 @@ void msc_destroy(msc_ctx *ctx) {
                   ctx->d_snap_u, ctx->d_clu, ctx->d_clu_flag, ctx->d_corr, ctx->d_corr_drop, ctx->d_order, ctx->d_rows, ctx->d_kblk, ctx->d_step,
                   ctx->d_tab, ctx->d_ph, ctx->d_amp, ctx->d_part, ctx->d_A, ctx->d_slot_out, ctx->d_task_out,
                   ctx->d_red, ctx->d_M2, ctx->d_Kin, ctx->d_gs, ctx->d_obs, ctx->d_counts, ctx->d_E,
-                  ctx->d_th, ctx->d_flags, ctx->d_rebuild, ctx->d_one, ctx->d_run_gs, ctx->d_run_obs, ctx->d_save,
-                  ctx->d_stamp, ctx->d_bop, ctx->d_bexp, ctx->d_lctr, ctx->d_dmax};
+                  ctx->d_th, ctx->d_flags, ctx->d_rebuild, ctx->d_one, ctx->d_run_gs, ctx->d_run_obs, ctx->d_save, ctx->d_tune_save,
+                  ctx->d_stamp, ctx->d_bop, ctx->d_bexp, ctx->d_lctr, ctx->d_dmax, ctx->d_rec_rod,
+                  ctx->d_rec_dip};
   for (void *b : bufs)
     if (b) hipFree(b);
   for (auto &kv : ctx->timers)
@@ -1618,6 +1631,25 @@ int msc_set_graph(msc_ctx *ctx, int on) {
   return MSC_OK;
 }
 
+int msc_get_tuning(msc_ctx *ctx, msc_tuning *t) {
+  SCOPE(ctx);
+  if (!ctx || !t) return MSC_E_ARG;
+  t->graph = ctx->graph_on ? 1 : 0;
+  t->fused_pairs = ctx->fused_pairs ? 1 : 0;
+  t->pair_waves = ctx->pair_waves;
+  return MSC_OK;
+}
+
+int msc_set_tuning(msc_ctx *ctx, const msc_tuning *t) {
+  SCOPE(ctx);
+  if (!ctx || !t || (t->pair_waves != 2 && t->pair_waves != 4 && t->pair_waves != 8)) return MSC_E_ARG;
+  drop_graph(ctx);  // the captured step holds the previous launch shapes
+  ctx->graph_on = t->graph != 0;
+  ctx->fused_pairs = t->fused_pairs != 0;
+  ctx->pair_waves = t->pair_waves;
+  return MSC_OK;
+}
+
 // One step of every replica, enqueued on the context stream, in the order of
 // v25/main.cpp:559-708 (the fused kernels are described in msc_kernels.hip).
 // The step reads its index from d_step, so the same launches serve every
@@ -1639,11 +1671,20 @@ static void enqueue_step(msc_ctx *ctx) {
     launch_drift(s, ctx->mixed, ctx->d_th, gs, ctx->d_step, rows + kRowKick1 * region, ctx->d_ph, ctx->d_amp,
                  rows + kRowDrift * region, ls, st);
   }
-  {
-    Bracket b(ctx, "wca");
-    launch_force_sc(s, !ctx->contact, rows + kRowSC * region, ctx->d_flags, st, &ls);
+  bool cut_all = true;
+  for (const RepDev &R : ctx->reps) cut_all &= R.rc > 0;
+  if (ctx->mixed && ctx->contact && cut_all && ctx->fused_pairs) {
+    // WCA over the contact rows and real-space pairs in one launch
+    Bracket b(ctx, "real_pairs");
+    launch_pairs_mixed(s, ctx->pair_waves, ctx->d_tab, rows + kRowSC * region, rows + kRowR * region,
+                       ctx->d_flags, ls, st);
+  } else {
+    {
+      Bracket b(ctx, "wca");
+      launch_force_sc(s, !ctx->contact, rows + kRowSC * region, ctx->d_flags, st, &ls);
+    }
+    dipR_launches(ctx, MSC_ALL, rows + kRowR * region);
   }
-  dipR_launches(ctx, MSC_ALL, rows + kRowR * region);
   dipK_launches(ctx, MSC_ALL, rows + kRowDrift * region + 2, ctx->d_kblk);
   // ext_field adds exactly nothing without a field: skipped then
   const bool field = any_field(ctx);
@@ -1663,7 +1704,7 @@ static void enqueue_step(msc_ctx *ctx) {
 // Device copy of everything a step carries into the next one (positions,
 // momenta, orientations, forces, list snapshot, thermostats, energies), for
 // rolling back a chunk of steps.
-static int state_copy(msc_ctx *ctx, bool restore) {
+static int state_copy(msc_ctx *ctx, bool restore, char *&save, size_t &cap) {
   const size_t N = ctx->Ntot, nrep = ctx->nrep;
   struct Item { void *p; size_t bytes; } items[] = {
       {ctx->d_x, 3 * N * 8}, {ctx->d_p, 3 * N * 8}, {ctx->d_l, 3 * N * 8}, {ctx->d_ori, 9 * N * 8},
@@ -1672,13 +1713,13 @@ static int state_copy(msc_ctx *ctx, bool restore) {
       {ctx->d_th, nrep * sizeof(Thermo)}, {ctx->d_E, nrep * sizeof(RepEnergy)}, {ctx->d_Kin, 2 * nrep * 8}};
   size_t total = 0;
   for (auto &it : items) total += (it.bytes + 255) / 256 * 256;
-  if (total > ctx->save_cap) {
-    if (int rc_ = dalloc(ctx, &ctx->d_save, total)) return rc_;
-    ctx->save_cap = total;
+  if (total > cap) {
+    if (int rc_ = dalloc(ctx, &save, total)) return rc_;
+    cap = total;
   }
   size_t off = 0;
   for (auto &it : items) {
-    char *b = ctx->d_save + off;
+    char *b = save + off;
     if (restore)
       HIPC(hipMemcpyAsync(it.p, b, it.bytes, hipMemcpyDeviceToDevice, ctx->stream));
     else
@@ -1687,6 +1728,7 @@ static int state_copy(msc_ctx *ctx, bool restore) {
   }
   return MSC_OK;
 }
+static int state_copy(msc_ctx *ctx, bool restore) { return state_copy(ctx, restore, ctx->d_save, ctx->save_cap); }
 
 // Steps run in chunks of kRunChunk.  A chunk in which a list rebuild found
 // its region too small is rolled back, the regions are enlarged to what the
@@ -1834,6 +1876,58 @@ int msc_run(msc_ctx *ctx, int step0, int nsteps, double *obs) {
   if (obs && nobs)
     if (int rc_ = d2h(ctx, obs, d_obs_all, sizeof(double) * nobs)) return rc_;
   if (ctx->timing) settle_timers(ctx);
+  return MSC_OK;
+}
+
+// The autotuner (no reference counterpart): each candidate launch choice
+// runs `steps` steps from the state at the call, after an untimed run of a
+// few steps from the same state (graph capture, first-touch costs); the state
+// is restored and the lists rebuilt after every run.
+int msc_autotune(msc_ctx *ctx, int steps, msc_tuning *best, double *ms, int nms, int *ntried) {
+  SCOPE(ctx);
+  if (!ctx || steps < 1 || (ms && nms < 0)) return MSC_E_ARG;
+  if (!ctx->have_lists) return fail(ctx, MSC_E_STATE, "call msc_init_run first");
+  bool cut_all = true;
+  for (const RepDev &R : ctx->reps) cut_all &= R.rc > 0;
+  const bool fusable = ctx->mixed && ctx->contact && cut_all;
+  std::vector<msc_tuning> cand;
+  for (int g = 0; g < 2; ++g) {
+    cand.push_back(msc_tuning{g, 0, 4});
+    if (fusable)
+      for (int w : {2, 4, 8}) cand.push_back(msc_tuning{g, 1, w});
+  }
+  const std::vector<MT> rng = ctx->thermo_rng;
+  if (int rc_ = state_copy(ctx, false, ctx->d_tune_save, ctx->tune_save_cap)) return rc_;
+  auto restore = [&]() -> int {
+    if (int rc_ = state_copy(ctx, true, ctx->d_tune_save, ctx->tune_save_cap)) return rc_;
+    ctx->thermo_rng = rng;
+    launch_fill_int(ctx->d_flags, 4 * ctx->nrep, 0, ctx->stream);
+    if (int rc_ = lists_build(ctx, ctx->d_one)) return rc_;
+    ctx->have_lists = true;
+    return MSC_OK;
+  };
+  int best_i = -1;
+  double best_ms = 0;
+  for (size_t i = 0; i < cand.size(); ++i) {
+    if (int rc_ = msc_set_tuning(ctx, &cand[i])) return rc_;
+    if (int rc_ = restore()) return rc_;
+    if (int rc_ = msc_run(ctx, 0, std::min(steps, 10), nullptr)) return rc_;
+    if (int rc_ = restore()) return rc_;
+    HIPC(hipStreamSynchronize(ctx->stream));
+    const auto t0 = std::chrono::steady_clock::now();
+    if (int rc_ = msc_run(ctx, 0, steps, nullptr)) return rc_;  // returns after its stream drained
+    const double t = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / steps;
+    if (ms && (int)i < nms) ms[i] = t;
+    if (best_i < 0 || t < best_ms) {
+      best_i = (int)i;
+      best_ms = t;
+    }
+  }
+  if (int rc_ = msc_set_tuning(ctx, &cand[best_i])) return rc_;
+  if (int rc_ = restore()) return rc_;
+  FINISH("msc_autotune");
+  if (best) *best = cand[best_i];
+  if (ntried) *ntried = (int)cand.size();
   return MSC_OK;
 }
 

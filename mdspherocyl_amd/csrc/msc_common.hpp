@@ -40,6 +40,7 @@ struct RepDev {
   int task0, ntask;     // wave tasks (64 particles) of this replica
   int rtask0, nrtask;   // reciprocal chunk tasks of this replica
   double L[3], V, P2[3];
+  double iL[3];         // 1 / L (single-precision pair sums' minimum image)
   double cw[3];         // cell widths
   double alpha, rc, sr_cut;
   double r_list_min, r_list_max;

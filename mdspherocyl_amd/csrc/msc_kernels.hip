@@ -154,6 +154,29 @@ __device__ __forceinline__ void dipoles_particle(const State &s, int g) {
       acc += O[2 * 3 + a] * c[2];
       s.dori[(3 * w + a) * N + g] = acc;
     }
+    if (s.rec_dip) {
+      double *o = s.rec_dip + (size_t)q * 8;
+      float f[8];
+      for (int a = 0; a < 3; ++a) {
+        o[a] = s.dpos[(3 * w + a) * N + g];
+        f[a] = (float)s.dori[(3 * w + a) * N + g];
+        f[3 + a] = (float)(pc * O[6 + a]);
+      }
+      f[6] = (float)s.dmu[q];
+      f[7] = 0.f;
+      float *of = reinterpret_cast<float *>(o + 4);
+      for (int k = 0; k < 8; ++k) of[k] = f[k];
+      o[3] = 0.0;
+    }
+  }
+  if (s.rec_rod) {
+    double *o = s.rec_rod + (size_t)g * 8;
+    for (int a = 0; a < 3; ++a) {
+      o[a] = r[a];
+      o[3 + a] = O[6 + a];
+    }
+    o[6] = s.len[g];
+    o[7] = s.dia[g];
   }
 }
 
@@ -257,7 +280,7 @@ __device__ __forceinline__ bool list_decision(const State &s, const RepDev &R, i
 // cell assignment of the task's particles (one per thread); the last block
 // of the replica to finish scans the counts (k_scan_cells).
 __device__ void cells_assign_task(const State &s, int t, const ListStep &ls) {
-  __shared__ int sh[64 * kPairSplit];
+  __shared__ int sh[kMaxPairWaves * 64];
   const int tid = s.task_begin + t;
   const Task tk = s.tasks[tid];
   const RepDev &R = s.reps[tk.rep];
@@ -911,8 +934,16 @@ __device__ __forceinline__ bool dip_pair(const R rij_in[3], const R oi[3], const
   R rij[3] = {rij_in[0], rij_in[1], rij_in[2]};
   R RIJSQ = rij[0] * rij[0] + rij[1] * rij[1] + rij[2] * rij[2];
   R uu = mi * mj;
-  R SR2 = (R)1 / RIJSQ;
-  R RIJ = sqrt(RIJSQ);
+  R SR2, RIJ;
+  if constexpr (sizeof(R) == sizeof(double)) {
+    SR2 = (R)1 / RIJSQ;
+    RIJ = sqrt(RIJSQ);
+  } else {
+    // hardware reciprocal square root (within 1 ulp): 1/r^2 and r from it
+    const R rs = __builtin_amdgcn_rsqf(RIJSQ);
+    SR2 = rs * rs;
+    RIJ = RIJSQ * rs;
+  }
   if (rc > 0 && !(RIJ < rc)) return false;
   R A, B, C;
   if constexpr (sizeof(R) == sizeof(double)) {
@@ -982,6 +1013,23 @@ __device__ __forceinline__ bool dip_pair(const R rij_in[3], const R oi[3], const
   return true;
 }
 
+// Minimum image of a separation component for the single-precision pair
+// sums: the nearest multiple of the box by a product with its reciprocal (the
+// FP64 paths divide, as the reference does; the two differ only at exact
+// half-box separations).
+__device__ __forceinline__ double minimg_mul(double d, double L, double iL) { return d - L * rint(d * iL); }
+
+// The real-space cut-off test sqrt(r2) < rc (rc > 0) of the FP64 sums (FP64
+// r2 and sqrt) for the single-precision pair sums, so that they drop and keep
+// the same pairs: the comparison of squares decides every pair outside a
+// band of 1e-13 relative around rc^2, the sqrt the rest.
+__device__ __forceinline__ bool within_cut(double r2, double rc) {
+  const double rc2 = rc * rc;
+  if (r2 < rc2 * (1.0 - 1e-13)) return true;
+  if (r2 > rc2 * (1.0 + 1e-13)) return false;
+  return sqrt(r2) < rc;
+}
+
 // Dipole data of particle g, slot w, in real type R.
 template <typename R>
 struct Dip {
@@ -1021,13 +1069,22 @@ __device__ __forceinline__ void dipR_partner(const State &s, const RepDev &Rp, c
       Dip<R> Dj = load_dip<R>(s, j, w);
       if (Dj.mu == (R)0) continue;
       R rij[3];
-      for (int a = 0; a < 3; ++a) {
-        double d = Di.pos[a] - Dj.pos[a];
-        d -= Rp.L[a] * floor(d / Rp.L[a] + 0.5);
-        rij[a] = (R)d;
+      R rc = (R)Rp.rc;
+      if constexpr (sizeof(R) == sizeof(double)) {
+        for (int a = 0; a < 3; ++a) {
+          double d = Di.pos[a] - Dj.pos[a];
+          d -= Rp.L[a] * floor(d / Rp.L[a] + 0.5);
+          rij[a] = d;
+        }
+      } else {
+        double d[3];
+        for (int a = 0; a < 3; ++a) d[a] = minimg_mul(Di.pos[a] - Dj.pos[a], Rp.L[a], Rp.iL[a]);
+        if (Rp.rc > 0 && !within_cut(d[0] * d[0] + d[1] * d[1] + d[2] * d[2], Rp.rc)) continue;
+        for (int a = 0; a < 3; ++a) rij[a] = (R)d[a];
+        rc = 0;  // decided above (rc <= 0: every pair)
       }
       PairOut<R> o;
-      if (!dip_pair<R>(rij, Di.o, Dj.o, Di.mu, Dj.mu, Di.d, Dj.d, tab, (R)Rp.rc, 0, gk, o)) continue;
+      if (!dip_pair<R>(rij, Di.o, Dj.o, Di.mu, Dj.mu, Di.d, Dj.d, tab, rc, 0, gk, o)) continue;
       for (int a = 0; a < 3; ++a) {
         F[a] += (double)o.f[a];
         T[a] += (double)o.t[a];
@@ -1110,6 +1167,227 @@ __global__ void __launch_bounds__(64 * kPairSplit) k_dipR_list(State s, const R 
     }
   }
   wave_store_sums(task_out + (size_t)tid * 16, acc, 11, lane);
+}
+
+// --------------------------------------------------------------------------
+// Mixed contexts in the step loop, replicas with a real-space cut-off: the
+// WCA over the contact rows (k_force_sc<false>) and the real-space dipole
+// pairs over the dipole rows (k_dipR_list<float>) of one task in one block,
+// every partner read as one 64-byte record (State::rec_rod, rec_dip) instead
+// of eleven scattered FP64 words.  Each sum is formed as in those kernels
+// (same row order, same wave split, same LDS order); forces and torques are
+// added to F and T WCA first, as the two launches add them.  Extra blocks
+// (ls.rebuild set): the list decision and cell assignment, as k_force_sc.
+// --------------------------------------------------------------------------
+__device__ __forceinline__ void load_rec8(const double *__restrict__ p, double o[8]) {
+  const double2 *q = reinterpret_cast<const double2 *>(p);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const double2 v = q[k];
+    o[2 * k] = v.x;
+    o[2 * k + 1] = v.y;
+  }
+}
+__device__ __forceinline__ Dip<float> load_dip_rec(const State &s, int g, int w) {
+  const double *p = s.rec_dip + ((size_t)w * s.Ntot + g) * 8;
+  const double2 *q = reinterpret_cast<const double2 *>(p);
+  const double2 a = q[0], b = q[1];
+  const float4 *f = reinterpret_cast<const float4 *>(p + 4);
+  const float4 c = f[0], d = f[1];
+  Dip<float> D;
+  D.pos[0] = a.x;
+  D.pos[1] = a.y;
+  D.pos[2] = b.x;
+  D.o[0] = c.x;
+  D.o[1] = c.y;
+  D.o[2] = c.z;
+  D.d[0] = c.w;
+  D.d[1] = d.x;
+  D.d[2] = d.y;
+  D.mu = d.z;
+  return D;
+}
+
+// Sum over the PS waves of a block of n per-lane values v (lane-wise, in wave
+// order), in chunks of 6 through LDS; the result is valid in wave 0.
+template <int PS>
+__device__ __forceinline__ void block_lane_sums(double (*red)[6][kWave], double *v, int n, int sw, int lane) {
+  for (int c0 = 0; c0 < n; c0 += 6) {
+    const int m = min(6, n - c0);
+    for (int k = 0; k < m; ++k) red[sw][k][lane] = v[c0 + k];
+    __syncthreads();
+    if (sw == 0)
+      for (int k = 0; k < m; ++k) {
+        double t = 0;
+        for (int c = 0; c < PS; ++c) t += red[c][k][lane];
+        v[c0 + k] = t;
+      }
+    __syncthreads();
+  }
+}
+
+template <int PS>
+__global__ void __launch_bounds__(64 * PS) k_pairs_mixed(State s, const float *__restrict__ tabs,
+                                                          double *__restrict__ sc_out, double *__restrict__ r_out,
+                                                          int *__restrict__ flags, ListStep ls) {
+  __shared__ double red[PS][6][kWave];
+  const int lane = threadIdx.x & 63, sw = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  if ((int)blockIdx.x >= s.ntask) {
+    if (ls.rebuild && (int)blockIdx.x < 2 * s.ntask) cells_assign_task(s, (int)blockIdx.x - s.ntask, ls);
+    return;
+  }
+  const int tid = s.task_begin + blockIdx.x;
+  const Task tk = s.tasks[tid];
+  const RepDev &R = s.reps[tk.rep];
+  const int N = s.Ntot;
+  const float *tab = tabs + (size_t)R.tab0 * 3;
+  double accS[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};  // U, W, PT[9], pairs
+  double accR[11] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};     // U, PT[9], pairs
+  double FS[3] = {0, 0, 0}, TS[3] = {0, 0, 0}, FR[3] = {0, 0, 0}, TR[3] = {0, 0, 0};
+  if (lane < tk.n) {
+    const int g = tk.g0 + lane;
+    // WCA over the contact rows (k_force_sc<false>)
+    {
+      const double srt = pow(2., 1. / 6);
+      const double srtsq = srt * srt;
+      const double r_cut_sq = R.sr_cut * R.sr_cut;
+      int code = 0;
+      double pg[8];
+      load_rec8(s.rec_rod + (size_t)g * 8, pg);
+      const int n = s.sl_cnt[g];
+      for (int q = sw; q < n; q += PS) {
+        const int j = s.sl_idx[row_at(R.sl_base, R.N, R.n0, q, g)];
+        double pj[8];
+        load_rec8(s.rec_rod + (size_t)j * 8, pj);
+        double sigma = (pg[7] + pj[7]) / 2;
+        double sig_sq = sigma * sigma;
+        const bool gA = g < j;
+        const double *a_ = gA ? pg : pj, *b_ = gA ? pj : pg;
+        double rA[3] = {a_[0], a_[1], a_[2]}, rB[3] = {b_[0], b_[1], b_[2]};
+        double uA[3] = {a_[3], a_[4], a_[5]}, uB[3] = {b_[3], b_[4], b_[5]};
+        double LA = a_[6], LB = b_[6];
+        double rsh = srtsq * sig_sq;
+        double rij[3], rsq = 0;
+        for (int a = 0; a < 3; ++a) {
+          rij[a] = rA[a] - rB[a];
+          rij[a] -= R.L[a] * floor(rij[a] / R.L[a] + 0.5);
+          rsq += rij[a] * rij[a];
+        }
+        double st[2], dij[3];
+        double dsq = segdist(rA, uA, LA, rB, uB, LB, R.L, rij, dij, st);
+        if (dsq < 0.49 * sig_sq) code = code ? code : 1;
+        if (dsq > rsq * 1.00001) code = code ? code : 2;
+        if (dsq < r_cut_sq && dsq < rsh) {
+          double SR2s = sig_sq / dsq;
+          double SR6s = SR2s * SR2s * SR2s;
+          double pot = 4 * SR6s * (SR6s - 1.);
+          double ddr = dot3(rij, dij);
+          double UIJ = pot + 1.;
+          double WIJ = 6 * (pot + 4 * SR6s * SR6s) * ddr / dsq;
+          double FIJ = WIJ / ddr;
+          double f[3];
+          for (int a = 0; a < 3; ++a) f[a] = FIJ * dij[a];
+          double icroj[3], icror[3], jcror[3];
+          cross3(uA, uB, icroj);
+          cross3(uA, rij, icror);
+          cross3(uB, rij, jcror);
+          double fact1 = (st[0] - 0.5) * LA;
+          double fact2 = -(st[1] - 0.5) * LB;
+          if (st[0] == -1.e+6) fact1 = fact2 = 0.;
+          if (st[0] == -1.e+10) { fact1 = 0.5 * LA; fact2 = 0.5 * LB; }
+          else if (st[0] == -2.e+10) { fact1 = -0.5 * LA; fact2 = -0.5 * LB; }
+          else if (st[0] == -3.e+10) { fact1 = 0.5 * LA; fact2 = -0.5 * LB; }
+          double fmul = fact1 * fact2;
+          if (gA) {
+            for (int a = 0; a < 3; ++a) FS[a] += f[a];
+            for (int a = 0; a < 3; ++a) TS[a] += FIJ * (fact1 * icror[a] + fmul * icroj[a]);
+            accS[0] += UIJ;
+            accS[1] += WIJ;
+            accS[11] += 1.0;
+            for (int a = 0; a < 3; ++a)
+              for (int c = 0; c < 3; ++c) accS[2 + 3 * a + c] += rij[a] * f[c];
+          } else {
+            for (int a = 0; a < 3; ++a) FS[a] += -f[a];
+            for (int a = 0; a < 3; ++a) TS[a] += FIJ * (fact2 * jcror[a] - fmul * icroj[a]);
+          }
+        }
+      }
+      if (code) atomicCAS(&flags[tk.rep * 4 + 0], 0, code);
+    }
+    // real-space dipole pairs over the dipole rows (k_dipR_list<float>)
+    {
+      const Dip<float> Dg0 = load_dip_rec(s, g, 0);
+      const int n = s.dl_cnt[g];
+      for (int q = sw; q < n; q += PS) {
+        const int j = s.dl_idx[row_at(R.dl_base, R.N, R.n0, q, g)];
+        const bool gk = g < j;
+        for (int v = 0; v < s.nd; ++v) {
+          const Dip<float> Di = s.nd == 1 ? Dg0 : load_dip_rec(s, g, v);
+          if (Di.mu == 0.f) continue;
+          for (int w = 0; w < s.nd; ++w) {
+            const Dip<float> Dj = load_dip_rec(s, j, w);
+            if (Dj.mu == 0.f) continue;
+            double d[3];
+            for (int a = 0; a < 3; ++a) d[a] = minimg_mul(Di.pos[a] - Dj.pos[a], R.L[a], R.iL[a]);
+            if (!within_cut(d[0] * d[0] + d[1] * d[1] + d[2] * d[2], R.rc)) continue;
+            const float rij[3] = {(float)d[0], (float)d[1], (float)d[2]};
+            PairOut<float> o;
+            dip_pair<float>(rij, Di.o, Dj.o, Di.mu, Dj.mu, Di.d, Dj.d, tab, 0.f, 0, gk, o);
+            for (int a = 0; a < 3; ++a) {
+              FR[a] += (double)o.f[a];
+              TR[a] += (double)o.t[a];
+            }
+            if (gk) {
+              accR[0] += (double)o.U;
+              for (int c = 0; c < 9; ++c) accR[1 + c] += (double)o.pt[c];
+              accR[10] += 1.0;
+            }
+          }
+        }
+      }
+    }
+  }
+  // the WCA partial sums meet in LDS, then the dipole ones
+  {
+    double v[18];
+    for (int a = 0; a < 3; ++a) {
+      v[a] = FS[a];
+      v[3 + a] = TS[a];
+    }
+    for (int q = 0; q < 12; ++q) v[6 + q] = accS[q];
+    block_lane_sums<PS>(red, v, 18, sw, lane);
+    for (int a = 0; a < 3; ++a) {
+      FS[a] = v[a];
+      TS[a] = v[3 + a];
+    }
+    for (int q = 0; q < 12; ++q) accS[q] = v[6 + q];
+    for (int a = 0; a < 3; ++a) {
+      v[a] = FR[a];
+      v[3 + a] = TR[a];
+    }
+    for (int q = 0; q < 11; ++q) v[6 + q] = accR[q];
+    block_lane_sums<PS>(red, v, 17, sw, lane);
+    for (int a = 0; a < 3; ++a) {
+      FR[a] = v[a];
+      TR[a] = v[3 + a];
+    }
+    for (int q = 0; q < 11; ++q) accR[q] = v[6 + q];
+  }
+  if (sw != 0) return;
+  if (lane < tk.n) {
+    const int g = tk.g0 + lane;
+    for (int a = 0; a < 3; ++a) {
+      double f = s.F[a * N + g], t = s.T[a * N + g];
+      f += FS[a];
+      t += TS[a];
+      f += FR[a];
+      t += TR[a];
+      s.F[a * N + g] = f;
+      s.T[a * N + g] = t;
+    }
+  }
+  wave_store_sums(sc_out + (size_t)tid * 16, accS, 12, lane);
+  wave_store_sums(r_out + (size_t)tid * 16, accR, 11, lane);
 }
 
 // --------------------------------------------------------------------------
@@ -2889,6 +3167,17 @@ void launch_dipR(const State &s, bool mixed, bool all_pairs, const void *tab, do
       MSC_LAUNCH(k_dipR_list<double>, dim3(s.ntask), dim3(64 * kPairSplit), 0, st, s, (const double *)tab,
                  task_out);
   }
+}
+void launch_pairs_mixed(const State &s, int waves, const void *tab, double *sc_out, double *r_out, int *flags,
+                        const ListStep &ls, hipStream_t st) {
+  const dim3 grid(ls.rebuild ? 2 * s.ntask : s.ntask);
+  const float *t = (const float *)tab;
+  if (waves == 2)
+    MSC_LAUNCH(k_pairs_mixed<2>, grid, dim3(128), 0, st, s, t, sc_out, r_out, flags, ls);
+  else if (waves == 8)
+    MSC_LAUNCH(k_pairs_mixed<8>, grid, dim3(512), 0, st, s, t, sc_out, r_out, flags, ls);
+  else
+    MSC_LAUNCH(k_pairs_mixed<4>, grid, dim3(256), 0, st, s, t, sc_out, r_out, flags, ls);
 }
 void launch_ext_field(const State &s, double *task_out, hipStream_t st) {
   MSC_LAUNCH(k_ext_field, dim3(nblk(s.ntask, 4)), dim3(256), 0, st, s, task_out);

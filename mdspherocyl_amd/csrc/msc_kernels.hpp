@@ -37,6 +37,11 @@ struct State {
   int *dl_cnt, *dl_idx;
   // reciprocal weights per slot (0 outside the half space)
   const double *kval;
+  // Partner records, rewritten with the dipoles whenever the particles move
+  // (null: not kept): rec_rod [N][8] centre, axis, length, diameter;
+  // rec_dip [nd][N][8] dipole position (3 doubles), then as 8 floats the
+  // orientation, the offset dpc u and the moment (one 64-byte line each).
+  double *rec_rod, *rec_dip;
 };
 
 // List maintenance of the step loop (v25/main.cpp:679-701).  k_drift
@@ -82,6 +87,13 @@ void launch_check(const State &s, double *task_out, int *rebuild, int *srebuild,
 void launch_force_sc(const State &s, bool strict, double *task_out, int *flags, hipStream_t st,
                      const ListStep *ls = nullptr);
 void launch_dipR(const State &s, bool mixed, bool all_pairs, const void *tab, double *task_out, hipStream_t st);
+// mixed contexts, every replica with a real-space cut-off and contact rows:
+// k_force_sc<false> and k_dipR_list<float> in one launch (step loop; ls as
+// launch_force_sc's)
+// waves: per 64-particle task, 2, 4 or 8 (the row entries q = w mod waves go
+// to wave w; the choice changes the rounding of the pair sums only)
+void launch_pairs_mixed(const State &s, int waves, const void *tab, double *sc_out, double *r_out, int *flags,
+                        const ListStep &ls, hipStream_t st);
 void launch_ext_field(const State &s, double *task_out, hipStream_t st);
 void launch_self_ener(const State &s, double *task_out, hipStream_t st);
 // out[rep*ostride + ooff + k] = replica sum of in[item*istride + k], k < nv;
@@ -162,6 +174,7 @@ void launch_pair_correl(const double *x, const double *ori, int Ntot, const Corr
 constexpr int kChunk = 64;  // particles per reciprocal structure-factor chunk
 constexpr int kRedStride = 16;  // doubles per replica in the reduction scratch d_red
 constexpr int kRfGroups = 8;    // waves sharing one task in the reciprocal back projection
+constexpr int kMaxPairWaves = 8;  // most waves per task of the fused mixed pair kernel
 constexpr int kPairSplit = 4;   // waves sharing one task in the WCA and real-space pair kernels
 
 }  // namespace msc

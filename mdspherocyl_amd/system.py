@@ -193,6 +193,38 @@ class MdSystem:
     def set_graph(self, on):
         self._chk(self.L.msc_set_graph(self._ctx, int(bool(on))))
 
+    def get_tuning(self):
+        """Launch choices (msc_tuning) as a dict."""
+        t = _lib.Tuning()
+        self._chk(self.L.msc_get_tuning(self._ctx, ctypes.byref(t)))
+        return t.as_dict()
+
+    def set_tuning(self, graph=None, fused_pairs=None, pair_waves=None):
+        t = _lib.Tuning()
+        self._chk(self.L.msc_get_tuning(self._ctx, ctypes.byref(t)))
+        if graph is not None:
+            t.graph = int(bool(graph))
+        if fused_pairs is not None:
+            t.fused_pairs = int(bool(fused_pairs))
+        if pair_waves is not None:
+            t.pair_waves = int(pair_waves)
+        self._chk(self.L.msc_set_tuning(self._ctx, ctypes.byref(t)))
+
+    def autotune(self, steps=100):
+        """msc_autotune: times every candidate launch choice over `steps`
+        steps from the current state (restored after each trial) and keeps
+        the fastest.  Returns (choice dict, [(candidate dict, ms/step)])."""
+        best = _lib.Tuning()
+        ms = (ctypes.c_double * 16)()
+        n = ctypes.c_int()
+        self._chk(self.L.msc_autotune(self._ctx, int(steps), ctypes.byref(best), ms, 16, ctypes.byref(n)))
+        cands = []
+        for g in (0, 1):
+            cands.append({"graph": g, "fused_pairs": 0, "pair_waves": 4})
+            if n.value > 2:
+                cands += [{"graph": g, "fused_pairs": 1, "pair_waves": w} for w in (2, 4, 8)]
+        return best.as_dict(), [(c, ms[i]) for i, c in enumerate(cands[:n.value])]
+
     # -- measurement --------------------------------------------------------
     def set_timing(self, on):
         self._chk(self.L.msc_set_timing(self._ctx, int(bool(on))))
