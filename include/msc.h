@@ -113,6 +113,13 @@ int msc_set_thermostat(msc_ctx *ctx, int rep, const double s[2], const double st
 /* neigh_list (v25/functions.cpp:1298-1334): build the neighbour pairs with
  * segment distance < r_list on the device; savec_list snapshot included. */
 int msc_neigh_list(msc_ctx *ctx, int rep, double r_list);
+/* The caller's list as the replica's neighbour rows: force_sc's nblist
+ * argument (v25/functions.cpp:1397-1398, built by neigh_list) in CSR form,
+ * off[N+1] and nbr[off[N]], row f holding ascending partners g > f.  The
+ * list snapshot of savec_list (v25/functions.cpp:1336-1353) is taken from the
+ * present state and r_list becomes the replica's list radius.  A row that is
+ * not ascending or names a partner outside (f, N) is MSC_E_ARG. */
+int msc_set_neigh_list(msc_ctx *ctx, int rep, double r_list, const int *off, const int *nbr);
 /* The list in the reference's form: for each f, ascending g > f.  off has
  * N+1 entries; nbr holds cap entries; *total receives the pair count. */
 int msc_get_neigh_list(msc_ctx *ctx, int rep, int *off, int *nbr, long cap, long *total);
@@ -124,7 +131,10 @@ int msc_check_list(msc_ctx *ctx, int rep, double r_list_min, double r_list_max,
 int msc_force_sc(msc_ctx *ctx, int rep, double r_cut, double *U, double *W, double PT[9]);
 /* force_dipR (v25/functions.cpp:1555-1678), table of tabul (1680-1704) */
 int msc_force_dipR(msc_ctx *ctx, int rep, double *U, double PT[9]);
-/* force_dipK (v25/functions.cpp:1706-1961), weights of wave_v (1963-1990) */
+/* force_dipK (v25/functions.cpp:1706-1961), weights of wave_v (1963-1990).
+ * With several dipole slots per particle the interactions between the slots
+ * of one particle are excluded (v25/functions.cpp:1780-1827) and the sum is
+ * FP64 in every context. */
 int msc_force_dipK(msc_ctx *ctx, int rep, double *U, double PT[9]);
 /* ext_field (v25/functions.cpp:1992-2025) */
 int msc_ext_field(msc_ctx *ctx, int rep, double *U);

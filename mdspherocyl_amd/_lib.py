@@ -26,7 +26,7 @@ EXPORTS = [
     "msc_kernel_time", "msc_reset_timing", "msc_work_counts", "msc_chunk_repeats", "msc_write_conf",
     "msc_read_conf", "msc_pair_correl", "msc_order_params", "msc_ini_state", "msc_set_contact_rows",
     "msc_pair_correl_noalloc", "msc_place_rods", "msc_cluster_labels_dd", "msc_get_tuning", "msc_set_tuning",
-    "msc_autotune",
+    "msc_autotune", "msc_set_neigh_list",
 ]
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -83,6 +83,7 @@ def load():
         "msc_neigh_list": ([vp, ctypes.c_int, ctypes.c_double], ctypes.c_int),
         "msc_get_neigh_list": ([vp, ctypes.c_int, _ip, _ip, ctypes.c_long, ctypes.POINTER(ctypes.c_long)],
                                ctypes.c_int),
+        "msc_set_neigh_list": ([vp, ctypes.c_int, ctypes.c_double, _ip, _ip], ctypes.c_int),
         "msc_check_list": ([vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, _ip], ctypes.c_int),
         "msc_force_sc": ([vp, ctypes.c_int, ctypes.c_double, _dp, _dp, _dp], ctypes.c_int),
         "msc_force_dipR": ([vp, ctypes.c_int, _dp, _dp], ctypes.c_int),

@@ -124,6 +124,10 @@ struct msc_ctx {
   double *d_kval = nullptr;
   void *d_tab = nullptr, *d_ph = nullptr, *d_amp = nullptr, *d_part = nullptr, *d_A = nullptr;
   void *d_bop = nullptr;  // matrix-core back-projection operands (mixed contexts)
+  // several dipole slots per particle (nd > 1): FP64 slot phase tables,
+  // chunk partials and amplitudes of the reciprocal sum (launch_recip_nd)
+  void *d_ph_nd = nullptr, *d_A_nd = nullptr;
+  double *d_part_nd = nullptr;
   long bop_elems = 0;     // their count, in groups of 4 fp16 values
   int *d_bexp = nullptr;  // their per-column scale exponents
   std::vector<double> kval_host;  // reciprocal weights of every slot (build_kspace)
@@ -821,7 +825,7 @@ int msc_create(int device, int precision, int nrep, const msc_replica_desc *reps
   int nd = reps[0].nd;
   for (int r = 0; r < nrep; ++r) {
     const msc_replica_desc &D = reps[r];
-    if (D.N <= 0 || D.nd != nd || nd < 1 || nd > 8 || D.K < 2 || D.K > kMaxKZ || D.L[0] <= 0 || D.L[1] <= 0 ||
+    if (D.N <= 0 || D.nd != nd || nd < 1 || nd > kMaxNd || D.K < 2 || D.K > kMaxKZ || D.L[0] <= 0 || D.L[1] <= 0 ||
         D.L[2] <= 0 || D.alpha <= 0) {
       delete ctx;
       return MSC_E_ARG;
@@ -951,6 +955,9 @@ int msc_create(int device, int precision, int nrep, const msc_replica_desc *reps
   if (ctx->mixed && rc == MSC_OK) rc = dalloc(ctx, &ctx->d_bexp, (size_t)3 * bexp_cols);
   if (ctx->mixed && rc == MSC_OK) rc = dalloc(ctx, &ctx->d_rec_rod, (size_t)8 * N);
   if (ctx->mixed && rc == MSC_OK) rc = dalloc(ctx, &ctx->d_rec_dip, (size_t)8 * nd * N);
+  if (nd > 1 && rc == MSC_OK) rc = dalloc(ctx, (unsigned char **)&ctx->d_ph_nd, (size_t)nd * 3 * kMaxKZ * N * 16);
+  if (nd > 1 && rc == MSC_OK) rc = dalloc(ctx, &ctx->d_part_nd, ctx->part_elems / 6 * 10);
+  if (nd > 1 && rc == MSC_OK) rc = dalloc(ctx, (unsigned char **)&ctx->d_A_nd, (size_t)ctx->nslot_tot * 16);
   if (rc == MSC_OK) {
     unsigned char **raw[] = {(unsigned char **)&ctx->d_tab, (unsigned char **)&ctx->d_ph,
                              (unsigned char **)&ctx->d_amp, (unsigned char **)&ctx->d_part,
@@ -1019,7 +1026,7 @@ void msc_destroy(msc_ctx *ctx) {
                   ctx->d_red, ctx->d_M2, ctx->d_Kin, ctx->d_gs, ctx->d_obs, ctx->d_counts, ctx->d_E,
                   ctx->d_th, ctx->d_flags, ctx->d_rebuild, ctx->d_one, ctx->d_run_gs, ctx->d_run_obs, ctx->d_save, ctx->d_tune_save,
                   ctx->d_stamp, ctx->d_bop, ctx->d_bexp, ctx->d_lctr, ctx->d_dmax, ctx->d_rec_rod,
-                  ctx->d_rec_dip};
+                  ctx->d_rec_dip, ctx->d_ph_nd, ctx->d_part_nd, ctx->d_A_nd};
   for (void *b : bufs)
     if (b) hipFree(b);
   for (auto &kv : ctx->timers)
@@ -1284,6 +1291,67 @@ int msc_neigh_list(msc_ctx *ctx, int rep, double r_list) {
   return MSC_OK;
 }
 
+// The caller's list in the reference's form (rows f of ascending g > f) as
+// replica rep's reference rows.  Every replica's rows and list snapshots are
+// built first (regions sized to hold the caller's longest row), then rep's
+// rows are replaced by the caller's pairs, both directions, in ascending
+// partner order, and its contact rows are refreshed from them.
+int msc_set_neigh_list(msc_ctx *ctx, int rep, double r_list, const int *off, const int *nbr) {
+  SCOPE(ctx);
+  if (!ctx || rep < 0 || rep >= ctx->nrep || !(r_list > 0) || !off) return MSC_E_ARG;
+  if (!ctx->have_state) return fail(ctx, MSC_E_STATE, "no state");
+  RepDev &R0 = ctx->reps[rep];
+  const int n = R0.N;
+  if (off[0] != 0) return fail(ctx, MSC_E_ARG, "off[0] must be 0");
+  for (int f = 0; f < n; ++f)
+    if (off[f + 1] < off[f]) return fail(ctx, MSC_E_ARG, "off is not ascending");
+  if (off[n] > 0 && !nbr) return fail(ctx, MSC_E_ARG, "no pairs given");
+  std::vector<int> cnt(n, 0);
+  for (int f = 0; f < n; ++f)
+    for (int q = off[f]; q < off[f + 1]; ++q) {
+      const int g = nbr[q];
+      if (g <= f || g >= n || (q > off[f] && g <= nbr[q - 1]))
+        return fail(ctx, MSC_E_ARG, "row f must hold ascending partners g > f below N");
+      ++cnt[f];
+      ++cnt[g];
+    }
+  long longest = 0;
+  for (int c : cnt) longest = std::max<long>(longest, c);
+  ctx->have_lists = false;
+  R0.r_list_max = r_list;
+  std::vector<long> need(ctx->nrep, 0);
+  need[rep] = longest;
+  int rc = lists_layout(ctx, need);
+  if (rc) return rc;
+  launch_fill_int(ctx->d_flags, 4 * ctx->nrep, 0, ctx->stream);
+  if ((rc = lists_build(ctx, ctx->d_one))) return rc;
+  const RepDev &R = ctx->reps[rep];
+  if (longest > R.nl_w) return fail(ctx, MSC_E_CAPACITY, "row region smaller than the caller's longest row");
+  // rows in the device layout: entry q of particle i at q * N + i (global
+  // partner indices); ascending partners, as f < g visits them
+  std::vector<int> idx((size_t)std::max<long>((long)R.nl_w * n, 1), 0), fill(n, 0);
+  for (int f = 0; f < n; ++f)
+    for (int q = off[f]; q < off[f + 1]; ++q) {
+      const int g = nbr[q];
+      idx[(size_t)fill[g]++ * n + g] = R.n0 + f;
+    }
+  // every row g so far holds its partners f < g in ascending order; the
+  // partners above follow, ascending
+  for (int f = 0; f < n; ++f)
+    for (int q = off[f]; q < off[f + 1]; ++q) idx[(size_t)fill[f]++ * n + f] = R.n0 + nbr[q];
+  if ((rc = h2d(ctx, ctx->d_nl_cnt + R.n0, cnt.data(), sizeof(int) * n))) return rc;
+  if ((rc = h2d(ctx, ctx->d_nl_idx + R.nl_base, idx.data(), sizeof(int) * (size_t)R.nl_w * n))) return rc;
+  if (ctx->contact) launch_short(sel(ctx, rep), ctx->d_one, ctx->d_flags, ctx->stream);
+  FINISH("msc_set_neigh_list");
+  std::vector<int> fl(4 * ctx->nrep);
+  if (int rc_ = d2h(ctx, fl.data(), ctx->d_flags, sizeof(int) * fl.size())) return rc_;
+  for (int r = 0; r < ctx->nrep; ++r)
+    if (fl[4 * r + 1] || fl[4 * r + 2] || fl[4 * r + 3])
+      return fail(ctx, MSC_E_CAPACITY, "neighbour rows longer than measured");
+  ctx->have_lists = true;
+  return MSC_OK;
+}
+
 int msc_get_neigh_list(msc_ctx *ctx, int rep, int *off, int *nbr, long cap, long *total) {
   SCOPE(ctx);
   if (!ctx || rep < 0 || rep >= ctx->nrep || !off || !total) return MSC_E_ARG;
@@ -1421,6 +1489,19 @@ int msc_force_dipR(msc_ctx *ctx, int rep, double *U, double PT[9]) {
 // rows come from k_drift (m2rows) and the energies go to kblk for k_end.
 static int dipK_launches(msc_ctx *ctx, int rep, const double *m2rows, double *kblk) {
   State s = sel(ctx, rep);
+  if (ctx->nd > 1) {
+    // several slots per particle: intra-particle exclusion by the particles'
+    // own sums (FP64 in every context)
+    const int rt0 = rep == MSC_ALL ? 0 : ctx->reps[rep].rtask0;
+    const int nrt = rep == MSC_ALL ? (int)ctx->rtasks.size() : ctx->reps[rep].nrtask;
+    {
+      Bracket b(ctx, "recip_force");
+      launch_recip_nd(s, rt0, nrt, ctx->maxslot, ctx->d_ph_nd, ctx->d_part_nd, ctx->d_A_nd, ctx->d_slot_out, kblk,
+                      ctx->stream);
+    }
+    if (!kblk) launch_reduce(s, ctx->d_slot_out, 10, 10, 1, (double *)ctx->d_E, kEstride, kEoff_K, ctx->stream);
+    return MSC_OK;
+  }
   if (!m2rows) {
     launch_recip_tables(s, ctx->mixed, ctx->d_ph, ctx->d_amp, ctx->d_task_out, ctx->stream);
     launch_reduce(s, ctx->d_task_out, 16, 6, 0, ctx->d_M2, 6, 0, ctx->stream);
@@ -1479,7 +1560,6 @@ static int do_dipK(msc_ctx *ctx, int rep) { return dipK_launches(ctx, rep, nullp
 int msc_force_dipK(msc_ctx *ctx, int rep, double *U, double PT[9]) {
   SCOPE(ctx);
   if (!ctx || rep < 0 || rep >= ctx->nrep) return MSC_E_ARG;
-  if (ctx->nd != 1) return fail(ctx, MSC_E_ARG, "the reciprocal kernels take one dipole slot per particle");
   int rc = do_dipK(ctx, rep);
   if (rc) return rc;
   FINISH("msc_force_dipK");
@@ -1597,7 +1677,6 @@ static void forces_all(msc_ctx *ctx) {
 int msc_init_run(msc_ctx *ctx) {
   SCOPE(ctx);
   if (!ctx || !ctx->have_state) return MSC_E_ARG;
-  if (ctx->nd != 1) return fail(ctx, MSC_E_ARG, "the step loop takes one dipole slot per particle");
   ctx->have_lists = false;
   int rc = lists_layout(ctx);
   if (rc) return rc;

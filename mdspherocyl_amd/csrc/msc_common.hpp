@@ -15,6 +15,7 @@ constexpr int kWave = 64;
 constexpr int kTabN = 32768;               // table length of tabul
 constexpr double kTabExt = 200 * 1.001;    // table span of tabul/force_dipR
 constexpr int kMaxKZ = 16;                 // largest reference K input supported
+constexpr int kMaxNd = 8;                  // most dipole slots per particle
 
 // Per-replica parameters as the kernels see them.
 struct RepDev {

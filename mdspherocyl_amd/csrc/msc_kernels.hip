@@ -1444,10 +1444,11 @@ __global__ void __launch_bounds__(256) k_self_ener(State s, double *__restrict__
 }
 
 // --------------------------------------------------------------------------
-// Ewald reciprocal sum over the half space of k (one dipole slot per
-// particle).  k = 2 pi (nx/Lx, ny/Ly, nz/Lz); column c = (nx, ny) with
-// nx, ny in [-kmax, kmax], slot = c * KZ + nz, nz in [0, kmax]; slots outside
-// the half space (nz = 0 with ny < 0, or ny = 0 and nx <= 0) carry weight 0.
+// Ewald reciprocal sum over the half space of k, one dipole slot per
+// particle (several slots: k_recip_*_nd below).  k = 2 pi (nx/Lx, ny/Ly,
+// nz/Lz); column c = (nx, ny) with nx, ny in [-kmax, kmax], slot = c * KZ +
+// nz, nz in [0, kmax]; slots outside the half space (nz = 0 with ny < 0, or
+// ny = 0 and nx <= 0) carry weight 0.
 // --------------------------------------------------------------------------
 template <typename R>
 struct Cpx { R re, im; };
@@ -2286,6 +2287,251 @@ __global__ void __launch_bounds__(256 * kBpGroups) k_recip_force_mfma(State s, c
       s.F[a * N + g] += F[a];
       s.T[a * N + g] += t[a];
     }
+  }
+}
+
+// --------------------------------------------------------------------------
+// Reciprocal sum with several dipole slots per particle (FP64 in every
+// context; v25/functions.cpp:1780-1925).  With sites (i, w), c = mu u and
+// S_i(k) = sum_w (c_w.k) e^{-ik.r_w} the particle's own sum, the energy per
+// k is val/2 (|S|^2 - sum_i |S_i|^2): the interactions between the slots of
+// one particle are removed.  The self sums sum_i |S_i|^2 and
+// sum_i Re(conj(S_i) Q_i,a) (pressure tensor) are structure-factor-like sums
+// over particles, taken next to Q_a(k) = sum_sites c_a e^{-ik.r}; the back
+// projection rebuilds S_i per particle and k from the particle's own phases.
+// Tables: ph[((w * 3 + a) * kMaxKZ + n) * Ntot + g] = e^{-i n theta_a} of
+// slot w's wrapped position.
+// --------------------------------------------------------------------------
+constexpr int kNdVals = 10;  // partial values per slot and chunk: Q (6), self, uvslf (3)
+
+__global__ void __launch_bounds__(256) k_recip_tables_nd(State s, Cpx<double> *__restrict__ ph) {
+  const int g = s.g_begin + blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= s.g_end) return;
+  const RepDev &Rp = s.reps[s.rep_of[g]];
+  const int N = s.Ntot;
+  for (int w = 0; w < s.nd; ++w)
+    for (int a = 0; a < 3; ++a) {
+      double x = s.dpos[(3 * w + a) * N + g];
+      x -= Rp.L[a] * floor(x / Rp.L[a] + 0.5);
+      double sn, cs;
+      sincos(Rp.P2[a] * x, &sn, &cs);
+      double er = 1.0, ei = 0.0;
+      Cpx<double> *o = ph + (size_t)(w * 3 + a) * kMaxKZ * N + g;
+      for (int n = 0; n < Rp.KZ; ++n) {
+        o[(size_t)n * N] = {er, ei};
+        const double nr = er * cs + ei * sn;
+        ei = ei * cs - er * sn;
+        er = nr;
+      }
+    }
+}
+
+// e^{-i k.r} of slot w of particle g from its tables, k = (nx, ny, nz) with
+// nz >= 0 (e^{-i n theta} for n < 0 is the conjugate of n > 0)
+__device__ __forceinline__ Cpx<double> phase_nd(const Cpx<double> *__restrict__ ph, int N, int g, int w, int nx,
+                                                int ny, int nz) {
+  Cpx<double> ex = ph[((size_t)(w * 3 + 0) * kMaxKZ + (nx < 0 ? -nx : nx)) * N + g];
+  Cpx<double> ey = ph[((size_t)(w * 3 + 1) * kMaxKZ + (ny < 0 ? -ny : ny)) * N + g];
+  const Cpx<double> ez = ph[((size_t)(w * 3 + 2) * kMaxKZ + nz) * N + g];
+  if (nx < 0) ex.im = -ex.im;
+  if (ny < 0) ey.im = -ey.im;
+  const double xr = ex.re * ey.re - ex.im * ey.im, xi = ex.re * ey.im + ex.im * ey.re;
+  return {xr * ez.re - xi * ez.im, xr * ez.im + xi * ez.re};
+}
+
+// Partials of one chunk of particles: one block per chunk task and 256
+// slots, one thread per slot; the particles' table entries are the same
+// address for every lane (broadcast loads).
+__global__ void __launch_bounds__(256) k_recip_struct_nd(State s, int rt0, const Cpx<double> *__restrict__ ph,
+                                                         double *__restrict__ part) {
+  const int rtid = rt0 + blockIdx.x;
+  const Task rt = s.rtasks[rtid];
+  const RepDev &Rp = s.reps[rt.rep];
+  const int nslot = Rp.ncol * Rp.KZ;
+  const int sl = blockIdx.y * blockDim.x + threadIdx.x;
+  if (sl >= nslot) return;
+  const int N = s.Ntot, kmax = Rp.kmax, side = 2 * kmax + 1;
+  const int col = sl / Rp.KZ, nz = sl % Rp.KZ;
+  const int nx = col % side - kmax, ny = col / side - kmax;
+  const double k[3] = {Rp.P2[0] * nx, Rp.P2[1] * ny, Rp.P2[2] * nz};
+  double v[kNdVals];
+  for (int q = 0; q < kNdVals; ++q) v[q] = 0.0;
+  for (int p = 0; p < rt.n; ++p) {
+    const int g = rt.g0 + p;
+    double Sr = 0, Si = 0, Qr[3] = {0, 0, 0}, Qi[3] = {0, 0, 0};
+    for (int w = 0; w < s.nd; ++w) {
+      const double mu = s.dmu[w * N + g];
+      if (mu == 0.0) continue;  // as force_dipK skips silent slots
+      const Cpx<double> e = phase_nd(ph, N, g, w, nx, ny, nz);
+      const double c[3] = {mu * s.dori[(3 * w) * N + g], mu * s.dori[(3 * w + 1) * N + g],
+                           mu * s.dori[(3 * w + 2) * N + g]};
+      const double uk = c[0] * k[0] + c[1] * k[1] + c[2] * k[2];
+      Sr += uk * e.re;
+      Si += uk * e.im;
+      for (int a = 0; a < 3; ++a) {
+        Qr[a] += c[a] * e.re;
+        Qi[a] += c[a] * e.im;
+      }
+    }
+    for (int a = 0; a < 3; ++a) {
+      v[2 * a] += Qr[a];
+      v[2 * a + 1] += Qi[a];
+      v[7 + a] += Sr * Qr[a] + Si * Qi[a];  // Re(conj(S_i) Q_i,a)
+    }
+    v[6] += Sr * Sr + Si * Si;
+  }
+  double *o = part + Rp.part0 / 6 * kNdVals + ((long)(rtid - Rp.rtask0) * nslot + sl) * kNdVals;
+  for (int q = 0; q < kNdVals; ++q) o[q] = v[q];
+}
+
+// Chunk sums (in chunk order), slot energies and pressure tensors, the
+// amplitude A = val conj(S) and the per-block sums for the step's k_end, as
+// k_recip_reduce.  One wave of 64 slots per block, grid.y over replicas.
+__global__ void __launch_bounds__(64) k_recip_reduce_nd(State s, const double *__restrict__ part,
+                                                        Cpx<double> *__restrict__ Aout, double *__restrict__ slot_out,
+                                                        double *__restrict__ kblk, int kblk_stride) {
+  const int rep = s.rep_begin + blockIdx.y;
+  const RepDev &Rp = s.reps[rep];
+  const int nslot = Rp.ncol * Rp.KZ;
+  if (blockIdx.x * 64 >= nslot) return;  // whole block idle (uniform)
+  const int lane = threadIdx.x;
+  const int sl = blockIdx.x * 64 + lane;
+  const bool act = sl < nslot;
+  double v[kNdVals];
+  for (int q = 0; q < kNdVals; ++q) v[q] = 0.0;
+  if (act)
+    for (int c = 0; c < Rp.nrtask; ++c) {
+      const double *p = part + Rp.part0 / 6 * kNdVals + ((long)c * nslot + sl) * kNdVals;
+      for (int q = 0; q < kNdVals; ++q) v[q] += p[q];
+    }
+  const double val = act ? s.kval[Rp.slot0 + sl] : 0.0;
+  double o[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  Cpx<double> A = {0.0, 0.0};
+  if (val != 0.0) {
+    const int kmax = Rp.kmax, side = 2 * kmax + 1;
+    const int col = sl / Rp.KZ, nz = sl % Rp.KZ;
+    const int nx = col % side - kmax, ny = col / side - kmax;
+    const double k[3] = {Rp.P2[0] * nx, Rp.P2[1] * ny, Rp.P2[2] * nz};
+    const double Sr = k[0] * v[0] + k[1] * v[2] + k[2] * v[4];
+    const double Si = k[0] * v[1] + k[1] * v[3] + k[2] * v[5];
+    const double nrm = Sr * Sr + Si * Si - v[6];
+    o[0] = val * 0.5 * nrm;
+    const double Ksq = k[0] * k[0] + k[1] * k[1] + k[2] * k[2];
+    const double pval = 2 * (1. / (4. * Rp.alpha) + 1. / Ksq);
+    for (int j = 0; j < 3; ++j)
+      for (int c = 0; c < 3; ++c) {
+        const double re = Sr * v[2 * j] + Si * v[2 * j + 1];  // Re(conj(S) Q_j)
+        o[1 + 3 * j + c] = val * 0.5 * (((j == c) ? 1.0 : 0.0) - pval * k[j] * k[c]) * nrm +
+                           val * 0.5 * 2 * (re - v[7 + j]) * k[c];
+      }
+    A.re = val * Sr;
+    A.im = -val * Si;
+  }
+  if (act) {
+    double *os = slot_out + (size_t)(Rp.slot0 + sl) * 10;
+    for (int q = 0; q < 10; ++q) os[q] = o[q];
+    Aout[Rp.slot0 + sl] = A;
+  }
+  if (kblk) {
+    double *kb = kblk + ((size_t)rep * kblk_stride + blockIdx.x) * 10;
+    for (int q = 0; q < 10; ++q) {
+      const double t = wave_sum(o[q]);
+      if (lane == 0) kb[q] = t;
+    }
+  }
+}
+
+// Back projection, one block per wave task, wave cg taking the columns
+// c = cg (mod kRfGroups).  Per particle and k: the slots' phases e_w, the
+// particle's own sum S_i, and with B = A - val conj(S_i) (the sum over the
+// other particles) per slot Z = e_w B, factF = -(c_w.k) Im Z and
+// factT = -Re Z: F_w = sum_k factF k, H_w = sum_k factT k, and the
+// particle's torque sum_w c_w x H_w + d_w x F_w with the lever
+// d_w = dpc_w u (force_dipK's icrok and dicrok terms).
+__global__ void __launch_bounds__(64 * kRfGroups) k_recip_force_nd(State s, const Cpx<double> *__restrict__ ph,
+                                                                   const Cpx<double> *__restrict__ Aall) {
+  __shared__ double red[kRfGroups][6][kWave];
+  const int lane = threadIdx.x & 63, cg = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  if ((int)blockIdx.x >= s.ntask) return;
+  const Task tk = s.tasks[s.task_begin + blockIdx.x];
+  const RepDev &Rp = s.reps[tk.rep];
+  const int N = s.Ntot, nd = s.nd;
+  const int kmax = Rp.kmax, side = 2 * kmax + 1, KZ = Rp.KZ;
+  const int g = tk.g0 + (lane < tk.n ? lane : 0);
+  double c[kMaxNd][3], Fw[kMaxNd][3], Hw[kMaxNd][3];
+#pragma unroll
+  for (int w = 0; w < kMaxNd; ++w)
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      c[w][a] = w < nd ? s.dmu[w * N + g] * s.dori[(3 * w + a) * N + g] : 0.0;
+      Fw[w][a] = Hw[w][a] = 0.0;
+    }
+  const Cpx<double> *A = Aall + Rp.slot0;
+  const double *kv = s.kval + Rp.slot0;
+  for (int col = cg; col < Rp.ncol; col += kRfGroups) {
+    const int nx = col % side - kmax, ny = col / side - kmax;
+    for (int nz = 0; nz < KZ; ++nz) {
+      const int sl = col * KZ + nz;
+      const double val = kv[sl];
+      if (val == 0.0) continue;  // outside the half space (uniform)
+      const Cpx<double> Ak = A[sl];
+      const double k[3] = {Rp.P2[0] * nx, Rp.P2[1] * ny, Rp.P2[2] * nz};
+      Cpx<double> e[kMaxNd];
+      double uk[kMaxNd];
+      double Sr = 0, Si = 0;
+#pragma unroll
+      for (int w = 0; w < kMaxNd; ++w) {
+        uk[w] = c[w][0] * k[0] + c[w][1] * k[1] + c[w][2] * k[2];
+        e[w] = {0.0, 0.0};
+        if (w < nd && s.dmu[w * N + g] != 0.0) {
+          e[w] = phase_nd(ph, N, g, w, nx, ny, nz);
+          Sr += uk[w] * e[w].re;
+          Si += uk[w] * e[w].im;
+        }
+      }
+      // B = A - val conj(S_i)
+      const double Br = Ak.re - val * Sr, Bi = Ak.im + val * Si;
+#pragma unroll
+      for (int w = 0; w < kMaxNd; ++w) {
+        if (w >= nd) break;
+        const double Zr = e[w].re * Br - e[w].im * Bi, Zi = e[w].re * Bi + e[w].im * Br;
+        const double fF = -uk[w] * Zi, fT = -Zr;
+        for (int a = 0; a < 3; ++a) {
+          Fw[w][a] += fF * k[a];
+          Hw[w][a] += fT * k[a];
+        }
+      }
+    }
+  }
+  double F[3] = {0, 0, 0}, T[3] = {0, 0, 0};
+  const double u[3] = {s.ori[6 * N + g], s.ori[7 * N + g], s.ori[8 * N + g]};
+#pragma unroll
+  for (int w = 0; w < kMaxNd; ++w) {
+    if (w >= nd) break;
+    const double sd = s.dpc[w * N + g];
+    const double d[3] = {sd * u[0], sd * u[1], sd * u[2]};
+    double t1[3], t2[3];
+    cross3(c[w], Hw[w], t1);
+    cross3(d, Fw[w], t2);
+    for (int a = 0; a < 3; ++a) {
+      F[a] += Fw[w][a];
+      T[a] += t1[a] + t2[a];
+    }
+  }
+  for (int a = 0; a < 3; ++a) {
+    red[cg][a][lane] = F[a];
+    red[cg][3 + a][lane] = T[a];
+  }
+  __syncthreads();
+  if (cg != 0 || lane >= tk.n) return;
+  for (int a = 0; a < 3; ++a) {
+    double f = 0, t = 0;
+    for (int q = 0; q < kRfGroups; ++q) {
+      f += red[q][a][lane];
+      t += red[q][3 + a][lane];
+    }
+    s.F[a * N + g] += f;
+    s.T[a * N + g] += t;
   }
 }
 
@@ -3269,6 +3515,16 @@ void launch_recip_reduce(const State &s, bool mixed, const void *part, const dou
   else
     MSC_LAUNCH(k_recip_reduce<double>, grid, dim3(512), 0, st, s, (const double *)part, M2, m2rows,
                (Cpx<double> *)A, slot_out, kblk, nblk(maxslot, 64), (_Float16 *)nullptr, (const int *)nullptr);
+}
+
+void launch_recip_nd(const State &s, int rt0, int nrt, int maxslot, void *ph, double *part, void *A,
+                     double *slot_out, double *kblk, hipStream_t st) {
+  Cpx<double> *t = (Cpx<double> *)ph;
+  MSC_LAUNCH(k_recip_tables_nd, dim3(nblk(s.g_end - s.g_begin, 256)), dim3(256), 0, st, s, t);
+  MSC_LAUNCH(k_recip_struct_nd, dim3(nrt, nblk(maxslot, 256)), dim3(256), 0, st, s, rt0, t, part);
+  MSC_LAUNCH(k_recip_reduce_nd, dim3(nblk(maxslot, 64), s.rep_end - s.rep_begin), dim3(64), 0, st, s, part,
+             (Cpx<double> *)A, slot_out, kblk, nblk(maxslot, 64));
+  MSC_LAUNCH(k_recip_force_nd, dim3(s.ntask), dim3(64 * kRfGroups), 0, st, s, t, (const Cpx<double> *)A);
 }
 
 void launch_moveP(const State &s, bool last, const Thermo *th, double *task_out, hipStream_t st) {

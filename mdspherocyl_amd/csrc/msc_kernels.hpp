@@ -115,6 +115,14 @@ void launch_recip_reduce(const State &s, bool mixed, const void *part, const dou
 constexpr int kBopPerTile = 12 * 64;  // 3 weights x 4 pieces x 64 lanes
 int launch_recip_force(const State &s, bool mixed, int KZ, const void *ph, const void *amp, const void *A,
                        const void *bop, const int *bexp, hipStream_t st);
+// Reciprocal sum of contexts with several dipole slots per particle, FP64:
+// slot phase tables ph (nd * 3 * kMaxKZ * Ntot complex doubles), chunk
+// partials of the chunk tasks rt0 .. rt0 + nrt - 1 (10 doubles per slot and
+// chunk, at RepDev::part0 / 6 * 10), slot energies and A (complex doubles per
+// slot), kblk as in launch_recip_reduce, and the forces and torques added to
+// F, T of the selected tasks.
+void launch_recip_nd(const State &s, int rt0, int nrt, int maxslot, void *ph, double *part, void *A,
+                     double *slot_out, double *kblk, hipStream_t st);
 void launch_moveP(const State &s, bool last, const Thermo *th, double *task_out, hipStream_t st);
 void launch_moveX(const State &s, hipStream_t st);
 void launch_renorm(const State &s, hipStream_t st);

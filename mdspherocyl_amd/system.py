@@ -132,6 +132,16 @@ class MdSystem:
                                             tot.value, ctypes.byref(tot)))
         return off, nbr[:tot.value]
 
+    def set_nblist(self, rep, r_list, off, nbr):
+        """Make the caller's list (CSR of ascending g > f, the reference's
+        nblist) the replica's neighbour rows."""
+        off = np.ascontiguousarray(off, dtype=np.int32)
+        nbr = np.ascontiguousarray(nbr, dtype=np.int32)
+        if nbr.size == 0:
+            nbr = np.zeros(1, dtype=np.int32)
+        self._chk(self.L.msc_set_neigh_list(self._ctx, rep, float(r_list), off.ctypes.data_as(_ip),
+                                            nbr.ctypes.data_as(_ip)))
+
     def check_list(self, rep, r_list_min, r_list_max):
         out = ctypes.c_int()
         self._chk(self.L.msc_check_list(self._ctx, rep, r_list_min, r_list_max, ctypes.byref(out)))

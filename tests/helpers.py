@@ -26,6 +26,45 @@ def v251_state():
     return S, P
 
 
+def v25_state():
+    """The system of v25/input.txt (two species of 50 rods, two off-centre
+    tilted dipoles each, K = 10, alpha = 0.75) as an oracle system before its
+    Signal=0 initialisation, its run parameters and the parsed input."""
+    import os
+    from mdspherocyl_amd import refio
+    gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    inp = refio.read_input(os.path.join(gold, "v25_input.txt"))
+    N = sum(inp["n_part"])
+    Lb = math.pow(N / (inp["ArLx"] * inp["ArLy"] * inp["dens"]), 1.0 / 3)
+    nd = len(inp["dip"][0])
+    S = O.OSys(N, nd, L=(inp["ArLx"] * Lb, inp["ArLy"] * Lb, Lb), counts=inp["n_part"])
+    b = 0
+    for n, NN in enumerate(inp["n_part"]):
+        S.len[b:b + NN] = inp["length"][n]
+        S.dia[b:b + NN] = inp["diameter"][n]
+        S.mass[b:b + NN] = inp["mass"][n]
+        for w, (pos, ori, mu) in enumerate(inp["dip"][n]):
+            S.dpc[b:b + NN, w] = pos
+            S.doc[b:b + NN, w] = ori
+            S.dmu[b:b + NN, w] = mu
+        b += NN
+    S.set_inertia()
+    P = O.run_params(dens=inp["dens"], temp=inp["temp"], dt=inp["Dt"], sr_cut=inp["sr_cut"], alpha=inp["alpha"],
+                     min_s=inp["min_s"], Q=inp["Q"], gamma=inp["gamma"], s0=0.0, B=tuple(inp["B"]),
+                     K=inp["K_max"], cl_fr=inp["cl_fr"])
+    return S, P, inp
+
+
+def v25_desc(S, inp, **over):
+    """Library descriptor of the v25/input.txt run (real space over every
+    minimum-image pair, as the reference sums)."""
+    from mdspherocyl_amd import replica_desc
+    kw = dict(rc=0.0, nd=S.nd, B=tuple(inp["B"]), temp=inp["temp"], dt=inp["Dt"], Q=inp["Q"], gamma=inp["gamma"],
+              s0=0.0, dens=inp["dens"], thermo_seed=(1, 2))
+    kw.update(over)
+    return replica_desc(S.N, tuple(S.L), inp["alpha"], inp["K_max"], inp["sr_cut"], **kw)
+
+
 def v251_desc(**over):
     from mdspherocyl_amd import replica_desc
     kw = dict(rc=0.0, B=(0.0, 0.0, 5.0), temp=1.0, dt=1e-4, Q=5.0, gamma=1.0, s0=1.0, dens=0.0013,

@@ -23,11 +23,42 @@
 #define __forceinline__ inline
 #define __launch_bounds__(...)
 #define __shared__ static
+#define __align__(n) __attribute__((aligned(n)))
+#define __builtin_amdgcn_rsqf(x) (1.0f / std::sqrt((float)(x)))
 #define __builtin_amdgcn_readfirstlane(x) (x)
+
+struct double2 { double x, y; };
+struct float2 { float x, y; };
+struct float4 { float x, y, z, w; };
+struct int2 { int x, y; };
+struct int4 { int x, y, z, w; };
+inline double __longlong_as_double(long long v) {
+  double d;
+  std::memcpy(&d, &v, sizeof d);
+  return d;
+}
+inline long long __double_as_longlong(double d) {
+  long long v;
+  std::memcpy(&v, &d, sizeof v);
+  return v;
+}
+inline float __int_as_float(int v) {
+  float f;
+  std::memcpy(&f, &v, sizeof f);
+  return f;
+}
+inline int __float_as_int(float f) {
+  int v;
+  std::memcpy(&v, &f, sizeof v);
+  return v;
+}
 
 using std::floor;
 using std::sqrt;
 using std::fabs;
+using std::isinf;
+using std::isnan;
+using std::isfinite;
 using std::fmax;
 using std::fma;
 using std::exp;
@@ -153,6 +184,12 @@ inline int atomicMax(int *p, int v) {
   }
   return old;
 }
+inline unsigned long long atomicMax(unsigned long long *p, unsigned long long v) {
+  unsigned long long old = __atomic_load_n(p, __ATOMIC_SEQ_CST);
+  while (old < v && !__atomic_compare_exchange_n(p, &old, v, false, __ATOMIC_SEQ_CST, __ATOMIC_SEQ_CST)) {
+  }
+  return old;
+}
 inline int atomicCAS(int *p, int cmp, int val) {
   __atomic_compare_exchange_n(p, &cmp, val, false, __ATOMIC_SEQ_CST, __ATOMIC_SEQ_CST);
   return cmp;
@@ -226,6 +263,13 @@ inline hipError_t hipEventDestroy(hipEvent_t) { return hipSuccess; }
 inline hipError_t hipEventRecord(hipEvent_t, hipStream_t) { return hipSuccess; }
 inline hipError_t hipEventElapsedTime(float *ms, hipEvent_t, hipEvent_t) { *ms = 0; return hipSuccess; }
 inline hipError_t hipGetLastError() { return hipSuccess; }
+inline hipError_t hipGetDevice(int *d) { *d = 0; return hipSuccess; }
+// a device symbol is host memory here: its address is the symbol's own
+#define HIP_SYMBOL(s) (&(s))
+inline hipError_t hipGetSymbolAddress(void **p, const void *sym) {
+  *p = const_cast<void *>(sym);
+  return hipSuccess;
+}
 inline hipError_t hipDeviceSynchronize() { return hipSuccess; }
 
 namespace hipemu {

@@ -27,7 +27,8 @@ def build():
     src = os.path.join(ROOT, "mdspherocyl_amd", "csrc")
     cmd = [CLANG, "-std=c++20", "-O1", "-g", "-fsanitize=address", "-shared-libasan", "-fno-omit-frame-pointer",
            "-fPIC", "-shared", "-I", HERE, "-x", "c++", os.path.join(src, "msc_kernels.hip"),
-           os.path.join(src, "msc_api.hip"), os.path.join(src, "msc_conf.cpp"), "-o", OUT, "-lpthread"]
+           os.path.join(src, "msc_api.hip"), os.path.join(src, "msc_conf.cpp"), os.path.join(src, "msc_place.cpp"),
+           "-o", OUT, "-lpthread"]
     subprocess.run(cmd, check=True)
 
 
