@@ -174,21 +174,31 @@ typedef struct msc_tuning {
                        rows and the real-space dipole pairs in one launch,
                        partners read as packed records */
   int pair_waves;   /* waves per task of that launch: 2, 4 or 8 */
+  double list_skin; /* MSC_MIXED: > 0 gives the context its own rows, every
+                       pair within 2^(1/6) D_max + list_skin (WCA and
+                       real-space pairs within r_c + list_skin), rebuilt
+                       together when twice the largest segment-point
+                       displacement exceeds list_skin; 0 keeps the
+                       reference's rows and rebuild test (v25/main.cpp:
+                       461-463, 679-701) with contact rows refreshed from
+                       them.  MSC_FP64: always 0.  A change takes effect at
+                       the next msc_init_run. */
 } msc_tuning;
 int msc_get_tuning(msc_ctx *ctx, msc_tuning *t);
 int msc_set_tuning(msc_ctx *ctx, const msc_tuning *t);
 /* The autotuner: times `steps` steps of msc_run under each candidate tuning
  * (graph off/on x, for contexts that can fuse, unfused or fused with 2, 4 or
- * 8 waves per task), each from the state at the call, and keeps the
- * fastest.  The state, thermostats and thermostat generators are restored
+ * 8 waves per task; then, for MSC_MIXED contexts, list skins 0.5, 0.75, 1.0
+ * and 1.5 with the fastest of those), each from the state at the call, and
+ * keeps the fastest.  The state, thermostats and thermostat generators are restored
  * after every trial and the lists rebuilt from the restored state (as after a
  * chunk rollback of msc_run), so a run continues from the same state; only
  * the summation order of the rebuilt lists can differ from an uninterrupted
- * run.  *best (may be NULL) receives the choice; ms (may be NULL, room for
- * nms values) the milliseconds per step of each candidate, in the order
- * tried; *ntried (may be NULL) their number.  No counterpart in the
- * reference. */
-int msc_autotune(msc_ctx *ctx, int steps, msc_tuning *best, double *ms, int nms, int *ntried);
+ * run.  *best (may be NULL) receives the choice; tried and ms (each may be
+ * NULL, room for ntry values) the candidates and their milliseconds per
+ * step, in the order tried; *ntried (may be NULL) their number.  No
+ * counterpart in the reference. */
+int msc_autotune(msc_ctx *ctx, int steps, msc_tuning *best, msc_tuning *tried, double *ms, int ntry, int *ntried);
 /* msc_run advances in chunks of 50 steps.  A chunk during which a list
  * rebuild outgrew its device region is rolled back, the regions are enlarged
  * and the lists rebuilt from the chunk's first state, and the chunk is run

@@ -35,10 +35,12 @@ _ip = ctypes.POINTER(ctypes.c_int)
 
 class Tuning(ctypes.Structure):
     """msc_tuning: launch choices of a context."""
-    _fields_ = [("graph", ctypes.c_int), ("fused_pairs", ctypes.c_int), ("pair_waves", ctypes.c_int)]
+    _fields_ = [("graph", ctypes.c_int), ("fused_pairs", ctypes.c_int), ("pair_waves", ctypes.c_int),
+                ("list_skin", ctypes.c_double)]
 
     def as_dict(self):
-        return {"graph": self.graph, "fused_pairs": self.fused_pairs, "pair_waves": self.pair_waves}
+        return {"graph": self.graph, "fused_pairs": self.fused_pairs, "pair_waves": self.pair_waves,
+                "list_skin": self.list_skin}
 
 
 class ReplicaDesc(ctypes.Structure):
@@ -99,7 +101,8 @@ def load():
         "msc_set_graph": ([vp, ctypes.c_int], ctypes.c_int),
         "msc_get_tuning": ([vp, ctypes.POINTER(Tuning)], ctypes.c_int),
         "msc_set_tuning": ([vp, ctypes.POINTER(Tuning)], ctypes.c_int),
-        "msc_autotune": ([vp, ctypes.c_int, ctypes.POINTER(Tuning), _dp, ctypes.c_int, _ip], ctypes.c_int),
+        "msc_autotune": ([vp, ctypes.c_int, ctypes.POINTER(Tuning), ctypes.POINTER(Tuning), _dp, ctypes.c_int, _ip],
+                         ctypes.c_int),
         "msc_cluster_labels": ([vp, ctypes.c_int, ctypes.c_double, ctypes.POINTER(ctypes.c_uint),
                                 _ip, _ip, _ip, ctypes.POINTER(ctypes.c_long)], ctypes.c_int),
         "msc_cluster_labels_dd": ([vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.POINTER(ctypes.c_uint),

@@ -76,6 +76,10 @@ struct TimerSet {
 
 }  // namespace
 
+// Default skin of a MSC_MIXED context's own rows (length units of the
+// state; the autotuner tries others).
+static constexpr double kDefaultListSkin = 1.0;
+
 struct msc_ctx {
   int device = 0, precision = MSC_FP64;
   bool mixed = false;
@@ -94,6 +98,11 @@ struct msc_ctx {
   // launch choices (msc_tuning): the fused mixed pair launch and its waves per task
   bool fused_pairs = true;
   int pair_waves = 4;
+  // MSC_MIXED: the skin of the context's own rows (0: the reference's rows
+  // and rebuild test, contact rows refreshed from them); per replica the
+  // smallest diameter and largest length + diameter the radii derive from
+  double list_skin = 0.0;
+  std::vector<double> sh_dmin, sh_dmax;
   long nl_alloc = 0, sl_alloc = 0, dl_alloc = 0;  // entries allocated for the row arrays
   size_t part_elems = 0;
   int nslot_tot = 0, maxslot = 0, maxcol = 0;
@@ -358,6 +367,40 @@ State sel(const msc_ctx *ctx, int rep) {
 
 bool rep_ok(msc_ctx *ctx, int rep) { return rep == MSC_ALL || (rep >= 0 && rep < ctx->nrep); }
 
+// Contact-row radius never beyond the rows it is taken from.
+void clamp_short(RepDev &R) { R.r_short = std::min(R.r_wca + R.skin_short, R.r_list_max); }
+
+// List radii of a replica.  The reference's (v25/main.cpp:461-463):
+// r_list_min = max(sr_cut + 0.1 D_min, 1.5 (L + D)_max), r_list_max =
+// r_list_min + 0.9 D_min, rebuilt when 2 sqrt(3) max|dx| exceeds their
+// difference; contact rows within the WCA range 2^(1/6) D_max plus half the
+// smallest diameter; real-space rows within r_c plus the list skin.  With a
+// list skin (MSC_MIXED) the rows are the context's own: every pair within
+// r_wca + skin, the real-space rows within r_c + skin, all rebuilt together
+// when twice the largest segment-point displacement exceeds the skin.
+void set_radii(msc_ctx *ctx, int rp) {
+  RepDev &R = ctx->reps[rp];
+  const msc_replica_desc &D = ctx->desc[rp];
+  const double dmin = ctx->sh_dmin[rp], dmax = ctx->sh_dmax[rp];
+  R.r_wca = std::pow(2.0, 1.0 / 6.0) * R.diamax;
+  if (ctx->list_skin > 0) {
+    const double skin = ctx->list_skin;
+    R.own_lists = 1;
+    R.r_list_min = R.r_wca;
+    R.r_list_max = R.r_wca + skin;
+    R.skin_short = skin;
+    R.rl_dip = D.rc > 0 ? D.rc + skin : 0;
+  } else {
+    R.own_lists = 0;
+    R.r_list_min = D.sr_cut + 0.1 * dmin;
+    if (R.r_list_min < 1.5 * dmax) R.r_list_min = 1.5 * dmax;
+    R.r_list_max = R.r_list_min + 0.9 * dmin;
+    R.rl_dip = D.rc > 0 ? D.rc + (R.r_list_max - R.r_list_min) : 0;
+    R.skin_short = 0.5 * dmin;
+  }
+  clamp_short(R);
+}
+
 // derived per-replica quantities that depend on the state (shapes, dipoles)
 void derive(msc_ctx *ctx, int rp, const double *len, const double *dia, const double *dpc) {
   RepDev &R = ctx->reps[rp];
@@ -374,16 +417,10 @@ void derive(msc_ctx *ctx, int rp, const double *len, const double *dia, const do
   if (D.dmin > 0) dmin = D.dmin;
   R.lmax = lmax;
   R.diamax = diamax;
-  // list radii, v25/main.cpp:461-463
-  R.r_list_min = D.sr_cut + 0.1 * dmin;
-  if (R.r_list_min < 1.5 * dmax) R.r_list_min = 1.5 * dmax;
-  R.r_list_max = R.r_list_min + 0.9 * dmin;
-  R.rl_dip = D.rc > 0 ? D.rc + (R.r_list_max - R.r_list_min) : 0;
-  // contact rows: every pair inside the WCA range 2^(1/6) sigma_ij plus a
-  // skin of half the smallest diameter (never beyond the reference list)
-  R.skin_short = 0.5 * dmin;
-  R.r_short = std::min(std::pow(2.0, 1.0 / 6.0) * diamax + R.skin_short, R.r_list_max);
+  ctx->sh_dmin[rp] = dmin;
+  ctx->sh_dmax[rp] = dmax;
   ctx->S.lever_max = std::max(ctx->S.lever_max, lev);
+  set_radii(ctx, rp);
 }
 
 // cell grid of a replica for the current list radii
@@ -820,6 +857,7 @@ int msc_create(int device, int precision, int nrep, const msc_replica_desc *reps
   ctx->precision = precision;
   ctx->mixed = precision == MSC_MIXED;
   ctx->contact = ctx->mixed;
+  ctx->list_skin = ctx->mixed ? kDefaultListSkin : 0.0;
   ctx->nrep = nrep;
   ctx->desc.assign(reps, reps + nrep);
   int nd = reps[0].nd;
@@ -852,6 +890,8 @@ int msc_create(int device, int precision, int nrep, const msc_replica_desc *reps
   int n0 = 0, slot0 = 0, bexp_cols = 0;
   long part0 = 0;
   ctx->reps.resize(nrep);
+  ctx->sh_dmin.assign(nrep, 0.0);
+  ctx->sh_dmax.assign(nrep, 0.0);
   for (int r = 0; r < nrep; ++r) {
     const msc_replica_desc &D = reps[r];
     RepDev &R = ctx->reps[r];
@@ -1276,7 +1316,10 @@ int msc_neigh_list(msc_ctx *ctx, int rep, double r_list) {
   // the rows are valid again only once this build has fitted its regions
   ctx->have_lists = false;
   for (int r = 0; r < ctx->nrep; ++r)
-    if (rep == MSC_ALL || r == rep) ctx->reps[r].r_list_max = r_list;
+    if (rep == MSC_ALL || r == rep) {
+      ctx->reps[r].r_list_max = r_list;
+      clamp_short(ctx->reps[r]);
+    }
   int rc = lists_layout(ctx);
   if (rc) return rc;
   launch_fill_int(ctx->d_flags, 4 * ctx->nrep, 0, ctx->stream);
@@ -1319,6 +1362,7 @@ int msc_set_neigh_list(msc_ctx *ctx, int rep, double r_list, const int *off, con
   for (int c : cnt) longest = std::max<long>(longest, c);
   ctx->have_lists = false;
   R0.r_list_max = r_list;
+  clamp_short(R0);
   std::vector<long> need(ctx->nrep, 0);
   need[rep] = longest;
   int rc = lists_layout(ctx, need);
@@ -1386,6 +1430,7 @@ int msc_check_list(msc_ctx *ctx, int rep, double r_list_min, double r_list_max, 
   if (!ctx->have_lists) return fail(ctx, MSC_E_STATE, "no neighbour list");
   ctx->reps[rep].r_list_min = r_list_min;
   ctx->reps[rep].r_list_max = r_list_max;
+  clamp_short(ctx->reps[rep]);
   int rc;
   if ((rc = upload_reps(ctx))) return rc;
   State s = sel(ctx, rep);
@@ -1716,16 +1761,28 @@ int msc_get_tuning(msc_ctx *ctx, msc_tuning *t) {
   t->graph = ctx->graph_on ? 1 : 0;
   t->fused_pairs = ctx->fused_pairs ? 1 : 0;
   t->pair_waves = ctx->pair_waves;
+  t->list_skin = ctx->list_skin;
   return MSC_OK;
 }
 
 int msc_set_tuning(msc_ctx *ctx, const msc_tuning *t) {
   SCOPE(ctx);
   if (!ctx || !t || (t->pair_waves != 2 && t->pair_waves != 4 && t->pair_waves != 8)) return MSC_E_ARG;
+  if (!(t->list_skin >= 0) || !std::isfinite(t->list_skin) || (!ctx->mixed && t->list_skin != 0))
+    return fail(ctx, MSC_E_ARG, "list_skin must be >= 0, and 0 for MSC_FP64 contexts");
   drop_graph(ctx);  // the captured step holds the previous launch shapes
   ctx->graph_on = t->graph != 0;
   ctx->fused_pairs = t->fused_pairs != 0;
   ctx->pair_waves = t->pair_waves;
+  if (t->list_skin != ctx->list_skin) {
+    // other rows: msc_init_run (or the autotuner) builds them again
+    ctx->list_skin = t->list_skin;
+    if (ctx->have_state) {
+      for (int r = 0; r < ctx->nrep; ++r) set_radii(ctx, r);
+      ctx->have_lists = false;
+      return upload_reps(ctx);
+    }
+  }
   return MSC_OK;
 }
 
@@ -1962,33 +2019,33 @@ int msc_run(msc_ctx *ctx, int step0, int nsteps, double *obs) {
 // runs `steps` steps from the state at the call, after an untimed run of a
 // few steps from the same state (graph capture, first-touch costs); the state
 // is restored and the lists rebuilt after every run.
-int msc_autotune(msc_ctx *ctx, int steps, msc_tuning *best, double *ms, int nms, int *ntried) {
+int msc_autotune(msc_ctx *ctx, int steps, msc_tuning *best, msc_tuning *tried, double *ms, int ntry,
+                 int *ntried) {
   SCOPE(ctx);
-  if (!ctx || steps < 1 || (ms && nms < 0)) return MSC_E_ARG;
+  if (!ctx || steps < 1 || ((tried || ms) && ntry < 0)) return MSC_E_ARG;
   if (!ctx->have_lists) return fail(ctx, MSC_E_STATE, "call msc_init_run first");
   bool cut_all = true;
   for (const RepDev &R : ctx->reps) cut_all &= R.rc > 0;
   const bool fusable = ctx->mixed && ctx->contact && cut_all;
-  std::vector<msc_tuning> cand;
-  for (int g = 0; g < 2; ++g) {
-    cand.push_back(msc_tuning{g, 0, 4});
-    if (fusable)
-      for (int w : {2, 4, 8}) cand.push_back(msc_tuning{g, 1, w});
-  }
+  msc_tuning start;
+  msc_get_tuning(ctx, &start);
   const std::vector<MT> rng = ctx->thermo_rng;
   if (int rc_ = state_copy(ctx, false, ctx->d_tune_save, ctx->tune_save_cap)) return rc_;
+  // the state at the call, with rows built for the present radii
   auto restore = [&]() -> int {
     if (int rc_ = state_copy(ctx, true, ctx->d_tune_save, ctx->tune_save_cap)) return rc_;
     ctx->thermo_rng = rng;
+    if (int rc_ = lists_layout(ctx)) return rc_;
     launch_fill_int(ctx->d_flags, 4 * ctx->nrep, 0, ctx->stream);
     if (int rc_ = lists_build(ctx, ctx->d_one)) return rc_;
     ctx->have_lists = true;
     return MSC_OK;
   };
-  int best_i = -1;
-  double best_ms = 0;
-  for (size_t i = 0; i < cand.size(); ++i) {
-    if (int rc_ = msc_set_tuning(ctx, &cand[i])) return rc_;
+  int n = 0;
+  msc_tuning win = start;
+  double win_ms = 0;
+  auto trial = [&](const msc_tuning &c) -> int {
+    if (int rc_ = msc_set_tuning(ctx, &c)) return rc_;
     if (int rc_ = restore()) return rc_;
     if (int rc_ = msc_run(ctx, 0, std::min(steps, 10), nullptr)) return rc_;
     if (int rc_ = restore()) return rc_;
@@ -1996,17 +2053,44 @@ int msc_autotune(msc_ctx *ctx, int steps, msc_tuning *best, double *ms, int nms,
     const auto t0 = std::chrono::steady_clock::now();
     if (int rc_ = msc_run(ctx, 0, steps, nullptr)) return rc_;  // returns after its stream drained
     const double t = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / steps;
-    if (ms && (int)i < nms) ms[i] = t;
-    if (best_i < 0 || t < best_ms) {
-      best_i = (int)i;
-      best_ms = t;
+    if (tried && n < ntry) tried[n] = c;
+    if (ms && n < ntry) ms[n] = t;
+    if (n == 0 || t < win_ms) {
+      win = c;
+      win_ms = t;
+    }
+    ++n;
+    return MSC_OK;
+  };
+  // launch choices at the present list skin, then (MSC_MIXED) list skins
+  // with the fastest launch choice
+  for (int g = 0; g < 2; ++g) {
+    msc_tuning c = start;
+    c.graph = g;
+    c.fused_pairs = 0;
+    c.pair_waves = 4;
+    if (int rc_ = trial(c)) return rc_;
+    if (fusable)
+      for (int w : {2, 4, 8}) {
+        c.fused_pairs = 1;
+        c.pair_waves = w;
+        if (int rc_ = trial(c)) return rc_;
+      }
+  }
+  if (ctx->mixed) {
+    const msc_tuning launch = win;
+    for (double skin : {0.5, 0.75, 1.0, 1.5}) {
+      if (skin == start.list_skin) continue;
+      msc_tuning c = launch;
+      c.list_skin = skin;
+      if (int rc_ = trial(c)) return rc_;
     }
   }
-  if (int rc_ = msc_set_tuning(ctx, &cand[best_i])) return rc_;
+  if (int rc_ = msc_set_tuning(ctx, &win)) return rc_;
   if (int rc_ = restore()) return rc_;
   FINISH("msc_autotune");
-  if (best) *best = cand[best_i];
-  if (ntried) *ntried = (int)cand.size();
+  if (best) *best = win;
+  if (ntried) *ntried = n;
   return MSC_OK;
 }
 
@@ -2350,18 +2434,26 @@ int msc_cluster_labels_dd(msc_ctx *ctx, int rep, double min_s, double u_cut, uns
   const double reach_max = R.diamax + min_s;
   if (!(reach_max < R.r_list_max))
     return fail(ctx, MSC_E_ARG, "min_s + largest diameter must stay below the list radius r_list_max");
-  const State s = sel(ctx, rep);
-  {
-    Bracket b(ctx, "clusters");
-    launch_clusters(s, min_s, reach_max, u_cut, ctx->d_clu, ctx->d_clu_flag, ctx->stream);
-  }
-  FINISH("msc_cluster_labels");
   std::vector<int> root(R.N);
   int fl[2];
-  if (int rc_ = d2h(ctx, root.data(), ctx->d_clu + R.n0, sizeof(int) * R.N)) return rc_;
-  if (int rc_ = d2h(ctx, fl, ctx->d_clu_flag, sizeof fl)) return rc_;
-  if (fl[0])
-    return fail(ctx, MSC_E_STATE, "particles moved too far since the neighbour list was built to find every contact");
+  for (int attempt = 0;; ++attempt) {
+    const State s = sel(ctx, rep);
+    {
+      Bracket b(ctx, "clusters");
+      launch_clusters(s, min_s, reach_max, u_cut, ctx->d_clu, ctx->d_clu_flag, ctx->stream);
+    }
+    FINISH("msc_cluster_labels");
+    if (int rc_ = d2h(ctx, root.data(), ctx->d_clu + R.n0, sizeof(int) * R.N)) return rc_;
+    if (int rc_ = d2h(ctx, fl, ctx->d_clu_flag, sizeof fl)) return rc_;
+    if (!fl[0]) break;
+    if (!R.own_lists || attempt > 0)
+      return fail(ctx, MSC_E_STATE, "particles moved too far since the neighbour list was built to find every contact");
+    // the context's own rows are short: built again from the present state
+    // (a run continues from them as after any rebuild)
+    if (int rc_ = lists_layout(ctx)) return rc_;
+    launch_fill_int(ctx->d_flags, 4 * ctx->nrep, 0, ctx->stream);
+    if (int rc_ = lists_build(ctx, ctx->d_one)) return rc_;
+  }
   for (int &r : root) r -= R.n0;
   Mt19937 mt;
   std::copy(mt_state, mt_state + Mt19937::kN, mt.mt);

@@ -33,7 +33,11 @@ struct RepDev {
   // row capacity.  nl: reference list (segment distance < r_list_max); sl:
   // contact list (segment distance < r_short); dl: real-space dipole pairs.
   int nl_w, sl_w, dl_w;
-  int pad_w;
+  // 1: the rows are the context's own (MSC_MIXED with a list skin): every
+  // pair within r_short = r_wca + skin, rebuilt when twice the largest
+  // segment-point displacement exceeds r_short - r_wca; 0: the reference's
+  // rows at r_list_max and its rebuild test, contact rows refreshed from them
+  int own_lists;
   long nl_base, sl_base, dl_base;
   long part0;           // first element of this replica's structure-factor partials
   long bop0;            // first element (4 fp16 values) of this replica's back-projection operands
@@ -47,6 +51,7 @@ struct RepDev {
   double r_list_min, r_list_max;
   double rl_dip;        // real-space pair-list radius (rc + skin)
   double r_short;       // contact-list radius: 2^(1/6) max diameter + skin_short
+  double r_wca;         // WCA range of the replica: 2^(1/6) max diameter
   double skin_short;
   double lmax;          // largest cylinder length
   double diamax;        // largest diameter

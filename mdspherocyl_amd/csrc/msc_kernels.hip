@@ -269,6 +269,12 @@ __device__ __forceinline__ bool list_decision(const State &s, const RepDev &R, i
   const unsigned long long *d = ls.dmax + (((size_t)(k & 1) * s.nrep + rp) * kDmaxBuckets + lane % kDmaxBuckets) * 2;
   double dm = wave_max(__longlong_as_double((long long)d[0]));
   const double me = wave_max(__longlong_as_double((long long)d[1]));
+  if (R.own_lists) {
+    // the context's own rows: every pair within r_short, valid while no two
+    // segment points (or dipoles) can have closed the skin
+    srb = false;
+    return 2. * me > R.r_short - R.r_wca;
+  }
   dm = 2. * sqrt(3. * dm * dm);
   const bool rb = dm > R.r_list_max - R.r_list_min;
   srb = !rb && 2. * me > R.skin_short;
@@ -3189,7 +3195,10 @@ __global__ void __launch_bounds__(256) k_drift(State s, const Thermo *__restrict
         du2 += U * U;
         th_sh[a][lane] = recip_theta(s, Rp, g, a);
       }
-      me = sqrt(dr2) + 0.5 * s.len[g] * sqrt(du2);
+      // segment points move by at most |dr| + L/2 |du|; with the context's
+      // own rows the bound also covers the dipoles (levers up to lever_max)
+      const double arm = Rp.own_lists ? fmax(0.5 * s.len[g], s.lever_max) : 0.5 * s.len[g];
+      me = sqrt(dr2) + arm * sqrt(du2);
       recip_amp_particle<R>(s, g, amp, m2);
     }
     wave_store_sums(rows1 + (size_t)tid * 16, acc, 2, lane);

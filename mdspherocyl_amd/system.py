@@ -209,7 +209,7 @@ class MdSystem:
         self._chk(self.L.msc_get_tuning(self._ctx, ctypes.byref(t)))
         return t.as_dict()
 
-    def set_tuning(self, graph=None, fused_pairs=None, pair_waves=None):
+    def set_tuning(self, graph=None, fused_pairs=None, pair_waves=None, list_skin=None):
         t = _lib.Tuning()
         self._chk(self.L.msc_get_tuning(self._ctx, ctypes.byref(t)))
         if graph is not None:
@@ -218,6 +218,8 @@ class MdSystem:
             t.fused_pairs = int(bool(fused_pairs))
         if pair_waves is not None:
             t.pair_waves = int(pair_waves)
+        if list_skin is not None:
+            t.list_skin = float(list_skin)
         self._chk(self.L.msc_set_tuning(self._ctx, ctypes.byref(t)))
 
     def autotune(self, steps=100):
@@ -225,15 +227,11 @@ class MdSystem:
         steps from the current state (restored after each trial) and keeps
         the fastest.  Returns (choice dict, [(candidate dict, ms/step)])."""
         best = _lib.Tuning()
-        ms = (ctypes.c_double * 16)()
+        tried = (_lib.Tuning * 32)()
+        ms = (ctypes.c_double * 32)()
         n = ctypes.c_int()
-        self._chk(self.L.msc_autotune(self._ctx, int(steps), ctypes.byref(best), ms, 16, ctypes.byref(n)))
-        cands = []
-        for g in (0, 1):
-            cands.append({"graph": g, "fused_pairs": 0, "pair_waves": 4})
-            if n.value > 2:
-                cands += [{"graph": g, "fused_pairs": 1, "pair_waves": w} for w in (2, 4, 8)]
-        return best.as_dict(), [(c, ms[i]) for i, c in enumerate(cands[:n.value])]
+        self._chk(self.L.msc_autotune(self._ctx, int(steps), ctypes.byref(best), tried, ms, 32, ctypes.byref(n)))
+        return best.as_dict(), [(tried[i].as_dict(), ms[i]) for i in range(min(n.value, 32))]
 
     # -- measurement --------------------------------------------------------
     def set_timing(self, on):

@@ -66,6 +66,25 @@ def test_fused_pairs_equal_separate_launches(msc):
         assert np.array_equal(x, y)
 
 
+def test_own_rows_follow_reference_rows(msc):
+    """The context's own rows (list skin 1.0 and 0.5) against the
+    reference's rows and rebuild test (list skin 0): the same pairs interact,
+    in another summation order, so the trajectories agree to 1e-9 over 130
+    steps; both schedules rebuild within the run."""
+    _say("case: own rows vs reference rows, mixed, 2 x N=400, 130 steps")
+    a = _run(msc, dict(fused_pairs=1, pair_waves=4, list_skin=0.0))
+    for skin in (1.0, 0.5):
+        b = _run(msc, dict(fused_pairs=1, pair_waves=4, list_skin=skin))
+        assert b[4]["list_skin"] == skin
+        _say("  skin %.2f: rebuilds %d (reference rows: %d)"
+             % (skin, int(b[0][:, 0, 11].sum()), int(a[0][:, 0, 11].sum())))
+        assert int(b[0][:, 0, 11].sum()) >= 1 and int(a[0][:, 0, 11].sum()) >= 1
+        assert np.allclose(b[0][:, :, 0], a[0][:, :, 0], rtol=1e-9, atol=0)
+        assert np.allclose(b[0][:, :, 1], a[0][:, :, 1], rtol=1e-9, atol=1e-12)
+        F, Fb = a[2][0], b[2][0]
+        assert np.abs(F - Fb).max() <= 1e-9 * np.abs(F).max()
+
+
 @pytest.mark.parametrize("waves", [2, 8])
 def test_pair_waves_change_rounding_only(msc, waves):
     """2 or 8 waves per task against 4: the same trajectory to 1e-9."""
@@ -89,7 +108,9 @@ def test_autotune_keeps_the_run(msc):
     best, tried = b[3]
     _say("  tried: %s" % ", ".join("%s %.4f ms" % (c, t) for c, t in tried))
     _say("  chose: %s" % best)
-    assert len(tried) == 8 and all(t > 0 for _, t in tried)
+    # 8 launch choices, then the three list skins other than the default
+    assert len(tried) == 11 and all(t > 0 for _, t in tried)
+    assert sorted({c["list_skin"] for c, _ in tried}) == [0.5, 0.75, 1.0, 1.5]
     assert best in [c for c, _ in tried]
     assert b[4] == best
     assert np.allclose(b[0][:, :, 0], a[0][:, :, 0], rtol=1e-9, atol=0)
