@@ -487,6 +487,7 @@ __device__ void build_lists_wave(const State &s, int t, const int *__restrict__ 
     }
   }
   const int nr = (2 * rz + 1) * (2 * ry + 1) * nrun;
+  int rlen = 0;
   if (lane < nr) {
     const int ru = lane % nrun, row = lane / nrun;
     const int dy = row % (2 * ry + 1) - ry, dz = row / (2 * ry + 1) - rz;
@@ -496,42 +497,52 @@ __device__ void build_lists_wave(const State &s, int t, const int *__restrict__ 
     const int crow = R.cell0 + (cz * R.nc[1] + cy) * ncx;
     const int cl = crow + (ru ? runs[1][0] : runs[0][0]), ch = crow + (ru ? runs[1][1] : runs[0][1]);
     RB[lane] = s.cell_start[cl];
-    RE[lane] = cell_end(s, R, ch);
+    rlen = cell_end(s, R, ch) - RB[lane];
   }
+  // Candidates are numbered across the runs in order (run r holds numbers
+  // RE[r] .. RE[r] + rlen - 1, RE holding the exclusive prefix sums of the
+  // run lengths), so each pass takes 64 of them whatever the run lengths.
+  int incl = rlen;
+  for (int off = 1; off < 64; off <<= 1) {
+    const int v = __shfl_up(incl, off);
+    if (lane >= off) incl += v;
+  }
+  RE[lane] = incl - rlen;
+  const int total = __builtin_amdgcn_readfirstlane(__shfl(incl, 63));
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  // Passes of 64 candidates over the runs in order; the records of the next
-  // pass are loaded while the current one is processed.
-  int r = 0, q0 = 0, e = 0;
-  for (; r < nr; ++r) {
-    q0 = __builtin_amdgcn_readfirstlane(RB[r]);
-    e = __builtin_amdgcn_readfirstlane(RE[r]);
-    if (q0 < e) break;
-  }
+  // record of candidate c (c < total): the last run starting at or before c
+  // (empty runs start where the next one does)
+  auto record_of = [&](int c) {
+    int lo = 0, hi = nr - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (RE[mid] <= c) lo = mid;
+      else hi = mid - 1;
+    }
+    return RB[lo] + (c - RE[lo]);
+  };
+  // Passes of 64 candidates; the records of the next pass are loaded while
+  // the current one is processed.
+  int qc = lane < total ? record_of(lane) : t;
   double cur[8];
   {
-    const double *pj = s.cell_pack + (size_t)(r < nr && q0 + lane < e ? q0 + lane : t) * 8;
+    const double *pj = s.cell_pack + (size_t)qc * 8;
 #pragma unroll
     for (int k = 0; k < 8; ++k) cur[k] = pj[k];
   }
-  while (r < nr) {
-    int r2 = r, q2 = q0 + 64, e2 = e;
-    if (q2 >= e2)
-      for (++r2; r2 < nr; ++r2) {
-        q2 = __builtin_amdgcn_readfirstlane(RB[r2]);
-        e2 = __builtin_amdgcn_readfirstlane(RE[r2]);
-        if (q2 < e2) break;
-      }
+  for (int c0 = 0; c0 < total; c0 += 64) {
+    const int qn2 = c0 + 64 + lane < total ? record_of(c0 + 64 + lane) : t;
     double nxt[8];
     {
-      const double *pj = s.cell_pack + (size_t)(r2 < nr && q2 + lane < e2 ? q2 + lane : t) * 8;
+      const double *pj = s.cell_pack + (size_t)qn2 * 8;
 #pragma unroll
       for (int k = 0; k < 8; ++k) nxt[k] = pj[k];
     }
     {
-      const int q = q0 + lane;
-      bool act = q < e;
+      const int q = qc;
+      bool act = c0 + lane < total;
       const int j = (int)cur[7];
       act = act && j != g;
       double rgj[3];
@@ -586,9 +597,7 @@ __device__ void build_lists_wave(const State &s, int t, const int *__restrict__ 
     }
 #pragma unroll
     for (int k = 0; k < 8; ++k) cur[k] = nxt[k];
-    r = r2;
-    q0 = q2;
-    e = e2;
+    qc = qn2;
   }
   if (qn > 0) flush(qn);
   if (lane == 0) {
@@ -717,13 +726,16 @@ __global__ void __launch_bounds__(256) k_lists_step(State s, ListStep ls, int *_
   for (int r = s.rep_begin + lane; r < s.rep_end; r += 64)
     any = any || ls.rebuild[r] || (ls.srebuild && ls.srebuild[r]);
   if (__ballot(any) == 0) return;
-  const int g = s.g_begin + __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + wv);
-  if (g >= s.g_end) return;
-  const int rp = __builtin_amdgcn_readfirstlane(s.rep_of[g]);
-  if (ls.rebuild[rp])
-    build_lists_wave(s, g, ls.rebuild, flags, queue[wv], run_b[wv], run_e[wv]);
-  else if (ls.srebuild && ls.srebuild[rp])
-    short_build_wave(s, g, ls.srebuild, flags, queue[wv]);
+  // a fixed grid (launch_lists_step) strides over the particles: the steps
+  // that rebuild nothing cost one small grid
+  for (int gi = blockIdx.x * 4 + wv; gi < s.g_end - s.g_begin; gi += gridDim.x * 4) {
+    const int g = s.g_begin + __builtin_amdgcn_readfirstlane(gi);
+    const int rp = __builtin_amdgcn_readfirstlane(s.rep_of[g]);
+    if (ls.rebuild[rp])
+      build_lists_wave(s, g, ls.rebuild, flags, queue[wv], run_b[wv], run_e[wv]);
+    else if (ls.srebuild && ls.srebuild[rp])
+      short_build_wave(s, g, ls.srebuild, flags, queue[wv]);
+  }
 }
 
 // --------------------------------------------------------------------------
@@ -3602,7 +3614,8 @@ void launch_end(const State &s, Thermo *th, const double *gs, const int *step, c
              region, has_field ? 1 : 0, kblk, nblk(maxslot, 64), E, Kin, counts, obs, ls);
 }
 void launch_lists_step(const State &s, const ListStep &ls, int *flags, int *step, hipStream_t st) {
-  MSC_LAUNCH(k_lists_step, dim3(nblk(s.g_end - s.g_begin, 4)), dim3(256), 0, st, s, ls, flags, step);
+  MSC_LAUNCH(k_lists_step, dim3(std::min(nblk(s.g_end - s.g_begin, 4), kListStepBlocks)), dim3(256), 0, st, s, ls,
+             flags, step);
 }
 void launch_stamp(double *p, double v, hipStream_t st) { MSC_LAUNCH(k_stamp, dim3(1), dim3(64), 0, st, p, v); }
 void launch_fill_int(int *p, int n, int v, hipStream_t st) {

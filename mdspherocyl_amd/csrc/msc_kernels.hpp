@@ -184,5 +184,6 @@ constexpr int kRedStride = 16;  // doubles per replica in the reduction scratch 
 constexpr int kRfGroups = 8;    // waves sharing one task in the reciprocal back projection
 constexpr int kMaxPairWaves = 8;  // most waves per task of the fused mixed pair kernel
 constexpr int kPairSplit = 4;   // waves sharing one task in the WCA and real-space pair kernels
+constexpr int kListStepBlocks = 2048;  // grid of the step loop's row launch (4 particles per wave at a time)
 
 }  // namespace msc

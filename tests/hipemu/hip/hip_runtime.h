@@ -101,6 +101,31 @@ inline T __shfl_xor(T v, int o) {
   return (T)r;
 }
 
+// value of lane (lane - d), or the caller's own where that is below 0
+template <typename T>
+inline T __shfl_up(T v, unsigned d) {
+  hipemu::Block *b = hipemu::cur_block();
+  unsigned t = hipemu::t_threadIdx.x, w = t / 64, lane = t % 64;
+  double *x = b->xch.data() + w * 64;
+  x[lane] = (double)v;
+  b->wbar[w]->arrive_and_wait();
+  double r = lane >= d ? x[lane - d] : (double)v;
+  b->wbar[w]->arrive_and_wait();
+  return (T)r;
+}
+// value of lane src
+template <typename T>
+inline T __shfl(T v, int src) {
+  hipemu::Block *b = hipemu::cur_block();
+  unsigned t = hipemu::t_threadIdx.x, w = t / 64, lane = t % 64;
+  double *x = b->xch.data() + w * 64;
+  x[lane] = (double)v;
+  b->wbar[w]->arrive_and_wait();
+  double r = x[src & 63];
+  b->wbar[w]->arrive_and_wait();
+  return (T)r;
+}
+
 // all 64 lanes of a wave take part (the kernels only call these from
 // wave-uniform control flow)
 inline unsigned long long __ballot(int pred) {
