@@ -183,13 +183,18 @@ typedef struct msc_tuning {
                        461-463, 679-701) with contact rows refreshed from
                        them.  MSC_FP64: always 0.  A change takes effect at
                        the next msc_init_run. */
+  int chunk;        /* particles per structure-factor chunk of the reciprocal
+                       sum (64, 128, 256 or 512; default 128): fewer, larger
+                       chunks write and re-read fewer partial sums.  Changes
+                       the summation order of S(k) only. */
 } msc_tuning;
 int msc_get_tuning(msc_ctx *ctx, msc_tuning *t);
 int msc_set_tuning(msc_ctx *ctx, const msc_tuning *t);
 /* The autotuner: times `steps` steps of msc_run under each candidate tuning
  * (graph off/on x, for contexts that can fuse, unfused or fused with 2, 4 or
  * 8 waves per task; then, for MSC_MIXED contexts, list skins 0.5, 0.75, 1.0
- * and 1.5 with the fastest of those), each from the state at the call, and
+ * and 1.5 with the fastest of those; then chunks of 64, 128, 256 and 512
+ * particles with the fastest so far), each from the state at the call, and
  * keeps the fastest.  The state, thermostats and thermostat generators are restored
  * after every trial and the lists rebuilt from the restored state (as after a
  * chunk rollback of msc_run), so a run continues from the same state; only

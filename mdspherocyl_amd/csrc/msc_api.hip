@@ -104,7 +104,9 @@ struct msc_ctx {
   double list_skin = 0.0;
   std::vector<double> sh_dmin, sh_dmax;
   long nl_alloc = 0, sl_alloc = 0, dl_alloc = 0;  // entries allocated for the row arrays
-  size_t part_elems = 0;
+  size_t part_elems = 0;      // structure-factor partials the buffers hold (at the present chunk)
+  size_t part_cap = 0;        // and at most (allocated)
+  size_t rtask_cap = 0;       // chunk tasks d_rtasks holds
   int nslot_tot = 0, maxslot = 0, maxcol = 0;
   // device buffers
   RepDev *d_reps = nullptr;
@@ -943,6 +945,8 @@ int msc_create(int device, int precision, int nrep, const msc_replica_desc *reps
   ctx->Ntot = n0;
   ctx->nslot_tot = slot0;
   ctx->part_elems = (size_t)part0;
+  ctx->part_cap = ctx->part_elems;
+  ctx->rtask_cap = ctx->rtasks.size();
   ctx->thermo_rng.resize(2 * nrep);
   for (int r = 0; r < nrep; ++r)
     for (int i = 0; i < 2; ++i) ctx->thermo_rng[2 * r + i].seed(reps[r].thermo_seed[i]);
@@ -1755,6 +1759,40 @@ int msc_set_graph(msc_ctx *ctx, int on) {
   return MSC_OK;
 }
 
+// Particles per structure-factor chunk task: the chunk tasks and each
+// replica's partial offsets laid out again, the buffers grown when needed.
+static int set_chunk(msc_ctx *ctx, int chunk) {
+  if (chunk == ctx->rchunk) return MSC_OK;
+  drop_graph(ctx);
+  std::vector<Task> rt;
+  long part0 = 0;
+  for (int r = 0; r < ctx->nrep; ++r) {
+    RepDev &R = ctx->reps[r];
+    R.rtask0 = (int)rt.size();
+    for (int g = 0; g < R.N; g += chunk) rt.push_back({r, R.n0 + g, std::min(chunk, R.N - g), 0});
+    R.nrtask = (int)rt.size() - R.rtask0;
+    R.part0 = part0;
+    part0 += (long)R.nrtask * R.ncol * R.KZ * 6;
+  }
+  const size_t rs = ctx->mixed ? sizeof(float) : sizeof(double);
+  if ((size_t)part0 > ctx->part_cap) {
+    if (int rc_ = dalloc(ctx, (unsigned char **)&ctx->d_part, (size_t)part0 * rs)) return rc_;
+    if (ctx->nd > 1)
+      if (int rc_ = dalloc(ctx, &ctx->d_part_nd, (size_t)part0 / 6 * 10)) return rc_;
+    ctx->part_cap = (size_t)part0;
+  }
+  if (rt.size() > ctx->rtask_cap) {
+    if (int rc_ = dalloc(ctx, &ctx->d_rtasks, rt.size())) return rc_;
+    ctx->rtask_cap = rt.size();
+  }
+  ctx->rtasks.swap(rt);
+  ctx->part_elems = (size_t)part0;
+  ctx->rchunk = chunk;
+  if (int rc_ = h2d(ctx, ctx->d_rtasks, ctx->rtasks.data(), sizeof(Task) * ctx->rtasks.size())) return rc_;
+  refresh_state_ptrs(ctx);
+  return upload_reps(ctx);
+}
+
 int msc_get_tuning(msc_ctx *ctx, msc_tuning *t) {
   SCOPE(ctx);
   if (!ctx || !t) return MSC_E_ARG;
@@ -1762,6 +1800,7 @@ int msc_get_tuning(msc_ctx *ctx, msc_tuning *t) {
   t->fused_pairs = ctx->fused_pairs ? 1 : 0;
   t->pair_waves = ctx->pair_waves;
   t->list_skin = ctx->list_skin;
+  t->chunk = ctx->rchunk;
   return MSC_OK;
 }
 
@@ -1770,6 +1809,9 @@ int msc_set_tuning(msc_ctx *ctx, const msc_tuning *t) {
   if (!ctx || !t || (t->pair_waves != 2 && t->pair_waves != 4 && t->pair_waves != 8)) return MSC_E_ARG;
   if (!(t->list_skin >= 0) || !std::isfinite(t->list_skin) || (!ctx->mixed && t->list_skin != 0))
     return fail(ctx, MSC_E_ARG, "list_skin must be >= 0, and 0 for MSC_FP64 contexts");
+  if (t->chunk != 64 && t->chunk != 128 && t->chunk != 256 && t->chunk != 512)
+    return fail(ctx, MSC_E_ARG, "chunk must be 64, 128, 256 or 512");
+  if (int rc_ = set_chunk(ctx, t->chunk)) return rc_;
   drop_graph(ctx);  // the captured step holds the previous launch shapes
   ctx->graph_on = t->graph != 0;
   ctx->fused_pairs = t->fused_pairs != 0;
@@ -2083,6 +2125,15 @@ int msc_autotune(msc_ctx *ctx, int steps, msc_tuning *best, msc_tuning *tried, d
       if (skin == start.list_skin) continue;
       msc_tuning c = launch;
       c.list_skin = skin;
+      if (int rc_ = trial(c)) return rc_;
+    }
+  }
+  {
+    const msc_tuning sofar = win;
+    for (int chunk : {64, 128, 256, 512}) {
+      if (chunk == start.chunk) continue;
+      msc_tuning c = sofar;
+      c.chunk = chunk;
       if (int rc_ = trial(c)) return rc_;
     }
   }

@@ -137,29 +137,36 @@ __device__ __forceinline__ R fmaR(R a, R b, R c) { return fma(a, b, c); }
 // dipole lab frame: position r + dpc * u, orientation O^T c
 // (Spherocyl::set_dip / translate / rotate, v25/classes.cpp:148-304)
 // --------------------------------------------------------------------------
-__device__ __forceinline__ void dipoles_particle(const State &s, int g) {
+// From the centre r and orientation O in registers; slot 0's lab position
+// and orientation are also returned in pos0, ori0 (when not null).
+__device__ __forceinline__ void dipoles_regs(const State &s, int g, const double r[3], const double O[9],
+                                             double *pos0, double *ori0) {
   const int N = s.Ntot;
-  double O[9];
-  for (int k = 0; k < 9; ++k) O[k] = s.ori[k * N + g];
-  double r[3] = {s.x[g], s.x[N + g], s.x[2 * N + g]};
   for (int w = 0; w < s.nd; ++w) {
     int q = w * N + g;
     double pc = s.dpc[q];
     double c[3] = {s.doc[(3 * w + 0) * N + g], s.doc[(3 * w + 1) * N + g], s.doc[(3 * w + 2) * N + g]};
+    double dp[3], dor[3];
     for (int a = 0; a < 3; ++a) {
-      s.dpos[(3 * w + a) * N + g] = r[a] + pc * O[6 + a];
+      dp[a] = r[a] + pc * O[6 + a];
       double acc = 0.0;
       acc += O[0 * 3 + a] * c[0];
       acc += O[1 * 3 + a] * c[1];
       acc += O[2 * 3 + a] * c[2];
+      dor[a] = acc;
+      s.dpos[(3 * w + a) * N + g] = dp[a];
       s.dori[(3 * w + a) * N + g] = acc;
+      if (w == 0 && pos0) {
+        pos0[a] = dp[a];
+        ori0[a] = acc;
+      }
     }
     if (s.rec_dip) {
       double *o = s.rec_dip + (size_t)q * 8;
       float f[8];
       for (int a = 0; a < 3; ++a) {
-        o[a] = s.dpos[(3 * w + a) * N + g];
-        f[a] = (float)s.dori[(3 * w + a) * N + g];
+        o[a] = dp[a];
+        f[a] = (float)dor[a];
         f[3 + a] = (float)(pc * O[6 + a]);
       }
       f[6] = (float)s.dmu[q];
@@ -178,6 +185,13 @@ __device__ __forceinline__ void dipoles_particle(const State &s, int g) {
     o[6] = s.len[g];
     o[7] = s.dia[g];
   }
+}
+__device__ __forceinline__ void dipoles_particle(const State &s, int g) {
+  const int N = s.Ntot;
+  double O[9];
+  for (int k = 0; k < 9; ++k) O[k] = s.ori[k * N + g];
+  const double r[3] = {s.x[g], s.x[N + g], s.x[2 * N + g]};
+  dipoles_regs(s, g, r, O, nullptr, nullptr);
 }
 
 __global__ void k_dipoles(State s) {
@@ -1476,10 +1490,12 @@ struct Cpx { R re, im; };
 // theta = 2 pi x_a / L_a of the wrapped dipole position (as force_dipK
 // wraps, v25/functions.cpp:1793-1794), by the recurrence
 // e^{-i (n+1) theta} = e^{-i n theta} e^{-i theta} in FP64 (one sincos).
-__device__ __forceinline__ double recip_theta(const State &s, const RepDev &Rp, int g, int a) {
-  double x = s.dpos[a * s.Ntot + g];
+__device__ __forceinline__ double recip_theta_of(const RepDev &Rp, int a, double x) {
   x -= Rp.L[a] * floor(x / Rp.L[a] + 0.5);
   return Rp.P2[a] * x;
+}
+__device__ __forceinline__ double recip_theta(const State &s, const RepDev &Rp, int g, int a) {
+  return recip_theta_of(Rp, a, s.dpos[a * s.Ntot + g]);
 }
 template <typename R>
 __device__ __forceinline__ void recip_phase_axis(const State &s, const RepDev &Rp, int g, int a, double th,
@@ -1499,13 +1515,19 @@ __device__ __forceinline__ void recip_phase_axis(const State &s, const RepDev &R
 }
 // amplitude c = mu u of particle g; m2 receives c c^T
 template <typename R>
-__device__ __forceinline__ void recip_amp_particle(const State &s, int g, R *__restrict__ amp, double m2[6]) {
+__device__ __forceinline__ void recip_amp_regs(const State &s, int g, double mu, const double o[3],
+                                               R *__restrict__ amp, double m2[6]) {
   const int N = s.Ntot;
-  const double mu = s.dmu[g];
-  const double c[3] = {mu * s.dori[g], mu * s.dori[N + g], mu * s.dori[2 * N + g]};
+  const double c[3] = {mu * o[0], mu * o[1], mu * o[2]};
   for (int a = 0; a < 3; ++a) amp[a * N + g] = (R)c[a];
   m2[0] = c[0] * c[0]; m2[1] = c[0] * c[1]; m2[2] = c[0] * c[2];
   m2[3] = c[1] * c[1]; m2[4] = c[1] * c[2]; m2[5] = c[2] * c[2];
+}
+template <typename R>
+__device__ __forceinline__ void recip_amp_particle(const State &s, int g, R *__restrict__ amp, double m2[6]) {
+  const int N = s.Ntot;
+  const double o[3] = {s.dori[g], s.dori[N + g], s.dori[2 * N + g]};
+  recip_amp_regs<R>(s, g, s.dmu[g], o, amp, m2);
 }
 
 // phase tables and amplitudes of every particle; per-task sums of c c^T
@@ -2922,38 +2944,37 @@ __device__ __forceinline__ void mv3(const double *A, const double *x, double *y)
 // moveP of one particle (v25/functions.cpp:2071-2184) with thermostat
 // coordinates ts[2]; acc receives [0] p^2/m, [1] l^T Ib l and, after the
 // last kick, [2..7] the kinetic pressure tensor and [8..13] p and l.
+// moveP of one particle in registers (p, l updated in place from the force
+// F and torque T; O the orientation, I the principal moments)
 template <bool LAST>
-__device__ __forceinline__ void kick_particle(const State &s, const RepDev &Rp, int g, const double ts[2],
-                                              double acc[14]) {
-  const int N = s.Ntot;
+__device__ __forceinline__ void kick_regs(const RepDev &Rp, double m, const double I[3], const double O[9],
+                                          const double F[3], const double T[3], const double ts[2], double p[3],
+                                          double l[3], double acc[14]) {
   const double Dt2 = Rp.dt / 2;
-  double m = s.mass[g];
-  double I[3] = {s.inert[g], s.inert[N + g], s.inert[2 * N + g]};
-  double O[9], Ot[9], I0[9] = {1. / I[0], 0., 0., 0., 1. / I[1], 0., 0., 0., 1. / I[2]}, Ax[9], Ib[9];
-  for (int k = 0; k < 9; ++k) O[k] = s.ori[k * N + g];
+  double Ot[9], I0[9] = {1. / I[0], 0., 0., 0., 1. / I[1], 0., 0., 0., 1. / I[2]}, Ax[9], Ib[9];
   for (int a = 0; a < 3; ++a)
     for (int b = 0; b < 3; ++b) Ot[a * 3 + b] = O[b * 3 + a];
   mm3(Ot, I0, Ax);
   mm3(Ax, O, Ib);
-  double p[3], l[3], psq = 0;
+  double psq = 0;
   if (!LAST) {
     double sfac = 1. + Dt2 * ts[0];
     for (int j = 0; j < 3; ++j) {
-      p[j] = (s.p[j * N + g] + Dt2 * s.F[j * N + g]) / sfac;
+      p[j] = (p[j] + Dt2 * F[j]) / sfac;
       psq += p[j] * p[j];
     }
     sfac = 1. + Dt2 * ts[1];
-    for (int j = 0; j < 3; ++j) l[j] = (s.l[j * N + g] + Dt2 * s.T[j * N + g]) / sfac;
+    for (int j = 0; j < 3; ++j) l[j] = (l[j] + Dt2 * T[j]) / sfac;
   } else {
     double sfac = 1. - Dt2 * ts[0];
     for (int j = 0; j < 3; ++j) {
-      p[j] = s.p[j * N + g] * sfac + Dt2 * s.F[j * N + g];
+      p[j] = p[j] * sfac + Dt2 * F[j];
       psq += p[j] * p[j];
     }
     acc[2] = p[0] * p[0] / m; acc[3] = p[0] * p[1] / m; acc[4] = p[0] * p[2] / m;
     acc[5] = p[1] * p[1] / m; acc[6] = p[1] * p[2] / m; acc[7] = p[2] * p[2] / m;
     sfac = 1. - Dt2 * ts[1];
-    for (int j = 0; j < 3; ++j) l[j] = s.l[j * N + g] * sfac + Dt2 * s.T[j * N + g];
+    for (int j = 0; j < 3; ++j) l[j] = l[j] * sfac + Dt2 * T[j];
   }
   double v[3];
   for (int n = 0; n < 3; ++n) {
@@ -2965,10 +2986,6 @@ __device__ __forceinline__ void kick_particle(const State &s, const RepDev &Rp, 
   for (int k = 0; k < 3; ++k) lres += v[k] * l[k];
   double lsq = 0.0;
   lsq += lres;
-  for (int j = 0; j < 3; ++j) {
-    s.p[j * N + g] = p[j];
-    s.l[j * N + g] = l[j];
-  }
   acc[0] = psq / m;
   acc[1] = lsq;
   if (LAST)
@@ -2976,6 +2993,26 @@ __device__ __forceinline__ void kick_particle(const State &s, const RepDev &Rp, 
       acc[8 + j] = p[j];
       acc[11 + j] = l[j];
     }
+}
+template <bool LAST>
+__device__ __forceinline__ void kick_particle(const State &s, const RepDev &Rp, int g, const double ts[2],
+                                              double acc[14]) {
+  const int N = s.Ntot;
+  const double m = s.mass[g];
+  const double I[3] = {s.inert[g], s.inert[N + g], s.inert[2 * N + g]};
+  double O[9], p[3], l[3], F[3], T[3];
+  for (int k = 0; k < 9; ++k) O[k] = s.ori[k * N + g];
+  for (int j = 0; j < 3; ++j) {
+    p[j] = s.p[j * N + g];
+    l[j] = s.l[j * N + g];
+    F[j] = s.F[j * N + g];
+    T[j] = s.T[j * N + g];
+  }
+  kick_regs<LAST>(Rp, m, I, O, F, T, ts, p, l, acc);
+  for (int j = 0; j < 3; ++j) {
+    s.p[j * N + g] = p[j];
+    s.l[j * N + g] = l[j];
+  }
 }
 
 // moveP for every particle; thermostat coordinates read from device memory.
@@ -2994,16 +3031,15 @@ __global__ void __launch_bounds__(256) k_moveP(State s, const Thermo *__restrict
   wave_store_sums(task_out + (size_t)tid * 16, acc, LAST ? 14 : 8, lane);
 }
 
-// translate + rotate (5-factor Rx Ry Rz Ry Rx splitting) + dipole update
-__device__ __forceinline__ void drift_particle(const State &s, const RepDev &Rp, int g) {
-  const int N = s.Ntot;
+// translate + rotate (5-factor Rx Ry Rz Ry Rx splitting) in registers: x
+// and the orientation Oio updated in place from the momenta p, l
+__device__ __forceinline__ void drift_regs(const RepDev &Rp, double m, const double I[3], const double p[3],
+                                           const double l[3], double x[3], double Oio[9]) {
   const double dt = Rp.dt, dt2 = dt / 2;
-  double m = s.mass[g];
-  for (int j = 0; j < 3; ++j) s.x[j * N + g] += dt * s.p[j * N + g] / m;
+  for (int j = 0; j < 3; ++j) x[j] += dt * p[j] / m;
   double O[9], Q[9], R[9], l0[3], l1[3];
-  double I[3] = {s.inert[g], s.inert[N + g], s.inert[2 * N + g]};
-  for (int k = 0; k < 9; ++k) O[k] = s.ori[k * N + g];
-  for (int k = 0; k < 3; ++k) l0[k] = s.l[k * N + g];
+  for (int k = 0; k < 9; ++k) O[k] = Oio[k];
+  for (int k = 0; k < 3; ++k) l0[k] = l[k];
   mv3(O, l0, l1);
   double ang, ca, sa;
 #define RX(c, s_) { R[0]=1.; R[1]=0.; R[2]=0.; R[3]=0.; R[4]=c; R[5]=-s_; R[6]=0.; R[7]=s_; R[8]=c; }
@@ -3022,7 +3058,22 @@ __device__ __forceinline__ void drift_particle(const State &s, const RepDev &Rp,
 #undef RX
 #undef RY
 #undef RZ
-  for (int k = 0; k < 9; ++k) s.ori[k * N + g] = Q[k];
+  for (int k = 0; k < 9; ++k) Oio[k] = Q[k];
+}
+__device__ __forceinline__ void drift_particle(const State &s, const RepDev &Rp, int g) {
+  const int N = s.Ntot;
+  const double m = s.mass[g];
+  const double I[3] = {s.inert[g], s.inert[N + g], s.inert[2 * N + g]};
+  double x[3], p[3], l[3], O[9];
+  for (int j = 0; j < 3; ++j) {
+    x[j] = s.x[j * N + g];
+    p[j] = s.p[j * N + g];
+    l[j] = s.l[j * N + g];
+  }
+  for (int k = 0; k < 9; ++k) O[k] = s.ori[k * N + g];
+  drift_regs(Rp, m, I, p, l, x, O);
+  for (int j = 0; j < 3; ++j) s.x[j * N + g] = x[j];
+  for (int k = 0; k < 9; ++k) s.ori[k * N + g] = O[k];
 }
 
 __global__ void k_moveX(State s) {
@@ -3030,10 +3081,8 @@ __global__ void k_moveX(State s) {
   if (g < s.g_end) drift_particle(s, s.reps[s.rep_of[g]], g);
 }
 
-__device__ __forceinline__ void renorm_particle(const State &s, int g) {
-  const int N = s.Ntot;
-  double o[9];
-  for (int k = 0; k < 9; ++k) o[k] = s.ori[k * N + g];
+// renorm (v25/classes.cpp:307-325) of an orientation in registers
+__device__ __forceinline__ void renorm_regs(double o[9]) {
   double norm = sqrt(o[6] * o[6] + o[7] * o[7] + o[8] * o[8]);
   o[6] /= norm; o[7] /= norm; o[8] /= norm;
   double Up[3] = {o[6] + o[3], o[7] + o[4], o[8] + o[5]};
@@ -3046,6 +3095,12 @@ __device__ __forceinline__ void renorm_particle(const State &s, int g) {
   o[0] = o[4] * o[8] - o[7] * o[5];
   o[1] = o[5] * o[6] - o[8] * o[3];
   o[2] = o[3] * o[7] - o[6] * o[4];
+}
+__device__ __forceinline__ void renorm_particle(const State &s, int g) {
+  const int N = s.Ntot;
+  double o[9];
+  for (int k = 0; k < 9; ++k) o[k] = s.ori[k * N + g];
+  renorm_regs(o);
   for (int k = 0; k < 9; ++k) s.ori[k * N + g] = o[k];
 }
 __global__ void k_renorm(State s) {
@@ -3185,33 +3240,56 @@ __global__ void __launch_bounds__(256) k_drift(State s, const Thermo *__restrict
     double acc[14] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     double m = 0.0, me = 0.0, m2[6] = {0, 0, 0, 0, 0, 0};
     if (lane < tk.n) {
+      // the particle's state is loaded once, advanced in registers (renorm,
+      // kick, drift, dipole frame) and stored once
       const int g = tk.g0 + lane;
-      if (renorm) renorm_particle(s, g);
-      const Thermo T = th[tk.rep];
-      const double ts[2] = {thermo_half(Rp, T.s[0], gs[tk.rep * 4 + 0]), thermo_half(Rp, T.s[1], gs[tk.rep * 4 + 2])};
-      kick_particle<false>(s, Rp, g, ts, acc);
-      drift_particle(s, Rp, g);
-      dipoles_particle(s, g);
-      double dr2 = 0, du2 = 0;
+      const double mass = s.mass[g], len = s.len[g];
+      const double I[3] = {s.inert[g], s.inert[N + g], s.inert[2 * N + g]};
+      double x[3], p[3], l[3], F[3], T[3], O[9], sn[3], se[6];
       for (int a = 0; a < 3; ++a) {
+        x[a] = s.x[a * N + g];
+        p[a] = s.p[a * N + g];
+        l[a] = s.l[a * N + g];
+        F[a] = s.F[a * N + g];
+        T[a] = s.T[a * N + g];
+        sn[a] = s.snap[a * N + g];
+        se[a] = s.snap_e[a * N + g];
+        se[3 + a] = s.snap_e[(3 + a) * N + g];
+      }
+      for (int q = 0; q < 9; ++q) O[q] = s.ori[q * N + g];
+      if (renorm) renorm_regs(O);
+      const Thermo Th = th[tk.rep];
+      const double ts[2] = {thermo_half(Rp, Th.s[0], gs[tk.rep * 4 + 0]),
+                            thermo_half(Rp, Th.s[1], gs[tk.rep * 4 + 2])};
+      kick_regs<false>(Rp, mass, I, O, F, T, ts, p, l, acc);
+      drift_regs(Rp, mass, I, p, l, x, O);
+      double pos0[3], ori0[3];
+      dipoles_regs(s, g, x, O, pos0, ori0);
+      for (int a = 0; a < 3; ++a) {
+        s.x[a * N + g] = x[a];
+        s.p[a * N + g] = p[a];
+        s.l[a * N + g] = l[a];
         s.F[a * N + g] = 0.0;
         s.T[a * N + g] = 0.0;
-        const double x = s.x[a * N + g];
-        double D = x - s.snap[a * N + g];
+      }
+      for (int q = 0; q < 9; ++q) s.ori[q * N + g] = O[q];
+      double dr2 = 0, du2 = 0;
+      for (int a = 0; a < 3; ++a) {
+        double D = x[a] - sn[a];
         D -= Rp.L[a] * floor(D / Rp.L[a] + 0.5);
         m = fmax(m, fabs(D));
-        double E = x - s.snap_e[a * N + g];
+        double E = x[a] - se[a];
         E -= Rp.L[a] * floor(E / Rp.L[a] + 0.5);
         dr2 += E * E;
-        double U = s.ori[(6 + a) * N + g] - s.snap_e[(3 + a) * N + g];
+        double U = O[6 + a] - se[3 + a];
         du2 += U * U;
-        th_sh[a][lane] = recip_theta(s, Rp, g, a);
+        th_sh[a][lane] = recip_theta_of(Rp, a, pos0[a]);
       }
       // segment points move by at most |dr| + L/2 |du|; with the context's
       // own rows the bound also covers the dipoles (levers up to lever_max)
-      const double arm = Rp.own_lists ? fmax(0.5 * s.len[g], s.lever_max) : 0.5 * s.len[g];
+      const double arm = Rp.own_lists ? fmax(0.5 * len, s.lever_max) : 0.5 * len;
       me = sqrt(dr2) + arm * sqrt(du2);
-      recip_amp_particle<R>(s, g, amp, m2);
+      recip_amp_regs<R>(s, g, s.dmu[g], ori0, amp, m2);
     }
     wave_store_sums(rows1 + (size_t)tid * 16, acc, 2, lane);
     m = wave_max(m);
