@@ -527,15 +527,12 @@ __device__ void build_lists_wave(const State &s, int t, const int *__restrict__ 
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   // record of candidate c (c < total): the last run starting at or before c
-  // (empty runs start where the next one does)
+  // (empty runs start where the next one does).  A lane's candidates only
+  // grow from pass to pass, so its run index only moves forward.
+  int rl = 0;
   auto record_of = [&](int c) {
-    int lo = 0, hi = nr - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (RE[mid] <= c) lo = mid;
-      else hi = mid - 1;
-    }
-    return RB[lo] + (c - RE[lo]);
+    while (rl + 1 < nr && RE[rl + 1] <= c) ++rl;
+    return RB[rl] + (c - RE[rl]);
   };
   // Passes of 64 candidates; the records of the next pass are loaded while
   // the current one is processed.
