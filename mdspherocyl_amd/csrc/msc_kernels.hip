@@ -1659,18 +1659,34 @@ __device__ __forceinline__ void wave_rows_sum(const double *__restrict__ rows, i
 // split as x = hi + lo with hi = fp16(x), lo = fp16(x - hi), and a real
 // product XY is taken as Xh Yh + Xh Yl + Xl Yh (the dropped Xl Yl and the
 // split itself are below 2^-21 |X||Y|) with FP32 accumulation in
-// v_mfma_f32_16x16x16f16; the complex product is four real products, so a
-// 16 x 16 output tile of (Re S, Im S) takes 12 instructions per 16 particles.
-// Operand layout of the instruction (checked on gfx950): A[i][k] in lane
-// i + 16 (k/4), element k % 4; B[k][n] in lane n + 16 (k/4), element k % 4;
-// D[i][n] in lane n + 16 (i/4), element i % 4.
+// v_mfma_f32_16x16x32_f16.  The 32-deep inner dimension holds 16 particles
+// twice, (Re, Im): Re S = [Re A | Im A] . [Re B ; -Im B] and
+// Im S = [Re A | Im A] . [Im B ; Re B], so a 16 x 16 output tile of
+// (Re S, Im S) takes 6 instructions per 16 particles (3 split products each).
+// Operand layout (D checked on gfx950): lane i + 16 q of A holds row i,
+// elements 0..3 the real parts of inner indices 4q..4q+3 and elements 4..7
+// their imaginary parts; B likewise with column n; D[i][n] in lane
+// n + 16 (i/4), element i % 4.  Any inner-index order gives the same product
+// as long as A and B share it.
 // Block: 4 waves over one chunk task; wave w owns M-tiles
 // 8 blockIdx.y + 2w + {0, 1} and every N-tile.  Per 64 particles the
 // phase tables are staged in LDS, the B operands of all four 16-particle
 // steps are built once for the block (wave w builds N-tiles w, w + 4, ...)
 // and read by all waves.
 typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
+typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
 typedef float float4_t __attribute__((ext_vector_type(4)));
+
+// (x, y) as one 8-wide operand
+__device__ __forceinline__ half8_t cat8(half4_t x, half4_t y) {
+  return __builtin_shufflevector(x, y, 0, 1, 2, 3, 4, 5, 6, 7);
+}
+// acc += X Y for split operands X = Xh + Xl, Y = Yh + Yl (Xl Yl dropped)
+__device__ __forceinline__ float4_t mfma_split3(half8_t xh, half8_t xl, half8_t yh, half8_t yl, float4_t acc) {
+  acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(xh, yl, acc, 0, 0, 0);
+  acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(xl, yh, acc, 0, 0, 0);
+  return __builtin_amdgcn_mfma_f32_16x16x32_f16(xh, yh, acc, 0, 0, 0);
+}
 
 // x = h + l with h = fp16(x) and l = fp16(x - h), x - h taken in FP64 (x is
 // an FP64 operand, see below); x - (h + l) is below 2^-22 |x| (2^-25
@@ -1686,7 +1702,7 @@ __device__ __forceinline__ void split_f16(double x, _Float16 &h, _Float16 &l) {
 // (N = 32768: relative L2 2e-7 against 6e-8; replicas of 3000 to 8000
 // rods: reciprocal torques 5.6e-5 against 1e-6 of their rms).
 template <int KZ>
-__global__ void __launch_bounds__(256) k_recip_struct_mfma(State s, int rt0, const Cpx<float> *__restrict__ ph,
+__global__ void __launch_bounds__(256, 2) k_recip_struct_mfma(State s, int rt0, const Cpx<float> *__restrict__ ph,
                                                            const float *__restrict__ amp, float *__restrict__ part) {
   constexpr int kmax = KZ - 1, side = 2 * kmax + 1;
   constexpr int MTOT = side * KZ, NTOT = 3 * side;
@@ -1761,12 +1777,8 @@ __global__ void __launch_bounds__(256) k_recip_struct_mfma(State s, int rt0, con
 #pragma unroll
     for (int step = 0; step < 4; ++step) {
       if (step * 16 >= np) break;  // block-uniform
-      // the step's B operands, read once and used for every owned M-tile
-      half4_t bop[NT][4];
-#pragma unroll
-      for (int t = 0; t < NT; ++t)
-#pragma unroll
-        for (int c = 0; c < 4; ++c) bop[t][c] = sb[step][t][c][lane];
+      // the A operands of the owned M-tiles (Re, Im of e_y e_z; hi and lo)
+      half8_t ah[MW], al[MW];
 #pragma unroll
       for (int u = 0; u < MW; ++u) {
         const int m = mrow[u];
@@ -1774,7 +1786,7 @@ __global__ void __launch_bounds__(256) k_recip_struct_mfma(State s, int rt0, con
         const int nyi = ok ? m / KZ : kmax, nz = ok ? m % KZ : 0;
         const int ny = nyi - kmax, ay = ny < 0 ? -ny : ny;
         const float sy = ny < 0 ? -1.f : 1.f;
-        half4_t arh, arl, aih, ail, naih, nail;
+        half4_t arh, arl, aih, ail;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
           const int p = step * 16 + 4 * lq + j;
@@ -1791,27 +1803,23 @@ __global__ void __launch_bounds__(256) k_recip_struct_mfma(State s, int rt0, con
           split_f16(ai, h, l);
           aih[j] = h;
           ail[j] = l;
-          naih[j] = -h;
-          nail[j] = -l;
         }
+        ah[u] = cat8(arh, aih);
+        al[u] = cat8(arl, ail);
+      }
+      // per N-tile the step's B operands, [Re B ; -Im B] for the real part
+      // of S and [Im B ; Re B] for the imaginary part, used by every owned
+      // M-tile
 #pragma unroll
-        for (int t = 0; t < NT; ++t) {
-          const half4_t brh = bop[t][0], brl = bop[t][1], bih = bop[t][2], bil = bop[t][3];
-          float4_t r = cr[u][t], i = ci[u][t];
-          r = __builtin_amdgcn_mfma_f32_16x16x16f16(arh, brh, r, 0, 0, 0);
-          r = __builtin_amdgcn_mfma_f32_16x16x16f16(arh, brl, r, 0, 0, 0);
-          r = __builtin_amdgcn_mfma_f32_16x16x16f16(arl, brh, r, 0, 0, 0);
-          r = __builtin_amdgcn_mfma_f32_16x16x16f16(naih, bih, r, 0, 0, 0);
-          r = __builtin_amdgcn_mfma_f32_16x16x16f16(naih, bil, r, 0, 0, 0);
-          r = __builtin_amdgcn_mfma_f32_16x16x16f16(nail, bih, r, 0, 0, 0);
-          i = __builtin_amdgcn_mfma_f32_16x16x16f16(arh, bih, i, 0, 0, 0);
-          i = __builtin_amdgcn_mfma_f32_16x16x16f16(arh, bil, i, 0, 0, 0);
-          i = __builtin_amdgcn_mfma_f32_16x16x16f16(arl, bih, i, 0, 0, 0);
-          i = __builtin_amdgcn_mfma_f32_16x16x16f16(aih, brh, i, 0, 0, 0);
-          i = __builtin_amdgcn_mfma_f32_16x16x16f16(aih, brl, i, 0, 0, 0);
-          i = __builtin_amdgcn_mfma_f32_16x16x16f16(ail, brh, i, 0, 0, 0);
-          cr[u][t] = r;
-          ci[u][t] = i;
+      for (int t = 0; t < NT; ++t) {
+        const half4_t brh = sb[step][t][0][lane], brl = sb[step][t][1][lane];
+        const half4_t bih = sb[step][t][2][lane], bil = sb[step][t][3][lane];
+        const half8_t breh = cat8(brh, -bih), brel = cat8(brl, -bil);
+        const half8_t bimh = cat8(bih, brh), biml = cat8(bil, brl);
+#pragma unroll
+        for (int u = 0; u < MW; ++u) {
+          cr[u][t] = mfma_split3(ah[u], al[u], breh, brel, cr[u][t]);
+          ci[u][t] = mfma_split3(ah[u], al[u], bimh, biml, ci[u][t]);
         }
       }
     }
@@ -1852,10 +1860,10 @@ __global__ void __launch_bounds__(256) k_recip_struct_mfma(State s, int rt0, con
 // With bop (mixed contexts) the kernel also writes the operands of the
 // matrix-core back projection, B_l(col, nz) = 2^e nz^l A(col, nz) for
 // l = 0, 1, 2 with the column tile's scale exponent e = bexp[3 (col / 16) + l], formed
-// in FP64 and split into fp16 (hi, lo) pieces, in the lane layout of a
-// 16x16x16 f16 matrix operand: column tile ct = col / 16, piece
+// in FP64 and split into fp16 (hi, lo) pieces, in the lane layout of the
+// matrix operands: column tile ct = col / 16, piece
 // q = (re hi, re lo, im hi, im lo), lane (col % 16) + 16 (nz / 4), element
-// nz % 4 (see k_recip_force_mfma).
+// nz % 4 (k_recip_force_mfma pairs the pieces into 32-deep operands).
 template <typename R>
 __global__ void __launch_bounds__(512) k_recip_reduce(State s, const R *__restrict__ part, const double *__restrict__ M2,
                                                        const double *__restrict__ m2rows, Cpx<R> *__restrict__ Aout,
@@ -2114,11 +2122,12 @@ __global__ void __launch_bounds__(64 * kRfGroups) k_recip_force(State s, const C
 
 // Back projection on the matrix cores (mixed contexts).  The nz sums of
 // k_recip_force, S_l(col) = sum_nz nz^l e_z(nz) A(col, nz), l = 0, 1, 2, are
-// complex matrix products with the particles as rows, nz as the inner
-// dimension (one 16-deep tile) and the columns (nx, ny) as columns:
+// complex matrix products with the particles as rows, (nz, Re/Im) as the
+// inner dimension (one 32-deep tile) and the columns (nx, ny) as columns:
 // A operand e_z of the particle (split fp16, built once), B operand
-// nz^l A (split fp16, written by k_recip_reduce).  Each product is
-// hi*hi + hi*lo + lo*hi in v_mfma_f32_16x16x16f16 with FP32 accumulation.
+// nz^l A (split fp16, written by k_recip_reduce, paired as [Re ; -Im] and
+// [Im ; Re] here).  Each product is hi*hi + hi*lo + lo*hi in
+// v_mfma_f32_16x16x32_f16 with FP32 accumulation.
 // The rest is k_recip_force's per-column combination on the vector ALUs:
 // x = e_x e_y, and the force and field sums from x S_l, in FP32 over the
 // column tiles a lane visits and FP64 from there on.  The operands of column
@@ -2171,7 +2180,8 @@ __global__ void __launch_bounds__(256 * kBpGroups) k_recip_force_mfma(State s, c
       ty[wv][p][n] = vy;
     }
   // A operand: lane (li, lq) holds e_z(nz = 4 lq + j) of the wave's particle li
-  half4_t arh, arl, aih, ail, naih, nail;
+  // (real parts, then imaginary parts: the inner dimension is (nz, Re/Im))
+  half4_t arh, arl, aih, ail;
   {
     const bool ok = pw + li < tk.n;
     const int g = tk.g0 + pw + (ok ? li : 0);
@@ -2188,10 +2198,9 @@ __global__ void __launch_bounds__(256 * kBpGroups) k_recip_force_mfma(State s, c
       split_f16((double)ez.im, h, l);
       aih[j] = h;
       ail[j] = l;
-      naih[j] = -h;
-      nail[j] = -l;
     }
   }
+  const half8_t ah = cat8(arh, aih), al = cat8(arl, ail);
   // the lane's output particles: 4 lq + j of the wave
   float c0[4], c1[4], c2z[4];
   const float P2x = (float)Rp.P2[0], P2y = (float)Rp.P2[1], P2z = (float)Rp.P2[2];
@@ -2225,21 +2234,9 @@ __global__ void __launch_bounds__(256 * kBpGroups) k_recip_force_mfma(State s, c
       for (int l = 0; l < 3; ++l) {
         const half4_t brh = sB[grp][buf][4 * l + 0][lane], brl = sB[grp][buf][4 * l + 1][lane];
         const half4_t bih = sB[grp][buf][4 * l + 2][lane], bil = sB[grp][buf][4 * l + 3][lane];
-        float4_t r = {0.f, 0.f, 0.f, 0.f}, i = {0.f, 0.f, 0.f, 0.f};
-        r = __builtin_amdgcn_mfma_f32_16x16x16f16(arh, brl, r, 0, 0, 0);
-        r = __builtin_amdgcn_mfma_f32_16x16x16f16(arl, brh, r, 0, 0, 0);
-        r = __builtin_amdgcn_mfma_f32_16x16x16f16(naih, bil, r, 0, 0, 0);
-        r = __builtin_amdgcn_mfma_f32_16x16x16f16(nail, bih, r, 0, 0, 0);
-        r = __builtin_amdgcn_mfma_f32_16x16x16f16(arh, brh, r, 0, 0, 0);
-        r = __builtin_amdgcn_mfma_f32_16x16x16f16(naih, bih, r, 0, 0, 0);
-        i = __builtin_amdgcn_mfma_f32_16x16x16f16(arh, bil, i, 0, 0, 0);
-        i = __builtin_amdgcn_mfma_f32_16x16x16f16(arl, bih, i, 0, 0, 0);
-        i = __builtin_amdgcn_mfma_f32_16x16x16f16(aih, brl, i, 0, 0, 0);
-        i = __builtin_amdgcn_mfma_f32_16x16x16f16(ail, brh, i, 0, 0, 0);
-        i = __builtin_amdgcn_mfma_f32_16x16x16f16(arh, bih, i, 0, 0, 0);
-        i = __builtin_amdgcn_mfma_f32_16x16x16f16(aih, brh, i, 0, 0, 0);
-        Sr[l] = r;
-        Si[l] = i;
+        const float4_t z = {0.f, 0.f, 0.f, 0.f};
+        Sr[l] = mfma_split3(ah, al, cat8(brh, -bih), cat8(brl, -bil), z);
+        Si[l] = mfma_split3(ah, al, cat8(bih, brh), cat8(bil, brl), z);
       }
       // the tile's scale factors (uniform), folded into the coefficients
       const float sc0 = scl[ct][0], sc1 = scl[ct][1], sc2 = scl[ct][2];
