@@ -1728,21 +1728,37 @@ __global__ void __launch_bounds__(256, 2) k_recip_struct_mfma(State s, int rt0, 
   int mrow[MW];
 #pragma unroll
   for (int u = 0; u < MW; ++u) mrow[u] = (4 * MW * blockIdx.y + MW * wv + u) * 16 + li;
-  for (int p0 = 0; p0 < rt.n; p0 += 64) {
-    const int np = min(64, rt.n - p0);
-    __syncthreads();
-    // consecutive lanes read consecutive particles of one table row
-    for (int q = threadIdx.x; q < 64 * 3 * KZ; q += 256) {
+  // the tables and amplitudes of the next 64 particles are loaded into
+  // registers while the current ones are in use; consecutive lanes read
+  // consecutive particles of one table row
+  constexpr int NQ = (64 * 3 * KZ + 255) / 256;
+  Cpx<float> pre[NQ];
+  float prea = 0.f;
+  auto fetch = [&](int q0) {
+    const int nq = min(64, rt.n - q0);
+#pragma unroll
+    for (int u = 0; u < NQ; ++u) {
+      const int q = threadIdx.x + 256 * u;
       const int p = q & 63, r = q >> 6, a = r / KZ, n = r % KZ;
       Cpx<float> v;
       v.re = v.im = 0.f;
-      if (p < np) v = ph[((size_t)a * kMaxKZ + n) * N + rt.g0 + p0 + p];
-      sh[p * 3 * KZ + r] = v;
+      if (q < 64 * 3 * KZ && p < nq) v = ph[((size_t)a * kMaxKZ + n) * N + rt.g0 + q0 + p];
+      pre[u] = v;
     }
-    for (int q = threadIdx.x; q < 64 * 3; q += 256) {
-      const int p = q / 3, a = q % 3;
-      sa[q] = p < np ? amp[a * N + rt.g0 + p0 + p] : 0.f;
+    const int p = threadIdx.x / 3, a = threadIdx.x % 3;
+    prea = (threadIdx.x < 64 * 3 && p < nq) ? amp[a * N + rt.g0 + q0 + p] : 0.f;
+  };
+  fetch(0);
+  for (int p0 = 0; p0 < rt.n; p0 += 64) {
+    const int np = min(64, rt.n - p0);
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < NQ; ++u) {
+      const int q = threadIdx.x + 256 * u;
+      if (q < 64 * 3 * KZ) sh[(q & 63) * 3 * KZ + (q >> 6)] = pre[u];
     }
+    if (threadIdx.x < 64 * 3) sa[threadIdx.x] = prea;
+    if (p0 + 64 < rt.n) fetch(p0 + 64);
     __syncthreads();
     // B = c_a e_x(nx) for the 4 steps (particles of padding have c = 0)
     for (int t = wv; t < NT; t += 4) {
@@ -1860,10 +1876,13 @@ __global__ void __launch_bounds__(256, 2) k_recip_struct_mfma(State s, int rt0, 
 // With bop (mixed contexts) the kernel also writes the operands of the
 // matrix-core back projection, B_l(col, nz) = 2^e nz^l A(col, nz) for
 // l = 0, 1, 2 with the column tile's scale exponent e = bexp[3 (col / 16) + l], formed
-// in FP64 and split into fp16 (hi, lo) pieces, in the lane layout of the
-// matrix operands: column tile ct = col / 16, piece
-// q = (re hi, re lo, im hi, im lo), lane (col % 16) + 16 (nz / 4), element
-// nz % 4 (k_recip_force_mfma pairs the pieces into 32-deep operands).
+// in FP64 and split into fp16 (hi, lo) pieces, as the 32-deep matrix
+// operands of k_recip_force_mfma: per column tile ct = col / 16 and weight l
+// four 8-wide operands q, [Re ; -Im] hi and lo, then [Im ; Re] hi and lo
+// (one for the real part of the product, one for the imaginary part), lane
+// (col % 16) + 16 (nz / 4), elements nz % 4 (first half) and 4 + nz % 4
+// (second half).  Entries of nz >= KZ and of columns past the last stay
+// zero from the allocation.
 template <typename R>
 __global__ void __launch_bounds__(512) k_recip_reduce(State s, const R *__restrict__ part, const double *__restrict__ M2,
                                                        const double *__restrict__ m2rows, Cpx<R> *__restrict__ Aout,
@@ -1973,15 +1992,19 @@ __global__ void __launch_bounds__(512) k_recip_reduce(State s, const R *__restri
 #pragma unroll
       for (int l = 0; l < 3; ++l) {
         const int e = bexp[Rp.bexp0 + 3 * ct + l];
-        const double v[2] = {ldexp(w * Ar, e), ldexp(w * Ai, e)};
-#pragma unroll
-        for (int c = 0; c < 2; ++c) {
-          const _Float16 h = (_Float16)(float)v[c];
-          const _Float16 lo = (_Float16)(float)(v[c] - (double)(float)h);
-          _Float16 *d = bop + 4 * (Rp.bop0 + ((long)ct * 12 + l * 4 + 2 * c) * 64 + bl) + be;
-          d[0] = h;
-          d[4 * 64] = lo;
-        }
+        _Float16 hi[2], lo[2];
+        split_f16(ldexp(w * Ar, e), hi[0], lo[0]);
+        split_f16(ldexp(w * Ai, e), hi[1], lo[1]);
+        // operands q = 0..3 of weight l, 8 fp16 per lane
+        _Float16 *d = bop + 4 * Rp.bop0 + 8 * (((long)ct * 12 + l * 4) * 64 + bl) + be;
+        d[0] = hi[0];
+        d[4] = -hi[1];
+        d[8 * 64] = lo[0];
+        d[8 * 64 + 4] = -lo[1];
+        d[16 * 64] = hi[1];
+        d[16 * 64 + 4] = hi[0];
+        d[24 * 64] = lo[1];
+        d[24 * 64 + 4] = lo[0];
         w *= (double)nz;
       }
     }
@@ -2124,15 +2147,24 @@ __global__ void __launch_bounds__(64 * kRfGroups) k_recip_force(State s, const C
 // k_recip_force, S_l(col) = sum_nz nz^l e_z(nz) A(col, nz), l = 0, 1, 2, are
 // complex matrix products with the particles as rows, (nz, Re/Im) as the
 // inner dimension (one 32-deep tile) and the columns (nx, ny) as columns:
-// A operand e_z of the particle (split fp16, built once), B operand
-// nz^l A (split fp16, written by k_recip_reduce, paired as [Re ; -Im] and
-// [Im ; Re] here).  Each product is hi*hi + hi*lo + lo*hi in
-// v_mfma_f32_16x16x32_f16 with FP32 accumulation.
-// The rest is k_recip_force's per-column combination on the vector ALUs:
-// x = e_x e_y, and the force and field sums from x S_l, in FP32 over the
-// column tiles a lane visits and FP64 from there on.  The operands of column
-// tile ct and weight l carry a factor 2^bexp[3 ct + l] (k_recip_reduce),
-// removed through the per-tile coefficients here.
+// A operand e_z of the particle (split fp16, built once), B operands
+// nz^l A as written by k_recip_reduce ([Re ; -Im] for the real part of the
+// product, [Im ; Re] for the imaginary part, hi and lo pieces).  Each
+// product is hi*hi + hi*lo + lo*hi in v_mfma_f32_16x16x32_f16 with FP32
+// accumulation.
+// The rest is on the vector ALUs.  With x = e_x e_y of the column,
+// I_l = Im(x S_l) and R_l = Re(x S_l), and c = mu u, the force
+// F = -sum_k k (c.k) Im(e^{-ik.r} A) is F_a = -sum_b c_b M_ab with the
+// symmetric sums M_ab = sum_k k_a k_b Im(...):
+//   M_xx = sum_col kx^2 I_0, M_xy = kx ky I_0, M_yy = ky^2 I_0,
+//   M_xz = kx P2z I_1, M_yz = ky P2z I_1, M_zz = P2z^2 I_2,
+// and the field H = sum_k k Re(...) is (kx R_0, ky R_0, P2z R_1) summed over
+// the columns.  The nine sums are FP32 over the column tiles a lane visits
+// and FP64 from there on.  The operands of column tile ct and weight l
+// carry a factor 2^bexp[3 ct + l] (k_recip_reduce), removed through the
+// per-tile coefficients.  e_x and e_y are staged in LDS for every n in
+// [-kmax, kmax] (conjugates for n < 0), so a column reads its factors
+// directly.
 // One block per wave task: 8 waves, two groups of 4; wave w of a group
 // takes the task's particles 16 w .. 16 w + 15, group g the column tiles
 // ct = g, g + 2, ...  A group's B operands of a column tile are staged in
@@ -2142,14 +2174,25 @@ constexpr int kBpGroups = 2;
 template <int KZ>
 __global__ void __launch_bounds__(256 * kBpGroups) k_recip_force_mfma(State s, const Cpx<float> *__restrict__ ph,
                                                                       const float *__restrict__ amp,
-                                                                      const half4_t *__restrict__ bop,
+                                                                      const half8_t *__restrict__ bop,
                                                                       const int *__restrict__ bexp) {
   constexpr int kmax = KZ - 1, side = 2 * kmax + 1, ncol = side * side, NCT = (ncol + 15) / 16;
   constexpr int NSTEP = (NCT + kBpGroups - 1) / kBpGroups;
-  __shared__ Cpx<float> tx[4][16][KZ], ty[4][16][KZ];
-  __shared__ half4_t sB[kBpGroups][2][12][64];
-  __shared__ float scl[NCT][3];
-  __shared__ double red[kBpGroups - 1][4][4][4][6];  // [group-1][wave][lq][j][value]
+  constexpr int NV = 9;  // M_xx, M_xy, M_yy, M_xz, M_yz, M_zz, H_x, H_y, H_z
+  constexpr int NW = 4 * kBpGroups;  // waves per block
+  // LDS: the loop's tables and operands, then (reusing the same bytes) the
+  // lanes' partial sums and the particles' totals
+  constexpr size_t oTy = sizeof(Cpx<float>) * 4 * 16 * side, oB = 2 * oTy;
+  constexpr size_t oScl = oB + sizeof(half8_t) * kBpGroups * 2 * 12 * 64, loopEnd = oScl + sizeof(float) * NCT * 3;
+  constexpr int kPs = 65;  // row stride of the partial sums (65: rows start on different banks)
+  constexpr size_t oTot = sizeof(float) * NW * 4 * NV * kPs, redEnd = oTot + sizeof(double) * 64 * NV;
+  __shared__ __align__(16) unsigned char lds[loopEnd > redEnd ? loopEnd : redEnd];
+  auto tx = reinterpret_cast<Cpx<float>(*)[16][side]>(lds);
+  auto ty = reinterpret_cast<Cpx<float>(*)[16][side]>(lds + oTy);
+  auto sB = reinterpret_cast<half8_t(*)[2][12][64]>(lds + oB);
+  auto scl = reinterpret_cast<float(*)[3]>(lds + oScl);
+  auto psum = reinterpret_cast<float(*)[4 * NV][kPs]>(lds);  // [wave][j * NV + value][lane]
+  auto tot = reinterpret_cast<double(*)[NV]>(lds + oTot);    // [particle][value]
   if ((int)blockIdx.x >= s.ntask) return;  // block-uniform
   const int tid = s.task_begin + blockIdx.x;
   const Task tk = s.tasks[tid];
@@ -2160,12 +2203,19 @@ __global__ void __launch_bounds__(256 * kBpGroups) k_recip_force_mfma(State s, c
   const int gt = threadIdx.x & 255;  // thread index within the group
   const int li = lane & 15, lq = lane >> 4;
   const int pw = wv * 16;  // the wave's first particle within the task
-  const half4_t *B = bop + Rp.bop0;
-  // each group's first column tile, every tile's scale factors
+  const half8_t *B = bop + Rp.bop0 / 2;
+  // each group's first column tile, every tile's scale factors; the
+  // group's second tile goes to registers (loads run two tiles ahead)
   if (grp < NCT)
     for (int q = gt; q < 12 * 64; q += 256) sB[grp][0][q >> 6][q & 63] = B[(size_t)grp * 12 * 64 + q];
+  half8_t nxt[3];
+  if (grp + kBpGroups < NCT) {
+#pragma unroll
+    for (int u = 0; u < 3; ++u) nxt[u] = B[(size_t)(grp + kBpGroups) * 12 * 64 + gt + 256 * u];
+  }
   for (int q = threadIdx.x; q < 3 * NCT; q += 256 * kBpGroups) scl[q / 3][q % 3] = ldexpf(1.f, -bexp[Rp.bexp0 + q]);
-  // e_x and e_y of the wave's 16 particles (group 0 loads them)
+  // e_x and e_y of the wave's 16 particles for n = -kmax..kmax (group 0
+  // loads them)
   if (grp == 0)
     for (int q = lane; q < 16 * KZ; q += 64) {
       const int p = q & 15, n = q >> 4;
@@ -2176,8 +2226,12 @@ __global__ void __launch_bounds__(256 * kBpGroups) k_recip_force_mfma(State s, c
         vx = ph[((size_t)0 * kMaxKZ + n) * N + g];
         vy = ph[((size_t)1 * kMaxKZ + n) * N + g];
       }
-      tx[wv][p][n] = vx;
-      ty[wv][p][n] = vy;
+      tx[wv][p][kmax + n] = vx;
+      ty[wv][p][kmax + n] = vy;
+      vx.im = -vx.im;
+      vy.im = -vy.im;
+      tx[wv][p][kmax - n] = vx;  // n = 0 writes the same value twice
+      ty[wv][p][kmax - n] = vy;
     }
   // A operand: lane (li, lq) holds e_z(nz = 4 lq + j) of the wave's particle li
   // (real parts, then imaginary parts: the inner dimension is (nz, Re/Im))
@@ -2201,76 +2255,72 @@ __global__ void __launch_bounds__(256 * kBpGroups) k_recip_force_mfma(State s, c
     }
   }
   const half8_t ah = cat8(arh, aih), al = cat8(arl, ail);
-  // the lane's output particles: 4 lq + j of the wave
-  float c0[4], c1[4], c2z[4];
   const float P2x = (float)Rp.P2[0], P2y = (float)Rp.P2[1], P2z = (float)Rp.P2[2];
-#pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    const int p = pw + 4 * lq + j;
-    const int g = tk.g0 + (p < tk.n ? p : 0);
-    c0[j] = amp[g];
-    c1[j] = amp[N + g];
-    c2z[j] = amp[2 * N + g] * P2z;
-  }
-  float part[4][6];
+  float part[4][NV];
 #pragma unroll
   for (int j = 0; j < 4; ++j)
 #pragma unroll
-    for (int q = 0; q < 6; ++q) part[j][q] = 0.f;
-  for (int st = 0; st < NSTEP; ++st) {
+    for (int q = 0; q < NV; ++q) part[j][q] = 0.f;
+  // the vector-ALU part of column tile ct from its products Sr, Si
+  auto combine = [&](const float4_t *Sr, const float4_t *Si, int ct) {
+    // lane: column col (past the last: a zero column, read as column 0),
+    // particles 4 lq + j; the tile's scale factors folded into the
+    // coefficients
+    const int col = ct * 16 + li;
+    const int cc = col < ncol ? col : 0;
+    const int iy = cc / side, ix = cc - iy * side;
+    const float kx = P2x * (float)(ix - kmax), ky = P2y * (float)(iy - kmax);
+    const float sc0 = scl[ct][0], sc1 = scl[ct][1], sc2 = scl[ct][2];
+    const float hx = kx * sc0, hy = ky * sc0, hz = P2z * sc1;
+    const float cxx = kx * hx, cxy = kx * hy, cyy = ky * hy;
+    const float cxz = kx * hz, cyz = ky * hz, czz = P2z * (P2z * sc2);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int p = 4 * lq + j;
+      const Cpx<float> exv = tx[wv][p][ix], ey = ty[wv][p][iy];
+      const float xr = fmaf(exv.re, ey.re, -exv.im * ey.im);
+      const float xi = fmaf(exv.re, ey.im, exv.im * ey.re);
+      const float r0 = fmaf(xr, Sr[0][j], -xi * Si[0][j]), i0 = fmaf(xr, Si[0][j], xi * Sr[0][j]);
+      const float r1 = fmaf(xr, Sr[1][j], -xi * Si[1][j]), i1 = fmaf(xr, Si[1][j], xi * Sr[1][j]);
+      const float i2 = fmaf(xr, Si[2][j], xi * Sr[2][j]);
+      part[j][0] = fmaf(cxx, i0, part[j][0]);
+      part[j][1] = fmaf(cxy, i0, part[j][1]);
+      part[j][2] = fmaf(cyy, i0, part[j][2]);
+      part[j][3] = fmaf(cxz, i1, part[j][3]);
+      part[j][4] = fmaf(cyz, i1, part[j][4]);
+      part[j][5] = fmaf(czz, i2, part[j][5]);
+      part[j][6] = fmaf(hx, r0, part[j][6]);
+      part[j][7] = fmaf(hy, r0, part[j][7]);
+      part[j][8] = fmaf(hz, r1, part[j][8]);
+    }
+  };
+  // Software pipeline: the matrix products of a step's tile are issued
+  // before the vector-ALU part of the previous step's tile, so the two
+  // units work at the same time; steps come in pairs so the products
+  // alternate between two register sets.
+  float4_t Pr[2][3], Pi[2][3];
+  int cprev = -1;  // the tile whose products are in the other set (-1: none)
+  auto step = [&](int st, int cur) {
     __syncthreads();  // this step's tiles staged; the other buffers free
     const int ct = st * kBpGroups + grp;
-    const int cn = ct + kBpGroups;  // the group's next tile
-    const bool have = ct < NCT, more = cn < NCT;
-    half4_t nxt[3];
-    if (more) {
+    const int cn = ct + kBpGroups;       // the group's next tile (in nxt)
+    const int cn2 = cn + kBpGroups;      // and the one after (loaded now)
+    const bool have = ct < NCT, more = cn < NCT, more2 = cn2 < NCT;
+    half8_t nxt2[3];
+    if (more2) {
 #pragma unroll
-      for (int u = 0; u < 3; ++u) nxt[u] = B[(size_t)cn * 12 * 64 + gt + 256 * u];
+      for (int u = 0; u < 3; ++u) nxt2[u] = B[(size_t)cn2 * 12 * 64 + gt + 256 * u];
     }
     const int buf = st & 1;
     if (have) {
-      float4_t Sr[3], Si[3];
+      const float4_t z = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
       for (int l = 0; l < 3; ++l) {
-        const half4_t brh = sB[grp][buf][4 * l + 0][lane], brl = sB[grp][buf][4 * l + 1][lane];
-        const half4_t bih = sB[grp][buf][4 * l + 2][lane], bil = sB[grp][buf][4 * l + 3][lane];
-        const float4_t z = {0.f, 0.f, 0.f, 0.f};
-        Sr[l] = mfma_split3(ah, al, cat8(brh, -bih), cat8(brl, -bil), z);
-        Si[l] = mfma_split3(ah, al, cat8(bih, brh), cat8(bil, brl), z);
-      }
-      // the tile's scale factors (uniform), folded into the coefficients
-      const float sc0 = scl[ct][0], sc1 = scl[ct][1], sc2 = scl[ct][2];
-      // lane: column col, particles 4 lq + j
-      const int col = ct * 16 + li;
-      const int cc = col < ncol ? col : 0;
-      const int iy = cc / side, ix = cc - iy * side;
-      const int nx = ix - kmax, ny = iy - kmax;
-      const int ax = nx < 0 ? -nx : nx, ay = ny < 0 ? -ny : ny;
-      const float kx = P2x * (float)nx, ky = P2y * (float)ny;
-      const float kx0 = kx * sc0, ky0 = ky * sc0, kz1 = P2z * sc1;
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int p = 4 * lq + j;
-        Cpx<float> exv = tx[wv][p][ax], ey = ty[wv][p][ay];
-        if (nx < 0) exv.im = -exv.im;
-        if (ny < 0) ey.im = -ey.im;
-        const float xr = fmaf(exv.re, ey.re, -exv.im * ey.im);
-        const float xi = fmaf(exv.re, ey.im, exv.im * ey.re);
-        // x S_l of the scaled sums
-        const float r0 = fmaf(xr, Sr[0][j], -xi * Si[0][j]), i0 = fmaf(xr, Si[0][j], xi * Sr[0][j]);
-        const float r1 = fmaf(xr, Sr[1][j], -xi * Si[1][j]), i1 = fmaf(xr, Si[1][j], xi * Sr[1][j]);
-        const float i2 = fmaf(xr, Si[2][j], xi * Sr[2][j]);
-        const float pcol = fmaf(c0[j], kx, c1[j] * ky);
-        const float Gc = fmaf(c2z[j] * sc1, i1, (pcol * sc0) * i0);
-        const float Gz = P2z * fmaf(c2z[j] * sc2, i2, (pcol * sc1) * i1);
-        part[j][0] = fmaf(Gc, kx, part[j][0]);
-        part[j][1] = fmaf(Gc, ky, part[j][1]);
-        part[j][2] += Gz;
-        part[j][3] = fmaf(r0, kx0, part[j][3]);
-        part[j][4] = fmaf(r0, ky0, part[j][4]);
-        part[j][5] = fmaf(r1, kz1, part[j][5]);
+        Pr[cur][l] = mfma_split3(ah, al, sB[grp][buf][4 * l + 0][lane], sB[grp][buf][4 * l + 1][lane], z);
+        Pi[cur][l] = mfma_split3(ah, al, sB[grp][buf][4 * l + 2][lane], sB[grp][buf][4 * l + 3][lane], z);
       }
     }
+    if (cprev >= 0) combine(Pr[cur ^ 1], Pi[cur ^ 1], cprev);
     if (more) {
 #pragma unroll
       for (int u = 0; u < 3; ++u) {
@@ -2278,48 +2328,61 @@ __global__ void __launch_bounds__(256 * kBpGroups) k_recip_force_mfma(State s, c
         sB[grp][buf ^ 1][q >> 6][q & 63] = nxt[u];
       }
     }
+#pragma unroll
+    for (int u = 0; u < 3; ++u) nxt[u] = nxt2[u];
+    cprev = have ? ct : -1;
+  };
+  int st = 0;
+  for (; st + 1 < NSTEP; st += 2) {
+    step(st, 0);
+    step(st + 1, 1);
   }
-  // sums over the 16 columns a lane group holds (fixed butterfly order, FP64)
-  double acc[4][6];
+  if (st < NSTEP) step(st, 0);
+  if (cprev >= 0) combine(Pr[(NSTEP - 1) & 1], Pi[(NSTEP - 1) & 1], cprev);
+  // Totals per particle: the lanes' FP32 partial sums go to LDS (over the
+  // loop's tables), then each (particle, value) is summed in FP64 in a
+  // fixed order: group 0's 16 lanes, then group 1's.
+  __syncthreads();
 #pragma unroll
   for (int j = 0; j < 4; ++j)
 #pragma unroll
-    for (int q = 0; q < 6; ++q) {
-      double v = (double)part[j][q];
-      for (int o = 1; o < 16; o <<= 1) v += __shfl_xor(v, o);
-      acc[j][q] = v;
-    }
-  // the groups' sums meet in group 0, in group order
-  if (grp > 0 && li == 0)
-#pragma unroll
-    for (int j = 0; j < 4; ++j)
-#pragma unroll
-      for (int q = 0; q < 6; ++q) red[grp - 1][wv][lq][j][q] = acc[j][q];
+    for (int q = 0; q < NV; ++q) psum[wave][j * NV + q][lane] = part[j][q];
   __syncthreads();
-  if (grp != 0 || li != 0) return;
+  for (int o = threadIdx.x; o < 64 * NV; o += 256 * kBpGroups) {
+    const int p = o / NV, q = o - p * NV;  // particle p of the task
+    const int w4 = p >> 4, pl = p & 15, lq4 = pl >> 2, j = pl & 3;
+    double t = 0.0;
+    for (int g2 = 0; g2 < kBpGroups; ++g2) {
+      const float *src = &psum[4 * g2 + w4][j * NV + q][16 * lq4];
 #pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    const int p = pw + 4 * lq + j;
-    if (p >= tk.n) continue;
-#pragma unroll
-    for (int q = 0; q < 6; ++q)
-      for (int g2 = 1; g2 < kBpGroups; ++g2) acc[j][q] += red[g2 - 1][wv][lq][j][q];
-    const int g = tk.g0 + p;
-    double F[3] = {-acc[j][0], -acc[j][1], -acc[j][2]};
-    double H[3] = {acc[j][3], acc[j][4], acc[j][5]};
-    double mu = s.dmu[g];
-    double u[3] = {s.dori[g], s.dori[N + g], s.dori[2 * N + g]};
-    double sd = s.dpc[g];
-    const double *G = Rp.G;
-    double Gu[3] = {G[0] * u[0] + G[1] * u[1] + G[2] * u[2], G[3] * u[0] + G[4] * u[1] + G[5] * u[2],
-                    G[6] * u[0] + G[7] * u[1] + G[8] * u[2]};
-    double w[3];
-    for (int a = 0; a < 3; ++a) w[a] = -mu * H[a] + mu * mu * Gu[a] + sd * F[a];
-    double t[3];
-    cross3(u, w, t);
-    for (int a = 0; a < 3; ++a) {
-      s.F[a * N + g] += F[a];
-      s.T[a * N + g] += t[a];
+      for (int l2 = 0; l2 < 16; ++l2) t += (double)src[l2];
+    }
+    tot[p][q] = t;
+  }
+  __syncthreads();
+  if (threadIdx.x < 64) {
+    const int p = threadIdx.x;
+    if (p < tk.n) {
+      const int g = tk.g0 + p;
+      double mu = s.dmu[g];
+      double u[3] = {s.dori[g], s.dori[N + g], s.dori[2 * N + g]};
+      const double c[3] = {mu * u[0], mu * u[1], mu * u[2]};
+      const double *M = tot[p];
+      double F[3] = {-(c[0] * M[0] + c[1] * M[1] + c[2] * M[3]), -(c[0] * M[1] + c[1] * M[2] + c[2] * M[4]),
+                     -(c[0] * M[3] + c[1] * M[4] + c[2] * M[5])};
+      double H[3] = {M[6], M[7], M[8]};
+      double sd = s.dpc[g];
+      const double *G = Rp.G;
+      double Gu[3] = {G[0] * u[0] + G[1] * u[1] + G[2] * u[2], G[3] * u[0] + G[4] * u[1] + G[5] * u[2],
+                      G[6] * u[0] + G[7] * u[1] + G[8] * u[2]};
+      double w[3];
+      for (int a = 0; a < 3; ++a) w[a] = -mu * H[a] + mu * mu * Gu[a] + sd * F[a];
+      double t[3];
+      cross3(u, w, t);
+      for (int a = 0; a < 3; ++a) {
+        s.F[a * N + g] += F[a];
+        s.T[a * N + g] += t[a];
+      }
     }
   }
 }
@@ -3561,7 +3624,7 @@ static void recip_force_kz(const State &s, const void *ph, const void *amp, cons
                            const int *bexp, hipStream_t st) {
   if constexpr (sizeof(R) == sizeof(float)) {
     MSC_LAUNCH((k_recip_force_mfma<KZ>), dim3(s.ntask), dim3(256 * kBpGroups), 0, st, s, (const Cpx<float> *)ph,
-               (const float *)amp, (const half4_t *)bop, bexp);
+               (const float *)amp, (const half8_t *)bop, bexp);
   } else {
     MSC_LAUNCH((k_recip_force<R, KZ>), dim3(s.ntask), dim3(64 * kRfGroups), 0, st, s, (const Cpx<R> *)ph,
                (const R *)amp, (const Cpx<R> *)A);

@@ -107,12 +107,13 @@ int launch_recip_struct(const State &s, bool mixed, int KZ, int rt0, int nrt, in
 // m2rows; kblk (may be null): per-block sums of the 10 slot values,
 // nblk(maxslot, 64) rows per replica
 // bop (mixed contexts): the operands of the matrix-core back projection,
-// kBopPerTile groups of 4 fp16 values per column tile of 16 (RepDev::bop0),
-// scaled by 2^bexp[3 col + l] (RepDev::bexp0)
+// kBopPerTile groups of 4 fp16 values per column tile of 16 (RepDev::bop0,
+// a multiple of 2: the kernels read them as groups of 8), scaled by
+// 2^bexp[3 col + l] (RepDev::bexp0)
 void launch_recip_reduce(const State &s, bool mixed, const void *part, const double *M2, const double *m2rows,
                          void *A, double *slot_out, double *kblk, int maxslot, void *bop, const int *bexp,
                          hipStream_t st);
-constexpr int kBopPerTile = 12 * 64;  // 3 weights x 4 pieces x 64 lanes
+constexpr int kBopPerTile = 24 * 64;  // 3 weights x 4 operands x 64 lanes x 8 fp16
 int launch_recip_force(const State &s, bool mixed, int KZ, const void *ph, const void *amp, const void *A,
                        const void *bop, const int *bexp, hipStream_t st);
 // Reciprocal sum of contexts with several dipole slots per particle, FP64:

@@ -75,7 +75,7 @@ struct Block {
   std::unique_ptr<std::barrier<>> bar;
   std::vector<std::unique_ptr<std::barrier<>>> wbar;
   std::vector<double> xch;  // [thread] exchange slots for shuffles
-  std::vector<float> mxa, mxb;  // [thread][4] operand exchange of matrix instructions
+  std::vector<float> mxa, mxb;  // [thread][8] operand exchange of matrix instructions
 };
 inline Block *&cur_block() { static Block *b = nullptr; return b; }
 inline thread_local dim3 t_threadIdx, t_blockIdx, t_blockDim, t_gridDim;
@@ -156,7 +156,7 @@ typedef float hipemu_float4 __attribute__((ext_vector_type(4)));
 inline hipemu_float4 hipemu_mfma16(hipemu_half4 a, hipemu_half4 b, hipemu_float4 c) {
   hipemu::Block *bk = hipemu::cur_block();
   const unsigned t = hipemu::t_threadIdx.x, w = t / 64, lane = t % 64;
-  float *xa = bk->mxa.data() + w * 256, *xb = bk->mxb.data() + w * 256;
+  float *xa = bk->mxa.data() + w * 512, *xb = bk->mxb.data() + w * 512;
   for (int j = 0; j < 4; ++j) {
     xa[lane * 4 + j] = (float)a[j];
     xb[lane * 4 + j] = (float)b[j];
@@ -173,6 +173,30 @@ inline hipemu_float4 hipemu_mfma16(hipemu_half4 a, hipemu_half4 b, hipemu_float4
   return d;
 }
 #define __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, cbsz, abid, blgp) hipemu_mfma16(a, b, c)
+// v_mfma_f32_16x16x32_f16: A[i][k] in lane i + 16 (k/8), element k % 8; B[k][n]
+// in lane n + 16 (k/8); D as above.  (The hardware's inner-index order may
+// differ; a product does not depend on it when A and B share it.)
+typedef _Float16 hipemu_half8 __attribute__((ext_vector_type(8)));
+inline hipemu_float4 hipemu_mfma32(hipemu_half8 a, hipemu_half8 b, hipemu_float4 c) {
+  hipemu::Block *bk = hipemu::cur_block();
+  const unsigned t = hipemu::t_threadIdx.x, w = t / 64, lane = t % 64;
+  float *xa = bk->mxa.data() + w * 512, *xb = bk->mxb.data() + w * 512;
+  for (int j = 0; j < 8; ++j) {
+    xa[lane * 8 + j] = (float)a[j];
+    xb[lane * 8 + j] = (float)b[j];
+  }
+  bk->wbar[w]->arrive_and_wait();
+  hipemu_float4 d;
+  for (int j = 0; j < 4; ++j) {
+    const int i = 4 * (lane / 16) + j, n = lane % 16;
+    float acc = c[j];
+    for (int k = 0; k < 32; ++k) acc += xa[(i + 16 * (k / 8)) * 8 + k % 8] * xb[(n + 16 * (k / 8)) * 8 + k % 8];
+    d[j] = acc;
+  }
+  bk->wbar[w]->arrive_and_wait();
+  return d;
+}
+#define __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, cbsz, abid, blgp) hipemu_mfma32(a, b, c)
 #endif
 #define __builtin_amdgcn_fence(...) ((void)0)
 inline void __threadfence() { std::atomic_thread_fence(std::memory_order_seq_cst); }
@@ -324,8 +348,8 @@ void launch(F f, dim3 grid, dim3 block, size_t shm, hipStream_t st, A... args) {
         unsigned nw = (nt + 63) / 64;
         for (unsigned w = 0; w < nw; ++w) B.wbar.push_back(std::make_unique<std::barrier<>>(std::min(64u, nt - 64 * w)));
         B.xch.assign(nw * 64, 0.0);
-        B.mxa.assign(nw * 64 * 4, 0.f);
-        B.mxb.assign(nw * 64 * 4, 0.f);
+        B.mxa.assign(nw * 64 * 8, 0.f);
+        B.mxb.assign(nw * 64 * 8, 0.f);
         cur_block() = &B;
         std::vector<std::thread> th;
         th.reserve(nt);
