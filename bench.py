@@ -92,8 +92,8 @@ def timed_run(sysm, steps, warmup, timing, step0=0):
 def autotune(sysm, name, steps=100):
     best, tried = sysm.autotune(steps)
     log("bench: autotune %s: %s" % (name, ", ".join(
-        "graph %d fused %d waves %d skin %.2f chunk %d %.4f ms" % (c["graph"], c["fused_pairs"], c["pair_waves"],
-                                                                    c["list_skin"], c["chunk"], t)
+        "graph %d fused %d waves %d skin %.2f chunk %d overlap %d %.4f ms" % (
+            c["graph"], c["fused_pairs"], c["pair_waves"], c["list_skin"], c["chunk"], c["overlap"], t)
         for c, t in tried)))
     log("bench: autotune %s chose %s" % (name, best))
     return {"choice": best, "steps": steps,

@@ -166,7 +166,7 @@ int msc_set_graph(msc_ctx *ctx, int on);
  * fused_pairs give the same results bit for bit; pair_waves changes how the
  * row entries of a 64-particle task are split over waves, so the rounding of
  * the real-space and WCA sums (each choice is deterministic).  Defaults:
- * graph 0, fused_pairs 1, pair_waves 4. */
+ * graph 0, fused_pairs 1, pair_waves 4, overlap 1. */
 typedef struct msc_tuning {
   int graph;        /* msc_run replays each step as one captured HIP graph */
   int fused_pairs;  /* MSC_MIXED contexts with contact rows whose replicas all
@@ -187,6 +187,12 @@ typedef struct msc_tuning {
                        sum (64, 128, 256 or 512; default 128): fewer, larger
                        chunks write and re-read fewer partial sums.  Changes
                        the summation order of S(k) only. */
+  int overlap;      /* 1: the fused pair launch runs on a second stream of the
+                       device, concurrently with the structure factors and
+                       their reduction, and is joined before the back
+                       projection (each writes its own outputs, so results
+                       are the same bit for bit); 0: one stream.  Runs with
+                       timing on or MSC_SYNC_CHECK set use one stream. */
 } msc_tuning;
 int msc_get_tuning(msc_ctx *ctx, msc_tuning *t);
 int msc_set_tuning(msc_ctx *ctx, const msc_tuning *t);

@@ -98,6 +98,11 @@ struct msc_ctx {
   // launch choices (msc_tuning): the fused mixed pair launch and its waves per task
   bool fused_pairs = true;
   int pair_waves = 4;
+  // msc_tuning.overlap: the fused pair launch on stream2, forked after the
+  // drift (ev_fork) and joined before the back projection (ev_join)
+  bool overlap = true;
+  hipStream_t stream2 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   // MSC_MIXED: the skin of the context's own rows (0: the reference's rows
   // and rebuild test, contact rows refreshed from them); per replica the
   // smallest diameter and largest length + diameter the radii derive from
@@ -183,6 +188,24 @@ static constexpr size_t kPinBytes = (size_t)4 << 20;
 // ends: contexts on a device share it, so creating and destroying contexts
 // never creates or destroys streams.
 static hipError_t device_stream(int device, hipStream_t *out) {
+  static std::mutex mu;
+  static std::map<int, hipStream_t> streams;
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = streams.find(device);
+  if (it != streams.end()) {
+    *out = it->second;
+    return hipSuccess;
+  }
+  hipStream_t st = nullptr;
+  hipError_t e = hipStreamCreate(&st);
+  if (e == hipSuccess) streams[device] = st;
+  *out = st;
+  return e;
+}
+
+// The second stream of a device (the pair launch of an overlapped step; see
+// msc_tuning.overlap), shared by the contexts on the device, default priority.
+static hipError_t device_stream2(int device, hipStream_t *out) {
   static std::mutex mu;
   static std::map<int, hipStream_t> streams;
   std::lock_guard<std::mutex> lock(mu);
@@ -875,8 +898,13 @@ int msc_create(int device, int precision, int nrep, const msc_replica_desc *reps
   {
     hipError_t e1 = hipSetDevice(device);
     hipError_t e2 = e1 == hipSuccess ? device_stream(device, &ctx->stream) : e1;
+    if (e2 == hipSuccess) e2 = device_stream2(device, &ctx->stream2);
+    if (e2 == hipSuccess) e2 = hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming);
+    if (e2 == hipSuccess) e2 = hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming);
     if (e2 != hipSuccess) {
-      g_err = std::string(e1 != hipSuccess ? "hipSetDevice: " : "hipStreamCreate: ") + hipGetErrorString(e2);
+      g_err = std::string(e1 != hipSuccess ? "hipSetDevice: " : "stream or event creation: ") + hipGetErrorString(e2);
+      if (ctx->ev_fork) hipEventDestroy(ctx->ev_fork);
+      if (ctx->ev_join) hipEventDestroy(ctx->ev_join);
       delete ctx;
       return MSC_E_HIP;
     }
@@ -1057,8 +1085,11 @@ int msc_create(int device, int precision, int nrep, const msc_replica_desc *reps
 void msc_destroy(msc_ctx *ctx) {
   if (!ctx) return;
   if (ctx->stream) hipStreamSynchronize(ctx->stream);
+  if (ctx->stream2) hipStreamSynchronize(ctx->stream2);
   hipDeviceSynchronize();
   if (ctx->graph_exec) hipGraphExecDestroy(ctx->graph_exec);
+  if (ctx->ev_fork) hipEventDestroy(ctx->ev_fork);
+  if (ctx->ev_join) hipEventDestroy(ctx->ev_join);
   void *bufs[] = {ctx->d_reps, ctx->d_tasks, ctx->d_rtasks, ctx->d_rep_of, ctx->d_cell_rep, ctx->d_x, ctx->d_p,
                   ctx->d_l, ctx->d_ori, ctx->d_F, ctx->d_T, ctx->d_len, ctx->d_dia, ctx->d_mass, ctx->d_inert,
                   ctx->d_dpc, ctx->d_doc, ctx->d_dmu, ctx->d_dpos, ctx->d_dori, ctx->d_snap, ctx->d_cell_of,
@@ -1536,7 +1567,7 @@ int msc_force_dipR(msc_ctx *ctx, int rep, double *U, double PT[9]) {
 // Reciprocal launches.  Standalone (m2rows null): phase tables, M2 and the
 // per-replica energy sums here.  In the step loop the tables and the c c^T
 // rows come from k_drift (m2rows) and the energies go to kblk for k_end.
-static int dipK_launches(msc_ctx *ctx, int rep, const double *m2rows, double *kblk) {
+static int dipK_launches(msc_ctx *ctx, int rep, const double *m2rows, double *kblk, bool join = false) {
   State s = sel(ctx, rep);
   if (ctx->nd > 1) {
     // several slots per particle: intra-particle exclusion by the particles'
@@ -1583,6 +1614,9 @@ static int dipK_launches(msc_ctx *ctx, int rep, const double *m2rows, double *kb
   launch_recip_reduce(s, ctx->mixed, ctx->d_part, m2rows ? nullptr : ctx->d_M2, m2rows, ctx->d_A, ctx->d_slot_out,
                       kblk, ctx->maxslot, ctx->d_bop, ctx->d_bexp, ctx->stream);
   if (!kblk) launch_reduce(s, ctx->d_slot_out, 10, 10, 1, (double *)ctx->d_E, kEstride, kEoff_K, ctx->stream);
+  // an overlapped step's pair launch adds into F and T as the back
+  // projection does: it must have finished first
+  if (join) note_stream_op("join of the pair stream", hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0));
   for (int kz : ctx->kz_groups) {
     for (int r = 0; r < ctx->nrep; ++r) {
       if (ctx->reps[r].KZ != kz || (rep != MSC_ALL && r != rep)) continue;
@@ -1801,6 +1835,7 @@ int msc_get_tuning(msc_ctx *ctx, msc_tuning *t) {
   t->pair_waves = ctx->pair_waves;
   t->list_skin = ctx->list_skin;
   t->chunk = ctx->rchunk;
+  t->overlap = ctx->overlap ? 1 : 0;
   return MSC_OK;
 }
 
@@ -1816,6 +1851,7 @@ int msc_set_tuning(msc_ctx *ctx, const msc_tuning *t) {
   ctx->graph_on = t->graph != 0;
   ctx->fused_pairs = t->fused_pairs != 0;
   ctx->pair_waves = t->pair_waves;
+  ctx->overlap = t->overlap != 0;
   if (t->list_skin != ctx->list_skin) {
     // other rows: msc_init_run (or the autotuner) builds them again
     ctx->list_skin = t->list_skin;
@@ -1851,11 +1887,23 @@ static void enqueue_step(msc_ctx *ctx) {
   }
   bool cut_all = true;
   for (const RepDev &R : ctx->reps) cut_all &= R.rc > 0;
-  if (ctx->mixed && ctx->contact && cut_all && ctx->fused_pairs) {
+  const bool fused = ctx->mixed && ctx->contact && cut_all && ctx->fused_pairs;
+  // the pair launch on the second stream, concurrent with the structure
+  // factors and their reduction (they read the drift's outputs only and
+  // write buffers of their own); joined before the back projection
+  const bool overlap = fused && ctx->overlap && ctx->nd == 1 && !ctx->timing && !launches_checked();
+  if (fused) {
     // WCA over the contact rows and real-space pairs in one launch
     Bracket b(ctx, "real_pairs");
+    hipStream_t sp = st;
+    if (overlap) {
+      note_stream_op("fork of the pair stream", hipEventRecord(ctx->ev_fork, st));
+      note_stream_op("fork of the pair stream", hipStreamWaitEvent(ctx->stream2, ctx->ev_fork, 0));
+      sp = ctx->stream2;
+    }
     launch_pairs_mixed(s, ctx->pair_waves, ctx->d_tab, rows + kRowSC * region, rows + kRowR * region,
-                       ctx->d_flags, ls, st);
+                       ctx->d_flags, ls, sp);
+    if (overlap) note_stream_op("join of the pair stream", hipEventRecord(ctx->ev_join, ctx->stream2));
   } else {
     {
       Bracket b(ctx, "wca");
@@ -1863,7 +1911,7 @@ static void enqueue_step(msc_ctx *ctx) {
     }
     dipR_launches(ctx, MSC_ALL, rows + kRowR * region);
   }
-  dipK_launches(ctx, MSC_ALL, rows + kRowDrift * region + 2, ctx->d_kblk);
+  dipK_launches(ctx, MSC_ALL, rows + kRowDrift * region + 2, ctx->d_kblk, overlap);
   // ext_field adds exactly nothing without a field: skipped then
   const bool field = any_field(ctx);
   if (field) launch_ext_field(s, rows + kRowExt * region, st);
@@ -2118,6 +2166,12 @@ int msc_autotune(msc_ctx *ctx, int steps, msc_tuning *best, msc_tuning *tried, d
         c.pair_waves = w;
         if (int rc_ = trial(c)) return rc_;
       }
+  }
+  if (fusable) {
+    // the pair launch on its own stream or not, with the fastest so far
+    msc_tuning c = win;
+    c.overlap = start.overlap ? 0 : 1;
+    if (int rc_ = trial(c)) return rc_;
   }
   if (ctx->mixed) {
     const msc_tuning launch = win;

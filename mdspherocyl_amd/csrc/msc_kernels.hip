@@ -66,6 +66,22 @@ static void note_launch(const char *kernel) {
     g_shared_record.kernel = kernel;
   }
 }
+void note_stream_op(const char *what, hipError_t e) {
+  if (e == hipSuccess) return;
+  (void)hipGetLastError();  // the same failure, already taken here
+  if (t_record) {
+    if (t_record->err == hipSuccess) {
+      t_record->err = e;
+      t_record->kernel = what;
+    }
+    return;
+  }
+  std::lock_guard<std::mutex> lk(g_shared_mu);
+  if (g_shared_record.err == hipSuccess) {
+    g_shared_record.err = e;
+    g_shared_record.kernel = what;
+  }
+}
 hipError_t take_launch_error(const char **kernel) {
   LaunchRecord *r = t_record;
   std::unique_lock<std::mutex> lk(g_shared_mu, std::defer_lock);

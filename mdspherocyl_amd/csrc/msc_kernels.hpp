@@ -73,6 +73,9 @@ LaunchRecord *set_launch_record(LaunchRecord *r);
 hipError_t take_launch_error(const char **kernel);
 // MSC_SYNC_CHECK is set: every launch is followed by a synchronisation
 bool launches_checked();
+// records a failed stream operation (event record or wait) like a failed
+// launch, under the name what
+void note_stream_op(const char *what, hipError_t e);
 
 void launch_dipoles(const State &s, hipStream_t st);
 // zero: clear every cell counter first (they are zero outside a rebuild:

@@ -209,7 +209,7 @@ class MdSystem:
         self._chk(self.L.msc_get_tuning(self._ctx, ctypes.byref(t)))
         return t.as_dict()
 
-    def set_tuning(self, graph=None, fused_pairs=None, pair_waves=None, list_skin=None, chunk=None):
+    def set_tuning(self, graph=None, fused_pairs=None, pair_waves=None, list_skin=None, chunk=None, overlap=None):
         t = _lib.Tuning()
         self._chk(self.L.msc_get_tuning(self._ctx, ctypes.byref(t)))
         if graph is not None:
@@ -222,6 +222,8 @@ class MdSystem:
             t.list_skin = float(list_skin)
         if chunk is not None:
             t.chunk = int(chunk)
+        if overlap is not None:
+            t.overlap = int(bool(overlap))
         self._chk(self.L.msc_set_tuning(self._ctx, ctypes.byref(t)))
 
     def autotune(self, steps=100):

@@ -66,6 +66,23 @@ def test_fused_pairs_equal_separate_launches(msc):
         assert np.array_equal(x, y)
 
 
+@pytest.mark.parametrize("graph", [0, 1])
+def test_overlapped_pair_stream_equals_one_stream(msc, graph):
+    """The pair launch on a second stream, concurrent with the structure
+    factors and their reduction, gives the one-stream step bit for bit
+    (direct launches and the captured graph): 130 steps, observables, state
+    and forces."""
+    _say("case: overlapped vs one stream, graph %d, mixed, 2 x N=400, 130 steps" % graph)
+    a = _run(msc, dict(fused_pairs=1, pair_waves=4, graph=graph, overlap=1))
+    b = _run(msc, dict(fused_pairs=1, pair_waves=4, graph=graph, overlap=0))
+    assert a[4]["overlap"] == 1 and b[4]["overlap"] == 0
+    assert np.array_equal(a[0], b[0])
+    for x, y in zip(a[1], b[1]):
+        assert np.array_equal(x, y)
+    for x, y in zip(a[2], b[2]):
+        assert np.array_equal(x, y)
+
+
 def test_own_rows_follow_reference_rows(msc):
     """The context's own rows (list skin 1.0 and 0.5) against the
     reference's rows and rebuild test (list skin 0): the same pairs interact,
@@ -128,9 +145,10 @@ def test_autotune_keeps_the_run(msc):
     best, tried = b[3]
     _say("  tried: %s" % ", ".join("%s %.4f ms" % (c, t) for c, t in tried))
     _say("  chose: %s" % best)
-    # 8 launch choices, the three list skins and three chunks other than the
-    # defaults
-    assert len(tried) == 14 and all(t > 0 for _, t in tried)
+    # 8 launch choices, the other overlap choice, the three list skins and
+    # three chunks other than the defaults
+    assert len(tried) == 15 and all(t > 0 for _, t in tried)
+    assert sorted({c["overlap"] for c, _ in tried}) == [0, 1]
     assert sorted({c["list_skin"] for c, _ in tried}) == [0.5, 0.75, 1.0, 1.5]
     assert sorted({c["chunk"] for c, _ in tried}) == [64, 128, 256, 512]
     assert best in [c for c, _ in tried]
