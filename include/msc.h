@@ -287,6 +287,16 @@ int msc_ini_state(int N, int nspec, const int *counts, const double L[3], const 
  * FP64 on the host: G1 and G2 differ from the reference's single-precision Q
  * by its rounding (about 1e-6 at N = 500). */
 int msc_order_params(msc_ctx *ctx, int rep, double *G1, double *G2, double M[3]);
+/* oo_params (v25/functions.cpp:2702-2760) computed as the reference computes
+ * it, for output-file parity (counters.dat, magmom.dat): the Q tensor of
+ * Qmat_3d (3426-3453) in single precision summed in particle order, the
+ * single-precision jacobi rotations (v25/jacobi.c:12-80, as compiled by the
+ * reference's C++ build), and G1 and M as double sums in particle order.
+ * Given the same state it returns the reference's numbers bit for bit.  One
+ * wave adds the terms in order: meant for output files, not for speed.  At
+ * most 4 dipole slots per particle; MSC_E_ARG if the rotations do not
+ * converge in 50 sweeps (as jacobi gives up). */
+int msc_order_params_ref(msc_ctx *ctx, int rep, double *G1, double *G2, double M[3]);
 
 /* acc_correl (v25/functions.cpp:2399-2466): orientational pair histograms
  * 000, 110, 112 and 220 of replica `rep` over every pair f < g, binned by the

@@ -26,7 +26,7 @@ EXPORTS = [
     "msc_kernel_time", "msc_reset_timing", "msc_work_counts", "msc_chunk_repeats", "msc_write_conf",
     "msc_read_conf", "msc_pair_correl", "msc_order_params", "msc_ini_state", "msc_set_contact_rows",
     "msc_pair_correl_noalloc", "msc_place_rods", "msc_cluster_labels_dd", "msc_get_tuning", "msc_set_tuning",
-    "msc_autotune", "msc_set_neigh_list",
+    "msc_autotune", "msc_set_neigh_list", "msc_order_params_ref",
 ]
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -111,6 +111,7 @@ def load():
         "msc_ini_state": ([ctypes.c_int, ctypes.c_int, _ip, _dp, _dp, _dp, _dp, ctypes.c_double, ctypes.c_int,
                            ctypes.POINTER(ctypes.c_uint), _ip, _dp, _dp, _dp, _dp], ctypes.c_int),
         "msc_order_params": ([vp, ctypes.c_int, _dp, _dp, _dp], ctypes.c_int),
+        "msc_order_params_ref": ([vp, ctypes.c_int, _dp, _dp, _dp], ctypes.c_int),
         "msc_pair_correl": ([vp, ctypes.c_int, ctypes.c_int, _ip, ctypes.c_double, ctypes.c_int, _dp,
                              ctypes.POINTER(ctypes.c_long)], ctypes.c_int),
         "msc_pair_correl_noalloc": ([vp, ctypes.c_int, ctypes.c_int, _ip, ctypes.c_double, ctypes.c_int, _dp,

@@ -2338,6 +2338,27 @@ int msc_order_params(msc_ctx *ctx, int rep, double *G1, double *G2, double M[3])
   return MSC_OK;
 }
 
+int msc_order_params_ref(msc_ctx *ctx, int rep, double *G1, double *G2, double M[3]) {
+  SCOPE(ctx);
+  if (!ctx || rep < 0 || rep >= ctx->nrep || !G1 || !G2 || !M) return MSC_E_ARG;
+  if (!ctx->have_state) return fail(ctx, MSC_E_STATE, "no state");
+  if (ctx->nd > kOrderRefSlots) return fail(ctx, MSC_E_ARG, "at most 4 dipole slots per particle");
+  if (!ctx->d_order)
+    if (int rc_ = dalloc(ctx, &ctx->d_order, kOrderSums)) return rc_;
+  {
+    Bracket b(ctx, "order_params");
+    launch_order_ref(sel(ctx, rep), rep, ctx->d_order, ctx->stream);
+  }
+  FINISH("msc_order_params_ref");
+  double o[6];
+  if (int rc_ = d2h(ctx, o, ctx->d_order, sizeof o)) return rc_;
+  if (o[5] < 0) return fail(ctx, MSC_E_ARG, "jacobi: no convergence in 50 sweeps");
+  *G1 = o[0];
+  *G2 = o[1];
+  for (int a = 0; a < 3; ++a) M[a] = o[2 + a];
+  return MSC_OK;
+}
+
 int msc_ini_state(int N, int nspec, const int *counts, const double L[3], const double *len, const double *dia,
                   const double *mass, double temp, int nmix, unsigned int *mt_state, int *mt_index, double *r,
                   double *p, double *l, double *ori) {

@@ -2869,6 +2869,159 @@ __global__ __launch_bounds__(kOrderThreads) void k_order_sums(State s, int rep, 
 }
 
 // --------------------------------------------------------------------------
+// oo_params as the reference computes it, for output-file parity
+// (v25/functions.cpp:2702-2760 with Qmat_3d, 3426-3453, and jacobi,
+// v25/jacobi.c:12-80, compiled as C++): the Q tensor in single precision,
+// summed particle by particle in the replica's order; the Jacobi rotations
+// in single precision with the double-precision constants promoting their
+// expressions; G1 and the total moment as double sums in the same order.
+// One wave: the lanes form 64 particles' terms at a time and lane 0 adds
+// them in order.  This file is compiled without contraction, and float
+// division and square root round correctly, as on the reference's host.
+// --------------------------------------------------------------------------
+// The cyclic Jacobi method of v25/jacobi.c on a 3x3 float matrix: a is
+// overwritten above the diagonal, d receives the eigenvalues, the columns of
+// v the eigenvectors; returns the number of rotations (-1 after 50 sweeps).
+__device__ int jacobi3_float(float a[3][3], float d[3], float v[3][3]) {
+  float b[3], z[3];
+  int nrot = 0;
+  for (int p = 0; p < 3; ++p) {
+    for (int q = 0; q < 3; ++q) v[p][q] = 0.0f;
+    v[p][p] = 1.0f;
+    b[p] = d[p] = a[p][p];
+    z[p] = 0.0f;
+  }
+  for (int sweep = 1; sweep <= 50; ++sweep) {
+    float sm = 0.0f;
+    for (int p = 0; p < 2; ++p)
+      for (int q = p + 1; q < 3; ++q) sm += fabsf(a[p][q]);
+    if (sm == 0.0f) return nrot;
+    // tresh = 0.2 sm / n^2 (double constant) for the first three sweeps
+    const float tresh = sweep < 4 ? (float)(0.2 * (double)sm / 9.0) : 0.0f;
+    for (int p = 0; p < 2; ++p) {
+      for (int q = p + 1; q < 3; ++q) {
+        const float g = (float)(100.0 * (double)fabsf(a[p][q]));
+        if (sweep > 4 && fabsf(d[p]) + g == fabsf(d[p]) && fabsf(d[q]) + g == fabsf(d[q])) {
+          a[p][q] = 0.0f;
+        } else if (fabsf(a[p][q]) > tresh) {
+          float h = d[q] - d[p], t;
+          if (fabsf(h) + g == fabsf(h)) {
+            t = a[p][q] / h;
+          } else {
+            const float theta = (float)(0.5 * (double)h / (double)a[p][q]);
+            const float th2 = theta * theta;  // a float product, then promoted
+            t = (float)(1.0 / ((double)fabsf(theta) + sqrt(1.0 + (double)th2)));
+            if ((double)theta < 0.0) t = -t;
+          }
+          const float c = (float)(1.0 / (double)sqrtf(1.0f + t * t));
+          const float sn = t * c;
+          const float tau = (float)((double)sn / (1.0 + (double)c));
+          h = t * a[p][q];
+          z[p] -= h;
+          z[q] += h;
+          d[p] -= h;
+          d[q] += h;
+          a[p][q] = 0.0f;
+          auto rot = [&](float &x, float &y) {
+            const float gx = x, hy = y;
+            x = gx - sn * (hy + gx * tau);
+            y = hy + sn * (gx - hy * tau);
+          };
+          for (int j = 0; j < p; ++j) rot(a[j][p], a[j][q]);
+          for (int j = p + 1; j < q; ++j) rot(a[p][j], a[j][q]);
+          for (int j = q + 1; j < 3; ++j) rot(a[p][j], a[q][j]);
+          for (int j = 0; j < 3; ++j) rot(v[j][p], v[j][q]);
+          ++nrot;
+        }
+      }
+    }
+    for (int p = 0; p < 3; ++p) {
+      b[p] += z[p];
+      d[p] = b[p];
+      z[p] = 0.0f;
+    }
+  }
+  return -1;
+}
+
+// out: G1, G2, M[3], rotations
+__global__ void __launch_bounds__(64) k_order_ref(State s, int rep, double *__restrict__ out) {
+  constexpr int kMaxSlots = 4;
+  __shared__ float tq[6][64];
+  __shared__ double tm[kMaxSlots * 3][64];
+  __shared__ double tg[64];
+  __shared__ float ve_sh[3];
+  const RepDev &R = s.reps[rep];
+  const int N = s.Ntot, lane = threadIdx.x;
+  const int nd = s.nd < kMaxSlots ? s.nd : kMaxSlots;  // the host refuses more
+  const float twoN = (float)(2 * R.N);
+  // Q (upper triangle and diagonal: the rotations read no other entry) and M
+  float Q[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  double M[3] = {0.0, 0.0, 0.0};
+  for (int b0 = 0; b0 < R.N; b0 += 64) {
+    const int nb = min(64, R.N - b0);
+    if (lane < nb) {
+      const int g = R.n0 + b0 + lane;
+      const double o[3] = {s.ori[6 * N + g], s.ori[7 * N + g], s.ori[8 * N + g]};
+      int k = 0;
+      for (int i = 0; i < 3; ++i)
+        for (int j = i; j < 3; ++j, ++k)
+          tq[k][lane] = ((float)(3 * o[i] * o[j]) - (float)(i == j ? 1 : 0)) / twoN;
+      for (int w = 0; w < nd; ++w) {
+        const double mu = s.dmu[w * N + g];
+        for (int a = 0; a < 3; ++a) tm[3 * w + a][lane] = mu * s.dori[(3 * w + a) * N + g];
+      }
+    }
+    __syncthreads();
+    if (lane == 0)
+      for (int l = 0; l < nb; ++l) {
+        for (int w = 0; w < nd; ++w)
+          for (int a = 0; a < 3; ++a) M[a] += tm[3 * w + a][l];
+        for (int k = 0; k < 6; ++k) Q[k] += tq[k][l];
+      }
+    __syncthreads();
+  }
+  if (lane == 0) {
+    float A[3][3] = {{Q[0], Q[1], Q[2]}, {Q[1], Q[3], Q[4]}, {Q[2], Q[4], Q[5]}}, d[3], V[3][3];
+    const int nrot = jacobi3_float(A, d, V);
+    const float e1 = fabsf(d[0]), e2 = fabsf(d[1]), e3 = fabsf(d[2]);
+    int col;
+    float G2;
+    if (e1 >= e2 && e1 >= e3) {
+      G2 = e1;
+      col = 0;
+    } else if (e2 > e1 && e2 >= e3) {
+      G2 = e2;
+      col = 1;
+    } else {
+      G2 = e3;
+      col = 2;
+    }
+    float Ve[3] = {V[0][col], V[1][col], V[2][col]};
+    const float Vesq = sqrtf(Ve[0] * Ve[0] + Ve[1] * Ve[1] + Ve[2] * Ve[2]);
+    for (int a = 0; a < 3; ++a) ve_sh[a] = Ve[a] / Vesq;
+    out[1] = (double)G2;
+    for (int a = 0; a < 3; ++a) out[2 + a] = M[a];
+    out[5] = (double)nrot;
+  }
+  __syncthreads();
+  const double Ve[3] = {(double)ve_sh[0], (double)ve_sh[1], (double)ve_sh[2]};
+  double G1 = 0.0;
+  for (int b0 = 0; b0 < R.N; b0 += 64) {
+    const int nb = min(64, R.N - b0);
+    if (lane < nb) {
+      const int g = R.n0 + b0 + lane;
+      tg[lane] = Ve[0] * s.ori[6 * N + g] + Ve[1] * s.ori[7 * N + g] + Ve[2] * s.ori[8 * N + g];
+    }
+    __syncthreads();
+    if (lane == 0)
+      for (int l = 0; l < nb; ++l) G1 += tg[l];
+    __syncthreads();
+  }
+  if (lane == 0) out[0] = fabs(G1) / R.N;
+}
+
+// --------------------------------------------------------------------------
 // pair correlations (acc_correl, v25/functions.cpp:2399-2466)
 // --------------------------------------------------------------------------
 // One block per (tile I <= tile J, bar range): thread t takes particle
@@ -3724,6 +3877,9 @@ void launch_zero_FT(const State &s, hipStream_t st) {
 }
 void launch_order_sums(const State &s, int rep, double *out, hipStream_t st) {
   MSC_LAUNCH(k_order_sums, dim3(1), dim3(kOrderThreads), 0, st, s, rep, out);
+}
+void launch_order_ref(const State &s, int rep, double *out, hipStream_t st) {
+  MSC_LAUNCH(k_order_ref, dim3(1), dim3(64), 0, st, s, rep, out);
 }
 void launch_pair_correl(const double *x, const double *ori, int Ntot, const CorrelArgs &a, double *hist,
                         unsigned long long *dropped, hipStream_t st) {

@@ -165,6 +165,11 @@ void launch_clusters(const State &s, double min_s, double reach_max, double u_cu
 constexpr int kOrderSums = 12;
 constexpr int kOrderThreads = 1024;  // one block; its LDS tree holds 12 x 1024 doubles
 void launch_order_sums(const State &s, int rep, double *out, hipStream_t st);
+// oo_params as the reference computes it (single-precision Q and Jacobi,
+// ordered sums): out = G1, G2, M[3], rotations; at most kOrderRefSlots
+// dipole slots per particle
+constexpr int kOrderRefSlots = 4;
+void launch_order_ref(const State &s, int rep, double *out, hipStream_t st);
 
 // pair correlations of one replica (acc_correl): histograms of the sets
 // [sets][4][n_bars] added into hist, pairs beyond the last bar counted in

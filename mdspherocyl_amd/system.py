@@ -261,6 +261,14 @@ class MdSystem:
         self._chk(self.L.msc_order_params(self._ctx, rep, ctypes.byref(g1), ctypes.byref(g2), _d(M)[1]))
         return g1.value, g2.value, M
 
+    def order_params_ref(self, rep):
+        """oo_params as the reference computes it (single-precision Q tensor
+        and jacobi, sums in particle order): the reference's (G1, G2, M) bit
+        for bit given the same state, for output-file parity."""
+        g1, g2, M = ctypes.c_double(), ctypes.c_double(), np.zeros(3)
+        self._chk(self.L.msc_order_params_ref(self._ctx, rep, ctypes.byref(g1), ctypes.byref(g2), _d(M)[1]))
+        return g1.value, g2.value, M
+
     def pair_correl(self, rep, bar_w, n_bars, counts=None, hist=None):
         """acc_correl (v25/functions.cpp:2399-2466): adds this replica's pair
         histograms into ``hist`` ([sets][4][n_bars], new zeros when None)

@@ -764,6 +764,35 @@ def test_order_params_v251(msc):
         R.free()
 
 
+def test_order_params_ref_v251_bit_exact(msc):
+    """msc_order_params_ref (the reference's single-precision Q tensor and
+    jacobi, sums in particle order) at steps 0, 10, ..., 100 of v25/v251/run1
+    equals the oracle's oo_params bit for bit: G1, G2 and M.  The oracle's
+    G1/G2 reproduce the G1, G1^2, G2, G2^2 sums of the reference's
+    counters.dat exactly (tests/test_oracle_fixtures.py), so the library
+    gives the reference's numbers for the same state."""
+    import ctypes
+    S, P = H.v251_state()
+    R = O.ORun(S, P, seed_init=True)
+    sysm = msc.MdSystem([H.v251_desc()], precision="fp64")
+    try:
+        for step in range(101):
+            R.run(1)
+            if step % 10:
+                continue
+            _say("case: reference-exact order parameters v251 after step %d" % step)
+            H.load_osys(sysm, 0, S)
+            G1, G2, M = sysm.order_params_ref(0)
+            g1, g2, Mo = ctypes.c_double(), ctypes.c_double(), np.zeros(3)
+            O.lib().o_oo_params(S.c(), ctypes.byref(g1), ctypes.byref(g2), O.dptr(Mo))
+            _say("  G1 %.17g (oracle %.17g)  G2 %.17g (oracle %.17g)" % (G1, g1.value, G2, g2.value))
+            assert G1 == g1.value and G2 == g2.value
+            assert np.array_equal(M, Mo)
+    finally:
+        sysm.close()
+        R.free()
+
+
 def test_order_params_two_dipoles_nematic(msc):
     """N=4096 with two dipoles per particle (moments 2 and -0.5, one off
     the axis) and axes spread about a director (G2 near 0.6): G1, G2 and M
@@ -794,10 +823,17 @@ def test_order_params_two_dipoles_nematic(msc):
         _say("case: order parameters N=4096 nd=2")
         H.load_osys(sysm, 0, S)
         G1, G2, M = sysm.order_params(0)
+        R1, R2, RM = sysm.order_params_ref(0)
     finally:
         sysm.close()
     g1, g2, Mo = ctypes.c_double(), ctypes.c_double(), np.zeros(3)
     O.lib().o_oo_params(S.c(), ctypes.byref(g1), ctypes.byref(g2), O.dptr(Mo))
+    _say("  reference-exact: G1 %.17g (oracle %.17g)  G2 %.17g (oracle %.17g)" % (R1, g1.value, R2, g2.value))
+    # the single-precision path equals the oracle's bit for bit; M within
+    # 1e-15 relative (the library forms the lab-frame dipole axes of the
+    # tilted slot from the body frame, the oracle stores them)
+    assert R1 == g1.value and R2 == g2.value
+    assert np.abs(RM - Mo).max() <= 1e-15 * np.abs(Mo).max()
     f1, f2, Mf = _order_params_fp64(S)
     assert 0.3 < G2 < 0.9 and 0.1 < G1 < 0.9
     assert abs(G1 - f1) <= 1e-12 and abs(G2 - f2) <= 1e-12

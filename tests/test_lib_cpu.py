@@ -59,6 +59,7 @@ def test_new_entry_points_reject_bad_arguments_without_a_device(tmp_path):
     assert L.msc_pair_correl(None, 0, 1, cnt, 0.1, 2, h, None) == E
     g1, g2, M = ctypes.c_double(), ctypes.c_double(), (ctypes.c_double * 3)()
     assert L.msc_order_params(None, 0, ctypes.byref(g1), ctypes.byref(g2), M) == E
+    assert L.msc_order_params_ref(None, 0, ctypes.byref(g1), ctypes.byref(g2), M) == E
     z = (ctypes.c_double * 64)()
     assert L.msc_read_conf(str(tmp_path / "missing.dat").encode(), 1, 1, z, z, z, z, z, z, z, z, z, None, z) == E
     assert L.msc_write_conf(None, 1, 1, z, z, z, z, z, z, z, z, z, None, z) == E
