@@ -485,6 +485,12 @@ static void drop_graph(msc_ctx *ctx) {
 int upload_reps(msc_ctx *ctx) {
   // replica parameters decide which launches a step makes (field, modes)
   drop_graph(ctx);
+  // the thermostat's constant factors, with the host's libm as the
+  // reference evaluates them once per call (v25/functions.cpp:2196-2197)
+  for (RepDev &R : ctx->reps) {
+    R.th_ex1 = std::exp(-R.gamma * R.dt / 2);
+    R.th_sq = std::sqrt(2. * R.temp * R.gamma * R.dt / R.Q);
+  }
   return h2d(ctx, ctx->d_reps, ctx->reps.data(), sizeof(RepDev) * ctx->nrep);
 }
 

@@ -57,6 +57,10 @@ struct RepDev {
   double diamax;        // largest diameter
   double B[3];
   double temp, dt, Q, gamma, dens;
+  // moveTher's factors (v25/functions.cpp:2196-2197), taken on the host
+  // from the four above whenever the replicas are uploaded:
+  // th_ex1 = exp(-gamma dt / 2), th_sq = sqrt(2 temp gamma dt / Q)
+  double th_ex1, th_sq;
   double G[9];          // sum over half-space k of val_k k k^T
   double Uself;
 };

@@ -3340,9 +3340,10 @@ __global__ void k_renorm(State s) {
 // moveTher mode 0 (v25/functions.cpp:2186-2218) for every replica (one
 // thread each): gs[rep*4 + 2*i + 0/1] hold the Box-Muller pair drawn on the
 // host from thermostat i's MT19937 for this step.
+// ex1 s + sqrt(2 T g dt / Q) r with both factors taken once on the host
+// (RepDev::th_ex1, th_sq)
 __device__ __forceinline__ double thermo_half(const RepDev &R, double s, double r) {
-  double ex1 = exp(-R.gamma * R.dt / 2);
-  return ex1 * s + sqrt(2. * R.temp * R.gamma * R.dt / R.Q) * r;
+  return R.th_ex1 * s + R.th_sq * r;
 }
 __device__ __forceinline__ double thermo_mode1(const RepDev &R, double s, double K2) {
   return s + R.dt * (2. * K2 - 3. * (double)R.N * R.temp) / R.Q;
