@@ -44,18 +44,24 @@ if ROOT not in sys.path:
 #   dipolar real-space pair within r_c: table lerp, force, torques        70
 FLOPS = {"recip_struct": 18, "recip_force": 21, "wca": 120, "real_pairs": 70}
 # What bounds each kernel of the mixed step: the reciprocal kernels run their
-# products on the matrix cores (v_mfma_f32_16x16x16_f16 on operands split into
-# two fp16 pieces, three products per FP32-accurate product); the pair
-# kernels are gathers.  The reciprocal kernels' algorithmic flops are FP32
-# products, priced against MI355X's dense FP32 matrix peak (157.3 TFLOP/s,
-# equal to its FP32 vector peak).
+# products on the matrix cores (v_mfma_f32_16x16x32_f16 on operands split into
+# two fp16 pieces, three products per FP32-accurate product), priced as FP32
+# products against MI355X's dense FP32 matrix peak (157.3 TFLOP/s, equal to
+# its FP32 vector peak).  The pair kernels are gathers, priced by their
+# algorithmic HBM bytes against 8 TB/s: each particle's records read once
+# (rod and dipole, 64 B each), its force and torque read and written (96 B),
+# and two 4-byte row entries per interacting pair (WCA inside 2^(1/6) sigma,
+# dipole pairs inside r_c).
 BOUND = {"recip_struct": "mfma", "recip_force": "mfma", "wca": "hbm", "real_pairs": "hbm"}
+PAIR_BYTES_PER_PARTICLE = {"wca": 64 + 96, "real_pairs": 64 + 64 + 96}  # real_pairs: the fused WCA + real-space launch
+PAIR_BYTES_PER_PAIR = 8
 PEAK_TFLOPS = {"fp32": 157.3, "fp64": 78.6}
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of a kernel from two rocprofv3 --pmc passes
 # (FETCH_SIZE, WRITE_SIZE) scaled by a copy kernel of known size, as
 # recorded in the profile summaries
-TRAFFIC_FILES = {"recip_force": "profiles/recip_force_mfma_n32768_counters.md"}
+TRAFFIC_FILES = {"recip_force": "profiles/recip_force_mfma_n32768_counters.md",
+                 "real_pairs": "profiles/pairs_mixed_n32768_counters.md"}
 
 
 def log(msg):
@@ -422,18 +428,44 @@ def main():
     mix.close()
     f64.close()
 
-    # --- dominant kernel roofline -----------------------------------------------
-    units = {"recip_struct": desc.N * nk, "recip_force": desc.N * nk, "wca": counts_mix[0],
-             "real_pairs": counts_mix[1]}
-    dom = max((n for n in ktab if n in FLOPS), key=lambda n: ktab[n][0])
-    ms, cnt = ktab[dom]
-    flop = FLOPS[dom] * units[dom]
-    achieved = flop / (ms / cnt * 1e-3) / 1e12 if ms > 0 else 0.0
-    traffic, traffic_src = traffic_from_profile(dom)
-    roofline = {"kernel": dom, "bound": BOUND[dom], "achieved": round(achieved, 3),
-                "peak": PEAK_TFLOPS["fp32"], "unit": "TFLOP/s", "frac": round(achieved / PEAK_TFLOPS["fp32"], 4),
-                "traffic": traffic, "traffic_source": traffic_src, "flop_per_launch": flop,
-                "ms_per_launch": round(ms / cnt, 4)}
+    # --- roofline of every force kernel; `roofline` is the dominant one ----------
+    # (the kernel with the most time per step; the fused launch "real_pairs"
+    # holds both the WCA and the real-space pairs)
+    wca_pairs, real_pairs = counts_mix[0], counts_mix[1]
+    fused = "wca" not in ktab or ktab["wca"][1] == 0
+    pair_flop = {"wca": FLOPS["wca"] * wca_pairs,
+                 "real_pairs": FLOPS["real_pairs"] * real_pairs + (FLOPS["wca"] * wca_pairs if fused else 0)}
+    pair_bytes = {"wca": PAIR_BYTES_PER_PARTICLE["wca"] * desc.N + PAIR_BYTES_PER_PAIR * wca_pairs,
+                  "real_pairs": (PAIR_BYTES_PER_PARTICLE["real_pairs"] * desc.N
+                                 + PAIR_BYTES_PER_PAIR * (real_pairs + (wca_pairs if fused else 0)))}
+
+    def roofline_of(name):
+        ms, cnt = ktab[name]
+        sec = ms / cnt * 1e-3
+        traffic, traffic_src = traffic_from_profile(name)
+        if BOUND[name] == "mfma":
+            flop = FLOPS[name] * desc.N * nk
+            ach = flop / sec / 1e12
+            r = {"kernel": name, "bound": "mfma", "achieved": round(ach, 3), "peak": PEAK_TFLOPS["fp32"],
+                 "unit": "TFLOP/s", "frac": round(ach / PEAK_TFLOPS["fp32"], 4), "flop_per_launch": flop}
+        else:
+            nbytes = pair_bytes[name]
+            ach = nbytes / sec / 1e9
+            r = {"kernel": name, "bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                 "frac": round(ach / HBM_PEAK_GBS, 4), "bytes_per_launch": nbytes,
+                 "flop_per_launch": pair_flop[name],
+                 "fp64_tflops": round(pair_flop[name] / sec / 1e12, 3)}
+        r.update({"traffic": traffic, "traffic_source": traffic_src, "ms_per_launch": round(ms / cnt, 4)})
+        return r
+
+    force_kernels = [n for n in ktab if n in FLOPS and ktab[n][1] > 0]
+    roofline_by_kernel = {n: roofline_of(n) for n in force_kernels}
+    dom = max(force_kernels, key=lambda n: ktab[n][0])
+    roofline = roofline_by_kernel[dom]
+    for n, r in roofline_by_kernel.items():
+        log("bench: roofline %-12s %8.4f ms/launch  %s %.3g %s of %.4g (frac %.4f)%s"
+            % (n, r["ms_per_launch"], r["bound"], r["achieved"], r["unit"], r["peak"], r["frac"],
+               "  <- dominant" if n == dom else ""))
 
     # --- batched replicas ---------------------------------------------------------
     batched = None
@@ -530,6 +562,7 @@ def main():
         "order_params": order,
         "kernels_ms_per_launch": {n: round(v[0] / v[1], 4) for n, v in ktab.items()},
         "roofline": roofline,
+        "roofline_by_kernel": roofline_by_kernel,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -12,7 +12,8 @@ once L > 3.2); replicas denser than phi = 0.22, where addition slows down
 near jamming, start as an orthorhombic lattice of rods along +-z with random
 polarity and a 0.02 rad random tilt (the reference's own ini_pos start is an
 aligned lattice too, v25/functions.cpp:491-584).  Maxwell momenta at T, zero
-total momentum.  One axial point dipole
+total momentum.  Randomly added rods are numbered along a Morton curve
+(neighbours in space are neighbours in index).  One axial point dipole
 per particle at the centre.  Lengths are Gaussian, truncated at +-3 sd, and
 the box follows from the packing fraction phi = sum(v_i) / V with
 v = pi D^2 L / 4 + pi D^3 / 6.
@@ -281,6 +282,19 @@ def inertia(length, dia, mass):
     return np.stack([I0, I0, I2], axis=1)
 
 
+def morton_order(r, box, bits=10):
+    """Permutation that numbers the centres r (in [0, box)^3) along a Morton
+    (Z-order) curve over a 2^bits grid per axis: consecutive indices are
+    neighbours in space, and each eighth of the index range is one octant
+    (the pair kernels give each XCD one contiguous eighth of the particles)."""
+    q = np.clip((np.mod(r, box) / box * (1 << bits)).astype(np.int64), 0, (1 << bits) - 1)
+    code = np.zeros(len(r), dtype=np.int64)
+    for b in range(bits - 1, -1, -1):
+        for a in (2, 1, 0):  # z, y, x from the top: the first split is along z
+            code = (code << 1) | ((q[:, a] >> b) & 1)
+    return np.argsort(code, kind="stable")
+
+
 def make_replica(rng, N, lengths, phi, mu, K, alpha_fn, rc, temp=1.0, dt=0.002, Q=5.0, gamma=0.0, seeds=(1, 2),
                  lattice=False, native_seed=None):
     dia = np.ones(N)
@@ -296,6 +310,13 @@ def make_replica(rng, N, lengths, phi, mu, K, alpha_fn, rc, temp=1.0, dt=0.002, 
         r, u = place_aligned(rng, N, lengths, box)
     else:
         r, u = place_rsa(rng, N, lengths, box)
+    if not lattice and not phi > 0.22:
+        # random addition numbers the rods in the order they were placed,
+        # which is random in space: renumber them along a space-filling curve
+        # (the same configuration, relabelled) so that neighbours in space
+        # are neighbours in index
+        o = morton_order(r, box)
+        r, u, lengths = r[o], u[o], lengths[o]
     ori = frames(u)
     p, l = maxwell(rng, N, mass, inertia(lengths, dia, mass), ori, temp)
     dmax = float((lengths + dia).max())

@@ -126,6 +126,18 @@ __device__ __forceinline__ void wave_store_sums(double *dst, const double *v, in
 // Task of the current wave (4 waves per 256-thread block).  The value is the
 // same in every lane; readfirstlane lets the compiler keep it, and all task
 // data derived from it, in scalar registers.
+// Work-groups are dispatched to the 8 XCDs in turn (work-group b to XCD
+// b mod 8), each XCD with its own L2.  xcd_block maps b in [0, n) to an
+// index so that each XCD takes one contiguous eighth of [0, n): neighbouring
+// tasks, spatial neighbours when the particles are numbered along a
+// space-filling curve, then share one L2 and their partners' records are
+// fetched once per XCD rather than once per XCD and task.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_block(int b, int n) {
+  const int x = b % kXcds, i = b / kXcds;
+  const int q = n / kXcds, r = n % kXcds;
+  return x * q + min(x, r) + i;
+}
 __device__ __forceinline__ int wave_task_id() {
   return __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave);
 }
@@ -860,7 +872,7 @@ __global__ void __launch_bounds__(64 * kPairSplit) k_force_sc(State s, double *_
     if (ls.rebuild && (int)blockIdx.x < 2 * s.ntask) cells_assign_task(s, (int)blockIdx.x - s.ntask, ls);
     return;
   }
-  const int tid = s.task_begin + blockIdx.x;
+  const int tid = s.task_begin + xcd_block(blockIdx.x, s.ntask);
   const Task tk = s.tasks[tid];
   const RepDev &R = s.reps[tk.rep];
   const int N = s.Ntot;
@@ -1177,7 +1189,7 @@ __global__ void __launch_bounds__(64 * kPairSplit) k_dipR_list(State s, const R 
   __shared__ double red[kPairSplit][17][kWave];
   const int lane = threadIdx.x & 63, sw = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   if ((int)blockIdx.x >= s.ntask) return;
-  const int tid = s.task_begin + blockIdx.x;
+  const int tid = s.task_begin + xcd_block(blockIdx.x, s.ntask);
   const Task tk = s.tasks[tid];
   const RepDev &Rp = s.reps[tk.rep];
   const R *tab = tabs + (size_t)Rp.tab0 * 3;
@@ -1281,7 +1293,7 @@ __global__ void __launch_bounds__(64 * PS) k_pairs_mixed(State s, const float *_
     if (ls.rebuild && (int)blockIdx.x < 2 * s.ntask) cells_assign_task(s, (int)blockIdx.x - s.ntask, ls);
     return;
   }
-  const int tid = s.task_begin + blockIdx.x;
+  const int tid = s.task_begin + xcd_block(blockIdx.x, s.ntask);
   const Task tk = s.tasks[tid];
   const RepDev &R = s.reps[tk.rep];
   const int N = s.Ntot;
