@@ -95,7 +95,7 @@ def timed_run(sysm, steps, warmup, timing, step0=0):
     return dt, obs
 
 
-def autotune(sysm, name, steps=100):
+def autotune(sysm, name, steps=300):
     best, tried = sysm.autotune(steps)
     log("bench: autotune %s: %s" % (name, ", ".join(
         "graph %d fused %d waves %d skin %.2f chunk %d overlap %d %.4f ms" % (

@@ -1499,7 +1499,7 @@ static void do_force_sc(msc_ctx *ctx, int rep) {
   }
   {
     Bracket b(ctx, "wca");
-    launch_force_sc(s, !ctx->contact, ctx->d_task_out, ctx->d_flags, ctx->stream);
+    launch_force_sc(s, !ctx->contact, ctx->mixed, ctx->d_task_out, ctx->d_flags, ctx->stream);
   }
   launch_reduce(s, ctx->d_task_out, 16, 11, 0, (double *)ctx->d_E, kEstride, kEoff_sc, ctx->stream, 1, ctx->d_counts,
                 4, 0);
@@ -1913,7 +1913,7 @@ static void enqueue_step(msc_ctx *ctx) {
   } else {
     {
       Bracket b(ctx, "wca");
-      launch_force_sc(s, !ctx->contact, rows + kRowSC * region, ctx->d_flags, st, &ls);
+      launch_force_sc(s, !ctx->contact, ctx->mixed, rows + kRowSC * region, ctx->d_flags, st, &ls);
     }
     dipR_launches(ctx, MSC_ALL, rows + kRowR * region);
   }

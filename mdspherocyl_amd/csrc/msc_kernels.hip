@@ -765,9 +765,13 @@ __global__ void __launch_bounds__(256) k_lists_step(State s, ListStep ls, int *_
   for (int r = s.rep_begin + lane; r < s.rep_end; r += 64)
     any = any || ls.rebuild[r] || (ls.srebuild && ls.srebuild[r]);
   if (__ballot(any) == 0) return;
-  // a fixed grid (launch_lists_step) strides over the particles: the steps
-  // that rebuild nothing cost one small grid
-  for (int gi = blockIdx.x * 4 + wv; gi < s.g_end - s.g_begin; gi += gridDim.x * 4) {
+  // a fixed grid (launch_lists_step) strides over the particles, the waves
+  // numbered so that each XCD holds one contiguous eighth of them (the cell
+  // records of neighbouring particles then share an L2; a replica that
+  // rebuilds alone still spreads over every wave): the steps that rebuild
+  // nothing cost one small grid
+  const int n = s.g_end - s.g_begin;
+  for (int gi = xcd_block(blockIdx.x, gridDim.x) * 4 + wv; gi < n; gi += gridDim.x * 4) {
     const int g = s.g_begin + __builtin_amdgcn_readfirstlane(gi);
     const int rp = __builtin_amdgcn_readfirstlane(s.rep_of[g]);
     if (ls.rebuild[rp])
@@ -858,10 +862,13 @@ __global__ void __launch_bounds__(256) k_check_decide(State s, const double *__r
 // abort tests, as the reference.  Otherwise only the contact rows (pairs
 // within r_short, which hold every pair inside 2^(1/6) sigma until the
 // contact skin is used up) are evaluated; the others contribute exactly zero.
-template <bool STRICT>
+// MIX (single-precision contexts): the minimum image and the pair force by
+// multiplications with reciprocals (1/L, 1/d^2) instead of the reference's
+// divisions, the segment distance as the reference.
 // Extra blocks (ls.rebuild set; block ntask + t for task t): the cell
 // assignment of rebuilt replicas; the replica's last such block scans its
 // cell counts into cell starts.
+template <bool STRICT, bool MIX>
 __global__ void __launch_bounds__(64 * kPairSplit) k_force_sc(State s, double *__restrict__ task_out,
                                                                int *__restrict__ flags, ListStep ls) {
   // one block per wave task; wave sw of the block takes row entries
@@ -903,7 +910,8 @@ __global__ void __launch_bounds__(64 * kPairSplit) k_force_sc(State s, double *_
       double rij[3], rsq = 0;
       for (int a = 0; a < 3; ++a) {
         rij[a] = rA[a] - rB[a];
-        rij[a] -= R.L[a] * floor(rij[a] / R.L[a] + 0.5);
+        if constexpr (MIX) rij[a] -= R.L[a] * floor(rij[a] * R.iL[a] + 0.5);
+        else rij[a] -= R.L[a] * floor(rij[a] / R.L[a] + 0.5);
         rsq += rij[a] * rij[a];
       }
       double st[2], dij[3];
@@ -911,13 +919,26 @@ __global__ void __launch_bounds__(64 * kPairSplit) k_force_sc(State s, double *_
       if (dsq < 0.49 * sig_sq) code = code ? code : 1;
       if (dsq > rsq * 1.00001) code = code ? code : 2;
       if (dsq < r_cut_sq && dsq < rsh) {
-        double SR2s = sig_sq / dsq;
-        double SR6s = SR2s * SR2s * SR2s;
-        double pot = 4 * SR6s * (SR6s - 1.);
-        double ddr = dot3(rij, dij);
-        double UIJ = pot + 1.;
-        double WIJ = 6 * (pot + 4 * SR6s * SR6s) * ddr / dsq;
-        double FIJ = WIJ / ddr;
+        double SR2s, SR6s, pot, ddr, UIJ, WIJ, FIJ;
+        if constexpr (MIX) {
+          // F_IJ = W_IJ / ddr taken directly as 6 (pot + 4 SR6^2) / d^2
+          const double idsq = 1.0 / dsq;
+          SR2s = sig_sq * idsq;
+          SR6s = SR2s * SR2s * SR2s;
+          pot = 4 * SR6s * (SR6s - 1.);
+          ddr = dot3(rij, dij);
+          UIJ = pot + 1.;
+          FIJ = 6 * (pot + 4 * SR6s * SR6s) * idsq;
+          WIJ = FIJ * ddr;
+        } else {
+          SR2s = sig_sq / dsq;
+          SR6s = SR2s * SR2s * SR2s;
+          pot = 4 * SR6s * (SR6s - 1.);
+          ddr = dot3(rij, dij);
+          UIJ = pot + 1.;
+          WIJ = 6 * (pot + 4 * SR6s * SR6s) * ddr / dsq;
+          FIJ = WIJ / ddr;
+        }
         double f[3];
         for (int a = 0; a < 3; ++a) f[a] = FIJ * dij[a];
         double icroj[3], icror[3], jcror[3];
@@ -1228,7 +1249,7 @@ __global__ void __launch_bounds__(64 * kPairSplit) k_dipR_list(State s, const R 
 
 // --------------------------------------------------------------------------
 // Mixed contexts in the step loop, replicas with a real-space cut-off: the
-// WCA over the contact rows (k_force_sc<false>) and the real-space dipole
+// WCA over the contact rows (k_force_sc<false, true>) and the real-space dipole
 // pairs over the dipole rows (k_dipR_list<float>) of one task in one block,
 // every partner read as one 64-byte record (State::rec_rod, rec_dip) instead
 // of eleven scattered FP64 words.  Each sum is formed as in those kernels
@@ -1303,7 +1324,7 @@ __global__ void __launch_bounds__(64 * PS) k_pairs_mixed(State s, const float *_
   double FS[3] = {0, 0, 0}, TS[3] = {0, 0, 0}, FR[3] = {0, 0, 0}, TR[3] = {0, 0, 0};
   if (lane < tk.n) {
     const int g = tk.g0 + lane;
-    // WCA over the contact rows (k_force_sc<false>)
+    // WCA over the contact rows (k_force_sc<false, true>)
     {
       const double srt = pow(2., 1. / 6);
       const double srtsq = srt * srt;
@@ -1327,7 +1348,7 @@ __global__ void __launch_bounds__(64 * PS) k_pairs_mixed(State s, const float *_
         double rij[3], rsq = 0;
         for (int a = 0; a < 3; ++a) {
           rij[a] = rA[a] - rB[a];
-          rij[a] -= R.L[a] * floor(rij[a] / R.L[a] + 0.5);
+          rij[a] -= R.L[a] * floor(rij[a] * R.iL[a] + 0.5);  // as k_force_sc<., true>
           rsq += rij[a] * rij[a];
         }
         double st[2], dij[3];
@@ -1335,13 +1356,15 @@ __global__ void __launch_bounds__(64 * PS) k_pairs_mixed(State s, const float *_
         if (dsq < 0.49 * sig_sq) code = code ? code : 1;
         if (dsq > rsq * 1.00001) code = code ? code : 2;
         if (dsq < r_cut_sq && dsq < rsh) {
-          double SR2s = sig_sq / dsq;
+          // as k_force_sc<., true>: one reciprocal
+          const double idsq = 1.0 / dsq;
+          double SR2s = sig_sq * idsq;
           double SR6s = SR2s * SR2s * SR2s;
           double pot = 4 * SR6s * (SR6s - 1.);
           double ddr = dot3(rij, dij);
           double UIJ = pot + 1.;
-          double WIJ = 6 * (pot + 4 * SR6s * SR6s) * ddr / dsq;
-          double FIJ = WIJ / ddr;
+          double FIJ = 6 * (pot + 4 * SR6s * SR6s) * idsq;
+          double WIJ = FIJ * ddr;
           double f[3];
           for (int a = 0; a < 3; ++a) f[a] = FIJ * dij[a];
           double icroj[3], icror[3], jcror[3];
@@ -3728,15 +3751,19 @@ void launch_check(const State &s, double *task_out, int *rebuild, int *srebuild,
   MSC_LAUNCH(k_check_decide, dim3(s.rep_end - s.rep_begin), dim3(256), 0, st, s, task_out, rebuild, srebuild,
              obs_rebuild, obs_stride, standalone ? 1 : 0);
 }
-void launch_force_sc(const State &s, bool strict, double *task_out, int *flags, hipStream_t st,
+void launch_force_sc(const State &s, bool strict, bool mixed, double *task_out, int *flags, hipStream_t st,
                      const ListStep *ls) {
   const ListStep none = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   const ListStep l = ls ? *ls : none;
   const dim3 grid(ls ? 2 * s.ntask : s.ntask);
-  if (strict)
-    MSC_LAUNCH(k_force_sc<true>, grid, dim3(64 * kPairSplit), 0, st, s, task_out, flags, l);
+  if (strict && mixed)
+    MSC_LAUNCH((k_force_sc<true, true>), grid, dim3(64 * kPairSplit), 0, st, s, task_out, flags, l);
+  else if (strict)
+    MSC_LAUNCH((k_force_sc<true, false>), grid, dim3(64 * kPairSplit), 0, st, s, task_out, flags, l);
+  else if (mixed)
+    MSC_LAUNCH((k_force_sc<false, true>), grid, dim3(64 * kPairSplit), 0, st, s, task_out, flags, l);
   else
-    MSC_LAUNCH(k_force_sc<false>, grid, dim3(64 * kPairSplit), 0, st, s, task_out, flags, l);
+    MSC_LAUNCH((k_force_sc<false, false>), grid, dim3(64 * kPairSplit), 0, st, s, task_out, flags, l);
 }
 void launch_dipR(const State &s, bool mixed, bool all_pairs, const void *tab, double *task_out, hipStream_t st) {
   if (mixed) {

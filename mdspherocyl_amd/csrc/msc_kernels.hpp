@@ -87,11 +87,11 @@ void launch_check(const State &s, double *task_out, int *rebuild, int *srebuild,
                   int obs_stride, bool standalone, hipStream_t st);
 // ls (step loop only): the cell assignment and scan of rebuilt replicas run
 // in extra blocks of the same launch
-void launch_force_sc(const State &s, bool strict, double *task_out, int *flags, hipStream_t st,
+void launch_force_sc(const State &s, bool strict, bool mixed, double *task_out, int *flags, hipStream_t st,
                      const ListStep *ls = nullptr);
 void launch_dipR(const State &s, bool mixed, bool all_pairs, const void *tab, double *task_out, hipStream_t st);
 // mixed contexts, every replica with a real-space cut-off and contact rows:
-// k_force_sc<false> and k_dipR_list<float> in one launch (step loop; ls as
+// k_force_sc<false, true> and k_dipR_list<float> in one launch (step loop; ls as
 // launch_force_sc's)
 // waves: per 64-particle task, 2, 4 or 8 (the row entries q = w mod waves go
 // to wave w; the choice changes the rounding of the pair sums only)
