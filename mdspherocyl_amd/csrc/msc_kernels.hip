@@ -1719,11 +1719,12 @@ __device__ __forceinline__ void wave_rows_sum(const double *__restrict__ rows, i
 // their imaginary parts; B likewise with column n; D[i][n] in lane
 // n + 16 (i/4), element i % 4.  Any inner-index order gives the same product
 // as long as A and B share it.
-// Block: 4 waves over one chunk task; wave w owns M-tiles
-// 8 blockIdx.y + 2w + {0, 1} and every N-tile.  Per 64 particles the
-// phase tables are staged in LDS, the B operands of all four 16-particle
-// steps are built once for the block (wave w builds N-tiles w, w + 4, ...)
-// and read by all waves.
+// Block: WPB waves over one chunk task; wave w owns the MW M-tiles from
+// MW (WPB blockIdx.y + w) and every N-tile.  Per 64 particles the phase
+// tables are staged in LDS (the next 64 particles' loaded into registers
+// meanwhile), the B operands of all four 16-particle steps are built once
+// for the block (wave w the (N-tile, step) pairs w, w + WPB, ...) and read
+// by all waves.
 typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
 typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
 typedef float float4_t __attribute__((ext_vector_type(4)));
@@ -1752,13 +1753,13 @@ __device__ __forceinline__ void split_f16(double x, _Float16 &h, _Float16 &l) {
 // after rounding them to FP32 left force errors 3 to 80 times larger
 // (N = 32768: relative L2 2e-7 against 6e-8; replicas of 3000 to 8000
 // rods: reciprocal torques 5.6e-5 against 1e-6 of their rms).
-template <int KZ>
-__global__ void __launch_bounds__(256, 2) k_recip_struct_mfma(State s, int rt0, const Cpx<float> *__restrict__ ph,
+template <int KZ, int WPB, int MW>
+__global__ void __launch_bounds__(64 * WPB) k_recip_struct_mfma(State s, int rt0, const Cpx<float> *__restrict__ ph,
                                                            const float *__restrict__ amp, float *__restrict__ part) {
   constexpr int kmax = KZ - 1, side = 2 * kmax + 1;
   constexpr int MTOT = side * KZ, NTOT = 3 * side;
-  constexpr int MT = (MTOT + 15) / 16, NT = (NTOT + 15) / 16;
-  constexpr int MW = 2;  // M-tiles per wave
+  constexpr int NT = (NTOT + 15) / 16;
+  constexpr int NTH = 64 * WPB;  // threads per block
   __shared__ Cpx<float> sh[64 * 3 * KZ];
   __shared__ float sa[64 * 3];
   // B operands of the 4 steps of a 64-particle stage: [step][N-tile][4][lane]
@@ -1778,18 +1779,18 @@ __global__ void __launch_bounds__(256, 2) k_recip_struct_mfma(State s, int rt0, 
   // this lane's A rows (one per owned M-tile) and B column
   int mrow[MW];
 #pragma unroll
-  for (int u = 0; u < MW; ++u) mrow[u] = (4 * MW * blockIdx.y + MW * wv + u) * 16 + li;
+  for (int u = 0; u < MW; ++u) mrow[u] = (WPB * MW * blockIdx.y + MW * wv + u) * 16 + li;
   // the tables and amplitudes of the next 64 particles are loaded into
   // registers while the current ones are in use; consecutive lanes read
   // consecutive particles of one table row
-  constexpr int NQ = (64 * 3 * KZ + 255) / 256;
+  constexpr int NQ = (64 * 3 * KZ + NTH - 1) / NTH;
   Cpx<float> pre[NQ];
   float prea = 0.f;
   auto fetch = [&](int q0) {
     const int nq = min(64, rt.n - q0);
 #pragma unroll
     for (int u = 0; u < NQ; ++u) {
-      const int q = threadIdx.x + 256 * u;
+      const int q = threadIdx.x + NTH * u;
       const int p = q & 63, r = q >> 6, a = r / KZ, n = r % KZ;
       Cpx<float> v;
       v.re = v.im = 0.f;
@@ -1805,40 +1806,39 @@ __global__ void __launch_bounds__(256, 2) k_recip_struct_mfma(State s, int rt0, 
     __syncthreads();
 #pragma unroll
     for (int u = 0; u < NQ; ++u) {
-      const int q = threadIdx.x + 256 * u;
+      const int q = threadIdx.x + NTH * u;
       if (q < 64 * 3 * KZ) sh[(q & 63) * 3 * KZ + (q >> 6)] = pre[u];
     }
     if (threadIdx.x < 64 * 3) sa[threadIdx.x] = prea;
     if (p0 + 64 < rt.n) fetch(p0 + 64);
     __syncthreads();
-    // B = c_a e_x(nx) for the 4 steps (particles of padding have c = 0)
-    for (int t = wv; t < NT; t += 4) {
+    // B = c_a e_x(nx) for the 4 steps (particles of padding have c = 0):
+    // wave w builds the (N-tile, step) pairs w, w + WPB, ...
+    for (int tq = wv; tq < NT * 4; tq += WPB) {
+      const int t = tq >> 2, step = tq & 3;
       const int n = t * 16 + li;
       const bool ok = n < NTOT;
       const int a = ok ? n / side : 0, nxi = ok ? n % side : kmax;
       const int nx = nxi - kmax, ax = nx < 0 ? -nx : nx;
       const float sx = nx < 0 ? -1.f : 1.f;
+      half4_t rh, rl, ih, il;
 #pragma unroll
-      for (int step = 0; step < 4; ++step) {
-        half4_t rh, rl, ih, il;
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          const int p = step * 16 + 4 * lq + j;
-          const Cpx<float> ex = sh[p * 3 * KZ + ax];
-          const double c = ok ? (double)sa[3 * p + a] : 0.0;
-          _Float16 h, l;
-          split_f16(c * (double)ex.re, h, l);
-          rh[j] = h;
-          rl[j] = l;
-          split_f16(c * (double)(sx * ex.im), h, l);
-          ih[j] = h;
-          il[j] = l;
-        }
-        sb[step][t][0][lane] = rh;
-        sb[step][t][1][lane] = rl;
-        sb[step][t][2][lane] = ih;
-        sb[step][t][3][lane] = il;
+      for (int j = 0; j < 4; ++j) {
+        const int p = step * 16 + 4 * lq + j;
+        const Cpx<float> ex = sh[p * 3 * KZ + ax];
+        const double c = ok ? (double)sa[3 * p + a] : 0.0;
+        _Float16 h, l;
+        split_f16(c * (double)ex.re, h, l);
+        rh[j] = h;
+        rl[j] = l;
+        split_f16(c * (double)(sx * ex.im), h, l);
+        ih[j] = h;
+        il[j] = l;
       }
+      sb[step][t][0][lane] = rh;
+      sb[step][t][1][lane] = rl;
+      sb[step][t][2][lane] = ih;
+      sb[step][t][3][lane] = il;
     }
     __syncthreads();
 #pragma unroll
@@ -1900,7 +1900,7 @@ __global__ void __launch_bounds__(256, 2) k_recip_struct_mfma(State s, int rt0, 
     for (int t = 0; t < NT; ++t)
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        const int m = (4 * MW * blockIdx.y + MW * wv + u) * 16 + 4 * lq + j, n = t * 16 + li;
+        const int m = (WPB * MW * blockIdx.y + MW * wv + u) * 16 + 4 * lq + j, n = t * 16 + li;
         if (m < MTOT && n < NTOT) {
           float *o = dst + (long)m * (2 * NTOT) + 2 * n;
           o[0] = cr[u][t][j];
@@ -3821,8 +3821,11 @@ static void recip_struct_kz(const State &s, int rt0, int nrt, int maxcol, const 
                             void *part, hipStream_t st) {
   if constexpr (sizeof(R) == sizeof(float)) {
     constexpr int MT = ((2 * KZ - 1) * KZ + 15) / 16;
-    MSC_LAUNCH((k_recip_struct_mfma<KZ>), dim3(nrt, (MT + 4 * 2 - 1) / (4 * 2)), dim3(256), 0, st, s, rt0,
-               (const Cpx<float> *)ph, (const float *)amp, (float *)part);
+    // 8 waves of one row tile each: the B operands built once for 8 row
+    // tiles, 2048 waves at 256-particle chunks of N = 32768
+    constexpr int WPB = 8, MW = 1;
+    MSC_LAUNCH((k_recip_struct_mfma<KZ, WPB, MW>), dim3(nrt, (MT + WPB * MW - 1) / (WPB * MW)), dim3(64 * WPB), 0,
+               st, s, rt0, (const Cpx<float> *)ph, (const float *)amp, (float *)part);
   } else {
     MSC_LAUNCH((k_recip_struct<R, KZ>), dim3(nrt, nblk(maxcol, 256)), dim3(256), 0, st, s, rt0,
                (const Cpx<R> *)ph, (const R *)amp, (R *)part);
