@@ -2221,7 +2221,7 @@ __global__ void __launch_bounds__(64 * kRfGroups) k_recip_force(State s, const C
 // ct = g, g + 2, ...  A group's B operands of a column tile are staged in
 // LDS for its 4 waves, the next tile's loaded while this one is used; the
 // two groups' sums meet in LDS in a fixed order.
-constexpr int kBpGroups = 2;
+constexpr int kBpGroups = 1;
 template <int KZ>
 __global__ void __launch_bounds__(256 * kBpGroups) k_recip_force_mfma(State s, const Cpx<float> *__restrict__ ph,
                                                                       const float *__restrict__ amp,
