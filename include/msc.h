@@ -210,7 +210,7 @@ int msc_set_tuning(msc_ctx *ctx, const msc_tuning *t);
  * step, in the order tried; *ntried (may be NULL) their number.  No
  * counterpart in the reference. */
 int msc_autotune(msc_ctx *ctx, int steps, msc_tuning *best, msc_tuning *tried, double *ms, int ntry, int *ntried);
-/* msc_run advances in chunks of 50 steps.  A chunk during which a list
+/* msc_run advances in chunks of 100 steps.  A chunk during which a list
  * rebuild outgrew its device region is rolled back, the regions are enlarged
  * and the lists rebuilt from the chunk's first state, and the chunk is run
  * again; *n receives how many chunks were repeated since msc_create.  No

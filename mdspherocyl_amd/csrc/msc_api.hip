@@ -1967,7 +1967,7 @@ static int state_copy(msc_ctx *ctx, bool restore) { return state_copy(ctx, resto
 // rebuild needed, every list is rebuilt from the chunk's first state, and the
 // chunk is run again (the redone steps agree with an uninterrupted run up to
 // the summation order of the rebuilt lists).
-static constexpr int kRunChunk = 50;
+static constexpr int kRunChunk = 100;
 
 // Capture one step (enqueue_step) as a graph on the context stream.
 static int capture_step(msc_ctx *ctx) {

@@ -1918,8 +1918,9 @@ __global__ void __launch_bounds__(64 * WPB) k_recip_struct_mfma(State s, int rt0
 // force_dipK (v25/functions.cpp:1926-1952, with its a4 = 1/(4 alpha));
 // A = val conj(S) for the back projection.
 // One block per (64 slots, replica): 8 waves each sum every 8th chunk
-// partial of their 64 slots in FP64 (coalesced rows of 64 slots), then one
-// wave combines the 8 sums in fixed order and finishes the slot.
+// partial of their 64 slots in FP64 (coalesced rows of 64 slots) while a
+// ninth sums M2, then one wave combines the 8 sums in fixed order and
+// finishes the slot.
 // M2 (sum of c c^T per replica) is read from M2 or, when M2 is null, summed
 // from the per-task rows m2rows (6 values each).  With kblk, wave 0 also
 // stores the sum of its 64 slots' 10 values as row blockIdx.x of the
@@ -1935,13 +1936,14 @@ __global__ void __launch_bounds__(64 * WPB) k_recip_struct_mfma(State s, int rt0
 // (second half).  Entries of nz >= KZ and of columns past the last stay
 // zero from the allocation.
 template <typename R>
-__global__ void __launch_bounds__(512) k_recip_reduce(State s, const R *__restrict__ part, const double *__restrict__ M2,
+__global__ void __launch_bounds__(576) k_recip_reduce(State s, const R *__restrict__ part, const double *__restrict__ M2,
                                                        const double *__restrict__ m2rows, Cpx<R> *__restrict__ Aout,
                                                        double *__restrict__ slot_out, double *__restrict__ kblk,
                                                        int kblk_stride, _Float16 *__restrict__ bop,
                                                        const int *__restrict__ bexp) {
-  constexpr int kGroups = 8;
+  constexpr int kGroups = 8;  // waves summing chunk partials; wave kGroups sums M2
   __shared__ double sh[kGroups][6][64];
+  __shared__ double msh[6];
   const int rep = s.rep_begin + blockIdx.y;
   const RepDev &Rp = s.reps[rep];
   const int nslot = Rp.ncol * Rp.KZ;
@@ -1954,7 +1956,19 @@ __global__ void __launch_bounds__(512) k_recip_reduce(State s, const R *__restri
   // offset of entry e's component 0 pair within a chunk; component a at + 2 side a
   const long eoff = (long)em * (6 * side) + 2 * enx;
   double Q[6] = {0, 0, 0, 0, 0, 0};
-  if (act) {
+  if (grp == kGroups) {
+    // M2 (sum of c c^T of the replica) while the other waves sum chunks
+    double m[6];
+    if (M2) {
+#pragma unroll
+      for (int q = 0; q < 6; ++q) m[q] = M2[rep * 6 + q];
+    } else {
+      wave_rows_sum<6>(m2rows, Rp.task0, Rp.ntask, lane, m);
+    }
+    if (lane == 0)
+#pragma unroll
+      for (int q = 0; q < 6; ++q) msh[q] = m[q];
+  } else if (act) {
     // four chunk partials in flight at a time, added in chunk order
     int c = grp;
     for (; c + 3 * kGroups < Rp.nrtask; c += 4 * kGroups) {
@@ -1982,8 +1996,9 @@ __global__ void __launch_bounds__(512) k_recip_reduce(State s, const R *__restri
       }
     }
   }
+  if (grp < kGroups)
 #pragma unroll
-  for (int q = 0; q < 6; ++q) sh[grp][q][lane] = Q[q];
+    for (int q = 0; q < 6; ++q) sh[grp][q][lane] = Q[q];
   __syncthreads();
   if (grp != 0) return;
 #pragma unroll
@@ -1993,12 +2008,8 @@ __global__ void __launch_bounds__(512) k_recip_reduce(State s, const R *__restri
     Q[q] = t;
   }
   double m[6];
-  if (M2) {
 #pragma unroll
-    for (int q = 0; q < 6; ++q) m[q] = M2[rep * 6 + q];
-  } else {
-    wave_rows_sum<6>(m2rows, Rp.task0, Rp.ntask, lane, m);
-  }
+  for (int q = 0; q < 6; ++q) m[q] = msh[q];
   const int nyi = em / Rp.KZ, nz = em % Rp.KZ;
   const int sl = (nyi * side + enx) * Rp.KZ + nz;  // the slot of entry e
   const int nx = enx - kmax, ny = nyi - kmax;
@@ -3882,10 +3893,10 @@ void launch_recip_reduce(const State &s, bool mixed, const void *part, const dou
                          hipStream_t st) {
   const dim3 grid(nblk(maxslot, 64), s.rep_end - s.rep_begin);
   if (mixed)
-    MSC_LAUNCH(k_recip_reduce<float>, grid, dim3(512), 0, st, s, (const float *)part, M2, m2rows,
+    MSC_LAUNCH(k_recip_reduce<float>, grid, dim3(576), 0, st, s, (const float *)part, M2, m2rows,
                (Cpx<float> *)A, slot_out, kblk, nblk(maxslot, 64), (_Float16 *)bop, bexp);
   else
-    MSC_LAUNCH(k_recip_reduce<double>, grid, dim3(512), 0, st, s, (const double *)part, M2, m2rows,
+    MSC_LAUNCH(k_recip_reduce<double>, grid, dim3(576), 0, st, s, (const double *)part, M2, m2rows,
                (Cpx<double> *)A, slot_out, kblk, nblk(maxslot, 64), (_Float16 *)nullptr, (const int *)nullptr);
 }
 
