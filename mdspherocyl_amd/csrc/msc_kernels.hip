@@ -167,13 +167,31 @@ __device__ __forceinline__ R fmaR(R a, R b, R c) { return fma(a, b, c); }
 // --------------------------------------------------------------------------
 // From the centre r and orientation O in registers; slot 0's lab position
 // and orientation are also returned in pos0, ori0 (when not null).
+// Constants of slot 0 and of the rod, for callers that load them with the
+// particle's state: the loads inside dipoles_regs come after stores, which
+// they cannot be moved above, so each would be a memory round trip of its own.
+struct Slot0 {
+  double pc, c[3], mu, len, dia;
+};
+__device__ __forceinline__ Slot0 load_slot0(const State &s, int g) {
+  const int N = s.Ntot;
+  Slot0 k;
+  k.pc = s.dpc[g];
+  for (int a = 0; a < 3; ++a) k.c[a] = s.doc[a * N + g];
+  k.mu = s.dmu[g];
+  k.len = s.len[g];
+  k.dia = s.dia[g];
+  return k;
+}
 __device__ __forceinline__ void dipoles_regs(const State &s, int g, const double r[3], const double O[9],
-                                             double *pos0, double *ori0) {
+                                             double *pos0, double *ori0, const Slot0 *k0 = nullptr) {
   const int N = s.Ntot;
   for (int w = 0; w < s.nd; ++w) {
     int q = w * N + g;
-    double pc = s.dpc[q];
-    double c[3] = {s.doc[(3 * w + 0) * N + g], s.doc[(3 * w + 1) * N + g], s.doc[(3 * w + 2) * N + g]};
+    const bool pre = k0 && w == 0;
+    double pc = pre ? k0->pc : s.dpc[q];
+    double c[3];
+    for (int a = 0; a < 3; ++a) c[a] = pre ? k0->c[a] : s.doc[(3 * w + a) * N + g];
     double dp[3], dor[3];
     for (int a = 0; a < 3; ++a) {
       dp[a] = r[a] + pc * O[6 + a];
@@ -197,7 +215,7 @@ __device__ __forceinline__ void dipoles_regs(const State &s, int g, const double
         f[a] = (float)dor[a];
         f[3 + a] = (float)(pc * O[6 + a]);
       }
-      f[6] = (float)s.dmu[q];
+      f[6] = (float)(pre ? k0->mu : s.dmu[q]);
       f[7] = 0.f;
       float *of = reinterpret_cast<float *>(o + 4);
       for (int k = 0; k < 8; ++k) of[k] = f[k];
@@ -210,8 +228,8 @@ __device__ __forceinline__ void dipoles_regs(const State &s, int g, const double
       o[a] = r[a];
       o[3 + a] = O[6 + a];
     }
-    o[6] = s.len[g];
-    o[7] = s.dia[g];
+    o[6] = k0 ? k0->len : s.len[g];
+    o[7] = k0 ? k0->dia : s.dia[g];
   }
 }
 __device__ __forceinline__ void dipoles_particle(const State &s, int g) {
@@ -3534,6 +3552,7 @@ __global__ void __launch_bounds__(256) k_drift(State s, const Thermo *__restrict
         se[3 + a] = s.snap_e[(3 + a) * N + g];
       }
       for (int q = 0; q < 9; ++q) O[q] = s.ori[q * N + g];
+      const Slot0 k0 = load_slot0(s, g);
       if (renorm) renorm_regs(O);
       const Thermo Th = th[tk.rep];
       const double ts[2] = {thermo_half(Rp, Th.s[0], gs[tk.rep * 4 + 0]),
@@ -3541,7 +3560,7 @@ __global__ void __launch_bounds__(256) k_drift(State s, const Thermo *__restrict
       kick_regs<false>(Rp, mass, I, O, F, T, ts, p, l, acc);
       drift_regs(Rp, mass, I, p, l, x, O);
       double pos0[3], ori0[3];
-      dipoles_regs(s, g, x, O, pos0, ori0);
+      dipoles_regs(s, g, x, O, pos0, ori0, &k0);
       for (int a = 0; a < 3; ++a) {
         s.x[a * N + g] = x[a];
         s.p[a * N + g] = p[a];
@@ -3566,7 +3585,7 @@ __global__ void __launch_bounds__(256) k_drift(State s, const Thermo *__restrict
       // own rows the bound also covers the dipoles (levers up to lever_max)
       const double arm = Rp.own_lists ? fmax(0.5 * len, s.lever_max) : 0.5 * len;
       me = sqrt(dr2) + arm * sqrt(du2);
-      recip_amp_regs<R>(s, g, s.dmu[g], ori0, amp, m2);
+      recip_amp_regs<R>(s, g, k0.mu, ori0, amp, m2);
     }
     wave_store_sums(rows1 + (size_t)tid * 16, acc, 2, lane);
     m = wave_max(m);
@@ -3655,6 +3674,14 @@ __global__ void __launch_bounds__(256) k_end(State s, Thermo *__restrict__ th, c
   const int rp = s.rep_begin + blockIdx.x;
   const RepDev &R = s.reps[rp];
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  // the thermostat and its deviates, loaded before the row sums so that
+  // their latency is not added to the serial part below
+  Thermo T;
+  double g4[4];
+  if (threadIdx.x == 0) {
+    T = th[rp];
+    for (int q = 0; q < 4; ++q) g4[q] = gs[rp * 4 + q];
+  }
   if (wv == 0) {
     double a[2], b[14];
     wave_rows_sum<2>(rows, R.task0, R.ntask, lane, a);
@@ -3689,10 +3716,9 @@ __global__ void __launch_bounds__(256) k_end(State s, Thermo *__restrict__ th, c
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    Thermo T = th[rp];
     for (int i = 0; i < 2; ++i) {
-      const double s1 = thermo_mode1(R, thermo_half(R, T.s[i], gs[rp * 4 + 2 * i]), v1[i] / 2);
-      T.strn[i] = gs[rp * 4 + 2 * i + 1];
+      const double s1 = thermo_mode1(R, thermo_half(R, T.s[i], g4[2 * i]), v1[i] / 2);
+      T.strn[i] = g4[2 * i + 1];
       T.s[i] = thermo_half(R, s1, T.strn[i]);
     }
     th[rp] = T;
