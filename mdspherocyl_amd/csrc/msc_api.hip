@@ -500,6 +500,7 @@ void refresh_state_ptrs(msc_ctx *ctx) {
   s.nd = ctx->nd;
   s.nrep = ctx->nrep;
   s.ntask = (int)ctx->tasks.size();
+  s.ntask_all = (int)ctx->tasks.size();
   s.nrtask = (int)ctx->rtasks.size();
   s.ncells = ctx->ncells;
   s.reps = ctx->d_reps;
@@ -1917,7 +1918,9 @@ static void enqueue_step(msc_ctx *ctx) {
     }
     dipR_launches(ctx, MSC_ALL, rows + kRowR * region);
   }
-  dipK_launches(ctx, MSC_ALL, rows + kRowDrift * region + 2, ctx->d_kblk, overlap);
+  // the drift's rows hold m2 as values 2..7 (value k of task t at
+  // k * ntask_all + t)
+  dipK_launches(ctx, MSC_ALL, rows + kRowDrift * region + 2 * (long)ctx->tasks.size(), ctx->d_kblk, overlap);
   // ext_field adds exactly nothing without a field: skipped then
   const bool field = any_field(ctx);
   if (field) launch_ext_field(s, rows + kRowExt * region, st);

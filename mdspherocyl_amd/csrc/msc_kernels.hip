@@ -114,12 +114,15 @@ __device__ __forceinline__ double wave_max(double v) {
   for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
   return v;
 }
-// fixed-order sum of a value over the 64 lanes of a wave (tree on lane
-// index), identical on every run
-__device__ __forceinline__ void wave_store_sums(double *dst, const double *v, int n, int lane) {
+// Per-task rows are value-major: value k of task t at rows[k * stride + t]
+// (stride = State::ntask_all), so the waves that re-sum a replica's tasks
+// read 64 consecutive tasks per load.
+// fixed-order sums of n values over the 64 lanes of a wave (tree on lane
+// index, identical on every run), stored as task tid's row
+__device__ __forceinline__ void wave_store_row(double *rows, int stride, int tid, const double *v, int n, int lane) {
   for (int k = 0; k < n; ++k) {
     double s = wave_sum(v[k]);
-    if (lane == 0) dst[k] = s;
+    if (lane == 0) rows[(size_t)k * stride + tid] = s;
   }
 }
 
@@ -1007,7 +1010,7 @@ __global__ void __launch_bounds__(64 * kPairSplit) k_force_sc(State s, double *_
       s.T[a * N + g] += T[a];
     }
   }
-  wave_store_sums(task_out + (size_t)tid * 16, acc, 12, lane);
+  wave_store_row(task_out, s.ntask_all, tid, acc, 12, lane);
 }
 
 // --------------------------------------------------------------------------
@@ -1217,7 +1220,7 @@ __global__ void __launch_bounds__(256) k_dipR_all(State s, const R *__restrict__
       s.T[a * s.Ntot + g] += T[a];
     }
   }
-  wave_store_sums(task_out + (size_t)tid * 16, acc, 11, lane);
+  wave_store_row(task_out, s.ntask_all, tid, acc, 11, lane);
 }
 
 // cut-off mode over the real-space pair list
@@ -1262,7 +1265,7 @@ __global__ void __launch_bounds__(64 * kPairSplit) k_dipR_list(State s, const R 
       s.T[a * s.Ntot + g] += T[a];
     }
   }
-  wave_store_sums(task_out + (size_t)tid * 16, acc, 11, lane);
+  wave_store_row(task_out, s.ntask_all, tid, acc, 11, lane);
 }
 
 // --------------------------------------------------------------------------
@@ -1484,8 +1487,8 @@ __global__ void __launch_bounds__(64 * PS) k_pairs_mixed(State s, const float *_
       s.T[a * N + g] = t;
     }
   }
-  wave_store_sums(sc_out + (size_t)tid * 16, accS, 12, lane);
-  wave_store_sums(r_out + (size_t)tid * 16, accR, 11, lane);
+  wave_store_row(sc_out, s.ntask_all, tid, accS, 12, lane);
+  wave_store_row(r_out, s.ntask_all, tid, accR, 11, lane);
 }
 
 // --------------------------------------------------------------------------
@@ -1511,7 +1514,7 @@ __global__ void __launch_bounds__(256) k_ext_field(State s, double *__restrict__
     for (int a = 0; a < 3; ++a) s.T[a * N + g] += T[a];
   }
   double v[1] = {U};
-  wave_store_sums(task_out + (size_t)tid * 16, v, 1, lane);
+  wave_store_row(task_out, s.ntask_all, tid, v, 1, lane);
 }
 
 __global__ void __launch_bounds__(256) k_self_ener(State s, double *__restrict__ task_out) {
@@ -1538,7 +1541,7 @@ __global__ void __launch_bounds__(256) k_self_ener(State s, double *__restrict__
       }
   }
   double vv[1] = {U};
-  wave_store_sums(task_out + (size_t)tid * 16, vv, 1, lane);
+  wave_store_row(task_out, s.ntask_all, tid, vv, 1, lane);
 }
 
 // --------------------------------------------------------------------------
@@ -1611,7 +1614,7 @@ __global__ void __launch_bounds__(256) k_recip_tables(State s, Cpx<R> *__restric
     for (int a = 0; a < 3; ++a) recip_phase_axis<R>(s, Rp, g, a, recip_theta(s, Rp, g, a), ph);
     recip_amp_particle<R>(s, g, amp, m2);
   }
-  wave_store_sums(task_out + (size_t)tid * 16, m2, 6, lane);
+  wave_store_row(task_out, s.ntask_all, tid, m2, 6, lane);
 }
 
 // Structure factors Q_a(k) = sum_i c_a,i e^{-i k.r_i} over a chunk of
@@ -1695,7 +1698,8 @@ __global__ void __launch_bounds__(256) k_recip_struct(State s, int rt0, const Cp
 // partials in either lane), and every wave that calls this with the same
 // rows gets the same result.
 template <int NV>
-__device__ __forceinline__ void wave_rows_sum(const double *__restrict__ rows, int t0, int nt, int lane, double *out) {
+__device__ __forceinline__ void wave_rows_sum(const double *__restrict__ rows, int stride, int t0, int nt, int lane,
+                                              double *out) {
   double v[NV];
 #pragma unroll
   for (int k = 0; k < NV; ++k) v[k] = 0.0;
@@ -1706,16 +1710,15 @@ __device__ __forceinline__ void wave_rows_sum(const double *__restrict__ rows, i
 #pragma unroll
     for (int u = 0; u < 8; ++u)
 #pragma unroll
-      for (int k = 0; k < NV; ++k) r[u][k] = rows[(size_t)(t0 + t + u * 64) * 16 + k];
+      for (int k = 0; k < NV; ++k) r[u][k] = rows[(size_t)k * stride + t0 + t + u * 64];
 #pragma unroll
     for (int u = 0; u < 8; ++u)
 #pragma unroll
       for (int k = 0; k < NV; ++k) v[k] += r[u][k];
   }
   for (; t < nt; t += 64) {
-    const double *r = rows + (size_t)(t0 + t) * 16;
 #pragma unroll
-    for (int k = 0; k < NV; ++k) v[k] += r[k];
+    for (int k = 0; k < NV; ++k) v[k] += rows[(size_t)k * stride + t0 + t];
   }
 #pragma unroll
   for (int k = 0; k < NV; ++k) out[k] = wave_sum(v[k]);
@@ -1981,7 +1984,7 @@ __global__ void __launch_bounds__(576) k_recip_reduce(State s, const R *__restri
 #pragma unroll
       for (int q = 0; q < 6; ++q) m[q] = M2[rep * 6 + q];
     } else {
-      wave_rows_sum<6>(m2rows, Rp.task0, Rp.ntask, lane, m);
+      wave_rows_sum<6>(m2rows, s.ntask_all, Rp.task0, Rp.ntask, lane, m);
     }
     if (lane == 0)
 #pragma unroll
@@ -2739,11 +2742,12 @@ __global__ void __launch_bounds__(256) k_reduce_items(const RepDev *__restrict__
   double v[kMaxV];
 #pragma unroll
   for (int k = 0; k < kMaxV; ++k) v[k] = 0.0;
+  // WHICH 0: value-major task rows, istride = the value stride; WHICH 1:
+  // slot rows of istride values
   for (int i = threadIdx.x; i < n; i += 256) {
-      const double *row = in + (size_t)(b + i) * istride;
 #pragma unroll
     for (int k = 0; k < kMaxV; ++k)
-      if (k < nt) v[k] += row[k];
+      if (k < nt) v[k] += WHICH == 0 ? in[(size_t)k * istride + b + i] : in[(size_t)(b + i) * istride + k];
   }
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
 #pragma unroll
@@ -3322,7 +3326,7 @@ __global__ void __launch_bounds__(256) k_moveP(State s, const Thermo *__restrict
     const Thermo T = th[tk.rep];
     kick_particle<LAST>(s, Rp, tk.g0 + lane, T.s, acc);
   }
-  wave_store_sums(task_out + (size_t)tid * 16, acc, LAST ? 14 : 8, lane);
+  wave_store_row(task_out, s.ntask_all, tid, acc, LAST ? 14 : 8, lane);
 }
 
 // translate + rotate (5-factor Rx Ry Rz Ry Rx splitting) in registers: x
@@ -3587,13 +3591,14 @@ __global__ void __launch_bounds__(256) k_drift(State s, const Thermo *__restrict
       me = sqrt(dr2) + arm * sqrt(du2);
       recip_amp_regs<R>(s, g, k0.mu, ori0, amp, m2);
     }
-    wave_store_sums(rows1 + (size_t)tid * 16, acc, 2, lane);
+    wave_store_row(rows1, s.ntask_all, tid, acc, 2, lane);
     m = wave_max(m);
     me = wave_max(me);
-    double *o = rows + (size_t)tid * 16;
+    const size_t T = s.ntask_all;
+    double *o = rows + tid;  // value k at o[k * T]
     if (lane == 0) {
       o[0] = m;
-      o[1] = me;
+      o[T] = me;
       if (ls.dmax) {
         // replica maxima for the list decision in kDmaxBuckets buckets
         // (fewer tasks per address; non-negative doubles order as their bit
@@ -3611,7 +3616,7 @@ __global__ void __launch_bounds__(256) k_drift(State s, const Thermo *__restrict
     }
     for (int q = 0; q < 6; ++q) {
       const double t = wave_sum(m2[q]);
-      if (lane == 0) o[2 + q] = t;
+      if (lane == 0) o[(2 + q) * T] = t;
     }
   }
   __syncthreads();
@@ -3639,13 +3644,13 @@ __global__ void __launch_bounds__(256) k_kick2(State s, const Thermo *__restrict
   const Task tk = s.tasks[tid];
   const RepDev &Rp = s.reps[tk.rep];
   double K1[2];
-  wave_rows_sum<2>(rows1, Rp.task0, Rp.ntask, lane, K1);
+  wave_rows_sum<2>(rows1, s.ntask_all, Rp.task0, Rp.ntask, lane, K1);
   const Thermo T = th[tk.rep];
   double ts[2];
   for (int i = 0; i < 2; ++i) ts[i] = thermo_mode1(Rp, thermo_half(Rp, T.s[i], gs[tk.rep * 4 + 2 * i]), K1[i] / 2);
   double acc[14] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
   if (lane < tk.n) kick_particle<true>(s, Rp, tk.g0 + lane, ts, acc);
-  wave_store_sums(rows2 + (size_t)tid * 16, acc, 14, lane);
+  wave_store_row(rows2, s.ntask_all, tid, acc, 14, lane);
 }
 
 // One block (4 waves) per replica.  rows + k*region is row set k (kick1,
@@ -3684,20 +3689,20 @@ __global__ void __launch_bounds__(256) k_end(State s, Thermo *__restrict__ th, c
   }
   if (wv == 0) {
     double a[2], b[14];
-    wave_rows_sum<2>(rows, R.task0, R.ntask, lane, a);
-    wave_rows_sum<14>(rows + 5 * region, R.task0, R.ntask, lane, b);
+    wave_rows_sum<2>(rows, s.ntask_all, R.task0, R.ntask, lane, a);
+    wave_rows_sum<14>(rows + 5 * region, s.ntask_all, R.task0, R.ntask, lane, b);
     if (lane < 2) v1[lane] = lane == 0 ? a[0] : a[1];
     if (lane == 0)
       for (int k = 0; k < 14; ++k) v2[k] = b[k];
   } else if (wv == 1) {
     double a[12];
-    wave_rows_sum<12>(rows + 2 * region, R.task0, R.ntask, lane, a);
+    wave_rows_sum<12>(rows + 2 * region, s.ntask_all, R.task0, R.ntask, lane, a);
     if (lane == 0)
       for (int k = 0; k < 12; ++k) vsc[k] = a[k];
   } else if (wv == 2) {
     double a[11], e[1] = {0.0};
-    wave_rows_sum<11>(rows + 3 * region, R.task0, R.ntask, lane, a);
-    if (has_field) wave_rows_sum<1>(rows + 4 * region, R.task0, R.ntask, lane, e);
+    wave_rows_sum<11>(rows + 3 * region, s.ntask_all, R.task0, R.ntask, lane, a);
+    if (has_field) wave_rows_sum<1>(rows + 4 * region, s.ntask_all, R.task0, R.ntask, lane, e);
     if (lane == 0) {
       for (int k = 0; k < 11; ++k) vr[k] = a[k];
       vext[0] = e[0];
@@ -3839,9 +3844,9 @@ void launch_self_ener(const State &s, double *task_out, hipStream_t st) {
 }
 void launch_reduce(const State &s, const double *in, int istride, int nv, int which, double *out, int ostride,
                    int ooff, hipStream_t st, int nv2, double *out2, int o2stride, int o2off) {
-  if (which == 0)
-    MSC_LAUNCH(k_reduce_items<0>, dim3(s.rep_end - s.rep_begin), dim3(256), 0, st, s.reps, s.rep_begin, in, istride,
-               nv, out, ostride, ooff, nv2, out2, o2stride, o2off);
+  if (which == 0)  // value-major task rows: the value stride is every task of the context
+    MSC_LAUNCH(k_reduce_items<0>, dim3(s.rep_end - s.rep_begin), dim3(256), 0, st, s.reps, s.rep_begin, in,
+               s.ntask_all, nv, out, ostride, ooff, nv2, out2, o2stride, o2off);
   else
     MSC_LAUNCH(k_reduce_items<1>, dim3(s.rep_end - s.rep_begin), dim3(256), 0, st, s.reps, s.rep_begin, in, istride,
                nv, out, ostride, ooff, nv2, out2, o2stride, o2off);

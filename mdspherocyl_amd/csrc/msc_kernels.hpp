@@ -10,6 +10,8 @@ namespace msc {
 struct State {
   int Ntot, nd, nrep;
   int ntask, nrtask, ncells;  // ntask: tasks in the selected range
+  int ntask_all;             // every wave task of the context: the value stride
+                             // of per-task rows (value k of task t at k * ntask_all + t)
   int task_begin;            // first selected wave task
   int g_begin, g_end;        // particle range of per-particle kernels
   int rep_begin, rep_end;    // replica range of per-replica reductions
@@ -99,8 +101,10 @@ void launch_pairs_mixed(const State &s, int waves, const void *tab, double *sc_o
                         const ListStep &ls, hipStream_t st);
 void launch_ext_field(const State &s, double *task_out, hipStream_t st);
 void launch_self_ener(const State &s, double *task_out, hipStream_t st);
-// out[rep*ostride + ooff + k] = replica sum of in[item*istride + k], k < nv;
-// with nv2 > 0 the following nv2 values go to out2[rep*o2stride + o2off + k]
+// out[rep*ostride + ooff + k] = replica sum of value k < nv of its items:
+// which 0, wave tasks, value-major rows in[k*ntask_all + task] (istride is
+// not used); which 1, k slots, in[slot*istride + k].  With nv2 > 0 the
+// following nv2 values go to out2[rep*o2stride + o2off + k]
 void launch_reduce(const State &s, const double *in, int istride, int nv, int which, double *out, int ostride,
                    int ooff, hipStream_t st, int nv2 = 0, double *out2 = nullptr, int o2stride = 0, int o2off = 0);
 void launch_recip_tables(const State &s, bool mixed, void *ph, void *amp, double *task_out, hipStream_t st);
