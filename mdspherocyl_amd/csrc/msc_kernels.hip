@@ -106,13 +106,43 @@ hipError_t take_launch_error(const char **kernel) {
 // --------------------------------------------------------------------------
 // small helpers
 // --------------------------------------------------------------------------
+// Wave reductions without the LDS crossbar.  Four DPP steps pair each lane
+// with lane ^ 1 (quad_perm [1,0,3,2]), lane ^ 2 (quad_perm [2,3,0,1]),
+// 7 - i within its 8 lanes (row_half_mirror) and 15 - i within its 16
+// lanes (row_mirror); both lanes of a pair add the same two values, so after
+// the four steps each 16-lane row holds its sum, bit-identical in every lane.
+// The row sums are read from lanes 0, 16, 32, 48 (v_readlane) and combined
+// as (r0 + r1) + (r2 + r3).  Every lane of the wave must be active.  A DPP
+// move costs a few cycles where a ds_bpermute round trip (__shfl_xor) costs
+// about a hundred, and the per-task row sums run one such reduction per
+// value.
+constexpr int kDppXor1 = 0xB1, kDppXor2 = 0x4E, kDppRowMirror = 0x140, kDppHalfMirror = 0x141;
+template <int CTRL>
+__device__ __forceinline__ double dpp_lane(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xf, 0xf, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ double read_lane(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
 __device__ __forceinline__ double wave_sum(double v) {
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-  return v;
+  v += dpp_lane<kDppXor1>(v);
+  v += dpp_lane<kDppXor2>(v);
+  v += dpp_lane<kDppHalfMirror>(v);
+  v += dpp_lane<kDppRowMirror>(v);
+  return (read_lane(v, 0) + read_lane(v, 16)) + (read_lane(v, 32) + read_lane(v, 48));
 }
 __device__ __forceinline__ double wave_max(double v) {
-  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
-  return v;
+  v = fmax(v, dpp_lane<kDppXor1>(v));
+  v = fmax(v, dpp_lane<kDppXor2>(v));
+  v = fmax(v, dpp_lane<kDppHalfMirror>(v));
+  v = fmax(v, dpp_lane<kDppRowMirror>(v));
+  return fmax(fmax(read_lane(v, 0), read_lane(v, 16)), fmax(read_lane(v, 32), read_lane(v, 48)));
 }
 // Per-task rows are value-major: value k of task t at rows[k * stride + t]
 // (stride = State::ntask_all), so the waves that re-sum a replica's tasks

@@ -126,6 +126,37 @@ inline T __shfl(T v, int src) {
   return (T)r;
 }
 
+// DPP moves (quad_perm, row_mirror, row_half_mirror) and v_readlane
+namespace hipemu {
+inline int update_dpp(int src, int ctrl) {
+  Block *b = cur_block();
+  unsigned t = t_threadIdx.x, w = t / 64, lane = t % 64;
+  double *x = b->xch.data() + w * 64;
+  x[lane] = (double)src;
+  b->wbar[w]->arrive_and_wait();
+  unsigned from = lane;
+  if (ctrl < 0x100) from = (lane & ~3u) | ((unsigned)(ctrl >> (2 * (lane & 3))) & 3u);
+  else if (ctrl == 0x140) from = (lane & ~15u) | (15u - (lane & 15u));
+  else if (ctrl == 0x141) from = (lane & ~7u) | (7u - (lane & 7u));
+  else std::abort();
+  int r = (int)x[from];
+  b->wbar[w]->arrive_and_wait();
+  return r;
+}
+inline int readlane(int v, int l) {
+  Block *b = cur_block();
+  unsigned t = t_threadIdx.x, w = t / 64, lane = t % 64;
+  double *x = b->xch.data() + w * 64;
+  x[lane] = (double)v;
+  b->wbar[w]->arrive_and_wait();
+  int r = (int)x[l & 63];
+  b->wbar[w]->arrive_and_wait();
+  return r;
+}
+}  // namespace hipemu
+#define __builtin_amdgcn_update_dpp(old, src, ctrl, rm, bm, bc) hipemu::update_dpp((src), (ctrl))
+#define __builtin_amdgcn_readlane(v, l) hipemu::readlane((v), (l))
+
 // all 64 lanes of a wave take part (the kernels only call these from
 // wave-uniform control flow)
 inline unsigned long long __ballot(int pred) {
