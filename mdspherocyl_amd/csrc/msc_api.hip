@@ -127,6 +127,7 @@ struct msc_ctx {
   unsigned long long *d_corr_drop = nullptr;
   double *d_order = nullptr;  // [kOrderSums] order-parameter sums
   double *d_rows = nullptr, *d_kblk = nullptr;  // per-task row sets and k-space block sums of the step loop
+  double *d_endsum = nullptr;  // end-of-step sums per replica (kEndSums each)
   int *d_step = nullptr;  // {step within the run, first absolute step of the run}
   int *d_cell_of = nullptr, *d_cell_cnt = nullptr, *d_cell_start = nullptr, *d_cell_fill = nullptr,
       *d_cell_list = nullptr;
@@ -1005,7 +1006,8 @@ int msc_create(int device, int precision, int nrep, const msc_replica_desc *reps
   A(&ctx->d_clu, N);
   A(&ctx->d_clu_flag, 2);
   A(&ctx->d_rows, ctx->tasks.size() * 16 * kRowSets);
-  A(&ctx->d_kblk, (size_t)nrep * ((ctx->maxslot + 63) / 64) * 10);
+  A(&ctx->d_kblk, (size_t)nrep * ((ctx->maxslot + kKblkSlots - 1) / kKblkSlots) * 10);
+  A(&ctx->d_endsum, (size_t)nrep * kEndSums);
   A(&ctx->d_step, 2);
   A(&ctx->d_cell_of, N); A(&ctx->d_cell_list, N);
   A(&ctx->d_cell_tmp, N);
@@ -1103,7 +1105,7 @@ void msc_destroy(msc_ctx *ctx) {
                   ctx->d_cell_cnt, ctx->d_cell_start, ctx->d_cell_fill, ctx->d_cell_list, ctx->d_nl_cnt,
                   ctx->d_nl_idx, ctx->d_dl_cnt, ctx->d_dl_idx, ctx->d_kval, ctx->d_cell_tmp, ctx->d_sl_cnt,
                   ctx->d_sl_idx, ctx->d_snap_e, ctx->d_srebuild, ctx->d_cell_pack, 
-                  ctx->d_snap_u, ctx->d_clu, ctx->d_clu_flag, ctx->d_corr, ctx->d_corr_drop, ctx->d_order, ctx->d_rows, ctx->d_kblk, ctx->d_step,
+                  ctx->d_snap_u, ctx->d_clu, ctx->d_clu_flag, ctx->d_corr, ctx->d_corr_drop, ctx->d_order, ctx->d_rows, ctx->d_kblk, ctx->d_endsum, ctx->d_step,
                   ctx->d_tab, ctx->d_ph, ctx->d_amp, ctx->d_part, ctx->d_A, ctx->d_slot_out, ctx->d_task_out,
                   ctx->d_red, ctx->d_M2, ctx->d_Kin, ctx->d_gs, ctx->d_obs, ctx->d_counts, ctx->d_E,
                   ctx->d_th, ctx->d_flags, ctx->d_rebuild, ctx->d_one, ctx->d_run_gs, ctx->d_run_obs, ctx->d_save, ctx->d_tune_save,
@@ -1927,12 +1929,13 @@ static void enqueue_step(msc_ctx *ctx) {
   {
     Bracket b(ctx, "integrate");
     launch_kick2(s, ctx->d_th, gs, ctx->d_step, rows + kRowKick1 * region, rows + kRowKick2 * region, ls, st);
-    launch_end(s, ctx->d_th, gs, ctx->d_step, rows, region, field, ctx->d_kblk, ctx->maxslot, ctx->d_E, ctx->d_Kin,
-               ctx->d_counts, ctx->d_run_obs, ls, st);
+    launch_end(s, ctx->d_step, gs, rows, region, field, ctx->d_kblk, ctx->maxslot, ctx->d_endsum, ls, st);
   }
   {
+    // the rows, and the thermostat, energies and observables from k_end's sums
     Bracket b(ctx, "nlist");
-    launch_lists_step(s, ls, ctx->d_flags, ctx->d_step, st);
+    const EndArgs ea = {ctx->d_endsum, ctx->d_th, ctx->d_E, ctx->d_Kin, ctx->d_counts, ctx->d_run_obs};
+    launch_lists_step(s, ls, ctx->d_flags, ctx->d_step, ea, st);
   }
 }
 

@@ -806,10 +806,18 @@ __global__ void __launch_bounds__(256) k_short_build(State s, const int *__restr
 // the replicas' flags (one load per lane), so that in a step without either
 // every wave ends after one load.  Advances the step counter: every kernel
 // of the step that reads it has finished.
+__device__ void end_serial(const State &s, int rp, const EndArgs &ea);
+// Extra blocks beyond the row grid (one per replica, when ea.sums is set)
+// finish the step's thermostat, energies and observables (end_serial).
 __global__ void __launch_bounds__(256) k_lists_step(State s, ListStep ls, int *__restrict__ flags,
-                                                    int *__restrict__ step) {
+                                                    int *__restrict__ step, EndArgs ea) {
   __shared__ int queue[4][128];
   __shared__ int run_b[4][64], run_e[4][64];
+  const int nmain = (int)gridDim.x - (ea.sums ? s.rep_end - s.rep_begin : 0);
+  if ((int)blockIdx.x >= nmain) {
+    if (threadIdx.x == 0) end_serial(s, s.rep_begin + (int)blockIdx.x - nmain, ea);
+    return;
+  }
   if (blockIdx.x == 0 && threadIdx.x == 0) step[0] += 1;
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   bool any = false;
@@ -822,7 +830,7 @@ __global__ void __launch_bounds__(256) k_lists_step(State s, ListStep ls, int *_
   // rebuilds alone still spreads over every wave): the steps that rebuild
   // nothing cost one small grid
   const int n = s.g_end - s.g_begin;
-  for (int gi = xcd_block(blockIdx.x, gridDim.x) * 4 + wv; gi < n; gi += gridDim.x * 4) {
+  for (int gi = xcd_block(blockIdx.x, nmain) * 4 + wv; gi < n; gi += nmain * 4) {
     const int g = s.g_begin + __builtin_amdgcn_readfirstlane(gi);
     const int rp = __builtin_amdgcn_readfirstlane(s.rep_of[g]);
     if (ls.rebuild[rp])
@@ -2123,6 +2131,7 @@ __global__ void __launch_bounds__(576) k_recip_reduce(State s, const R *__restri
     }
   }
   if (kblk) {
+    static_assert(kKblkSlots == 64, "one k-space block row per 64-slot block");
     double *kb = kblk + ((size_t)rep * kblk_stride + blockIdx.x) * 10;
 #pragma unroll
     for (int q = 0; q < 10; ++q) {
@@ -2647,6 +2656,7 @@ __global__ void __launch_bounds__(64) k_recip_reduce_nd(State s, const double *_
     Aout[Rp.slot0 + sl] = A;
   }
   if (kblk) {
+    static_assert(kKblkSlots == 64, "one k-space block row per 64-slot block");
     double *kb = kblk + ((size_t)rep * kblk_stride + blockIdx.x) * 10;
     for (int q = 0; q < 10; ++q) {
       const double t = wave_sum(o[q]);
@@ -3683,106 +3693,152 @@ __global__ void __launch_bounds__(256) k_kick2(State s, const Thermo *__restrict
   wave_store_row(rows2, s.ntask_all, tid, acc, 14, lane);
 }
 
-// One block (4 waves) per replica.  rows + k*region is row set k (kick1,
-// drift, sc, R, ext, kick2).  Writes the thermostat after modes 0, 1 and 2
-// (k_thermo0 / k_step_scalars), the energies in the layout of RepEnergy,
-// Kin, the pair counts and the observables of print_out but the rebuild flag
-// (written with the decision in k_drift).  Extra blocks (ls.rebuild set,
-// beyond the replicas' blocks): the cell rank and packed records of rebuilt
-// replicas, one particle per thread.
-__global__ void __launch_bounds__(256) k_end(State s, Thermo *__restrict__ th, const double *__restrict__ gs_base,
-                                             const int *__restrict__ step, const double *__restrict__ rows,
-                                             long region, int has_field, const double *__restrict__ kblk,
-                                             int kblk_stride, RepEnergy *__restrict__ E, double *__restrict__ Kin,
-                                             double *__restrict__ counts, double *__restrict__ obs_base,
-                                             ListStep ls) {
-  const int nmain = s.rep_end - s.rep_begin;
+// Sums of the end of a step, kEndBlocks blocks (4 waves each) per replica
+// (rows + k*region is row set k: kick1, drift, sc, R, ext, kick2).  Wave w
+// of the replica sums values w, w + W, w + 2 W, w + 3 W (W = 4 kEndBlocks)
+// of the list below, each over the replica's tasks (or k-space block rows)
+// in the order of wave_rows_sum, and stores them in sums[kEndSums rp + v]:
+//   0, 1    kick1 rows 0, 1 (twice the kinetic energies after the first kick)
+//   2 .. 9  kick2 rows 0, 1, 8 .. 13 (the same after the last kick; p and l)
+//   10 .. 21  sc rows 0 .. 11,  22 .. 32  R rows 0 .. 10,  33  ext row 0
+//   34 .. 43  the 10 k-space values over the kblk rows
+// and wave 0 stores the step index k as value kEndVals (k_lists_step, which
+// finishes the step from these sums, advances the step counter) and the
+// replica's four thermostat deviates of the step after it.  One wave
+// reading a replica's rows alone waits for many memory round trips: spread
+// over 12 waves on 3 CUs the reads take about one.
+// Extra blocks (ls.rebuild set, beyond the replicas' blocks): the cell rank
+// and packed records of rebuilt replicas, one particle per thread.
+__device__ __forceinline__ void end_value(const State &s, const RepDev &R, int rp, int v, const double *rows,
+                                          long region, int has_field, const double *kblk, int kblk_stride,
+                                          const double *&p, int &st, int &n) {
+  const long T = s.ntask_all;
+  st = 1;
+  n = R.ntask;
+  if (v < 2) {
+    p = rows + kRowKick1 * region + v * T + R.task0;
+  } else if (v < 10) {
+    const int k = v < 4 ? v - 2 : v + 4;
+    p = rows + kRowKick2 * region + k * T + R.task0;
+  } else if (v < 22) {
+    p = rows + kRowSC * region + (v - 10) * T + R.task0;
+  } else if (v < 33) {
+    p = rows + kRowR * region + (v - 22) * T + R.task0;
+  } else if (v < 34) {
+    p = rows + kRowExt * region + R.task0;
+    if (!has_field) n = 0;
+  } else {
+    p = kblk + (size_t)rp * kblk_stride * 10 + (v - 34);
+    st = 10;
+    n = (R.ncol * R.KZ + kKblkSlots - 1) / kKblkSlots;
+  }
+}
+__global__ void __launch_bounds__(256) k_end(State s, const int *__restrict__ step, const double *__restrict__ gs,
+                                             const double *__restrict__ rows, long region, int has_field,
+                                             const double *__restrict__ kblk, int kblk_stride,
+                                             double *__restrict__ sums, ListStep ls) {
+  const int nmain = (s.rep_end - s.rep_begin) * kEndBlocks;
   if ((int)blockIdx.x >= nmain) {
     const int g = s.g_begin + ((int)blockIdx.x - nmain) * 256 + (int)threadIdx.x;
     if (ls.rebuild && g < s.g_end) cells_rank_one(s, g, ls.rebuild);
     return;
   }
-  const int kstep = step[0];
-  const double *gs = gs_base + (size_t)kstep * s.nrep * 4;
-  double *obs = obs_base + (size_t)kstep * s.nrep * 12;
-  __shared__ double v1[2], v2[14], vsc[12], vr[11], vext[1], vk[10];
-  const int rp = s.rep_begin + blockIdx.x;
+  constexpr int W = 4 * kEndBlocks, PER = (kEndVals + W - 1) / W;
+  static_assert(PER <= 4, "at most four values per wave");
+  const int rp = s.rep_begin + (int)blockIdx.x / kEndBlocks;
   const RepDev &R = s.reps[rp];
-  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  // the thermostat and its deviates, loaded before the row sums so that
-  // their latency is not added to the serial part below
-  Thermo T;
-  double g4[4];
-  if (threadIdx.x == 0) {
-    T = th[rp];
-    for (int q = 0; q < 4; ++q) g4[q] = gs[rp * 4 + q];
+  const int w = ((int)blockIdx.x % kEndBlocks) * 4 + (int)(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (w == 0 && lane < 5) {
+    // the step index and the deviates, so that the finishing thread reads
+    // everything it needs in one round trip
+    const int k = step[0];
+    sums[(size_t)rp * kEndSums + kEndVals + lane] = lane == 0 ? (double)k : gs[((size_t)k * s.nrep + rp) * 4 + lane - 1];
   }
-  if (wv == 0) {
-    double a[2], b[14];
-    wave_rows_sum<2>(rows, s.ntask_all, R.task0, R.ntask, lane, a);
-    wave_rows_sum<14>(rows + 5 * region, s.ntask_all, R.task0, R.ntask, lane, b);
-    if (lane < 2) v1[lane] = lane == 0 ? a[0] : a[1];
-    if (lane == 0)
-      for (int k = 0; k < 14; ++k) v2[k] = b[k];
-  } else if (wv == 1) {
-    double a[12];
-    wave_rows_sum<12>(rows + 2 * region, s.ntask_all, R.task0, R.ntask, lane, a);
-    if (lane == 0)
-      for (int k = 0; k < 12; ++k) vsc[k] = a[k];
-  } else if (wv == 2) {
-    double a[11], e[1] = {0.0};
-    wave_rows_sum<11>(rows + 3 * region, s.ntask_all, R.task0, R.ntask, lane, a);
-    if (has_field) wave_rows_sum<1>(rows + 4 * region, s.ntask_all, R.task0, R.ntask, lane, e);
-    if (lane == 0) {
-      for (int k = 0; k < 11; ++k) vr[k] = a[k];
-      vext[0] = e[0];
+  const double *p[PER];
+  int st[PER], n[PER], nmax = 0;
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
+    const int v = w + j * W;
+    if (v < kEndVals) {
+      end_value(s, R, rp, v, rows, region, has_field, kblk, kblk_stride, p[j], st[j], n[j]);
+    } else {
+      p[j] = rows;
+      st[j] = 1;
+      n[j] = 0;
     }
-  } else {
-    const int nb = (R.ncol * R.KZ + 63) / 64;
-    double k[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for (int b = lane; b < nb; b += 64) {
-      const double *r = kblk + ((size_t)rp * kblk_stride + b) * 10;
-      for (int q = 0; q < 10; ++q) k[q] += r[q];
-    }
-    for (int q = 0; q < 10; ++q) {
-      const double t = wave_sum(k[q]);
-      if (lane == 0) vk[q] = t;
-    }
+    nmax = max(nmax, n[j]);
   }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    for (int i = 0; i < 2; ++i) {
-      const double s1 = thermo_mode1(R, thermo_half(R, T.s[i], g4[2 * i]), v1[i] / 2);
-      T.strn[i] = g4[2 * i + 1];
-      T.s[i] = thermo_half(R, s1, T.strn[i]);
-    }
-    th[rp] = T;
-    const double K2[2] = {v2[0] / 2, v2[1] / 2};
-    Kin[rp * 2 + 0] = K2[0];
-    Kin[rp * 2 + 1] = K2[1];
-    double *e = reinterpret_cast<double *>(E + rp);
-    for (int k = 0; k < 11; ++k) e[k] = vsc[k];
-    for (int k = 0; k < 10; ++k) e[11 + k] = vr[k];
-    for (int k = 0; k < 10; ++k) e[21 + k] = vk[k];
-    e[31] = vext[0];
-    counts[rp * 4 + 0] = vsc[11];
-    counts[rp * 4 + 1] = vr[10];
-    // observables of print_out (as k_step_scalars<2>)
-    const double Np = (double)R.N;
-    const double UR = vr[0], UK = vk[0];
-    const double WdipR = 3 * UR, WdipK = 3 * UK;
-    const double K = (K2[0] + K2[1]) / Np;
-    const double Uss = vsc[0] / Np, UdipR = UR / Np, UdipK = UK / Np, Uext = vext[0] / Np;
-    const double U = R.Uself + Uss + UdipR + UdipK + Uext;
-    const double Pss = vsc[1] / (3 * R.V), PdipR = WdipR / (3 * R.V), PdipK = WdipK / (3 * R.V);
-    const double P = K / 3 * R.dens + Pss + PdipR + PdipK;
-    double *o = obs + (size_t)rp * 12;
-    o[0] = K; o[1] = U; o[2] = P; o[3] = Uss; o[4] = Pss; o[5] = UdipR; o[6] = PdipR;
-    o[7] = UdipK; o[8] = PdipK;
-    const double *mo = v2 + 8;
-    o[9] = mo[0] * mo[0] + mo[1] * mo[1] + mo[2] * mo[2];
-    o[10] = mo[3] * mo[3] + mo[4] * mo[4] + mo[5] * mo[5];
+  // per lane: items lane, lane + 64, ... in order, eight per value in flight
+  double acc[PER];
+#pragma unroll
+  for (int j = 0; j < PER; ++j) acc[j] = 0.0;
+  for (int t = lane; t < nmax; t += 8 * 64) {
+    double r[PER][8];
+#pragma unroll
+    for (int j = 0; j < PER; ++j)
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int tt = t + 64 * u;
+        r[j][u] = tt < n[j] ? p[j][(size_t)tt * st[j]] : 0.0;
+      }
+#pragma unroll
+    for (int j = 0; j < PER; ++j)
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (t + 64 * u < n[j]) acc[j] += r[j][u];
   }
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
+    const int v = w + j * W;
+    const double t = wave_sum(acc[j]);
+    if (v < kEndVals && lane == 0) sums[(size_t)rp * kEndSums + v] = t;
+  }
+}
+
+// The end of a step for replica rp from k_end's sums (one thread): moveTher
+// mode 2 (k_thermo0 / k_step_scalars), Kin, the energies in the layout of
+// RepEnergy, the pair counts and the observables of print_out but the
+// rebuild flag (written with the decision).
+__device__ void end_serial(const State &s, int rp, const EndArgs &ea) {
+  const RepDev &R = s.reps[rp];
+  // every input loaded before the first store (the stores may alias them)
+  double v[kEndVals + 5];
+  for (int q = 0; q < kEndVals + 5; ++q) v[q] = ea.sums[(size_t)rp * kEndSums + q];
+  Thermo T = ea.th[rp];
+  const double Np = (double)R.N, V = R.V, dens = R.dens, Uself = R.Uself;
+  const int kstep = (int)v[kEndVals];
+  const double *g4 = v + kEndVals + 1;
+  for (int i = 0; i < 2; ++i) {
+    const double s1 = thermo_mode1(R, thermo_half(R, T.s[i], g4[2 * i]), v[i] / 2);
+    T.strn[i] = g4[2 * i + 1];
+    T.s[i] = thermo_half(R, s1, T.strn[i]);
+  }
+  ea.th[rp] = T;
+  const double K2[2] = {v[2] / 2, v[3] / 2};
+  ea.Kin[rp * 2 + 0] = K2[0];
+  ea.Kin[rp * 2 + 1] = K2[1];
+  const double *mo = v + 4, *vsc = v + 10, *vr = v + 22, *vk = v + 34;
+  const double vext = v[33];
+  double *e = reinterpret_cast<double *>(ea.E + rp);
+  for (int k = 0; k < 11; ++k) e[k] = vsc[k];
+  for (int k = 0; k < 10; ++k) e[11 + k] = vr[k];
+  for (int k = 0; k < 10; ++k) e[21 + k] = vk[k];
+  e[31] = vext;
+  ea.counts[rp * 4 + 0] = vsc[11];
+  ea.counts[rp * 4 + 1] = vr[10];
+  // observables of print_out (as k_step_scalars<2>)
+  const double UR = vr[0], UK = vk[0];
+  const double WdipR = 3 * UR, WdipK = 3 * UK;
+  const double K = (K2[0] + K2[1]) / Np;
+  const double Uss = vsc[0] / Np, UdipR = UR / Np, UdipK = UK / Np, Uext = vext / Np;
+  const double U = Uself + Uss + UdipR + UdipK + Uext;
+  const double Pss = vsc[1] / (3 * V), PdipR = WdipR / (3 * V), PdipK = WdipK / (3 * V);
+  const double P = K / 3 * dens + Pss + PdipR + PdipK;
+  double *o = ea.obs + ((size_t)kstep * s.nrep + rp) * 12;
+  o[0] = K; o[1] = U; o[2] = P; o[3] = Uss; o[4] = Pss; o[5] = UdipR; o[6] = PdipR;
+  o[7] = UdipK; o[8] = PdipK;
+  o[9] = mo[0] * mo[0] + mo[1] * mo[1] + mo[2] * mo[2];
+  o[10] = mo[3] * mo[3] + mo[4] * mo[4] + mo[5] * mo[5];
 }
 
 // completion stamp: written after an entry point's kernels, read back by the
@@ -3952,13 +4008,13 @@ int launch_recip_force(const State &s, bool mixed, int KZ, const void *ph, const
 void launch_recip_reduce(const State &s, bool mixed, const void *part, const double *M2, const double *m2rows,
                          void *A, double *slot_out, double *kblk, int maxslot, void *bop, const int *bexp,
                          hipStream_t st) {
-  const dim3 grid(nblk(maxslot, 64), s.rep_end - s.rep_begin);
+  const dim3 grid(nblk(maxslot, kKblkSlots), s.rep_end - s.rep_begin);
   if (mixed)
     MSC_LAUNCH(k_recip_reduce<float>, grid, dim3(576), 0, st, s, (const float *)part, M2, m2rows,
-               (Cpx<float> *)A, slot_out, kblk, nblk(maxslot, 64), (_Float16 *)bop, bexp);
+               (Cpx<float> *)A, slot_out, kblk, nblk(maxslot, kKblkSlots), (_Float16 *)bop, bexp);
   else
     MSC_LAUNCH(k_recip_reduce<double>, grid, dim3(576), 0, st, s, (const double *)part, M2, m2rows,
-               (Cpx<double> *)A, slot_out, kblk, nblk(maxslot, 64), (_Float16 *)nullptr, (const int *)nullptr);
+               (Cpx<double> *)A, slot_out, kblk, nblk(maxslot, kKblkSlots), (_Float16 *)nullptr, (const int *)nullptr);
 }
 
 void launch_recip_nd(const State &s, int rt0, int nrt, int maxslot, void *ph, double *part, void *A,
@@ -3966,8 +4022,8 @@ void launch_recip_nd(const State &s, int rt0, int nrt, int maxslot, void *ph, do
   Cpx<double> *t = (Cpx<double> *)ph;
   MSC_LAUNCH(k_recip_tables_nd, dim3(nblk(s.g_end - s.g_begin, 256)), dim3(256), 0, st, s, t);
   MSC_LAUNCH(k_recip_struct_nd, dim3(nrt, nblk(maxslot, 256)), dim3(256), 0, st, s, rt0, t, part);
-  MSC_LAUNCH(k_recip_reduce_nd, dim3(nblk(maxslot, 64), s.rep_end - s.rep_begin), dim3(64), 0, st, s, part,
-             (Cpx<double> *)A, slot_out, kblk, nblk(maxslot, 64));
+  MSC_LAUNCH(k_recip_reduce_nd, dim3(nblk(maxslot, kKblkSlots), s.rep_end - s.rep_begin), dim3(64), 0, st, s, part,
+             (Cpx<double> *)A, slot_out, kblk, nblk(maxslot, kKblkSlots));
   MSC_LAUNCH(k_recip_force_nd, dim3(s.ntask), dim3(64 * kRfGroups), 0, st, s, t, (const Cpx<double> *)A);
 }
 
@@ -4033,15 +4089,17 @@ void launch_kick2(const State &s, const Thermo *th, const double *gs, const int 
   MSC_LAUNCH(k_kick2, dim3(nblk(s.ntask, 4) + cell_blocks(s, ls)), dim3(256), 0, st, s, th, gs, step, rows1, rows2,
              ls);
 }
-void launch_end(const State &s, Thermo *th, const double *gs, const int *step, const double *rows, long region,
-                bool has_field, const double *kblk, int maxslot, RepEnergy *E, double *Kin, double *counts,
-                double *obs, const ListStep &ls, hipStream_t st) {
-  MSC_LAUNCH(k_end, dim3(s.rep_end - s.rep_begin + cell_blocks(s, ls)), dim3(256), 0, st, s, th, gs, step, rows,
-             region, has_field ? 1 : 0, kblk, nblk(maxslot, 64), E, Kin, counts, obs, ls);
+void launch_end(const State &s, const int *step, const double *gs, const double *rows, long region,
+                bool has_field, const double *kblk, int maxslot, double *sums, const ListStep &ls,
+                hipStream_t st) {
+  MSC_LAUNCH(k_end, dim3((s.rep_end - s.rep_begin) * kEndBlocks + cell_blocks(s, ls)), dim3(256), 0, st, s, step,
+             gs, rows, region, has_field ? 1 : 0, kblk, nblk(maxslot, kKblkSlots), sums, ls);
 }
-void launch_lists_step(const State &s, const ListStep &ls, int *flags, int *step, hipStream_t st) {
-  MSC_LAUNCH(k_lists_step, dim3(std::min(nblk(s.g_end - s.g_begin, 4), kListStepBlocks)), dim3(256), 0, st, s, ls,
-             flags, step);
+void launch_lists_step(const State &s, const ListStep &ls, int *flags, int *step, const EndArgs &ea,
+                       hipStream_t st) {
+  const int rows = std::min(nblk(s.g_end - s.g_begin, 4), kListStepBlocks);
+  MSC_LAUNCH(k_lists_step, dim3(rows + (ea.sums ? s.rep_end - s.rep_begin : 0)), dim3(256), 0, st, s, ls, flags,
+             step, ea);
 }
 void launch_stamp(double *p, double v, hipStream_t st) { MSC_LAUNCH(k_stamp, dim3(1), dim3(64), 0, st, p, v); }
 void launch_fill_int(int *p, int n, int v, hipStream_t st) {

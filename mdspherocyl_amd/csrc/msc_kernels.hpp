@@ -111,8 +111,9 @@ void launch_recip_tables(const State &s, bool mixed, void *ph, void *amp, double
 int launch_recip_struct(const State &s, bool mixed, int KZ, int rt0, int nrt, int maxcol, const void *ph,
                         const void *amp, void *part, hipStream_t st);
 // M2: per-replica sums of c c^T, or null to sum them from the per-task rows
-// m2rows; kblk (may be null): per-block sums of the 10 slot values,
-// nblk(maxslot, 64) rows per replica
+// m2rows; kblk (may be null): per-block sums of the 10 slot values over
+// kKblkSlots slots, nblk(maxslot, kKblkSlots) rows per replica
+constexpr int kKblkSlots = 64;
 // bop (mixed contexts): the operands of the matrix-core back projection,
 // kBopPerTile groups of 4 fp16 values per column tile of 16 (RepDev::bop0,
 // a multiple of 2: the kernels read them as groups of 8), scaled by
@@ -150,12 +151,29 @@ void launch_drift(const State &s, bool mixed, const Thermo *th, const double *gs
                   void *ph, void *amp, double *rows, const ListStep &ls, hipStream_t st);
 void launch_kick2(const State &s, const Thermo *th, const double *gs, const int *step, const double *rows1,
                   double *rows2, const ListStep &ls, hipStream_t st);
-void launch_end(const State &s, Thermo *th, const double *gs, const int *step, const double *rows, long region,
-                bool has_field, const double *kblk, int maxslot, RepEnergy *E, double *Kin, double *counts,
-                double *obs, const ListStep &ls, hipStream_t st);
+// End-of-step sums per replica (sums[kEndSums rp + v]): v < kEndVals the
+// replica sums of the per-task rows and k-space block rows the step's
+// thermostat, energies and observables need (k_end, several blocks per
+// replica so that the row reads spread over several CUs), v = kEndVals the
+// step index k, then the step's four thermostat deviates of the replica
+constexpr int kEndVals = 44, kEndSums = 56, kEndBlocks = 3;  // kEndBlocks * 4 waves >= kEndVals / 4
+void launch_end(const State &s, const int *step, const double *gs, const double *rows, long region,
+                bool has_field, const double *kblk, int maxslot, double *sums, const ListStep &ls,
+                hipStream_t st);
+// Outputs of the end of a step, written from the sums by one extra block
+// per replica of k_lists_step: moveTher mode 2 into th, the energies E in
+// the layout of RepEnergy, Kin, the pair counts and the observables of
+// print_out (obs + 12 (k nrep + rp)) but the rebuild flag
+struct EndArgs {
+  const double *sums;  // launch_end's sums
+  Thermo *th;
+  RepEnergy *E;
+  double *Kin, *counts, *obs;
+};
 // the step's last launch: rows of rebuilt replicas, contact rows of
-// refreshed ones; advances step[0]
-void launch_lists_step(const State &s, const ListStep &ls, int *flags, int *step, hipStream_t st);
+// refreshed ones, and the end-of-step outputs (EndArgs); advances step[0]
+void launch_lists_step(const State &s, const ListStep &ls, int *flags, int *step, const EndArgs &ea,
+                       hipStream_t st);
 // contact-graph components of the selected replica: par[g] = smallest
 // particle of g's component (global index); flags[0] counts particles that
 // moved too far for the reference rows to hold every contact, flags[1] pairs
