@@ -128,6 +128,7 @@ struct msc_ctx {
   double *d_order = nullptr;  // [kOrderSums] order-parameter sums
   double *d_rows = nullptr, *d_kblk = nullptr;  // per-task row sets and k-space block sums of the step loop
   double *d_endsum = nullptr;  // end-of-step sums per replica (kEndSums each)
+  double *d_kick_ts = nullptr;  // thermostat coordinates of the fused last kick, 2 per replica
   int *d_step = nullptr;  // {step within the run, first absolute step of the run}
   int *d_cell_of = nullptr, *d_cell_cnt = nullptr, *d_cell_start = nullptr, *d_cell_fill = nullptr,
       *d_cell_list = nullptr;
@@ -1008,6 +1009,7 @@ int msc_create(int device, int precision, int nrep, const msc_replica_desc *reps
   A(&ctx->d_rows, ctx->tasks.size() * 16 * kRowSets);
   A(&ctx->d_kblk, (size_t)nrep * ((ctx->maxslot + kKblkSlots - 1) / kKblkSlots) * 10);
   A(&ctx->d_endsum, (size_t)nrep * kEndSums);
+  A(&ctx->d_kick_ts, (size_t)nrep * 2);
   A(&ctx->d_step, 2);
   A(&ctx->d_cell_of, N); A(&ctx->d_cell_list, N);
   A(&ctx->d_cell_tmp, N);
@@ -1105,7 +1107,7 @@ void msc_destroy(msc_ctx *ctx) {
                   ctx->d_cell_cnt, ctx->d_cell_start, ctx->d_cell_fill, ctx->d_cell_list, ctx->d_nl_cnt,
                   ctx->d_nl_idx, ctx->d_dl_cnt, ctx->d_dl_idx, ctx->d_kval, ctx->d_cell_tmp, ctx->d_sl_cnt,
                   ctx->d_sl_idx, ctx->d_snap_e, ctx->d_srebuild, ctx->d_cell_pack, 
-                  ctx->d_snap_u, ctx->d_clu, ctx->d_clu_flag, ctx->d_corr, ctx->d_corr_drop, ctx->d_order, ctx->d_rows, ctx->d_kblk, ctx->d_endsum, ctx->d_step,
+                  ctx->d_snap_u, ctx->d_clu, ctx->d_clu_flag, ctx->d_corr, ctx->d_corr_drop, ctx->d_order, ctx->d_rows, ctx->d_kblk, ctx->d_endsum, ctx->d_kick_ts, ctx->d_step,
                   ctx->d_tab, ctx->d_ph, ctx->d_amp, ctx->d_part, ctx->d_A, ctx->d_slot_out, ctx->d_task_out,
                   ctx->d_red, ctx->d_M2, ctx->d_Kin, ctx->d_gs, ctx->d_obs, ctx->d_counts, ctx->d_E,
                   ctx->d_th, ctx->d_flags, ctx->d_rebuild, ctx->d_one, ctx->d_run_gs, ctx->d_run_obs, ctx->d_save, ctx->d_tune_save,
@@ -1576,7 +1578,8 @@ int msc_force_dipR(msc_ctx *ctx, int rep, double *U, double PT[9]) {
 // Reciprocal launches.  Standalone (m2rows null): phase tables, M2 and the
 // per-replica energy sums here.  In the step loop the tables and the c c^T
 // rows come from k_drift (m2rows) and the energies go to kblk for k_end.
-static int dipK_launches(msc_ctx *ctx, int rep, const double *m2rows, double *kblk, bool join = false) {
+static int dipK_launches(msc_ctx *ctx, int rep, const double *m2rows, double *kblk, bool join = false,
+                         const KickFuse *kf = nullptr) {
   State s = sel(ctx, rep);
   if (ctx->nd > 1) {
     // several slots per particle: intra-particle exclusion by the particles'
@@ -1621,7 +1624,7 @@ static int dipK_launches(msc_ctx *ctx, int rep, const double *m2rows, double *kb
     }
   }
   launch_recip_reduce(s, ctx->mixed, ctx->d_part, m2rows ? nullptr : ctx->d_M2, m2rows, ctx->d_A, ctx->d_slot_out,
-                      kblk, ctx->maxslot, ctx->d_bop, ctx->d_bexp, ctx->stream);
+                      kblk, ctx->maxslot, ctx->d_bop, ctx->d_bexp, ctx->stream, kf);
   if (!kblk) launch_reduce(s, ctx->d_slot_out, 10, 10, 1, (double *)ctx->d_E, kEstride, kEoff_K, ctx->stream);
   // an overlapped step's pair launch adds into F and T as the back
   // projection does: it must have finished first
@@ -1639,7 +1642,7 @@ static int dipK_launches(msc_ctx *ctx, int rep, const double *m2rows, double *kb
       {
         Bracket b(ctx, "recip_force");
         launch_recip_force(s1, ctx->mixed, kz, ctx->d_ph, ctx->d_amp, ctx->d_A, ctx->d_bop, ctx->d_bexp,
-                           ctx->stream);
+                           ctx->stream, kf);
       }
       r = r1;
     }
@@ -1884,9 +1887,10 @@ static void enqueue_step(msc_ctx *ctx) {
   double *rows = ctx->d_rows;
   const double *gs = ctx->d_run_gs;
   // list maintenance (v25/main.cpp:679-701): displacements reduced by
-  // k_drift, decision and cell grid by extra blocks of k_force_sc, k_kick2
-  // and k_end, the rows by k_lists_step; for replicas that keep their rows
-  // these return at once
+  // k_drift, decision and cell grid by extra blocks of k_force_sc (or the
+  // fused pair launch), k_kick2 (or the back projection that does the last
+  // kick) and k_end, the rows by k_lists_step; for replicas that keep their
+  // rows these return at once
   const ListStep ls = {ctx->d_rebuild, ctx->contact ? ctx->d_srebuild : nullptr, ctx->d_lctr, ctx->d_dmax,
                        ctx->d_run_obs, ctx->d_step};
   {
@@ -1920,15 +1924,22 @@ static void enqueue_step(msc_ctx *ctx) {
     }
     dipR_launches(ctx, MSC_ALL, rows + kRowR * region);
   }
-  // the drift's rows hold m2 as values 2..7 (value k of task t at
-  // k * ntask_all + t)
-  dipK_launches(ctx, MSC_ALL, rows + kRowDrift * region + 2 * (long)ctx->tasks.size(), ctx->d_kblk, overlap);
   // ext_field adds exactly nothing without a field: skipped then
   const bool field = any_field(ctx);
+  // mixed contexts without a field: the back projection is the step's last
+  // force launch and does the last kick (k_kick2's work) in its epilogue
+  const bool fuse_kick = ctx->mixed && ctx->nd == 1 && !field;
+  const KickFuse kf = {ctx->d_th, gs, ctx->d_step, rows + kRowKick1 * region, ctx->d_kick_ts,
+                       rows + kRowKick2 * region, ls};
+  // the drift's rows hold m2 as values 2..7 (value k of task t at
+  // k * ntask_all + t)
+  dipK_launches(ctx, MSC_ALL, rows + kRowDrift * region + 2 * (long)ctx->tasks.size(), ctx->d_kblk, overlap,
+                fuse_kick ? &kf : nullptr);
   if (field) launch_ext_field(s, rows + kRowExt * region, st);
   {
     Bracket b(ctx, "integrate");
-    launch_kick2(s, ctx->d_th, gs, ctx->d_step, rows + kRowKick1 * region, rows + kRowKick2 * region, ls, st);
+    if (!fuse_kick)
+      launch_kick2(s, ctx->d_th, gs, ctx->d_step, rows + kRowKick1 * region, rows + kRowKick2 * region, ls, st);
     launch_end(s, ctx->d_step, gs, rows, region, field, ctx->d_kblk, ctx->maxslot, ctx->d_endsum, ls, st);
   }
   {

@@ -156,6 +156,17 @@ __device__ __forceinline__ void wave_store_row(double *rows, int stride, int tid
   }
 }
 
+// Thermostat half steps (moveTher, v25/functions.cpp:2186-2218):
+// thermo_half = ex1 s + sqrt(2 T g dt / Q) r with both factors taken once on
+// the host (RepDev::th_ex1, th_sq); thermo_mode1 the drive by the kinetic
+// energy K2.
+__device__ __forceinline__ double thermo_half(const RepDev &R, double s, double r) {
+  return R.th_ex1 * s + R.th_sq * r;
+}
+__device__ __forceinline__ double thermo_mode1(const RepDev &R, double s, double K2) {
+  return s + R.dt * (2. * K2 - 3. * (double)R.N * R.temp) / R.Q;
+}
+
 // Task of the current wave (4 waves per 256-thread block).  The value is the
 // same in every lane; readfirstlane lets the compiler keep it, and all task
 // data derived from it, in scalar registers.
@@ -1999,7 +2010,7 @@ __global__ void __launch_bounds__(576) k_recip_reduce(State s, const R *__restri
                                                        const double *__restrict__ m2rows, Cpx<R> *__restrict__ Aout,
                                                        double *__restrict__ slot_out, double *__restrict__ kblk,
                                                        int kblk_stride, _Float16 *__restrict__ bop,
-                                                       const int *__restrict__ bexp) {
+                                                       const int *__restrict__ bexp, KickFuse kf) {
   constexpr int kGroups = 8;  // waves summing chunk partials; wave kGroups sums M2
   __shared__ double sh[kGroups][6][64];
   __shared__ double msh[6];
@@ -2027,6 +2038,18 @@ __global__ void __launch_bounds__(576) k_recip_reduce(State s, const R *__restri
     if (lane == 0)
 #pragma unroll
       for (int q = 0; q < 6; ++q) msh[q] = m[q];
+    if (kf.ts && blockIdx.x == 0) {
+      // moveTher mode 1 for the fused last kick (as k_kick2)
+      double K1[2];
+      wave_rows_sum<2>(kf.rows1, s.ntask_all, Rp.task0, Rp.ntask, lane, K1);
+      const double *gs = kf.gs + (size_t)kf.step[0] * s.nrep * 4;
+      const Thermo T = kf.th[rep];
+      if (lane < 2) {
+        const int i = lane;
+        kf.ts[rep * 2 + i] = thermo_mode1(Rp, thermo_half(Rp, i ? T.s[1] : T.s[0], gs[rep * 4 + 2 * i]),
+                                          (i ? K1[1] : K1[0]) / 2);
+      }
+    }
   } else if (act) {
     // four chunk partials in flight at a time, added in chunk order
     int c = grp;
@@ -2297,11 +2320,15 @@ __global__ void __launch_bounds__(64 * kRfGroups) k_recip_force(State s, const C
 // vector work (two groups in one block kept every wave of a CU in step:
 // 41.5 against 39.8 us at N = 32768).
 constexpr int kBpGroups = 1;
+template <bool LAST>
+__device__ __forceinline__ void kick_regs(const RepDev &Rp, double m, const double I[3], const double O[9],
+                                          const double F[3], const double T[3], const double ts[2], double p[3],
+                                          double l[3], double acc[14]);
 template <int KZ>
 __global__ void __launch_bounds__(256 * kBpGroups) k_recip_force_mfma(State s, const Cpx<float> *__restrict__ ph,
                                                                       const float *__restrict__ amp,
                                                                       const half8_t *__restrict__ bop,
-                                                                      const int *__restrict__ bexp) {
+                                                                      const int *__restrict__ bexp, KickFuse kf) {
   constexpr int kmax = KZ - 1, side = 2 * kmax + 1, ncol = side * side, NCT = (ncol + 15) / 16;
   constexpr int NSTEP = (NCT + kBpGroups - 1) / kBpGroups;
   constexpr int NV = 9;  // M_xx, M_xy, M_yy, M_xz, M_yz, M_zz, H_x, H_y, H_z
@@ -2319,7 +2346,12 @@ __global__ void __launch_bounds__(256 * kBpGroups) k_recip_force_mfma(State s, c
   auto scl = reinterpret_cast<float(*)[3]>(lds + oScl);
   auto psum = reinterpret_cast<float(*)[4 * NV][kPs]>(lds);  // [wave][j * NV + value][lane]
   auto tot = reinterpret_cast<double(*)[NV]>(lds + oTot);    // [particle][value]
-  if ((int)blockIdx.x >= s.ntask) return;  // block-uniform
+  if ((int)blockIdx.x >= s.ntask) {
+    // extra blocks of the fused last kick: the cell fill of rebuilt replicas
+    const int g = s.g_begin + ((int)blockIdx.x - s.ntask) * 256 + (int)threadIdx.x;
+    if (kf.ls.rebuild && threadIdx.x < 256 && g < s.g_end) cells_fill_one(s, g, kf.ls.rebuild);
+    return;
+  }
   const int tid = s.task_begin + blockIdx.x;
   const Task tk = s.tasks[tid];
   const RepDev &Rp = s.reps[tk.rep];
@@ -2488,6 +2520,7 @@ __global__ void __launch_bounds__(256 * kBpGroups) k_recip_force_mfma(State s, c
   __syncthreads();
   if (threadIdx.x < 64) {
     const int p = threadIdx.x;
+    double acc[14] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     if (p < tk.n) {
       const int g = tk.g0 + p;
       double mu = s.dmu[g];
@@ -2505,11 +2538,32 @@ __global__ void __launch_bounds__(256 * kBpGroups) k_recip_force_mfma(State s, c
       for (int a = 0; a < 3; ++a) w[a] = -mu * H[a] + mu * mu * Gu[a] + sd * F[a];
       double t[3];
       cross3(u, w, t);
+      double Ff[3], Tf[3];
       for (int a = 0; a < 3; ++a) {
-        s.F[a * N + g] += F[a];
-        s.T[a * N + g] += t[a];
+        Ff[a] = s.F[a * N + g] + F[a];
+        Tf[a] = s.T[a * N + g] + t[a];
+        s.F[a * N + g] = Ff[a];
+        s.T[a * N + g] = Tf[a];
+      }
+      if (kf.ts) {
+        // the last moveP with the final force and torque (as k_kick2)
+        const double m = s.mass[g];
+        const double I[3] = {s.inert[g], s.inert[N + g], s.inert[2 * N + g]};
+        double O[9], pp[3], ll[3];
+        for (int k = 0; k < 9; ++k) O[k] = s.ori[k * N + g];
+        for (int a = 0; a < 3; ++a) {
+          pp[a] = s.p[a * N + g];
+          ll[a] = s.l[a * N + g];
+        }
+        const double ts[2] = {kf.ts[tk.rep * 2 + 0], kf.ts[tk.rep * 2 + 1]};
+        kick_regs<true>(Rp, m, I, O, Ff, Tf, ts, pp, ll, acc);
+        for (int a = 0; a < 3; ++a) {
+          s.p[a * N + g] = pp[a];
+          s.l[a * N + g] = ll[a];
+        }
       }
     }
+    if (kf.ts) wave_store_row(kf.rows2, s.ntask_all, tid, acc, 14, lane);
   }
 }
 
@@ -3446,21 +3500,10 @@ __global__ void k_renorm(State s) {
   if (g < s.g_end) renorm_particle(s, g);
 }
 
-// moveTher for every replica (one thread each).  gs[rep*4 + 2*i + 0/1] hold
-// the Box-Muller pair drawn on the host from thermostat i's MT19937 for this
-// step (mode 0 uses the first, mode 2 the stored second).
 // moveTher mode 0 (v25/functions.cpp:2186-2218) for every replica (one
 // thread each): gs[rep*4 + 2*i + 0/1] hold the Box-Muller pair drawn on the
-// host from thermostat i's MT19937 for this step.
-// ex1 s + sqrt(2 T g dt / Q) r with both factors taken once on the host
-// (RepDev::th_ex1, th_sq)
-__device__ __forceinline__ double thermo_half(const RepDev &R, double s, double r) {
-  return R.th_ex1 * s + R.th_sq * r;
-}
-__device__ __forceinline__ double thermo_mode1(const RepDev &R, double s, double K2) {
-  return s + R.dt * (2. * K2 - 3. * (double)R.N * R.temp) / R.Q;
-}
-
+// host from thermostat i's MT19937 for this step (mode 0 uses the first,
+// mode 2 the stored second).
 __global__ void k_thermo0(State s, Thermo *__restrict__ th, const double *__restrict__ gs) {
   int rp = blockIdx.x * blockDim.x + threadIdx.x;
   if (rp >= s.nrep) return;
@@ -3965,10 +4008,16 @@ static void recip_struct_kz(const State &s, int rt0, int nrt, int maxcol, const 
 }
 template <typename R, int KZ>
 static void recip_force_kz(const State &s, const void *ph, const void *amp, const void *A, const void *bop,
-                           const int *bexp, hipStream_t st) {
+                           const int *bexp, hipStream_t st, const KickFuse *kf) {
   if constexpr (sizeof(R) == sizeof(float)) {
-    MSC_LAUNCH((k_recip_force_mfma<KZ>), dim3(s.ntask), dim3(256 * kBpGroups), 0, st, s, (const Cpx<float> *)ph,
-               (const float *)amp, (const half8_t *)bop, bexp);
+    KickFuse k0 = {};
+    int extra = 0;
+    if (kf) {
+      k0 = *kf;
+      if (k0.ls.rebuild) extra = nblk(s.g_end - s.g_begin, 256);
+    }
+    MSC_LAUNCH((k_recip_force_mfma<KZ>), dim3(s.ntask + extra), dim3(256 * kBpGroups), 0, st, s,
+               (const Cpx<float> *)ph, (const float *)amp, (const half8_t *)bop, bexp, k0);
   } else {
     MSC_LAUNCH((k_recip_force<R, KZ>), dim3(s.ntask), dim3(64 * kRfGroups), 0, st, s, (const Cpx<R> *)ph,
                (const R *)amp, (const Cpx<R> *)A);
@@ -3992,12 +4041,13 @@ int launch_recip_struct(const State &s, bool mixed, int KZ, int rt0, int nrt, in
   }
 }
 int launch_recip_force(const State &s, bool mixed, int KZ, const void *ph, const void *amp, const void *A,
-                       const void *bop, const int *bexp, hipStream_t st) {
+                       const void *bop, const int *bexp, hipStream_t st, const KickFuse *kf) {
+  if (kf && !mixed) return -1;  // the fused kick exists in the mixed back projection only
   switch (KZ) {
-#define CASE(k)                                                        \
-  case k:                                                              \
-    if (mixed) recip_force_kz<float, k>(s, ph, amp, A, bop, bexp, st); \
-    else recip_force_kz<double, k>(s, ph, amp, A, bop, bexp, st);      \
+#define CASE(k)                                                            \
+  case k:                                                                  \
+    if (mixed) recip_force_kz<float, k>(s, ph, amp, A, bop, bexp, st, kf); \
+    else recip_force_kz<double, k>(s, ph, amp, A, bop, bexp, st, kf);      \
     return 0;
     MSC_KZ_CASES(CASE)
 #undef CASE
@@ -4007,14 +4057,17 @@ int launch_recip_force(const State &s, bool mixed, int KZ, const void *ph, const
 }
 void launch_recip_reduce(const State &s, bool mixed, const void *part, const double *M2, const double *m2rows,
                          void *A, double *slot_out, double *kblk, int maxslot, void *bop, const int *bexp,
-                         hipStream_t st) {
+                         hipStream_t st, const KickFuse *kf) {
   const dim3 grid(nblk(maxslot, kKblkSlots), s.rep_end - s.rep_begin);
+  KickFuse k0 = {};
+  if (kf) k0 = *kf;
   if (mixed)
     MSC_LAUNCH(k_recip_reduce<float>, grid, dim3(576), 0, st, s, (const float *)part, M2, m2rows,
-               (Cpx<float> *)A, slot_out, kblk, nblk(maxslot, kKblkSlots), (_Float16 *)bop, bexp);
+               (Cpx<float> *)A, slot_out, kblk, nblk(maxslot, kKblkSlots), (_Float16 *)bop, bexp, k0);
   else
     MSC_LAUNCH(k_recip_reduce<double>, grid, dim3(576), 0, st, s, (const double *)part, M2, m2rows,
-               (Cpx<double> *)A, slot_out, kblk, nblk(maxslot, kKblkSlots), (_Float16 *)nullptr, (const int *)nullptr);
+               (Cpx<double> *)A, slot_out, kblk, nblk(maxslot, kKblkSlots), (_Float16 *)nullptr, (const int *)nullptr,
+               k0);
 }
 
 void launch_recip_nd(const State &s, int rt0, int nrt, int maxslot, void *ph, double *part, void *A,

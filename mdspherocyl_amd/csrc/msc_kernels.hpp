@@ -118,12 +118,28 @@ constexpr int kKblkSlots = 64;
 // kBopPerTile groups of 4 fp16 values per column tile of 16 (RepDev::bop0,
 // a multiple of 2: the kernels read them as groups of 8), scaled by
 // 2^bexp[3 col + l] (RepDev::bexp0)
+// The last kick fused into the back projection (mixed contexts of the step
+// loop without an external field, where the back projection is the step's
+// last force launch): k_recip_reduce takes moveTher mode 1 of each replica
+// (from the first kick's rows, as k_kick2) into ts[2 rep + i]; the back
+// projection then kicks its task's particles with the final F and T (the
+// last moveP, as k_kick2), writes the kick2 rows, and its extra blocks fill
+// the cells of rebuilt replicas (k_kick2's extra blocks).
+struct KickFuse {
+  const Thermo *th;
+  const double *gs;   // the run's deviates
+  const int *step;
+  const double *rows1;  // kick1 rows (k_drift)
+  double *ts;           // [nrep][2]
+  double *rows2;        // kick2 rows
+  ListStep ls;
+};
 void launch_recip_reduce(const State &s, bool mixed, const void *part, const double *M2, const double *m2rows,
                          void *A, double *slot_out, double *kblk, int maxslot, void *bop, const int *bexp,
-                         hipStream_t st);
+                         hipStream_t st, const KickFuse *kf = nullptr);
 constexpr int kBopPerTile = 24 * 64;  // 3 weights x 4 operands x 64 lanes x 8 fp16
 int launch_recip_force(const State &s, bool mixed, int KZ, const void *ph, const void *amp, const void *A,
-                       const void *bop, const int *bexp, hipStream_t st);
+                       const void *bop, const int *bexp, hipStream_t st, const KickFuse *kf = nullptr);
 // Reciprocal sum of contexts with several dipole slots per particle, FP64:
 // slot phase tables ph (nd * 3 * kMaxKZ * Ntot complex doubles), chunk
 // partials of the chunk tasks rt0 .. rt0 + nrt - 1 (10 doubles per slot and
