@@ -166,7 +166,7 @@ int msc_set_graph(msc_ctx *ctx, int on);
  * fused_pairs give the same results bit for bit; pair_waves changes how the
  * row entries of a 64-particle task are split over waves, so the rounding of
  * the real-space and WCA sums (each choice is deterministic).  Defaults:
- * graph 0, fused_pairs 1, pair_waves 4, overlap 1. */
+ * graph 0, fused_pairs 1, pair_waves 4, overlap 1, fuse_kick 1. */
 typedef struct msc_tuning {
   int graph;        /* msc_run replays each step as one captured HIP graph */
   int fused_pairs;  /* MSC_MIXED contexts with contact rows whose replicas all
@@ -193,6 +193,11 @@ typedef struct msc_tuning {
                        projection (each writes its own outputs, so results
                        are the same bit for bit); 0: one stream.  Runs with
                        timing on or MSC_SYNC_CHECK set use one stream. */
+  int fuse_kick;    /* 1: in MSC_MIXED contexts without an external field the
+                       back projection, the step's last force launch, also
+                       does the last moveP of its particles (one launch
+                       fewer per step; the same results bit for bit); 0: a
+                       launch of its own. */
 } msc_tuning;
 int msc_get_tuning(msc_ctx *ctx, msc_tuning *t);
 int msc_set_tuning(msc_ctx *ctx, const msc_tuning *t);

@@ -36,11 +36,13 @@ _ip = ctypes.POINTER(ctypes.c_int)
 class Tuning(ctypes.Structure):
     """msc_tuning: launch choices of a context."""
     _fields_ = [("graph", ctypes.c_int), ("fused_pairs", ctypes.c_int), ("pair_waves", ctypes.c_int),
-                ("list_skin", ctypes.c_double), ("chunk", ctypes.c_int), ("overlap", ctypes.c_int)]
+                ("list_skin", ctypes.c_double), ("chunk", ctypes.c_int), ("overlap", ctypes.c_int),
+                ("fuse_kick", ctypes.c_int)]
 
     def as_dict(self):
         return {"graph": self.graph, "fused_pairs": self.fused_pairs, "pair_waves": self.pair_waves,
-                "list_skin": self.list_skin, "chunk": self.chunk, "overlap": self.overlap}
+                "list_skin": self.list_skin, "chunk": self.chunk, "overlap": self.overlap,
+                "fuse_kick": self.fuse_kick}
 
 
 class ReplicaDesc(ctypes.Structure):

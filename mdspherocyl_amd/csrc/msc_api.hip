@@ -101,6 +101,8 @@ struct msc_ctx {
   // msc_tuning.overlap: the fused pair launch on stream2, forked after the
   // drift (ev_fork) and joined before the back projection (ev_join)
   bool overlap = true;
+  // msc_tuning.fuse_kick: the last kick in the mixed back projection's epilogue
+  bool fuse_kick = true;
   hipStream_t stream2 = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   // MSC_MIXED: the skin of the context's own rows (0: the reference's rows
@@ -1848,6 +1850,7 @@ int msc_get_tuning(msc_ctx *ctx, msc_tuning *t) {
   t->list_skin = ctx->list_skin;
   t->chunk = ctx->rchunk;
   t->overlap = ctx->overlap ? 1 : 0;
+  t->fuse_kick = ctx->fuse_kick ? 1 : 0;
   return MSC_OK;
 }
 
@@ -1864,6 +1867,7 @@ int msc_set_tuning(msc_ctx *ctx, const msc_tuning *t) {
   ctx->fused_pairs = t->fused_pairs != 0;
   ctx->pair_waves = t->pair_waves;
   ctx->overlap = t->overlap != 0;
+  ctx->fuse_kick = t->fuse_kick != 0;
   if (t->list_skin != ctx->list_skin) {
     // other rows: msc_init_run (or the autotuner) builds them again
     ctx->list_skin = t->list_skin;
@@ -1928,7 +1932,7 @@ static void enqueue_step(msc_ctx *ctx) {
   const bool field = any_field(ctx);
   // mixed contexts without a field: the back projection is the step's last
   // force launch and does the last kick (k_kick2's work) in its epilogue
-  const bool fuse_kick = ctx->mixed && ctx->nd == 1 && !field;
+  const bool fuse_kick = ctx->fuse_kick && ctx->mixed && ctx->nd == 1 && !field;
   const KickFuse kf = {ctx->d_th, gs, ctx->d_step, rows + kRowKick1 * region, ctx->d_kick_ts,
                        rows + kRowKick2 * region, ls};
   // the drift's rows hold m2 as values 2..7 (value k of task t at

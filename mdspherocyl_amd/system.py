@@ -209,7 +209,8 @@ class MdSystem:
         self._chk(self.L.msc_get_tuning(self._ctx, ctypes.byref(t)))
         return t.as_dict()
 
-    def set_tuning(self, graph=None, fused_pairs=None, pair_waves=None, list_skin=None, chunk=None, overlap=None):
+    def set_tuning(self, graph=None, fused_pairs=None, pair_waves=None, list_skin=None, chunk=None, overlap=None,
+                   fuse_kick=None):
         t = _lib.Tuning()
         self._chk(self.L.msc_get_tuning(self._ctx, ctypes.byref(t)))
         if graph is not None:
@@ -224,6 +225,8 @@ class MdSystem:
             t.chunk = int(chunk)
         if overlap is not None:
             t.overlap = int(bool(overlap))
+        if fuse_kick is not None:
+            t.fuse_kick = int(bool(fuse_kick))
         self._chk(self.L.msc_set_tuning(self._ctx, ctypes.byref(t)))
 
     def autotune(self, steps=100):

@@ -83,6 +83,25 @@ def test_overlapped_pair_stream_equals_one_stream(msc, graph):
         assert np.array_equal(x, y)
 
 
+@pytest.mark.parametrize("graph", [0, 1])
+def test_fused_last_kick_equals_kick_launch(msc, graph):
+    """The last kick in the back projection's epilogue (thermostat
+    coordinates from the reduction, cell fill in its extra blocks) gives the
+    step with the kick as a launch of its own bit for bit (direct launches
+    and the captured graph): 130 steps with list rebuilds, observables,
+    state and forces."""
+    _say("case: fused last kick vs kick launch, graph %d, mixed, 2 x N=400, 130 steps" % graph)
+    a = _run(msc, dict(fused_pairs=1, pair_waves=4, graph=graph, fuse_kick=1))
+    b = _run(msc, dict(fused_pairs=1, pair_waves=4, graph=graph, fuse_kick=0))
+    assert a[4]["fuse_kick"] == 1 and b[4]["fuse_kick"] == 0
+    assert int(a[0][:, 0, 11].sum()) >= 1
+    assert np.array_equal(a[0], b[0])
+    for x, y in zip(a[1], b[1]):
+        assert np.array_equal(x, y)
+    for x, y in zip(a[2], b[2]):
+        assert np.array_equal(x, y)
+
+
 def test_own_rows_follow_reference_rows(msc):
     """The context's own rows (list skin 1.0 and 0.5) against the
     reference's rows and rebuild test (list skin 0): the same pairs interact,
