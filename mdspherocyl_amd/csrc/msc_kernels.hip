@@ -2497,6 +2497,36 @@ __global__ void __launch_bounds__(256 * kBpGroups) k_recip_force_mfma(State s, c
   }
   if (st < NSTEP) step(st, 0);
   if (cprev >= 0) combine(Pr[(NSTEP - 1) & 1], Pi[(NSTEP - 1) & 1], cprev);
+  // The particle's own inputs of the epilogue (one particle per lane of wave
+  // 0), loaded while the partial sums are combined: its dipole, the force
+  // and torque of the other launches and, with the fused kick, the state the
+  // last moveP reads.
+  const int p = threadIdx.x;
+  const bool own = threadIdx.x < 64 && p < tk.n;
+  const int g = tk.g0 + p;
+  double F0[3] = {0, 0, 0}, T0[3] = {0, 0, 0}, u[3] = {0, 0, 0}, mu = 0.0, sd = 0.0;
+  double m = 0.0, I[3] = {0, 0, 0}, O[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, pp[3] = {0, 0, 0}, ll[3] = {0, 0, 0};
+  double ts[2] = {0, 0};
+  if (own) {
+    mu = s.dmu[g];
+    sd = s.dpc[g];
+    for (int a = 0; a < 3; ++a) {
+      u[a] = s.dori[a * N + g];
+      F0[a] = s.F[a * N + g];
+      T0[a] = s.T[a * N + g];
+    }
+    if (kf.ts) {
+      m = s.mass[g];
+      for (int a = 0; a < 3; ++a) I[a] = s.inert[a * N + g];
+      for (int k = 0; k < 9; ++k) O[k] = s.ori[k * N + g];
+      for (int a = 0; a < 3; ++a) {
+        pp[a] = s.p[a * N + g];
+        ll[a] = s.l[a * N + g];
+      }
+      ts[0] = kf.ts[tk.rep * 2 + 0];
+      ts[1] = kf.ts[tk.rep * 2 + 1];
+    }
+  }
   // Totals per particle: the lanes' FP32 partial sums go to LDS (over the
   // loop's tables), then each (particle, value) is summed in FP64 in a
   // fixed order: group 0's 16 lanes, then group 1's.
@@ -2507,30 +2537,25 @@ __global__ void __launch_bounds__(256 * kBpGroups) k_recip_force_mfma(State s, c
     for (int q = 0; q < NV; ++q) psum[wave][j * NV + q][lane] = part[j][q];
   __syncthreads();
   for (int o = threadIdx.x; o < 64 * NV; o += 256 * kBpGroups) {
-    const int p = o / NV, q = o - p * NV;  // particle p of the task
-    const int w4 = p >> 4, pl = p & 15, lq4 = pl >> 2, j = pl & 3;
+    const int pt = o / NV, q = o - pt * NV;  // particle pt of the task
+    const int w4 = pt >> 4, pl = pt & 15, lq4 = pl >> 2, j = pl & 3;
     double t = 0.0;
     for (int g2 = 0; g2 < kBpGroups; ++g2) {
       const float *src = &psum[4 * g2 + w4][j * NV + q][16 * lq4];
 #pragma unroll
       for (int l2 = 0; l2 < 16; ++l2) t += (double)src[l2];
     }
-    tot[p][q] = t;
+    tot[pt][q] = t;
   }
   __syncthreads();
   if (threadIdx.x < 64) {
-    const int p = threadIdx.x;
     double acc[14] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-    if (p < tk.n) {
-      const int g = tk.g0 + p;
-      double mu = s.dmu[g];
-      double u[3] = {s.dori[g], s.dori[N + g], s.dori[2 * N + g]};
+    if (own) {
       const double c[3] = {mu * u[0], mu * u[1], mu * u[2]};
       const double *M = tot[p];
       double F[3] = {-(c[0] * M[0] + c[1] * M[1] + c[2] * M[3]), -(c[0] * M[1] + c[1] * M[2] + c[2] * M[4]),
                      -(c[0] * M[3] + c[1] * M[4] + c[2] * M[5])};
       double H[3] = {M[6], M[7], M[8]};
-      double sd = s.dpc[g];
       const double *G = Rp.G;
       double Gu[3] = {G[0] * u[0] + G[1] * u[1] + G[2] * u[2], G[3] * u[0] + G[4] * u[1] + G[5] * u[2],
                       G[6] * u[0] + G[7] * u[1] + G[8] * u[2]};
@@ -2540,22 +2565,13 @@ __global__ void __launch_bounds__(256 * kBpGroups) k_recip_force_mfma(State s, c
       cross3(u, w, t);
       double Ff[3], Tf[3];
       for (int a = 0; a < 3; ++a) {
-        Ff[a] = s.F[a * N + g] + F[a];
-        Tf[a] = s.T[a * N + g] + t[a];
+        Ff[a] = F0[a] + F[a];
+        Tf[a] = T0[a] + t[a];
         s.F[a * N + g] = Ff[a];
         s.T[a * N + g] = Tf[a];
       }
       if (kf.ts) {
         // the last moveP with the final force and torque (as k_kick2)
-        const double m = s.mass[g];
-        const double I[3] = {s.inert[g], s.inert[N + g], s.inert[2 * N + g]};
-        double O[9], pp[3], ll[3];
-        for (int k = 0; k < 9; ++k) O[k] = s.ori[k * N + g];
-        for (int a = 0; a < 3; ++a) {
-          pp[a] = s.p[a * N + g];
-          ll[a] = s.l[a * N + g];
-        }
-        const double ts[2] = {kf.ts[tk.rep * 2 + 0], kf.ts[tk.rep * 2 + 1]};
         kick_regs<true>(Rp, m, I, O, Ff, Tf, ts, pp, ll, acc);
         for (int a = 0; a < 3; ++a) {
           s.p[a * N + g] = pp[a];
@@ -2567,7 +2583,6 @@ __global__ void __launch_bounds__(256 * kBpGroups) k_recip_force_mfma(State s, c
   }
 }
 
-// --------------------------------------------------------------------------
 // Reciprocal sum with several dipole slots per particle (FP64 in every
 // context; v25/functions.cpp:1780-1925).  With sites (i, w), c = mu u and
 // S_i(k) = sum_w (c_w.k) e^{-ik.r_w} the particle's own sum, the energy per
