@@ -167,6 +167,22 @@ __device__ __forceinline__ double thermo_mode1(const RepDev &R, double s, double
   return s + R.dt * (2. * K2 - 3. * (double)R.N * R.temp) / R.Q;
 }
 
+// F += dF, T += dT for particle g.  All six loads come before the first
+// store: the compiler cannot prove the six addresses distinct, so a load
+// after a store would wait for it, one memory round trip per component.
+__device__ __forceinline__ void add_FT(const State &s, int g, const double dF[3], const double dT[3]) {
+  const int N = s.Ntot;
+  double f[3], t[3];
+  for (int a = 0; a < 3; ++a) {
+    f[a] = s.F[a * N + g];
+    t[a] = s.T[a * N + g];
+  }
+  for (int a = 0; a < 3; ++a) {
+    s.F[a * N + g] = f[a] + dF[a];
+    s.T[a * N + g] = t[a] + dT[a];
+  }
+}
+
 // Task of the current wave (4 waves per 256-thread block).  The value is the
 // same in every lane; readfirstlane lets the compiler keep it, and all task
 // data derived from it, in scalar registers.
@@ -1054,10 +1070,7 @@ __global__ void __launch_bounds__(64 * kPairSplit) k_force_sc(State s, double *_
   }
   if (lane < tk.n) {
     const int g = tk.g0 + lane;
-    for (int a = 0; a < 3; ++a) {
-      s.F[a * N + g] += F[a];
-      s.T[a * N + g] += T[a];
-    }
+    add_FT(s, g, F, T);
   }
   wave_store_row(task_out, s.ntask_all, tid, acc, 12, lane);
 }
@@ -1264,10 +1277,7 @@ __global__ void __launch_bounds__(256) k_dipR_all(State s, const R *__restrict__
       if (j == g) continue;
       dipR_partner<R>(s, Rp, tab, Dg0, g, j, F, T, acc);
     }
-    for (int a = 0; a < 3; ++a) {
-      s.F[a * s.Ntot + g] += F[a];
-      s.T[a * s.Ntot + g] += T[a];
-    }
+    add_FT(s, g, F, T);
   }
   wave_store_row(task_out, s.ntask_all, tid, acc, 11, lane);
 }
@@ -1309,10 +1319,7 @@ __global__ void __launch_bounds__(64 * kPairSplit) k_dipR_list(State s, const R 
   }
   if (lane < tk.n) {
     const int g = tk.g0 + lane;
-    for (int a = 0; a < 3; ++a) {
-      s.F[a * s.Ntot + g] += F[a];
-      s.T[a * s.Ntot + g] += T[a];
-    }
+    add_FT(s, g, F, T);
   }
   wave_store_row(task_out, s.ntask_all, tid, acc, 11, lane);
 }
@@ -1525,15 +1532,16 @@ __global__ void __launch_bounds__(64 * PS) k_pairs_mixed(State s, const float *_
   }
   if (sw != 0) return;
   if (lane < tk.n) {
+    // (F + F_WCA) + F_R per component, every load before the first store
     const int g = tk.g0 + lane;
+    double f[3], t[3];
     for (int a = 0; a < 3; ++a) {
-      double f = s.F[a * N + g], t = s.T[a * N + g];
-      f += FS[a];
-      t += TS[a];
-      f += FR[a];
-      t += TR[a];
-      s.F[a * N + g] = f;
-      s.T[a * N + g] = t;
+      f[a] = s.F[a * N + g];
+      t[a] = s.T[a * N + g];
+    }
+    for (int a = 0; a < 3; ++a) {
+      s.F[a * N + g] = (f[a] + FS[a]) + FR[a];
+      s.T[a * N + g] = (t[a] + TS[a]) + TR[a];
     }
   }
   wave_store_row(sc_out, s.ntask_all, tid, accS, 12, lane);
@@ -1560,7 +1568,9 @@ __global__ void __launch_bounds__(256) k_ext_field(State s, double *__restrict__
       cross3(o, Rp.B, t);
       for (int a = 0; a < 3; ++a) T[a] += t[a] * mu;
     }
-    for (int a = 0; a < 3; ++a) s.T[a * N + g] += T[a];
+    double t0[3];
+    for (int a = 0; a < 3; ++a) t0[a] = s.T[a * N + g];
+    for (int a = 0; a < 3; ++a) s.T[a * N + g] = t0[a] + T[a];
   }
   double v[1] = {U};
   wave_store_row(task_out, s.ntask_all, tid, v, 1, lane);
@@ -2282,10 +2292,7 @@ __global__ void __launch_bounds__(64 * kRfGroups) k_recip_force(State s, const C
   for (int a = 0; a < 3; ++a) w[a] = -mu * H[a] + mu * mu * Gu[a] + sd * F[a];
   double t[3];
   cross3(u, w, t);
-  for (int a = 0; a < 3; ++a) {
-    s.F[a * N + g] += F[a];
-    s.T[a * N + g] += t[a];
-  }
+  add_FT(s, g, F, t);
 }
 
 // Back projection on the matrix cores (mixed contexts).  The nz sums of
@@ -2817,15 +2824,16 @@ __global__ void __launch_bounds__(64 * kRfGroups) k_recip_force_nd(State s, cons
   }
   __syncthreads();
   if (cg != 0 || lane >= tk.n) return;
+  double f[3], t[3];
   for (int a = 0; a < 3; ++a) {
-    double f = 0, t = 0;
+    f[a] = 0;
+    t[a] = 0;
     for (int q = 0; q < kRfGroups; ++q) {
-      f += red[q][a][lane];
-      t += red[q][3 + a][lane];
+      f[a] += red[q][a][lane];
+      t[a] += red[q][3 + a][lane];
     }
-    s.F[a * N + g] += f;
-    s.T[a * N + g] += t;
   }
+  add_FT(s, g, f, t);
 }
 
 // --------------------------------------------------------------------------
