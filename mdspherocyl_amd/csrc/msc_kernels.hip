@@ -3622,8 +3622,9 @@ __global__ void k_zero_FT(State s) {
 // rows1: [0..1] twice the kinetic energies after the kick, and rows: [0]
 // largest centre-component displacement since savec_list, [1] largest
 // segment-point displacement since the contact rows were built (as
-// k_check_list), [2..7] sums of c c^T (as k_recip_tables); waves 0-2 then
-// write the phase tables of axes 0-2 (one particle per lane).  In the step
+// k_check_list), [2..7] sums of c c^T (as k_recip_tables); waves 1-3
+// write the phase tables of axes 0-2 (one particle per lane) once wave 0
+// has the dipole positions, while wave 0 stores and sums.  In the step
 // loop (ls.dmax set) wave 0 also reduces the displacements into the
 // replica's maxima for the list decision.
 template <typename R>
@@ -3638,95 +3639,106 @@ __global__ void __launch_bounds__(256) k_drift(State s, const Thermo *__restrict
   const RepDev &Rp = s.reps[tk.rep];
   const int N = s.Ntot;
   const int k = __builtin_amdgcn_readfirstlane(step[0]);
-  if (wv == 0) {
+  // wave 0, first part: the particle's state is loaded once and advanced
+  // in registers (renorm, kick, drift, dipole frame); the phase angles go
+  // to LDS so that waves 1-3 write the phase tables while wave 0 finishes
+  const int g = tk.g0 + lane;
+  const bool own = wv == 0 && lane < tk.n;
+  double acc[14] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  double x[3], p[3], l[3], O[9], sn[3], se[6], pos0[3], ori0[3], len = 0.0;
+  Slot0 k0;
+  if (own) {
     const double *gs = gs_base + (size_t)k * s.nrep * 4;
     // renorm every 100 steps at the start of the step
     const bool renorm = fmodf((float)(step[1] + k), 100.f) == 0.f;
-    double acc[14] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-    double m = 0.0, me = 0.0, m2[6] = {0, 0, 0, 0, 0, 0};
-    if (lane < tk.n) {
-      // the particle's state is loaded once, advanced in registers (renorm,
-      // kick, drift, dipole frame) and stored once
-      const int g = tk.g0 + lane;
-      const double mass = s.mass[g], len = s.len[g];
-      const double I[3] = {s.inert[g], s.inert[N + g], s.inert[2 * N + g]};
-      double x[3], p[3], l[3], F[3], T[3], O[9], sn[3], se[6];
-      for (int a = 0; a < 3; ++a) {
-        x[a] = s.x[a * N + g];
-        p[a] = s.p[a * N + g];
-        l[a] = s.l[a * N + g];
-        F[a] = s.F[a * N + g];
-        T[a] = s.T[a * N + g];
-        sn[a] = s.snap[a * N + g];
-        se[a] = s.snap_e[a * N + g];
-        se[3 + a] = s.snap_e[(3 + a) * N + g];
-      }
-      for (int q = 0; q < 9; ++q) O[q] = s.ori[q * N + g];
-      const Slot0 k0 = load_slot0(s, g);
-      if (renorm) renorm_regs(O);
-      const Thermo Th = th[tk.rep];
-      const double ts[2] = {thermo_half(Rp, Th.s[0], gs[tk.rep * 4 + 0]),
-                            thermo_half(Rp, Th.s[1], gs[tk.rep * 4 + 2])};
-      kick_regs<false>(Rp, mass, I, O, F, T, ts, p, l, acc);
-      drift_regs(Rp, mass, I, p, l, x, O);
-      double pos0[3], ori0[3];
-      dipoles_regs(s, g, x, O, pos0, ori0, &k0);
-      for (int a = 0; a < 3; ++a) {
-        s.x[a * N + g] = x[a];
-        s.p[a * N + g] = p[a];
-        s.l[a * N + g] = l[a];
-        s.F[a * N + g] = 0.0;
-        s.T[a * N + g] = 0.0;
-      }
-      for (int q = 0; q < 9; ++q) s.ori[q * N + g] = O[q];
-      double dr2 = 0, du2 = 0;
-      for (int a = 0; a < 3; ++a) {
-        double D = x[a] - sn[a];
-        D -= Rp.L[a] * floor(D / Rp.L[a] + 0.5);
-        m = fmax(m, fabs(D));
-        double E = x[a] - se[a];
-        E -= Rp.L[a] * floor(E / Rp.L[a] + 0.5);
-        dr2 += E * E;
-        double U = O[6 + a] - se[3 + a];
-        du2 += U * U;
-        th_sh[a][lane] = recip_theta_of(Rp, a, pos0[a]);
-      }
-      // segment points move by at most |dr| + L/2 |du|; with the context's
-      // own rows the bound also covers the dipoles (levers up to lever_max)
-      const double arm = Rp.own_lists ? fmax(0.5 * len, s.lever_max) : 0.5 * len;
-      me = sqrt(dr2) + arm * sqrt(du2);
-      recip_amp_regs<R>(s, g, k0.mu, ori0, amp, m2);
+    const double mass = s.mass[g];
+    len = s.len[g];
+    const double I[3] = {s.inert[g], s.inert[N + g], s.inert[2 * N + g]};
+    double F[3], T[3];
+    for (int a = 0; a < 3; ++a) {
+      x[a] = s.x[a * N + g];
+      p[a] = s.p[a * N + g];
+      l[a] = s.l[a * N + g];
+      F[a] = s.F[a * N + g];
+      T[a] = s.T[a * N + g];
+      sn[a] = s.snap[a * N + g];
+      se[a] = s.snap_e[a * N + g];
+      se[3 + a] = s.snap_e[(3 + a) * N + g];
     }
-    wave_store_row(rows1, s.ntask_all, tid, acc, 2, lane);
-    m = wave_max(m);
-    me = wave_max(me);
-    const size_t T = s.ntask_all;
-    double *o = rows + tid;  // value k at o[k * T]
-    if (lane == 0) {
-      o[0] = m;
-      o[T] = me;
-      if (ls.dmax) {
-        // replica maxima for the list decision in kDmaxBuckets buckets
-        // (fewer tasks per address; non-negative doubles order as their bit
-        // patterns)
-        unsigned long long *d = ls.dmax + (((size_t)(k & 1) * s.nrep + tk.rep) * kDmaxBuckets + (tid & (kDmaxBuckets - 1))) * 2;
-        atomicMax(d, (unsigned long long)__double_as_longlong(m));
-        atomicMax(d + 1, (unsigned long long)__double_as_longlong(me));
-      }
-    }
-    // the replica's first task clears the buckets of the next step
-    if (ls.dmax && tid == Rp.task0 && lane < kDmaxBuckets) {
-      unsigned long long *dn = ls.dmax + (((size_t)((k + 1) & 1) * s.nrep + tk.rep) * kDmaxBuckets + lane) * 2;
-      dn[0] = 0;
-      dn[1] = 0;
-    }
-    for (int q = 0; q < 6; ++q) {
-      const double t = wave_sum(m2[q]);
-      if (lane == 0) o[(2 + q) * T] = t;
-    }
+    for (int q = 0; q < 9; ++q) O[q] = s.ori[q * N + g];
+    k0 = load_slot0(s, g);
+    if (renorm) renorm_regs(O);
+    const Thermo Th = th[tk.rep];
+    const double ts[2] = {thermo_half(Rp, Th.s[0], gs[tk.rep * 4 + 0]),
+                          thermo_half(Rp, Th.s[1], gs[tk.rep * 4 + 2])};
+    kick_regs<false>(Rp, mass, I, O, F, T, ts, p, l, acc);
+    drift_regs(Rp, mass, I, p, l, x, O);
+    dipoles_regs(s, g, x, O, pos0, ori0, &k0);
+    for (int a = 0; a < 3; ++a) th_sh[a][lane] = recip_theta_of(Rp, a, pos0[a]);
   }
   __syncthreads();
-  if (wv < 3 && lane < tk.n) recip_phase_axis<R>(s, Rp, tk.g0 + lane, wv, th_sh[wv][lane], ph);
+  if (wv > 0) {
+    if (lane < tk.n) recip_phase_axis<R>(s, Rp, g, wv - 1, th_sh[wv - 1][lane], ph);
+    return;
+  }
+  // wave 0, second part: the state stored, rows1: [0..1] twice the kinetic
+  // energies after the kick, rows: [0] largest centre-component
+  // displacement since savec_list, [1] largest segment-point displacement
+  // since the contact rows were built, [2..7] sums of c c^T
+  double m = 0.0, me = 0.0, m2[6] = {0, 0, 0, 0, 0, 0};
+  if (own) {
+    for (int a = 0; a < 3; ++a) {
+      s.x[a * N + g] = x[a];
+      s.p[a * N + g] = p[a];
+      s.l[a * N + g] = l[a];
+      s.F[a * N + g] = 0.0;
+      s.T[a * N + g] = 0.0;
+    }
+    for (int q = 0; q < 9; ++q) s.ori[q * N + g] = O[q];
+    double dr2 = 0, du2 = 0;
+    for (int a = 0; a < 3; ++a) {
+      double D = x[a] - sn[a];
+      D -= Rp.L[a] * floor(D / Rp.L[a] + 0.5);
+      m = fmax(m, fabs(D));
+      double E = x[a] - se[a];
+      E -= Rp.L[a] * floor(E / Rp.L[a] + 0.5);
+      dr2 += E * E;
+      double U = O[6 + a] - se[3 + a];
+      du2 += U * U;
+    }
+    // segment points move by at most |dr| + L/2 |du|; with the context's
+    // own rows the bound also covers the dipoles (levers up to lever_max)
+    const double arm = Rp.own_lists ? fmax(0.5 * len, s.lever_max) : 0.5 * len;
+    me = sqrt(dr2) + arm * sqrt(du2);
+    recip_amp_regs<R>(s, g, k0.mu, ori0, amp, m2);
+  }
+  wave_store_row(rows1, s.ntask_all, tid, acc, 2, lane);
+  m = wave_max(m);
+  me = wave_max(me);
+  const size_t T = s.ntask_all;
+  double *o = rows + tid;  // value k at o[k * T]
+  if (lane == 0) {
+    o[0] = m;
+    o[T] = me;
+    if (ls.dmax) {
+      // replica maxima for the list decision in kDmaxBuckets buckets
+      // (fewer tasks per address; non-negative doubles order as their bit
+      // patterns)
+      unsigned long long *d = ls.dmax + (((size_t)(k & 1) * s.nrep + tk.rep) * kDmaxBuckets + (tid & (kDmaxBuckets - 1))) * 2;
+      atomicMax(d, (unsigned long long)__double_as_longlong(m));
+      atomicMax(d + 1, (unsigned long long)__double_as_longlong(me));
+    }
+  }
+  // the replica's first task clears the buckets of the next step
+  if (ls.dmax && tid == Rp.task0 && lane < kDmaxBuckets) {
+    unsigned long long *dn = ls.dmax + (((size_t)((k + 1) & 1) * s.nrep + tk.rep) * kDmaxBuckets + lane) * 2;
+    dn[0] = 0;
+    dn[1] = 0;
+  }
+  for (int q = 0; q < 6; ++q) {
+    const double t = wave_sum(m2[q]);
+    if (lane == 0) o[(2 + q) * T] = t;
+  }
 }
 
 // moveTher mode 1 from the first kick's energies, re-summed by each wave,
