@@ -3784,7 +3784,7 @@ __global__ void __launch_bounds__(256) k_kick2(State s, const Thermo *__restrict
 // finishes the step from these sums, advances the step counter) and the
 // replica's four thermostat deviates of the step after it.  One wave
 // reading a replica's rows alone waits for many memory round trips: spread
-// over 12 waves on 3 CUs the reads take about one.
+// over 16 waves on 4 CUs (three values each) the reads take about one.
 // Extra blocks (ls.rebuild set, beyond the replicas' blocks): the cell rank
 // and packed records of rebuilt replicas, one particle per thread.
 __device__ __forceinline__ void end_value(const State &s, const RepDev &R, int rp, int v, const double *rows,

@@ -172,7 +172,7 @@ void launch_kick2(const State &s, const Thermo *th, const double *gs, const int 
 // thermostat, energies and observables need (k_end, several blocks per
 // replica so that the row reads spread over several CUs), v = kEndVals the
 // step index k, then the step's four thermostat deviates of the replica
-constexpr int kEndVals = 44, kEndSums = 56, kEndBlocks = 3;  // kEndBlocks * 4 waves >= kEndVals / 4
+constexpr int kEndVals = 44, kEndSums = 56, kEndBlocks = 4;  // kEndBlocks * 4 waves >= kEndVals / 4
 void launch_end(const State &s, const int *step, const double *gs, const double *rows, long region,
                 bool has_field, const double *kblk, int maxslot, double *sums, const ListStep &ls,
                 hipStream_t st);
